@@ -1,0 +1,183 @@
+"""Throughput of the detection hot path on one MI355X.
+
+Workload (default "full"): 2000 synthetic 1024x1024 uint16 frames, 1000
+particles per frame (sigma ~ U[1.2, 2], A ~ lognormal(ln 600, 0.5), Brownian
+D = 1, 5% blinking, background ramp 80..140 + Poisson), parameters
+diameter 9, separation 7, minmass 400, characterize on.  The frames are
+rendered on the GPU and stay in HBM; one step is one pass of the pipeline
+(bandpass -> local max -> walk + characterize) over all frames.
+
+Prints ONE JSON line: frames/s, the roofline of the dominant kernel (from
+HIP events around each stage on the launch stream), and the CPU oracle's
+frames/s on a bounded sample of the same frames on the host cores, plus a
+bit-for-bit parity check of that sample.
+
+    python bench.py [--steps K] [--warmup W] [--workload full|small]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+WORKLOADS = {
+    # name: (synth spec, frames, params)
+    "full": ("full", 2000, dict(diameter=9, separation=7, minmass=400.0, characterize=True)),
+    "small": ("small", 100, dict(diameter=9, separation=9, minmass=500.0, noise_size=1.0,
+                                 smoothing_size=9, max_iterations=10, characterize=True)),
+}
+
+
+def _cpu_baseline(video_host, params, budget_s, gpu_rows):
+    import numpy as np
+
+    import oracle
+    T, H, W = video_host.shape
+    try:
+        ncores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        ncores = os.cpu_count() or 1
+    nthreads = max(1, min(16, ncores))
+    op = oracle.derive_trackpy(H, W, diameter=params["diameter"], separation=params["separation"],
+                               minmass=params["minmass"])
+    done = 0
+    t0 = time.perf_counter()
+    results = []
+    while done < T and time.perf_counter() - t0 < budget_s:
+        n = min(T - done, 4 * nthreads)
+        rows, _ = oracle.detect_batch(video_host[done:done + n], op, characterize=True,
+                                      nthreads=nthreads, cap_per_frame=H * W // 4)
+        results.extend(rows)
+        done += n
+    dt = time.perf_counter() - t0
+    # parity of the GPU rows on the sampled frames
+    mismatched = 0
+    fields = ["x", "y", "mass", "max_intensity", "Rg", "raw_mass", "signal", "ecc", "count"]
+    for t in range(done):
+        ref = np.sort(results[t], order="seed")
+        got = gpu_rows[t]
+        if len(ref) != len(got) or not np.array_equal(ref["seed"], got["seed"]) or not all(
+                np.array_equal(ref[f].view(np.uint32), got[f].view(np.uint32)) for f in fields):
+            mismatched += 1
+    return {
+        "value": round(done / dt, 3), "unit": "frames/s", "cores": nthreads, "kind": "port",
+        "sample": f"first {done} frames of the workload, detect+characterize, oracle C code "
+                  f"(-O2, one frame per thread) on {nthreads} host threads, {dt:.1f} s",
+    }, done, mismatched
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="full", choices=sorted(WORKLOADS))
+    ap.add_argument("--frames", type=int, default=0, help="override the frame count")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU; multi-GPU runs are not part of it "
+              "(frames shard across GPUs with no exchange; see the design notes)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    import gpu_tracking_amd as gt
+    from gpu_tracking_amd import synth
+
+    spec_name, T, params = WORKLOADS[args.workload]
+    if args.frames:
+        T = args.frames
+    s = synth.spec(spec_name)
+    H, W = s.nrows, s.ncols
+    print(f"[bench] rendering {T} frames {H}x{W} ({spec_name}) on the GPU", file=sys.stderr,
+          flush=True)
+    frames = synth.video_torch(s, T, seed=20261203 + 2000, device="cuda")
+    torch.cuda.synchronize()
+
+    p = gt.make_params(params["diameter"], **{k: v for k, v in params.items() if k != "diameter"})
+    ctx = gt.Context(p, H, W)
+    cap = 8192
+    out = torch.empty((T, cap, 12), dtype=torch.int32, device="cuda")
+    counts = torch.empty((T,), dtype=torch.int32, device="cuda")
+    stream = torch.cuda.current_stream().cuda_stream
+
+    def step():
+        ctx.detect_device(frames, out=out, counts=counts, cap_per_frame=cap, stream=stream)
+
+    print(f"[bench] warmup {args.warmup}", file=sys.stderr, flush=True)
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    if int(counts.max().item()) > cap:
+        raise SystemExit(f"detection capacity {cap} exceeded: {int(counts.max().item())}")
+
+    ctx.set_timing(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    stage_ms, nlaunch, chunk = ctx.stage_times()   # device time of the last step
+    ctx.set_timing(False)
+    fps = T * args.steps / dt
+    print(f"[bench] {fps:.0f} frames/s, stages (last step, ms) {stage_ms}", file=sys.stderr,
+          flush=True)
+
+    # roofline of the dominant kernel; algorithmic bytes per pixel:
+    # bandpass 2 (u16 in) + 4 (f32 out), local max 4 (f32 in), walk ~0 per pixel
+    names = ["bandpass", "localmax", "walk"]
+    alg_bpp = {"bandpass": 6.0, "localmax": 4.0, "walk": 0.0}
+    k = int(np.argmax(stage_ms)) if stage_ms else 0
+    kname = names[k]
+    launch_ms = stage_ms[k] / nlaunch if stage_ms else float("nan")
+    bytes_launch = alg_bpp[kname] * H * W * chunk
+    achieved = bytes_launch / (launch_ms * 1e-3) / 1e9 if stage_ms and alg_bpp[kname] else 0.0
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", f"traffic_{kname}_{args.workload}.json")
+    if os.path.exists(tf):
+        with open(tf) as f:
+            tj = json.load(f)
+        traffic = round(tj["bytes_per_frame"] * chunk)
+    roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+            "kernel": kname, "launch_ms": round(launch_ms, 4), "frames_per_launch": chunk,
+            "stage_ms_per_step": [round(x, 3) for x in stage_ms],
+            "pipeline_alg_bytes_per_frame": 10 * H * W,
+            "pipeline_frac": round(10 * H * W * fps / (HBM_PEAK_GBS * 1e9), 4)}
+
+    cpu = None
+    parity = None
+    if not args.no_cpu:
+        rows = gt.detections_to_numpy(out[:64], counts[:64])
+        host = frames[:64].cpu().numpy().view(np.uint16)
+        print("[bench] CPU baseline", file=sys.stderr, flush=True)
+        cpu, nchk, bad = _cpu_baseline(host, params, args.cpu_seconds, rows)
+        parity = {"frames_checked": nchk, "frames_bit_identical": nchk - bad}
+
+    res = {
+        "metric": "frames/s (detect+characterize, 1024^2 u16)" if args.workload == "full"
+        else f"frames/s (detect+characterize, {H}x{W} u16)",
+        "value": round(fps, 1), "unit": "frames/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(dt / args.steps * 1e3, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded, rendered on the GPU)",
+        "config": {"workload": f"{args.workload}: {T} frames {H}x{W} u16, "
+                               f"{s.n_particles} particles/frame, " +
+                               ", ".join(f"{k}={v}" for k, v in params.items())},
+        "roofline": roof, "cpu_baseline": cpu, "parity": parity,
+        "mean_detections_per_frame": round(float(counts.float().mean().item()), 1),
+    }
+    print(json.dumps(res), flush=True)
+
+
+if __name__ == "__main__":
+    main()

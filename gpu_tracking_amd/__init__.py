@@ -1,0 +1,17 @@
+"""gpu_tracking_amd -- gpu_tracking's per-frame particle detection on MI355X.
+
+Hand-written HIP kernels for gfx950 behind a C ABI (include/gtrack.h,
+libgtrack.so), with the reference's Python entry points on top:
+
+    import gpu_tracking_amd as gt
+    df = gt.batch(video, 9, minmass=500)
+
+The package needs the built library; importing the entry points raises when
+it is missing (no CPU fallback).
+"""
+from .api import (Context, LoG, batch, characterize_points, detections_to_numpy,  # noqa: F401
+                  make_params)
+from ._native import GtError, column_names, default_params  # noqa: F401
+
+__all__ = ["batch", "characterize_points", "LoG", "Context", "make_params", "GtError",
+           "column_names", "default_params", "detections_to_numpy"]

@@ -1,0 +1,316 @@
+"""Python entry points with the reference's names, arguments and defaults.
+
+``batch``, ``characterize_points`` and ``LoG`` mirror gpu_tracking's
+``python/gpu_tracking/generated.py`` (batch :2-57, characterize_points
+:59-162, LoG :164-227) and the defaults its binding macro applies
+(gpu_tracking_macros/src/lib.rs:187-683).  They return a pandas DataFrame
+with the reference's columns (column_names, src/execute_gpu.rs:482-519) and
+dtypes.  Frames are given as a numpy array (T, Y, X) or (Y, X); uint16,
+uint8 and float32 frames go to the GPU without a host-side cast.
+
+Fixes of reference wrapper bugs (documented divergences):
+  - characterize_points: ``has_frames = None,`` made a tuple
+    (generated.py:127) and the function called a missing
+    ``characterize_rust`` (generated.py:154);
+  - LoG passed undefined names ``min_r, max_r`` (generated.py:222-225).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from typing import Iterable, Optional
+
+import numpy as np
+
+from . import _native as N
+
+# Python-level defaults of the reference binding (macros lib.rs:187-461, 487-569).
+_TRACKPY_DEFAULTS = dict(
+    noise_size=1.0, smoothing_size=None, minmass=0.0, max_iterations=10, characterize=True,
+    search_range=None, memory=None, doughnut_correction=True, bg_radius=None, gap_radius=None,
+    snr=None, minmass_snr=None, truncate_preprocessed=True, correct_illumination=None,
+    illumination_sigma=None, illumination_correction_per_frame=False, adaptive_background=None,
+    shift_threshold=0.6, linker_reset_points=None, frames=None, tqdm=True, channel=None,
+    maxsize=math.inf, separation=None, threshold=1.0 / 255.0, invert=False, percentile=64.0,
+    topn=0xFFFFFFFF, preprocess=True, filter_close=True,
+)
+
+
+def _dtype_code(arr: np.ndarray) -> int:
+    if arr.dtype == np.uint16:
+        return N.GT_U16
+    if arr.dtype == np.uint8:
+        return N.GT_U8
+    if arr.dtype == np.float32:
+        return N.GT_F32
+    raise TypeError(f"unsupported frame dtype {arr.dtype}")
+
+
+def _as_video(file_or_array) -> np.ndarray:
+    if isinstance(file_or_array, (str, bytes)) or hasattr(file_or_array, "__fspath__"):
+        raise N.GtError(12, "file input is not supported in this build; pass a numpy array")
+    arr = np.asarray(file_or_array)
+    if arr.ndim == 2:
+        arr = arr[None]
+    if arr.ndim != 3:
+        raise ValueError(f"expected a (T, Y, X) array, got shape {arr.shape}")
+    if arr.dtype not in (np.uint16, np.uint8, np.float32):
+        # the reference's np.atleast_3d(arr).astype("float32") (python_bindings.rs:144)
+        arr = arr.astype(np.float32)
+    return np.ascontiguousarray(arr)
+
+
+def make_params(diameter: int, **kw) -> N.GtParams:
+    """TrackingParams for the Trackpy style with the Python binding's defaults."""
+    unknown = set(kw) - set(_TRACKPY_DEFAULTS)
+    if unknown:
+        raise TypeError(f"unexpected keyword arguments: {sorted(unknown)}")
+    a = dict(_TRACKPY_DEFAULTS)
+    a.update({k: v for k, v in kw.items() if v is not None or k in ("search_range", "memory")})
+    p = N.default_params()
+    p.style = N.GT_STYLE_TRACKPY
+    p.diameter = int(diameter)
+    p.separation = int(a["separation"]) if a["separation"] is not None else int(diameter) + 1
+    p.maxsize = float(a["maxsize"])
+    p.threshold = float(a["threshold"])
+    p.invert = int(bool(a["invert"]))
+    p.percentile = float(a["percentile"])
+    p.topn = int(a["topn"])
+    p.preprocess = int(bool(a["preprocess"]))
+    p.filter_close = int(bool(a["filter_close"]))
+    _common(p, a)
+    return p
+
+
+def _common(p: N.GtParams, a: dict) -> None:
+    p.noise_size = float(a["noise_size"])
+    if a["smoothing_size"] is not None:
+        p.has_smoothing_size, p.smoothing_size = 1, int(a["smoothing_size"])
+    p.minmass = float(a["minmass"])
+    p.max_iterations = int(a["max_iterations"])
+    p.characterize = int(bool(a["characterize"]))
+    if a["search_range"] is not None:
+        p.has_search_range, p.search_range = 1, float(a["search_range"])
+    if a["memory"] is not None:
+        p.has_memory, p.memory = 1, int(a["memory"])
+    p.doughnut_correction = int(bool(a["doughnut_correction"]))
+    if a["bg_radius"] is not None:
+        p.has_bg_radius, p.bg_radius = 1, float(a["bg_radius"])
+        # gap_radius only takes effect together with bg_radius (macros lib.rs:312)
+        p.has_gap_radius = 1
+        p.gap_radius = float(a["gap_radius"]) if a["gap_radius"] is not None else 0.0
+    if a["snr"] is not None:
+        p.has_snr, p.snr = 1, float(a["snr"])
+    if a["minmass_snr"] is not None:
+        p.has_minmass_snr, p.minmass_snr = 1, float(a["minmass_snr"])
+    p.truncate_preprocessed = int(bool(a["truncate_preprocessed"]))
+    sigma = a["illumination_sigma"]
+    if sigma is None and a.get("correct_illumination"):
+        sigma = 30.0
+    if sigma is not None:
+        p.has_illumination_sigma, p.illumination_sigma = 1, float(sigma)
+    p.illumination_correction_per_frame = int(bool(a["illumination_correction_per_frame"]))
+    if a["adaptive_background"] is not None:
+        p.has_adaptive_background, p.adaptive_background = 1, int(a["adaptive_background"])
+    p.shift_threshold = float(a["shift_threshold"])
+
+
+class Context:
+    """One detection context (setup_state) for frames of a fixed size."""
+
+    def __init__(self, params: N.GtParams, nrows: int, ncols: int,
+                 characterize_points: bool = False):
+        self._lib = N.lib()
+        self.params = params
+        self.shape = (int(nrows), int(ncols))
+        h = C.c_void_p()
+        N.check(self._lib.gt_context_create(C.byref(params), nrows, ncols,
+                                            int(characterize_points), C.byref(h)))
+        self._h = h
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) and self._h.value:
+            self._lib.gt_context_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def handle(self) -> C.c_void_p:
+        return self._h
+
+    def execute(self, video: np.ndarray, keys: Optional[Iterable[int]] = None,
+                points: Optional[np.ndarray] = None, points_has_frames: bool = False,
+                points_has_r: bool = False, linker_reset_points=None):
+        """gt_execute over a host video; returns (matrix, names, types)."""
+        video = np.ascontiguousarray(video)
+        if video.shape[1:] != self.shape:
+            raise N.GtError(4, f"dimension mismatch. provided dimensions are {self.shape}, "
+                               f"but the frames are {video.shape[1:]}")
+        keys_arr = None if keys is None else np.ascontiguousarray(np.asarray(list(keys), np.uint64))
+        pts = None if points is None else np.ascontiguousarray(points, np.float32)
+        reset = None if linker_reset_points is None else np.ascontiguousarray(
+            np.asarray(list(linker_reset_points), np.uint64))
+        out = N.GtOutput()
+        st = self._lib.gt_execute(
+            self._h, video.ctypes.data, _dtype_code(video), video.shape[0],
+            None if keys_arr is None else keys_arr.ctypes.data,
+            0 if keys_arr is None else len(keys_arr),
+            None if pts is None else pts.ctypes.data, 0 if pts is None else pts.shape[0],
+            0 if pts is None else pts.shape[1], int(points_has_frames), int(points_has_r),
+            None if reset is None else reset.ctypes.data, 0 if reset is None else len(reset),
+            C.byref(out))
+        N.check(st)
+        try:
+            n, m = int(out.nrows), int(out.ncols)
+            names = [out.col_names[i].decode() for i in range(m)]
+            types = [out.col_types[i].decode() for i in range(m)]
+            data = np.ctypeslib.as_array(out.data, shape=(n * m,)).copy().reshape(n, m) \
+                if n * m else np.zeros((0, m), np.float32)
+        finally:
+            self._lib.gt_output_free(C.byref(out))
+        return data, names, types
+
+    # device-resident path -------------------------------------------------
+    def detect_device(self, frames, out=None, counts=None, cap_per_frame: Optional[int] = None,
+                      stream=None):
+        """Runs the pipeline on frames already on the GPU (a torch tensor of
+        shape (T, Y, X), dtype int16/uint16 (read as u16), uint8 or float32).
+        Returns (detections tensor (T, cap, 12) of 32-bit words, counts (T,)
+        int32); the call is asynchronous on `stream` (default: torch's current
+        stream)."""
+        import torch
+        if not frames.is_cuda:
+            raise ValueError("detect_device needs a device tensor")
+        if frames.dim() == 2:
+            frames = frames.unsqueeze(0)
+        if tuple(frames.shape[1:]) != self.shape or not frames.is_contiguous():
+            raise ValueError("frames must be contiguous with shape (T,) + context shape")
+        if frames.dtype in (torch.int16, torch.uint16):
+            code = N.GT_U16
+        elif frames.dtype == torch.uint8:
+            code = N.GT_U8
+        elif frames.dtype == torch.float32:
+            code = N.GT_F32
+        else:
+            raise TypeError(f"unsupported dtype {frames.dtype}")
+        T = frames.shape[0]
+        cap = cap_per_frame or max(1024, (self.shape[0] * self.shape[1]) // 16)
+        if out is None:
+            out = torch.empty((T, cap, 12), dtype=torch.int32, device=frames.device)
+        if counts is None:
+            counts = torch.empty((T,), dtype=torch.int32, device=frames.device)
+        if stream is None:
+            stream = torch.cuda.current_stream(frames.device).cuda_stream
+        N.check(self._lib.gt_detect_device(self._h, frames.data_ptr(), code, T, out.data_ptr(),
+                                           cap, counts.data_ptr(), stream))
+        return out, counts
+
+    def set_timing(self, enable: bool) -> None:
+        N.check(self._lib.gt_set_timing(self._h, int(enable)))
+
+    def stage_times(self):
+        """(per-stage ms of the last detect_device call, launches per stage,
+        frames per launch); stages are bandpass, local max, walk."""
+        buf = (C.c_double * 3)()
+        nl, fpl = C.c_uint32(0), C.c_uint32(0)
+        n = self._lib.gt_stage_times(self._h, buf, 3, C.byref(nl), C.byref(fpl))
+        return [buf[i] for i in range(n)], nl.value, fpl.value
+
+
+def detections_to_numpy(out, counts) -> list:
+    """Device detections -> list of per-frame structured arrays sorted by seed."""
+    dt = np.dtype(N.DETECTION_FIELDS)
+    o = out.cpu().numpy()
+    c = counts.cpu().numpy().astype(np.int64)
+    res = []
+    for t in range(o.shape[0]):
+        n = int(c[t])
+        if n > o.shape[1]:
+            raise N.GtError(101, f"frame {t}: {n} detections above capacity {o.shape[1]}")
+        a = np.ascontiguousarray(o[t, :n]).view(dt).reshape(-1)
+        res.append(np.sort(a, order="seed"))
+    return res
+
+
+def _to_dataframe(data: np.ndarray, names, types):
+    import pandas as pd
+    df = pd.DataFrame(data, columns=names)
+    return df.astype({n: ("int64" if t == "int" else "float32") for n, t in zip(names, types)})
+
+
+def batch(file_or_array, diameter, **kwargs):
+    """Trackpy-like detection (and, with search_range, linking) on the GPU.
+
+    Same arguments and defaults as gpu_tracking.batch (generated.py:2-57).
+    Returns a pandas DataFrame with columns frame, y, x, mass, [Rg, raw,
+    signal, ecc], [raw_mass, raw_bg_median, raw_mass_corrected],
+    [particle, duration]."""
+    video = _as_video(file_or_array)
+    frames = kwargs.pop("frames", None)
+    reset = kwargs.pop("linker_reset_points", None)
+    kwargs.pop("tqdm", None)
+    kwargs.pop("channel", None)
+    p = make_params(diameter, **kwargs)
+    with Context(p, video.shape[1], video.shape[2]) as ctx:
+        data, names, types = ctx.execute(video, keys=frames, linker_reset_points=reset)
+    return _to_dataframe(data, names, types)
+
+
+def characterize_points(file_or_array, points_to_characterize, diameter=None, has_frames=None,
+                        has_r=None, **kwargs):
+    """Characterize given positions (generated.py:59-162)."""
+    import pandas as pd
+    if isinstance(points_to_characterize, pd.DataFrame):
+        cols = ["y", "x"]
+        if "frame" in points_to_characterize.columns:
+            cols = ["frame"] + cols
+            if has_frames is None:
+                has_frames = True
+        if "r" in points_to_characterize.columns:
+            cols = cols + ["r"]
+            if has_r is None:
+                has_r = True
+        points_arr = points_to_characterize[cols].to_numpy().astype("float32")
+    else:
+        points_arr = np.asarray(points_to_characterize, np.float32)
+    has_frames = bool(has_frames)
+    has_r = bool(has_r)
+    if diameter is None:
+        if has_r:
+            diameter = 2 * int(points_arr[:, -1].max() + 0.5) + 1
+        else:
+            raise ValueError("Diameter needs to be specified if the supplied points don't have "
+                             "associated radiuses")
+    video = _as_video(file_or_array)
+    frames = kwargs.pop("frames", None)
+    reset = kwargs.pop("linker_reset_points", None)
+    kwargs.pop("tqdm", None)
+    kwargs.pop("channel", None)
+    p = make_params(diameter, **kwargs)
+    # characterize_points_rust forces characterize and disables filter_close
+    # (macros lib.rs:671-681); r is reported when the points carry it.
+    p.characterize = 1
+    p.filter_close = 0
+    if has_r:
+        p.include_r_in_output = 1
+    with Context(p, video.shape[1], video.shape[2], characterize_points=True) as ctx:
+        data, names, types = ctx.execute(video, keys=frames, points=points_arr,
+                                         points_has_frames=has_frames, points_has_r=has_r,
+                                         linker_reset_points=reset)
+    return _to_dataframe(data, names, types)
+
+
+def LoG(file_or_array, min_radius, max_radius, **kwargs):
+    """Laplacian-of-Gaussian detection (generated.py:164-227)."""
+    raise N.GtError(100, "LoG is not available in this build yet")

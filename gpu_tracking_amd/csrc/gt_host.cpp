@@ -1,0 +1,466 @@
+// gt_host.cpp -- host logic of the runtime (no device calls).
+#include "gt_host.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <unordered_map>
+
+namespace {
+
+int32_t f2i_sat(float x) {
+    // f32 -> i32 as Rust's `as i32`: NaN -> 0, saturating.
+    if (x != x) return 0;
+    if (x >= 2147483648.0f) return 2147483647;
+    if (x <= -2147483648.0f) return -2147483647 - 1;
+    return static_cast<int32_t>(x);
+}
+
+uint32_t f2u_sat(float x) {
+    // f32 -> u32 as Rust's `as u32`.
+    if (!(x > 0.0f)) return 0u;
+    if (x >= 4294967296.0f) return 0xFFFFFFFFu;
+    return static_cast<uint32_t>(x);
+}
+
+}  // namespace
+
+void gt_host_params_default(gt_params *p) {
+    // TrackingParams::default() (src/gpu_setup.rs:61-97)
+    std::memset(p, 0, sizeof(*p));
+    p->style = GT_STYLE_TRACKPY;
+    p->diameter = 9;
+    p->separation = 10;
+    p->threshold = 0.0f;
+    p->invert = 0;
+    p->percentile = 0.0f;
+    p->topn = 0;
+    p->maxsize = 0.0f;
+    p->preprocess = 1;
+    p->filter_close = 1;
+    p->n_radii = 10;
+    p->characterize = 0;
+    p->doughnut_correction = 0;
+    p->max_iterations = 10;
+    p->minmass = 0.0f;
+    p->truncate_preprocessed = 1;
+    p->include_r_in_output = 0;
+    p->shift_threshold = 0.6f;
+    p->noise_size = 1.0f;
+    p->illumination_correction_per_frame = 0;
+}
+
+bool gt_host_derive(const gt_params *p, uint32_t H, uint32_t W, gt_derived *d, std::string *why) {
+    (void)H;
+    (void)W;
+    *d = gt_derived();
+    if (p->style != GT_STYLE_TRACKPY && p->style != GT_STYLE_LOG) {
+        *why = "unknown style";
+        return false;
+    }
+    if (p->style == GT_STYLE_LOG) {
+        *why = "LoG style is not available in this build yet";
+        return false;
+    }
+    if (p->has_snr || p->has_minmass_snr || p->has_adaptive_background) {
+        *why = "snr, minmass_snr and adaptive_background need the per-frame noise estimate, "
+               "which this build does not provide";
+        return false;
+    }
+    if (p->has_illumination_sigma) {
+        *why = "illumination correction is not available in this build";
+        return false;
+    }
+    if (p->diameter == 0) {
+        *why = "diameter must be at least 1";
+        return false;
+    }
+    if (!(p->noise_size > 0.0f)) {
+        *why = "noise_size must be positive";
+        return false;
+    }
+    d->style = p->style;
+    // gpu_setup.rs:535-547
+    const uint32_t s = p->has_smoothing_size ? p->smoothing_size : p->diameter;
+    if (s == 0) {
+        *why = "smoothing_size must be at least 1";
+        return false;
+    }
+    if (s / 2 > GT_MAX_R) {
+        *why = "smoothing_size above 33 is not supported";
+        return false;
+    }
+    d->smoothing = s;
+    d->sigma2 = p->noise_size;
+    d->circle = p->diameter;
+    d->separation = p->separation;
+    d->dilation = f2u_sat(2.0f * static_cast<float>(p->separation) / std::sqrt(2.0f));
+    uint32_t m = p->diameter / 2u;
+    const uint32_t m1 = p->separation / 2u - 1u;  // wraps for separation < 2, as the u32 code does
+    if (m1 > m) m = m1;
+    if (s / 2u > m) m = s / 2u;
+    d->margin = static_cast<int32_t>(m);
+    d->max_iterations = p->max_iterations;
+    d->shift_threshold = p->shift_threshold;
+    d->minmass = p->minmass;
+    d->snr = 0.0f;
+    d->minmass_snr = 0.0f;
+    d->rough = 1.0f;
+    d->truncate = p->truncate_preprocessed ? 1 : 0;
+    d->characterize = p->characterize ? 1 : 0;
+    d->filter_close = p->filter_close ? 1 : 0;
+    d->doughnut = p->doughnut_correction ? 1 : 0;
+    d->include_r = p->include_r_in_output ? 1 : 0;
+    // execute_gpu.rs:925-931 and :743-750: radius = diameter / 2; the ring
+    // spans [gap_radius + radius, bg_radius (default 2 * radius)].
+    const float radius = static_cast<float>(p->diameter) / 2.0f;
+    d->bg_radius = p->has_bg_radius ? p->bg_radius : 2.0f * radius;
+    d->gap_radius = p->has_gap_radius ? p->gap_radius : 0.0f;
+    return true;
+}
+
+void gt_host_fill_kparams(const gt_derived *d, uint32_t H, uint32_t W, GtKParams *kp) {
+    std::memset(kp, 0, sizeof(*kp));
+    kp->H = static_cast<int32_t>(H);
+    kp->W = static_cast<int32_t>(W);
+    const int32_t R = static_cast<int32_t>(d->smoothing / 2);
+    kp->R = R;
+    // preprocess_rows.wgsl:30-47 evaluates exp per tap in f32; the table is
+    // computed once here with the same f32 argument.
+    float norm = 0.0f;
+    for (int32_t i = -R; i <= R; ++i) {
+        const float x = static_cast<float>(i);
+        const float g = std::exp(-x * x / (2.0f * d->sigma2));
+        kp->w[i + R] = g;
+        norm = norm + g;
+    }
+    kp->gnorm = norm;
+    kp->box_area = static_cast<float>(static_cast<int32_t>(d->smoothing) * static_cast<int32_t>(d->smoothing));
+    kp->truncate = d->truncate;
+    kp->k = static_cast<int32_t>(d->dilation);
+    kp->a = f2i_sat(static_cast<float>(kp->k) / 2.0f - 0.5f);
+    kp->margin = d->margin;
+    kp->max_thr = 0.0f * d->snr * d->rough;
+    kp->rint = (static_cast<int32_t>(d->circle) - 1) / 2;
+    kp->r2 = static_cast<float>(kp->rint) * static_cast<float>(kp->rint);
+    kp->max_iter = d->max_iterations;
+    kp->shift_thr = d->shift_threshold;
+    kp->minmass = d->minmass;
+    kp->minmass_snr = d->minmass_snr;
+    kp->image_std = 0.0f;
+    kp->rough = d->rough;
+    kp->characterize = d->characterize;
+    kp->circle = static_cast<int32_t>(d->circle);
+}
+
+uint32_t gt_host_column_names(const gt_params *p, const char **names, const char **types) {
+    // column_names (src/execute_gpu.rs:482-519)
+    uint32_t n = 0;
+    auto push = [&](const char *a, const char *b) {
+        if (names) names[n] = a;
+        if (types) types[n] = b;
+        ++n;
+    };
+    push("frame", "int");
+    push("y", "float");
+    push("x", "float");
+    push("mass", "float");
+    if (p->include_r_in_output) push("r", "float");
+    if (p->characterize) {
+        push("Rg", "float");
+        push("raw", "float");
+        push("signal", "float");
+        push("ecc", "float");
+    }
+    if (p->doughnut_correction) {
+        push("raw_mass", "float");
+        push("raw_bg_median", "float");
+        push("raw_mass_corrected", "float");
+    }
+    if (p->has_search_range) {
+        push("particle", "int");
+        push("duration", "int");
+    }
+    return n;
+}
+
+void gt_host_widen(const void *src, int32_t dtype, size_t n, float *dst) {
+    switch (dtype) {
+    case GT_U16: {
+        const uint16_t *s = static_cast<const uint16_t *>(src);
+        for (size_t i = 0; i < n; ++i) dst[i] = static_cast<float>(s[i]);
+        break;
+    }
+    case GT_U8: {
+        const uint8_t *s = static_cast<const uint8_t *>(src);
+        for (size_t i = 0; i < n; ++i) dst[i] = static_cast<float>(s[i]);
+        break;
+    }
+    default: std::memcpy(dst, src, n * sizeof(float)); break;
+    }
+}
+
+gt_status gt_host_plan_frames(int32_t characterize_points, uint64_t nframes, const uint64_t *keys,
+                              uint64_t nkeys, const float *points, uint64_t npoints,
+                              uint32_t point_cols, int32_t has_frames, int32_t has_r,
+                              gt_host_plan *plan, std::string *why) {
+    plan->frames.clear();
+    plan->pt_begin.clear();
+    plan->pt_end.clear();
+    plan->pts.clear();
+    if (characterize_points && points) {
+        const uint32_t need = 2u + (has_frames ? 1u : 0u) + (has_r ? 1u : 0u);
+        if (point_cols != need) {
+            *why = "unsupported characterize array dimensions.\naccepted dimensions: (Nx2) or (Nx3).";
+            return GT_ERR_ARRAY_DIMENSIONS;
+        }
+        const uint32_t cy = has_frames ? 1 : 0, cx = cy + 1, cr = cx + 1;
+        plan->pts.resize(npoints);
+        for (uint64_t i = 0; i < npoints; ++i) {
+            const float *row = points + i * point_cols;
+            gt_result_row r;
+            std::memset(&r, 0, sizeof(r));
+            r.x = row[cy];
+            r.y = row[cx];
+            if (has_r) r.r = row[cr];
+            plan->pts[i] = r;
+        }
+        if (!has_frames) {
+            // FrameSubsetter without a frame column cycles the same positions
+            // over every frame (execute_gpu.rs:1010-1017).
+            if (nframes == 0) return GT_OK;
+            for (uint64_t f = 0; f < nframes; ++f) {
+                plan->frames.push_back(f);
+                plan->pt_begin.push_back(0);
+                plan->pt_end.push_back(npoints);
+            }
+            return GT_OK;
+        }
+        // With a frame column: groups in ascending frame order; a frame value
+        // that decreases is an error; the first group starts at frame 0 even
+        // when it is empty (linking.rs:104-138).
+        float cur = 0.0f;
+        size_t begin = 0;
+        for (uint64_t i = 0; i < npoints; ++i) {
+            const float fr = points[i * point_cols];
+            if (fr > cur) {
+                plan->frames.push_back(static_cast<uint64_t>(cur));
+                plan->pt_begin.push_back(begin);
+                plan->pt_end.push_back(i);
+                begin = i;
+                cur = fr;
+            }
+            if (fr < cur) {
+                *why = "the points provided for characterization aren't sorted in ascending frame number.";
+                return GT_ERR_NON_SORTED_CHARACTERIZATION;
+            }
+        }
+        if (npoints > begin) {
+            plan->frames.push_back(static_cast<uint64_t>(cur));
+            plan->pt_begin.push_back(begin);
+            plan->pt_end.push_back(npoints);
+        }
+        return GT_OK;
+    }
+    if (keys) {
+        plan->frames.assign(keys, keys + nkeys);
+    } else {
+        plan->frames.resize(nframes);
+        for (uint64_t f = 0; f < nframes; ++f) plan->frames[f] = f;
+    }
+    return GT_OK;
+}
+
+gt_host_post::gt_host_post(const gt_params *p, const gt_derived *d, uint32_t H, uint32_t W,
+                           const uint64_t *reset_points, uint64_t nreset)
+    : p_(*p), d_(*d), H_(H), W_(W) {
+    ncols_ = gt_host_column_names(p, nullptr, nullptr);
+    (void)reset_points;
+    if (p->has_search_range) error_ = "linking (search_range) is not available in this build yet";
+    (void)nreset;
+}
+
+gt_status gt_host_post::process(std::vector<gt_detection> &rows, uint64_t frame_index,
+                                const float *raw, std::atomic<int32_t> *interrupt,
+                                std::string *err) {
+    (void)interrupt;
+    (void)err;
+    std::sort(rows.begin(), rows.end(),
+              [](const gt_detection &a, const gt_detection &b) { return a.seed < b.seed; });
+    tmp_rows_.resize(rows.size());
+    for (size_t i = 0; i < rows.size(); ++i) tmp_rows_[i] = rows[i].row;
+    size_t n = tmp_rows_.size();
+    if (d_.style == GT_STYLE_TRACKPY && d_.filter_close && n > 0) {
+        keep_.assign(n, 1);
+        gt_filter_close(tmp_rows_.data(), n, static_cast<float>(d_.separation), keep_.data());
+        size_t m = 0;
+        for (size_t i = 0; i < n; ++i)
+            if (keep_[i]) tmp_rows_[m++] = tmp_rows_[i];
+        n = m;
+        tmp_rows_.resize(n);
+    }
+    if (d_.doughnut) {
+        dough_.assign(3 * n, 0.0f);
+        if (n) gt_doughnut(raw, H_, W_, tmp_rows_.data(), n, static_cast<float>(d_.circle) / 2.0f,
+                           d_.bg_radius, d_.gap_radius, dough_.data());
+    }
+    const float fidx = static_cast<float>(frame_index);
+    for (size_t i = 0; i < n; ++i) {
+        const gt_result_row &r = tmp_rows_[i];
+        out_.push_back(fidx);
+        out_.push_back(r.x);
+        out_.push_back(r.y);
+        out_.push_back(r.mass);
+        if (d_.include_r) out_.push_back(r.r);
+        if (d_.characterize) {
+            out_.push_back(r.Rg);
+            out_.push_back(r.raw_mass);
+            out_.push_back(r.signal);
+            out_.push_back(r.ecc);
+        }
+        if (d_.doughnut) {
+            out_.push_back(dough_[3 * i + 0]);
+            out_.push_back(dough_[3 * i + 1]);
+            out_.push_back(dough_[3 * i + 2]);
+        }
+    }
+    return GT_OK;
+}
+
+gt_status gt_host_post::finish(gt_output *out, std::string *err) {
+    (void)err;
+    // One static name/type list per combination of the four optional column
+    // groups, so the pointers handed out stay valid and are never rewritten.
+    struct Lists {
+        const char *names[16][16];
+        const char *types[16][16];
+    };
+    static const Lists lists = [] {
+        Lists l;
+        std::memset(&l, 0, sizeof(l));
+        for (int m = 0; m < 16; ++m) {
+            gt_params q;
+            gt_host_params_default(&q);
+            q.include_r_in_output = m & 1;
+            q.characterize = (m >> 1) & 1;
+            q.doughnut_correction = (m >> 2) & 1;
+            q.has_search_range = (m >> 3) & 1;
+            gt_host_column_names(&q, l.names[m], l.types[m]);
+        }
+        return l;
+    }();
+    const int combo = (p_.include_r_in_output ? 1 : 0) | (p_.characterize ? 2 : 0) |
+                      (p_.doughnut_correction ? 4 : 0) | (p_.has_search_range ? 8 : 0);
+    const char *const *names = lists.names[combo];
+    const char *const *types = lists.types[combo];
+    const uint32_t nc = ncols_;
+    out->ncols = nc;
+    out->nrows = nc ? out_.size() / nc : 0;
+    out->data = static_cast<float *>(std::malloc(sizeof(float) * (out_.size() ? out_.size() : 1)));
+    if (!out->data) return GT_ERR_GPU_DEVICE;
+    if (!out_.empty()) std::memcpy(out->data, out_.data(), sizeof(float) * out_.size());
+    out->col_names = names;
+    out->col_types = types;
+    return GT_OK;
+}
+
+extern "C" gt_status gt_filter_close(const gt_result_row *rows, uint64_t n, float separation,
+                                     uint8_t *keep) {
+    if (n == 0) return GT_OK;
+    if (!rows || !keep) return GT_ERR_INVALID_ARGUMENT;
+    const float r2 = separation * separation;
+    // Uniform grid with cells of side `separation`; +-2 cells are searched so
+    // rounding of x / separation near a cell edge cannot hide a neighbour.
+    const float cell = separation > 0.0f ? separation : 1.0f;
+    std::unordered_map<int64_t, std::vector<uint32_t>> grid;
+    grid.reserve(static_cast<size_t>(n));
+    auto key = [](int64_t cx, int64_t cy) { return (cx << 32) ^ (cy & 0xFFFFFFFFll); };
+    std::vector<int64_t> cxs(n), cys(n);
+    std::vector<uint8_t> finite(n);
+    for (uint64_t i = 0; i < n; ++i) {
+        finite[i] = std::isfinite(rows[i].x) && std::isfinite(rows[i].y);
+        if (!finite[i]) continue;
+        cxs[i] = static_cast<int64_t>(std::floor(static_cast<double>(rows[i].x) / cell));
+        cys[i] = static_cast<int64_t>(std::floor(static_cast<double>(rows[i].y) / cell));
+        grid[key(cxs[i], cys[i])].push_back(static_cast<uint32_t>(i));
+    }
+    for (uint64_t i = 0; i < n; ++i) {
+        keep[i] = 1;
+        if (!finite[i]) continue;  // NaN/inf distances never fall below r^2
+        const float mi = rows[i].mass;
+        bool drop = false;
+        for (int64_t dx = -2; dx <= 2 && !drop; ++dx)
+            for (int64_t dy = -2; dy <= 2 && !drop; ++dy) {
+                auto it = grid.find(key(cxs[i] + dx, cys[i] + dy));
+                if (it == grid.end()) continue;
+                for (uint32_t j : it->second) {
+                    if (j == i) continue;
+                    const float ddx = rows[j].x - rows[i].x, ddy = rows[j].y - rows[i].y;
+                    float d2 = 0.0f;
+                    d2 = d2 + ddx * ddx;
+                    d2 = d2 + ddy * ddy;
+                    if (!(d2 < r2)) continue;
+                    const float mj = rows[j].mass;
+                    if (mi < mj || (mi == mj && i > j)) {
+                        drop = true;
+                        break;
+                    }
+                }
+            }
+        if (drop) keep[i] = 0;
+    }
+    return GT_OK;
+}
+
+extern "C" gt_status gt_doughnut(const float *raw, uint32_t H, uint32_t W, const gt_result_row *rows,
+                                 uint64_t n, float radius, float bg_radius, float gap_radius,
+                                 float *out3) {
+    if (n == 0) return GT_OK;
+    if (!raw || !rows || !out3) return GT_ERR_INVALID_ARGUMENT;
+    // circle_inds(radius) / annulus_inds(bg_radius, gap_radius + radius)
+    // (kernels.rs:138-171), in their i-major, j-minor order.
+    std::vector<std::pair<int32_t, int32_t>> sig, ring;
+    const int32_t cb = f2i_sat(std::ceil(radius));
+    const float rr = radius * radius;
+    for (int32_t i = -cb; i <= cb; ++i)
+        for (int32_t j = -cb; j <= cb; ++j)
+            if (static_cast<float>(i * i + j * j) <= rr) sig.emplace_back(i, j);
+    const float inner = gap_radius + radius;
+    const int32_t ab = f2i_sat(std::ceil(bg_radius));
+    const float or2 = bg_radius * bg_radius, ir2 = inner * inner;
+    for (int32_t i = -ab; i <= ab; ++i)
+        for (int32_t j = -ab; j <= ab; ++j) {
+            const float q = static_cast<float>(i * i + j * j);
+            if (q <= or2 && q >= ir2) ring.emplace_back(i, j);
+        }
+    std::vector<float> bg;
+    bg.reserve(ring.size());
+    for (uint64_t k = 0; k < n; ++k) {
+        // centre rounded half away from zero (f32::round, execute_gpu.rs:752)
+        const int64_t mu = f2i_sat(std::round(rows[k].x)), mv = f2i_sat(std::round(rows[k].y));
+        float s = 0.0f;
+        size_t ns = 0;
+        for (const auto &o : sig) {
+            const int64_t pu = mu + o.first, pv = mv + o.second;
+            if (pu < 0 || pu >= H || pv < 0 || pv >= W) continue;
+            s = s + raw[pu * W + pv];
+            ++ns;
+        }
+        bg.clear();
+        for (const auto &o : ring) {
+            const int64_t pu = mu + o.first, pv = mv + o.second;
+            if (pu < 0 || pu >= H || pv < 0 || pv >= W) continue;
+            bg.push_back(raw[pu * W + pv]);
+        }
+        float med = 0.0f;
+        if (!bg.empty()) {
+            std::nth_element(bg.begin(), bg.begin() + bg.size() / 2, bg.end());
+            med = bg[bg.size() / 2];
+        }
+        out3[3 * k + 0] = s;
+        out3[3 * k + 1] = med;
+        out3[3 * k + 2] = s - med * static_cast<float>(ns);
+    }
+    return GT_OK;
+}

@@ -1,0 +1,85 @@
+// gt_host.h -- host-side logic of the runtime that needs no GPU: parameter
+// defaults and derivation, column names, frame planning, and the per-frame
+// post-processing of execute_gpu (filter_close, doughnut, output assembly).
+#pragma once
+
+#include <atomic>
+#include <string>
+#include <vector>
+
+#include "gt_kparams.h"
+#include "gtrack.h"
+
+// Parameters after validation and derivation (gpuparams_from_tracking_params,
+// src/gpu_setup.rs:528-580).
+struct gt_derived {
+    int32_t style = GT_STYLE_TRACKPY;
+    uint32_t smoothing = 9;     // preprocess kernel side s
+    float sigma2 = 1.0f;        // noise_size (used as a variance)
+    uint32_t circle = 9;        // diameter
+    uint32_t dilation = 12;     // k
+    int32_t margin = 4;
+    uint32_t separation = 10;
+    uint32_t max_iterations = 10;
+    float shift_threshold = 0.6f;
+    float minmass = 0.0f;
+    float snr = 0.0f, minmass_snr = 0.0f, rough = 1.0f;
+    int32_t truncate = 1;
+    int32_t characterize = 0;
+    int32_t filter_close = 1;
+    int32_t doughnut = 0;
+    float bg_radius = 0.0f;     // resolved doughnut outer radius
+    float gap_radius = 0.0f;
+    int32_t include_r = 0;
+};
+
+void gt_host_params_default(gt_params *p);
+bool gt_host_derive(const gt_params *p, uint32_t H, uint32_t W, gt_derived *d, std::string *why);
+void gt_host_fill_kparams(const gt_derived *d, uint32_t H, uint32_t W, GtKParams *kp);
+uint32_t gt_host_column_names(const gt_params *p, const char **names, const char **types);
+void gt_host_widen(const void *src, int32_t dtype, size_t n, float *dst);
+
+// Which frames to run, in order, and (characterize_points) which positions
+// each of them gets (FrameSubsetter, src/linking.rs:94-196; the keys/frames
+// subset, src/execute_gpu.rs:1020-1056).
+struct gt_host_plan {
+    std::vector<uint64_t> frames;
+    std::vector<size_t> pt_begin, pt_end;   // per planned frame, into pts
+    std::vector<gt_result_row> pts;
+};
+
+gt_status gt_host_plan_frames(int32_t characterize_points, uint64_t nframes, const uint64_t *keys,
+                              uint64_t nkeys, const float *points, uint64_t npoints,
+                              uint32_t point_cols, int32_t has_frames, int32_t has_r,
+                              gt_host_plan *plan, std::string *why);
+
+// Per-frame post-processing and output assembly (post_process,
+// src/execute_gpu.rs:671-817, and the tail of execute_gpu, :1165-1181).
+class gt_host_post {
+  public:
+    gt_host_post(const gt_params *p, const gt_derived *d, uint32_t H, uint32_t W,
+                 const uint64_t *reset_points, uint64_t nreset);
+    bool ok() const { return error_.empty(); }
+    const std::string &error() const { return error_; }
+    bool needs_raw() const { return d_.doughnut != 0; }
+    // rows: the frame's detections in any order (sorted here by seed).
+    gt_status process(std::vector<gt_detection> &rows, uint64_t frame_index, const float *raw,
+                      std::atomic<int32_t> *interrupt, std::string *err);
+    gt_status finish(gt_output *out, std::string *err);
+
+  private:
+    gt_params p_;
+    gt_derived d_;
+    uint32_t H_, W_;
+    uint32_t ncols_ = 0;
+    std::vector<float> out_;
+    std::string error_;
+    std::vector<gt_result_row> tmp_rows_;
+    std::vector<uint8_t> keep_;
+    std::vector<float> dough_;
+};
+
+// Implemented in gt_runtime.cpp (needs the device).
+struct gt_context;
+gt_status gt_host_characterize_points(gt_context *c, size_t base, uint32_t nf, int32_t dtype,
+                                      const gt_host_plan &plan);

@@ -1,0 +1,37 @@
+// gt_kparams.h -- kernel parameter block shared by the host code and the
+// HIP kernels (no HIP headers, so host-only translation units can use it).
+#pragma once
+#include <stdint.h>
+
+// Largest preprocess radius s/2 the LDS-tiled bandpass handles (s <= 33).
+#define GT_MAX_R 16
+// Largest dilation side k the LDS-tiled local-max kernel handles; larger
+// windows run the untiled kernel.
+#define GT_MAX_K_TILED 48
+
+// Derived per-frame parameters, passed by value as a kernel argument.  The
+// first block is the reference's WGSL Params uniform (src/shaders/params.wgsl),
+// extended with the f32 tap table the reference recomputes per tap
+// (preprocess_rows.wgsl:46) and the walk/characterize constants.
+struct GtKParams {
+    int32_t H, W;
+    int32_t R;                    // preprocess radius s/2; taps i = -R..R
+    float w[2 * GT_MAX_R + 1];    // exp(-i^2 / (2 sigma2)) as f32
+    float gnorm;                  // sequential f32 sum of w
+    float box_area;               // f32(s*s)
+    int32_t truncate;             // truncate_preprocessed
+    int32_t k;                    // dilation side
+    int32_t a;                    // window start offset i32(k/2 - 0.5)
+    int32_t margin;
+    float max_thr;                // image_std * snr * rough_snr_factor
+    int32_t rint;                 // walk radius (circle-1)/2
+    float r2;                     // f32(rint)^2
+    uint32_t max_iter;
+    float shift_thr;
+    float minmass;
+    float minmass_snr;
+    float image_std;
+    float rough;
+    int32_t characterize;
+    int32_t circle;
+};

@@ -1,0 +1,386 @@
+// gt_runtime.cpp -- host runtime behind include/gtrack.h.
+//
+// Parameter derivation (src/gpu_setup.rs:528-580), the context (setup_state,
+// :611), the device-resident batch path, and execute_gpu's whole-video driver
+// with its host post-processing (src/execute_gpu.rs:556-590, 671-817,
+// 886-1184).  HIP runtime calls only; kernels live in gt_kernels.hip.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "gt_internal.h"
+#include "gt_host.h"
+
+namespace {
+
+thread_local std::string g_last_error = "no error";
+
+gt_status fail(gt_status st, const char *fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+    return st;
+}
+
+#define GT_HIP(call)                                                                       \
+    do {                                                                                   \
+        hipError_t e_ = (call);                                                            \
+        if (e_ != hipSuccess)                                                              \
+            return fail(GT_ERR_GPU_DEVICE, "%s failed: %s", #call, hipGetErrorString(e_)); \
+    } while (0)
+
+size_t dtype_size(int32_t dtype) {
+    switch (dtype) {
+    case GT_U16: return 2;
+    case GT_F32: return 4;
+    case GT_U8: return 1;
+    default: return 0;
+    }
+}
+
+}  // namespace
+
+struct gt_context {
+    gt_params params;
+    uint32_t H = 0, W = 0;
+    int32_t characterize_points = 0;
+    GtKParams kp;
+    gt_derived der;
+    int device = 0;
+    hipStream_t stream = nullptr;
+
+    // device scratch shared by gt_detect_device and gt_execute
+    uint32_t chunk = 1;            // frames per pipeline chunk
+    float *d_proc = nullptr;       // chunk * HW
+    uint32_t *d_cand = nullptr;    // chunk * cand_cap
+    uint32_t *d_ccount = nullptr;  // chunk
+    uint32_t cand_cap = 0;
+
+    // gt_execute staging
+    void *d_frames = nullptr;      // chunk * HW pixels (largest dtype)
+    gt_detection *d_rows = nullptr;  // chunk * det_cap
+    uint32_t *d_counts = nullptr;  // chunk
+    uint32_t det_cap = 0;
+    void *h_stage = nullptr;       // pinned, chunk * HW * 4 bytes
+    uint32_t *h_counts = nullptr;  // pinned, chunk
+
+    // timing
+    bool timing = false;
+    std::vector<hipEvent_t> events;
+    double stage_ms[3] = {0, 0, 0};
+    int n_timed_chunks = 0;
+    uint32_t timed_chunk_frames = 0;
+
+    std::atomic<int32_t> interrupt{0};
+    std::atomic<uint64_t> prog_cur{0}, prog_total{0};
+};
+
+namespace {
+
+gt_status alloc_scratch(gt_context *c) {
+    const size_t HW = static_cast<size_t>(c->H) * c->W;
+    GT_HIP(hipMalloc(&c->d_proc, sizeof(float) * HW * c->chunk));
+    GT_HIP(hipMalloc(&c->d_cand, sizeof(uint32_t) * static_cast<size_t>(c->cand_cap) * c->chunk));
+    GT_HIP(hipMalloc(&c->d_ccount, sizeof(uint32_t) * c->chunk));
+    return GT_OK;
+}
+
+gt_status alloc_execute_buffers(gt_context *c) {
+    if (c->d_frames) return GT_OK;
+    const size_t HW = static_cast<size_t>(c->H) * c->W;
+    GT_HIP(hipMalloc(&c->d_frames, 4 * HW * c->chunk));
+    GT_HIP(hipMalloc(&c->d_rows, sizeof(gt_detection) * static_cast<size_t>(c->det_cap) * c->chunk));
+    GT_HIP(hipMalloc(&c->d_counts, sizeof(uint32_t) * c->chunk));
+    GT_HIP(hipHostMalloc(&c->h_stage, 4 * HW * c->chunk, hipHostMallocDefault));
+    GT_HIP(hipHostMalloc(reinterpret_cast<void **>(&c->h_counts), sizeof(uint32_t) * c->chunk,
+                         hipHostMallocDefault));
+    return GT_OK;
+}
+
+void free_all(gt_context *c) {
+    if (c->d_proc) (void)hipFree(c->d_proc);
+    if (c->d_cand) (void)hipFree(c->d_cand);
+    if (c->d_ccount) (void)hipFree(c->d_ccount);
+    if (c->d_frames) (void)hipFree(c->d_frames);
+    if (c->d_rows) (void)hipFree(c->d_rows);
+    if (c->d_counts) (void)hipFree(c->d_counts);
+    if (c->h_stage) (void)hipHostFree(c->h_stage);
+    if (c->h_counts) (void)hipHostFree(c->h_counts);
+    for (hipEvent_t e : c->events) (void)hipEventDestroy(e);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+}
+
+// Runs the three pipeline stages over `nframes` device frames in chunks.
+gt_status run_pipeline(gt_context *c, const void *d_frames, int32_t dtype, uint32_t nframes,
+                       gt_detection *d_out, uint32_t cap, uint32_t *d_counts, hipStream_t s) {
+    const size_t HW = static_cast<size_t>(c->H) * c->W;
+    const size_t esz = dtype_size(dtype);
+    GT_HIP(gt_launch_zero_u32(d_counts, nframes, s));
+    if (c->timing) {
+        for (double &v : c->stage_ms) v = 0.0;
+        c->n_timed_chunks = 0;
+    }
+    const uint32_t nchunks = (nframes + c->chunk - 1) / c->chunk;
+    if (c->timing && c->events.size() < 4u * nchunks) {
+        while (c->events.size() < 4u * nchunks) {
+            hipEvent_t e;
+            GT_HIP(hipEventCreate(&e));
+            c->events.push_back(e);
+        }
+    }
+    for (uint32_t ci = 0; ci < nchunks; ++ci) {
+        const uint32_t f0 = ci * c->chunk;
+        const uint32_t nf = std::min(c->chunk, nframes - f0);
+        const char *frames = static_cast<const char *>(d_frames) + esz * HW * f0;
+        hipEvent_t *ev = c->timing ? &c->events[4 * ci] : nullptr;
+        GT_HIP(gt_launch_zero_u32(c->d_ccount, nf, s));
+        if (ev) GT_HIP(hipEventRecord(ev[0], s));
+        GT_HIP(gt_launch_bandpass(frames, dtype, c->d_proc, nf, c->kp, s));
+        if (ev) GT_HIP(hipEventRecord(ev[1], s));
+        GT_HIP(gt_launch_localmax(c->d_proc, nf, c->kp, c->d_cand, c->cand_cap, c->d_ccount, s));
+        if (ev) GT_HIP(hipEventRecord(ev[2], s));
+        GT_HIP(gt_launch_walk(c->d_proc, frames, dtype, nf, c->kp, c->d_cand, c->cand_cap,
+                              c->d_ccount, d_out + static_cast<size_t>(f0) * cap, cap,
+                              d_counts + f0, f0, s));
+        if (ev) GT_HIP(hipEventRecord(ev[3], s));
+    }
+    if (c->timing) {
+        c->n_timed_chunks = static_cast<int>(nchunks);
+        c->timed_chunk_frames = std::min(c->chunk, nframes);
+    }
+    return GT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t gt_abi_version(void) { return GT_ABI_VERSION; }
+
+const char *gt_last_error(void) { return g_last_error.c_str(); }
+
+void gt_params_default(gt_params *p) { gt_host_params_default(p); }
+
+uint32_t gt_column_names(const gt_params *params, const char **names, const char **types) {
+    return gt_host_column_names(params, names, types);
+}
+
+gt_status gt_context_create(const gt_params *params, uint32_t nrows, uint32_t ncols,
+                            int32_t characterize_points, gt_context **out) {
+    if (!params || !out) return fail(GT_ERR_INVALID_ARGUMENT, "null argument");
+    *out = nullptr;
+    if (nrows == 0 || ncols == 0)
+        return fail(GT_ERR_INVALID_ARGUMENT, "frame dimensions must be positive");
+    if (static_cast<uint64_t>(nrows) * ncols > 0xFFFFFFFFull)
+        return fail(GT_ERR_INVALID_ARGUMENT, "frames above 2^32 pixels are not supported");
+    gt_derived der;
+    std::string why;
+    if (!gt_host_derive(params, nrows, ncols, &der, &why))
+        return fail(GT_ERR_INVALID_ARGUMENT, "%s", why.c_str());
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+        return fail(GT_ERR_GPU_ADAPTER, "no HIP device is available");
+    gt_context *c = new (std::nothrow) gt_context();
+    if (!c) return fail(GT_ERR_GPU_DEVICE, "out of host memory");
+    c->params = *params;
+    c->H = nrows;
+    c->W = ncols;
+    c->characterize_points = characterize_points;
+    c->der = der;
+    gt_host_fill_kparams(&der, nrows, ncols, &c->kp);
+    if (hipGetDevice(&c->device) != hipSuccess) c->device = 0;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return fail(GT_ERR_GPU_DEVICE, "hipStreamCreate failed");
+    }
+    const size_t HW = static_cast<size_t>(nrows) * ncols;
+    // Frames per chunk: bounded by a 4 GiB scratch budget and by 64 frames.
+    const size_t per_frame = HW * (4 + 4 + 4 + sizeof(gt_detection));
+    size_t chunk = (size_t(4) << 30) / (per_frame ? per_frame : 1);
+    c->chunk = static_cast<uint32_t>(std::max<size_t>(1, std::min<size_t>(chunk, 64)));
+    c->cand_cap = static_cast<uint32_t>(HW);
+    c->det_cap = static_cast<uint32_t>(HW);
+    gt_status st = alloc_scratch(c);
+    if (st != GT_OK) {
+        std::string msg = g_last_error;
+        free_all(c);
+        delete c;
+        return fail(st, "%s", msg.c_str());
+    }
+    *out = c;
+    return GT_OK;
+}
+
+void gt_context_destroy(gt_context *c) {
+    if (!c) return;
+    (void)hipStreamSynchronize(c->stream);
+    free_all(c);
+    delete c;
+}
+
+gt_status gt_detect_device(gt_context *c, const void *d_frames, int32_t dtype, uint32_t nframes,
+                           gt_detection *d_out, uint32_t cap_per_frame, uint32_t *d_counts,
+                           void *stream) {
+    if (!c) return fail(GT_ERR_INVALID_ARGUMENT, "null context");
+    if (!dtype_size(dtype)) return fail(GT_ERR_INVALID_ARGUMENT, "unknown dtype %d", dtype);
+    if (c->der.style != GT_STYLE_TRACKPY)
+        return fail(GT_ERR_INVALID_ARGUMENT, "gt_detect_device runs the Trackpy style only");
+    if (nframes == 0) return fail(GT_ERR_EMPTY_ITERATOR, "the provided video doesn't contain any frames.");
+    if (!d_frames || !d_out || !d_counts || cap_per_frame == 0)
+        return fail(GT_ERR_INVALID_ARGUMENT, "null device buffer or zero capacity");
+    hipStream_t s = stream ? static_cast<hipStream_t>(stream) : c->stream;
+    return run_pipeline(c, d_frames, dtype, nframes, d_out, cap_per_frame, d_counts, s);
+}
+
+gt_status gt_set_timing(gt_context *c, int32_t enable) {
+    if (!c) return fail(GT_ERR_INVALID_ARGUMENT, "null context");
+    c->timing = enable != 0;
+    return GT_OK;
+}
+
+int32_t gt_stage_times(gt_context *c, double *stage_ms, int32_t n, uint32_t *launches,
+                       uint32_t *frames_per_launch) {
+    if (!c || !c->timing || c->n_timed_chunks == 0) return 0;
+    if (launches) *launches = static_cast<uint32_t>(c->n_timed_chunks);
+    if (frames_per_launch) *frames_per_launch = c->timed_chunk_frames;
+    double acc[3] = {0, 0, 0};
+    for (int ci = 0; ci < c->n_timed_chunks; ++ci) {
+        hipEvent_t *ev = &c->events[4 * ci];
+        if (hipEventSynchronize(ev[3]) != hipSuccess) return 0;
+        for (int k = 0; k < 3; ++k) {
+            float ms = 0.0f;
+            if (hipEventElapsedTime(&ms, ev[k], ev[k + 1]) != hipSuccess) return 0;
+            acc[k] += ms;
+        }
+    }
+    const int m = std::min<int>(n, 3);
+    for (int k = 0; k < m; ++k) stage_ms[k] = acc[k];
+    return m;
+}
+
+gt_status gt_set_interrupt(gt_context *c, int32_t value) {
+    if (!c) return fail(GT_ERR_INVALID_ARGUMENT, "null context");
+    c->interrupt.store(value);
+    return GT_OK;
+}
+
+gt_status gt_progress(gt_context *c, uint64_t *cur, uint64_t *total) {
+    if (!c) return fail(GT_ERR_INVALID_ARGUMENT, "null context");
+    if (cur) *cur = c->prog_cur.load();
+    if (total) *total = c->prog_total.load();
+    return GT_OK;
+}
+
+void gt_output_free(gt_output *out) {
+    if (!out) return;
+    free(out->data);
+    out->data = nullptr;
+    out->nrows = 0;
+}
+
+gt_status gt_execute(gt_context *c, const void *frames, int32_t dtype, uint64_t nframes,
+                     const uint64_t *keys, uint64_t nkeys, const float *points, uint64_t npoints,
+                     uint32_t point_cols, int32_t points_has_frames, int32_t points_has_r,
+                     const uint64_t *linker_reset_points, uint64_t nreset, gt_output *out) {
+    if (!c || !out) return fail(GT_ERR_INVALID_ARGUMENT, "null argument");
+    out->data = nullptr;
+    out->nrows = 0;
+    out->ncols = 0;
+    if (!dtype_size(dtype)) return fail(GT_ERR_INVALID_ARGUMENT, "unknown dtype %d", dtype);
+    if (nframes > 0 && !frames) return fail(GT_ERR_INVALID_ARGUMENT, "null frames");
+    c->interrupt.store(0);
+    gt_status st = alloc_execute_buffers(c);
+    if (st != GT_OK) return st;
+
+    gt_host_plan plan;
+    std::string why;
+    gt_status pst = gt_host_plan_frames(c->characterize_points, nframes, keys, nkeys, points,
+                                        npoints, point_cols, points_has_frames, points_has_r,
+                                        &plan, &why);
+    if (pst != GT_OK) return fail(pst, "%s", why.c_str());
+    if (plan.frames.empty()) return fail(GT_ERR_EMPTY_ITERATOR, "the provided video doesn't contain any frames.");
+    if (c->characterize_points && !points)
+        return fail(GT_ERR_INVALID_ARGUMENT, "context was created for characterize_points but no points were given");
+    if (!c->characterize_points && points)
+        return fail(GT_ERR_INVALID_ARGUMENT, "points given to a context created without characterize_points");
+
+    const size_t HW = static_cast<size_t>(c->H) * c->W;
+    const size_t esz = dtype_size(dtype);
+    gt_host_post post(&c->params, &c->der, c->H, c->W, linker_reset_points, nreset);
+    if (!post.ok()) return fail(GT_ERR_INVALID_ARGUMENT, "%s", post.error().c_str());
+    c->prog_total.store(plan.frames.size());
+    c->prog_cur.store(0);
+
+    std::vector<gt_detection> rows;
+    std::vector<float> rawf;
+    for (size_t base = 0; base < plan.frames.size(); base += c->chunk) {
+        if (c->interrupt.load()) return fail(GT_ERR_INTERRUPTED, "Keyboard Interrupt, my custom version");
+        const uint32_t nf = static_cast<uint32_t>(std::min<size_t>(c->chunk, plan.frames.size() - base));
+        for (uint32_t f = 0; f < nf; ++f) {
+            const uint64_t idx = plan.frames[base + f];
+            if (idx >= nframes)
+                return fail(GT_ERR_FRAME_OUT_OF_BOUNDS,
+                            "Requested a frame outside the length\n of the video. Video length is %llu, requested frame %llu",
+                            static_cast<unsigned long long>(nframes), static_cast<unsigned long long>(idx));
+            std::memcpy(static_cast<char *>(c->h_stage) + esz * HW * f,
+                        static_cast<const char *>(frames) + esz * HW * idx, esz * HW);
+        }
+        GT_HIP(hipMemcpyAsync(c->d_frames, c->h_stage, esz * HW * nf, hipMemcpyHostToDevice, c->stream));
+        if (c->characterize_points) {
+            st = gt_host_characterize_points(c, base, nf, dtype, plan);
+            if (st != GT_OK) return st;
+        } else {
+            st = run_pipeline(c, c->d_frames, dtype, nf, c->d_rows, c->det_cap, c->d_counts, c->stream);
+            if (st != GT_OK) return st;
+        }
+        GT_HIP(hipMemcpyAsync(c->h_counts, c->d_counts, sizeof(uint32_t) * nf, hipMemcpyDeviceToHost, c->stream));
+        GT_HIP(hipStreamSynchronize(c->stream));
+        for (uint32_t f = 0; f < nf; ++f) {
+            const uint32_t n = c->h_counts[f];
+            if (n > c->det_cap)
+                return fail(GT_ERR_CAPACITY, "frame %llu produced %u detections, above the capacity of %u",
+                            static_cast<unsigned long long>(plan.frames[base + f]), n, c->det_cap);
+            rows.resize(n);
+            if (n)
+                GT_HIP(hipMemcpy(rows.data(), c->d_rows + static_cast<size_t>(f) * c->det_cap,
+                                 sizeof(gt_detection) * n, hipMemcpyDeviceToHost));
+            const char *raw = static_cast<const char *>(frames) + esz * HW * plan.frames[base + f];
+            const float *rawp = nullptr;
+            if (post.needs_raw()) {
+                rawf.resize(HW);
+                gt_host_widen(raw, dtype, HW, rawf.data());
+                rawp = rawf.data();
+            }
+            std::string perr;
+            gt_status pst2 = post.process(rows, plan.frames[base + f], rawp, &c->interrupt, &perr);
+            if (pst2 != GT_OK) return fail(pst2, "%s", perr.c_str());
+            c->prog_cur.fetch_add(1);
+        }
+    }
+    std::string ferr;
+    gt_status fst = post.finish(out, &ferr);
+    if (fst != GT_OK) return fail(fst, "%s", ferr.c_str());
+    return GT_OK;
+}
+
+}  // extern "C"
+
+// Used by gt_host.cpp for characterize_points: upload the positions of the
+// chunk's frames and run characterize only.
+gt_status gt_host_characterize_points(gt_context *c, size_t base, uint32_t nf, int32_t dtype,
+                                      const gt_host_plan &plan) {
+    (void)c; (void)base; (void)nf; (void)dtype; (void)plan;
+    return fail(GT_ERR_INVALID_ARGUMENT, "characterize_points is not available yet");
+}

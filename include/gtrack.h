@@ -1,0 +1,230 @@
+/*
+ * gtrack.h -- C ABI of the MI355X (gfx950) detection pipeline.
+ *
+ * This is the drop-in boundary for gpu_tracking's per-frame detection path
+ * (bandpass -> grey-dilation local max -> centroid walk -> characterize, and
+ * the host post-processing that follows it).  Plain C: pointers, sizes and
+ * POD structs only.  Each declaration names the reference interface it
+ * replaces (paths relative to the gpu_tracking repository).
+ *
+ * Threading: a gt_context is used by one host thread at a time; only
+ * gt_set_interrupt() and gt_progress() may be called from another thread
+ * (reference: GpuState is !Send, interruption is an Arc<AtomicBool>,
+ * src/gpu_setup.rs:502-514, src/execute_gpu.rs:886-895).
+ * Ownership: inputs and outputs belong to the caller; the library owns its
+ * device buffers and pinned staging and keeps no caller pointer after a call
+ * returns (gt_output memory is released with gt_output_free).
+ */
+#ifndef GTRACK_H
+#define GTRACK_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GT_ABI_VERSION 1
+
+/* One code per variant of the reference's error enum (src/error.rs:6-94),
+ * in declaration order, plus two codes that have no reference counterpart. */
+typedef enum gt_status {
+    GT_OK = 0,
+    GT_ERR_GPU_ADAPTER = 1,              /* Error::GpuAdapterError: no HIP device */
+    GT_ERR_GPU_DEVICE = 2,               /* Error::GpuDeviceError: HIP runtime failure */
+    GT_ERR_FILE_NOT_FOUND = 3,           /* Error::FileNotFound */
+    GT_ERR_DIMENSION_MISMATCH = 4,       /* Error::DimensionMismatch */
+    GT_ERR_EMPTY_ITERATOR = 5,           /* Error::EmptyIterator */
+    GT_ERR_NON_SORTED_CHARACTERIZATION = 6, /* Error::NonSortedCharacterization */
+    GT_ERR_FRAME_OUT_OF_BOUNDS = 7,      /* Error::FrameOutOfBounds */
+    GT_ERR_FRAME_OOB = 8,                /* Error::FrameOOB */
+    GT_ERR_THREAD = 9,                   /* Error::ThreadError */
+    GT_ERR_NON_STANDARD_ARRAY_LAYOUT = 10, /* Error::NonStandardArrayLayout */
+    GT_ERR_NO_EXTENSION = 11,            /* Error::NoExtensionError */
+    GT_ERR_UNSUPPORTED_FILEFORMAT = 12,  /* Error::UnsupportedFileformat */
+    GT_ERR_ARRAY_DIMENSIONS = 13,        /* Error::ArrayDimensionsError */
+    GT_ERR_CAST = 14,                    /* Error::CastError */
+    GT_ERR_READ = 15,                    /* Error::ReadError */
+    GT_ERR_INVALID_FILENAME = 16,        /* Error::InvalidFileName */
+    GT_ERR_CHANNEL_NOT_FOUND = 17,       /* Error::ChannelNotFound */
+    GT_ERR_TOO_DENSE_TO_LINK = 18,       /* Error::TooDenseToLink */
+    GT_ERR_INTERRUPTED = 19,             /* Error::Interrupted */
+    GT_ERR_POLLED_AFTER_TERMINATION = 20, /* Error::PolledAfterTermination */
+    GT_ERR_TIFF_WRITE = 21,              /* Error::TiffWrite */
+    GT_ERR_WRONG_BUILDER = 22,           /* Error::WrongBuilder */
+    GT_ERR_INVALID_ARGUMENT = 100,       /* no counterpart: rejected argument */
+    GT_ERR_CAPACITY = 101                /* no counterpart: a frame exceeded the
+                                            candidate/detection capacity (the
+                                            reference writes past its buffers) */
+} gt_status;
+
+/* ParamStyle (src/gpu_setup.rs:10-33) */
+typedef enum gt_style { GT_STYLE_TRACKPY = 0, GT_STYLE_LOG = 1 } gt_style;
+
+/* Pixel type of the frames handed in.  The reference casts every frame to f32
+ * on the host (src/decoderiter.rs:16-19); here u16/u8 frames go to the GPU as
+ * they are and are widened in the kernel (exact). */
+typedef enum gt_dtype { GT_U16 = 0, GT_F32 = 1, GT_U8 = 2 } gt_dtype;
+
+/* TrackingParams + ParamStyle, field for field (src/gpu_setup.rs:10-59).
+ * Option<T> fields are a has_* flag plus the value.  keys/frames and
+ * linker_reset_points are arrays passed to gt_execute instead. */
+typedef struct gt_params {
+    int32_t style;                    /* gt_style */
+    /* ParamStyle::Trackpy */
+    uint32_t separation;
+    uint32_t diameter;
+    float maxsize;                    /* accepted, unused (as in the reference) */
+    float threshold;                  /* accepted, unused */
+    int32_t invert;                   /* accepted, unused */
+    float percentile;                 /* accepted, unused */
+    uint32_t topn;                    /* accepted, unused */
+    int32_t preprocess;               /* accepted, unused */
+    int32_t filter_close;
+    /* ParamStyle::Log */
+    float min_radius;
+    float max_radius;
+    uint32_t n_radii;
+    int32_t log_spacing;
+    float overlap_threshold;
+    /* common */
+    float minmass;
+    uint32_t max_iterations;
+    int32_t characterize;
+    int32_t has_search_range;  float search_range;
+    int32_t has_memory;        uint32_t memory;
+    int32_t doughnut_correction;
+    int32_t has_bg_radius;     float bg_radius;
+    int32_t has_gap_radius;    float gap_radius;
+    int32_t has_snr;           float snr;
+    int32_t has_minmass_snr;   float minmass_snr;
+    int32_t truncate_preprocessed;
+    int32_t has_illumination_sigma; float illumination_sigma;
+    int32_t has_adaptive_background; uint32_t adaptive_background;
+    int32_t include_r_in_output;
+    float shift_threshold;
+    float noise_size;
+    int32_t has_smoothing_size; uint32_t smoothing_size;
+    int32_t illumination_correction_per_frame;
+} gt_params;
+
+/* ResultRow (src/execute_gpu.rs:48-61): 10 x f32, x = row, y = column. */
+typedef struct gt_result_row {
+    float x, y, mass, r, max_intensity, Rg, raw_mass, signal, ecc, count;
+} gt_result_row;
+
+/* A detection as the device path emits it: the reference's ResultRow plus
+ * the raster index (row * ncols + col) of the local maximum the walk started
+ * from, which orders detections deterministically, and the frame index. */
+typedef struct gt_detection {
+    gt_result_row row;
+    uint32_t seed;
+    uint32_t frame;
+} gt_detection;
+
+typedef struct gt_context gt_context;
+
+/* Whole-video result (execute_gpu's (Vec<f32>, Vec<(&str,&str)>) return,
+ * src/execute_gpu.rs:886-895, with column_names at :482-519). */
+typedef struct gt_output {
+    float *data;               /* nrows x ncols, row-major */
+    uint64_t nrows;
+    uint32_t ncols;
+    const char *const *col_names;  /* static strings */
+    const char *const *col_types;  /* "int" or "float" */
+} gt_output;
+
+/* TrackingParams::default() (src/gpu_setup.rs:61-97). */
+void gt_params_default(gt_params *p);
+
+/* ABI version of the loaded library (GT_ABI_VERSION). */
+int32_t gt_abi_version(void);
+
+/* Message for the last error raised on the calling thread (never NULL). */
+const char *gt_last_error(void);
+
+/* setup_state(&TrackingParams, &[u32;2] dims, characterize_new_points, ..)
+ * (src/gpu_setup.rs:611-616): validates and derives the parameters
+ * (gpuparams_from_tracking_params, :528-580), selects the HIP device that is
+ * current on the calling thread and creates the context's stream. */
+gt_status gt_context_create(const gt_params *params, uint32_t nrows, uint32_t ncols,
+                            int32_t characterize_points, gt_context **out);
+void gt_context_destroy(gt_context *ctx);
+
+/* The device-resident hot path: the reference's per-frame GPU pass sequence
+ * (submit_work's order preprocess_rows, preprocess_cols, max_rows,
+ * extract_max, walk, characterize -- src/execute_gpu.rs:231-235,368-375)
+ * over nframes frames already in device memory.
+ *   d_frames: nframes * nrows * ncols pixels of `dtype`, contiguous.
+ *   d_out:    nframes * cap_per_frame gt_detection slots; frame f's rows go
+ *             to d_out[f * cap_per_frame ...] in unspecified order.
+ *   d_counts: nframes uint32; d_counts[f] = number of detections of frame f
+ *             (it may exceed cap_per_frame, in which case only cap_per_frame
+ *             rows were written and the caller must treat it as an error).
+ *   stream:   hipStream_t to enqueue on (NULL = the context's own stream).
+ * Asynchronous: returns after enqueueing.  No per-frame D2H copy. */
+gt_status gt_detect_device(gt_context *ctx, const void *d_frames, int32_t dtype,
+                           uint32_t nframes, gt_detection *d_out, uint32_t cap_per_frame,
+                           uint32_t *d_counts, void *stream);
+
+/* execute_gpu over an in-memory video (execute_ndarray -> execute_provider ->
+ * execute_gpu, src/execute_gpu.rs:1378,1272,886): frames are staged through
+ * pinned host memory to the GPU in chunks, detected, read back (counts and
+ * rows only) and post-processed on a host thread (filter_close, doughnut
+ * photometry, column assembly; src/execute_gpu.rs:671-817).
+ *   frames:  nframes * nrows * ncols pixels of `dtype`, contiguous (host).
+ *   keys:    optional list of frame indices to process, in order (the
+ *            reference's `keys`/`frames`, :1020-1056); NULL = all frames.
+ *   points:  characterize_points input, npoints rows of point_cols floats
+ *            laid out [frame?, y, x, r?] (src/linking.rs:94-196); NULL
+ *            otherwise.  Requires a context created with
+ *            characterize_points = 1.
+ * On success *out holds the output matrix; release it with gt_output_free. */
+gt_status gt_execute(gt_context *ctx, const void *frames, int32_t dtype, uint64_t nframes,
+                     const uint64_t *keys, uint64_t nkeys, const float *points,
+                     uint64_t npoints, uint32_t point_cols, int32_t points_has_frames,
+                     int32_t points_has_r, const uint64_t *linker_reset_points,
+                     uint64_t nreset, gt_output *out);
+void gt_output_free(gt_output *out);
+
+/* Column names and types for a parameter set (column_names,
+ * src/execute_gpu.rs:482-519).  Returns the column count; the arrays point
+ * to static strings and hold at most 16 entries. */
+uint32_t gt_column_names(const gt_params *params, const char **names, const char **types);
+
+/* filter_close_trackpy (src/execute_gpu.rs:556-590) over one frame's rows:
+ * keep[i] = 0 when another row j lies strictly closer than `separation`
+ * (squared distance < separation^2, src/linking.rs:238-243) and has a larger
+ * mass, or the same mass and j < i.  Host-only, uniform-grid neighbour search. */
+gt_status gt_filter_close(const gt_result_row *rows, uint64_t n, float separation,
+                          uint8_t *keep);
+
+/* Doughnut photometry of post_process (src/execute_gpu.rs:717-790) with the
+ * stencils circle_inds/annulus_inds (src/kernels.rs:138-171): for each row,
+ * out3[3i..3i+2] = raw_mass, raw_bg_median, raw_mass_corrected over the raw
+ * f32 frame.  Host-only. */
+gt_status gt_doughnut(const float *raw, uint32_t nrows, uint32_t ncols, const gt_result_row *rows,
+                      uint64_t n, float radius, float bg_radius, float gap_radius, float *out3);
+
+/* Interruption and progress (the Arc<AtomicBool> / Arc<Mutex<(usize,
+ * Option<usize>)>> pair of execute_gpu, src/execute_gpu.rs:1094-1099,
+ * :974-977).  Safe to call from any thread while gt_execute runs. */
+gt_status gt_set_interrupt(gt_context *ctx, int32_t value);
+gt_status gt_progress(gt_context *ctx, uint64_t *cur, uint64_t *total);
+
+/* Per-stage device time of the most recent gt_detect_device call when timing
+ * is enabled (hipEvents on the launch stream around each stage; adds a few
+ * microseconds per chunk).  stage_ms[0..n) receives the summed time of
+ * bandpass, local max, walk+characterize, in that order; *launches the
+ * number of launches per stage and *frames_per_launch the chunk size.
+ * Returns the number of stages written (0 when nothing was timed).  There is
+ * no reference counterpart (the reference has no GPU timestamps). */
+gt_status gt_set_timing(gt_context *ctx, int32_t enable);
+int32_t gt_stage_times(gt_context *ctx, double *stage_ms, int32_t n, uint32_t *launches,
+                       uint32_t *frames_per_launch);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GTRACK_H */

@@ -1,0 +1,504 @@
+/*
+ * gt_oracle.c -- CPU restatement of gpu_tracking's per-frame detection
+ * pipeline (the oracle).  TEST INFRASTRUCTURE ONLY; see gt_oracle.h.
+ *
+ * Build: gcc -O2 -ffp-contract=off (no implicit contraction; the explicit
+ * fmaf calls are the only fused operations, matching the HIP kernels).
+ */
+#include "gt_oracle.h"
+
+#include <math.h>
+#include <pthread.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* WGSL `1./255.` as an f32 constant (preprocess_cols.wgsl:54). */
+#define OR_TRUNC_THRESHOLD 0.003921568859368562698364257812f /* 0x3B808081 */
+
+static inline int32_t or_f2i(float x) {
+    /* f32 -> i32 with NaN -> 0 and saturation. */
+    if (x != x) return 0;
+    if (x >= 2147483648.0f) return INT32_MAX;
+    if (x <= -2147483648.0f) return INT32_MIN;
+    return (int32_t)x;
+}
+
+static inline float or_max(float a, float b) { return b > a ? b : a; }
+
+void or_derive_trackpy(uint32_t diameter, uint32_t separation, int32_t smoothing_size,
+                       float noise_size, float minmass, uint32_t max_iterations,
+                       float shift_threshold, float snr, float minmass_snr,
+                       int adaptive_background, int truncate, int32_t nrows,
+                       int32_t ncols, or_params *out) {
+    /* gpu_setup.rs:536-547: smoothing_size defaults to diameter; dilation side
+     * is (2*sep/sqrt(2)) as u32; margin = max(d/2, sep/2 - 1, s/2) in u32
+     * arithmetic (sep/2 - 1 wraps for sep < 2) cast to i32. */
+    uint32_t s = smoothing_size > 0 ? (uint32_t)smoothing_size : diameter;
+    float dil = 2.0f * (float)separation / sqrtf(2.0f);
+    uint32_t k = dil <= 0.0f ? 0u : (dil >= 4294967296.0f ? 0xFFFFFFFFu : (uint32_t)dil);
+    uint32_t m0 = diameter / 2u, m1 = separation / 2u - 1u, m2 = s / 2u;
+    uint32_t m = m0;
+    if (m1 > m) m = m1;
+    if (m2 > m) m = m2;
+    memset(out, 0, sizeof(*out));
+    out->nrows = nrows;
+    out->ncols = ncols;
+    out->smoothing = (int32_t)s;
+    out->sigma2 = noise_size;
+    out->circle = (int32_t)diameter;
+    out->dilation = (int32_t)k;
+    out->max_iterations = max_iterations;
+    out->threshold = shift_threshold;
+    out->minmass = minmass;
+    out->margin = (int32_t)m;
+    out->snr = snr;
+    out->minmass_snr = minmass_snr;
+    out->rough_snr_factor = adaptive_background ? 0.5f : 1.0f;
+    out->image_std = 0.0f;
+    out->truncate = truncate;
+}
+
+void or_gauss_weights(int32_t s, float sigma2, float *w, float *norm) {
+    /* preprocess_rows.wgsl:25,30-47: x = f32(i); exp(-x*x / (2*sigma2)). */
+    int32_t r = s / 2;
+    float n = 0.0f;
+    for (int32_t i = -r; i <= r; i++) {
+        float x = (float)i;
+        float g = expf(-x * x / (2.0f * sigma2));
+        w[i + r] = g;
+        n = n + g;
+    }
+    *norm = n;
+}
+
+void or_bandpass(const float *frame, const or_params *p, float *processed, float *scratch) {
+    const int32_t H = p->nrows, W = p->ncols, s = p->smoothing, r = s / 2;
+    float w[1024];
+    float norm;
+    or_gauss_weights(s, p->sigma2, w, &norm);
+    float *tg = scratch, *tc = scratch + (size_t)H * W;
+    /* Row pass (preprocess_rows.wgsl:21-54): Gaussian taps skip out-of-bounds
+     * columns but the norm keeps every tap; boxcar clamps to the edge. */
+    for (int32_t u = 0; u < H; u++) {
+        const float *row = frame + (size_t)u * W;
+        for (int32_t v = 0; v < W; v++) {
+            float gs = 0.0f, cs = 0.0f;
+            for (int32_t i = -r; i <= r; i++) {
+                int32_t pv = v + i;
+                int inb = 1;
+                if (pv < 0) { pv = 0; inb = 0; }
+                else if (pv >= W) { pv = W - 1; inb = 0; }
+                if (inb) gs = fmaf(w[i + r], row[pv], gs);
+                cs = cs + row[pv];
+            }
+            tg[(size_t)u * W + v] = gs / norm;
+            tc[(size_t)u * W + v] = cs;
+        }
+    }
+    /* Column pass (preprocess_cols.wgsl:18-59). */
+    const float box_area = (float)(s * s);
+    for (int32_t u = 0; u < H; u++) {
+        for (int32_t v = 0; v < W; v++) {
+            float gs = 0.0f, cs = 0.0f;
+            for (int32_t i = -r; i <= r; i++) {
+                int32_t pu = u + i;
+                int inb = 1;
+                if (pu < 0) { pu = 0; inb = 0; }
+                else if (pu >= H) { pu = H - 1; inb = 0; }
+                if (inb) gs = fmaf(w[i + r], tg[(size_t)pu * W + v], gs);
+                cs = cs + tc[(size_t)pu * W + v];
+            }
+            float res = gs / norm - cs / box_area;
+            if (p->truncate && res < OR_TRUNC_THRESHOLD) res = 0.0f;
+            processed[(size_t)u * W + v] = res;
+        }
+    }
+}
+
+size_t or_local_max(const float *processed, const or_params *p, uint32_t *cand, size_t cap,
+                    float *M) {
+    const int32_t H = p->nrows, W = p->ncols, k = p->dilation;
+    /* window start: i32(f32(k)/2 - 0.5) (max_rows.wgsl:21, extract_max.wgsl:40) */
+    const int32_t a = or_f2i((float)k / 2.0f - 0.5f);
+    for (int32_t u = 0; u < H; u++) {
+        for (int32_t v = 0; v < W; v++) {
+            float m = processed[(size_t)u * W + v];
+            int32_t pv = v - a;
+            for (int32_t j = 0; j < k; j++, pv++)
+                if (pv >= 0 && pv < W) m = or_max(m, processed[(size_t)u * W + pv]);
+            M[(size_t)u * W + v] = m;
+        }
+    }
+    const float thr = p->image_std * p->snr * p->rough_snr_factor;
+    size_t n = 0;
+    for (int32_t u = 0; u < H; u++) {
+        for (int32_t v = 0; v < W; v++) {
+            if (u < p->margin || u >= H - p->margin || v < p->margin || v >= W - p->margin)
+                continue;
+            const float c = processed[(size_t)u * W + v];
+            if (c < thr) continue;
+            int ismax = 1;
+            int32_t pu = u - a;
+            for (int32_t i = 0; i < k; i++, pu++) {
+                if (pu >= 0 && pu < H && M[(size_t)pu * W + v] > c) { ismax = 0; break; }
+            }
+            if (!ismax) continue;
+            if (n < cap) cand[n] = (uint32_t)((size_t)u * W + v);
+            n++;
+        }
+    }
+    return n;
+}
+
+typedef struct { float x, y, mass, max, count; } or_walkres;
+
+/* get_center (walk.wgsl:67-102) */
+static or_walkres or_get_center(const float *P, const or_params *p, int32_t u, int32_t v,
+                                int32_t rint, float r2) {
+    or_walkres res = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    for (int32_t i = -rint; i <= rint; i++) {
+        const float x = (float)i, x2 = x * x;
+        for (int32_t j = -rint; j <= rint; j++) {
+            const float y = (float)j, y2 = y * y;
+            if (x2 + y2 > r2) continue;
+            const int32_t pu = u + i, pv = v + j;
+            if (pu < 0 || pu >= p->nrows || pv < 0 || pv >= p->ncols) continue;
+            const float d = P[(size_t)pu * p->ncols + pv];
+            res.x = fmaf(d, x, res.x);
+            res.y = fmaf(d, y, res.y);
+            res.mass = res.mass + d;
+            res.count = res.count + 1.0f;
+            res.max = or_max(res.max, d);
+        }
+    }
+    res.x = res.x / res.mass;
+    res.y = res.y / res.mass;
+    return res;
+}
+
+size_t or_walk(const float *P, const or_params *p, const uint32_t *cand, size_t ncand,
+               or_row *rows) {
+    /* walk.wgsl:104-169 with r = -1 (rint = (circle-1)/2), then main :171-229 */
+    const int32_t rint = (p->circle - 1) / 2;
+    const float rf = (float)rint, r2 = rf * rf;
+    const int32_t margin = rint;
+    size_t n = 0;
+    for (size_t c = 0; c < ncand; c++) {
+        int32_t pu = (int32_t)(cand[c] / (uint32_t)p->ncols);
+        int32_t pv = (int32_t)(cand[c] % (uint32_t)p->ncols);
+        or_walkres res = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+        for (uint32_t it = 0; it < p->max_iterations; it++) {
+            res = or_get_center(P, p, pu, pv, rint, r2);
+            int changed = 0;
+            if (res.x > p->threshold && pu < p->nrows - 1 - margin) { pu += 1; changed = 1; }
+            else if (res.x < -p->threshold && pu > margin) { pu -= 1; changed = 1; }
+            if (res.y > p->threshold && pv < p->ncols - 1 - margin) { pv += 1; changed = 1; }
+            else if (res.y < -p->threshold && pv > margin) { pv -= 1; changed = 1; }
+            if (!changed) break;
+        }
+        res.x = res.x + (float)pu;
+        res.y = res.y + (float)pv;
+        float mm;
+        if (p->minmass > 0.0f) mm = p->minmass;
+        else mm = p->minmass_snr * res.count * p->image_std * p->rough_snr_factor;
+        if (res.mass >= mm) {
+            or_row *o = &rows[n++];
+            memset(o, 0, sizeof(*o));
+            o->x = res.x;
+            o->y = res.y;
+            o->mass = res.mass;
+            o->count = res.count;
+            o->max_intensity = res.max;
+            o->seed = cand[c];
+        }
+    }
+    return n;
+}
+
+void or_characterize(const float *P, const float *raw, const or_params *p, or_row *rows,
+                     size_t n, int use_f64, int recompute_mass) {
+    const int32_t H = p->nrows, W = p->ncols;
+    for (size_t k = 0; k < n; k++) {
+        or_row *row = &rows[k];
+        /* characterize.wgsl:62-77 */
+        float r = row->r;
+        int32_t rint;
+        if (r <= 0.0f) { rint = (p->circle - 1) / 2; r = (float)rint; }
+        else rint = or_f2i(ceilf(r));
+        const float r2 = r * r;
+        const int32_t u = or_f2i(nearbyintf(row->x)), v = or_f2i(nearbyintf(row->y));
+        if (recompute_mass) {
+            /* get_mass (characterize.wgsl:30-58) */
+            float m = 0.0f;
+            for (int32_t i = -rint; i <= rint; i++) {
+                const float x = (float)i, x2 = x * x;
+                for (int32_t j = -rint; j <= rint; j++) {
+                    const float y = (float)j, y2 = y * y;
+                    if (x2 + y2 > r2) continue;
+                    const int32_t pu = u + i, pv = v + j;
+                    if (pu < 0 || pu >= H || pv < 0 || pv >= W) continue;
+                    m = m + P[(size_t)pu * W + pv];
+                }
+            }
+            row->mass = m;
+        }
+        const float mass = row->mass;
+        float rg32 = 0.0f, raw_mass = 0.0f, signal = 0.0f, es32 = 0.0f, ec32 = 0.0f;
+        double rg64 = 0.0, es64 = 0.0, ec64 = 0.0;
+        for (int32_t i = -rint; i <= rint; i++) {
+            const float x = (float)i, x2 = x * x;
+            for (int32_t j = -rint; j <= rint; j++) {
+                const float y = (float)j, y2 = y * y;
+                const float mask = x2 + y2;
+                if (mask > r2) continue;
+                const int32_t pu = u + i, pv = v + j;
+                if (pu < 0 || pu >= H || pv < 0 || pv >= W) continue;
+                const size_t idx = (size_t)pu * W + pv;
+                const float d = P[idx];
+                raw_mass = raw_mass + raw[idx];
+                signal = or_max(signal, d);
+                if (i != 0 || j != 0) {
+                    /* sin(2 atan2(y,x)) = 2xy/(x^2+y^2), cos(2 atan2(y,x)) = (x^2-y^2)/(x^2+y^2) */
+                    const double q = (double)(i * i + j * j);
+                    const double s2 = 2.0 * (double)i * (double)j / q;
+                    const double c2 = ((double)(i * i) - (double)(j * j)) / q;
+                    if (use_f64) {
+                        es64 = fma((double)d, s2, es64);
+                        ec64 = fma((double)d, c2, ec64);
+                    } else {
+                        es32 = fmaf(d, (float)s2, es32);
+                        ec32 = fmaf(d, (float)c2, ec32);
+                    }
+                } else {
+                    if (use_f64) ec64 += (double)d;
+                    else ec32 = ec32 + d;
+                }
+                if (use_f64) rg64 = fma((double)d, (double)mask, rg64);
+                else rg32 = fmaf(d, mask, rg32);
+            }
+        }
+        /* the centre sample is read without a bounds check in the reference
+         * (characterize.wgsl:79,131); clamp so the read stays inside the frame */
+        int32_t cu = u < 0 ? 0 : (u >= H ? H - 1 : u);
+        int32_t cv = v < 0 ? 0 : (v >= W ? W - 1 : v);
+        const float den = mass - P[(size_t)cu * W + cv];
+        if (use_f64) {
+            row->ecc = (float)(sqrt(es64 * es64 + ec64 * ec64) / (double)den);
+            row->Rg = (float)sqrt(rg64 / (double)mass);
+        } else {
+            es32 = es32 * es32;
+            ec32 = ec32 * ec32;
+            row->ecc = sqrtf(es32 + ec32) / den;
+            row->Rg = sqrtf(rg32 / mass);
+        }
+        row->raw_mass = raw_mass;
+        row->signal = signal;
+    }
+}
+
+void or_filter_close(const or_row *rows, size_t n, float separation, uint8_t *keep) {
+    const float r2 = separation * separation;
+    for (size_t i = 0; i < n; i++) {
+        keep[i] = 1;
+        const float mi = rows[i].mass;
+        for (size_t j = 0; j < n; j++) {
+            if (j == i) continue;
+            const float dx = rows[j].x - rows[i].x, dy = rows[j].y - rows[i].y;
+            float d2 = 0.0f;
+            d2 = d2 + dx * dx;
+            d2 = d2 + dy * dy;
+            if (!(d2 < r2)) continue;
+            const float mj = rows[j].mass;
+            if (mi < mj || (mi == mj && i > j)) { keep[i] = 0; break; }
+        }
+    }
+}
+
+static int or_cmp_float(const void *a, const void *b) {
+    float x = *(const float *)a, y = *(const float *)b;
+    return (x > y) - (x < y);
+}
+
+static inline int32_t or_round_away(float x) { return or_f2i(roundf(x)); }
+
+void or_doughnut(const float *raw, int32_t H, int32_t W, const or_row *rows, size_t n,
+                 float radius, float bg_radius, float gap_radius, float *out3) {
+    /* circle_inds(radius) and annulus_inds(bg_radius, gap_radius + radius)
+     * (kernels.rs:138-171); the row's centre is rounded half away from zero
+     * (execute_gpu.rs:752); out-of-frame samples are skipped (:754-774). */
+    const int32_t cb = or_f2i(ceilf(radius));
+    const float rr = radius * radius;
+    const float inner = gap_radius + radius;
+    const int32_t ab = or_f2i(ceilf(bg_radius));
+    const float or2 = bg_radius * bg_radius, ir2 = inner * inner;
+    size_t bgcap = (size_t)(2 * ab + 1) * (size_t)(2 * ab + 1);
+    float *bg = (float *)malloc(sizeof(float) * (bgcap ? bgcap : 1));
+    for (size_t k = 0; k < n; k++) {
+        const int32_t mu = or_round_away(rows[k].x), mv = or_round_away(rows[k].y);
+        float sig = 0.0f;
+        size_t nsig = 0, nbg = 0;
+        for (int32_t i = -cb; i <= cb; i++)
+            for (int32_t j = -cb; j <= cb; j++) {
+                if (!((float)(i * i + j * j) <= rr)) continue;
+                int64_t pu = (int64_t)mu + i, pv = (int64_t)mv + j;
+                if (pu < 0 || pu >= H || pv < 0 || pv >= W) continue;
+                sig = sig + raw[pu * W + pv];
+                nsig++;
+            }
+        for (int32_t i = -ab; i <= ab; i++)
+            for (int32_t j = -ab; j <= ab; j++) {
+                const float q = (float)(i * i + j * j);
+                if (!(q <= or2 && q >= ir2)) continue;
+                int64_t pu = (int64_t)mu + i, pv = (int64_t)mv + j;
+                if (pu < 0 || pu >= H || pv < 0 || pv >= W) continue;
+                bg[nbg++] = raw[pu * W + pv];
+            }
+        qsort(bg, nbg, sizeof(float), or_cmp_float);
+        const float med = nbg ? bg[nbg / 2] : 0.0f;
+        out3[3 * k + 0] = sig;
+        out3[3 * k + 1] = med;
+        out3[3 * k + 2] = sig - med * (float)nsig;
+    }
+    free(bg);
+}
+
+static int or_cmp_row(const void *a, const void *b) {
+    uint32_t x = ((const or_row *)a)->seed, y = ((const or_row *)b)->seed;
+    return (x > y) - (x < y);
+}
+
+void or_sort_rows(or_row *rows, size_t n) { qsort(rows, n, sizeof(or_row), or_cmp_row); }
+
+size_t or_detect_frame(const float *frame, const or_params *p, int characterize, or_row *rows,
+                       size_t cap, size_t *ncand_out) {
+    const size_t hw = (size_t)p->nrows * p->ncols;
+    float *proc = (float *)malloc(sizeof(float) * hw * 3);
+    uint32_t *cand = (uint32_t *)malloc(sizeof(uint32_t) * (hw ? hw : 1));
+    or_bandpass(frame, p, proc, proc + hw);
+    size_t nc = or_local_max(proc, p, cand, hw, proc + hw);
+    if (ncand_out) *ncand_out = nc;
+    or_row *tmp = (or_row *)malloc(sizeof(or_row) * (nc ? nc : 1));
+    size_t nr = or_walk(proc, p, cand, nc, tmp);
+    if (characterize) or_characterize(proc, frame, p, tmp, nr, 1, 0);
+    if (nr <= cap) memcpy(rows, tmp, sizeof(or_row) * nr);
+    else nr = (size_t)-1;
+    free(tmp);
+    free(cand);
+    free(proc);
+    return nr;
+}
+
+typedef struct {
+    const void *frames;
+    int dtype;
+    size_t nframes;
+    const or_params *p;
+    int characterize;
+    int nthreads, tid;
+    or_row *rows;
+    size_t cap;
+    uint64_t *counts, *ncands;
+    int status;
+} or_job;
+
+static void *or_worker(void *arg) {
+    or_job *j = (or_job *)arg;
+    const size_t hw = (size_t)j->p->nrows * j->p->ncols;
+    float *f32 = (float *)malloc(sizeof(float) * (hw ? hw : 1));
+    j->status = 0;
+    for (size_t f = (size_t)j->tid; f < j->nframes; f += (size_t)j->nthreads) {
+        const float *fr;
+        if (j->dtype == 0) {
+            const uint16_t *src = (const uint16_t *)j->frames + f * hw;
+            for (size_t i = 0; i < hw; i++) f32[i] = (float)src[i];
+            fr = f32;
+        } else {
+            fr = (const float *)j->frames + f * hw;
+        }
+        size_t nc = 0;
+        size_t nr = or_detect_frame(fr, j->p, j->characterize, j->rows + f * j->cap, j->cap, &nc);
+        if (nr == (size_t)-1) { j->status = -1; nr = 0; }
+        j->counts[f] = nr;
+        if (j->ncands) j->ncands[f] = nc;
+    }
+    free(f32);
+    return NULL;
+}
+
+int or_detect_batch(const void *frames, int dtype, size_t nframes, const or_params *p,
+                    int characterize, int nthreads, or_row *rows, size_t cap_per_frame,
+                    uint64_t *counts, uint64_t *ncands) {
+    if (nthreads < 1) nthreads = 1;
+    pthread_t *th = (pthread_t *)malloc(sizeof(pthread_t) * nthreads);
+    or_job *jobs = (or_job *)malloc(sizeof(or_job) * nthreads);
+    for (int t = 0; t < nthreads; t++) {
+        or_job j = {frames, dtype, nframes, p, characterize, nthreads, t, rows, cap_per_frame,
+                    counts, ncands, 0};
+        jobs[t] = j;
+        pthread_create(&th[t], NULL, or_worker, &jobs[t]);
+    }
+    int st = 0;
+    for (int t = 0; t < nthreads; t++) {
+        pthread_join(th[t], NULL);
+        if (jobs[t].status) st = -1;
+    }
+    free(jobs);
+    free(th);
+    return st;
+}
+
+void or_bandpass_f64(const float *frame, const or_params *p, double *processed, double *scratch) {
+    const int32_t H = p->nrows, W = p->ncols, s = p->smoothing, r = s / 2;
+    double w[1024], norm = 0.0;
+    for (int32_t i = -r; i <= r; i++) {
+        w[i + r] = exp(-(double)i * (double)i / (2.0 * (double)p->sigma2));
+        norm += w[i + r];
+    }
+    double *tg = scratch, *tc = scratch + (size_t)H * W;
+    for (int32_t u = 0; u < H; u++)
+        for (int32_t v = 0; v < W; v++) {
+            double gs = 0.0, cs = 0.0;
+            for (int32_t i = -r; i <= r; i++) {
+                int32_t pv = v + i, inb = 1;
+                if (pv < 0) { pv = 0; inb = 0; }
+                else if (pv >= W) { pv = W - 1; inb = 0; }
+                if (inb) gs += w[i + r] * (double)frame[(size_t)u * W + pv];
+                cs += (double)frame[(size_t)u * W + pv];
+            }
+            tg[(size_t)u * W + v] = gs / norm;
+            tc[(size_t)u * W + v] = cs;
+        }
+    for (int32_t u = 0; u < H; u++)
+        for (int32_t v = 0; v < W; v++) {
+            double gs = 0.0, cs = 0.0;
+            for (int32_t i = -r; i <= r; i++) {
+                int32_t pu = u + i, inb = 1;
+                if (pu < 0) { pu = 0; inb = 0; }
+                else if (pu >= H) { pu = H - 1; inb = 0; }
+                if (inb) gs += w[i + r] * tg[(size_t)pu * W + v];
+                cs += tc[(size_t)pu * W + v];
+            }
+            double res = gs / norm - cs / (double)(s * s);
+            if (p->truncate && res < 1.0 / 255.0) res = 0.0;
+            processed[(size_t)u * W + v] = res;
+        }
+}
+
+void or_centroid_f64(const double *P, const or_params *p, int32_t pu0, int32_t pv0, double *dx,
+                     double *dy, double *mass) {
+    const int32_t rint = (p->circle - 1) / 2;
+    double sx = 0.0, sy = 0.0, m = 0.0;
+    for (int32_t i = -rint; i <= rint; i++)
+        for (int32_t j = -rint; j <= rint; j++) {
+            if (i * i + j * j > rint * rint) continue;
+            int32_t pu = pu0 + i, pv = pv0 + j;
+            if (pu < 0 || pu >= p->nrows || pv < 0 || pv >= p->ncols) continue;
+            double d = P[(size_t)pu * p->ncols + pv];
+            sx += d * i;
+            sy += d * j;
+            m += d;
+        }
+    *dx = sx / m;
+    *dy = sy / m;
+    *mass = m;
+}

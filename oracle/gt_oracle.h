@@ -1,0 +1,119 @@
+/*
+ * gt_oracle.h -- CPU restatement of gpu_tracking's per-frame detection
+ * pipeline.  TEST INFRASTRUCTURE ONLY: the product (libgtrack.so and the
+ * gpu_tracking_amd package) never links, loads or calls this code.  Only
+ * tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg use it, as
+ * the checker the HIP path is compared against.
+ *
+ * Every function names the reference file:line it restates.  The reference's
+ * arithmetic is WGSL compiled by naga/wgpu; the few places where WGSL leaves
+ * the result to the driver are fixed here as follows (and the HIP kernels
+ * follow the same choices, so the two agree bit for bit):
+ *   - per-tap multiply-accumulates `acc += w * x` are contracted (fmaf);
+ *     SPIR-V without NoContraction lets a driver do this;
+ *   - the Gaussian tap weights are one f32 table computed with expf;
+ *   - sin(2*theta), cos(2*theta) of atan2(j, i) use the exact rational forms
+ *     2ij/(i^2+j^2) and (i^2-j^2)/(i^2+j^2);
+ *   - NaN converts to integer 0 (what AMD v_cvt_i32_f32 does).
+ */
+#ifndef GT_ORACLE_H
+#define GT_ORACLE_H
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Derived per-frame parameters: the 16-word WGSL `Params` uniform
+ * (src/shaders/params.wgsl:1-20) plus the frame std and the truncation flag. */
+typedef struct {
+    int32_t nrows, ncols;
+    int32_t smoothing;        /* preprocess kernel side s (composite_dims) */
+    float sigma2;             /* noise_size, used as a variance */
+    int32_t circle;           /* diameter (circle_dims) */
+    int32_t dilation;         /* dilation window side k */
+    uint32_t max_iterations;
+    float threshold;          /* shift threshold */
+    float minmass;
+    int32_t margin;
+    float snr;
+    float minmass_snr;
+    float rough_snr_factor;
+    float image_std;          /* 0 unless an snr criterion is active */
+    int32_t truncate;         /* truncate_preprocessed */
+} or_params;
+
+/* ResultRow (src/execute_gpu.rs:48-61) plus the raster index of the
+ * candidate pixel the walk started from (the deterministic order key). */
+typedef struct {
+    float x, y, mass, r, max_intensity, Rg, raw_mass, signal, ecc, count;
+    uint32_t seed;
+    uint32_t pad;
+} or_row;
+
+/* gpu_setup.rs:528-580 (Trackpy branch). smoothing_size <= 0 means "diameter". */
+void or_derive_trackpy(uint32_t diameter, uint32_t separation, int32_t smoothing_size,
+                       float noise_size, float minmass, uint32_t max_iterations,
+                       float shift_threshold, float snr, float minmass_snr,
+                       int adaptive_background, int truncate, int32_t nrows,
+                       int32_t ncols, or_params *out);
+
+/* Tap weights of the preprocess Gaussian, i = -s/2..s/2 (preprocess_rows.wgsl:30-47).
+ * w must hold 2*(s/2)+1 floats; *norm receives their sequential f32 sum. */
+void or_gauss_weights(int32_t s, float sigma2, float *w, float *norm);
+
+/* preprocess_rows.wgsl:14-55 then preprocess_cols.wgsl:11-60.
+ * frame, processed: nrows*ncols f32.  scratch: 2*nrows*ncols f32. */
+void or_bandpass(const float *frame, const or_params *p, float *processed, float *scratch);
+
+/* max_rows.wgsl:10-32 + extract_max.wgsl:35-78.  Writes candidate raster
+ * indices (u*ncols+v) in raster order; returns the count. */
+size_t or_local_max(const float *processed, const or_params *p, uint32_t *cand,
+                    size_t cap, float *scratch_rows);
+
+/* walk.wgsl:67-229 for every candidate (r = -1, Trackpy).  Returns the
+ * number of rows kept (mass >= minmass) in candidate order. */
+size_t or_walk(const float *processed, const or_params *p, const uint32_t *cand,
+               size_t ncand, or_row *rows);
+
+/* characterize.wgsl:30-155 on rows[0..n).  use_f64 = 1 accumulates the
+ * Rg and eccentricity moment sums in f64; 0 keeps the reference's f32 sums.
+ * recompute_mass = 1 is the characterize_points feature (characterize.wgsl:75). */
+void or_characterize(const float *processed, const float *raw, const or_params *p,
+                     or_row *rows, size_t n, int use_f64, int recompute_mass);
+
+/* filter_close_trackpy (execute_gpu.rs:556-590) with within_radius_rd2
+ * (linking.rs:208-247).  keep[i] = 1 if row i survives. O(n^2). */
+void or_filter_close(const or_row *rows, size_t n, float separation, uint8_t *keep);
+
+/* Doughnut photometry (execute_gpu.rs:717-790; kernels.rs:138-171).
+ * out3[3*i..] = raw_mass, raw_bg_median, raw_mass_corrected. */
+void or_doughnut(const float *raw, int32_t nrows, int32_t ncols, const or_row *rows,
+                 size_t n, float radius, float bg_radius, float gap_radius, float *out3);
+
+/* Sort rows by seed (the deterministic order used by both implementations). */
+void or_sort_rows(or_row *rows, size_t n);
+
+/* Whole frame: bandpass -> local max -> walk -> (characterize, f64 moments).
+ * Returns the number of rows (sorted by seed), or (size_t)-1 on overflow. */
+size_t or_detect_frame(const float *frame, const or_params *p, int characterize,
+                       or_row *rows, size_t cap, size_t *ncand_out);
+
+/* Frame-parallel driver for the CPU baseline: nthreads POSIX threads, frame
+ * f handled by thread f % nthreads.  dtype 0 = uint16, 1 = float32.
+ * rows: nframes*cap_per_frame; counts: nframes.  Returns 0 or -1. */
+int or_detect_batch(const void *frames, int dtype, size_t nframes, const or_params *p,
+                    int characterize, int nthreads, or_row *rows, size_t cap_per_frame,
+                    uint64_t *counts, uint64_t *ncands);
+
+/* FP64 twin for error reporting: f64 bandpass (exp in f64, exact divisions). */
+void or_bandpass_f64(const float *frame, const or_params *p, double *processed, double *scratch);
+/* FP64 centroid of the disk at integer position (pu, pv) of a f64 frame. */
+void or_centroid_f64(const double *processed, const or_params *p, int32_t pu, int32_t pv,
+                     double *dx, double *dy, double *mass);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
