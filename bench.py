@@ -80,6 +80,9 @@ def main():
     ap.add_argument("--frames", type=int, default=0, help="override the frame count")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="also run a 1 GiB device-to-device copy (a known-size kernel that "
+                         "calibrates rocprof's FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; multi-GPU runs are not part of it "
@@ -101,6 +104,13 @@ def main():
           flush=True)
     frames = synth.video_torch(s, T, seed=20261203 + 2000, device="cuda")
     torch.cuda.synchronize()
+
+    if args.copy_probe:
+        src = torch.empty(1 << 28, dtype=torch.float32, device="cuda")   # 1 GiB
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     p = gt.make_params(params["diameter"], **{k: v for k, v in params.items() if k != "diameter"})
     ctx = gt.Context(p, H, W)

@@ -8,6 +8,10 @@
 
 #include "gt_kparams.h"
 
+// Per-frame detection counters live GT_COUNTER_STRIDE words apart so that
+// frames processed at the same time do not share a cache line.
+#define GT_COUNTER_STRIDE 64
+
 // Kernel launchers (gt_kernels.hip).  All are asynchronous on `s`.
 hipError_t gt_launch_zero_u32(uint32_t *p, size_t n, hipStream_t s);
 hipError_t gt_launch_bandpass(const void *frames, int dtype, float *proc, uint32_t nframes,
@@ -19,3 +23,15 @@ hipError_t gt_launch_walk(const float *proc, const void *raw, int dtype, uint32_
                           const GtKParams &p, const uint32_t *cand, uint32_t cand_cap,
                           const uint32_t *cand_count, gt_detection *out, uint32_t out_cap,
                           uint32_t *out_count, uint32_t frame0, hipStream_t s);
+hipError_t gt_launch_characterize_rows(const float *proc, const void *raw, int dtype,
+                                       uint32_t nframes, const GtKParams &p, gt_detection *rows,
+                                       uint32_t cap, const uint32_t *counts, uint32_t max_count,
+                                       hipStream_t s);
+// Fused local max + walk + characterize over LDS tiles (the main path).
+int gt_tile_halo(const GtKParams &p);
+size_t gt_tile_lds_bytes(const GtKParams &p);
+hipError_t gt_launch_detect_tile(const float *proc, const void *raw, int dtype, uint32_t nframes,
+                                 const GtKParams &p, gt_detection *out, uint32_t out_cap,
+                                 uint32_t *ocount, uint32_t frame0, hipStream_t s);
+hipError_t gt_launch_copy_counts(const uint32_t *strided, uint32_t *dst, uint32_t n,
+                                 hipStream_t s);

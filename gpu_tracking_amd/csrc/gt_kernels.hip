@@ -18,6 +18,8 @@
 #include <hip/hip_runtime.h>
 #include <math.h>
 
+#include <algorithm>
+
 #include "gt_internal.h"
 
 namespace {
@@ -26,7 +28,7 @@ namespace {
 constexpr float kTrunc = 0.003921568859368562698364257812f;
 
 constexpr int BP_TW = 64, BP_TH = 32, BP_THREADS = 256;
-constexpr int LM_TW = 64, LM_TH = 16, LM_THREADS = 256;
+constexpr int LM_TW = 64, LM_THREADS = 256;
 constexpr int WK_THREADS = 256;
 
 template <typename T>
@@ -148,68 +150,7 @@ __global__ __launch_bounds__(BP_THREADS) void k_bandpass(const T *__restrict__ f
 // ---------------------------------------------------------------------------
 // Local maximum: a pixel inside the margin is a candidate when no in-frame
 // value of its k x k window [-a, k-1-a]^2 is greater than it, and it is not
-// below the snr threshold.  Out-of-frame window entries are -inf.
-template <int KT>
-__global__ __launch_bounds__(LM_THREADS) void k_localmax(const float *__restrict__ proc,
-                                                         GtKParams p, uint32_t *__restrict__ cand,
-                                                         uint32_t cap,
-                                                         uint32_t *__restrict__ count) {
-    extern __shared__ float lds[];
-    const int k = KT > 0 ? KT : p.k;
-    const int a = p.a;
-    const int RW = LM_TW + k - 1, RH = LM_TH + k - 1;
-    float *Sp = lds;
-    float *Sm = Sp + RH * RW;
-    const int H = p.H, W = p.W;
-    const size_t HW = static_cast<size_t>(H) * W;
-    const int x0 = blockIdx.x * LM_TW, y0 = blockIdx.y * LM_TH;
-    const int fr = blockIdx.z;
-    const float *P = proc + static_cast<size_t>(fr) * HW;
-    const int tid = threadIdx.x;
-    for (int idx = tid; idx < RH * RW; idx += LM_THREADS) {
-        const int ry = idx / RW, rx = idx - ry * RW;
-        const int gy = y0 - a + ry, gx = x0 - a + rx;
-        Sp[idx] = (gy >= 0 && gy < H && gx >= 0 && gx < W) ? P[static_cast<size_t>(gy) * W + gx]
-                                                           : -INFINITY;
-    }
-    __syncthreads();
-    // Row maxima over the k columns of the window.
-    for (int idx = tid; idx < RH * LM_TW; idx += LM_THREADS) {
-        const int ry = idx / LM_TW, cx = idx - ry * LM_TW;
-        const float *s = Sp + ry * RW + cx;
-        float m = s[a];
-        if (KT > 0) {
-#pragma unroll
-            for (int j = 0; j < KT; ++j) m = maxf_ref(m, s[j]);
-        } else {
-            for (int j = 0; j < k; ++j) m = maxf_ref(m, s[j]);
-        }
-        Sm[idx] = m;
-    }
-    __syncthreads();
-    const int margin = p.margin;
-    const float thr = p.max_thr;
-    uint32_t *fcand = cand + static_cast<size_t>(fr) * cap;
-    for (int base = 0; base < LM_TH * LM_TW; base += LM_THREADS) {
-        const int idx = base + tid;
-        const int ty = idx / LM_TW, cx = idx - ty * LM_TW;
-        const int gy = y0 + ty, gx = x0 + cx;
-        bool flag = false;
-        if (gy < H && gx < W && gy >= margin && gy < H - margin && gx >= margin &&
-            gx < W - margin) {
-            const float c = Sp[(ty + a) * RW + cx + a];
-            if (!(c < thr)) {
-                flag = true;
-                const float *col = Sm + ty * LM_TW + cx;
-                for (int i = 0; i < k; ++i)
-                    if (col[i * LM_TW] > c) { flag = false; break; }
-            }
-        }
-        const uint32_t pos = wave_append(flag, &count[fr]);
-        if (flag && pos < cap) fcand[pos] = static_cast<uint32_t>(gy) * static_cast<uint32_t>(W) + gx;
-    }
-}
-
+// below the snr threshold.
 // Untiled local maximum for dilation windows too large for the LDS tile.
 __global__ __launch_bounds__(LM_THREADS) void k_localmax_direct(const float *__restrict__ proc,
                                                                 GtKParams p,
@@ -241,7 +182,7 @@ __global__ __launch_bounds__(LM_THREADS) void k_localmax_direct(const float *__r
             }
         }
     }
-    const uint32_t pos = wave_append(flag, &count[fr]);
+    const uint32_t pos = wave_append(flag, &count[fr * GT_COUNTER_STRIDE]);
     if (flag && pos < cap)
         cand[static_cast<size_t>(fr) * cap + pos] = static_cast<uint32_t>(gy) * static_cast<uint32_t>(W) + gx;
 }
@@ -252,16 +193,35 @@ struct WalkRes {
     float x, y, mass, max, count;
 };
 
+// Where the processed samples of a disk come from: the frame in global
+// memory, or the block's LDS copy of a region around its tile.  Both give
+// the same values; only the load differs.
+struct GlobalSrc {
+    const float *p;
+    int W;
+    __device__ __forceinline__ float at(int u, int v) const {
+        return p[static_cast<size_t>(u) * W + v];
+    }
+};
+struct LdsSrc {
+    const float *p;  // LDS
+    int oy, ox, pitch;
+    __device__ __forceinline__ float at(int u, int v) const {
+        return p[(u - oy) * pitch + (v - ox)];
+    }
+};
+
 // get_center (walk.wgsl:67-102): centroid offset, mass, count and maximum of
 // the processed frame over the disk x^2 + y^2 <= r2 around (u, v).
-__device__ __forceinline__ void disk_accumulate(WalkRes &res, const float *__restrict__ P, int H,
-                                                int W, int u, int v, int i, int j, float r2) {
+template <class S>
+__device__ __forceinline__ void disk_accumulate(WalkRes &res, const S &src, int H, int W, int u,
+                                                int v, int i, int j, float r2) {
     const float x = static_cast<float>(i), x2 = x * x;
     const float y = static_cast<float>(j), y2 = y * y;
     if (x2 + y2 > r2) return;
     const int pu = u + i, pv = v + j;
     if (pu < 0 || pu >= H || pv < 0 || pv >= W) return;
-    const float d = P[static_cast<size_t>(pu) * W + pv];
+    const float d = src.at(pu, pv);
     res.x = fmaf(d, x, res.x);
     res.y = fmaf(d, y, res.y);
     res.mass = res.mass + d;
@@ -269,30 +229,42 @@ __device__ __forceinline__ void disk_accumulate(WalkRes &res, const float *__res
     res.max = maxf_ref(res.max, d);
 }
 
-template <int RI>
-__device__ __forceinline__ WalkRes get_center(const float *__restrict__ P, int H, int W, int u,
-                                              int v, int rint_rt, float r2_rt) {
+template <int RI, class S>
+__device__ __forceinline__ WalkRes get_center(const S &src, int H, int W, int u, int v,
+                                              int rint_rt, float r2_rt) {
     WalkRes res = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
     if constexpr (RI > 0) {
         constexpr float r2 = static_cast<float>(RI) * static_cast<float>(RI);
 #pragma unroll
         for (int i = -RI; i <= RI; ++i)
 #pragma unroll
-            for (int j = -RI; j <= RI; ++j) disk_accumulate(res, P, H, W, u, v, i, j, r2);
+            for (int j = -RI; j <= RI; ++j) disk_accumulate(res, src, H, W, u, v, i, j, r2);
     } else {
         for (int i = -rint_rt; i <= rint_rt; ++i)
-            for (int j = -rint_rt; j <= rint_rt; ++j) disk_accumulate(res, P, H, W, u, v, i, j, r2_rt);
+            for (int j = -rint_rt; j <= rint_rt; ++j)
+                disk_accumulate(res, src, H, W, u, v, i, j, r2_rt);
     }
     res.x = res.x / res.mass;
     res.y = res.y / res.mass;
     return res;
 }
 
+// One walk step's move rule (walk.wgsl:141-158); returns whether it moved.
+__device__ __forceinline__ bool walk_move(const WalkRes &res, int &pu, int &pv, int H, int W,
+                                          int margin, float thr) {
+    bool changed = false;
+    if (res.x > thr && pu < H - 1 - margin) { pu += 1; changed = true; }
+    else if (res.x < -thr && pu > margin) { pu -= 1; changed = true; }
+    if (res.y > thr && pv < W - 1 - margin) { pv += 1; changed = true; }
+    else if (res.y < -thr && pv > margin) { pv -= 1; changed = true; }
+    return changed;
+}
+
 // characterize (characterize.wgsl:61-143) at round-half-even of the walked
 // centre, with the Rg and eccentricity moment sums accumulated in f64.
-template <typename T>
-__device__ void characterize_row(const float *__restrict__ P, const T *__restrict__ Rw, int H,
-                                 int W, int circle, gt_result_row &row) {
+template <typename T, class S>
+__device__ void characterize_row(const S &src, const T *__restrict__ Rw, int H, int W, int circle,
+                                 gt_result_row &row) {
     float r = row.r;
     int rint;
     if (r <= 0.0f) {
@@ -313,9 +285,8 @@ __device__ void characterize_row(const float *__restrict__ P, const T *__restric
             if (mask > r2) continue;
             const int pu = u + i, pv = v + j;
             if (pu < 0 || pu >= H || pv < 0 || pv >= W) continue;
-            const size_t idx = static_cast<size_t>(pu) * W + pv;
-            const float d = P[idx];
-            raw_mass = raw_mass + to_f32(Rw[idx]);
+            const float d = src.at(pu, pv);
+            raw_mass = raw_mass + to_f32(Rw[static_cast<size_t>(pu) * W + pv]);
             signal = maxf_ref(signal, d);
             if (i != 0 || j != 0) {
                 // sin(2 atan2(j, i)) and cos(2 atan2(j, i)) in closed form
@@ -330,14 +301,41 @@ __device__ void characterize_row(const float *__restrict__ P, const T *__restric
             rg = fma(static_cast<double>(d), static_cast<double>(mask), rg);
         }
     }
+    // the reference reads the centre sample without a bounds check
     const int cu = min(max(u, 0), H - 1), cv = min(max(v, 0), W - 1);
-    const float den = row.mass - P[static_cast<size_t>(cu) * W + cv];
+    const float den = row.mass - src.at(cu, cv);
     row.ecc = static_cast<float>(sqrt(es * es + ec * ec) / static_cast<double>(den));
     row.Rg = static_cast<float>(sqrt(rg / static_cast<double>(row.mass)));
     row.raw_mass = raw_mass;
     row.signal = signal;
 }
 
+// minmass rule (walk.wgsl:200-213)
+__device__ __forceinline__ bool passes_minmass(const GtKParams &p, const WalkRes &res) {
+    float mm;
+    if (p.minmass > 0.0f) mm = p.minmass;
+    else mm = p.minmass_snr * res.count * p.image_std * p.rough;
+    return res.mass >= mm;
+}
+
+__device__ __forceinline__ void fill_row(gt_detection &d, const WalkRes &res, uint32_t seed,
+                                         uint32_t frame) {
+    d.row.x = res.x;
+    d.row.y = res.y;
+    d.row.mass = res.mass;
+    d.row.r = 0.0f;
+    d.row.max_intensity = res.max;
+    d.row.Rg = 0.0f;
+    d.row.raw_mass = 0.0f;
+    d.row.signal = 0.0f;
+    d.row.ecc = 0.0f;
+    d.row.count = res.count;
+    d.seed = seed;
+    d.frame = frame;
+}
+
+// Walk from global memory, one lane per candidate (generic path, used when
+// the dilation window is too large for the fused tile kernel).
 template <typename T, int RI>
 __global__ __launch_bounds__(WK_THREADS) void k_walk(const float *__restrict__ proc,
                                                      const T *__restrict__ raw, GtKParams p,
@@ -351,13 +349,12 @@ __global__ __launch_bounds__(WK_THREADS) void k_walk(const float *__restrict__ p
     const int fr = blockIdx.y;
     const int H = p.H, W = p.W;
     const size_t HW = static_cast<size_t>(H) * W;
-    const float *P = proc + static_cast<size_t>(fr) * HW;
+    const GlobalSrc src{proc + static_cast<size_t>(fr) * HW, W};
     const T *Rw = raw + static_cast<size_t>(fr) * HW;
     const uint32_t n = min(ccount[fr], cap);
     const uint32_t *fc = cand + static_cast<size_t>(fr) * cap;
     gt_detection *fout = out + static_cast<size_t>(fr) * out_cap;
-    const int rint = RI > 0 ? RI : p.rint;
-    const int margin = rint;
+    const int margin = RI > 0 ? RI : p.rint;
     const uint32_t lane = __lane_id();
     const uint32_t stride = gridDim.x * WK_THREADS;
     // i0 is the first candidate of this wavefront: the loop condition is
@@ -372,38 +369,237 @@ __global__ __launch_bounds__(WK_THREADS) void k_walk(const float *__restrict__ p
             int pv = static_cast<int>(seed - static_cast<uint32_t>(pu) * static_cast<uint32_t>(W));
             WalkRes res = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
             for (uint32_t it = 0; it < p.max_iter; ++it) {
-                res = get_center<RI>(P, H, W, pu, pv, p.rint, p.r2);
-                bool changed = false;
-                if (res.x > p.shift_thr && pu < H - 1 - margin) { pu += 1; changed = true; }
-                else if (res.x < -p.shift_thr && pu > margin) { pu -= 1; changed = true; }
-                if (res.y > p.shift_thr && pv < W - 1 - margin) { pv += 1; changed = true; }
-                else if (res.y < -p.shift_thr && pv > margin) { pv -= 1; changed = true; }
-                if (!changed) break;
+                res = get_center<RI>(src, H, W, pu, pv, p.rint, p.r2);
+                if (!walk_move(res, pu, pv, H, W, margin, p.shift_thr)) break;
             }
             res.x = res.x + static_cast<float>(pu);
             res.y = res.y + static_cast<float>(pv);
-            float mm;
-            if (p.minmass > 0.0f) mm = p.minmass;
-            else mm = p.minmass_snr * res.count * p.image_std * p.rough;
-            keep = res.mass >= mm;
+            keep = passes_minmass(p, res);
             if (keep) {
-                d.row.x = res.x;
-                d.row.y = res.y;
-                d.row.mass = res.mass;
-                d.row.r = 0.0f;
-                d.row.max_intensity = res.max;
-                d.row.Rg = 0.0f;
-                d.row.raw_mass = 0.0f;
-                d.row.signal = 0.0f;
-                d.row.ecc = 0.0f;
-                d.row.count = res.count;
-                d.seed = seed;
-                d.frame = frame0 + fr;
-                if (p.characterize) characterize_row<T>(P, Rw, H, W, p.circle, d.row);
+                fill_row(d, res, seed, frame0 + fr);
+                if (p.characterize) characterize_row<T>(src, Rw, H, W, p.circle, d.row);
             }
         }
-        const uint32_t pos = wave_append(keep, &ocount[fr]);
+        const uint32_t pos = wave_append(keep, &ocount[fr * GT_COUNTER_STRIDE]);
         if (keep && pos < out_cap) fout[pos] = d;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Fused local max + walk + characterize.  One block per FT_TH x FT_TW tile:
+//   1. the processed frame over the tile plus an h-pixel halo is staged in LDS
+//      (out-of-frame entries -inf);
+//   2. row maxima over the k-wide dilation window, then the column test, give
+//      the tile's candidates, compacted into an LDS list in raster order with
+//      one ballot per wave and a block prefix (no atomics);
+//   3. one lane per candidate walks; every disk evaluation that lies inside
+//      the staged region reads LDS, the rest read the frame in global memory
+//      (same arithmetic, so results do not depend on the tile);
+//   4. detections are characterized and appended to the frame's row list with
+//      one atomic per block.
+constexpr int FT_TW = 64, FT_TH = 32, FT_THREADS = 256;
+constexpr int FT_WAVES = FT_THREADS / 64;
+constexpr int FT_ITERS = FT_TH * FT_TW / FT_THREADS;
+
+template <typename T, int KT, int RI>
+__global__ __launch_bounds__(FT_THREADS) void k_detect_tile(const float *__restrict__ proc,
+                                                            const T *__restrict__ raw,
+                                                            GtKParams p, int h,
+                                                            gt_detection *__restrict__ out,
+                                                            uint32_t out_cap,
+                                                            uint32_t *__restrict__ ocount,
+                                                            uint32_t frame0) {
+    extern __shared__ float lds[];
+    __shared__ uint32_t s_wave_cnt[FT_ITERS * FT_WAVES];
+    __shared__ uint32_t s_wave_off[FT_ITERS * FT_WAVES + 1];
+    __shared__ uint16_t s_cand[FT_TH * FT_TW];
+    __shared__ uint32_t s_keep_cnt[FT_WAVES];
+    __shared__ uint32_t s_base;
+    const int k = KT > 0 ? KT : p.k;
+    const int a = p.a;
+    const int rint = RI > 0 ? RI : p.rint;
+    const int RW = FT_TW + 2 * h, RH = FT_TH + 2 * h;
+    const int MR = FT_TH + k - 1;  // rows of row-maxima needed by the tile
+    float *Sp = lds;               // RH x RW
+    float *Sm = Sp + RH * RW;      // MR x FT_TW
+    const int H = p.H, W = p.W;
+    const size_t HW = static_cast<size_t>(H) * W;
+    const int x0 = blockIdx.x * FT_TW, y0 = blockIdx.y * FT_TH;
+    const int fr = blockIdx.z;
+    const float *P = proc + static_cast<size_t>(fr) * HW;
+    const T *Rw = raw + static_cast<size_t>(fr) * HW;
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int oy = y0 - h, ox = x0 - h;
+
+    for (int idx = tid; idx < RH * RW; idx += FT_THREADS) {
+        const int ry = idx / RW, rx = idx - ry * RW;
+        const int gy = oy + ry, gx = ox + rx;
+        Sp[idx] = (gy >= 0 && gy < H && gx >= 0 && gx < W) ? P[static_cast<size_t>(gy) * W + gx]
+                                                           : -INFINITY;
+    }
+    __syncthreads();
+    // row maxima: Sm[m][cx] = max over j < k of Sp[h - a + m][h + cx - a + j]
+    for (int idx = tid; idx < MR * FT_TW; idx += FT_THREADS) {
+        const int m = idx / FT_TW, cx = idx - m * FT_TW;
+        const float *s = Sp + (h - a + m) * RW + (h + cx - a);
+        float mx = s[a];
+        if (KT > 0) {
+#pragma unroll
+            for (int j = 0; j < KT; ++j) mx = maxf_ref(mx, s[j]);
+        } else {
+            for (int j = 0; j < k; ++j) mx = maxf_ref(mx, s[j]);
+        }
+        Sm[idx] = mx;
+    }
+    __syncthreads();
+    // candidate test; pixel idx = it * 256 + tid is row it*4 + wave, column lane
+    uint64_t masks[FT_ITERS];
+    const int margin = p.margin;
+    const float thr = p.max_thr;
+#pragma unroll
+    for (int it = 0; it < FT_ITERS; ++it) {
+        const int ty = it * FT_WAVES + wave, cx = lane;
+        const int gy = y0 + ty, gx = x0 + cx;
+        bool flag = false;
+        if (gy < H && gx < W && gy >= margin && gy < H - margin && gx >= margin &&
+            gx < W - margin) {
+            const float c = Sp[(h + ty) * RW + h + cx];
+            if (!(c < thr)) {
+                flag = true;
+                const float *col = Sm + ty * FT_TW + cx;
+                if (KT > 0) {
+#pragma unroll
+                    for (int i = 0; i < KT; ++i) flag = flag && !(col[i * FT_TW] > c);
+                } else {
+                    for (int i = 0; i < k; ++i) flag = flag && !(col[i * FT_TW] > c);
+                }
+            }
+        }
+        masks[it] = __ballot(flag);
+        if (lane == 0) s_wave_cnt[it * FT_WAVES + wave] = __popcll(masks[it]);
+    }
+    __syncthreads();
+    if (tid == 0) {
+        uint32_t acc = 0;
+        for (int q = 0; q < FT_ITERS * FT_WAVES; ++q) {
+            s_wave_off[q] = acc;
+            acc += s_wave_cnt[q];
+        }
+        s_wave_off[FT_ITERS * FT_WAVES] = acc;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < FT_ITERS; ++it) {
+        const uint64_t m = masks[it];
+        if ((m >> lane) & 1ull) {
+            const uint32_t pos = s_wave_off[it * FT_WAVES + wave] +
+                                 __popcll(m & ((1ull << lane) - 1ull));
+            s_cand[pos] = static_cast<uint16_t>((it * FT_WAVES + wave) * FT_TW + lane);
+        }
+    }
+    __syncthreads();
+    const uint32_t ncand = s_wave_off[FT_ITERS * FT_WAVES];
+    const GlobalSrc gsrc{P, W};
+    const LdsSrc lsrc{Sp, oy, ox, RW};
+    // a disk around (u, v) lies in the staged region when all of it does
+    const int ylo = oy + rint, yhi = oy + RH - 1 - rint;
+    const int xlo = ox + rint, xhi = ox + RW - 1 - rint;
+    const uint32_t frame = frame0 + fr;
+    uint32_t *counter = &ocount[fr * GT_COUNTER_STRIDE];
+    gt_detection *fout = out + static_cast<size_t>(fr) * out_cap;
+    for (uint32_t c0 = 0; c0 < ncand; c0 += FT_THREADS) {
+        const uint32_t ci = c0 + tid;
+        bool keep = false;
+        gt_detection d;
+        if (ci < ncand) {
+            const int li = s_cand[ci];
+            const int ty = li / FT_TW, cx = li - ty * FT_TW;
+            int pu = y0 + ty, pv = x0 + cx;
+            const uint32_t seed = static_cast<uint32_t>(pu) * static_cast<uint32_t>(W) + pv;
+            WalkRes res = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+            for (uint32_t it = 0; it < p.max_iter; ++it) {
+                if (pu >= ylo && pu <= yhi && pv >= xlo && pv <= xhi)
+                    res = get_center<RI>(lsrc, H, W, pu, pv, p.rint, p.r2);
+                else
+                    res = get_center<RI>(gsrc, H, W, pu, pv, p.rint, p.r2);
+                if (!walk_move(res, pu, pv, H, W, rint, p.shift_thr)) break;
+            }
+            res.x = res.x + static_cast<float>(pu);
+            res.y = res.y + static_cast<float>(pv);
+            keep = passes_minmass(p, res);
+            if (keep) {
+                fill_row(d, res, seed, frame);
+                if (p.characterize) {
+                    const int cu = f2i(rintf(d.row.x)), cv = f2i(rintf(d.row.y));
+                    if (cu >= ylo && cu <= yhi && cv >= xlo && cv <= xhi)
+                        characterize_row<T>(lsrc, Rw, H, W, p.circle, d.row);
+                    else
+                        characterize_row<T>(gsrc, Rw, H, W, p.circle, d.row);
+                }
+            }
+        }
+        const uint64_t km = __ballot(keep);
+        if (lane == 0) s_keep_cnt[wave] = __popcll(km);
+        __syncthreads();
+        if (tid == 0) {
+            uint32_t tot = 0;
+            for (int q = 0; q < FT_WAVES; ++q) tot += s_keep_cnt[q];
+            s_base = tot ? atomicAdd(counter, tot) : 0u;
+        }
+        __syncthreads();
+        if (keep) {
+            uint32_t pos = s_base + __popcll(km & ((1ull << lane) - 1ull));
+            for (int q = 0; q < wave; ++q) pos += s_keep_cnt[q];
+            if (pos < out_cap) fout[pos] = d;
+        }
+        __syncthreads();
+    }
+}
+
+// characterize_points: rows hold caller positions (x = row, y = column, r);
+// mass is recomputed over the disk at the rounded position (get_mass,
+// characterize.wgsl:30-58,75) and the characterize columns are filled in place.
+template <typename T>
+__global__ __launch_bounds__(WK_THREADS) void k_characterize_rows(const float *__restrict__ proc,
+                                                                  const T *__restrict__ raw,
+                                                                  GtKParams p,
+                                                                  gt_detection *__restrict__ rows,
+                                                                  uint32_t cap,
+                                                                  const uint32_t *__restrict__ counts) {
+    const int fr = blockIdx.y;
+    const int H = p.H, W = p.W;
+    const size_t HW = static_cast<size_t>(H) * W;
+    const float *P = proc + static_cast<size_t>(fr) * HW;
+    const GlobalSrc gsrc{P, W};
+    const T *Rw = raw + static_cast<size_t>(fr) * HW;
+    const uint32_t n = min(counts[fr], cap);
+    gt_detection *fr_rows = rows + static_cast<size_t>(fr) * cap;
+    for (uint32_t i = blockIdx.x * WK_THREADS + threadIdx.x; i < n; i += gridDim.x * WK_THREADS) {
+        gt_result_row row = fr_rows[i].row;
+        float r = row.r;
+        int rint;
+        if (r <= 0.0f) {
+            rint = (p.circle - 1) / 2;
+            r = static_cast<float>(rint);
+        } else {
+            rint = f2i(ceilf(r));
+        }
+        const float r2 = r * r;
+        const int u = f2i(rintf(row.x)), v = f2i(rintf(row.y));
+        float m = 0.0f;
+        for (int a = -rint; a <= rint; ++a) {
+            const float x = static_cast<float>(a), x2 = x * x;
+            for (int b = -rint; b <= rint; ++b) {
+                const float y = static_cast<float>(b), y2 = y * y;
+                if (x2 + y2 > r2) continue;
+                const int pu = u + a, pv = v + b;
+                if (pu < 0 || pu >= H || pv < 0 || pv >= W) continue;
+                m = m + P[static_cast<size_t>(pu) * W + pv];
+            }
+        }
+        row.mass = m;
+        characterize_row<T>(gsrc, Rw, H, W, p.circle, row);
+        fr_rows[i].row = row;
     }
 }
 
@@ -481,20 +677,85 @@ hipError_t gt_launch_bandpass(const void *frames, int dtype, float *proc, uint32
 hipError_t gt_launch_localmax(const float *proc, uint32_t nframes, const GtKParams &p,
                               uint32_t *cand, uint32_t cand_cap, uint32_t *cand_count,
                               hipStream_t s) {
-    if (p.k > GT_MAX_K_TILED || p.k < 1) {
-        const dim3 grid((p.W + LM_TW - 1) / LM_TW, (p.H + 3) / 4, nframes);
-        k_localmax_direct<<<grid, LM_THREADS, 0, s>>>(proc, p, cand, cand_cap, cand_count);
-        return hipGetLastError();
+    const dim3 grid((p.W + LM_TW - 1) / LM_TW, (p.H + 3) / 4, nframes);
+    k_localmax_direct<<<grid, LM_THREADS, 0, s>>>(proc, p, cand, cand_cap, cand_count);
+    return hipGetLastError();
+}
+
+int gt_tile_halo(const GtKParams &p) {
+    // the staged region must hold the dilation window of every tile pixel and
+    // the walk disk of a candidate that moved up to two pixels off the tile
+    int h = p.a;
+    h = std::max(h, p.k - 1 - p.a);
+    h = std::max(h, p.rint + 2);
+    return h;
+}
+
+size_t gt_tile_lds_bytes(const GtKParams &p) {
+    const int h = gt_tile_halo(p);
+    const size_t dyn = sizeof(float) * (static_cast<size_t>(FT_TH + 2 * h) * (FT_TW + 2 * h) +
+                                        static_cast<size_t>(FT_TH + p.k - 1) * FT_TW);
+    const size_t stat = sizeof(uint16_t) * FT_TH * FT_TW + 1024;
+    return dyn + stat;
+}
+
+namespace {
+template <typename T>
+hipError_t launch_detect_tile_t(const float *proc, const T *raw, uint32_t nframes,
+                                const GtKParams &p, gt_detection *out, uint32_t out_cap,
+                                uint32_t *ocount, uint32_t frame0, hipStream_t s) {
+    const int h = gt_tile_halo(p);
+    const dim3 grid((p.W + FT_TW - 1) / FT_TW, (p.H + FT_TH - 1) / FT_TH, nframes);
+    const size_t lds = sizeof(float) * (static_cast<size_t>(FT_TH + 2 * h) * (FT_TW + 2 * h) +
+                                        static_cast<size_t>(FT_TH + p.k - 1) * FT_TW);
+#define GT_TILE(KV, RV)                                                                      \
+    k_detect_tile<T, KV, RV><<<grid, FT_THREADS, lds, s>>>(proc, raw, p, h, out, out_cap, \
+                                                            ocount, frame0)
+    if (p.rint == 4) {
+        switch (p.k) {
+        case 9: GT_TILE(9, 4); break;
+        case 12: GT_TILE(12, 4); break;
+        default: GT_TILE(0, 4); break;
+        }
+    } else {
+        switch (p.k) {
+        case 9: GT_TILE(9, 0); break;
+        case 12: GT_TILE(12, 0); break;
+        default: GT_TILE(0, 0); break;
+        }
     }
-    const dim3 grid((p.W + LM_TW - 1) / LM_TW, (p.H + LM_TH - 1) / LM_TH, nframes);
-    const int RW = LM_TW + p.k - 1, RH = LM_TH + p.k - 1;
-    const size_t lds = sizeof(float) * (static_cast<size_t>(RH) * RW + static_cast<size_t>(RH) * LM_TW);
-    switch (p.k) {
-    case 9: k_localmax<9><<<grid, LM_THREADS, lds, s>>>(proc, p, cand, cand_cap, cand_count); break;
-    case 12: k_localmax<12><<<grid, LM_THREADS, lds, s>>>(proc, p, cand, cand_cap, cand_count); break;
-    case 14: k_localmax<14><<<grid, LM_THREADS, lds, s>>>(proc, p, cand, cand_cap, cand_count); break;
-    default: k_localmax<0><<<grid, LM_THREADS, lds, s>>>(proc, p, cand, cand_cap, cand_count); break;
+#undef GT_TILE
+    return hipGetLastError();
+}
+
+__global__ void k_copy_counts(const uint32_t *__restrict__ strided, uint32_t *__restrict__ dst,
+                              uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = strided[i * GT_COUNTER_STRIDE];
+}
+}  // namespace
+
+hipError_t gt_launch_detect_tile(const float *proc, const void *raw, int dtype, uint32_t nframes,
+                                 const GtKParams &p, gt_detection *out, uint32_t out_cap,
+                                 uint32_t *ocount, uint32_t frame0, hipStream_t s) {
+    switch (dtype) {
+    case GT_U16:
+        return launch_detect_tile_t(proc, static_cast<const uint16_t *>(raw), nframes, p, out,
+                                    out_cap, ocount, frame0, s);
+    case GT_F32:
+        return launch_detect_tile_t(proc, static_cast<const float *>(raw), nframes, p, out,
+                                    out_cap, ocount, frame0, s);
+    case GT_U8:
+        return launch_detect_tile_t(proc, static_cast<const uint8_t *>(raw), nframes, p, out,
+                                    out_cap, ocount, frame0, s);
+    default: return hipErrorInvalidValue;
     }
+}
+
+hipError_t gt_launch_copy_counts(const uint32_t *strided, uint32_t *dst, uint32_t n,
+                                 hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    k_copy_counts<<<(n + 255) / 256, 256, 0, s>>>(strided, dst, n);
     return hipGetLastError();
 }
 
@@ -514,4 +775,29 @@ hipError_t gt_launch_walk(const float *proc, const void *raw, int dtype, uint32_
                              cand_count, out, out_cap, out_count, frame0, s);
     default: return hipErrorInvalidValue;
     }
+}
+
+hipError_t gt_launch_characterize_rows(const float *proc, const void *raw, int dtype,
+                                       uint32_t nframes, const GtKParams &p, gt_detection *rows,
+                                       uint32_t cap, const uint32_t *counts, uint32_t max_count,
+                                       hipStream_t s) {
+    uint32_t blocks = (max_count + WK_THREADS - 1) / WK_THREADS;
+    blocks = blocks < 1 ? 1 : (blocks > 1024 ? 1024 : blocks);
+    const dim3 grid(blocks, nframes, 1);
+    switch (dtype) {
+    case GT_U16:
+        k_characterize_rows<uint16_t><<<grid, WK_THREADS, 0, s>>>(
+            proc, static_cast<const uint16_t *>(raw), p, rows, cap, counts);
+        break;
+    case GT_F32:
+        k_characterize_rows<float><<<grid, WK_THREADS, 0, s>>>(
+            proc, static_cast<const float *>(raw), p, rows, cap, counts);
+        break;
+    case GT_U8:
+        k_characterize_rows<uint8_t><<<grid, WK_THREADS, 0, s>>>(
+            proc, static_cast<const uint8_t *>(raw), p, rows, cap, counts);
+        break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
 }

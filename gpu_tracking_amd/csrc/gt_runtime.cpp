@@ -65,8 +65,10 @@ struct gt_context {
     uint32_t chunk = 1;            // frames per pipeline chunk
     float *d_proc = nullptr;       // chunk * HW
     uint32_t *d_cand = nullptr;    // chunk * cand_cap
-    uint32_t *d_ccount = nullptr;  // chunk
+    uint32_t *d_ccount = nullptr;  // chunk * GT_COUNTER_STRIDE (candidates, untiled path)
+    uint32_t *d_ocount = nullptr;  // chunk * GT_COUNTER_STRIDE (detections)
     uint32_t cand_cap = 0;
+    bool tiled = true;             // fused tile kernel fits in LDS
 
     // gt_execute staging
     void *d_frames = nullptr;      // chunk * HW pixels (largest dtype)
@@ -92,8 +94,10 @@ namespace {
 gt_status alloc_scratch(gt_context *c) {
     const size_t HW = static_cast<size_t>(c->H) * c->W;
     GT_HIP(hipMalloc(&c->d_proc, sizeof(float) * HW * c->chunk));
-    GT_HIP(hipMalloc(&c->d_cand, sizeof(uint32_t) * static_cast<size_t>(c->cand_cap) * c->chunk));
-    GT_HIP(hipMalloc(&c->d_ccount, sizeof(uint32_t) * c->chunk));
+    if (!c->tiled)
+        GT_HIP(hipMalloc(&c->d_cand, sizeof(uint32_t) * static_cast<size_t>(c->cand_cap) * c->chunk));
+    GT_HIP(hipMalloc(&c->d_ccount, sizeof(uint32_t) * GT_COUNTER_STRIDE * c->chunk));
+    GT_HIP(hipMalloc(&c->d_ocount, sizeof(uint32_t) * GT_COUNTER_STRIDE * c->chunk));
     return GT_OK;
 }
 
@@ -113,6 +117,7 @@ void free_all(gt_context *c) {
     if (c->d_proc) (void)hipFree(c->d_proc);
     if (c->d_cand) (void)hipFree(c->d_cand);
     if (c->d_ccount) (void)hipFree(c->d_ccount);
+    if (c->d_ocount) (void)hipFree(c->d_ocount);
     if (c->d_frames) (void)hipFree(c->d_frames);
     if (c->d_rows) (void)hipFree(c->d_rows);
     if (c->d_counts) (void)hipFree(c->d_counts);
@@ -145,15 +150,25 @@ gt_status run_pipeline(gt_context *c, const void *d_frames, int32_t dtype, uint3
         const uint32_t nf = std::min(c->chunk, nframes - f0);
         const char *frames = static_cast<const char *>(d_frames) + esz * HW * f0;
         hipEvent_t *ev = c->timing ? &c->events[4 * ci] : nullptr;
-        GT_HIP(gt_launch_zero_u32(c->d_ccount, nf, s));
+        GT_HIP(gt_launch_zero_u32(c->d_ocount, static_cast<size_t>(nf) * GT_COUNTER_STRIDE, s));
+        if (!c->tiled)
+            GT_HIP(gt_launch_zero_u32(c->d_ccount, static_cast<size_t>(nf) * GT_COUNTER_STRIDE, s));
         if (ev) GT_HIP(hipEventRecord(ev[0], s));
         GT_HIP(gt_launch_bandpass(frames, dtype, c->d_proc, nf, c->kp, s));
         if (ev) GT_HIP(hipEventRecord(ev[1], s));
-        GT_HIP(gt_launch_localmax(c->d_proc, nf, c->kp, c->d_cand, c->cand_cap, c->d_ccount, s));
-        if (ev) GT_HIP(hipEventRecord(ev[2], s));
-        GT_HIP(gt_launch_walk(c->d_proc, frames, dtype, nf, c->kp, c->d_cand, c->cand_cap,
-                              c->d_ccount, d_out + static_cast<size_t>(f0) * cap, cap,
-                              d_counts + f0, f0, s));
+        gt_detection *out = d_out + static_cast<size_t>(f0) * cap;
+        if (c->tiled) {
+            // local max, walk and characterize in one kernel
+            GT_HIP(gt_launch_detect_tile(c->d_proc, frames, dtype, nf, c->kp, out, cap,
+                                         c->d_ocount, f0, s));
+            if (ev) GT_HIP(hipEventRecord(ev[2], s));
+        } else {
+            GT_HIP(gt_launch_localmax(c->d_proc, nf, c->kp, c->d_cand, c->cand_cap, c->d_ccount, s));
+            if (ev) GT_HIP(hipEventRecord(ev[2], s));
+            GT_HIP(gt_launch_walk(c->d_proc, frames, dtype, nf, c->kp, c->d_cand, c->cand_cap,
+                                  c->d_ccount, out, cap, c->d_ocount, f0, s));
+        }
+        GT_HIP(gt_launch_copy_counts(c->d_ocount, d_counts + f0, nf, s));
         if (ev) GT_HIP(hipEventRecord(ev[3], s));
     }
     if (c->timing) {
@@ -200,6 +215,8 @@ gt_status gt_context_create(const gt_params *params, uint32_t nrows, uint32_t nc
     c->characterize_points = characterize_points;
     c->der = der;
     gt_host_fill_kparams(&der, nrows, ncols, &c->kp);
+    // the fused tile kernel needs its staged region in 64 KiB of LDS
+    c->tiled = c->kp.k >= 1 && gt_tile_lds_bytes(c->kp) <= 65536;
     if (hipGetDevice(&c->device) != hipSuccess) c->device = 0;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
         delete c;
@@ -377,10 +394,41 @@ gt_status gt_execute(gt_context *c, const void *frames, int32_t dtype, uint64_t 
 
 }  // extern "C"
 
-// Used by gt_host.cpp for characterize_points: upload the positions of the
-// chunk's frames and run characterize only.
+// characterize_points for one chunk (submit_work's characterization branch,
+// src/execute_gpu.rs:190-229, which drops max_rows/extract_max/walk from the
+// pass order, src/gpu_setup.rs:877-891): the chunk's frames are already on the
+// device; upload each frame's positions as rows, bandpass, then characterize
+// the rows in place.  Row seeds are the positions' order within the frame.
 gt_status gt_host_characterize_points(gt_context *c, size_t base, uint32_t nf, int32_t dtype,
                                       const gt_host_plan &plan) {
-    (void)c; (void)base; (void)nf; (void)dtype; (void)plan;
-    return fail(GT_ERR_INVALID_ARGUMENT, "characterize_points is not available yet");
+    std::vector<gt_detection> staged;
+    std::vector<uint32_t> counts(nf, 0);
+    uint32_t max_count = 0;
+    for (uint32_t f = 0; f < nf; ++f) {
+        const size_t b = plan.pt_begin[base + f], e = plan.pt_end[base + f];
+        const size_t n = e - b;
+        if (n > c->det_cap)
+            return fail(GT_ERR_CAPACITY, "%zu positions for one frame exceed the capacity of %u", n,
+                        c->det_cap);
+        counts[f] = static_cast<uint32_t>(n);
+        max_count = std::max<uint32_t>(max_count, static_cast<uint32_t>(n));
+        staged.resize(n);
+        for (size_t i = 0; i < n; ++i) {
+            staged[i].row = plan.pts[b + i];
+            staged[i].seed = static_cast<uint32_t>(i);
+            staged[i].frame = static_cast<uint32_t>(plan.frames[base + f]);
+        }
+        if (n)
+            GT_HIP(hipMemcpyAsync(c->d_rows + static_cast<size_t>(f) * c->det_cap, staged.data(),
+                                  sizeof(gt_detection) * n, hipMemcpyHostToDevice, c->stream));
+        // the staging vector is reused: wait for this copy before refilling it
+        GT_HIP(hipStreamSynchronize(c->stream));
+    }
+    std::memcpy(c->h_counts, counts.data(), sizeof(uint32_t) * nf);
+    GT_HIP(hipMemcpyAsync(c->d_counts, c->h_counts, sizeof(uint32_t) * nf, hipMemcpyHostToDevice,
+                          c->stream));
+    GT_HIP(gt_launch_bandpass(c->d_frames, dtype, c->d_proc, nf, c->kp, c->stream));
+    GT_HIP(gt_launch_characterize_rows(c->d_proc, c->d_frames, dtype, nf, c->kp, c->d_rows,
+                                       c->det_cap, c->d_counts, max_count, c->stream));
+    return GT_OK;
 }

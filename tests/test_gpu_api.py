@@ -1,0 +1,143 @@
+"""The reference's Python entry points (batch, characterize_points) end to end
+through gt_execute: host staging, GPU detection, host post-processing
+(filter_close, doughnut photometry) and column assembly, against the oracle
+run frame by frame on the same video."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def gt(gpu_ok):
+    import gpu_tracking_amd
+    return gpu_tracking_amd
+
+
+def _video(T=4, seed=3, H=200, W=232, n=30):
+    from gpu_tracking_amd import synth
+    return synth.video_numpy(synth.spec("small", nrows=H, ncols=W, n_particles=n), T, seed)
+
+
+def _expected(oracle, video, diameter, separation, minmass, characterize, doughnut,
+              filter_close=True, frames=None):
+    T, H, W = video.shape
+    p = oracle.derive_trackpy(H, W, diameter=diameter, separation=separation, minmass=minmass)
+    out = []
+    for t in (range(T) if frames is None else frames):
+        rows, _ = oracle.detect_frame(video[t], p, characterize=characterize)
+        rows = np.sort(rows, order="seed")
+        if filter_close:
+            rows = rows[oracle.filter_close(rows, float(separation))]
+        cols = [np.full(len(rows), t, np.float32), rows["x"], rows["y"], rows["mass"]]
+        if characterize:
+            cols += [rows["Rg"], rows["raw_mass"], rows["signal"], rows["ecc"]]
+        if doughnut:
+            d = oracle.doughnut(video[t], rows, diameter / 2.0, float(diameter), 0.0)
+            cols += [d[:, 0], d[:, 1], d[:, 2]]
+        out.append(np.stack(cols, 1).astype(np.float32) if len(rows) else
+                   np.zeros((0, len(cols)), np.float32))
+    return np.concatenate(out, 0)
+
+
+def _frame_matrix(df):
+    return df.to_numpy(dtype=np.float64).astype(np.float32)
+
+
+def test_batch_default_columns_and_values(gt, oracle):
+    v = _video()
+    df = gt.batch(v, 9, minmass=300)
+    assert list(df.columns) == ["frame", "y", "x", "mass", "Rg", "raw", "signal", "ecc",
+                                "raw_mass", "raw_bg_median", "raw_mass_corrected"]
+    assert str(df["frame"].dtype) == "int64" and str(df["y"].dtype) == "float32"
+    ref = _expected(oracle, v, 9, 10, 300.0, True, True)
+    got = _frame_matrix(df)
+    assert got.shape == ref.shape
+    assert np.array_equal(got.view(np.uint32), ref.view(np.uint32))
+
+
+def test_batch_without_characterize_or_doughnut(gt, oracle):
+    v = _video(T=3, seed=8)
+    df = gt.batch(v, 7, separation=6, minmass=150, characterize=False,
+                  doughnut_correction=False, filter_close=False)
+    assert list(df.columns) == ["frame", "y", "x", "mass"]
+    ref = _expected(oracle, v, 7, 6, 150.0, False, False, filter_close=False)
+    assert np.array_equal(_frame_matrix(df).view(np.uint32), ref.view(np.uint32))
+
+
+def test_batch_frames_subset_and_order(gt, oracle):
+    v = _video(T=5, seed=9)
+    df = gt.batch(v, 9, minmass=300, frames=[4, 1, 2])
+    assert df["frame"].drop_duplicates().tolist() == [4, 1, 2]
+    ref = _expected(oracle, v, 9, 10, 300.0, True, True, frames=[4, 1, 2])
+    assert np.array_equal(_frame_matrix(df).view(np.uint32), ref.view(np.uint32))
+
+
+def test_batch_errors(gt):
+    v = _video(T=2)
+    with pytest.raises(gt.GtError) as e:
+        gt.batch(v, 9, frames=[0, 7])
+    assert e.value.status == 7  # FrameOutOfBounds
+    with pytest.raises(gt.GtError) as e:
+        gt.batch(v[:0], 9)
+    assert e.value.status == 5  # EmptyIterator
+
+
+def test_batch_accepts_2d_and_float_input(gt, oracle):
+    v = _video(T=1, seed=12)
+    a = gt.batch(v[0], 9, minmass=300)
+    b = gt.batch(v.astype(np.float64), 9, minmass=300)   # cast to f32 like the reference
+    assert np.array_equal(_frame_matrix(a).view(np.uint32), _frame_matrix(b).view(np.uint32))
+
+
+def test_characterize_points_matches_oracle(gt, oracle):
+    import pandas as pd
+    v = _video(T=3, seed=14)
+    df = gt.batch(v, 9, minmass=300, doughnut_correction=False)
+    pts = df[["frame", "y", "x"]].copy()
+    pts["y"] += 0.3
+    out = gt.characterize_points(v, pts, diameter=9, doughnut_correction=False)
+    assert list(out.columns) == ["frame", "y", "x", "mass", "Rg", "raw", "signal", "ecc"]
+    assert len(out) == len(pts)
+    T, H, W = v.shape
+    p = oracle.derive_trackpy(H, W, diameter=9)
+    exp = []
+    for t in range(T):
+        sel = pts[pts["frame"] == t]
+        rows = np.zeros(len(sel), oracle.ROW_DTYPE)
+        rows["x"] = sel["y"].to_numpy(np.float32)
+        rows["y"] = sel["x"].to_numpy(np.float32)
+        P = oracle.bandpass(v[t], p)
+        rows = oracle.characterize(P, v[t], p, rows, use_f64=True, recompute_mass=True)
+        exp.append(np.stack([np.full(len(rows), t, np.float32), rows["x"], rows["y"], rows["mass"],
+                             rows["Rg"], rows["raw_mass"], rows["signal"], rows["ecc"]], 1))
+    exp = np.concatenate(exp).astype(np.float32)
+    got = _frame_matrix(out)
+    assert np.array_equal(got.view(np.uint32), exp.view(np.uint32))
+
+
+def test_characterize_points_without_frames_and_with_r(gt, oracle):
+    v = _video(T=2, seed=15)
+    pts = np.array([[50.2, 60.7, 3.0], [100.5, 120.5, 4.5], [20.0, 30.0, 2.2]], np.float32)
+    out = gt.characterize_points(v, pts, has_frames=False, has_r=True,
+                                 doughnut_correction=False)
+    # every frame gets every point; diameter defaults to 2*int(max r + 0.5) + 1
+    assert list(out.columns) == ["frame", "y", "x", "mass", "r", "Rg", "raw", "signal", "ecc"]
+    assert out["frame"].tolist() == [0, 0, 0, 1, 1, 1]
+    assert np.allclose(out["r"].to_numpy()[:3], pts[:, 2])
+    p = oracle.derive_trackpy(v.shape[1], v.shape[2], diameter=2 * int(4.5 + 0.5) + 1)
+    rows = np.zeros(3, oracle.ROW_DTYPE)
+    rows["x"], rows["y"], rows["r"] = pts[:, 0], pts[:, 1], pts[:, 2]
+    P = oracle.bandpass(v[1], p)
+    ref = oracle.characterize(P, v[1], p, rows, use_f64=True, recompute_mass=True)
+    got = out[out["frame"] == 1]
+    assert np.array_equal(got["mass"].to_numpy(np.float32), ref["mass"])
+    assert np.array_equal(got["ecc"].to_numpy(np.float32), ref["ecc"])
+
+
+def test_characterize_points_unsorted_frames_rejected(gt):
+    v = _video(T=3, seed=16)
+    pts = np.array([[1, 50, 50], [0, 60, 60]], np.float32)
+    with pytest.raises(gt.GtError) as e:
+        gt.characterize_points(v, pts, diameter=9, has_frames=True)
+    assert e.value.status == 6  # NonSortedCharacterization
