@@ -137,6 +137,14 @@ void gt_host_fill_kparams(const gt_derived *d, uint32_t H, uint32_t W, GtKParams
     }
     kp->gnorm = norm;
     kp->box_area = static_cast<float>(static_cast<int32_t>(d->smoothing) * static_cast<int32_t>(d->smoothing));
+    kp->rgnorm = 1.0f / kp->gnorm;
+    kp->rbox = 1.0f / kp->box_area;
+    // Integer frames let the bandpass divide by the constant norms with a
+    // reciprocal and two fma corrections; that needs normal (not tiny)
+    // dividends, which holds when every nonzero weight is >= 2^-100.
+    kp->fast_div = 1;
+    for (int32_t i = 0; i <= 2 * R; ++i)
+        if (kp->w[i] != 0.0f && kp->w[i] < 7.888609052210118e-31f) kp->fast_div = 0;
     kp->truncate = d->truncate;
     kp->k = static_cast<int32_t>(d->dilation);
     kp->a = f2i_sat(static_cast<float>(kp->k) / 2.0f - 0.5f);

@@ -32,6 +32,14 @@ int gt_tile_halo(const GtKParams &p);
 size_t gt_tile_lds_bytes(const GtKParams &p);
 hipError_t gt_launch_detect_tile(const float *proc, const void *raw, int dtype, uint32_t nframes,
                                  const GtKParams &p, gt_detection *out, uint32_t out_cap,
-                                 uint32_t *ocount, uint32_t frame0, hipStream_t s);
+                                 uint32_t *ocount, uint2 *slow, uint32_t slow_cap,
+                                 uint32_t *scount, uint32_t frame0, hipStream_t s);
+// Finishes the walks the tile kernel handed over (slow list: seed, position).
+hipError_t gt_launch_walk_slow(const float *proc, const void *raw, int dtype, uint32_t nframes,
+                               const GtKParams &p, const uint2 *slow, uint32_t slow_cap,
+                               const uint32_t *scount, gt_detection *out, uint32_t out_cap,
+                               uint32_t *ocount, uint32_t frame0, hipStream_t s);
 hipError_t gt_launch_copy_counts(const uint32_t *strided, uint32_t *dst, uint32_t n,
                                  hipStream_t s);
+// Fills the constant tables the kernels read (once per device).
+hipError_t gt_init_tables();

@@ -148,6 +148,151 @@ __global__ __launch_bounds__(BP_THREADS) void k_bandpass(const T *__restrict__ f
 }
 
 // ---------------------------------------------------------------------------
+// Bandpass for integer frames (u8/u16) and radius R <= 7, register blocked.
+//
+// Integer input makes two things exact that the generic kernel cannot assume:
+//   * every boxcar partial sum is an integer below 2^24 ((2R+1)^2 * 65535 <
+//     2^24 for R <= 7), so running sums give the sequential sum's bits;
+//   * every dividend is 0 or at least min(w) > 2^-100, so division by the
+//     constant norm can use the correctly rounded reciprocal-and-two-
+//     corrections sequence (exact for these operands; checked against IEEE
+//     division in the CPU test suite) instead of the general division.
+// The Gaussian taps stay sequential fmaf chains, as in the oracle.
+constexpr int B2_TW = 64, B2_TH = 32, B2_THREADS = 256, B2_ROWS = 8;
+
+// a / b correctly rounded, given rb = RN(1/b); a >= 0 and not subnormal.
+__device__ __forceinline__ float div_const(float a, float b, float rb) {
+    float q = a * rb;
+    float r = fmaf(-q, b, a);
+    q = fmaf(r, rb, q);
+    r = fmaf(-q, b, a);
+    return fmaf(r, rb, q);
+}
+
+template <typename T, int R>
+__global__ __launch_bounds__(B2_THREADS) void k_bandpass_int(const T *__restrict__ frames,
+                                                             float *__restrict__ proc,
+                                                             GtKParams p) {
+    constexpr int NT = 2 * R + 1;
+    constexpr int RH = B2_TH + 2 * R;
+    constexpr int RWP = (B2_TW + 2 * R + 3) & ~3;   // staged row pitch, 16-byte aligned
+    constexpr int NV = (NT + 3 + 3) / 4;            // float4 reads per row-pass item
+    static_assert(B2_TW - 4 + 4 * NV <= RWP, "row-pass reads stay inside the staged row");
+    static_assert(B2_TH % B2_ROWS == 0 && (B2_TH / B2_ROWS) * B2_TW == B2_THREADS, "mapping");
+    __shared__ __align__(16) float Sraw[RH * RWP];
+    __shared__ __align__(16) float Sg[RH * B2_TW];
+    __shared__ __align__(16) float Sc[RH * B2_TW];
+    const int H = p.H, W = p.W;
+    const size_t HW = static_cast<size_t>(H) * W;
+    const int x0 = blockIdx.x * B2_TW, y0 = blockIdx.y * B2_TH;
+    const T *f = frames + static_cast<size_t>(blockIdx.z) * HW;
+    float *out = proc + static_cast<size_t>(blockIdx.z) * HW;
+    const int tid = threadIdx.x;
+    {
+        // issue every global load of the thread before the first LDS store
+        constexpr int CN = RH * RWP, NL = (CN + B2_THREADS - 1) / B2_THREADS;
+        T vals[NL];
+#pragma unroll
+        for (int q = 0; q < NL; ++q) {
+            const int idx = min(tid + q * B2_THREADS, CN - 1);
+            const int ry = idx / RWP, rx = idx - ry * RWP;
+            const int gy = min(max(y0 - R + ry, 0), H - 1);
+            const int gx = min(max(x0 - R + rx, 0), W - 1);
+            vals[q] = f[static_cast<size_t>(gy) * W + gx];
+        }
+#pragma unroll
+        for (int q = 0; q < NL; ++q) {
+            const int idx = tid + q * B2_THREADS;
+            if (idx < CN) Sraw[idx] = to_f32(vals[q]);
+        }
+    }
+    __syncthreads();
+    const float gnorm = p.gnorm, rgnorm = p.rgnorm;
+    // Row pass: 4 adjacent columns of one staged row per item.
+    for (int item = tid; item < RH * (B2_TW / 4); item += B2_THREADS) {
+        const int ry = item / (B2_TW / 4), cx = (item % (B2_TW / 4)) * 4;
+        const float4 *src4 = reinterpret_cast<const float4 *>(Sraw + ry * RWP + cx);
+        float v[4 * NV];
+#pragma unroll
+        for (int k = 0; k < NV; ++k) {
+            const float4 t = src4[k];
+            v[4 * k + 0] = t.x;
+            v[4 * k + 1] = t.y;
+            v[4 * k + 2] = t.z;
+            v[4 * k + 3] = t.w;
+        }
+        float gs[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+        const int gx0 = x0 + cx - R;   // frame column of v[0]
+        if (gx0 >= 0 && gx0 + 3 + 2 * R < W) {
+#pragma unroll
+            for (int t = 0; t < NT; ++t)
+#pragma unroll
+                for (int k = 0; k < 4; ++k) gs[k] = fmaf(p.w[t], v[k + t], gs[k]);
+        } else {
+#pragma unroll
+            for (int t = 0; t < NT; ++t)
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    const int px = gx0 + k + t;
+                    if (px >= 0 && px < W) gs[k] = fmaf(p.w[t], v[k + t], gs[k]);
+                }
+        }
+        float cs[4];
+        cs[0] = 0.0f;
+#pragma unroll
+        for (int t = 0; t < NT; ++t) cs[0] = cs[0] + v[t];
+#pragma unroll
+        for (int k = 1; k < 4; ++k) cs[k] = cs[k - 1] - v[k - 1] + v[k - 1 + NT];
+        float4 og, oc;
+        og.x = div_const(gs[0], gnorm, rgnorm);
+        og.y = div_const(gs[1], gnorm, rgnorm);
+        og.z = div_const(gs[2], gnorm, rgnorm);
+        og.w = div_const(gs[3], gnorm, rgnorm);
+        oc.x = cs[0];
+        oc.y = cs[1];
+        oc.z = cs[2];
+        oc.w = cs[3];
+        *reinterpret_cast<float4 *>(Sg + ry * B2_TW + cx) = og;
+        *reinterpret_cast<float4 *>(Sc + ry * B2_TW + cx) = oc;
+    }
+    __syncthreads();
+    // Column pass: B2_ROWS consecutive rows of one column per thread.
+    const int cx = tid % B2_TW, r0 = (tid / B2_TW) * B2_ROWS;
+    float g[B2_ROWS + 2 * R], c[B2_ROWS + 2 * R];
+#pragma unroll
+    for (int t = 0; t < B2_ROWS + 2 * R; ++t) {
+        g[t] = Sg[(r0 + t) * B2_TW + cx];
+        c[t] = Sc[(r0 + t) * B2_TW + cx];
+    }
+    const int gx = x0 + cx;
+    const int gy0 = y0 + r0;           // frame row of output 0; its taps start at gy0 - R
+    const bool inner = gy0 - R >= 0 && gy0 + B2_ROWS - 1 + R < H;
+    const float box = p.box_area, rbox = p.rbox;
+    float csum = 0.0f;
+#pragma unroll
+    for (int t = 0; t < NT; ++t) csum = csum + c[t];
+#pragma unroll
+    for (int r = 0; r < B2_ROWS; ++r) {
+        if (r > 0) csum = csum - c[r - 1] + c[r - 1 + NT];
+        float gs = 0.0f;
+        if (inner) {
+#pragma unroll
+            for (int t = 0; t < NT; ++t) gs = fmaf(p.w[t], g[r + t], gs);
+        } else {
+#pragma unroll
+            for (int t = 0; t < NT; ++t) {
+                const int py = gy0 + r - R + t;
+                if (py >= 0 && py < H) gs = fmaf(p.w[t], g[r + t], gs);
+            }
+        }
+        float res = div_const(gs, gnorm, rgnorm) - div_const(csum, box, rbox);
+        if (p.truncate && res < kTrunc) res = 0.0f;
+        const int gy = gy0 + r;
+        if (gy < H && gx < W) out[static_cast<size_t>(gy) * W + gx] = res;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Local maximum: a pixel inside the margin is a candidate when no in-frame
 // value of its k x k window [-a, k-1-a]^2 is greater than it, and it is not
 // below the snr threshold.
@@ -260,9 +405,66 @@ __device__ __forceinline__ bool walk_move(const WalkRes &res, int &pu, int &pv, 
     return changed;
 }
 
+}  // namespace
+
+// sin(2 atan2(j, i)) and cos(2 atan2(j, i)) for |i|, |j| <= GT_SC_R in closed
+// form, 2ij/(i^2+j^2) and (i^2-j^2)/(i^2+j^2), filled by gt_init_tables().
+__constant__ double2 c_sincos2[(2 * GT_SC_R + 1) * (2 * GT_SC_R + 1)];
+
+namespace {
+
+__device__ __forceinline__ double2 sincos2(int i, int j) {
+    if (i >= -GT_SC_R && i <= GT_SC_R && j >= -GT_SC_R && j <= GT_SC_R)
+        return c_sincos2[(i + GT_SC_R) * (2 * GT_SC_R + 1) + (j + GT_SC_R)];
+    const double q = static_cast<double>(i * i + j * j);
+    return make_double2(2.0 * static_cast<double>(i) * static_cast<double>(j) / q,
+                        (static_cast<double>(i * i) - static_cast<double>(j * j)) / q);
+}
+
 // characterize (characterize.wgsl:61-143) at round-half-even of the walked
 // centre, with the Rg and eccentricity moment sums accumulated in f64.
-template <typename T, class S>
+struct CharAcc {
+    float raw_mass, signal;
+    double rg, es, ec;
+};
+
+// One disk pixel of characterize; `fold` marks a compile-time (i, j), for
+// which the closed-form sin/cos(2 theta) fold to constants.
+template <typename T, class S, bool fold>
+__device__ __forceinline__ void char_accumulate(CharAcc &acc, const S &src,
+                                                const T *__restrict__ Rw, int H, int W, int u,
+                                                int v, int i, int j, float r2) {
+    const float x = static_cast<float>(i), x2 = x * x;
+    const float y = static_cast<float>(j), y2 = y * y;
+    const float mask = x2 + y2;
+    if (mask > r2) return;
+    const int pu = u + i, pv = v + j;
+    if (pu < 0 || pu >= H || pv < 0 || pv >= W) return;
+    const float d = src.at(pu, pv);
+    acc.raw_mass = acc.raw_mass + to_f32(Rw[static_cast<size_t>(pu) * W + pv]);
+    acc.signal = maxf_ref(acc.signal, d);
+    if (i != 0 || j != 0) {
+        double2 sc;
+        if constexpr (fold) {
+            const double q = static_cast<double>(i * i + j * j);
+            sc = make_double2(2.0 * static_cast<double>(i) * static_cast<double>(j) / q,
+                              (static_cast<double>(i * i) - static_cast<double>(j * j)) / q);
+        } else {
+            sc = sincos2(i, j);
+        }
+        acc.es = fma(static_cast<double>(d), sc.x, acc.es);
+        acc.ec = fma(static_cast<double>(d), sc.y, acc.ec);
+    } else {
+        acc.ec = acc.ec + static_cast<double>(d);
+    }
+    acc.rg = fma(static_cast<double>(d), static_cast<double>(mask), acc.rg);
+}
+
+// characterize (characterize.wgsl:61-143) at round-half-even of the walked
+// centre, with the Rg and eccentricity moment sums accumulated in f64.  RI > 0
+// unrolls the disk of rows without their own radius, so its loads issue
+// together.
+template <typename T, int RI = 0, class S>
 __device__ void characterize_row(const S &src, const T *__restrict__ Rw, int H, int W, int circle,
                                  gt_result_row &row) {
     float r = row.r;
@@ -275,39 +477,31 @@ __device__ void characterize_row(const S &src, const T *__restrict__ Rw, int H, 
     }
     const float r2 = r * r;
     const int u = f2i(rintf(row.x)), v = f2i(rintf(row.y));
-    float raw_mass = 0.0f, signal = 0.0f;
-    double rg = 0.0, es = 0.0, ec = 0.0;
-    for (int i = -rint; i <= rint; ++i) {
-        const float x = static_cast<float>(i), x2 = x * x;
-        for (int j = -rint; j <= rint; ++j) {
-            const float y = static_cast<float>(j), y2 = y * y;
-            const float mask = x2 + y2;
-            if (mask > r2) continue;
-            const int pu = u + i, pv = v + j;
-            if (pu < 0 || pu >= H || pv < 0 || pv >= W) continue;
-            const float d = src.at(pu, pv);
-            raw_mass = raw_mass + to_f32(Rw[static_cast<size_t>(pu) * W + pv]);
-            signal = maxf_ref(signal, d);
-            if (i != 0 || j != 0) {
-                // sin(2 atan2(j, i)) and cos(2 atan2(j, i)) in closed form
-                const double q = static_cast<double>(i * i + j * j);
-                const double s2 = 2.0 * static_cast<double>(i) * static_cast<double>(j) / q;
-                const double c2 = (static_cast<double>(i * i) - static_cast<double>(j * j)) / q;
-                es = fma(static_cast<double>(d), s2, es);
-                ec = fma(static_cast<double>(d), c2, ec);
-            } else {
-                ec = ec + static_cast<double>(d);
-            }
-            rg = fma(static_cast<double>(d), static_cast<double>(mask), rg);
+    CharAcc acc = {0.0f, 0.0f, 0.0, 0.0, 0.0};
+    bool done = false;
+    if constexpr (RI > 0) {
+        if (row.r <= 0.0f && rint == RI) {
+            constexpr float r2c = static_cast<float>(RI) * static_cast<float>(RI);
+#pragma unroll
+            for (int i = -RI; i <= RI; ++i)
+#pragma unroll
+                for (int j = -RI; j <= RI; ++j)
+                    char_accumulate<T, S, true>(acc, src, Rw, H, W, u, v, i, j, r2c);
+            done = true;
         }
+    }
+    if (!done) {
+        for (int i = -rint; i <= rint; ++i)
+            for (int j = -rint; j <= rint; ++j)
+                char_accumulate<T, S, false>(acc, src, Rw, H, W, u, v, i, j, r2);
     }
     // the reference reads the centre sample without a bounds check
     const int cu = min(max(u, 0), H - 1), cv = min(max(v, 0), W - 1);
     const float den = row.mass - src.at(cu, cv);
-    row.ecc = static_cast<float>(sqrt(es * es + ec * ec) / static_cast<double>(den));
-    row.Rg = static_cast<float>(sqrt(rg / static_cast<double>(row.mass)));
-    row.raw_mass = raw_mass;
-    row.signal = signal;
+    row.ecc = static_cast<float>(sqrt(acc.es * acc.es + acc.ec * acc.ec) / static_cast<double>(den));
+    row.Rg = static_cast<float>(sqrt(acc.rg / static_cast<double>(row.mass)));
+    row.raw_mass = acc.raw_mass;
+    row.signal = acc.signal;
 }
 
 // minmass rule (walk.wgsl:200-213)
@@ -377,7 +571,7 @@ __global__ __launch_bounds__(WK_THREADS) void k_walk(const float *__restrict__ p
             keep = passes_minmass(p, res);
             if (keep) {
                 fill_row(d, res, seed, frame0 + fr);
-                if (p.characterize) characterize_row<T>(src, Rw, H, W, p.circle, d.row);
+                if (p.characterize) characterize_row<T, RI>(src, Rw, H, W, p.circle, d.row);
             }
         }
         const uint32_t pos = wave_append(keep, &ocount[fr * GT_COUNTER_STRIDE]);
@@ -389,39 +583,130 @@ __global__ __launch_bounds__(WK_THREADS) void k_walk(const float *__restrict__ p
 // Fused local max + walk + characterize.  One block per FT_TH x FT_TW tile:
 //   1. the processed frame over the tile plus an h-pixel halo is staged in LDS
 //      (out-of-frame entries -inf);
-//   2. row maxima over the k-wide dilation window, then the column test, give
-//      the tile's candidates, compacted into an LDS list in raster order with
-//      one ballot per wave and a block prefix (no atomics);
-//   3. one lane per candidate walks; every disk evaluation that lies inside
-//      the staged region reads LDS, the rest read the frame in global memory
-//      (same arithmetic, so results do not depend on the tile);
-//   4. detections are characterized and appended to the frame's row list with
-//      one atomic per block.
-constexpr int FT_TW = 64, FT_TH = 32, FT_THREADS = 256;
+//   2. wave w owns tile rows [16w, 16w + 16); each lane takes one column,
+//      folds the row maxima of the dilation window into its pixels' window
+//      maxima in registers and tests them; the block's candidates are
+//      compacted into one LDS list in raster order (ballots + 4-entry prefix);
+//   3. the first ceil(n/64) waves walk the list, one lane per candidate, for
+//      at most FT_WALK_STEPS steps; a disk that lies inside the staged region
+//      is read from LDS, any other from the frame in global memory (same
+//      arithmetic either way).  A walk still moving after those steps is
+//      handed to k_walk_slow through a per-frame list (or, when that list is
+//      full, walked on here);
+//   4. finished walks are characterized and appended to the frame's rows
+//      with one atomic per wave.  Waves without candidates leave early.
+constexpr int FT_TW = 64, FT_TH = 64, FT_THREADS = 256;
 constexpr int FT_WAVES = FT_THREADS / 64;
-constexpr int FT_ITERS = FT_TH * FT_TW / FT_THREADS;
+constexpr int FT_WROWS = FT_TH / FT_WAVES;
+constexpr uint32_t FT_WALK_STEPS = 2;
 
-template <typename T, int KT, int RI>
+// Centroid of the disk around (u, v) when the whole disk is inside the frame:
+// no per-pixel bounds test, every offset a compile-time constant.  Same
+// operations in the same order as get_center, so the same bits.
+template <int RI, class B>
+__device__ __forceinline__ WalkRes get_center_inner(const B &base) {
+    WalkRes res = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    constexpr float r2 = static_cast<float>(RI) * static_cast<float>(RI);
+#pragma unroll
+    for (int i = -RI; i <= RI; ++i) {
+#pragma unroll
+        for (int j = -RI; j <= RI; ++j) {
+            const float x = static_cast<float>(i), y = static_cast<float>(j);
+            if (x * x + y * y > r2) continue;
+            const float d = base(i, j);
+            res.x = fmaf(d, x, res.x);
+            res.y = fmaf(d, y, res.y);
+            res.mass = res.mass + d;
+            res.count = res.count + 1.0f;
+            // res.max starts at +0 and is never NaN, so fmaxf agrees with the
+            // reference's max() here
+            res.max = fmaxf(res.max, d);
+        }
+    }
+    res.x = res.x / res.mass;
+    res.y = res.y / res.mass;
+    return res;
+}
+
+template <int PITCH>
+struct LdsRow {
+    const float *c;  // LDS address of the disk centre
+    __device__ __forceinline__ float operator()(int i, int j) const { return c[i * PITCH + j]; }
+};
+struct GlobalRow {
+    const float *c;
+    int W;
+    __device__ __forceinline__ float operator()(int i, int j) const {
+        return c[static_cast<long long>(i) * W + j];
+    }
+};
+
+// One walk evaluation at (pu, pv) from the best available source.
+template <int RI, int HT>
+__device__ __forceinline__ WalkRes walk_eval(const GtKParams &p, const float *__restrict__ P,
+                                             const float *Sp, int oy, int ox, int RW, int ylo,
+                                             int yhi, int xlo, int xhi, int pu, int pv) {
+    const int H = p.H, W = p.W;
+    if constexpr (RI > 0) {
+        const bool in_frame = pu >= RI && pu + RI < H && pv >= RI && pv + RI < W;
+        if (in_frame) {
+            if (pu >= ylo && pu <= yhi && pv >= xlo && pv <= xhi) {
+                if constexpr (HT > 0) {
+                    constexpr int PITCH = FT_TW + 2 * HT;
+                    return get_center_inner<RI>(LdsRow<PITCH>{Sp + (pu - oy) * PITCH + (pv - ox)});
+                }
+            }
+            return get_center_inner<RI>(GlobalRow{P + static_cast<size_t>(pu) * W + pv, W});
+        }
+    }
+    if (pu >= ylo && pu <= yhi && pv >= xlo && pv <= xhi)
+        return get_center<RI>(LdsSrc{Sp, oy, ox, RW}, H, W, pu, pv, p.rint, p.r2);
+    return get_center<RI>(GlobalSrc{P, W}, H, W, pu, pv, p.rint, p.r2);
+}
+
+// Finish a walk at integer position (pu, pv) with last evaluation `res`:
+// minmass test, then the row (characterized when asked).
+template <typename T, int RI>
+__device__ __forceinline__ bool finish_walk(const GtKParams &p, const float *__restrict__ P,
+                                            const T *__restrict__ Rw, const float *Sp, int oy,
+                                            int ox, int RW, int ylo, int yhi, int xlo, int xhi,
+                                            WalkRes res, int pu, int pv, uint32_t seed,
+                                            uint32_t frame, gt_detection &d) {
+    res.x = res.x + static_cast<float>(pu);
+    res.y = res.y + static_cast<float>(pv);
+    if (!passes_minmass(p, res)) return false;
+    fill_row(d, res, seed, frame);
+    if (p.characterize) {
+        const int cu = f2i(rintf(d.row.x)), cv = f2i(rintf(d.row.y));
+        if (Sp && cu >= ylo && cu <= yhi && cv >= xlo && cv <= xhi)
+            characterize_row<T, RI>(LdsSrc{Sp, oy, ox, RW}, Rw, p.H, p.W, p.circle, d.row);
+        else
+            characterize_row<T, RI>(GlobalSrc{P, p.W}, Rw, p.H, p.W, p.circle, d.row);
+    }
+    return true;
+}
+
+template <typename T, int KT, int RI, int HT>
 __global__ __launch_bounds__(FT_THREADS) void k_detect_tile(const float *__restrict__ proc,
                                                             const T *__restrict__ raw,
-                                                            GtKParams p, int h,
+                                                            GtKParams p, int h_rt,
                                                             gt_detection *__restrict__ out,
                                                             uint32_t out_cap,
                                                             uint32_t *__restrict__ ocount,
+                                                            uint2 *__restrict__ slow,
+                                                            uint32_t slow_cap,
+                                                            uint32_t *__restrict__ scount,
                                                             uint32_t frame0) {
     extern __shared__ float lds[];
-    __shared__ uint32_t s_wave_cnt[FT_ITERS * FT_WAVES];
-    __shared__ uint32_t s_wave_off[FT_ITERS * FT_WAVES + 1];
     __shared__ uint16_t s_cand[FT_TH * FT_TW];
-    __shared__ uint32_t s_keep_cnt[FT_WAVES];
-    __shared__ uint32_t s_base;
+    __shared__ uint32_t s_cnt[FT_WAVES];
+    constexpr bool kInt = sizeof(T) <= 2;  // integer frames: no NaN in the processed frame
     const int k = KT > 0 ? KT : p.k;
     const int a = p.a;
     const int rint = RI > 0 ? RI : p.rint;
+    const int h = HT > 0 ? HT : h_rt;
     const int RW = FT_TW + 2 * h, RH = FT_TH + 2 * h;
-    const int MR = FT_TH + k - 1;  // rows of row-maxima needed by the tile
-    float *Sp = lds;               // RH x RW
-    float *Sm = Sp + RH * RW;      // MR x FT_TW
+    float *Sp = lds;  // RH x RW, origin (y0 - h, x0 - h)
     const int H = p.H, W = p.W;
     const size_t HW = static_cast<size_t>(H) * W;
     const int x0 = blockIdx.x * FT_TW, y0 = blockIdx.y * FT_TH;
@@ -430,129 +715,211 @@ __global__ __launch_bounds__(FT_THREADS) void k_detect_tile(const float *__restr
     const T *Rw = raw + static_cast<size_t>(fr) * HW;
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int oy = y0 - h, ox = x0 - h;
+    const bool interior = oy >= 0 && ox >= 0 && oy + RH <= H && ox + RW <= W;
 
-    for (int idx = tid; idx < RH * RW; idx += FT_THREADS) {
-        const int ry = idx / RW, rx = idx - ry * RW;
-        const int gy = oy + ry, gx = ox + rx;
-        Sp[idx] = (gy >= 0 && gy < H && gx >= 0 && gx < W) ? P[static_cast<size_t>(gy) * W + gx]
-                                                           : -INFINITY;
-    }
-    __syncthreads();
-    // row maxima: Sm[m][cx] = max over j < k of Sp[h - a + m][h + cx - a + j]
-    for (int idx = tid; idx < MR * FT_TW; idx += FT_THREADS) {
-        const int m = idx / FT_TW, cx = idx - m * FT_TW;
-        const float *s = Sp + (h - a + m) * RW + (h + cx - a);
-        float mx = s[a];
-        if (KT > 0) {
+    if constexpr (HT > 0) {
+        // fixed region: every load of the thread is issued before the first
+        // LDS store, so their latencies overlap
+        constexpr int CRW = FT_TW + 2 * HT, CN = (FT_TH + 2 * HT) * CRW;
+        constexpr int NL = (CN + FT_THREADS - 1) / FT_THREADS;
+        float vals[NL];
+        if (interior) {
+            const float *src = P + static_cast<size_t>(oy) * W + ox;
 #pragma unroll
-            for (int j = 0; j < KT; ++j) mx = maxf_ref(mx, s[j]);
+            for (int q = 0; q < NL; ++q) {
+                const int idx = min(tid + q * FT_THREADS, CN - 1);
+                const int ry = idx / CRW, rx = idx - ry * CRW;
+                vals[q] = src[static_cast<size_t>(ry) * W + rx];
+            }
         } else {
-            for (int j = 0; j < k; ++j) mx = maxf_ref(mx, s[j]);
+#pragma unroll
+            for (int q = 0; q < NL; ++q) {
+                const int idx = tid + q * FT_THREADS;
+                const int ry = idx / CRW, rx = idx - ry * CRW;
+                const int gy = oy + ry, gx = ox + rx;
+                vals[q] = (idx < CN && gy >= 0 && gy < H && gx >= 0 && gx < W)
+                              ? P[static_cast<size_t>(gy) * W + gx]
+                              : -INFINITY;
+            }
         }
-        Sm[idx] = mx;
+#pragma unroll
+        for (int q = 0; q < NL; ++q) {
+            const int idx = tid + q * FT_THREADS;
+            if (idx < CN) Sp[idx] = vals[q];
+        }
+    } else {
+        for (int idx = tid; idx < RH * RW; idx += FT_THREADS) {
+            const int ry = idx / RW, rx = idx - ry * RW;
+            const int gy = oy + ry, gx = ox + rx;
+            Sp[idx] = (gy >= 0 && gy < H && gx >= 0 && gx < W)
+                          ? P[static_cast<size_t>(gy) * W + gx]
+                          : -INFINITY;
+        }
     }
     __syncthreads();
-    // candidate test; pixel idx = it * 256 + tid is row it*4 + wave, column lane
-    uint64_t masks[FT_ITERS];
+
+    // Candidate test for tile rows r0..r0+15 of column `lane`: window row q
+    // (frame row y0 + r0 - a + q) enters the window maximum of output row r
+    // when 0 <= q - r < k.
+    const int r0 = wave * FT_WROWS;
+    float wmax[FT_WROWS];
+#pragma unroll
+    for (int r = 0; r < FT_WROWS; ++r) wmax[r] = -INFINITY;
+    const float *rowbase = Sp + (h + r0 - a) * RW + (h + lane - a);
+    auto vmax = [](float x, float y) { return kInt ? fmaxf(x, y) : maxf_ref(x, y); };
+    if constexpr (KT > 0) {
+#pragma unroll
+        for (int q = 0; q < FT_WROWS + KT - 1; ++q) {
+            const float *s = rowbase + q * RW;
+            float m = s[a];
+#pragma unroll
+            for (int j = 0; j < KT; ++j) m = vmax(m, s[j]);
+#pragma unroll
+            for (int r = 0; r < FT_WROWS; ++r)
+                if (q - r >= 0 && q - r < KT) wmax[r] = vmax(wmax[r], m);
+        }
+    } else {
+        for (int q = 0; q < FT_WROWS + k - 1; ++q) {
+            const float *s = rowbase + q * RW;
+            float m = s[a];
+            for (int j = 0; j < k; ++j) m = vmax(m, s[j]);
+#pragma unroll
+            for (int r = 0; r < FT_WROWS; ++r)
+                if (q - r >= 0 && q - r < k) wmax[r] = vmax(wmax[r], m);
+        }
+    }
     const int margin = p.margin;
     const float thr = p.max_thr;
+    const int gx = x0 + lane;
+    uint64_t masks[FT_WROWS];
+    uint32_t nwave = 0;  // wave-uniform
 #pragma unroll
-    for (int it = 0; it < FT_ITERS; ++it) {
-        const int ty = it * FT_WAVES + wave, cx = lane;
-        const int gy = y0 + ty, gx = x0 + cx;
+    for (int r = 0; r < FT_WROWS; ++r) {
+        const int gy = y0 + r0 + r;
         bool flag = false;
         if (gy < H && gx < W && gy >= margin && gy < H - margin && gx >= margin &&
             gx < W - margin) {
-            const float c = Sp[(h + ty) * RW + h + cx];
-            if (!(c < thr)) {
-                flag = true;
-                const float *col = Sm + ty * FT_TW + cx;
-                if (KT > 0) {
+            const float c = Sp[(h + r0 + r) * RW + h + lane];
+            flag = !(c < thr) && !(wmax[r] > c);
+        }
+        masks[r] = __ballot(flag);
+        nwave += static_cast<uint32_t>(__popcll(masks[r]));
+    }
+    if (lane == 0) s_cnt[wave] = nwave;
+    __syncthreads();
+    uint32_t off = 0, ncand = 0;
 #pragma unroll
-                    for (int i = 0; i < KT; ++i) flag = flag && !(col[i * FT_TW] > c);
-                } else {
-                    for (int i = 0; i < k; ++i) flag = flag && !(col[i * FT_TW] > c);
-                }
-            }
-        }
-        masks[it] = __ballot(flag);
-        if (lane == 0) s_wave_cnt[it * FT_WAVES + wave] = __popcll(masks[it]);
+    for (int w = 0; w < FT_WAVES; ++w) {
+        const uint32_t cw = s_cnt[w];
+        off += w < wave ? cw : 0u;
+        ncand += cw;
     }
-    __syncthreads();
-    if (tid == 0) {
-        uint32_t acc = 0;
-        for (int q = 0; q < FT_ITERS * FT_WAVES; ++q) {
-            s_wave_off[q] = acc;
-            acc += s_wave_cnt[q];
-        }
-        s_wave_off[FT_ITERS * FT_WAVES] = acc;
-    }
-    __syncthreads();
 #pragma unroll
-    for (int it = 0; it < FT_ITERS; ++it) {
-        const uint64_t m = masks[it];
-        if ((m >> lane) & 1ull) {
-            const uint32_t pos = s_wave_off[it * FT_WAVES + wave] +
-                                 __popcll(m & ((1ull << lane) - 1ull));
-            s_cand[pos] = static_cast<uint16_t>((it * FT_WAVES + wave) * FT_TW + lane);
-        }
+    for (int r = 0; r < FT_WROWS; ++r) {
+        if ((masks[r] >> lane) & 1ull)
+            s_cand[off + __popcll(masks[r] & ((1ull << lane) - 1ull))] =
+                static_cast<uint16_t>((r0 + r) * FT_TW + lane);
+        off += static_cast<uint32_t>(__popcll(masks[r]));
     }
     __syncthreads();
-    const uint32_t ncand = s_wave_off[FT_ITERS * FT_WAVES];
-    const GlobalSrc gsrc{P, W};
-    const LdsSrc lsrc{Sp, oy, ox, RW};
-    // a disk around (u, v) lies in the staged region when all of it does
+    if (static_cast<uint32_t>(wave) * 64u >= ncand) return;  // nothing left for this wave
+
     const int ylo = oy + rint, yhi = oy + RH - 1 - rint;
     const int xlo = ox + rint, xhi = ox + RW - 1 - rint;
     const uint32_t frame = frame0 + fr;
     uint32_t *counter = &ocount[fr * GT_COUNTER_STRIDE];
+    uint32_t *slow_counter = &scount[fr * GT_COUNTER_STRIDE];
+    uint2 *fslow = slow + static_cast<size_t>(fr) * slow_cap;
     gt_detection *fout = out + static_cast<size_t>(fr) * out_cap;
-    for (uint32_t c0 = 0; c0 < ncand; c0 += FT_THREADS) {
-        const uint32_t ci = c0 + tid;
-        bool keep = false;
+    for (uint32_t c0 = static_cast<uint32_t>(wave) * 64u; c0 < ncand; c0 += FT_THREADS) {
+        const uint32_t ci = c0 + static_cast<uint32_t>(lane);
+        bool keep = false, defer = false;
         gt_detection d;
+        int pu = 0, pv = 0;
+        uint32_t seed = 0;
         if (ci < ncand) {
             const int li = s_cand[ci];
             const int ty = li / FT_TW, cx = li - ty * FT_TW;
-            int pu = y0 + ty, pv = x0 + cx;
-            const uint32_t seed = static_cast<uint32_t>(pu) * static_cast<uint32_t>(W) + pv;
+            pu = y0 + ty;
+            pv = x0 + cx;
+            seed = static_cast<uint32_t>(pu) * static_cast<uint32_t>(W) + pv;
             WalkRes res = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
-            for (uint32_t it = 0; it < p.max_iter; ++it) {
-                if (pu >= ylo && pu <= yhi && pv >= xlo && pv <= xhi)
-                    res = get_center<RI>(lsrc, H, W, pu, pv, p.rint, p.r2);
-                else
-                    res = get_center<RI>(gsrc, H, W, pu, pv, p.rint, p.r2);
-                if (!walk_move(res, pu, pv, H, W, rint, p.shift_thr)) break;
-            }
-            res.x = res.x + static_cast<float>(pu);
-            res.y = res.y + static_cast<float>(pv);
-            keep = passes_minmass(p, res);
-            if (keep) {
-                fill_row(d, res, seed, frame);
-                if (p.characterize) {
-                    const int cu = f2i(rintf(d.row.x)), cv = f2i(rintf(d.row.y));
-                    if (cu >= ylo && cu <= yhi && cv >= xlo && cv <= xhi)
-                        characterize_row<T>(lsrc, Rw, H, W, p.circle, d.row);
-                    else
-                        characterize_row<T>(gsrc, Rw, H, W, p.circle, d.row);
+            bool moving = true;
+            uint32_t it = 0;
+            for (; it < p.max_iter && it < FT_WALK_STEPS; ++it) {
+                res = walk_eval<RI, HT>(p, P, Sp, oy, ox, RW, ylo, yhi, xlo, xhi, pu, pv);
+                if (!walk_move(res, pu, pv, H, W, rint, p.shift_thr)) {
+                    moving = false;
+                    break;
                 }
             }
+            if (moving && it < p.max_iter) defer = true;
+            else keep = finish_walk<T, RI>(p, P, Rw, Sp, oy, ox, RW, ylo, yhi, xlo, xhi, res, pu,
+                                           pv, seed, frame, d);
         }
-        const uint64_t km = __ballot(keep);
-        if (lane == 0) s_keep_cnt[wave] = __popcll(km);
-        __syncthreads();
-        if (tid == 0) {
-            uint32_t tot = 0;
-            for (int q = 0; q < FT_WAVES; ++q) tot += s_keep_cnt[q];
-            s_base = tot ? atomicAdd(counter, tot) : 0u;
+        // hand long walks to k_walk_slow; if its list is full, walk on here
+        const uint32_t spos = wave_append(defer, slow_counter);
+        if (defer) {
+            if (spos < slow_cap) {
+                fslow[spos] = make_uint2(seed, static_cast<uint32_t>(pu) * static_cast<uint32_t>(W) + pv);
+            } else {
+                WalkRes res = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+                for (uint32_t it = FT_WALK_STEPS; it < p.max_iter; ++it) {
+                    res = walk_eval<RI, HT>(p, P, Sp, oy, ox, RW, ylo, yhi, xlo, xhi, pu, pv);
+                    if (!walk_move(res, pu, pv, H, W, rint, p.shift_thr)) break;
+                }
+                keep = finish_walk<T, RI>(p, P, Rw, Sp, oy, ox, RW, ylo, yhi, xlo, xhi, res, pu,
+                                          pv, seed, frame, d);
+            }
         }
-        __syncthreads();
-        if (keep) {
-            uint32_t pos = s_base + __popcll(km & ((1ull << lane) - 1ull));
-            for (int q = 0; q < wave; ++q) pos += s_keep_cnt[q];
-            if (pos < out_cap) fout[pos] = d;
+        const uint32_t pos = wave_append(keep, counter);
+        if (keep && pos < out_cap) fout[pos] = d;
+    }
+}
+
+// Continues the walks k_detect_tile handed over: each entry is the seed and
+// the position reached after FT_WALK_STEPS steps; the remaining steps read the
+// frame in global memory.
+template <typename T, int RI>
+__global__ __launch_bounds__(WK_THREADS) void k_walk_slow(const float *__restrict__ proc,
+                                                          const T *__restrict__ raw, GtKParams p,
+                                                          const uint2 *__restrict__ slow,
+                                                          uint32_t slow_cap,
+                                                          const uint32_t *__restrict__ scount,
+                                                          gt_detection *__restrict__ out,
+                                                          uint32_t out_cap,
+                                                          uint32_t *__restrict__ ocount,
+                                                          uint32_t frame0) {
+    const int fr = blockIdx.y;
+    const int H = p.H, W = p.W;
+    const size_t HW = static_cast<size_t>(H) * W;
+    const float *P = proc + static_cast<size_t>(fr) * HW;
+    const T *Rw = raw + static_cast<size_t>(fr) * HW;
+    const uint32_t n = min(scount[fr * GT_COUNTER_STRIDE], slow_cap);
+    const uint2 *fs = slow + static_cast<size_t>(fr) * slow_cap;
+    gt_detection *fout = out + static_cast<size_t>(fr) * out_cap;
+    uint32_t *counter = &ocount[fr * GT_COUNTER_STRIDE];
+    const int rint = RI > 0 ? RI : p.rint;
+    const uint32_t lane = __lane_id();
+    const uint32_t stride = gridDim.x * WK_THREADS;
+    for (uint32_t i0 = blockIdx.x * WK_THREADS + (threadIdx.x & ~63u); i0 < n; i0 += stride) {
+        const uint32_t i = i0 + lane;
+        bool keep = false;
+        gt_detection d;
+        if (i < n) {
+            const uint2 e = fs[i];
+            int pu = static_cast<int>(e.y / static_cast<uint32_t>(W));
+            int pv = static_cast<int>(e.y - static_cast<uint32_t>(pu) * static_cast<uint32_t>(W));
+            WalkRes res = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+            for (uint32_t it = FT_WALK_STEPS; it < p.max_iter; ++it) {
+                res = walk_eval<RI, 0>(p, P, nullptr, 0, 0, 0, 1, 0, 1, 0, pu, pv);
+                if (!walk_move(res, pu, pv, H, W, rint, p.shift_thr)) break;
+            }
+            keep = finish_walk<T, RI>(p, P, Rw, nullptr, 0, 0, 0, 1, 0, 1, 0, res, pu, pv, e.x,
+                                      frame0 + fr, d);
         }
-        __syncthreads();
+        const uint32_t pos = wave_append(keep, counter);
+        if (keep && pos < out_cap) fout[pos] = d;
     }
 }
 
@@ -611,6 +978,21 @@ __global__ void k_zero_u32(uint32_t *p, size_t n) {
 template <typename T>
 hipError_t launch_bandpass_t(const T *frames, float *proc, uint32_t nframes, const GtKParams &p,
                              hipStream_t s) {
+    if constexpr (sizeof(T) <= 2) {
+        if (p.fast_div && p.R >= 1 && p.R <= 7) {
+            const dim3 g2((p.W + B2_TW - 1) / B2_TW, (p.H + B2_TH - 1) / B2_TH, nframes);
+            switch (p.R) {
+            case 1: k_bandpass_int<T, 1><<<g2, B2_THREADS, 0, s>>>(frames, proc, p); break;
+            case 2: k_bandpass_int<T, 2><<<g2, B2_THREADS, 0, s>>>(frames, proc, p); break;
+            case 3: k_bandpass_int<T, 3><<<g2, B2_THREADS, 0, s>>>(frames, proc, p); break;
+            case 4: k_bandpass_int<T, 4><<<g2, B2_THREADS, 0, s>>>(frames, proc, p); break;
+            case 5: k_bandpass_int<T, 5><<<g2, B2_THREADS, 0, s>>>(frames, proc, p); break;
+            case 6: k_bandpass_int<T, 6><<<g2, B2_THREADS, 0, s>>>(frames, proc, p); break;
+            default: k_bandpass_int<T, 7><<<g2, B2_THREADS, 0, s>>>(frames, proc, p); break;
+            }
+            return hipGetLastError();
+        }
+    }
     const dim3 grid((p.W + BP_TW - 1) / BP_TW, (p.H + BP_TH - 1) / BP_TH, nframes);
     const int RH = BP_TH + 2 * p.R, RW = BP_TW + 2 * p.R;
     const size_t lds = sizeof(float) * (static_cast<size_t>(RH) * RW + 2u * RH * BP_TW);
@@ -693,9 +1075,8 @@ int gt_tile_halo(const GtKParams &p) {
 
 size_t gt_tile_lds_bytes(const GtKParams &p) {
     const int h = gt_tile_halo(p);
-    const size_t dyn = sizeof(float) * (static_cast<size_t>(FT_TH + 2 * h) * (FT_TW + 2 * h) +
-                                        static_cast<size_t>(FT_TH + p.k - 1) * FT_TW);
-    const size_t stat = sizeof(uint16_t) * FT_TH * FT_TW + 1024;
+    const size_t dyn = sizeof(float) * static_cast<size_t>(FT_TH + 2 * h) * (FT_TW + 2 * h);
+    const size_t stat = sizeof(uint16_t) * FT_TH * FT_TW + 64;
     return dyn + stat;
 }
 
@@ -703,28 +1084,44 @@ namespace {
 template <typename T>
 hipError_t launch_detect_tile_t(const float *proc, const T *raw, uint32_t nframes,
                                 const GtKParams &p, gt_detection *out, uint32_t out_cap,
-                                uint32_t *ocount, uint32_t frame0, hipStream_t s) {
+                                uint32_t *ocount, uint2 *slow, uint32_t slow_cap,
+                                uint32_t *scount, uint32_t frame0, hipStream_t s) {
     const int h = gt_tile_halo(p);
     const dim3 grid((p.W + FT_TW - 1) / FT_TW, (p.H + FT_TH - 1) / FT_TH, nframes);
-    const size_t lds = sizeof(float) * (static_cast<size_t>(FT_TH + 2 * h) * (FT_TW + 2 * h) +
-                                        static_cast<size_t>(FT_TH + p.k - 1) * FT_TW);
-#define GT_TILE(KV, RV)                                                                      \
-    k_detect_tile<T, KV, RV><<<grid, FT_THREADS, lds, s>>>(proc, raw, p, h, out, out_cap, \
-                                                            ocount, frame0)
-    if (p.rint == 4) {
-        switch (p.k) {
-        case 9: GT_TILE(9, 4); break;
-        case 12: GT_TILE(12, 4); break;
-        default: GT_TILE(0, 4); break;
-        }
+    const size_t lds = sizeof(float) * static_cast<size_t>(FT_TH + 2 * h) * (FT_TW + 2 * h);
+#define GT_TILE(KV, RV, HV)                                                                       \
+    k_detect_tile<T, KV, RV, HV><<<grid, FT_THREADS, lds, s>>>(proc, raw, p, h, out, out_cap, \
+                                                                ocount, slow, slow_cap, scount, \
+                                                                frame0)
+    // diameter 9 with separation 7 (k = 9) or 9 (k = 12): halo 6
+    if (p.rint == 4 && h == 6 && p.k == 9) {
+        GT_TILE(9, 4, 6);
+    } else if (p.rint == 4 && h == 6 && p.k == 12) {
+        GT_TILE(12, 4, 6);
+    } else if (p.rint == 4) {
+        GT_TILE(0, 4, 0);
     } else {
-        switch (p.k) {
-        case 9: GT_TILE(9, 0); break;
-        case 12: GT_TILE(12, 0); break;
-        default: GT_TILE(0, 0); break;
-        }
+        GT_TILE(0, 0, 0);
     }
 #undef GT_TILE
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_walk_slow_t(const float *proc, const T *raw, uint32_t nframes,
+                              const GtKParams &p, const uint2 *slow, uint32_t slow_cap,
+                              const uint32_t *scount, gt_detection *out, uint32_t out_cap,
+                              uint32_t *ocount, uint32_t frame0, hipStream_t s) {
+    const size_t hw = static_cast<size_t>(p.H) * p.W;
+    uint32_t blocks = static_cast<uint32_t>((hw / 1024 + WK_THREADS - 1) / WK_THREADS);
+    blocks = blocks < 1 ? 1 : (blocks > 256 ? 256 : blocks);
+    const dim3 grid(blocks, nframes, 1);
+    if (p.rint == 4)
+        k_walk_slow<T, 4><<<grid, WK_THREADS, 0, s>>>(proc, raw, p, slow, slow_cap, scount, out,
+                                                      out_cap, ocount, frame0);
+    else
+        k_walk_slow<T, 0><<<grid, WK_THREADS, 0, s>>>(proc, raw, p, slow, slow_cap, scount, out,
+                                                      out_cap, ocount, frame0);
     return hipGetLastError();
 }
 
@@ -737,17 +1134,36 @@ __global__ void k_copy_counts(const uint32_t *__restrict__ strided, uint32_t *__
 
 hipError_t gt_launch_detect_tile(const float *proc, const void *raw, int dtype, uint32_t nframes,
                                  const GtKParams &p, gt_detection *out, uint32_t out_cap,
-                                 uint32_t *ocount, uint32_t frame0, hipStream_t s) {
+                                 uint32_t *ocount, uint2 *slow, uint32_t slow_cap,
+                                 uint32_t *scount, uint32_t frame0, hipStream_t s) {
     switch (dtype) {
     case GT_U16:
         return launch_detect_tile_t(proc, static_cast<const uint16_t *>(raw), nframes, p, out,
-                                    out_cap, ocount, frame0, s);
+                                    out_cap, ocount, slow, slow_cap, scount, frame0, s);
     case GT_F32:
         return launch_detect_tile_t(proc, static_cast<const float *>(raw), nframes, p, out,
-                                    out_cap, ocount, frame0, s);
+                                    out_cap, ocount, slow, slow_cap, scount, frame0, s);
     case GT_U8:
         return launch_detect_tile_t(proc, static_cast<const uint8_t *>(raw), nframes, p, out,
-                                    out_cap, ocount, frame0, s);
+                                    out_cap, ocount, slow, slow_cap, scount, frame0, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t gt_launch_walk_slow(const float *proc, const void *raw, int dtype, uint32_t nframes,
+                               const GtKParams &p, const uint2 *slow, uint32_t slow_cap,
+                               const uint32_t *scount, gt_detection *out, uint32_t out_cap,
+                               uint32_t *ocount, uint32_t frame0, hipStream_t s) {
+    switch (dtype) {
+    case GT_U16:
+        return launch_walk_slow_t(proc, static_cast<const uint16_t *>(raw), nframes, p, slow,
+                                  slow_cap, scount, out, out_cap, ocount, frame0, s);
+    case GT_F32:
+        return launch_walk_slow_t(proc, static_cast<const float *>(raw), nframes, p, slow,
+                                  slow_cap, scount, out, out_cap, ocount, frame0, s);
+    case GT_U8:
+        return launch_walk_slow_t(proc, static_cast<const uint8_t *>(raw), nframes, p, slow,
+                                  slow_cap, scount, out, out_cap, ocount, frame0, s);
     default: return hipErrorInvalidValue;
     }
 }
@@ -800,4 +1216,20 @@ hipError_t gt_launch_characterize_rows(const float *proc, const void *raw, int d
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
+}
+
+hipError_t gt_init_tables() {
+    // the same expressions as the oracle's closed forms, in f64
+    static double2 host[(2 * GT_SC_R + 1) * (2 * GT_SC_R + 1)];
+    for (int i = -GT_SC_R; i <= GT_SC_R; ++i)
+        for (int j = -GT_SC_R; j <= GT_SC_R; ++j) {
+            double2 v = make_double2(0.0, 0.0);
+            if (i != 0 || j != 0) {
+                const double q = static_cast<double>(i * i + j * j);
+                v.x = 2.0 * static_cast<double>(i) * static_cast<double>(j) / q;
+                v.y = (static_cast<double>(i * i) - static_cast<double>(j * j)) / q;
+            }
+            host[(i + GT_SC_R) * (2 * GT_SC_R + 1) + (j + GT_SC_R)] = v;
+        }
+    return hipMemcpyToSymbol(HIP_SYMBOL(c_sincos2), host, sizeof(host));
 }

@@ -8,6 +8,8 @@
 // Largest dilation side k the LDS-tiled local-max kernel handles; larger
 // windows run the untiled kernel.
 #define GT_MAX_K_TILED 48
+// Radius up to which characterize reads sin/cos(2 theta) from a table.
+#define GT_SC_R 24
 
 // Derived per-frame parameters, passed by value as a kernel argument.  The
 // first block is the reference's WGSL Params uniform (src/shaders/params.wgsl),
@@ -19,6 +21,9 @@ struct GtKParams {
     float w[2 * GT_MAX_R + 1];    // exp(-i^2 / (2 sigma2)) as f32
     float gnorm;                  // sequential f32 sum of w
     float box_area;               // f32(s*s)
+    float rgnorm;                 // RN(1 / gnorm)
+    float rbox;                   // RN(1 / box_area)
+    int32_t fast_div;             // every nonzero tap weight >= 2^-100 (see k_bandpass_int)
     int32_t truncate;             // truncate_preprocessed
     int32_t k;                    // dilation side
     int32_t a;                    // window start offset i32(k/2 - 0.5)

@@ -67,6 +67,9 @@ struct gt_context {
     uint32_t *d_cand = nullptr;    // chunk * cand_cap
     uint32_t *d_ccount = nullptr;  // chunk * GT_COUNTER_STRIDE (candidates, untiled path)
     uint32_t *d_ocount = nullptr;  // chunk * GT_COUNTER_STRIDE (detections)
+    uint2 *d_slow = nullptr;       // chunk * slow_cap walks handed to k_walk_slow
+    uint32_t *d_scount = nullptr;  // chunk * GT_COUNTER_STRIDE
+    uint32_t slow_cap = 0;
     uint32_t cand_cap = 0;
     bool tiled = true;             // fused tile kernel fits in LDS
 
@@ -98,6 +101,12 @@ gt_status alloc_scratch(gt_context *c) {
         GT_HIP(hipMalloc(&c->d_cand, sizeof(uint32_t) * static_cast<size_t>(c->cand_cap) * c->chunk));
     GT_HIP(hipMalloc(&c->d_ccount, sizeof(uint32_t) * GT_COUNTER_STRIDE * c->chunk));
     GT_HIP(hipMalloc(&c->d_ocount, sizeof(uint32_t) * GT_COUNTER_STRIDE * c->chunk));
+    if (c->tiled) {
+        // one slot per 8 pixels; a tile whose frame list is full walks on itself
+        c->slow_cap = static_cast<uint32_t>(std::max<size_t>(1024, HW / 8));
+        GT_HIP(hipMalloc(&c->d_slow, sizeof(uint2) * static_cast<size_t>(c->slow_cap) * c->chunk));
+        GT_HIP(hipMalloc(&c->d_scount, sizeof(uint32_t) * GT_COUNTER_STRIDE * c->chunk));
+    }
     return GT_OK;
 }
 
@@ -118,6 +127,8 @@ void free_all(gt_context *c) {
     if (c->d_cand) (void)hipFree(c->d_cand);
     if (c->d_ccount) (void)hipFree(c->d_ccount);
     if (c->d_ocount) (void)hipFree(c->d_ocount);
+    if (c->d_slow) (void)hipFree(c->d_slow);
+    if (c->d_scount) (void)hipFree(c->d_scount);
     if (c->d_frames) (void)hipFree(c->d_frames);
     if (c->d_rows) (void)hipFree(c->d_rows);
     if (c->d_counts) (void)hipFree(c->d_counts);
@@ -159,9 +170,12 @@ gt_status run_pipeline(gt_context *c, const void *d_frames, int32_t dtype, uint3
         gt_detection *out = d_out + static_cast<size_t>(f0) * cap;
         if (c->tiled) {
             // local max, walk and characterize in one kernel
+            GT_HIP(gt_launch_zero_u32(c->d_scount, static_cast<size_t>(nf) * GT_COUNTER_STRIDE, s));
             GT_HIP(gt_launch_detect_tile(c->d_proc, frames, dtype, nf, c->kp, out, cap,
-                                         c->d_ocount, f0, s));
+                                         c->d_ocount, c->d_slow, c->slow_cap, c->d_scount, f0, s));
             if (ev) GT_HIP(hipEventRecord(ev[2], s));
+            GT_HIP(gt_launch_walk_slow(c->d_proc, frames, dtype, nf, c->kp, c->d_slow, c->slow_cap,
+                                       c->d_scount, out, cap, c->d_ocount, f0, s));
         } else {
             GT_HIP(gt_launch_localmax(c->d_proc, nf, c->kp, c->d_cand, c->cand_cap, c->d_ccount, s));
             if (ev) GT_HIP(hipEventRecord(ev[2], s));
@@ -221,6 +235,11 @@ gt_status gt_context_create(const gt_params *params, uint32_t nrows, uint32_t nc
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
         delete c;
         return fail(GT_ERR_GPU_DEVICE, "hipStreamCreate failed");
+    }
+    if (gt_init_tables() != hipSuccess) {
+        free_all(c);
+        delete c;
+        return fail(GT_ERR_GPU_DEVICE, "could not initialise the device tables");
     }
     const size_t HW = static_cast<size_t>(nrows) * ncols;
     // Frames per chunk: bounded by a 4 GiB scratch budget and by 64 frames.
