@@ -481,12 +481,14 @@ __device__ void characterize_row(const S &src, const T *__restrict__ Rw, int H, 
     bool done = false;
     if constexpr (RI > 0) {
         if (row.r <= 0.0f && rint == RI) {
+            // rows one at a time (bounded register use), columns unrolled so a
+            // row's loads issue together
             constexpr float r2c = static_cast<float>(RI) * static_cast<float>(RI);
-#pragma unroll
+#pragma unroll 1
             for (int i = -RI; i <= RI; ++i)
 #pragma unroll
                 for (int j = -RI; j <= RI; ++j)
-                    char_accumulate<T, S, true>(acc, src, Rw, H, W, u, v, i, j, r2c);
+                    char_accumulate<T, S, false>(acc, src, Rw, H, W, u, v, i, j, r2c);
             done = true;
         }
     }
@@ -598,7 +600,10 @@ __global__ __launch_bounds__(WK_THREADS) void k_walk(const float *__restrict__ p
 constexpr int FT_TW = 64, FT_TH = 64, FT_THREADS = 256;
 constexpr int FT_WAVES = FT_THREADS / 64;
 constexpr int FT_WROWS = FT_TH / FT_WAVES;
+constexpr int FT_PROWS = 8;  // output rows per window-maximum pass
 constexpr uint32_t FT_WALK_STEPS = 2;
+// At least FT_MIN_WAVES waves per SIMD: the walk phase is latency bound.
+constexpr int FT_MIN_WAVES = 4;
 
 // Centroid of the disk around (u, v) when the whole disk is inside the frame:
 // no per-pixel bounds test, every offset a compile-time constant.  Same
@@ -687,7 +692,7 @@ __device__ __forceinline__ bool finish_walk(const GtKParams &p, const float *__r
 }
 
 template <typename T, int KT, int RI, int HT>
-__global__ __launch_bounds__(FT_THREADS) void k_detect_tile(const float *__restrict__ proc,
+__global__ __launch_bounds__(FT_THREADS, FT_MIN_WAVES) void k_detect_tile(const float *__restrict__ proc,
                                                             const T *__restrict__ raw,
                                                             GtKParams p, int h_rt,
                                                             gt_detection *__restrict__ out,
@@ -700,6 +705,7 @@ __global__ __launch_bounds__(FT_THREADS) void k_detect_tile(const float *__restr
     extern __shared__ float lds[];
     __shared__ uint16_t s_cand[FT_TH * FT_TW];
     __shared__ uint32_t s_cnt[FT_WAVES];
+    __shared__ uint64_t s_masks[FT_WAVES][FT_WROWS];
     constexpr bool kInt = sizeof(T) <= 2;  // integer frames: no NaN in the processed frame
     const int k = KT > 0 ? KT : p.k;
     const int a = p.a;
@@ -762,48 +768,52 @@ __global__ __launch_bounds__(FT_THREADS) void k_detect_tile(const float *__restr
     // (frame row y0 + r0 - a + q) enters the window maximum of output row r
     // when 0 <= q - r < k.
     const int r0 = wave * FT_WROWS;
-    float wmax[FT_WROWS];
-#pragma unroll
-    for (int r = 0; r < FT_WROWS; ++r) wmax[r] = -INFINITY;
-    const float *rowbase = Sp + (h + r0 - a) * RW + (h + lane - a);
     auto vmax = [](float x, float y) { return kInt ? fmaxf(x, y) : maxf_ref(x, y); };
-    if constexpr (KT > 0) {
-#pragma unroll
-        for (int q = 0; q < FT_WROWS + KT - 1; ++q) {
-            const float *s = rowbase + q * RW;
-            float m = s[a];
-#pragma unroll
-            for (int j = 0; j < KT; ++j) m = vmax(m, s[j]);
-#pragma unroll
-            for (int r = 0; r < FT_WROWS; ++r)
-                if (q - r >= 0 && q - r < KT) wmax[r] = vmax(wmax[r], m);
-        }
-    } else {
-        for (int q = 0; q < FT_WROWS + k - 1; ++q) {
-            const float *s = rowbase + q * RW;
-            float m = s[a];
-            for (int j = 0; j < k; ++j) m = vmax(m, s[j]);
-#pragma unroll
-            for (int r = 0; r < FT_WROWS; ++r)
-                if (q - r >= 0 && q - r < k) wmax[r] = vmax(wmax[r], m);
-        }
-    }
     const int margin = p.margin;
     const float thr = p.max_thr;
     const int gx = x0 + lane;
-    uint64_t masks[FT_WROWS];
     uint32_t nwave = 0;  // wave-uniform
+    // two passes of FT_PROWS output rows keep the live window maxima few
+#pragma unroll 1
+    for (int rp = 0; rp < FT_WROWS; rp += FT_PROWS) {
+        float wmax[FT_PROWS];
 #pragma unroll
-    for (int r = 0; r < FT_WROWS; ++r) {
-        const int gy = y0 + r0 + r;
-        bool flag = false;
-        if (gy < H && gx < W && gy >= margin && gy < H - margin && gx >= margin &&
-            gx < W - margin) {
-            const float c = Sp[(h + r0 + r) * RW + h + lane];
-            flag = !(c < thr) && !(wmax[r] > c);
+        for (int r = 0; r < FT_PROWS; ++r) wmax[r] = -INFINITY;
+        const float *rowbase = Sp + (h + r0 + rp - a) * RW + (h + lane - a);
+        if constexpr (KT > 0) {
+#pragma unroll
+            for (int q = 0; q < FT_PROWS + KT - 1; ++q) {
+                const float *s = rowbase + q * RW;
+                float m = s[a];
+#pragma unroll
+                for (int j = 0; j < KT; ++j) m = vmax(m, s[j]);
+#pragma unroll
+                for (int r = 0; r < FT_PROWS; ++r)
+                    if (q - r >= 0 && q - r < KT) wmax[r] = vmax(wmax[r], m);
+            }
+        } else {
+            for (int q = 0; q < FT_PROWS + k - 1; ++q) {
+                const float *s = rowbase + q * RW;
+                float m = s[a];
+                for (int j = 0; j < k; ++j) m = vmax(m, s[j]);
+#pragma unroll
+                for (int r = 0; r < FT_PROWS; ++r)
+                    if (q - r >= 0 && q - r < k) wmax[r] = vmax(wmax[r], m);
+            }
         }
-        masks[r] = __ballot(flag);
-        nwave += static_cast<uint32_t>(__popcll(masks[r]));
+#pragma unroll
+        for (int r = 0; r < FT_PROWS; ++r) {
+            const int gy = y0 + r0 + rp + r;
+            bool flag = false;
+            if (gy < H && gx < W && gy >= margin && gy < H - margin && gx >= margin &&
+                gx < W - margin) {
+                const float c = Sp[(h + r0 + rp + r) * RW + h + lane];
+                flag = !(c < thr) && !(wmax[r] > c);
+            }
+            const uint64_t m = __ballot(flag);
+            if (lane == 0) s_masks[wave][rp + r] = m;
+            nwave += static_cast<uint32_t>(__popcll(m));
+        }
     }
     if (lane == 0) s_cnt[wave] = nwave;
     __syncthreads();
@@ -816,10 +826,11 @@ __global__ __launch_bounds__(FT_THREADS) void k_detect_tile(const float *__restr
     }
 #pragma unroll
     for (int r = 0; r < FT_WROWS; ++r) {
-        if ((masks[r] >> lane) & 1ull)
-            s_cand[off + __popcll(masks[r] & ((1ull << lane) - 1ull))] =
+        const uint64_t m = s_masks[wave][r];
+        if ((m >> lane) & 1ull)
+            s_cand[off + __popcll(m & ((1ull << lane) - 1ull))] =
                 static_cast<uint16_t>((r0 + r) * FT_TW + lane);
-        off += static_cast<uint32_t>(__popcll(masks[r]));
+        off += static_cast<uint32_t>(__popcll(m));
     }
     __syncthreads();
     if (static_cast<uint32_t>(wave) * 64u >= ncand) return;  // nothing left for this wave
