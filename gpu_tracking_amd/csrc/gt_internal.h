@@ -41,5 +41,10 @@ hipError_t gt_launch_walk_slow(const float *proc, const void *raw, int dtype, ui
                                uint32_t *ocount, uint32_t frame0, hipStream_t s);
 hipError_t gt_launch_copy_counts(const uint32_t *strided, uint32_t *dst, uint32_t n,
                                  hipStream_t s);
+// Fills the characterize columns of every detection of the chunk (rows of
+// frame f at out[f * out_cap], count at ocount[f * GT_COUNTER_STRIDE]).
+hipError_t gt_launch_characterize_det(const float *proc, const void *raw, int dtype,
+                                      uint32_t nframes, const GtKParams &p, gt_detection *out,
+                                      uint32_t out_cap, const uint32_t *ocount, hipStream_t s);
 // Fills the constant tables the kernels read (once per device).
 hipError_t gt_init_tables();

@@ -464,7 +464,7 @@ __device__ __forceinline__ void char_accumulate(CharAcc &acc, const S &src,
 // centre, with the Rg and eccentricity moment sums accumulated in f64.  RI > 0
 // unrolls the disk of rows without their own radius, so its loads issue
 // together.
-template <typename T, int RI = 0, class S>
+template <typename T, int RI = 0, bool kAllRows = false, class S>
 __device__ void characterize_row(const S &src, const T *__restrict__ Rw, int H, int W, int circle,
                                  gt_result_row &row) {
     float r = row.r;
@@ -481,14 +481,23 @@ __device__ void characterize_row(const S &src, const T *__restrict__ Rw, int H, 
     bool done = false;
     if constexpr (RI > 0) {
         if (row.r <= 0.0f && rint == RI) {
-            // rows one at a time (bounded register use), columns unrolled so a
-            // row's loads issue together
             constexpr float r2c = static_cast<float>(RI) * static_cast<float>(RI);
-#pragma unroll 1
-            for (int i = -RI; i <= RI; ++i)
+            if constexpr (kAllRows) {
+                // whole disk unrolled: every load issues at once
 #pragma unroll
-                for (int j = -RI; j <= RI; ++j)
-                    char_accumulate<T, S, false>(acc, src, Rw, H, W, u, v, i, j, r2c);
+                for (int i = -RI; i <= RI; ++i)
+#pragma unroll
+                    for (int j = -RI; j <= RI; ++j)
+                        char_accumulate<T, S, true>(acc, src, Rw, H, W, u, v, i, j, r2c);
+            } else {
+                // rows one at a time (bounded register use), columns unrolled
+                // so a row's loads issue together
+#pragma unroll 1
+                for (int i = -RI; i <= RI; ++i)
+#pragma unroll
+                    for (int j = -RI; j <= RI; ++j)
+                        char_accumulate<T, S, false>(acc, src, Rw, H, W, u, v, i, j, r2c);
+            }
             done = true;
         }
     }
@@ -670,23 +679,24 @@ __device__ __forceinline__ WalkRes walk_eval(const GtKParams &p, const float *__
 }
 
 // Finish a walk at integer position (pu, pv) with last evaluation `res`:
-// minmass test, then the row (characterized when asked).
+// minmass test, then the row (characterized here when `inline_char`;
+// otherwise k_characterize_det fills those columns afterwards).
 template <typename T, int RI>
 __device__ __forceinline__ bool finish_walk(const GtKParams &p, const float *__restrict__ P,
                                             const T *__restrict__ Rw, const float *Sp, int oy,
                                             int ox, int RW, int ylo, int yhi, int xlo, int xhi,
                                             WalkRes res, int pu, int pv, uint32_t seed,
-                                            uint32_t frame, gt_detection &d) {
+                                            uint32_t frame, gt_detection &d, bool inline_char) {
     res.x = res.x + static_cast<float>(pu);
     res.y = res.y + static_cast<float>(pv);
     if (!passes_minmass(p, res)) return false;
     fill_row(d, res, seed, frame);
-    if (p.characterize) {
+    if (inline_char && p.characterize) {
         const int cu = f2i(rintf(d.row.x)), cv = f2i(rintf(d.row.y));
         if (Sp && cu >= ylo && cu <= yhi && cv >= xlo && cv <= xhi)
-            characterize_row<T, RI>(LdsSrc{Sp, oy, ox, RW}, Rw, p.H, p.W, p.circle, d.row);
+            characterize_row<T, RI, true>(LdsSrc{Sp, oy, ox, RW}, Rw, p.H, p.W, p.circle, d.row);
         else
-            characterize_row<T, RI>(GlobalSrc{P, p.W}, Rw, p.H, p.W, p.circle, d.row);
+            characterize_row<T, RI, true>(GlobalSrc{P, p.W}, Rw, p.H, p.W, p.circle, d.row);
     }
     return true;
 }
@@ -866,7 +876,7 @@ __global__ __launch_bounds__(FT_THREADS, FT_MIN_WAVES) void k_detect_tile(const 
             }
             if (moving && it < p.max_iter) defer = true;
             else keep = finish_walk<T, RI>(p, P, Rw, Sp, oy, ox, RW, ylo, yhi, xlo, xhi, res, pu,
-                                           pv, seed, frame, d);
+                                           pv, seed, frame, d, true);
         }
         // hand long walks to k_walk_slow; if its list is full, walk on here
         const uint32_t spos = wave_append(defer, slow_counter);
@@ -880,7 +890,7 @@ __global__ __launch_bounds__(FT_THREADS, FT_MIN_WAVES) void k_detect_tile(const 
                     if (!walk_move(res, pu, pv, H, W, rint, p.shift_thr)) break;
                 }
                 keep = finish_walk<T, RI>(p, P, Rw, Sp, oy, ox, RW, ylo, yhi, xlo, xhi, res, pu,
-                                          pv, seed, frame, d);
+                                          pv, seed, frame, d, true);
             }
         }
         const uint32_t pos = wave_append(keep, counter);
@@ -927,10 +937,34 @@ __global__ __launch_bounds__(WK_THREADS) void k_walk_slow(const float *__restric
                 if (!walk_move(res, pu, pv, H, W, rint, p.shift_thr)) break;
             }
             keep = finish_walk<T, RI>(p, P, Rw, nullptr, 0, 0, 0, 1, 0, 1, 0, res, pu, pv, e.x,
-                                      frame0 + fr, d);
+                                      frame0 + fr, d, true);
         }
         const uint32_t pos = wave_append(keep, counter);
         if (keep && pos < out_cap) fout[pos] = d;
+    }
+}
+
+// Characterize every detection of the chunk, one lane per row, from the
+// frames in global memory (rows of frame f: out[f * out_cap ...], count in
+// the strided per-frame counter).
+template <typename T, int RI>
+__global__ __launch_bounds__(WK_THREADS) void k_characterize_det(const float *__restrict__ proc,
+                                                                 const T *__restrict__ raw,
+                                                                 GtKParams p,
+                                                                 gt_detection *__restrict__ out,
+                                                                 uint32_t out_cap,
+                                                                 const uint32_t *__restrict__ ocount) {
+    const int fr = blockIdx.y;
+    const int H = p.H, W = p.W;
+    const size_t HW = static_cast<size_t>(H) * W;
+    const GlobalSrc src{proc + static_cast<size_t>(fr) * HW, W};
+    const T *Rw = raw + static_cast<size_t>(fr) * HW;
+    const uint32_t n = min(ocount[fr * GT_COUNTER_STRIDE], out_cap);
+    gt_detection *rows = out + static_cast<size_t>(fr) * out_cap;
+    for (uint32_t i = blockIdx.x * WK_THREADS + threadIdx.x; i < n; i += gridDim.x * WK_THREADS) {
+        gt_result_row row = rows[i].row;
+        characterize_row<T, RI, true>(src, Rw, H, W, p.circle, row);
+        rows[i].row = row;
     }
 }
 
@@ -1243,4 +1277,30 @@ hipError_t gt_init_tables() {
             host[(i + GT_SC_R) * (2 * GT_SC_R + 1) + (j + GT_SC_R)] = v;
         }
     return hipMemcpyToSymbol(HIP_SYMBOL(c_sincos2), host, sizeof(host));
+}
+
+hipError_t gt_launch_characterize_det(const float *proc, const void *raw, int dtype,
+                                      uint32_t nframes, const GtKParams &p, gt_detection *out,
+                                      uint32_t out_cap, const uint32_t *ocount, hipStream_t s) {
+    const size_t hw = static_cast<size_t>(p.H) * p.W;
+    uint32_t blocks = static_cast<uint32_t>(hw / 65536);
+    blocks = blocks < 1 ? 1 : (blocks > 64 ? 64 : blocks);
+    const dim3 grid(blocks, nframes, 1);
+#define GT_CHAR(TT)                                                                            \
+    do {                                                                                       \
+        if (p.rint == 4)                                                                       \
+            k_characterize_det<TT, 4><<<grid, WK_THREADS, 0, s>>>(                             \
+                proc, static_cast<const TT *>(raw), p, out, out_cap, ocount);                  \
+        else                                                                                   \
+            k_characterize_det<TT, 0><<<grid, WK_THREADS, 0, s>>>(                             \
+                proc, static_cast<const TT *>(raw), p, out, out_cap, ocount);                  \
+    } while (0)
+    switch (dtype) {
+    case GT_U16: GT_CHAR(uint16_t); break;
+    case GT_F32: GT_CHAR(float); break;
+    case GT_U8: GT_CHAR(uint8_t); break;
+    default: return hipErrorInvalidValue;
+    }
+#undef GT_CHAR
+    return hipGetLastError();
 }
