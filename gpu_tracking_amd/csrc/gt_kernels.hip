@@ -44,6 +44,23 @@ __device__ __forceinline__ int32_t f2i(float x) {
 
 __device__ __forceinline__ float maxf_ref(float a, float b) { return b > a ? b : a; }
 
+// Workgroups are handed to the 8 XCDs (each with its own L2) round-robin in
+// launch order.  Tile t of a launch of `ntiles` tiles is given to workgroup
+// id w with t = (w % 8) * per + w / 8, per = ceil(ntiles / 8): each XCD works
+// through one contiguous range of tiles, so a tile's halo lines are usually
+// in the L2 its neighbour filled.  The launch has 8 * per workgroups; ids
+// that map past the last tile return at once.
+constexpr int kXcds = 8;
+__device__ __forceinline__ int xcd_tile(int ntiles) {
+    const int per = (ntiles + kXcds - 1) / kXcds;
+    const int w = blockIdx.x;
+    return (w % kXcds) * per + w / kXcds;
+}
+inline unsigned xcd_grid(size_t ntiles) {
+    const size_t per = (ntiles + kXcds - 1) / kXcds;
+    return static_cast<unsigned>(per * kXcds);
+}
+
 // Appends the lanes with `flag` set to a list guarded by *counter: one
 // atomicAdd per wavefront.  Must be reached by every lane of the wave.
 __device__ __forceinline__ uint32_t wave_append(bool flag, uint32_t *counter) {
@@ -172,7 +189,7 @@ __device__ __forceinline__ float div_const(float a, float b, float rb) {
 template <typename T, int R>
 __global__ __launch_bounds__(B2_THREADS) void k_bandpass_int(const T *__restrict__ frames,
                                                              float *__restrict__ proc,
-                                                             GtKParams p) {
+                                                             GtKParams p, int nframes) {
     constexpr int NT = 2 * R + 1;
     constexpr int RH = B2_TH + 2 * R;
     constexpr int RWP = (B2_TW + 2 * R + 3) & ~3;   // staged row pitch, 16-byte aligned
@@ -184,11 +201,44 @@ __global__ __launch_bounds__(B2_THREADS) void k_bandpass_int(const T *__restrict
     __shared__ __align__(16) float Sc[RH * B2_TW];
     const int H = p.H, W = p.W;
     const size_t HW = static_cast<size_t>(H) * W;
-    const int x0 = blockIdx.x * B2_TW, y0 = blockIdx.y * B2_TH;
-    const T *f = frames + static_cast<size_t>(blockIdx.z) * HW;
-    float *out = proc + static_cast<size_t>(blockIdx.z) * HW;
+    const int ntx = (W + B2_TW - 1) / B2_TW, nty = (H + B2_TH - 1) / B2_TH;
+    const int tile = xcd_tile(ntx * nty * nframes);
+    if (tile >= ntx * nty * nframes) return;
+    const int fr = tile / (ntx * nty), trest = tile - fr * (ntx * nty);
+    const int x0 = (trest % ntx) * B2_TW, y0 = (trest / ntx) * B2_TH;
+    const T *f = frames + static_cast<size_t>(fr) * HW;
+    float *out = proc + static_cast<size_t>(fr) * HW;
     const int tid = threadIdx.x;
-    {
+    constexpr int PAIRS = (B2_TW + 2 * R + 1) / 2;  // u16 pairs per staged row
+    static_assert(2 * PAIRS <= RWP, "pairs fit the staged row");
+    if (sizeof(T) == 2 && (R % 2) == 0 && (W % 2) == 0 &&
+        (reinterpret_cast<uintptr_t>(frames) & 3u) == 0 && y0 - R >= 0 && y0 + B2_TH + R <= H &&
+        x0 - R >= 0 && x0 - R + 2 * PAIRS <= W) {
+        // interior tile of a u16 frame: the staged region is whole in the
+        // frame and starts on an even column, so rows load as u32 pairs
+        constexpr int CN = RH * PAIRS, NL = (CN + B2_THREADS - 1) / B2_THREADS;
+        const uint32_t *src = reinterpret_cast<const uint32_t *>(
+            f + static_cast<size_t>(y0 - R) * W + (x0 - R));
+        const int W2 = W / 2;
+        uint32_t vals[NL];
+#pragma unroll
+        for (int q = 0; q < NL; ++q) {
+            const int idx = min(tid + q * B2_THREADS, CN - 1);
+            const int ry = idx / PAIRS, rx = idx - ry * PAIRS;
+            vals[q] = src[static_cast<size_t>(ry) * W2 + rx];
+        }
+#pragma unroll
+        for (int q = 0; q < NL; ++q) {
+            const int idx = tid + q * B2_THREADS;
+            if (idx < CN) {
+                const int ry = idx / PAIRS, rx = idx - ry * PAIRS;
+                float2 v2;
+                v2.x = static_cast<float>(vals[q] & 0xFFFFu);
+                v2.y = static_cast<float>(vals[q] >> 16);
+                *reinterpret_cast<float2 *>(Sraw + ry * RWP + 2 * rx) = v2;
+            }
+        }
+    } else {
         // issue every global load of the thread before the first LDS store
         constexpr int CN = RH * RWP, NL = (CN + B2_THREADS - 1) / B2_THREADS;
         T vals[NL];
@@ -711,7 +761,7 @@ __global__ __launch_bounds__(FT_THREADS, FT_MIN_WAVES) void k_detect_tile(const 
                                                             uint2 *__restrict__ slow,
                                                             uint32_t slow_cap,
                                                             uint32_t *__restrict__ scount,
-                                                            uint32_t frame0) {
+                                                            uint32_t frame0, int nframes) {
     extern __shared__ float lds[];
     __shared__ uint16_t s_cand[FT_TH * FT_TW];
     __shared__ uint32_t s_cnt[FT_WAVES];
@@ -725,8 +775,11 @@ __global__ __launch_bounds__(FT_THREADS, FT_MIN_WAVES) void k_detect_tile(const 
     float *Sp = lds;  // RH x RW, origin (y0 - h, x0 - h)
     const int H = p.H, W = p.W;
     const size_t HW = static_cast<size_t>(H) * W;
-    const int x0 = blockIdx.x * FT_TW, y0 = blockIdx.y * FT_TH;
-    const int fr = blockIdx.z;
+    const int ntx = (W + FT_TW - 1) / FT_TW, nty = (H + FT_TH - 1) / FT_TH;
+    const int tile = xcd_tile(ntx * nty * nframes);
+    if (tile >= ntx * nty * nframes) return;
+    const int fr = tile / (ntx * nty), trest = tile - fr * (ntx * nty);
+    const int x0 = (trest % ntx) * FT_TW, y0 = (trest / ntx) * FT_TH;
     const float *P = proc + static_cast<size_t>(fr) * HW;
     const T *Rw = raw + static_cast<size_t>(fr) * HW;
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
@@ -1025,15 +1078,18 @@ hipError_t launch_bandpass_t(const T *frames, float *proc, uint32_t nframes, con
                              hipStream_t s) {
     if constexpr (sizeof(T) <= 2) {
         if (p.fast_div && p.R >= 1 && p.R <= 7) {
-            const dim3 g2((p.W + B2_TW - 1) / B2_TW, (p.H + B2_TH - 1) / B2_TH, nframes);
+            const size_t ntiles = static_cast<size_t>((p.W + B2_TW - 1) / B2_TW) *
+                                  ((p.H + B2_TH - 1) / B2_TH) * nframes;
+            const dim3 g2(xcd_grid(ntiles));
+            const int nf = static_cast<int>(nframes);
             switch (p.R) {
-            case 1: k_bandpass_int<T, 1><<<g2, B2_THREADS, 0, s>>>(frames, proc, p); break;
-            case 2: k_bandpass_int<T, 2><<<g2, B2_THREADS, 0, s>>>(frames, proc, p); break;
-            case 3: k_bandpass_int<T, 3><<<g2, B2_THREADS, 0, s>>>(frames, proc, p); break;
-            case 4: k_bandpass_int<T, 4><<<g2, B2_THREADS, 0, s>>>(frames, proc, p); break;
-            case 5: k_bandpass_int<T, 5><<<g2, B2_THREADS, 0, s>>>(frames, proc, p); break;
-            case 6: k_bandpass_int<T, 6><<<g2, B2_THREADS, 0, s>>>(frames, proc, p); break;
-            default: k_bandpass_int<T, 7><<<g2, B2_THREADS, 0, s>>>(frames, proc, p); break;
+            case 1: k_bandpass_int<T, 1><<<g2, B2_THREADS, 0, s>>>(frames, proc, p, nf); break;
+            case 2: k_bandpass_int<T, 2><<<g2, B2_THREADS, 0, s>>>(frames, proc, p, nf); break;
+            case 3: k_bandpass_int<T, 3><<<g2, B2_THREADS, 0, s>>>(frames, proc, p, nf); break;
+            case 4: k_bandpass_int<T, 4><<<g2, B2_THREADS, 0, s>>>(frames, proc, p, nf); break;
+            case 5: k_bandpass_int<T, 5><<<g2, B2_THREADS, 0, s>>>(frames, proc, p, nf); break;
+            case 6: k_bandpass_int<T, 6><<<g2, B2_THREADS, 0, s>>>(frames, proc, p, nf); break;
+            default: k_bandpass_int<T, 7><<<g2, B2_THREADS, 0, s>>>(frames, proc, p, nf); break;
             }
             return hipGetLastError();
         }
@@ -1132,12 +1188,15 @@ hipError_t launch_detect_tile_t(const float *proc, const T *raw, uint32_t nframe
                                 uint32_t *ocount, uint2 *slow, uint32_t slow_cap,
                                 uint32_t *scount, uint32_t frame0, hipStream_t s) {
     const int h = gt_tile_halo(p);
-    const dim3 grid((p.W + FT_TW - 1) / FT_TW, (p.H + FT_TH - 1) / FT_TH, nframes);
+    const size_t ntiles = static_cast<size_t>((p.W + FT_TW - 1) / FT_TW) *
+                          ((p.H + FT_TH - 1) / FT_TH) * nframes;
+    const dim3 grid(xcd_grid(ntiles));
+    const int nf = static_cast<int>(nframes);
     const size_t lds = sizeof(float) * static_cast<size_t>(FT_TH + 2 * h) * (FT_TW + 2 * h);
 #define GT_TILE(KV, RV, HV)                                                                       \
     k_detect_tile<T, KV, RV, HV><<<grid, FT_THREADS, lds, s>>>(proc, raw, p, h, out, out_cap, \
                                                                 ocount, slow, slow_cap, scount, \
-                                                                frame0)
+                                                                frame0, nf)
     // diameter 9 with separation 7 (k = 9) or 9 (k = 12): halo 6
     if (p.rint == 4 && h == 6 && p.k == 9) {
         GT_TILE(9, 4, 6);
