@@ -28,7 +28,6 @@ hipError_t gt_launch_characterize_rows(const float *proc, const void *raw, int d
                                        uint32_t cap, const uint32_t *counts, uint32_t max_count,
                                        hipStream_t s);
 // Fused local max + walk + characterize over LDS tiles (the main path).
-int gt_tile_halo(const GtKParams &p);
 size_t gt_tile_lds_bytes(const GtKParams &p);
 hipError_t gt_launch_detect_tile(const float *proc, const void *raw, int dtype, uint32_t nframes,
                                  const GtKParams &p, gt_detection *out, uint32_t out_cap,

@@ -642,27 +642,47 @@ __global__ __launch_bounds__(WK_THREADS) void k_walk(const float *__restrict__ p
 
 // ---------------------------------------------------------------------------
 // Fused local max + walk + characterize.  One block per FT_TH x FT_TW tile:
-//   1. the processed frame over the tile plus an h-pixel halo is staged in LDS
-//      (out-of-frame entries -inf);
-//   2. wave w owns tile rows [16w, 16w + 16); each lane takes one column,
-//      folds the row maxima of the dilation window into its pixels' window
-//      maxima in registers and tests them; the block's candidates are
-//      compacted into one LDS list in raster order (ballots + 4-entry prefix);
-//   3. the first ceil(n/64) waves walk the list, one lane per candidate, for
-//      at most FT_WALK_STEPS steps; a disk that lies inside the staged region
-//      is read from LDS, any other from the frame in global memory (same
-//      arithmetic either way).  A walk still moving after those steps is
-//      handed to k_walk_slow through a per-frame list (or, when that list is
-//      full, walked on here);
-//   4. finished walks are characterized and appended to the frame's rows
-//      with one atomic per wave.  Waves without candidates leave early.
-constexpr int FT_TW = 64, FT_TH = 64, FT_THREADS = 256;
+//   1. the processed frame over the tile plus halos (TileGeom) is staged in
+//      LDS, out-of-frame entries -inf;
+//   2. horizontal window maxima: one item = 4 adjacent output columns of one
+//      row, read as float4 from the staged row (the left halo is chosen so
+//      these reads are 16-byte aligned), written to Hm;
+//   3. vertical window maxima and the test: each lane owns a column and
+//      FT_WROWS output rows, folding Hm rows in registers; the block's
+//      candidates are compacted into an LDS list in raster order (ballots and
+//      a 4-entry prefix; the list reuses Hm's storage);
+//   4. the first ceil(n/64) waves walk the list, one lane per candidate, for
+//      at most FT_WALK_STEPS steps; a disk inside the staged region is read
+//      from LDS, any other from the frame in global memory (same arithmetic).
+//      Walks still moving go to k_walk_slow through a per-frame list (or walk
+//      on here when that list is full);
+//   5. finished walks are characterized and appended to the frame's rows with
+//      one atomic per wave.  Waves without candidates leave early.
+constexpr int FT_TW = 64, FT_TH = 32, FT_THREADS = 256;
 constexpr int FT_WAVES = FT_THREADS / 64;
 constexpr int FT_WROWS = FT_TH / FT_WAVES;
-constexpr int FT_PROWS = 8;  // output rows per window-maximum pass
 constexpr uint32_t FT_WALK_STEPS = 2;
 // At least FT_MIN_WAVES waves per SIMD: the walk phase is latency bound.
 constexpr int FT_MIN_WAVES = 4;
+
+}  // namespace
+
+// Halo layout of a detect tile for dilation side k and walk radius rint.
+// The window of output column c starts at staged column hl - a + c, and
+// (hl - a) % 4 == 0 keeps the horizontal float4 reads aligned.
+struct TileGeom {
+    int a, ht, hb, hl, hr, width, pitch, rows;
+};
+__host__ __device__ constexpr TileGeom tile_geom(int k, int rint) {
+    const int a = (k - 1) / 2;  // = i32(k/2 - 0.5) for k >= 1
+    const int lo = a > rint + 2 ? a : rint + 2;
+    const int hi = (k - 1 - a) > rint + 2 ? (k - 1 - a) : rint + 2;
+    const int hl = a + 4 * ((lo - a + 3) / 4);
+    const int width = hl + FT_TW + hi;
+    return TileGeom{a, lo, hi, hl, hi, width, ((width + 3) & ~3) + 4, lo + FT_TH + hi};
+}
+
+namespace {
 
 // Centroid of the disk around (u, v) when the whole disk is inside the frame:
 // no per-pixel bounds test, every offset a compile-time constant.  Same
@@ -705,8 +725,9 @@ struct GlobalRow {
     }
 };
 
-// One walk evaluation at (pu, pv) from the best available source.
-template <int RI, int HT>
+// One walk evaluation at (pu, pv) from the best available source; PITCH is
+// the staged region's compile-time row pitch (0: none or not known).
+template <int RI, int PITCH>
 __device__ __forceinline__ WalkRes walk_eval(const GtKParams &p, const float *__restrict__ P,
                                              const float *Sp, int oy, int ox, int RW, int ylo,
                                              int yhi, int xlo, int xhi, int pu, int pv) {
@@ -714,11 +735,9 @@ __device__ __forceinline__ WalkRes walk_eval(const GtKParams &p, const float *__
     if constexpr (RI > 0) {
         const bool in_frame = pu >= RI && pu + RI < H && pv >= RI && pv + RI < W;
         if (in_frame) {
-            if (pu >= ylo && pu <= yhi && pv >= xlo && pv <= xhi) {
-                if constexpr (HT > 0) {
-                    constexpr int PITCH = FT_TW + 2 * HT;
+            if constexpr (PITCH > 0) {
+                if (pu >= ylo && pu <= yhi && pv >= xlo && pv <= xhi)
                     return get_center_inner<RI>(LdsRow<PITCH>{Sp + (pu - oy) * PITCH + (pv - ox)});
-                }
             }
             return get_center_inner<RI>(GlobalRow{P + static_cast<size_t>(pu) * W + pv, W});
         }
@@ -728,51 +747,114 @@ __device__ __forceinline__ WalkRes walk_eval(const GtKParams &p, const float *__
     return get_center<RI>(GlobalSrc{P, W}, H, W, pu, pv, p.rint, p.r2);
 }
 
+// characterize for a disk of radius RI that lies wholly inside the frame:
+// the same operations in the same order as characterize_row, without the
+// per-pixel bounds test, so every load is unconditional and they issue
+// together (one memory round trip instead of one per pixel).
+template <typename T, int RI, class B>
+__device__ __forceinline__ void characterize_inner(const B &base, const T *__restrict__ rc, int W,
+                                                   gt_result_row &row) {
+    constexpr float r2 = static_cast<float>(RI) * static_cast<float>(RI);
+    float raw[2 * RI + 1][2 * RI + 1];
+#pragma unroll
+    for (int i = -RI; i <= RI; ++i) {
+        const T *rrow = rc + static_cast<long long>(i) * W;
+#pragma unroll
+        for (int j = -RI; j <= RI; ++j) {
+            const float x = static_cast<float>(i), y = static_cast<float>(j);
+            if (x * x + y * y > r2) continue;
+            raw[i + RI][j + RI] = to_f32(rrow[j]);
+        }
+    }
+    float raw_mass = 0.0f, signal = 0.0f;
+    double rg = 0.0, es = 0.0, ec = 0.0;
+#pragma unroll
+    for (int i = -RI; i <= RI; ++i) {
+#pragma unroll
+        for (int j = -RI; j <= RI; ++j) {
+            const float x = static_cast<float>(i), y = static_cast<float>(j);
+            const float mask = x * x + y * y;
+            if (mask > r2) continue;
+            const float d = base(i, j);
+            raw_mass = raw_mass + raw[i + RI][j + RI];
+            // signal starts at +0 and is never NaN: fmaxf matches max() here
+            signal = fmaxf(signal, d);
+            if (i != 0 || j != 0) {
+                const double q = static_cast<double>(i * i + j * j);
+                es = fma(static_cast<double>(d), 2.0 * static_cast<double>(i) * static_cast<double>(j) / q, es);
+                ec = fma(static_cast<double>(d), (static_cast<double>(i * i) - static_cast<double>(j * j)) / q, ec);
+            } else {
+                ec = ec + static_cast<double>(d);
+            }
+            rg = fma(static_cast<double>(d), static_cast<double>(mask), rg);
+        }
+    }
+    const float den = row.mass - base(0, 0);
+    row.ecc = static_cast<float>(sqrt(es * es + ec * ec) / static_cast<double>(den));
+    row.Rg = static_cast<float>(sqrt(rg / static_cast<double>(row.mass)));
+    row.raw_mass = raw_mass;
+    row.signal = signal;
+}
+
 // Finish a walk at integer position (pu, pv) with last evaluation `res`:
-// minmass test, then the row (characterized here when `inline_char`;
-// otherwise k_characterize_det fills those columns afterwards).
-template <typename T, int RI>
+// minmass test, then the row, characterized when asked.
+template <typename T, int RI, int PITCH = 0>
 __device__ __forceinline__ bool finish_walk(const GtKParams &p, const float *__restrict__ P,
                                             const T *__restrict__ Rw, const float *Sp, int oy,
                                             int ox, int RW, int ylo, int yhi, int xlo, int xhi,
                                             WalkRes res, int pu, int pv, uint32_t seed,
-                                            uint32_t frame, gt_detection &d, bool inline_char) {
+                                            uint32_t frame, gt_detection &d) {
     res.x = res.x + static_cast<float>(pu);
     res.y = res.y + static_cast<float>(pv);
     if (!passes_minmass(p, res)) return false;
     fill_row(d, res, seed, frame);
-    if (inline_char && p.characterize) {
+    if (p.characterize) {
         const int cu = f2i(rintf(d.row.x)), cv = f2i(rintf(d.row.y));
-        if (Sp && cu >= ylo && cu <= yhi && cv >= xlo && cv <= xhi)
-            characterize_row<T, RI, true>(LdsSrc{Sp, oy, ox, RW}, Rw, p.H, p.W, p.circle, d.row);
+        const bool in_lds = Sp && cu >= ylo && cu <= yhi && cv >= xlo && cv <= xhi;
+        if constexpr (RI > 0) {
+            if (cu >= RI && cu + RI < p.H && cv >= RI && cv + RI < p.W) {
+                const T *rc = Rw + static_cast<size_t>(cu) * p.W + cv;
+                if constexpr (PITCH > 0) {
+                    if (in_lds) {
+                        characterize_inner<T, RI>(LdsRow<PITCH>{Sp + (cu - oy) * PITCH + (cv - ox)},
+                                                  rc, p.W, d.row);
+                        return true;
+                    }
+                }
+                characterize_inner<T, RI>(GlobalRow{P + static_cast<size_t>(cu) * p.W + cv, p.W},
+                                          rc, p.W, d.row);
+                return true;
+            }
+        }
+        if (in_lds)
+            characterize_row<T, RI>(LdsSrc{Sp, oy, ox, RW}, Rw, p.H, p.W, p.circle, d.row);
         else
-            characterize_row<T, RI, true>(GlobalSrc{P, p.W}, Rw, p.H, p.W, p.circle, d.row);
+            characterize_row<T, RI>(GlobalSrc{P, p.W}, Rw, p.H, p.W, p.circle, d.row);
     }
     return true;
 }
 
-template <typename T, int KT, int RI, int HT>
-__global__ __launch_bounds__(FT_THREADS, FT_MIN_WAVES) void k_detect_tile(const float *__restrict__ proc,
-                                                            const T *__restrict__ raw,
-                                                            GtKParams p, int h_rt,
-                                                            gt_detection *__restrict__ out,
-                                                            uint32_t out_cap,
-                                                            uint32_t *__restrict__ ocount,
-                                                            uint2 *__restrict__ slow,
-                                                            uint32_t slow_cap,
-                                                            uint32_t *__restrict__ scount,
-                                                            uint32_t frame0, int nframes) {
+template <typename T, int KT, int RI>
+__global__ __launch_bounds__(FT_THREADS, FT_MIN_WAVES) void k_detect_tile(
+    const float *__restrict__ proc, const T *__restrict__ raw, GtKParams p, TileGeom g_rt,
+    gt_detection *__restrict__ out, uint32_t out_cap, uint32_t *__restrict__ ocount,
+    uint2 *__restrict__ slow, uint32_t slow_cap, uint32_t *__restrict__ scount, uint32_t frame0,
+    int nframes) {
+    constexpr bool kFixed = KT > 0 && RI > 0;
+    constexpr TileGeom gc = tile_geom(KT > 0 ? KT : 1, RI > 0 ? RI : 1);
+    const TileGeom g = kFixed ? gc : g_rt;
     extern __shared__ float lds[];
-    __shared__ uint16_t s_cand[FT_TH * FT_TW];
     __shared__ uint32_t s_cnt[FT_WAVES];
     __shared__ uint64_t s_masks[FT_WAVES][FT_WROWS];
     constexpr bool kInt = sizeof(T) <= 2;  // integer frames: no NaN in the processed frame
     const int k = KT > 0 ? KT : p.k;
-    const int a = p.a;
+    const int a = g.a;
     const int rint = RI > 0 ? RI : p.rint;
-    const int h = HT > 0 ? HT : h_rt;
-    const int RW = FT_TW + 2 * h, RH = FT_TH + 2 * h;
-    float *Sp = lds;  // RH x RW, origin (y0 - h, x0 - h)
+    const int RW = g.pitch, RH = g.rows;
+    const int MR = FT_TH + k - 1;      // rows of horizontal maxima the tile needs
+    float *Sp = lds;                   // RH x pitch, origin (y0 - ht, x0 - hl)
+    float *Hm = lds + RH * RW;         // MR x FT_TW; later the candidate list
+    uint16_t *s_cand = reinterpret_cast<uint16_t *>(Hm);
     const int H = p.H, W = p.W;
     const size_t HW = static_cast<size_t>(H) * W;
     const int ntx = (W + FT_TW - 1) / FT_TW, nty = (H + FT_TH - 1) / FT_TH;
@@ -783,13 +865,12 @@ __global__ __launch_bounds__(FT_THREADS, FT_MIN_WAVES) void k_detect_tile(const 
     const float *P = proc + static_cast<size_t>(fr) * HW;
     const T *Rw = raw + static_cast<size_t>(fr) * HW;
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-    const int oy = y0 - h, ox = x0 - h;
-    const bool interior = oy >= 0 && ox >= 0 && oy + RH <= H && ox + RW <= W;
+    const int oy = y0 - g.ht, ox = x0 - g.hl;
 
-    if constexpr (HT > 0) {
-        // fixed region: every load of the thread is issued before the first
-        // LDS store, so their latencies overlap
-        constexpr int CRW = FT_TW + 2 * HT, CN = (FT_TH + 2 * HT) * CRW;
+    // 1. staging (the pitch padding is loaded too; it is never compared)
+    const bool interior = oy >= 0 && ox >= 0 && oy + RH <= H && ox + RW <= W;
+    if constexpr (kFixed) {
+        constexpr int CN = gc.rows * gc.pitch;
         constexpr int NL = (CN + FT_THREADS - 1) / FT_THREADS;
         float vals[NL];
         if (interior) {
@@ -797,14 +878,14 @@ __global__ __launch_bounds__(FT_THREADS, FT_MIN_WAVES) void k_detect_tile(const 
 #pragma unroll
             for (int q = 0; q < NL; ++q) {
                 const int idx = min(tid + q * FT_THREADS, CN - 1);
-                const int ry = idx / CRW, rx = idx - ry * CRW;
+                const int ry = idx / gc.pitch, rx = idx - ry * gc.pitch;
                 vals[q] = src[static_cast<size_t>(ry) * W + rx];
             }
         } else {
 #pragma unroll
             for (int q = 0; q < NL; ++q) {
                 const int idx = tid + q * FT_THREADS;
-                const int ry = idx / CRW, rx = idx - ry * CRW;
+                const int ry = idx / gc.pitch, rx = idx - ry * gc.pitch;
                 const int gy = oy + ry, gx = ox + rx;
                 vals[q] = (idx < CN && gy >= 0 && gy < H && gx >= 0 && gx < W)
                               ? P[static_cast<size_t>(gy) * W + gx]
@@ -827,59 +908,96 @@ __global__ __launch_bounds__(FT_THREADS, FT_MIN_WAVES) void k_detect_tile(const 
     }
     __syncthreads();
 
-    // Candidate test for tile rows r0..r0+15 of column `lane`: window row q
-    // (frame row y0 + r0 - a + q) enters the window maximum of output row r
-    // when 0 <= q - r < k.
-    const int r0 = wave * FT_WROWS;
     auto vmax = [](float x, float y) { return kInt ? fmaxf(x, y) : maxf_ref(x, y); };
+    // 2. horizontal maxima: Hm[m][c] = max_{j<k} Sp[ht - a + m][hl - a + c + j]
+    //    (float frames keep the oracle's sequential order: NaN inputs)
+    if constexpr (kFixed && kInt) {
+        constexpr int NV = (KT + 3 + 3) / 4;  // float4 reads per item
+        for (int item = tid; item < MR * (FT_TW / 4); item += FT_THREADS) {
+            const int m = item / (FT_TW / 4), c4 = (item % (FT_TW / 4)) * 4;
+            const float4 *src =
+                reinterpret_cast<const float4 *>(Sp + (gc.ht - gc.a + m) * gc.pitch + (gc.hl - gc.a) + c4);
+            float v[4 * NV];
+#pragma unroll
+            for (int q = 0; q < NV; ++q) {
+                const float4 t = src[q];
+                v[4 * q] = t.x;
+                v[4 * q + 1] = t.y;
+                v[4 * q + 2] = t.z;
+                v[4 * q + 3] = t.w;
+            }
+            // the reference starts each row maximum from the centre value
+            float core = v[a + 3];
+#pragma unroll
+            for (int j = 3; j < KT; ++j) core = vmax(core, v[j]);
+            float4 o;
+            float m0 = vmax(v[a], core), m1 = vmax(v[a + 1], core);
+            float m2 = vmax(v[a + 2], core), m3 = vmax(v[a + 3], core);
+            m0 = vmax(vmax(vmax(m0, v[0]), v[1]), v[2]);
+            m1 = vmax(vmax(vmax(m1, v[1]), v[2]), v[KT]);
+            m2 = vmax(vmax(vmax(m2, v[2]), v[KT]), v[KT + 1]);
+            m3 = vmax(vmax(vmax(m3, v[KT]), v[KT + 1]), v[KT + 2]);
+            o.x = m0;
+            o.y = m1;
+            o.z = m2;
+            o.w = m3;
+            *reinterpret_cast<float4 *>(Hm + m * FT_TW + c4) = o;
+        }
+    } else {
+        for (int item = tid; item < MR * FT_TW; item += FT_THREADS) {
+            const int m = item / FT_TW, c = item - m * FT_TW;
+            const float *s = Sp + (g.ht - a + m) * RW + (g.hl - a) + c;
+            float mx = s[a];
+            for (int j = 0; j < k; ++j) mx = vmax(mx, s[j]);
+            Hm[item] = mx;
+        }
+    }
+    __syncthreads();
+
+    // 3. vertical maxima and the candidate test for rows r0..r0+7 of column
+    //    `lane`: Hm row q feeds output row r when 0 <= q - r < k
+    const int r0 = wave * FT_WROWS;
     const int margin = p.margin;
     const float thr = p.max_thr;
     const int gx = x0 + lane;
     uint32_t nwave = 0;  // wave-uniform
-    // two passes of FT_PROWS output rows keep the live window maxima few
-#pragma unroll 1
-    for (int rp = 0; rp < FT_WROWS; rp += FT_PROWS) {
-        float wmax[FT_PROWS];
+    {
+        float wmax[FT_WROWS];
 #pragma unroll
-        for (int r = 0; r < FT_PROWS; ++r) wmax[r] = -INFINITY;
-        const float *rowbase = Sp + (h + r0 + rp - a) * RW + (h + lane - a);
+        for (int r = 0; r < FT_WROWS; ++r) wmax[r] = -INFINITY;
+        const float *col = Hm + r0 * FT_TW + lane;
         if constexpr (KT > 0) {
 #pragma unroll
-            for (int q = 0; q < FT_PROWS + KT - 1; ++q) {
-                const float *s = rowbase + q * RW;
-                float m = s[a];
+            for (int q = 0; q < FT_WROWS + KT - 1; ++q) {
+                const float hq = col[q * FT_TW];
 #pragma unroll
-                for (int j = 0; j < KT; ++j) m = vmax(m, s[j]);
-#pragma unroll
-                for (int r = 0; r < FT_PROWS; ++r)
-                    if (q - r >= 0 && q - r < KT) wmax[r] = vmax(wmax[r], m);
+                for (int r = 0; r < FT_WROWS; ++r)
+                    if (q - r >= 0 && q - r < KT) wmax[r] = vmax(wmax[r], hq);
             }
         } else {
-            for (int q = 0; q < FT_PROWS + k - 1; ++q) {
-                const float *s = rowbase + q * RW;
-                float m = s[a];
-                for (int j = 0; j < k; ++j) m = vmax(m, s[j]);
+            for (int q = 0; q < FT_WROWS + k - 1; ++q) {
+                const float hq = col[q * FT_TW];
 #pragma unroll
-                for (int r = 0; r < FT_PROWS; ++r)
-                    if (q - r >= 0 && q - r < k) wmax[r] = vmax(wmax[r], m);
+                for (int r = 0; r < FT_WROWS; ++r)
+                    if (q - r >= 0 && q - r < k) wmax[r] = vmax(wmax[r], hq);
             }
         }
 #pragma unroll
-        for (int r = 0; r < FT_PROWS; ++r) {
-            const int gy = y0 + r0 + rp + r;
+        for (int r = 0; r < FT_WROWS; ++r) {
+            const int gy = y0 + r0 + r;
             bool flag = false;
             if (gy < H && gx < W && gy >= margin && gy < H - margin && gx >= margin &&
                 gx < W - margin) {
-                const float c = Sp[(h + r0 + rp + r) * RW + h + lane];
+                const float c = Sp[(g.ht + r0 + r) * RW + g.hl + lane];
                 flag = !(c < thr) && !(wmax[r] > c);
             }
-            const uint64_t m = __ballot(flag);
-            if (lane == 0) s_masks[wave][rp + r] = m;
-            nwave += static_cast<uint32_t>(__popcll(m));
+            const uint64_t mk = __ballot(flag);
+            if (lane == 0) s_masks[wave][r] = mk;
+            nwave += static_cast<uint32_t>(__popcll(mk));
         }
     }
     if (lane == 0) s_cnt[wave] = nwave;
-    __syncthreads();
+    __syncthreads();  // Hm is dead from here on: the list takes its place
     uint32_t off = 0, ncand = 0;
 #pragma unroll
     for (int w = 0; w < FT_WAVES; ++w) {
@@ -889,22 +1007,24 @@ __global__ __launch_bounds__(FT_THREADS, FT_MIN_WAVES) void k_detect_tile(const 
     }
 #pragma unroll
     for (int r = 0; r < FT_WROWS; ++r) {
-        const uint64_t m = s_masks[wave][r];
-        if ((m >> lane) & 1ull)
-            s_cand[off + __popcll(m & ((1ull << lane) - 1ull))] =
+        const uint64_t mk = s_masks[wave][r];
+        if ((mk >> lane) & 1ull)
+            s_cand[off + __popcll(mk & ((1ull << lane) - 1ull))] =
                 static_cast<uint16_t>((r0 + r) * FT_TW + lane);
-        off += static_cast<uint32_t>(__popcll(m));
+        off += static_cast<uint32_t>(__popcll(mk));
     }
     __syncthreads();
     if (static_cast<uint32_t>(wave) * 64u >= ncand) return;  // nothing left for this wave
 
+    // 4-5. walks
     const int ylo = oy + rint, yhi = oy + RH - 1 - rint;
-    const int xlo = ox + rint, xhi = ox + RW - 1 - rint;
+    const int xlo = ox + rint, xhi = ox + g.width - 1 - rint;
     const uint32_t frame = frame0 + fr;
     uint32_t *counter = &ocount[fr * GT_COUNTER_STRIDE];
     uint32_t *slow_counter = &scount[fr * GT_COUNTER_STRIDE];
     uint2 *fslow = slow + static_cast<size_t>(fr) * slow_cap;
     gt_detection *fout = out + static_cast<size_t>(fr) * out_cap;
+    constexpr int PITCH = kFixed ? gc.pitch : 0;
     for (uint32_t c0 = static_cast<uint32_t>(wave) * 64u; c0 < ncand; c0 += FT_THREADS) {
         const uint32_t ci = c0 + static_cast<uint32_t>(lane);
         bool keep = false, defer = false;
@@ -921,15 +1041,15 @@ __global__ __launch_bounds__(FT_THREADS, FT_MIN_WAVES) void k_detect_tile(const 
             bool moving = true;
             uint32_t it = 0;
             for (; it < p.max_iter && it < FT_WALK_STEPS; ++it) {
-                res = walk_eval<RI, HT>(p, P, Sp, oy, ox, RW, ylo, yhi, xlo, xhi, pu, pv);
+                res = walk_eval<RI, PITCH>(p, P, Sp, oy, ox, RW, ylo, yhi, xlo, xhi, pu, pv);
                 if (!walk_move(res, pu, pv, H, W, rint, p.shift_thr)) {
                     moving = false;
                     break;
                 }
             }
             if (moving && it < p.max_iter) defer = true;
-            else keep = finish_walk<T, RI>(p, P, Rw, Sp, oy, ox, RW, ylo, yhi, xlo, xhi, res, pu,
-                                           pv, seed, frame, d, true);
+            else keep = finish_walk<T, RI, PITCH>(p, P, Rw, Sp, oy, ox, RW, ylo, yhi, xlo, xhi, res, pu,
+                                           pv, seed, frame, d);
         }
         // hand long walks to k_walk_slow; if its list is full, walk on here
         const uint32_t spos = wave_append(defer, slow_counter);
@@ -939,11 +1059,11 @@ __global__ __launch_bounds__(FT_THREADS, FT_MIN_WAVES) void k_detect_tile(const 
             } else {
                 WalkRes res = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
                 for (uint32_t it = FT_WALK_STEPS; it < p.max_iter; ++it) {
-                    res = walk_eval<RI, HT>(p, P, Sp, oy, ox, RW, ylo, yhi, xlo, xhi, pu, pv);
+                    res = walk_eval<RI, PITCH>(p, P, Sp, oy, ox, RW, ylo, yhi, xlo, xhi, pu, pv);
                     if (!walk_move(res, pu, pv, H, W, rint, p.shift_thr)) break;
                 }
-                keep = finish_walk<T, RI>(p, P, Rw, Sp, oy, ox, RW, ylo, yhi, xlo, xhi, res, pu,
-                                          pv, seed, frame, d, true);
+                keep = finish_walk<T, RI, PITCH>(p, P, Rw, Sp, oy, ox, RW, ylo, yhi, xlo, xhi, res, pu,
+                                          pv, seed, frame, d);
             }
         }
         const uint32_t pos = wave_append(keep, counter);
@@ -990,7 +1110,7 @@ __global__ __launch_bounds__(WK_THREADS) void k_walk_slow(const float *__restric
                 if (!walk_move(res, pu, pv, H, W, rint, p.shift_thr)) break;
             }
             keep = finish_walk<T, RI>(p, P, Rw, nullptr, 0, 0, 0, 1, 0, 1, 0, res, pu, pv, e.x,
-                                      frame0 + fr, d, true);
+                                      frame0 + fr, d);
         }
         const uint32_t pos = wave_append(keep, counter);
         if (keep && pos < out_cap) fout[pos] = d;
@@ -1165,20 +1285,11 @@ hipError_t gt_launch_localmax(const float *proc, uint32_t nframes, const GtKPara
     return hipGetLastError();
 }
 
-int gt_tile_halo(const GtKParams &p) {
-    // the staged region must hold the dilation window of every tile pixel and
-    // the walk disk of a candidate that moved up to two pixels off the tile
-    int h = p.a;
-    h = std::max(h, p.k - 1 - p.a);
-    h = std::max(h, p.rint + 2);
-    return h;
-}
-
 size_t gt_tile_lds_bytes(const GtKParams &p) {
-    const int h = gt_tile_halo(p);
-    const size_t dyn = sizeof(float) * static_cast<size_t>(FT_TH + 2 * h) * (FT_TW + 2 * h);
-    const size_t stat = sizeof(uint16_t) * FT_TH * FT_TW + 64;
-    return dyn + stat;
+    const TileGeom g = tile_geom(p.k, p.rint);
+    const size_t dyn = sizeof(float) * (static_cast<size_t>(g.rows) * g.pitch +
+                                        static_cast<size_t>(FT_TH + p.k - 1) * FT_TW);
+    return dyn + 256;
 }
 
 namespace {
@@ -1187,25 +1298,25 @@ hipError_t launch_detect_tile_t(const float *proc, const T *raw, uint32_t nframe
                                 const GtKParams &p, gt_detection *out, uint32_t out_cap,
                                 uint32_t *ocount, uint2 *slow, uint32_t slow_cap,
                                 uint32_t *scount, uint32_t frame0, hipStream_t s) {
-    const int h = gt_tile_halo(p);
+    const TileGeom g = tile_geom(p.k, p.rint);
     const size_t ntiles = static_cast<size_t>((p.W + FT_TW - 1) / FT_TW) *
                           ((p.H + FT_TH - 1) / FT_TH) * nframes;
     const dim3 grid(xcd_grid(ntiles));
     const int nf = static_cast<int>(nframes);
-    const size_t lds = sizeof(float) * static_cast<size_t>(FT_TH + 2 * h) * (FT_TW + 2 * h);
-#define GT_TILE(KV, RV, HV)                                                                       \
-    k_detect_tile<T, KV, RV, HV><<<grid, FT_THREADS, lds, s>>>(proc, raw, p, h, out, out_cap, \
-                                                                ocount, slow, slow_cap, scount, \
-                                                                frame0, nf)
-    // diameter 9 with separation 7 (k = 9) or 9 (k = 12): halo 6
-    if (p.rint == 4 && h == 6 && p.k == 9) {
-        GT_TILE(9, 4, 6);
-    } else if (p.rint == 4 && h == 6 && p.k == 12) {
-        GT_TILE(12, 4, 6);
+    const size_t lds = sizeof(float) * (static_cast<size_t>(g.rows) * g.pitch +
+                                        static_cast<size_t>(FT_TH + p.k - 1) * FT_TW);
+#define GT_TILE(KV, RV)                                                                         \
+    k_detect_tile<T, KV, RV><<<grid, FT_THREADS, lds, s>>>(proc, raw, p, g, out, out_cap, ocount, \
+                                                            slow, slow_cap, scount, frame0, nf)
+    // diameter 9 with separation 7 (k = 9) or 9 (k = 12)
+    if (p.rint == 4 && p.k == 9) {
+        GT_TILE(9, 4);
+    } else if (p.rint == 4 && p.k == 12) {
+        GT_TILE(12, 4);
     } else if (p.rint == 4) {
-        GT_TILE(0, 4, 0);
+        GT_TILE(0, 4);
     } else {
-        GT_TILE(0, 0, 0);
+        GT_TILE(0, 0);
     }
 #undef GT_TILE
     return hipGetLastError();
