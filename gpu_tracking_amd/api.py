@@ -311,6 +311,46 @@ def characterize_points(file_or_array, points_to_characterize, diameter=None, ha
     return _to_dataframe(data, names, types)
 
 
+_LOG_ONLY = dict(n_radii=10, log_spacing=False, overlap_threshold=0.0)
+_LOG_DEFAULTS = dict({k: v for k, v in _TRACKPY_DEFAULTS.items()
+                      if k not in ("maxsize", "separation", "threshold", "invert", "percentile",
+                                   "topn", "preprocess", "filter_close")}, **_LOG_ONLY)
+
+
+def make_log_params(min_radius: float, max_radius: float, **kw) -> N.GtParams:
+    """TrackingParams for the LoG style with the Python binding's defaults
+    (macros lib.rs:571-629: n_radii 10, linear spacing, overlap_threshold 0,
+    the r column on)."""
+    unknown = set(kw) - set(_LOG_DEFAULTS)
+    if unknown:
+        raise TypeError(f"unexpected keyword arguments: {sorted(unknown)}")
+    a = dict(_LOG_DEFAULTS)
+    a.update({k: v for k, v in kw.items() if v is not None or k in ("search_range", "memory")})
+    p = N.default_params()
+    p.style = N.GT_STYLE_LOG
+    p.min_radius = float(min_radius)
+    p.max_radius = float(max_radius)
+    p.n_radii = int(a["n_radii"])
+    p.log_spacing = int(bool(a["log_spacing"]))
+    p.overlap_threshold = float(a["overlap_threshold"])
+    p.include_r_in_output = 1
+    _common(p, a)
+    return p
+
+
 def LoG(file_or_array, min_radius, max_radius, **kwargs):
-    """Laplacian-of-Gaussian detection (generated.py:164-227)."""
-    raise N.GtError(100, "LoG is not available in this build yet")
+    """Laplacian-of-Gaussian blob detection (generated.py:164-227; the
+    reference wrapper passes the undefined names min_r, max_r, fixed here).
+
+    Returns a pandas DataFrame with columns frame, y, x, mass, r, [Rg, raw,
+    signal, ecc], [raw_mass, raw_bg_median, raw_mass_corrected],
+    [particle, duration]; rows of a frame in descending r, then mass."""
+    video = _as_video(file_or_array)
+    frames = kwargs.pop("frames", None)
+    reset = kwargs.pop("linker_reset_points", None)
+    kwargs.pop("tqdm", None)
+    kwargs.pop("channel", None)
+    p = make_log_params(min_radius, max_radius, **kwargs)
+    with Context(p, video.shape[1], video.shape[2]) as ctx:
+        data, names, types = ctx.execute(video, keys=frames, linker_reset_points=reset)
+    return _to_dataframe(data, names, types)

@@ -59,10 +59,6 @@ bool gt_host_derive(const gt_params *p, uint32_t H, uint32_t W, gt_derived *d, s
         *why = "unknown style";
         return false;
     }
-    if (p->style == GT_STYLE_LOG) {
-        *why = "LoG style is not available in this build yet";
-        return false;
-    }
     if (p->has_snr || p->has_minmass_snr || p->has_adaptive_background) {
         *why = "snr, minmass_snr and adaptive_background need the per-frame noise estimate, "
                "which this build does not provide";
@@ -72,35 +68,94 @@ bool gt_host_derive(const gt_params *p, uint32_t H, uint32_t W, gt_derived *d, s
         *why = "illumination correction is not available in this build";
         return false;
     }
-    if (p->diameter == 0) {
-        *why = "diameter must be at least 1";
-        return false;
-    }
     if (!(p->noise_size > 0.0f)) {
         *why = "noise_size must be positive";
         return false;
     }
     d->style = p->style;
-    // gpu_setup.rs:535-547
-    const uint32_t s = p->has_smoothing_size ? p->smoothing_size : p->diameter;
-    if (s == 0) {
-        *why = "smoothing_size must be at least 1";
-        return false;
-    }
-    if (s / 2 > GT_MAX_R) {
-        *why = "smoothing_size above 33 is not supported";
-        return false;
-    }
-    d->smoothing = s;
     d->sigma2 = p->noise_size;
-    d->circle = p->diameter;
-    d->separation = p->separation;
-    d->dilation = f2u_sat(2.0f * static_cast<float>(p->separation) / std::sqrt(2.0f));
-    uint32_t m = p->diameter / 2u;
-    const uint32_t m1 = p->separation / 2u - 1u;  // wraps for separation < 2, as the u32 code does
-    if (m1 > m) m = m1;
-    if (s / 2u > m) m = s / 2u;
-    d->margin = static_cast<int32_t>(m);
+    if (p->style == GT_STYLE_LOG) {
+        // gpu_setup.rs:549-557: smoothing side 2 * u32(max_radius + 0.5) + 1
+        if (!(p->min_radius > 0.0f) || !(p->max_radius > 0.0f) || !std::isfinite(p->min_radius) ||
+            !std::isfinite(p->max_radius)) {
+            *why = "min_radius and max_radius must be positive and finite";
+            return false;
+        }
+        if (p->n_radii < 1 || p->n_radii > GT_LOG_MAX_SCALES) {
+            *why = "n_radii must be between 1 and 64";
+            return false;
+        }
+        const uint32_t s = p->has_smoothing_size ? p->smoothing_size
+                                                 : 2u * f2u_sat(p->max_radius + 0.5f) + 1u;
+        if (s == 0) {
+            *why = "smoothing_size must be at least 1";
+            return false;
+        }
+        if (s / 2 > GT_MAX_R) {
+            *why = "smoothing_size above 33 is not supported";
+            return false;
+        }
+        d->smoothing = s;
+        d->circle = 0;
+        d->separation = 0;
+        d->dilation = 0;
+        d->margin = 0;
+        // gpu_setup.rs:958-984; the start value advances by f32 addition
+        const uint32_t n = p->n_radii;
+        d->radii.assign(n, 0.0f);
+        if (p->log_spacing) {
+            float start = ::log10f(p->min_radius);
+            const float end = ::log10f(p->max_radius);
+            const float diff = (end - start) / static_cast<float>(n - 1);
+            for (uint32_t k = 0; k + 1 < n; ++k) {
+                const float t = start;
+                start = start + diff;
+                d->radii[k] = ::powf(10.0f, t);
+            }
+            d->radii[n - 1] = ::powf(10.0f, start);
+        } else {
+            float start = p->min_radius;
+            const float diff = (p->max_radius - p->min_radius) / static_cast<float>(n - 1);
+            for (uint32_t k = 0; k + 1 < n; ++k) {
+                const float t = start;
+                start = start + diff;
+                d->radii[k] = t;
+            }
+            d->radii[n - 1] = start;
+        }
+        for (float r : d->radii) {
+            const float sigma = r / ::sqrtf(2.0f);
+            if (!(r > 0.0f) || f2i_sat(4.0f * sigma + 0.5f) > GT_LOG_MAX_R) {
+                *why = "every radius of the scan must be positive and at most 16.9";
+                return false;
+            }
+        }
+        d->overlap_threshold = p->overlap_threshold;
+    } else {
+        if (p->diameter == 0) {
+            *why = "diameter must be at least 1";
+            return false;
+        }
+        // gpu_setup.rs:535-547
+        const uint32_t s = p->has_smoothing_size ? p->smoothing_size : p->diameter;
+        if (s == 0) {
+            *why = "smoothing_size must be at least 1";
+            return false;
+        }
+        if (s / 2 > GT_MAX_R) {
+            *why = "smoothing_size above 33 is not supported";
+            return false;
+        }
+        d->smoothing = s;
+        d->circle = p->diameter;
+        d->separation = p->separation;
+        d->dilation = f2u_sat(2.0f * static_cast<float>(p->separation) / std::sqrt(2.0f));
+        uint32_t m = p->diameter / 2u;
+        const uint32_t m1 = p->separation / 2u - 1u;  // wraps for separation < 2, as the u32 code does
+        if (m1 > m) m = m1;
+        if (s / 2u > m) m = s / 2u;
+        d->margin = static_cast<int32_t>(m);
+    }
     d->max_iterations = p->max_iterations;
     d->shift_threshold = p->shift_threshold;
     d->minmass = p->minmass;
@@ -116,8 +171,38 @@ bool gt_host_derive(const gt_params *p, uint32_t H, uint32_t W, gt_derived *d, s
     // spans [gap_radius + radius, bg_radius (default 2 * radius)].
     const float radius = static_cast<float>(p->diameter) / 2.0f;
     d->bg_radius = p->has_bg_radius ? p->bg_radius : 2.0f * radius;
+    d->bg_radius_given = p->has_bg_radius ? 1 : 0;
     d->gap_radius = p->has_gap_radius ? p->gap_radius : 0.0f;
     return true;
+}
+
+void gt_host_fill_logtable(const gt_derived *d, GtLogTable *t) {
+    std::memset(t, 0, sizeof(*t));
+    t->n = static_cast<int32_t>(d->radii.size());
+    t->snr_thr = 0.0f * d->snr * d->rough;
+    int32_t off = 0;
+    for (int32_t k = 0; k < t->n; ++k) {
+        // separable_log.wgsl:22-89 with sigma = radius / sqrt(2)
+        // (execute_gpu.rs:253): taps exp(-x^2 / (2 sigma^2)) and that times
+        // (x^2 - sigma^2), norms 1/(sqrt(2 pi) sigma), 1/(sqrt(2 pi) sigma^3)
+        const float sigma = d->radii[k] / ::sqrtf(2.0f);
+        const float sigma2 = sigma * sigma;
+        const int32_t R = f2i_sat(4.0f * sigma + 0.5f);
+        t->radius[k] = d->radii[k];
+        t->R[k] = R;
+        t->off[k] = off;
+        for (int32_t i = -R; i <= R; ++i) {
+            const float x = static_cast<float>(i), x2 = x * x;
+            const float e = ::expf(-x2 / (2.0f * sigma2));
+            t->taps[off + i + R] = e;
+            t->taps[off + 2 * R + 1 + i + R] = e * (x2 - sigma2);
+        }
+        off += 2 * (2 * R + 1);
+        const float sqrt2pi = 2.5066282746310002f;
+        t->n1[k] = 1.0f / (sqrt2pi * sigma);
+        t->n3[k] = 1.0f / (sqrt2pi * sigma * sigma2);
+        if (R > t->Rm) t->Rm = R;
+    }
 }
 
 void gt_host_fill_kparams(const gt_derived *d, uint32_t H, uint32_t W, GtKParams *kp) {
@@ -308,10 +393,25 @@ gt_status gt_host_post::process(std::vector<gt_detection> &rows, uint64_t frame_
         n = m;
         tmp_rows_.resize(n);
     }
+    if (d_.style == GT_STYLE_LOG && n > 0) {
+        n = gt_host_prune_blobs(tmp_rows_.data(), n, d_.overlap_threshold);
+        tmp_rows_.resize(n);
+    }
     if (d_.doughnut) {
         dough_.assign(3 * n, 0.0f);
-        if (n) gt_doughnut(raw, H_, W_, tmp_rows_.data(), n, static_cast<float>(d_.circle) / 2.0f,
-                           d_.bg_radius, d_.gap_radius, dough_.data());
+        if (n && d_.style == GT_STYLE_LOG) {
+            // per detection: radius r, ring [gap + r, bg_radius or 2 r]
+            // (execute_gpu.rs:740-747 with :925-931)
+            for (size_t i = 0; i < n; ++i) {
+                const float r = tmp_rows_[i].r;
+                gt_doughnut(raw, H_, W_, &tmp_rows_[i], 1, r,
+                            d_.bg_radius_given ? d_.bg_radius : 2.0f * r, d_.gap_radius,
+                            &dough_[3 * i]);
+            }
+        } else if (n) {
+            gt_doughnut(raw, H_, W_, tmp_rows_.data(), n, static_cast<float>(d_.circle) / 2.0f,
+                        d_.bg_radius, d_.gap_radius, dough_.data());
+        }
     }
     const float fidx = static_cast<float>(frame_index);
     for (size_t i = 0; i < n; ++i) {
@@ -470,5 +570,97 @@ extern "C" gt_status gt_doughnut(const float *raw, uint32_t H, uint32_t W, const
         out3[3 * k + 1] = med;
         out3[3 * k + 2] = s - med * static_cast<float>(ns);
     }
+    return GT_OK;
+}
+
+namespace {
+
+// blob_overlap / disk_overlap (execute_gpu.rs:597-630), f32 throughout as
+// the Rust code (no contraction).
+float make_acos(float d, float r1, float r1s, float r2s) {
+    float ratio = (d * d + r1s - r2s) / (2.0f * d * r1);
+    ratio = std::min(std::max(ratio, -1.0f), 1.0f);
+    return ::acosf(ratio);
+}
+
+float blob_overlap(float r1, float r2, float distance) {
+    if (distance > r1 + r2) return 0.0f;
+    if (distance <= std::fabs(r1 - r2)) return 1.0f;
+    const float d = distance;
+    const float r1s = r1 * r1, r2s = r2 * r2;
+    const float acos1 = make_acos(d, r1, r1s, r2s);
+    const float acos2 = make_acos(d, r2, r2s, r1s);
+    const float a = -d + r2 + r1;
+    const float b = d - r2 + r1;
+    const float c = d + r2 - r1;
+    const float e = d + r2 + r1;
+    const float area = r1s * acos1 + r2s * acos2 - 0.5f * ::sqrtf(std::fabs(a * b * c * e));
+    const float pi = 3.14159265358979323846f;
+    return area / (pi * std::min(r1s, r2s));
+}
+
+}  // namespace
+
+size_t gt_host_prune_blobs(gt_result_row *rows, size_t n, float overlap_threshold) {
+    if (!(overlap_threshold < 1.0f) || n == 0) return n;
+    // Stable sort, r descending then mass descending.  The reference's
+    // partial_cmp().unwrap() would panic on NaN; here NaN sorts last.
+    auto kf = [](float v) { return v != v ? -INFINITY : v; };
+    std::stable_sort(rows, rows + n, [&](const gt_result_row &a, const gt_result_row &b) {
+        const float ar = kf(a.r), br = kf(b.r);
+        if (ar != br) return ar > br;
+        return kf(a.mass) > kf(b.mass);
+    });
+    // prune_blobs_log: every surviving blob in order drops every surviving
+    // other blob with distance^2 < (2 r)^2 whose overlap exceeds the
+    // threshold.  Neighbours come from a uniform grid whose cell is the
+    // largest query radius, so the 3 x 3 cells around a query cover it.
+    float rmax = 0.0f;
+    for (size_t i = 0; i < n; ++i)
+        if (rows[i].r > rmax) rmax = rows[i].r;
+    // (slightly enlarged so rounding of x / cell cannot push a neighbour two
+    // cells away)
+    const double cell = rmax > 0.0f ? 2.0001 * static_cast<double>(rmax) : 1.0;
+    std::unordered_map<int64_t, std::vector<uint32_t>> grid;
+    grid.reserve(n);
+    auto key = [](int64_t cx, int64_t cy) { return (cx << 32) ^ (cy & 0xFFFFFFFFll); };
+    std::vector<int64_t> cxs(n), cys(n);
+    std::vector<uint8_t> finite(n), keep(n, 1);
+    for (size_t i = 0; i < n; ++i) {
+        finite[i] = std::isfinite(rows[i].x) && std::isfinite(rows[i].y);
+        if (!finite[i]) continue;
+        cxs[i] = static_cast<int64_t>(std::floor(static_cast<double>(rows[i].x) / cell));
+        cys[i] = static_cast<int64_t>(std::floor(static_cast<double>(rows[i].y) / cell));
+        grid[key(cxs[i], cys[i])].push_back(static_cast<uint32_t>(i));
+    }
+    for (size_t q = 0; q < n; ++q) {
+        if (!keep[q] || !finite[q]) continue;
+        const float rad = rows[q].r * 2.0f, r2 = rad * rad;
+        for (int64_t dx = -1; dx <= 1; ++dx)
+            for (int64_t dy = -1; dy <= 1; ++dy) {
+                auto it = grid.find(key(cxs[q] + dx, cys[q] + dy));
+                if (it == grid.end()) continue;
+                for (uint32_t k : it->second) {
+                    if (k == q || !keep[k]) continue;
+                    const float ddx = rows[k].x - rows[q].x, ddy = rows[k].y - rows[q].y;
+                    float d2 = 0.0f;
+                    d2 = d2 + ddx * ddx;
+                    d2 = d2 + ddy * ddy;
+                    if (!(d2 < r2)) continue;
+                    if (blob_overlap(rows[q].r, rows[k].r, ::sqrtf(d2)) > overlap_threshold)
+                        keep[k] = 0;
+                }
+            }
+    }
+    size_t m = 0;
+    for (size_t i = 0; i < n; ++i)
+        if (keep[i]) rows[m++] = rows[i];
+    return m;
+}
+
+extern "C" gt_status gt_prune_blobs(gt_result_row *rows, uint64_t n, float overlap_threshold,
+                                    uint64_t *n_out) {
+    if (!n_out || (n && !rows)) return GT_ERR_INVALID_ARGUMENT;
+    *n_out = gt_host_prune_blobs(rows, static_cast<size_t>(n), overlap_threshold);
     return GT_OK;
 }

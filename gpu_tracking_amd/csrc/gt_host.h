@@ -31,11 +31,21 @@ struct gt_derived {
     float bg_radius = 0.0f;     // resolved doughnut outer radius
     float gap_radius = 0.0f;
     int32_t include_r = 0;
+    // LoG style
+    std::vector<float> radii;   // the radius scan (gpu_setup.rs:958-984)
+    float overlap_threshold = 0.0f;
+    int32_t bg_radius_given = 0;  // LoG: else the ring's outer radius is 2 r per row
 };
 
 void gt_host_params_default(gt_params *p);
 bool gt_host_derive(const gt_params *p, uint32_t H, uint32_t W, gt_derived *d, std::string *why);
 void gt_host_fill_kparams(const gt_derived *d, uint32_t H, uint32_t W, GtKParams *kp);
+// The LoG scale table (radii, tap radii, tap weights and norms).
+void gt_host_fill_logtable(const gt_derived *d, GtLogTable *t);
+// LoG post-processing of one frame (execute_gpu.rs:699-712): stable sort by
+// r descending then mass descending, then prune_blobs_log (:592-669) when
+// overlap_threshold < 1.  Compacts rows in place; returns the survivor count.
+size_t gt_host_prune_blobs(gt_result_row *rows, size_t n, float overlap_threshold);
 uint32_t gt_host_column_names(const gt_params *p, const char **names, const char **types);
 void gt_host_widen(const void *src, int32_t dtype, size_t n, float *dst);
 

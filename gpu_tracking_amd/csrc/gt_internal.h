@@ -45,5 +45,18 @@ hipError_t gt_launch_copy_counts(const uint32_t *strided, uint32_t *dst, uint32_
 hipError_t gt_launch_characterize_det(const float *proc, const void *raw, int dtype,
                                       uint32_t nframes, const GtKParams &p, gt_detection *out,
                                       uint32_t out_cap, const uint32_t *ocount, hipStream_t s);
+// LoG style: scale-space maxima of every scale of a chunk (planes kept in
+// LDS), then their walks.  Candidate seeds are scale * H * W + raster index.
+size_t gt_log_lds_bytes(int Rm);
+hipError_t gt_launch_log_tile(const void *frames, int dtype, const float *proc, uint32_t nframes,
+                              const GtKParams &p, const GtLogTable *tab, int Rm, uint32_t *cand,
+                              uint32_t cap, uint32_t *ccount, hipStream_t s);
+hipError_t gt_launch_walk_log(const float *proc, const void *raw, int dtype, uint32_t nframes,
+                              const GtKParams &p, const GtLogTable *tab, const uint32_t *cand,
+                              uint32_t cap, const uint32_t *ccount, gt_detection *out,
+                              uint32_t out_cap, uint32_t *ocount, uint32_t frame0, hipStream_t s);
+// Per-frame detection counts, 0xFFFFFFFF where the candidate list overflowed.
+hipError_t gt_launch_copy_counts_checked(const uint32_t *ocount, const uint32_t *ccount,
+                                         uint32_t ccap, uint32_t *dst, uint32_t n, hipStream_t s);
 // Fills the constant tables the kernels read (once per device).
 hipError_t gt_init_tables();

@@ -1474,3 +1474,392 @@ hipError_t gt_launch_characterize_det(const float *proc, const void *raw, int dt
 #undef GT_CHAR
     return hipGetLastError();
 }
+
+// ---------------------------------------------------------------------------
+// LoG style (execute_gpu.rs:237-286).  One block per LG_TH x LG_TW tile of
+// one frame computes every scale's LoG plane over the tile plus a one-pixel
+// ring (separable_log.wgsl through Laplace<2>, utils.rs:114-296) from one
+// LDS copy of the raw tile, keeps the last three planes in LDS, and tests the
+// scale-space maxima (logspace_max.wgsl:62-154) as soon as a scale's upper
+// neighbour exists.  The planes never go to HBM.  Candidates (seed = scale *
+// H * W + raster index) go to a per-frame list; k_walk_log walks them.
+namespace {
+
+constexpr int LG_TW = 64, LG_TH = 14;
+constexpr int LG_RR = LG_TH + 2, LG_RC = LG_TW + 2;  // plane region: tile + 1-pixel ring
+constexpr int LG_MV = 8;                             // rows per lane in the vertical pass
+constexpr int LG_MH = 6;                             // columns per lane in the horizontal pass
+constexpr int LG_THREADS = 256;
+static_assert(LG_RR % LG_MV == 0, "vertical chunks must tile the region rows");
+static_assert(LG_RC % LG_MH == 0, "horizontal chunks must tile the region columns");
+
+// LDS layout for a largest tap radius Rm:
+//   raw   RRows x RP  (region rows + 2 Rm, plus LG_MV rows the sliding window
+//                      may load past the last tap; those are never used)
+//   Vd,Vg LG_RR x VP  (vertical-pass outputs, both tap sets; VP odd)
+//   L     3 x LG_RR x LG_RC (ring of the last three scales' planes)
+struct LogGeom {
+    int Rm, RP, RRows, VP;
+};
+inline LogGeom log_geom(int Rm) {
+    LogGeom g;
+    g.Rm = Rm;
+    g.RP = LG_RC + 2 * Rm;
+    g.RRows = LG_RR + 2 * Rm + LG_MV;
+    g.VP = (LG_RC + 2 * Rm + LG_MH) | 1;
+    return g;
+}
+inline size_t log_lds_bytes(const LogGeom &g) {
+    return sizeof(float) * (static_cast<size_t>(g.RRows) * g.RP + 2u * LG_RR * g.VP +
+                            3u * LG_RR * LG_RC);
+}
+
+// Vertical 1-D pass of one scale for LG_MV consecutive outputs of one column:
+// ad = sum_i td[i] * x[i + m], ag = sum_i tg[i] * x[i + m], taps in order
+// i = 0..ntaps-1 (the reference's -R..R).  The inputs slide through a
+// register window; inside a block of LG_MV taps the window index rotates at
+// compile time, so no register moves are needed there.
+__device__ __forceinline__ void log_vpass(const float *colp, int RP, int ntaps,
+                                          const float *__restrict__ tg,
+                                          const float *__restrict__ td, float (&ag)[LG_MV],
+                                          float (&ad)[LG_MV]) {
+    float w[LG_MV];
+#pragma unroll
+    for (int m = 0; m < LG_MV; ++m) {
+        w[m] = colp[m * RP];
+        ag[m] = 0.0f;
+        ad[m] = 0.0f;
+    }
+    int i = 0;
+    for (; i + LG_MV <= ntaps; i += LG_MV) {
+#pragma unroll
+        for (int k = 0; k < LG_MV; ++k) {
+            const float g = tg[i + k], d = td[i + k];
+#pragma unroll
+            for (int m = 0; m < LG_MV; ++m) {
+                const float v = w[(m + k) % LG_MV];
+                ad[m] = fmaf(d, v, ad[m]);
+                ag[m] = fmaf(g, v, ag[m]);
+            }
+            w[k] = colp[(i + k + LG_MV) * RP];
+        }
+    }
+    for (; i < ntaps; ++i) {
+        const float g = tg[i], d = td[i];
+#pragma unroll
+        for (int m = 0; m < LG_MV; ++m) {
+            ad[m] = fmaf(d, w[m], ad[m]);
+            ag[m] = fmaf(g, w[m], ag[m]);
+        }
+#pragma unroll
+        for (int m = 0; m + 1 < LG_MV; ++m) w[m] = w[m + 1];
+        w[LG_MV - 1] = colp[(i + LG_MV) * RP];
+    }
+}
+
+// Horizontal 1-D pass for LG_MH consecutive outputs of one row:
+// a0 = sum_j tg[j] * vd[j + m], a1 = sum_j td[j] * vg[j + m].
+__device__ __forceinline__ void log_hpass(const float *vd, const float *vg, int ntaps,
+                                          const float *__restrict__ tg,
+                                          const float *__restrict__ td, float (&a0)[LG_MH],
+                                          float (&a1)[LG_MH]) {
+    float wa[LG_MH], wb[LG_MH];
+#pragma unroll
+    for (int m = 0; m < LG_MH; ++m) {
+        wa[m] = vd[m];
+        wb[m] = vg[m];
+        a0[m] = 0.0f;
+        a1[m] = 0.0f;
+    }
+    int j = 0;
+    for (; j + LG_MH <= ntaps; j += LG_MH) {
+#pragma unroll
+        for (int k = 0; k < LG_MH; ++k) {
+            const float g = tg[j + k], d = td[j + k];
+#pragma unroll
+            for (int m = 0; m < LG_MH; ++m) {
+                const int q = (m + k) % LG_MH;
+                a0[m] = fmaf(g, wa[q], a0[m]);
+                a1[m] = fmaf(d, wb[q], a1[m]);
+            }
+            wa[k] = vd[j + k + LG_MH];
+            wb[k] = vg[j + k + LG_MH];
+        }
+    }
+    for (; j < ntaps; ++j) {
+        const float g = tg[j], d = td[j];
+#pragma unroll
+        for (int m = 0; m < LG_MH; ++m) {
+            a0[m] = fmaf(g, wa[m], a0[m]);
+            a1[m] = fmaf(d, wb[m], a1[m]);
+        }
+#pragma unroll
+        for (int m = 0; m + 1 < LG_MH; ++m) {
+            wa[m] = wa[m + 1];
+            wb[m] = wb[m + 1];
+        }
+        wa[LG_MH - 1] = vd[j + LG_MH];
+        wb[LG_MH - 1] = vg[j + LG_MH];
+    }
+}
+
+// is_max_in_plane (logspace_max.wgsl:62-92) for a pixel whose 8 neighbours
+// are in the frame: no neighbour >= the centre.
+__device__ __forceinline__ bool log_plane_max(const float *L, int r, int c, float ctr) {
+#pragma unroll
+    for (int a = -1; a <= 1; ++a)
+#pragma unroll
+        for (int b = -1; b <= 1; ++b) {
+            if (a == 0 && b == 0) continue;
+            if (L[(r + a) * LG_RC + (c + b)] >= ctr) return false;
+        }
+    return true;
+}
+
+template <typename T>
+__global__ __launch_bounds__(LG_THREADS) void k_log_tile(const T *__restrict__ raw,
+                                                         const float *__restrict__ proc,
+                                                         const GtLogTable *__restrict__ tab,
+                                                         LogGeom g, int H, int W,
+                                                         uint32_t *__restrict__ cand, uint32_t cap,
+                                                         uint32_t *__restrict__ ccount,
+                                                         int nframes) {
+    extern __shared__ float lds[];
+    const int ntx = (W + LG_TW - 1) / LG_TW, nty = (H + LG_TH - 1) / LG_TH;
+    const int per_frame = ntx * nty;
+    const int t = xcd_tile(per_frame * nframes);
+    if (t >= per_frame * nframes) return;
+    const int fr = t / per_frame;
+    const int tt = t - fr * per_frame;
+    const int ty = tt / ntx, tx = tt - ty * ntx;
+    const int y0 = ty * LG_TH, x0 = tx * LG_TW;
+    const size_t HW = static_cast<size_t>(H) * W;
+    const T *f = raw + static_cast<size_t>(fr) * HW;
+    const float *P = proc + static_cast<size_t>(fr) * HW;
+    const int Rm = g.Rm, RP = g.RP, VP = g.VP;
+    float *Sraw = lds;
+    float *Vd = Sraw + g.RRows * RP;
+    float *Vg = Vd + LG_RR * VP;
+    float *Lr = Vg + LG_RR * VP;
+    const int tid = threadIdx.x;
+    // raw tile: region origin (y0 - 1, x0 - 1) less the largest tap radius;
+    // outside the frame 0 (a zero sample adds nothing to a tap sum, as the
+    // reference's skipped out-of-frame taps)
+    const int oy = y0 - 1 - Rm, ox = x0 - 1 - Rm;
+    const int nload = (LG_RR + 2 * Rm) * RP;
+    for (int idx = tid; idx < nload; idx += LG_THREADS) {
+        const int ry = idx / RP, rx = idx - ry * RP;
+        const int gy = oy + ry, gx = ox + rx;
+        float v = 0.0f;
+        if (gy >= 0 && gy < H && gx >= 0 && gx < W) v = to_f32(f[static_cast<size_t>(gy) * W + gx]);
+        Sraw[idx] = v;
+    }
+    __syncthreads();
+    const int n = tab->n;
+    const float snr_thr = tab->snr_thr;
+    uint32_t *cnt = &ccount[fr * GT_COUNTER_STRIDE];
+    uint32_t *fc = cand + static_cast<size_t>(fr) * cap;
+    // scale-space maximum test of scale m on the tile's interior pixels
+    auto test = [&](int m) {
+        const float *Lm = Lr + (m % 3) * (LG_RR * LG_RC);
+        const float *Lb = m >= 1 ? Lr + ((m + 2) % 3) * (LG_RR * LG_RC) : nullptr;
+        const float *Lt = m + 1 < n ? Lr + ((m + 1) % 3) * (LG_RR * LG_RC) : nullptr;
+        // LG_TH * LG_TW is a multiple of 128, so each wave runs whole iterations
+        for (int pi = tid; pi < LG_TH * LG_TW; pi += LG_THREADS) {
+            const int r = 1 + pi / LG_TW, c = 1 + (pi % LG_TW);
+            const int gy = y0 + r - 1, gx = x0 + c - 1;
+            bool flag = gy >= 1 && gy <= H - 2 && gx >= 1 && gx <= W - 2;
+            if (flag) {
+                const float ctr = Lm[r * LG_RC + c];
+                flag = !(ctr <= 0.0f) && log_plane_max(Lm, r, c, ctr);
+                if (flag && Lb) flag = log_plane_max(Lb, r, c, ctr);
+                if (flag && Lt) flag = log_plane_max(Lt, r, c, ctr);
+                if (flag) flag = !(P[static_cast<size_t>(gy) * W + gx] < snr_thr);
+            }
+            const uint32_t pos = wave_append(flag, cnt);
+            if (flag && pos < cap)
+                fc[pos] = static_cast<uint32_t>(m) * static_cast<uint32_t>(HW) +
+                          static_cast<uint32_t>(gy) * static_cast<uint32_t>(W) +
+                          static_cast<uint32_t>(gx);
+        }
+    };
+    for (int k = 0; k < n; ++k) {
+        const int R = tab->R[k], ntaps = 2 * R + 1;
+        const float *tg = tab->taps + tab->off[k];
+        const float *td = tg + ntaps;
+        const float n1 = tab->n1[k], n3 = tab->n3[k];
+        // vertical pass over columns -R .. LG_RC + R of the region
+        const int ncv = LG_RC + 2 * R;
+        for (int item = tid; item < (LG_RR / LG_MV) * ncv; item += LG_THREADS) {
+            const int q = item / ncv, vc = item - q * ncv, r0 = q * LG_MV;
+            const float *colp = Sraw + (r0 + Rm - R) * RP + vc + (Rm - R);
+            float ag[LG_MV], ad[LG_MV];
+            log_vpass(colp, RP, ntaps, tg, td, ag, ad);
+#pragma unroll
+            for (int m = 0; m < LG_MV; ++m) {
+                Vd[(r0 + m) * VP + vc] = ad[m] * n3;
+                Vg[(r0 + m) * VP + vc] = ag[m] * n1;
+            }
+        }
+        __syncthreads();
+        // horizontal pass; L = (0 - R0) - R1 as the reference subtracts both
+        // passes from a zeroed plane
+        float *Lk = Lr + (k % 3) * (LG_RR * LG_RC);
+        for (int item = tid; item < LG_RR * (LG_RC / LG_MH); item += LG_THREADS) {
+            const int r = item % LG_RR, c0 = (item / LG_RR) * LG_MH;
+            float a0[LG_MH], a1[LG_MH];
+            log_hpass(Vd + r * VP + c0, Vg + r * VP + c0, ntaps, tg, td, a0, a1);
+#pragma unroll
+            for (int m = 0; m < LG_MH; ++m) {
+                float l = 0.0f - a0[m] * n1;
+                l = l - a1[m] * n3;
+                Lk[r * LG_RC + c0 + m] = l;
+            }
+        }
+        __syncthreads();
+        if (k >= 1) test(k - 1);
+    }
+    test(n - 1);
+}
+
+// Walk + characterize of the LoG candidates (walk.wgsl with r > 0: rint =
+// ceil(r), r2 = r*r, move band rint; characterize.wgsl at the same radius).
+// A frame whose candidate list overflowed reports count 0xFFFFFFFF.
+template <typename T>
+__global__ __launch_bounds__(WK_THREADS) void k_walk_log(const float *__restrict__ proc,
+                                                         const T *__restrict__ raw, GtKParams p,
+                                                         const GtLogTable *__restrict__ tab,
+                                                         const uint32_t *__restrict__ cand,
+                                                         uint32_t cap,
+                                                         const uint32_t *__restrict__ ccount,
+                                                         gt_detection *__restrict__ out,
+                                                         uint32_t out_cap,
+                                                         uint32_t *__restrict__ ocount,
+                                                         uint32_t frame0) {
+    const int fr = blockIdx.y;
+    const int H = p.H, W = p.W;
+    const uint32_t HW = static_cast<uint32_t>(H) * static_cast<uint32_t>(W);
+    const GlobalSrc src{proc + static_cast<size_t>(fr) * HW, W};
+    const T *Rw = raw + static_cast<size_t>(fr) * HW;
+    const uint32_t nc = ccount[fr * GT_COUNTER_STRIDE];
+    const uint32_t n = min(nc, cap);
+    const uint32_t *fc = cand + static_cast<size_t>(fr) * cap;
+    gt_detection *fout = out + static_cast<size_t>(fr) * out_cap;
+    const uint32_t lane = __lane_id();
+    const uint32_t stride = gridDim.x * WK_THREADS;
+    for (uint32_t i0 = blockIdx.x * WK_THREADS + (threadIdx.x & ~63u); i0 < n; i0 += stride) {
+        const uint32_t i = i0 + lane;
+        bool keep = false;
+        gt_detection d;
+        if (i < n) {
+            const uint32_t seed = fc[i];
+            const uint32_t m = seed / HW;
+            const uint32_t ras = seed - m * HW;
+            int pu = static_cast<int>(ras / static_cast<uint32_t>(W));
+            int pv = static_cast<int>(ras - static_cast<uint32_t>(pu) * static_cast<uint32_t>(W));
+            const float r = tab->radius[m];
+            const int rint = f2i(ceilf(r));
+            const float r2 = r * r;
+            WalkRes res = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+            for (uint32_t it = 0; it < p.max_iter; ++it) {
+                res = get_center<0>(src, H, W, pu, pv, rint, r2);
+                if (!walk_move(res, pu, pv, H, W, rint, p.shift_thr)) break;
+            }
+            res.x = res.x + static_cast<float>(pu);
+            res.y = res.y + static_cast<float>(pv);
+            keep = passes_minmass(p, res);
+            if (keep) {
+                fill_row(d, res, seed, frame0 + fr);
+                d.row.r = r;
+                if (p.characterize) characterize_row<T>(src, Rw, H, W, p.circle, d.row);
+            }
+        }
+        const uint32_t pos = wave_append(keep, &ocount[fr * GT_COUNTER_STRIDE]);
+        if (keep && pos < out_cap) fout[pos] = d;
+    }
+}
+
+__global__ void k_copy_counts_checked(const uint32_t *__restrict__ ocount,
+                                      const uint32_t *__restrict__ ccount, uint32_t ccap,
+                                      uint32_t *__restrict__ dst, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        dst[i] = ccount[i * GT_COUNTER_STRIDE] > ccap ? 0xFFFFFFFFu : ocount[i * GT_COUNTER_STRIDE];
+}
+
+template <typename T>
+hipError_t launch_log_tile_t(const T *raw, const float *proc, uint32_t nframes, const GtKParams &p,
+                             const GtLogTable *tab, int Rm, uint32_t *cand, uint32_t cap,
+                             uint32_t *ccount, hipStream_t s) {
+    const LogGeom g = log_geom(Rm);
+    const size_t lds = log_lds_bytes(g);
+    if (lds > 65536) {
+        const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_log_tile<T>),
+                                                 hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                 static_cast<int>(lds));
+        if (e != hipSuccess) return e;
+    }
+    const size_t ntiles = static_cast<size_t>((p.W + LG_TW - 1) / LG_TW) *
+                          ((p.H + LG_TH - 1) / LG_TH) * nframes;
+    k_log_tile<T><<<xcd_grid(ntiles), LG_THREADS, lds, s>>>(raw, proc, tab, g, p.H, p.W, cand, cap,
+                                                            ccount, static_cast<int>(nframes));
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_walk_log_t(const float *proc, const T *raw, uint32_t nframes, const GtKParams &p,
+                             const GtLogTable *tab, const uint32_t *cand, uint32_t cap,
+                             const uint32_t *ccount, gt_detection *out, uint32_t out_cap,
+                             uint32_t *ocount, uint32_t frame0, hipStream_t s) {
+    // enough blocks to cover a typical frame's list in one pass
+    const uint32_t blocks = std::min<uint32_t>(1024u, std::max<uint32_t>(1u, cap / WK_THREADS / 4u));
+    const dim3 grid(blocks, nframes, 1);
+    k_walk_log<T><<<grid, WK_THREADS, 0, s>>>(proc, raw, p, tab, cand, cap, ccount, out, out_cap,
+                                              ocount, frame0);
+    return hipGetLastError();
+}
+
+}  // namespace
+
+size_t gt_log_lds_bytes(int Rm) { return log_lds_bytes(log_geom(Rm)); }
+
+hipError_t gt_launch_log_tile(const void *frames, int dtype, const float *proc, uint32_t nframes,
+                              const GtKParams &p, const GtLogTable *tab, int Rm, uint32_t *cand,
+                              uint32_t cap, uint32_t *ccount, hipStream_t s) {
+    switch (dtype) {
+    case GT_U16:
+        return launch_log_tile_t(static_cast<const uint16_t *>(frames), proc, nframes, p, tab, Rm,
+                                 cand, cap, ccount, s);
+    case GT_F32:
+        return launch_log_tile_t(static_cast<const float *>(frames), proc, nframes, p, tab, Rm,
+                                 cand, cap, ccount, s);
+    case GT_U8:
+        return launch_log_tile_t(static_cast<const uint8_t *>(frames), proc, nframes, p, tab, Rm,
+                                 cand, cap, ccount, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t gt_launch_walk_log(const float *proc, const void *raw, int dtype, uint32_t nframes,
+                              const GtKParams &p, const GtLogTable *tab, const uint32_t *cand,
+                              uint32_t cap, const uint32_t *ccount, gt_detection *out,
+                              uint32_t out_cap, uint32_t *ocount, uint32_t frame0, hipStream_t s) {
+    switch (dtype) {
+    case GT_U16:
+        return launch_walk_log_t(proc, static_cast<const uint16_t *>(raw), nframes, p, tab, cand,
+                                 cap, ccount, out, out_cap, ocount, frame0, s);
+    case GT_F32:
+        return launch_walk_log_t(proc, static_cast<const float *>(raw), nframes, p, tab, cand, cap,
+                                 ccount, out, out_cap, ocount, frame0, s);
+    case GT_U8:
+        return launch_walk_log_t(proc, static_cast<const uint8_t *>(raw), nframes, p, tab, cand,
+                                 cap, ccount, out, out_cap, ocount, frame0, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t gt_launch_copy_counts_checked(const uint32_t *ocount, const uint32_t *ccount,
+                                         uint32_t ccap, uint32_t *dst, uint32_t n, hipStream_t s) {
+    k_copy_counts_checked<<<(n + 255) / 256, 256, 0, s>>>(ocount, ccount, ccap, dst, n);
+    return hipGetLastError();
+}

@@ -40,3 +40,21 @@ struct GtKParams {
     int32_t characterize;
     int32_t circle;
 };
+
+// LoG style: the radius scan and each scale's tap table (separable_log.wgsl),
+// one device copy per context, read by the LoG kernels through a pointer.
+#define GT_LOG_MAX_SCALES 64
+#define GT_LOG_MAX_R 48
+#define GT_LOG_MAX_TAPS (2 * GT_LOG_MAX_SCALES * (2 * GT_LOG_MAX_R + 1))
+struct GtLogTable {
+    int32_t n;                         // scales
+    int32_t Rm;                        // largest tap radius over the scales
+    float snr_thr;                     // image_std * snr * rough_snr_factor
+    int32_t pad;
+    float radius[GT_LOG_MAX_SCALES];   // the radius scan
+    int32_t R[GT_LOG_MAX_SCALES];      // i32(4 sigma + 0.5), sigma = radius / sqrt(2)
+    int32_t off[GT_LOG_MAX_SCALES];    // Gaussian taps at taps[off], second derivative after them
+    float n1[GT_LOG_MAX_SCALES];       // 1 / (sqrt(2 pi) sigma)
+    float n3[GT_LOG_MAX_SCALES];       // 1 / (sqrt(2 pi) sigma sigma^2)
+    float taps[GT_LOG_MAX_TAPS];
+};

@@ -72,6 +72,9 @@ struct gt_context {
     uint32_t slow_cap = 0;
     uint32_t cand_cap = 0;
     bool tiled = true;             // fused tile kernel fits in LDS
+    bool log = false;              // LoG style
+    GtLogTable *d_logtab = nullptr;  // LoG scale table (device copy)
+    int32_t log_Rm = 0;
 
     // gt_execute staging
     void *d_frames = nullptr;      // chunk * HW pixels (largest dtype)
@@ -129,6 +132,7 @@ void free_all(gt_context *c) {
     if (c->d_ocount) (void)hipFree(c->d_ocount);
     if (c->d_slow) (void)hipFree(c->d_slow);
     if (c->d_scount) (void)hipFree(c->d_scount);
+    if (c->d_logtab) (void)hipFree(c->d_logtab);
     if (c->d_frames) (void)hipFree(c->d_frames);
     if (c->d_rows) (void)hipFree(c->d_rows);
     if (c->d_counts) (void)hipFree(c->d_counts);
@@ -162,12 +166,24 @@ gt_status run_pipeline(gt_context *c, const void *d_frames, int32_t dtype, uint3
         const char *frames = static_cast<const char *>(d_frames) + esz * HW * f0;
         hipEvent_t *ev = c->timing ? &c->events[4 * ci] : nullptr;
         GT_HIP(gt_launch_zero_u32(c->d_ocount, static_cast<size_t>(nf) * GT_COUNTER_STRIDE, s));
-        if (!c->tiled)
+        if (!c->tiled || c->log)
             GT_HIP(gt_launch_zero_u32(c->d_ccount, static_cast<size_t>(nf) * GT_COUNTER_STRIDE, s));
         if (ev) GT_HIP(hipEventRecord(ev[0], s));
         GT_HIP(gt_launch_bandpass(frames, dtype, c->d_proc, nf, c->kp, s));
         if (ev) GT_HIP(hipEventRecord(ev[1], s));
         gt_detection *out = d_out + static_cast<size_t>(f0) * cap;
+        if (c->log) {
+            // scale-space maxima of all scales, then their walks
+            GT_HIP(gt_launch_log_tile(frames, dtype, c->d_proc, nf, c->kp, c->d_logtab, c->log_Rm,
+                                      c->d_cand, c->cand_cap, c->d_ccount, s));
+            if (ev) GT_HIP(hipEventRecord(ev[2], s));
+            GT_HIP(gt_launch_walk_log(c->d_proc, frames, dtype, nf, c->kp, c->d_logtab, c->d_cand,
+                                      c->cand_cap, c->d_ccount, out, cap, c->d_ocount, f0, s));
+            GT_HIP(gt_launch_copy_counts_checked(c->d_ocount, c->d_ccount, c->cand_cap,
+                                                 d_counts + f0, nf, s));
+            if (ev) GT_HIP(hipEventRecord(ev[3], s));
+            continue;
+        }
         if (c->tiled) {
             // local max, walk and characterize in one kernel
             GT_HIP(gt_launch_zero_u32(c->d_scount, static_cast<size_t>(nf) * GT_COUNTER_STRIDE, s));
@@ -230,7 +246,14 @@ gt_status gt_context_create(const gt_params *params, uint32_t nrows, uint32_t nc
     c->der = der;
     gt_host_fill_kparams(&der, nrows, ncols, &c->kp);
     // the fused tile kernel needs its staged region in 64 KiB of LDS
-    c->tiled = c->kp.k >= 1 && gt_tile_lds_bytes(c->kp) <= 65536;
+    c->tiled = der.style == GT_STYLE_TRACKPY && c->kp.k >= 1 && gt_tile_lds_bytes(c->kp) <= 65536;
+    c->log = der.style == GT_STYLE_LOG;
+    if (c->log && static_cast<uint64_t>(nrows) * ncols * der.radii.size() > 0xFFFFFFFFull) {
+        delete c;
+        return fail(GT_ERR_INVALID_ARGUMENT,
+                    "LoG candidates are keyed by scale * nrows * ncols + pixel, which must stay "
+                    "below 2^32");
+    }
     if (hipGetDevice(&c->device) != hipSuccess) c->device = 0;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
         delete c;
@@ -249,6 +272,19 @@ gt_status gt_context_create(const gt_params *params, uint32_t nrows, uint32_t nc
     c->cand_cap = static_cast<uint32_t>(HW);
     c->det_cap = static_cast<uint32_t>(HW);
     gt_status st = alloc_scratch(c);
+    if (st == GT_OK && c->log) {
+        GtLogTable *tab = new (std::nothrow) GtLogTable;
+        if (!tab) {
+            st = fail(GT_ERR_GPU_DEVICE, "out of host memory");
+        } else {
+            gt_host_fill_logtable(&der, tab);
+            c->log_Rm = tab->Rm;
+            hipError_t e = hipMalloc(&c->d_logtab, sizeof(GtLogTable));
+            if (e == hipSuccess) e = hipMemcpy(c->d_logtab, tab, sizeof(GtLogTable), hipMemcpyHostToDevice);
+            delete tab;
+            if (e != hipSuccess) st = fail(GT_ERR_GPU_DEVICE, "LoG table upload failed: %s", hipGetErrorString(e));
+        }
+    }
     if (st != GT_OK) {
         std::string msg = g_last_error;
         free_all(c);
@@ -271,8 +307,6 @@ gt_status gt_detect_device(gt_context *c, const void *d_frames, int32_t dtype, u
                            void *stream) {
     if (!c) return fail(GT_ERR_INVALID_ARGUMENT, "null context");
     if (!dtype_size(dtype)) return fail(GT_ERR_INVALID_ARGUMENT, "unknown dtype %d", dtype);
-    if (c->der.style != GT_STYLE_TRACKPY)
-        return fail(GT_ERR_INVALID_ARGUMENT, "gt_detect_device runs the Trackpy style only");
     if (nframes == 0) return fail(GT_ERR_EMPTY_ITERATOR, "the provided video doesn't contain any frames.");
     if (!d_frames || !d_out || !d_counts || cap_per_frame == 0)
         return fail(GT_ERR_INVALID_ARGUMENT, "null device buffer or zero capacity");

@@ -207,6 +207,15 @@ gt_status gt_filter_close(const gt_result_row *rows, uint64_t n, float separatio
 gt_status gt_doughnut(const float *raw, uint32_t nrows, uint32_t ncols, const gt_result_row *rows,
                       uint64_t n, float radius, float bg_radius, float gap_radius, float *out3);
 
+/* LoG post-processing of one frame (src/execute_gpu.rs:699-712 with
+ * prune_blobs_log, :592-669): when overlap_threshold < 1, rows are stably
+ * sorted by r descending then mass descending, and each surviving row in that
+ * order drops every surviving other row with squared distance < (2 r)^2 whose
+ * disk overlap (:597-630) exceeds the threshold.  Survivors are compacted in
+ * sorted order into rows[0..*n_out).  Host-only. */
+gt_status gt_prune_blobs(gt_result_row *rows, uint64_t n, float overlap_threshold,
+                         uint64_t *n_out);
+
 /* Interruption and progress (the Arc<AtomicBool> / Arc<Mutex<(usize,
  * Option<usize>)>> pair of execute_gpu, src/execute_gpu.rs:1094-1099,
  * :974-977).  Safe to call from any thread while gt_execute runs. */

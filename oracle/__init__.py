@@ -74,6 +74,24 @@ def lib():
         L.or_bandpass_f64.argtypes = [C.c_void_p, P(Params), C.c_void_p, C.c_void_p]
         L.or_centroid_f64.argtypes = [C.c_void_p, P(Params), C.c_int32, C.c_int32, C.c_void_p,
                                       C.c_void_p, C.c_void_p]
+        L.or_walk_r.argtypes = [C.c_void_p, P(Params), C.c_void_p, C.c_void_p, C.c_size_t,
+                                C.c_void_p]
+        L.or_walk_r.restype = C.c_size_t
+        L.or_log_radii.argtypes = [C.c_float, C.c_float, C.c_uint32, C.c_int, C.c_void_p]
+        L.or_log_taps.argtypes = [C.c_float, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.or_log_taps.restype = C.c_int32
+        L.or_log_plane.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_float, C.c_void_p,
+                                   C.c_void_p, C.c_void_p]
+        L.or_log_max.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32,
+                                 C.c_int32, C.c_float, C.c_void_p, C.c_size_t]
+        L.or_log_max.restype = C.c_size_t
+        L.or_blob_overlap.argtypes = [C.c_float, C.c_float, C.c_float]
+        L.or_blob_overlap.restype = C.c_float
+        L.or_prune_blobs.argtypes = [C.c_void_p, C.c_size_t, C.c_float]
+        L.or_prune_blobs.restype = C.c_size_t
+        L.or_log_detect_frame.argtypes = [C.c_void_p, P(Params), C.c_void_p, C.c_uint32, C.c_int,
+                                          C.c_float, C.c_void_p, C.c_size_t, C.c_void_p]
+        L.or_log_detect_frame.restype = C.c_size_t
         _lib = L
     return _lib
 
@@ -209,3 +227,102 @@ def centroid_f64(processed64, p: Params, pu: int, pv: int):
     dx, dy, m = C.c_double(), C.c_double(), C.c_double()
     lib().or_centroid_f64(_ptr(P_), C.byref(p), pu, pv, C.byref(dx), C.byref(dy), C.byref(m))
     return dx.value, dy.value, m.value
+
+
+# ---------------------------------------------------------------------------
+# LoG style
+
+
+def derive_log(nrows: int, ncols: int, max_radius: float, smoothing_size: int | None = None,
+               noise_size: float = 1.0, minmass: float = 0.0, max_iterations: int = 10,
+               shift_threshold: float = 0.6, truncate_preprocessed: bool = True) -> Params:
+    """gpu_setup.rs:549-557 + :562-579 (Log branch): smoothing side
+    2 * u32(max_radius + 0.5) + 1 unless given; no circle, dilation or margin."""
+    p = Params()
+    if smoothing_size is None:
+        rad = int(np.float32(max_radius) + np.float32(0.5)) if max_radius > -0.5 else 0
+        smoothing_size = 2 * rad + 1
+    p.nrows, p.ncols = nrows, ncols
+    p.smoothing = smoothing_size
+    p.sigma2 = noise_size
+    p.circle = 0
+    p.dilation = 0
+    p.max_iterations = max_iterations
+    p.threshold = shift_threshold
+    p.minmass = minmass
+    p.margin = 0
+    p.snr = 0.0
+    p.minmass_snr = 0.0
+    p.rough_snr_factor = 1.0
+    p.image_std = 0.0
+    p.truncate = int(truncate_preprocessed)
+    return p
+
+
+def log_radii(min_r: float, max_r: float, n: int, log_spacing: bool) -> np.ndarray:
+    out = np.zeros(max(n, 1), np.float32)
+    lib().or_log_radii(min_r, max_r, n, int(log_spacing), _ptr(out))
+    return out[:n]
+
+
+def log_taps(sigma: float):
+    """(R, g, d, norm1, norm3) of one LoG scale."""
+    R = lib().or_log_taps(sigma, None, None, None, None)
+    g = np.zeros(2 * R + 1, np.float32)
+    d = np.zeros(2 * R + 1, np.float32)
+    n1, n3 = C.c_float(), C.c_float()
+    lib().or_log_taps(sigma, _ptr(g), _ptr(d), C.byref(n1), C.byref(n3))
+    return R, g, d, n1.value, n3.value
+
+
+def log_plane(raw, sigma: float) -> np.ndarray:
+    f = _f32(raw)
+    L = np.empty_like(f)
+    t1 = np.empty_like(f)
+    t2 = np.empty_like(f)
+    lib().or_log_plane(_ptr(f), f.shape[0], f.shape[1], sigma, _ptr(L), _ptr(t1), _ptr(t2))
+    return L
+
+
+def log_max(bottom, middle, top, processed, snr_threshold: float = 0.0) -> np.ndarray:
+    M = _f32(middle)
+    B = None if bottom is None else _f32(bottom)
+    T = None if top is None else _f32(top)
+    P_ = _f32(processed)
+    cand = np.empty(M.size, np.uint32)
+    n = lib().or_log_max(None if B is None else _ptr(B), _ptr(M), None if T is None else _ptr(T),
+                         _ptr(P_), M.shape[0], M.shape[1], snr_threshold, _ptr(cand), cand.size)
+    return cand[:n].copy()
+
+
+def walk_r(processed, p: Params, cand, radii) -> np.ndarray:
+    P_ = _f32(processed)
+    cand = np.ascontiguousarray(cand, np.uint32)
+    rr = np.ascontiguousarray(radii, np.float32)
+    rows = np.zeros(max(len(cand), 1), ROW_DTYPE)
+    n = lib().or_walk_r(_ptr(P_), C.byref(p), _ptr(cand), _ptr(rr), len(cand), _ptr(rows))
+    return rows[:n].copy()
+
+
+def blob_overlap(r1: float, r2: float, distance: float) -> float:
+    return lib().or_blob_overlap(r1, r2, distance)
+
+
+def prune_blobs(rows, overlap_threshold: float) -> np.ndarray:
+    rows = np.ascontiguousarray(rows, ROW_DTYPE).copy()
+    n = lib().or_prune_blobs(_ptr(rows), len(rows), overlap_threshold)
+    return rows[:n].copy()
+
+
+def log_detect_frame(frame, p: Params, radii, characterize: bool = True,
+                     overlap_threshold: float = 0.0):
+    """LoG frame: rows in the pruned (r desc, mass desc) order.  Returns
+    (rows, n_candidates); seed = scale * H * W + raster index."""
+    f = _f32(frame)
+    rr = np.ascontiguousarray(radii, np.float32)
+    cap = f.size * max(len(rr), 1)
+    rows = np.zeros(cap, ROW_DTYPE)
+    nc = C.c_size_t(0)
+    n = lib().or_log_detect_frame(_ptr(f), C.byref(p), _ptr(rr), len(rr), int(characterize),
+                                  overlap_threshold, _ptr(rows), cap, C.byref(nc))
+    return rows[:n].copy(), nc.value

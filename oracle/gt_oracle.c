@@ -178,12 +178,28 @@ static or_walkres or_get_center(const float *P, const or_params *p, int32_t u, i
 
 size_t or_walk(const float *P, const or_params *p, const uint32_t *cand, size_t ncand,
                or_row *rows) {
-    /* walk.wgsl:104-169 with r = -1 (rint = (circle-1)/2), then main :171-229 */
-    const int32_t rint = (p->circle - 1) / 2;
-    const float rf = (float)rint, r2 = rf * rf;
-    const int32_t margin = rint;
+    return or_walk_r(P, p, cand, NULL, ncand, rows);
+}
+
+size_t or_walk_r(const float *P, const or_params *p, const uint32_t *cand, const float *cand_r,
+                 size_t ncand, or_row *rows) {
+    /* walk.wgsl:104-169 (radius: r <= 0 -> rint = (circle-1)/2, r2 = rint^2;
+     * r > 0 -> rint = ceil(r), r2 = r*r; the move band is rint), then main
+     * :171-229 (minmass; row.r = r when r > 0) */
     size_t n = 0;
     for (size_t c = 0; c < ncand; c++) {
+        const float r = cand_r ? cand_r[c] : -1.0f;
+        int32_t rint;
+        float r2;
+        if (r <= 0.0f) {
+            rint = (p->circle - 1) / 2;
+            const float rf = (float)rint;
+            r2 = rf * rf;
+        } else {
+            rint = or_f2i(ceilf(r));
+            r2 = r * r;
+        }
+        const int32_t margin = rint;
         int32_t pu = (int32_t)(cand[c] / (uint32_t)p->ncols);
         int32_t pv = (int32_t)(cand[c] % (uint32_t)p->ncols);
         or_walkres res = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
@@ -209,6 +225,7 @@ size_t or_walk(const float *P, const or_params *p, const uint32_t *cand, size_t 
             o->mass = res.mass;
             o->count = res.count;
             o->max_intensity = res.max;
+            if (r > 0.0f) o->r = r;
             o->seed = cand[c];
         }
     }
@@ -501,4 +518,246 @@ void or_centroid_f64(const double *P, const or_params *p, int32_t pu0, int32_t p
     *dx = sx / m;
     *dy = sy / m;
     *mass = m;
+}
+
+/* ------------------------------------------------------------------------- */
+/* LoG style                                                                 */
+
+void or_log_radii(float min_r, float max_r, uint32_t n, int log_spacing, float *out) {
+    /* gpu_setup.rs:958-984: the start value advances by repeated f32
+     * addition; with log spacing each radius is 10^start. */
+    if (n == 0) return;
+    if (log_spacing) {
+        float start = log10f(min_r);
+        const float end = log10f(max_r);
+        const float diff = (end - start) / (float)(n - 1);
+        for (uint32_t k = 0; k + 1 < n; k++) {
+            const float t = start;
+            start = start + diff;
+            out[k] = powf(10.0f, t);
+        }
+        out[n - 1] = powf(10.0f, start);
+    } else {
+        float start = min_r;
+        const float diff = (max_r - min_r) / (float)(n - 1);
+        for (uint32_t k = 0; k + 1 < n; k++) {
+            const float t = start;
+            start = start + diff;
+            out[k] = t;
+        }
+        out[n - 1] = start;
+    }
+}
+
+int32_t or_log_taps(float sigma, float *g, float *d, float *norm1, float *norm3) {
+    /* separable_log.wgsl:22-89: rint = i32(4 sigma + 0.5); tap weights
+     * exp(-x^2 / (2 sigma^2)) and exp(-x^2 / (2 sigma^2)) * (x^2 - sigma^2);
+     * norms 1/(sqrt(2 pi) sigma) and 1/(sqrt(2 pi) sigma sigma^2). */
+    const float sigma2 = sigma * sigma;
+    const int32_t R = or_f2i(4.0f * sigma + 0.5f);
+    for (int32_t i = -R; i <= R; i++) {
+        const float x = (float)i, x2 = x * x;
+        const float e = expf(-x2 / (2.0f * sigma2));
+        if (g) g[i + R] = e;
+        if (d) d[i + R] = e * (x2 - sigma2);
+    }
+    const float sqrt2pi = 2.5066282746310002f;
+    if (norm1) *norm1 = 1.0f / (sqrt2pi * sigma);
+    if (norm3) *norm3 = 1.0f / (sqrt2pi * sigma * sigma2);
+    return R;
+}
+
+/* One 1-D pass of separable_log.wgsl: along rows (dim 0) or columns (dim 1),
+ * taps w[0..2R], out-of-frame taps skipped, result = sum * norm. */
+static void or_log_pass(const float *in, int32_t H, int32_t W, int dim, const float *w, int32_t R,
+                        float norm, float *out) {
+    for (int32_t u = 0; u < H; u++)
+        for (int32_t v = 0; v < W; v++) {
+            float s = 0.0f;
+            for (int32_t i = -R; i <= R; i++) {
+                const int32_t pu = dim == 0 ? u + i : u, pv = dim == 0 ? v : v + i;
+                if (pu < 0 || pu >= H || pv < 0 || pv >= W) continue;
+                s = fmaf(w[i + R], in[(size_t)pu * W + pv], s);
+            }
+            out[(size_t)u * W + v] = s * norm;
+        }
+}
+
+void or_log_plane(const float *raw, int32_t H, int32_t W, float sigma, float *L, float *tmp,
+                  float *tmp2) {
+    /* Laplace<2> over SeparableConvolution<2> (utils.rs:114-296) with
+     * separable_log.wgsl: for diff_dim 0 then 1, a dim-0 pass of the raw
+     * frame into tmp and a dim-1 pass of tmp subtracted from the output
+     * (zeroed by the first pass).  The second-derivative taps go on the
+     * pass whose dim equals diff_dim. */
+    int32_t R = or_log_taps(sigma, NULL, NULL, NULL, NULL);
+    float *g = (float *)malloc(sizeof(float) * (2 * R + 1));
+    float *d = (float *)malloc(sizeof(float) * (2 * R + 1));
+    float n1, n3;
+    or_log_taps(sigma, g, d, &n1, &n3);
+    const size_t hw = (size_t)H * W;
+    /* diff_dim 0 */
+    or_log_pass(raw, H, W, 0, d, R, n3, tmp);
+    or_log_pass(tmp, H, W, 1, g, R, n1, tmp2);
+    for (size_t k = 0; k < hw; k++) L[k] = 0.0f - tmp2[k];
+    /* diff_dim 1 */
+    or_log_pass(raw, H, W, 0, g, R, n1, tmp);
+    or_log_pass(tmp, H, W, 1, d, R, n3, tmp2);
+    for (size_t k = 0; k < hw; k++) L[k] = L[k] - tmp2[k];
+    free(g);
+    free(d);
+}
+
+/* is_max_in_plane (logspace_max.wgsl:62-92) */
+static int or_is_max_in_plane(const float *P, int32_t H, int32_t W, int32_t i, int32_t j,
+                              float center) {
+    for (int32_t a = -1; a < 2; a++)
+        for (int32_t b = -1; b < 2; b++) {
+            if (a == 0 && b == 0) continue;
+            const int32_t x = i + a, y = j + b;
+            if (x < 0 || x >= H || y < 0 || y >= W) return 0;
+            if (P[(size_t)x * W + y] >= center) return 0;
+        }
+    return 1;
+}
+
+size_t or_log_max(const float *bottom, const float *middle, const float *top,
+                  const float *processed, int32_t H, int32_t W, float snr_threshold,
+                  uint32_t *cand, size_t cap) {
+    /* logspace_max.wgsl:94-154: processed >= std*snr*rough, centre > 0 and
+     * strictly above its 8 neighbours in its own plane and in the planes
+     * below / above (NULL = no such plane); every neighbour in the frame. */
+    size_t n = 0;
+    for (int32_t i = 0; i < H; i++)
+        for (int32_t j = 0; j < W; j++) {
+            const size_t idx = (size_t)i * W + j;
+            if (processed[idx] < snr_threshold) continue;
+            const float c = middle[idx];
+            if (c <= 0.0f) continue;
+            if (!or_is_max_in_plane(middle, H, W, i, j, c)) continue;
+            if (bottom && !or_is_max_in_plane(bottom, H, W, i, j, c)) continue;
+            if (top && !or_is_max_in_plane(top, H, W, i, j, c)) continue;
+            if (n < cap) cand[n] = (uint32_t)idx;
+            n++;
+        }
+    return n;
+}
+
+/* disk_overlap / blob_overlap (execute_gpu.rs:597-630), f32 as in Rust */
+static float or_make_acos(float d, float r1, float r1s, float r2s) {
+    float ratio = (d * d + r1s - r2s) / (2.0f * d * r1);
+    ratio = ratio < -1.0f ? -1.0f : (ratio > 1.0f ? 1.0f : ratio);
+    return acosf(ratio);
+}
+
+float or_blob_overlap(float r1, float r2, float distance) {
+    if (distance > r1 + r2) return 0.0f;
+    if (distance <= fabsf(r1 - r2)) return 1.0f;
+    const float d = distance;
+    const float r1s = r1 * r1, r2s = r2 * r2;
+    const float acos1 = or_make_acos(d, r1, r1s, r2s);
+    const float acos2 = or_make_acos(d, r2, r2s, r1s);
+    const float a = -d + r2 + r1;
+    const float b = d - r2 + r1;
+    const float c = d + r2 - r1;
+    const float e = d + r2 + r1;
+    const float area = r1s * acos1 + r2s * acos2 - 0.5f * sqrtf(fabsf(a * b * c * e));
+    const float pi = 3.14159265358979323846f;
+    return area / (pi * (r1s < r2s ? r1s : r2s));
+}
+
+size_t or_prune_blobs(or_row *rows, size_t n, float overlap_threshold) {
+    /* post_process (execute_gpu.rs:699-712): when overlap_threshold < 1,
+     * a stable sort by r descending, mass descending, then prune_blobs_log
+     * (:592-669): each surviving blob in that order drops every surviving
+     * other blob within distance^2 < (2 r)^2 whose overlap exceeds the
+     * threshold.  Survivors are compacted in sorted order; returns their
+     * count.  O(n^2). */
+    if (!(overlap_threshold < 1.0f)) return n;
+    /* stable insertion-merge sort on (r desc, mass desc) */
+    or_row *tmp = (or_row *)malloc(sizeof(or_row) * (n ? n : 1));
+    for (size_t w = 1; w < n; w *= 2) {
+        for (size_t lo = 0; lo < n; lo += 2 * w) {
+            size_t mid = lo + w < n ? lo + w : n, hi = lo + 2 * w < n ? lo + 2 * w : n;
+            size_t a = lo, b = mid, k = lo;
+            while (a < mid && b < hi) {
+                const or_row *x = &rows[a], *y = &rows[b];
+                /* take y first only if it sorts strictly before x */
+                int ybefore = (y->r > x->r) || (y->r == x->r && y->mass > x->mass);
+                tmp[k++] = ybefore ? rows[b++] : rows[a++];
+            }
+            while (a < mid) tmp[k++] = rows[a++];
+            while (b < hi) tmp[k++] = rows[b++];
+        }
+        memcpy(rows, tmp, sizeof(or_row) * n);
+    }
+    free(tmp);
+    uint8_t *keep = (uint8_t *)malloc(n ? n : 1);
+    memset(keep, 1, n);
+    for (size_t q = 0; q < n; q++) {
+        if (!keep[q]) continue;
+        const float rad = rows[q].r * 2.0f, r2 = rad * rad;
+        for (size_t k = 0; k < n; k++) {
+            if (k == q || !keep[k]) continue;
+            const float dx = rows[k].x - rows[q].x, dy = rows[k].y - rows[q].y;
+            const float d2 = 0.0f + dx * dx + dy * dy;
+            if (!(d2 < r2)) continue;
+            if (or_blob_overlap(rows[q].r, rows[k].r, sqrtf(d2)) > overlap_threshold) keep[k] = 0;
+        }
+    }
+    size_t m = 0;
+    for (size_t k = 0; k < n; k++)
+        if (keep[k]) rows[m++] = rows[k];
+    free(keep);
+    return m;
+}
+
+size_t or_log_detect_frame(const float *frame, const or_params *p, const float *radii,
+                           uint32_t n_radii, int characterize, float overlap_threshold,
+                           or_row *rows, size_t cap, size_t *ncand_out) {
+    /* execute_gpu.rs:237-286 (bandpass, then per scale the LoG plane, the
+     * scale-space maximum of the previous scale and its walk) and the LoG
+     * branch of post_process.  Rows before pruning are in (scale, raster)
+     * order; seed = scale * H * W + raster. */
+    const int32_t H = p->nrows, W = p->ncols;
+    const size_t hw = (size_t)H * W;
+    float *proc = (float *)malloc(sizeof(float) * hw * 3);
+    or_bandpass(frame, p, proc, proc + hw);
+    float *planes = (float *)malloc(sizeof(float) * hw * 3);
+    float *t1 = (float *)malloc(sizeof(float) * hw), *t2 = (float *)malloc(sizeof(float) * hw);
+    uint32_t *cand = (uint32_t *)malloc(sizeof(uint32_t) * (hw ? hw : 1));
+    float *cr = (float *)malloc(sizeof(float) * (hw ? hw : 1));
+    size_t total = 0, ncand = 0;
+    or_row *all = NULL;
+    const float sq2 = sqrtf(2.0f);
+    for (uint32_t m = 0; m < n_radii; m++) {
+        /* planes needed: m-1, m, m+1 (ring of 3 by scale index) */
+        if (m == 0) or_log_plane(frame, H, W, radii[0] / sq2, planes, t1, t2);
+        if (m + 1 < n_radii)
+            or_log_plane(frame, H, W, radii[m + 1] / sq2, planes + ((m + 1) % 3) * hw, t1, t2);
+        const float *mid = planes + (m % 3) * hw;
+        const float *bot = m > 0 ? planes + ((m + 2) % 3) * hw : NULL;
+        const float *top = m + 1 < n_radii ? planes + ((m + 1) % 3) * hw : NULL;
+        const float thr = p->image_std * p->snr * p->rough_snr_factor;
+        size_t nc = or_log_max(bot, mid, top, proc, H, W, thr, cand, hw);
+        ncand += nc;
+        for (size_t c = 0; c < nc; c++) cr[c] = radii[m];
+        all = (or_row *)realloc(all, sizeof(or_row) * (total + nc + 1));
+        size_t nr = or_walk_r(proc, p, cand, cr, nc, all + total);
+        if (characterize) or_characterize(proc, frame, p, all + total, nr, 1, 0);
+        for (size_t k = 0; k < nr; k++) all[total + k].seed += (uint32_t)(m * hw);
+        total += nr;
+    }
+    if (ncand_out) *ncand_out = ncand;
+    size_t n = or_prune_blobs(all, total, overlap_threshold);
+    if (n <= cap) memcpy(rows, all, sizeof(or_row) * n);
+    else n = (size_t)-1;
+    free(all);
+    free(cr);
+    free(cand);
+    free(t1);
+    free(t2);
+    free(planes);
+    free(proc);
+    return n;
 }

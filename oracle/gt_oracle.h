@@ -107,6 +107,37 @@ int or_detect_batch(const void *frames, int dtype, size_t nframes, const or_para
                     int characterize, int nthreads, or_row *rows, size_t cap_per_frame,
                     uint64_t *counts, uint64_t *ncands);
 
+/* walk.wgsl:104-229 with a radius per candidate (cand_r[c] > 0: LoG,
+ * rint = ceil(r), r2 = r*r; cand_r == NULL: Trackpy, r = -1). */
+size_t or_walk_r(const float *processed, const or_params *p, const uint32_t *cand,
+                 const float *cand_r, size_t ncand, or_row *rows);
+
+/* ---- LoG style ---- */
+/* The radius scan (gpu_setup.rs:958-984). */
+void or_log_radii(float min_r, float max_r, uint32_t n, int log_spacing, float *out);
+/* Tap weights of one scale (separable_log.wgsl:22-89): returns R; g, d hold
+ * 2R+1 floats (Gaussian and second-derivative taps) when not NULL. */
+int32_t or_log_taps(float sigma, float *g, float *d, float *norm1, float *norm3);
+/* Scale-normalised LoG plane L = -(sigma^2)(Gxx + Gyy) * raw, by the four 1-D
+ * passes of Laplace<2> (utils.rs:114-296).  tmp, tmp2: H*W floats each. */
+void or_log_plane(const float *raw, int32_t H, int32_t W, float sigma, float *L, float *tmp,
+                  float *tmp2);
+/* Scale-space maxima of `middle` (logspace_max.wgsl:62-154); bottom / top
+ * NULL at the ends of the scan.  Raster order; returns the count (may exceed cap). */
+size_t or_log_max(const float *bottom, const float *middle, const float *top,
+                  const float *processed, int32_t H, int32_t W, float snr_threshold,
+                  uint32_t *cand, size_t cap);
+/* blob_overlap (execute_gpu.rs:597-630). */
+float or_blob_overlap(float r1, float r2, float distance);
+/* Sort (r desc, mass desc, stable) + prune_blobs_log (execute_gpu.rs:592-712);
+ * compacts rows in place and returns the survivor count. */
+size_t or_prune_blobs(or_row *rows, size_t n, float overlap_threshold);
+/* Whole LoG frame (execute_gpu.rs:237-286 + the LoG branch of post_process);
+ * rows come out in the pruned, sorted order; returns their count or (size_t)-1. */
+size_t or_log_detect_frame(const float *frame, const or_params *p, const float *radii,
+                           uint32_t n_radii, int characterize, float overlap_threshold,
+                           or_row *rows, size_t cap, size_t *ncand_out);
+
 /* FP64 twin for error reporting: f64 bandpass (exp in f64, exact divisions). */
 void or_bandpass_f64(const float *frame, const or_params *p, double *processed, double *scratch);
 /* FP64 centroid of the disk at integer position (pu, pv) of a f64 frame. */

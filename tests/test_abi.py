@@ -134,3 +134,38 @@ def test_invalid_parameters_rejected():
     p.has_smoothing_size, p.smoothing_size = 1, 41
     assert N.lib().gt_context_create(C.byref(p), 64, 64, 0, C.byref(h)) == 100
     assert b"smoothing_size" in N.lib().gt_last_error()
+
+
+@pytest.mark.parametrize("seed,n,thr", [(0, 500, 0.0), (1, 300, 0.3), (2, 40, 0.0), (3, 1, 0.0),
+                                        (4, 200, 1.0)])
+def test_prune_blobs_matches_oracle(oracle, seed, n, thr):
+    rng = np.random.default_rng(seed)
+    rows = np.zeros(n, oracle.ROW_DTYPE)
+    rows["x"] = rng.uniform(0, 80, n)
+    rows["y"] = rng.uniform(0, 80, n)
+    rows["r"] = rng.choice(np.float32([1.5, 1.75, 2.04, 3.0, 6.0]), n)
+    rows["mass"] = rng.integers(0, 4, n)             # ties in r and mass
+    rows["seed"] = np.arange(n)
+    if n > 20:
+        rows["x"][::19] = np.nan                      # zero-mass walks: never neighbours
+    ref = oracle.prune_blobs(rows, thr)
+    rr = _as_result_rows(rows)
+    nout = C.c_uint64()
+    N.check(N.lib().gt_prune_blobs(rr.ctypes.data, n, thr, C.byref(nout)))
+    got = rr[:nout.value]
+    assert nout.value == len(ref)
+    for f in ("x", "y", "r", "mass"):
+        assert np.array_equal(got[f].view(np.uint32), ref[f].view(np.uint32)), f
+
+
+def test_log_parameters_validated():
+    import gpu_tracking_amd as gt
+    h = C.c_void_p()
+    for kw in (dict(n_radii=0), dict(n_radii=65)):
+        p = gt.make_log_params(1.5, 6.0, **kw)
+        assert N.lib().gt_context_create(C.byref(p), 64, 64, 0, C.byref(h)) == 100
+    p = gt.make_log_params(0.0, 6.0)
+    assert N.lib().gt_context_create(C.byref(p), 64, 64, 0, C.byref(h)) == 100
+    p = gt.make_log_params(1.5, 40.0, smoothing_size=9)     # sigma 28: taps beyond 48
+    assert N.lib().gt_context_create(C.byref(p), 64, 64, 0, C.byref(h)) == 100
+    assert b"radius" in N.lib().gt_last_error()
