@@ -12,7 +12,12 @@ HIP events around each stage on the launch stream), and the CPU oracle's
 frames/s on a bounded sample of the same frames on the host cores, plus a
 bit-for-bit parity check of that sample.
 
-    python bench.py [--steps K] [--warmup W] [--workload full|small]
+    python bench.py [--steps K] [--warmup W] [--workload full|small|log]
+
+Workload "log": 500 synthetic 2048x2048 float32 frames, 3000 particles per
+frame (sigma ~ U[1, 5], A ~ U[200, 2000], background 50 + N(0, 5^2)), LoG
+style with radii 1.5..6 in 10 log-spaced scales, characterize on; one step is
+bandpass -> scale-space maxima of all scales -> walk + characterize.
 """
 import argparse
 import json
@@ -25,8 +30,12 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 
+LOG_PARAMS = dict(min_radius=1.5, max_radius=6.0, n_radii=10, log_spacing=True,
+                  characterize=True)
+
 WORKLOADS = {
     # name: (synth spec, frames, params)
+    "log": ("log", 500, LOG_PARAMS),
     "full": ("full", 2000, dict(diameter=9, separation=7, minmass=400.0, characterize=True)),
     "small": ("small", 100, dict(diameter=9, separation=9, minmass=500.0, noise_size=1.0,
                                  smoothing_size=9, max_iterations=10, characterize=True)),
@@ -68,6 +77,81 @@ def _cpu_baseline(video_host, params, budget_s, gpu_rows):
         "value": round(done / dt, 3), "unit": "frames/s", "cores": nthreads, "kind": "port",
         "sample": f"first {done} frames of the workload, detect+characterize, oracle C code "
                   f"(-O2, one frame per thread) on {nthreads} host threads, {dt:.1f} s",
+    }, done, mismatched
+
+
+def _log_taps_per_pixel(params):
+    """Sum over the scales of 2R+1, R = i32(4 sigma + 0.5), sigma = r / sqrt(2)
+    (the radius scan of gpu_setup.rs:958-984 in f32)."""
+    import numpy as np
+    f = np.float32
+    n = params["n_radii"]
+    lo, hi = f(params["min_radius"]), f(params["max_radius"])
+    if params["log_spacing"]:
+        start, end = np.log10(lo), np.log10(hi)
+        diff = f((end - start) / f(n - 1))
+        radii = []
+        for _ in range(n - 1):
+            radii.append(f(10.0) ** start)
+            start = f(start + diff)
+        radii.append(f(10.0) ** start)
+    else:
+        diff = f((hi - lo) / f(n - 1))
+        radii, start = [], lo
+        for _ in range(n - 1):
+            radii.append(start)
+            start = f(start + diff)
+        radii.append(start)
+    return sum(2 * int(f(4.0) * f(r / np.sqrt(f(2.0))) + f(0.5)) + 1 for r in radii)
+
+
+def _cpu_baseline_log(video_host, params, budget_s, gpu_rows):
+    """The oracle's LoG frame (single-threaded C per frame) on host threads."""
+    import threading
+
+    import numpy as np
+
+    import oracle
+    T, H, W = video_host.shape
+    try:
+        ncores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        ncores = os.cpu_count() or 1
+    nthreads = max(1, min(16, ncores))
+    op = oracle.derive_log(H, W, params["max_radius"])
+    radii = oracle.log_radii(params["min_radius"], params["max_radius"], params["n_radii"],
+                             params["log_spacing"])
+    results = {}
+
+    def work(t):
+        # unpruned rows in seed order: the device path's output before the
+        # host post-processing
+        results[t], _ = oracle.log_detect_frame(video_host[t], op, radii, characterize=True,
+                                                overlap_threshold=1.0)
+
+    done = 0
+    t0 = time.perf_counter()
+    while done < T and time.perf_counter() - t0 < budget_s:
+        n = min(T - done, nthreads)
+        ths = [threading.Thread(target=work, args=(done + k,)) for k in range(n)]
+        for th in ths:
+            th.start()
+        for th in ths:
+            th.join()
+        done += n
+    dt = time.perf_counter() - t0
+    mismatched = 0
+    fields = ["x", "y", "mass", "r", "max_intensity", "Rg", "raw_mass", "signal", "ecc", "count"]
+    for t in range(done):
+        ref, got = results[t], gpu_rows[t]
+        if len(ref) != len(got) or not np.array_equal(ref["seed"], got["seed"]) or not all(
+                np.array_equal(ref[f], got[f], equal_nan=True) for f in fields):
+            mismatched += 1
+    return {
+        "value": round(done / dt, 3), "unit": "frames/s", "cores": nthreads, "kind": "port",
+        "sample": f"first {done} frames of the workload, LoG detect+characterize (no pruning), "
+                  f"oracle C code (-O2, one frame per thread) on {nthreads} host threads, "
+                  f"{dt:.1f} s",
     }, done, mismatched
 
 
@@ -115,9 +199,18 @@ def main():
         torch.cuda.synchronize()
         del probe_src, dst
 
-    p = gt.make_params(params["diameter"], **{k: v for k, v in params.items() if k != "diameter"})
+    is_log = args.workload == "log"
+    if is_log:
+        p = gt.make_log_params(params["min_radius"], params["max_radius"],
+                               **{k: v for k, v in params.items()
+                                  if k not in ("min_radius", "max_radius")})
+    else:
+        p = gt.make_params(params["diameter"],
+                           **{k: v for k, v in params.items() if k != "diameter"})
     ctx = gt.Context(p, H, W)
-    cap = 8192
+    # detections per frame: ~1000 on the plain workloads; LoG keeps every
+    # scale-space maximum of a noisy frame (no threshold), ~7% of the pixels
+    cap = (H * W) // 8 if is_log else 8192
     out = torch.empty((T, cap, 12), dtype=torch.int32, device="cuda")
     counts = torch.zeros((T,), dtype=torch.int32, device="cuda")
     stream = torch.cuda.current_stream().cuda_stream
@@ -146,10 +239,18 @@ def main():
         raise SystemExit(f"detection capacity {cap} exceeded: {int(counts.max().item())}")
 
     # roofline of the dominant kernel; algorithmic bytes per pixel:
-    # bandpass 2 (u16 in) + 4 (f32 out); detect (local max + walk + characterize)
-    # 4 (f32 in); finish (walks handed over + counts) ~0 per pixel
-    names = ["bandpass", "detect", "finish"]
-    alg_bpp = {"bandpass": 6.0, "detect": 4.0, "finish": 0.0}
+    # plain: bandpass 2 (u16 in) + 4 (f32 out); detect (local max + walk +
+    # characterize) 4 (f32 in); finish (walks handed over + counts) ~0.
+    # LoG: bandpass 4 + 4; log_tile 4 (raw f32 read once; the scale planes
+    # stay in LDS); walk ~0.
+    if is_log:
+        names = ["bandpass", "log_tile", "walk"]
+        alg_bpp = {"bandpass": 8.0, "log_tile": 4.0, "walk": 0.0}
+        pipe_bpp = 4 + 8 + 8 * params["n_radii"]
+    else:
+        names = ["bandpass", "detect", "finish"]
+        alg_bpp = {"bandpass": 6.0, "detect": 4.0, "finish": 0.0}
+        pipe_bpp = 10
     k = int(np.argmax(stage_ms)) if stage_ms else 0
     kname = names[k]
     launch_ms = stage_ms[k] / nlaunch if stage_ms else float("nan")
@@ -165,26 +266,44 @@ def main():
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
             "kernel": kname, "launch_ms": round(launch_ms, 4), "frames_per_launch": chunk,
             "stage_ms_per_step": [round(x, 3) for x in stage_ms],
-            "pipeline_alg_bytes_per_frame": 10 * H * W,
-            "pipeline_frac": round(10 * H * W * fps / (HBM_PEAK_GBS * 1e9), 4)}
+            "pipeline_alg_bytes_per_frame": pipe_bpp * H * W,
+            "pipeline_frac": round(pipe_bpp * H * W * fps / (HBM_PEAK_GBS * 1e9), 4)}
+    if is_log and stage_ms:
+        # the scale-space kernel is FP32-FMA bound: 4 separable passes of
+        # 2R+1 taps per pixel per scale (8 (2R+1) flop), against the vector
+        # FP32 peak 256 CU x 64 lanes x 2 flop x 2.4 GHz = 78.6 TFLOP/s
+        taps = _log_taps_per_pixel(params)
+        flop_frame = 8.0 * taps * H * W
+        tk = stage_ms[1] / nlaunch * 1e-3
+        roof["valu"] = {"kernel": "log_tile", "alg_flop_per_frame": flop_frame,
+                        "achieved_tflops": round(flop_frame * chunk / tk / 1e12, 2),
+                        "peak_tflops": 78.6,
+                        "frac": round(flop_frame * chunk / tk / 1e12 / 78.6, 4)}
 
     cpu = None
     parity = None
     if not args.no_cpu:
         rows = gt.detections_to_numpy(out[:64], counts[:64])
-        host = frames[:64].cpu().numpy().view(np.uint16)
-        print("[bench] CPU baseline", file=sys.stderr, flush=True)
-        cpu, nchk, bad = _cpu_baseline(host, params, args.cpu_seconds, rows)
+        if is_log:
+            host = frames[:64].cpu().numpy()
+            print("[bench] CPU baseline", file=sys.stderr, flush=True)
+            cpu, nchk, bad = _cpu_baseline_log(host, params, args.cpu_seconds, rows)
+        else:
+            host = frames[:64].cpu().numpy().view(np.uint16)
+            print("[bench] CPU baseline", file=sys.stderr, flush=True)
+            cpu, nchk, bad = _cpu_baseline(host, params, args.cpu_seconds, rows)
         parity = {"frames_checked": nchk, "frames_bit_identical": nchk - bad}
 
     res = {
         "metric": "frames/s (detect+characterize, 1024^2 u16)" if args.workload == "full"
-        else f"frames/s (detect+characterize, {H}x{W} u16)",
+        else (f"frames/s (LoG detect+characterize, {H}x{W} f32)" if is_log
+              else f"frames/s (detect+characterize, {H}x{W} u16)"),
         "value": round(fps, 1), "unit": "frames/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(dt / args.steps * 1e3, 4),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic (seeded, rendered on the GPU)",
-        "config": {"workload": f"{args.workload}: {T} frames {H}x{W} u16, "
+        "config": {"workload": f"{args.workload}: {T} frames {H}x{W} "
+                               f"{'f32' if is_log else 'u16'}, "
                                f"{s.n_particles} particles/frame, " +
                                ", ".join(f"{k}={v}" for k, v in params.items())},
         "roofline": roof, "cpu_baseline": cpu, "parity": parity,
