@@ -10,8 +10,9 @@ The package needs the built library; importing the entry points raises when
 it is missing (no CPU fallback).
 """
 from .api import (Context, LoG, batch, characterize_points, detections_to_numpy,  # noqa: F401
-                  make_log_params, make_params)
+                  link, make_log_params, make_params)
 from ._native import GtError, column_names, default_params  # noqa: F401
 
-__all__ = ["batch", "characterize_points", "LoG", "Context", "make_params", "make_log_params", "GtError",
+__all__ = ["batch", "characterize_points", "LoG", "link", "Context", "make_params",
+           "make_log_params", "GtError",
            "column_names", "default_params", "detections_to_numpy"]

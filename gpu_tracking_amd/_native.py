@@ -84,7 +84,7 @@ EXPORTS = [
     "gt_params_default", "gt_abi_version", "gt_last_error", "gt_context_create",
     "gt_context_destroy", "gt_detect_device", "gt_execute", "gt_output_free",
     "gt_column_names", "gt_filter_close", "gt_doughnut", "gt_set_interrupt", "gt_progress",
-    "gt_set_timing", "gt_stage_times", "gt_prune_blobs",
+    "gt_set_timing", "gt_stage_times", "gt_prune_blobs", "gt_link",
 ]
 
 _lib = None
@@ -134,6 +134,8 @@ def lib() -> C.CDLL:
     L.gt_doughnut.restype = C.c_int
     L.gt_prune_blobs.argtypes = [C.c_void_p, C.c_uint64, C.c_float, C.c_void_p]
     L.gt_prune_blobs.restype = C.c_int
+    L.gt_link.argtypes = [C.c_void_p, C.c_uint64, C.c_float, C.c_uint32, C.c_void_p]
+    L.gt_link.restype = C.c_int
     L.gt_set_interrupt.argtypes = [C.c_void_p, C.c_int32]
     L.gt_set_interrupt.restype = C.c_int
     L.gt_progress.argtypes = [C.c_void_p, P(C.c_uint64), P(C.c_uint64)]

@@ -354,3 +354,27 @@ def LoG(file_or_array, min_radius, max_radius, **kwargs):
     with Context(p, video.shape[1], video.shape[2]) as ctx:
         data, names, types = ctx.execute(video, keys=frames, linker_reset_points=reset)
     return _to_dataframe(data, names, types)
+
+
+def link(to_link, search_range, memory=0):
+    """Links detections into trajectories (lib.py:74-106): to_link is a
+    DataFrame with columns frame, y, x (sorted by frame) or an (N, 3) array
+    of the same.  Returns the DataFrame with a "particle" column added, or
+    the array of particle ids.  (The reference wrapper calls the
+    non-existent ndarray.as_type; fixed here.)"""
+    import pandas as pd
+    if isinstance(to_link, pd.DataFrame):
+        arr = to_link[["frame", "y", "x"]].to_numpy()
+    else:
+        arr = to_link
+    arr = np.ascontiguousarray(arr, dtype=np.float32)
+    if arr.ndim != 2 or arr.shape[1] != 3:
+        raise ValueError("expected an (N, 3) array of frame, y, x")
+    ids = np.zeros(arr.shape[0], np.uint64)
+    N.check(N.lib().gt_link(arr.ctypes.data, arr.shape[0], float(search_range),
+                            int(memory or 0), ids.ctypes.data))
+    if isinstance(to_link, pd.DataFrame):
+        out = to_link.copy()
+        out["particle"] = ids
+        return out
+    return ids

@@ -369,9 +369,17 @@ gt_host_post::gt_host_post(const gt_params *p, const gt_derived *d, uint32_t H, 
                            const uint64_t *reset_points, uint64_t nreset)
     : p_(*p), d_(*d), H_(H), W_(W) {
     ncols_ = gt_host_column_names(p, nullptr, nullptr);
-    (void)reset_points;
-    if (p->has_search_range) error_ = "linking (search_range) is not available in this build yet";
-    (void)nreset;
+    if (p->has_search_range) {
+        if (!(p->search_range > 0.0f) || !std::isfinite(p->search_range)) {
+            error_ = "search_range must be positive and finite";
+            return;
+        }
+        linker_.reset(new gt_linker(p->search_range, p->has_memory ? p->memory : 0u));
+        if (reset_points && nreset) {
+            reset_.assign(reset_points, reset_points + nreset);
+            std::sort(reset_.begin(), reset_.end());
+        }
+    }
 }
 
 gt_status gt_host_post::process(std::vector<gt_detection> &rows, uint64_t frame_index,
@@ -413,6 +421,22 @@ gt_status gt_host_post::process(std::vector<gt_detection> &rows, uint64_t frame_
                         d_.bg_radius, d_.gap_radius, dough_.data());
         }
     }
+    if (linker_) {
+        // linker_reset_points (execute_gpu.rs:792-797): a new linker, ids
+        // start over
+        if (std::binary_search(reset_.begin(), reset_.end(), frame_index)) {
+            seg_books_.push_back(linker_->finish());
+            linker_.reset(new gt_linker(p_.search_range, p_.has_memory ? p_.memory : 0u));
+        }
+        xy_.resize(2 * n);
+        for (size_t i = 0; i < n; ++i) {
+            xy_[2 * i] = tmp_rows_[i].x;
+            xy_[2 * i + 1] = tmp_rows_[i].y;
+        }
+        ids_.resize(n);
+        gt_status ls = linker_->advance(xy_.data(), n, ids_.data(), interrupt, err);
+        if (ls != GT_OK) return ls;
+    }
     const float fidx = static_cast<float>(frame_index);
     for (size_t i = 0; i < n; ++i) {
         const gt_result_row &r = tmp_rows_[i];
@@ -432,12 +456,30 @@ gt_status gt_host_post::process(std::vector<gt_detection> &rows, uint64_t frame_
             out_.push_back(dough_[3 * i + 1]);
             out_.push_back(dough_[3 * i + 2]);
         }
+        if (linker_) {
+            out_.push_back(static_cast<float>(ids_[i]));
+            out_.push_back(0.0f);    // duration, filled in by finish()
+            row_seg_.push_back(static_cast<uint32_t>(seg_books_.size()));
+        }
     }
     return GT_OK;
 }
 
 gt_status gt_host_post::finish(gt_output *out, std::string *err) {
     (void)err;
+    if (linker_) {
+        // duration column (execute_gpu.rs:1165-1181), the last column.  Each
+        // segment between reset points keeps its own durations (the
+        // reference reads every row's id in the last segment's table).
+        seg_books_.push_back(linker_->finish());
+        const uint32_t nc = ncols_;
+        for (size_t r = 0; r < row_seg_.size(); ++r) {
+            float *row = &out_[r * nc];
+            const auto &books = seg_books_[row_seg_[r]];
+            const size_t id = static_cast<size_t>(row[nc - 2]);
+            row[nc - 1] = id < books.size() ? static_cast<float>(books[id].duration) : 0.0f;
+        }
+    }
     // One static name/type list per combination of the four optional column
     // groups, so the pointers handed out stay valid and are never rewritten.
     struct Lists {
@@ -662,5 +704,39 @@ extern "C" gt_status gt_prune_blobs(gt_result_row *rows, uint64_t n, float overl
                                     uint64_t *n_out) {
     if (!n_out || (n && !rows)) return GT_ERR_INVALID_ARGUMENT;
     *n_out = gt_host_prune_blobs(rows, static_cast<size_t>(n), overlap_threshold);
+    return GT_OK;
+}
+
+extern "C" gt_status gt_link(const float *points, uint64_t n, float search_range, uint32_t memory,
+                             uint64_t *ids) {
+    // linker_all over FrameSubsetter's linking groups (linking.rs:94-196,
+    // :818-831): rows [frame, y, x] sorted by frame; consecutive rows of one
+    // frame value form a frame, the first frame is frame 0 even when empty.
+    if (n && (!points || !ids)) return GT_ERR_INVALID_ARGUMENT;
+    if (!(search_range > 0.0f) || !std::isfinite(search_range)) return GT_ERR_INVALID_ARGUMENT;
+    gt_linker lk(search_range, memory);
+    std::vector<float> xy;
+    std::string err;
+    float cur = 0.0f;
+    uint64_t begin = 0;
+    auto flush = [&](uint64_t end) -> gt_status {
+        xy.resize(2 * (end - begin));
+        for (uint64_t i = begin; i < end; ++i) {
+            xy[2 * (i - begin)] = points[3 * i + 1];
+            xy[2 * (i - begin) + 1] = points[3 * i + 2];
+        }
+        return lk.advance(xy.data(), end - begin, ids + begin, nullptr, &err);
+    };
+    for (uint64_t i = 0; i < n; ++i) {
+        const float fr = points[3 * i];
+        if (fr > cur) {
+            gt_status st = flush(i);
+            if (st != GT_OK) return st;
+            begin = i;
+            cur = fr;
+        }
+        if (fr < cur) return GT_ERR_NON_SORTED_CHARACTERIZATION;
+    }
+    if (n > begin) return flush(n);
     return GT_OK;
 }

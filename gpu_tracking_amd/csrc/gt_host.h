@@ -4,10 +4,12 @@
 #pragma once
 
 #include <atomic>
+#include <memory>
 #include <string>
 #include <vector>
 
 #include "gt_kparams.h"
+#include "gt_linker.h"
 #include "gtrack.h"
 
 // Parameters after validation and derivation (gpuparams_from_tracking_params,
@@ -87,6 +89,13 @@ class gt_host_post {
     std::vector<gt_result_row> tmp_rows_;
     std::vector<uint8_t> keep_;
     std::vector<float> dough_;
+    // linking (search_range): one linker per segment between reset points
+    std::vector<uint64_t> reset_;          // sorted frame indices
+    std::unique_ptr<gt_linker> linker_;
+    std::vector<std::vector<gt_linker::Book>> seg_books_;   // finished segments
+    std::vector<uint32_t> row_seg_;        // segment of each output row
+    std::vector<uint64_t> ids_;
+    std::vector<float> xy_;
 };
 
 // Implemented in gt_runtime.cpp (needs the device).

@@ -216,6 +216,15 @@ gt_status gt_doughnut(const float *raw, uint32_t nrows, uint32_t ncols, const gt
 gt_status gt_prune_blobs(gt_result_row *rows, uint64_t n, float overlap_threshold,
                          uint64_t *n_out);
 
+/* Frame-to-frame linking of detections given as rows [frame, y, x] sorted by
+ * frame (linker_all over FrameSubsetter groups, src/linking.rs:94-196 and
+ * :818-831, with Linker at :346-521 and link at :564-816): ids[i] receives
+ * row i's particle id.  search_range > 0; memory = frames a particle may be
+ * missing.  GT_ERR_TOO_DENSE_TO_LINK when a subnetwork is too large,
+ * GT_ERR_NON_SORTED_CHARACTERIZATION when frames decrease.  Host-only. */
+gt_status gt_link(const float *points, uint64_t n, float search_range, uint32_t memory,
+                  uint64_t *ids);
+
 /* Interruption and progress (the Arc<AtomicBool> / Arc<Mutex<(usize,
  * Option<usize>)>> pair of execute_gpu, src/execute_gpu.rs:1094-1099,
  * :974-977).  Safe to call from any thread while gt_execute runs. */

@@ -141,3 +141,35 @@ def test_characterize_points_unsorted_frames_rejected(gt):
     with pytest.raises(gt.GtError) as e:
         gt.characterize_points(v, pts, diameter=9, has_frames=True)
     assert e.value.status == 6  # NonSortedCharacterization
+
+
+def test_batch_with_linking(gt, oracle):
+    # search_range adds particle and duration columns (column_names,
+    # execute_gpu.rs:482-519; durations :1165-1181); ids must equal the
+    # oracle linker run on the oracle's per-frame rows in the same order
+    from oracle import linker as OL
+    v = _video(T=8, seed=12, n=40)
+    df = gt.batch(v, 9, minmass=300, search_range=5.0, memory=2, doughnut_correction=False)
+    assert list(df.columns)[-2:] == ["particle", "duration"]
+    ref = _expected(oracle, v, 9, 10, 300.0, True, False)
+    got = _frame_matrix(df)
+    assert np.array_equal(got[:, :-2].view(np.uint32), ref.view(np.uint32))
+    ids = OL.link_frames(ref[:, :3], 5.0, 2)
+    assert df["particle"].tolist() == ids
+    # duration: frames from first to last appearance of the particle, inclusive
+    first, last = {}, {}
+    for f, i in zip(df["frame"].tolist(), ids):
+        first.setdefault(i, f)
+        last[i] = f
+    assert df["duration"].tolist() == [last[i] - first[i] + 1 for i in ids]
+
+
+def test_batch_linker_reset_points(gt, oracle):
+    v = _video(T=6, seed=13, n=25)
+    df = gt.batch(v, 9, minmass=300, search_range=5.0, linker_reset_points=[3],
+                  doughnut_correction=False, characterize=False)
+    a = df[df.frame < 3]
+    b = df[df.frame >= 3]
+    # ids start over at the reset point
+    assert b["particle"].min() == 0 and a["particle"].min() == 0
+    assert set(b[b.frame == 3]["particle"]) == set(range(len(b[b.frame == 3])))

@@ -1,0 +1,157 @@
+"""Linking: the oracle's restatement of src/linking.rs pinned against brute
+force and invariants, and the product's C++ linker (gt_link) against the
+oracle, id for id.  All on the CPU."""
+import itertools
+
+import numpy as np
+import pytest
+
+from oracle import linker as OL
+
+
+def _cost(src, dest, assign, r2):
+    """Total cost of an assignment dest -> src index (or None)."""
+    c = np.float64(0)
+    for d, s in enumerate(assign):
+        if s is None:
+            c += r2
+        else:
+            c += float(OL._d2(dest[d], src[s][0]))
+    return c
+
+
+def _brute(src, dest, r):
+    r2 = float(np.float32(np.float32(r) * np.float32(r)))
+    best = None
+    opts = [[None] + [s for s in range(len(src)) if OL._d2(dest[d], src[s][0]) < np.float32(r2)]
+            for d in range(len(dest))]
+    for assign in itertools.product(*opts):
+        used = [s for s in assign if s is not None]
+        if len(used) != len(set(used)):
+            continue
+        c = _cost(src, dest, assign, r2)
+        if best is None or c < best:
+            best = c
+    return best
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_oracle_subnet_assignment_is_optimal(seed):
+    rng = np.random.default_rng(seed)
+    ns, nd = rng.integers(1, 5), rng.integers(1, 5)
+    src = [((np.float32(rng.uniform(0, 4)), np.float32(rng.uniform(0, 4))), 100 + k)
+           for k in range(ns)]
+    dest = [(np.float32(rng.uniform(0, 4)), np.float32(rng.uniform(0, 4))) for _ in range(nd)]
+    r = 2.5
+    ids, unused, counter = OL.link(src, dest, None, r, 1000)
+    idmap = {e[1]: k for k, e in enumerate(src)}
+    assign = [idmap.get(i) for i in ids]
+    r2 = float(np.float32(np.float32(r) * np.float32(r)))
+    assert abs(_cost(src, dest, assign, r2) - _brute(src, dest, r)) < 1e-4
+    # linked sources are distinct, new ids count up from the counter in dest order
+    linked = [i for i in ids if i < 1000]
+    assert len(linked) == len(set(linked))
+    assert [i for i in ids if i >= 1000] == list(range(1000, counter))
+    assert set(e[1] for e in unused) == set(e[1] for e in src) - set(linked)
+
+
+def _tracks(rng, n, T, step, blink=0.0, box=200.0):
+    pos = rng.uniform(10, box - 10, (n, 2))
+    frames, truth = [], []
+    for t in range(T):
+        if t:
+            pos = pos + rng.normal(0, step, pos.shape)
+        vis = rng.random(n) >= blink
+        frames.append(pos[vis].astype(np.float32))
+        truth.append(np.flatnonzero(vis))
+    return frames, truth
+
+
+def test_oracle_keeps_ids_of_separated_particles():
+    rng = np.random.default_rng(1)
+    frames, truth = _tracks(rng, 30, 12, 0.3, box=400.0)
+    lk = OL.Linker(3.0, 0)
+    first = None
+    for t, f in enumerate(frames):
+        ids = lk.advance(f)
+        if first is None:
+            first = ids
+        assert ids == first          # every particle keeps its id
+    books = lk.finish()
+    assert all(d == 12 for _, d in books)
+
+
+def test_oracle_memory_and_durations():
+    # particle A: frames 0-1, absent 2-3, back at 4; B: frames 0-5; C: 0-1 then gone
+    A, B, C = (10.0, 10.0), (50.0, 50.0), (90.0, 90.0)
+    seq = [[A, B, C], [A, B, C], [B], [B], [A, B], [B]]
+    for memory, a_back in [(2, True), (1, False)]:
+        lk = OL.Linker(5.0, memory)
+        out = [lk.advance(f) for f in seq]
+        a0 = out[0][0]
+        if a_back:
+            assert out[4][0] == a0       # within memory: same id
+        else:
+            assert out[4][0] != a0       # beyond memory: a new particle
+        books = lk.finish()
+        # duration = frames from first to last appearance, inclusive
+        assert books[out[0][1]][1] == 6              # B
+        assert books[out[0][2]][1] == 2              # C
+        if a_back:
+            assert books[a0][1] == 5                 # A: frames 0..4
+        else:
+            assert books[a0][1] == 2 and books[out[4][0]][1] == 1
+
+
+def test_oracle_too_dense():
+    src = [((np.float32(i * 0.01), np.float32(0)), i) for i in range(80)]
+    dest = [(np.float32(i * 0.01), np.float32(0.001)) for i in range(80)]
+    with pytest.raises(OL.TooDenseToLink):
+        OL.link(src, dest, None, 5.0, 80)
+
+
+# --------------------------------------------------------------------------
+# product (gt_link, C++) against the oracle
+def _as_rows(frames):
+    rows = []
+    for t, f in enumerate(frames):
+        for p in f:
+            rows.append((t, p[0], p[1]))
+    return np.array(rows, np.float32).reshape(-1, 3)
+
+
+@pytest.mark.parametrize("seed,n,T,step,blink,sr,mem", [
+    (0, 60, 15, 1.0, 0.05, 4.0, 0),
+    (1, 80, 20, 1.5, 0.1, 5.0, 2),
+    (2, 40, 10, 2.5, 0.0, 6.0, 1),    # crowded: subnetworks of several particles
+    (3, 120, 8, 0.8, 0.2, 3.0, 3),
+])
+def test_gt_link_matches_oracle(seed, n, T, step, blink, sr, mem):
+    from gpu_tracking_amd import _native as N
+    rng = np.random.default_rng(seed)
+    frames, _ = _tracks(rng, n, T, step, blink, box=120.0)
+    rows = _as_rows(frames)
+    ref = OL.link_frames(rows, sr, mem)
+    ids = np.zeros(len(rows), np.uint64)
+    N.check(N.lib().gt_link(rows.ctypes.data, len(rows), sr, mem, ids.ctypes.data))
+    assert ids.tolist() == ref
+
+
+def test_gt_link_groups_and_errors():
+    import gpu_tracking_amd as gt
+    import pandas as pd
+    from gpu_tracking_amd import _native as N
+    # frames 2 and 5 only: an empty frame 0 comes first, skipped numbers are not frames
+    df = pd.DataFrame({"frame": [2, 2, 5, 5], "y": [1.0, 20.0, 1.5, 20.5], "x": [1, 1, 1, 1]})
+    out = gt.link(df, 2.0, 0)
+    assert list(out["particle"]) == [0, 1, 0, 1]
+    ref = OL.link_frames(df[["frame", "y", "x"]].to_numpy(), 2.0, 0)
+    assert list(out["particle"]) == ref
+    with pytest.raises(N.GtError) as e:
+        gt.link(np.array([[1, 0, 0], [0, 1, 1]], np.float32), 2.0)
+    assert e.value.status == 6
+    dense = np.array([[0, i * 0.01, 0] for i in range(80)] + [[1, i * 0.01, 0.001]
+                                                               for i in range(80)], np.float32)
+    with pytest.raises(N.GtError) as e:
+        gt.link(dense, 5.0)
+    assert e.value.status == 18
