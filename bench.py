@@ -60,7 +60,7 @@ def _cpu_baseline(video_host, params, budget_s, gpu_rows):
     while done < T and time.perf_counter() - t0 < budget_s:
         n = min(T - done, 4 * nthreads)
         rows, _ = oracle.detect_batch(video_host[done:done + n], op, characterize=True,
-                                      nthreads=nthreads, cap_per_frame=H * W // 4)
+                                      nthreads=nthreads, cap_per_frame=8192)
         results.extend(rows)
         done += n
     dt = time.perf_counter() - t0
@@ -164,6 +164,9 @@ def main():
     ap.add_argument("--frames", type=int, default=0, help="override the frame count")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--e2e-frames", type=int, default=256,
+                    help="frames of the end-to-end run (host frames in, DataFrame columns out, "
+                         "with linking; plain workloads only; 0 = skip)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a 1 GiB device-to-device copy (a known-size kernel that "
                          "calibrates rocprof's FETCH_SIZE/WRITE_SIZE counters)")
@@ -280,16 +283,41 @@ def main():
                         "peak_tflops": 78.6,
                         "frac": round(flop_frame * chunk / tk / 1e12 / 78.6, 4)}
 
+    e2e = None
+    if not is_log and args.e2e_frames > 0:
+        # whole-video path: host frames staged through pinned memory, detection
+        # on the GPU, read-back, and filter_close + doughnut + linking
+        # (search_range 5, memory 2) on the host thread, overlapped with the
+        # next chunk's detection
+        n = min(T, args.e2e_frames)
+        host = frames[:n].cpu().numpy().view(np.uint16)
+        kw = {k: v for k, v in params.items() if k != "diameter"}
+        p2 = gt.make_params(params["diameter"], search_range=5.0, memory=2, **kw)
+        with gt.Context(p2, H, W) as ctx2:
+            ctx2.execute(host[:min(n, 8)])           # warm-up (allocations, first launch)
+            t0 = time.perf_counter()
+            data, names, _ = ctx2.execute(host)
+            dt2 = time.perf_counter() - t0
+        e2e = {"frames": n, "value": round(n / dt2, 1), "unit": "frames/s",
+               "rows": int(data.shape[0]), "columns": list(names),
+               "includes": "host frames -> pinned staging -> H2D -> detection -> D2H -> "
+                           "filter_close, doughnut, linking (search_range 5, memory 2)"}
+        print(f"[bench] end to end {e2e['value']} frames/s", file=sys.stderr, flush=True)
+
     cpu = None
     parity = None
     if not args.no_cpu:
-        rows = gt.detections_to_numpy(out[:64], counts[:64])
         if is_log:
+            rows = gt.detections_to_numpy(out[:64], counts[:64])
             host = frames[:64].cpu().numpy()
             print("[bench] CPU baseline", file=sys.stderr, flush=True)
             cpu, nchk, bad = _cpu_baseline_log(host, params, args.cpu_seconds, rows)
         else:
-            host = frames[:64].cpu().numpy().view(np.uint16)
+            # enough frames for the time budget; the parity check covers all
+            # the frames the CPU finishes
+            nh = min(T, 2048)
+            rows = gt.detections_to_numpy(out[:nh], counts[:nh])
+            host = frames[:nh].cpu().numpy().view(np.uint16)
             print("[bench] CPU baseline", file=sys.stderr, flush=True)
             cpu, nchk, bad = _cpu_baseline(host, params, args.cpu_seconds, rows)
         parity = {"frames_checked": nchk, "frames_bit_identical": nchk - bad}
@@ -306,7 +334,7 @@ def main():
                                f"{'f32' if is_log else 'u16'}, "
                                f"{s.n_particles} particles/frame, " +
                                ", ".join(f"{k}={v}" for k, v in params.items())},
-        "roofline": roof, "cpu_baseline": cpu, "parity": parity,
+        "roofline": roof, "cpu_baseline": cpu, "parity": parity, "end_to_end": e2e,
         "mean_detections_per_frame": round(float(counts.float().mean().item()), 1),
     }
     print(json.dumps(res), flush=True)

@@ -100,5 +100,5 @@ class gt_host_post {
 
 // Implemented in gt_runtime.cpp (needs the device).
 struct gt_context;
-gt_status gt_host_characterize_points(gt_context *c, size_t base, uint32_t nf, int32_t dtype,
-                                      const gt_host_plan &plan);
+gt_status gt_host_characterize_points(gt_context *c, int slot, size_t base, uint32_t nf,
+                                      int32_t dtype, const gt_host_plan &plan);

@@ -12,8 +12,12 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <condition_variable>
+#include <deque>
+#include <mutex>
 #include <new>
 #include <string>
+#include <thread>
 #include <unordered_map>
 #include <vector>
 
@@ -76,13 +80,20 @@ struct gt_context {
     GtLogTable *d_logtab = nullptr;  // LoG scale table (device copy)
     int32_t log_Rm = 0;
 
-    // gt_execute staging
-    void *d_frames = nullptr;      // chunk * HW pixels (largest dtype)
-    gt_detection *d_rows = nullptr;  // chunk * det_cap
-    uint32_t *d_counts = nullptr;  // chunk
+    // gt_execute staging: two slots, so one chunk's detections come back
+    // and are post-processed while the next chunk runs on the GPU
+    struct Slot {
+        void *d_frames = nullptr;      // chunk * HW pixels (largest dtype)
+        gt_detection *d_rows = nullptr;  // chunk * det_cap
+        uint32_t *d_counts = nullptr;  // chunk
+        void *h_stage = nullptr;       // pinned, chunk * HW * 4 bytes
+        uint32_t *h_counts = nullptr;  // pinned, chunk
+        hipEvent_t done = nullptr;     // counts of the slot's chunk are on the host
+    } slot[2];
     uint32_t det_cap = 0;
-    void *h_stage = nullptr;       // pinned, chunk * HW * 4 bytes
-    uint32_t *h_counts = nullptr;  // pinned, chunk
+    hipStream_t copy_stream = nullptr;  // detection read-back
+    gt_detection *h_rows = nullptr;     // pinned read-back buffer
+    size_t h_rows_cap = 0;
 
     // timing
     bool timing = false;
@@ -114,14 +125,30 @@ gt_status alloc_scratch(gt_context *c) {
 }
 
 gt_status alloc_execute_buffers(gt_context *c) {
-    if (c->d_frames) return GT_OK;
+    if (c->slot[0].d_frames) return GT_OK;
     const size_t HW = static_cast<size_t>(c->H) * c->W;
-    GT_HIP(hipMalloc(&c->d_frames, 4 * HW * c->chunk));
-    GT_HIP(hipMalloc(&c->d_rows, sizeof(gt_detection) * static_cast<size_t>(c->det_cap) * c->chunk));
-    GT_HIP(hipMalloc(&c->d_counts, sizeof(uint32_t) * c->chunk));
-    GT_HIP(hipHostMalloc(&c->h_stage, 4 * HW * c->chunk, hipHostMallocDefault));
-    GT_HIP(hipHostMalloc(reinterpret_cast<void **>(&c->h_counts), sizeof(uint32_t) * c->chunk,
+    for (auto &sl : c->slot) {
+        GT_HIP(hipMalloc(&sl.d_frames, 4 * HW * c->chunk));
+        GT_HIP(hipMalloc(&sl.d_rows, sizeof(gt_detection) * static_cast<size_t>(c->det_cap) * c->chunk));
+        GT_HIP(hipMalloc(&sl.d_counts, sizeof(uint32_t) * c->chunk));
+        GT_HIP(hipHostMalloc(&sl.h_stage, 4 * HW * c->chunk, hipHostMallocDefault));
+        GT_HIP(hipHostMalloc(reinterpret_cast<void **>(&sl.h_counts), sizeof(uint32_t) * c->chunk,
+                             hipHostMallocDefault));
+        GT_HIP(hipEventCreateWithFlags(&sl.done, hipEventDisableTiming));
+    }
+    GT_HIP(hipStreamCreateWithFlags(&c->copy_stream, hipStreamNonBlocking));
+    return GT_OK;
+}
+
+gt_status reserve_host_rows(gt_context *c, size_t n) {
+    if (n <= c->h_rows_cap) return GT_OK;
+    if (c->h_rows) (void)hipHostFree(c->h_rows);
+    c->h_rows = nullptr;
+    c->h_rows_cap = 0;
+    const size_t cap = std::max<size_t>(n, 2 * c->h_rows_cap + 4096);
+    GT_HIP(hipHostMalloc(reinterpret_cast<void **>(&c->h_rows), sizeof(gt_detection) * cap,
                          hipHostMallocDefault));
+    c->h_rows_cap = cap;
     return GT_OK;
 }
 
@@ -133,11 +160,16 @@ void free_all(gt_context *c) {
     if (c->d_slow) (void)hipFree(c->d_slow);
     if (c->d_scount) (void)hipFree(c->d_scount);
     if (c->d_logtab) (void)hipFree(c->d_logtab);
-    if (c->d_frames) (void)hipFree(c->d_frames);
-    if (c->d_rows) (void)hipFree(c->d_rows);
-    if (c->d_counts) (void)hipFree(c->d_counts);
-    if (c->h_stage) (void)hipHostFree(c->h_stage);
-    if (c->h_counts) (void)hipHostFree(c->h_counts);
+    for (auto &sl : c->slot) {
+        if (sl.d_frames) (void)hipFree(sl.d_frames);
+        if (sl.d_rows) (void)hipFree(sl.d_rows);
+        if (sl.d_counts) (void)hipFree(sl.d_counts);
+        if (sl.h_stage) (void)hipHostFree(sl.h_stage);
+        if (sl.h_counts) (void)hipHostFree(sl.h_counts);
+        if (sl.done) (void)hipEventDestroy(sl.done);
+    }
+    if (c->h_rows) (void)hipHostFree(c->h_rows);
+    if (c->copy_stream) (void)hipStreamDestroy(c->copy_stream);
     for (hipEvent_t e : c->events) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
 }
@@ -266,7 +298,10 @@ gt_status gt_context_create(const gt_params *params, uint32_t nrows, uint32_t nc
     }
     const size_t HW = static_cast<size_t>(nrows) * ncols;
     // Frames per chunk: bounded by a 4 GiB scratch budget and by 64 frames.
-    const size_t per_frame = HW * (4 + 4 + 4 + sizeof(gt_detection));
+    // per frame of device scratch: processed plane and candidate list (the
+    // two gt_execute staging slots, frame + detection buffer each, are
+    // allocated on first use)
+    const size_t per_frame = HW * (4 + 4);
     size_t chunk = (size_t(4) << 30) / (per_frame ? per_frame : 1);
     c->chunk = static_cast<uint32_t>(std::max<size_t>(1, std::min<size_t>(chunk, 64)));
     c->cand_cap = static_cast<uint32_t>(HW);
@@ -393,52 +428,159 @@ gt_status gt_execute(gt_context *c, const void *frames, int32_t dtype, uint64_t 
     c->prog_total.store(plan.frames.size());
     c->prog_cur.store(0);
 
-    std::vector<gt_detection> rows;
-    std::vector<float> rawf;
-    for (size_t base = 0; base < plan.frames.size(); base += c->chunk) {
-        if (c->interrupt.load()) return fail(GT_ERR_INTERRUPTED, "Keyboard Interrupt, my custom version");
-        const uint32_t nf = static_cast<uint32_t>(std::min<size_t>(c->chunk, plan.frames.size() - base));
+    // Host post-processing (filter_close / pruning, doughnut, linking) runs
+    // on its own thread, frame by frame in order, as the reference's
+    // post-process thread (execute_gpu.rs:914-987), while this thread keeps
+    // the GPU busy: chunk k+1 is submitted before chunk k's detections are
+    // read back and handed over.
+    struct Item {
+        uint64_t frame;
+        std::vector<gt_detection> rows;
+        const char *raw;
+    };
+    std::deque<Item> queue;
+    std::mutex mu;
+    std::condition_variable cv_item, cv_space;
+    bool producer_done = false;
+    gt_status post_status = GT_OK;
+    std::string post_err;
+    const size_t max_queued = 4 * static_cast<size_t>(c->chunk) + 8;
+    std::thread worker([&]() {
+        std::vector<float> rawf;
+        for (;;) {
+            Item it;
+            {
+                std::unique_lock<std::mutex> lk(mu);
+                cv_item.wait(lk, [&] { return !queue.empty() || producer_done; });
+                if (queue.empty()) return;
+                it = std::move(queue.front());
+                queue.pop_front();
+            }
+            cv_space.notify_one();
+            bool failed;
+            {
+                std::lock_guard<std::mutex> lk(mu);
+                failed = post_status != GT_OK;
+            }
+            if (failed) continue;   // drain without work after an error
+            const float *rawp = nullptr;
+            if (post.needs_raw()) {
+                rawf.resize(HW);
+                gt_host_widen(it.raw, dtype, HW, rawf.data());
+                rawp = rawf.data();
+            }
+            std::string perr;
+            gt_status ps = post.process(it.rows, it.frame, rawp, &c->interrupt, &perr);
+            if (ps != GT_OK) {
+                std::lock_guard<std::mutex> lk(mu);
+                post_status = ps;
+                post_err = perr;
+            }
+            c->prog_cur.fetch_add(1);
+        }
+    });
+    auto post_failed = [&]() {
+        std::lock_guard<std::mutex> lk(mu);
+        return post_status != GT_OK;
+    };
+    auto finish_worker = [&]() {
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            producer_done = true;
+        }
+        cv_item.notify_all();
+        worker.join();
+    };
+
+    const size_t nplan = plan.frames.size();
+    const size_t nchunks = (nplan + c->chunk - 1) / c->chunk;
+    // stage chunk k in slot k % 2 and queue its GPU work
+    auto submit = [&](size_t k) -> gt_status {
+        auto &sl = c->slot[k % 2];
+        const size_t base = k * c->chunk;
+        const uint32_t nf = static_cast<uint32_t>(std::min<size_t>(c->chunk, nplan - base));
         for (uint32_t f = 0; f < nf; ++f) {
             const uint64_t idx = plan.frames[base + f];
             if (idx >= nframes)
                 return fail(GT_ERR_FRAME_OUT_OF_BOUNDS,
                             "Requested a frame outside the length\n of the video. Video length is %llu, requested frame %llu",
                             static_cast<unsigned long long>(nframes), static_cast<unsigned long long>(idx));
-            std::memcpy(static_cast<char *>(c->h_stage) + esz * HW * f,
+            std::memcpy(static_cast<char *>(sl.h_stage) + esz * HW * f,
                         static_cast<const char *>(frames) + esz * HW * idx, esz * HW);
         }
-        GT_HIP(hipMemcpyAsync(c->d_frames, c->h_stage, esz * HW * nf, hipMemcpyHostToDevice, c->stream));
-        if (c->characterize_points) {
-            st = gt_host_characterize_points(c, base, nf, dtype, plan);
-            if (st != GT_OK) return st;
-        } else {
-            st = run_pipeline(c, c->d_frames, dtype, nf, c->d_rows, c->det_cap, c->d_counts, c->stream);
-            if (st != GT_OK) return st;
-        }
-        GT_HIP(hipMemcpyAsync(c->h_counts, c->d_counts, sizeof(uint32_t) * nf, hipMemcpyDeviceToHost, c->stream));
-        GT_HIP(hipStreamSynchronize(c->stream));
+        GT_HIP(hipMemcpyAsync(sl.d_frames, sl.h_stage, esz * HW * nf, hipMemcpyHostToDevice, c->stream));
+        gt_status st2;
+        if (c->characterize_points)
+            st2 = gt_host_characterize_points(c, k % 2, base, nf, dtype, plan);
+        else
+            st2 = run_pipeline(c, sl.d_frames, dtype, nf, sl.d_rows, c->det_cap, sl.d_counts, c->stream);
+        if (st2 != GT_OK) return st2;
+        GT_HIP(hipMemcpyAsync(sl.h_counts, sl.d_counts, sizeof(uint32_t) * nf, hipMemcpyDeviceToHost, c->stream));
+        GT_HIP(hipEventRecord(sl.done, c->stream));
+        return GT_OK;
+    };
+    // read chunk k's detections back and hand them to the post thread
+    auto drain = [&](size_t k) -> gt_status {
+        auto &sl = c->slot[k % 2];
+        const size_t base = k * c->chunk;
+        const uint32_t nf = static_cast<uint32_t>(std::min<size_t>(c->chunk, nplan - base));
+        GT_HIP(hipEventSynchronize(sl.done));
+        size_t total = 0;
         for (uint32_t f = 0; f < nf; ++f) {
-            const uint32_t n = c->h_counts[f];
+            const uint32_t n = sl.h_counts[f];
             if (n > c->det_cap)
                 return fail(GT_ERR_CAPACITY, "frame %llu produced %u detections, above the capacity of %u",
                             static_cast<unsigned long long>(plan.frames[base + f]), n, c->det_cap);
-            rows.resize(n);
-            if (n)
-                GT_HIP(hipMemcpy(rows.data(), c->d_rows + static_cast<size_t>(f) * c->det_cap,
-                                 sizeof(gt_detection) * n, hipMemcpyDeviceToHost));
-            const char *raw = static_cast<const char *>(frames) + esz * HW * plan.frames[base + f];
-            const float *rawp = nullptr;
-            if (post.needs_raw()) {
-                rawf.resize(HW);
-                gt_host_widen(raw, dtype, HW, rawf.data());
-                rawp = rawf.data();
-            }
-            std::string perr;
-            gt_status pst2 = post.process(rows, plan.frames[base + f], rawp, &c->interrupt, &perr);
-            if (pst2 != GT_OK) return fail(pst2, "%s", perr.c_str());
-            c->prog_cur.fetch_add(1);
+            total += n;
         }
+        gt_status rs = reserve_host_rows(c, total);
+        if (rs != GT_OK) return rs;
+        size_t off = 0;
+        for (uint32_t f = 0; f < nf; ++f) {
+            const uint32_t n = sl.h_counts[f];
+            if (n)
+                GT_HIP(hipMemcpyAsync(c->h_rows + off, sl.d_rows + static_cast<size_t>(f) * c->det_cap,
+                                      sizeof(gt_detection) * n, hipMemcpyDeviceToHost, c->copy_stream));
+            off += n;
+        }
+        GT_HIP(hipStreamSynchronize(c->copy_stream));
+        off = 0;
+        for (uint32_t f = 0; f < nf; ++f) {
+            const uint32_t n = sl.h_counts[f];
+            Item it;
+            it.frame = plan.frames[base + f];
+            it.rows.assign(c->h_rows + off, c->h_rows + off + n);
+            it.raw = static_cast<const char *>(frames) + esz * HW * it.frame;
+            off += n;
+            std::unique_lock<std::mutex> lk(mu);
+            cv_space.wait(lk, [&] { return queue.size() < max_queued || post_status != GT_OK; });
+            queue.push_back(std::move(it));
+            lk.unlock();
+            cv_item.notify_one();
+        }
+        return GT_OK;
+    };
+
+    gt_status lst = GT_OK;
+    if (nchunks) lst = submit(0);
+    for (size_t k = 0; k < nchunks && lst == GT_OK; ++k) {
+        if (c->interrupt.load()) {
+            lst = fail(GT_ERR_INTERRUPTED, "Keyboard Interrupt, my custom version");
+            break;
+        }
+        if (post_failed()) break;
+        if (k + 1 < nchunks) {
+            // slot (k+1) % 2 last held chunk k-1, drained in the previous pass
+            lst = submit(k + 1);
+            if (lst != GT_OK) break;
+        }
+        lst = drain(k);
     }
+    // leave no work in flight on the slots before returning
+    (void)hipStreamSynchronize(c->stream);
+    finish_worker();
+    if (lst != GT_OK) return lst;
+    if (post_status != GT_OK) return fail(post_status, "%s", post_err.c_str());
     std::string ferr;
     gt_status fst = post.finish(out, &ferr);
     if (fst != GT_OK) return fail(fst, "%s", ferr.c_str());
@@ -452,8 +594,9 @@ gt_status gt_execute(gt_context *c, const void *frames, int32_t dtype, uint64_t 
 // pass order, src/gpu_setup.rs:877-891): the chunk's frames are already on the
 // device; upload each frame's positions as rows, bandpass, then characterize
 // the rows in place.  Row seeds are the positions' order within the frame.
-gt_status gt_host_characterize_points(gt_context *c, size_t base, uint32_t nf, int32_t dtype,
-                                      const gt_host_plan &plan) {
+gt_status gt_host_characterize_points(gt_context *c, int slot, size_t base, uint32_t nf,
+                                      int32_t dtype, const gt_host_plan &plan) {
+    auto &sl = c->slot[slot];
     std::vector<gt_detection> staged;
     std::vector<uint32_t> counts(nf, 0);
     uint32_t max_count = 0;
@@ -472,16 +615,16 @@ gt_status gt_host_characterize_points(gt_context *c, size_t base, uint32_t nf, i
             staged[i].frame = static_cast<uint32_t>(plan.frames[base + f]);
         }
         if (n)
-            GT_HIP(hipMemcpyAsync(c->d_rows + static_cast<size_t>(f) * c->det_cap, staged.data(),
+            GT_HIP(hipMemcpyAsync(sl.d_rows + static_cast<size_t>(f) * c->det_cap, staged.data(),
                                   sizeof(gt_detection) * n, hipMemcpyHostToDevice, c->stream));
         // the staging vector is reused: wait for this copy before refilling it
         GT_HIP(hipStreamSynchronize(c->stream));
     }
-    std::memcpy(c->h_counts, counts.data(), sizeof(uint32_t) * nf);
-    GT_HIP(hipMemcpyAsync(c->d_counts, c->h_counts, sizeof(uint32_t) * nf, hipMemcpyHostToDevice,
+    std::memcpy(sl.h_counts, counts.data(), sizeof(uint32_t) * nf);
+    GT_HIP(hipMemcpyAsync(sl.d_counts, sl.h_counts, sizeof(uint32_t) * nf, hipMemcpyHostToDevice,
                           c->stream));
-    GT_HIP(gt_launch_bandpass(c->d_frames, dtype, c->d_proc, nf, c->kp, c->stream));
-    GT_HIP(gt_launch_characterize_rows(c->d_proc, c->d_frames, dtype, nf, c->kp, c->d_rows,
-                                       c->det_cap, c->d_counts, max_count, c->stream));
+    GT_HIP(gt_launch_bandpass(sl.d_frames, dtype, c->d_proc, nf, c->kp, c->stream));
+    GT_HIP(gt_launch_characterize_rows(c->d_proc, sl.d_frames, dtype, nf, c->kp, sl.d_rows,
+                                       c->det_cap, sl.d_counts, max_count, c->stream));
     return GT_OK;
 }
