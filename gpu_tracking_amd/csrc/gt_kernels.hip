@@ -641,7 +641,7 @@ __global__ __launch_bounds__(WK_THREADS) void k_walk(const float *__restrict__ p
 }
 
 // ---------------------------------------------------------------------------
-// Fused local max + walk + characterize.  One block per FT_TH x FT_TW tile:
+// Fused local max + walk.  One block per FT_TH x FT_TW tile:
 //   1. the processed frame over the tile plus halos (TileGeom) is staged in
 //      LDS, out-of-frame entries -inf;
 //   2. horizontal window maxima: one item = 4 adjacent output columns of one
@@ -656,8 +656,10 @@ __global__ __launch_bounds__(WK_THREADS) void k_walk(const float *__restrict__ p
 //      from LDS, any other from the frame in global memory (same arithmetic).
 //      Walks still moving go to k_walk_slow through a per-frame list (or walk
 //      on here when that list is full);
-//   5. finished walks are characterized and appended to the frame's rows with
+//   5. finished walks that pass minmass are appended to the frame's rows with
 //      one atomic per wave.  Waves without candidates leave early.
+// characterize runs afterwards over each frame's compacted rows
+// (k_characterize_det), whole wavefronts of detections at a time.
 constexpr int FT_TW = 64, FT_TH = 32, FT_THREADS = 256;
 constexpr int FT_WAVES = FT_THREADS / 64;
 constexpr int FT_WROWS = FT_TH / FT_WAVES;
@@ -798,7 +800,7 @@ __device__ __forceinline__ void characterize_inner(const B &base, const T *__res
 
 // Finish a walk at integer position (pu, pv) with last evaluation `res`:
 // minmass test, then the row, characterized when asked.
-template <typename T, int RI, int PITCH = 0>
+template <typename T, int RI, int PITCH = 0, bool kChar = true>
 __device__ __forceinline__ bool finish_walk(const GtKParams &p, const float *__restrict__ P,
                                             const T *__restrict__ Rw, const float *Sp, int oy,
                                             int ox, int RW, int ylo, int yhi, int xlo, int xhi,
@@ -808,7 +810,7 @@ __device__ __forceinline__ bool finish_walk(const GtKParams &p, const float *__r
     res.y = res.y + static_cast<float>(pv);
     if (!passes_minmass(p, res)) return false;
     fill_row(d, res, seed, frame);
-    if (p.characterize) {
+    if (kChar && p.characterize) {
         const int cu = f2i(rintf(d.row.x)), cv = f2i(rintf(d.row.y));
         const bool in_lds = Sp && cu >= ylo && cu <= yhi && cv >= xlo && cv <= xhi;
         if constexpr (RI > 0) {
@@ -1048,7 +1050,7 @@ __global__ __launch_bounds__(FT_THREADS, FT_MIN_WAVES) void k_detect_tile(
                 }
             }
             if (moving && it < p.max_iter) defer = true;
-            else keep = finish_walk<T, RI, PITCH>(p, P, Rw, Sp, oy, ox, RW, ylo, yhi, xlo, xhi, res, pu,
+            else keep = finish_walk<T, RI, PITCH, false>(p, P, Rw, Sp, oy, ox, RW, ylo, yhi, xlo, xhi, res, pu,
                                            pv, seed, frame, d);
         }
         // hand long walks to k_walk_slow; if its list is full, walk on here
@@ -1062,7 +1064,7 @@ __global__ __launch_bounds__(FT_THREADS, FT_MIN_WAVES) void k_detect_tile(
                     res = walk_eval<RI, PITCH>(p, P, Sp, oy, ox, RW, ylo, yhi, xlo, xhi, pu, pv);
                     if (!walk_move(res, pu, pv, H, W, rint, p.shift_thr)) break;
                 }
-                keep = finish_walk<T, RI, PITCH>(p, P, Rw, Sp, oy, ox, RW, ylo, yhi, xlo, xhi, res, pu,
+                keep = finish_walk<T, RI, PITCH, false>(p, P, Rw, Sp, oy, ox, RW, ylo, yhi, xlo, xhi, res, pu,
                                           pv, seed, frame, d);
             }
         }
@@ -1109,8 +1111,8 @@ __global__ __launch_bounds__(WK_THREADS) void k_walk_slow(const float *__restric
                 res = walk_eval<RI, 0>(p, P, nullptr, 0, 0, 0, 1, 0, 1, 0, pu, pv);
                 if (!walk_move(res, pu, pv, H, W, rint, p.shift_thr)) break;
             }
-            keep = finish_walk<T, RI>(p, P, Rw, nullptr, 0, 0, 0, 1, 0, 1, 0, res, pu, pv, e.x,
-                                      frame0 + fr, d);
+            keep = finish_walk<T, RI, 0, false>(p, P, Rw, nullptr, 0, 0, 0, 1, 0, 1, 0, res, pu, pv,
+                                                e.x, frame0 + fr, d);
         }
         const uint32_t pos = wave_append(keep, counter);
         if (keep && pos < out_cap) fout[pos] = d;
@@ -1136,6 +1138,17 @@ __global__ __launch_bounds__(WK_THREADS) void k_characterize_det(const float *__
     gt_detection *rows = out + static_cast<size_t>(fr) * out_cap;
     for (uint32_t i = blockIdx.x * WK_THREADS + threadIdx.x; i < n; i += gridDim.x * WK_THREADS) {
         gt_result_row row = rows[i].row;
+        if constexpr (RI > 0) {
+            // disk wholly inside the frame: every load issues at once
+            const int cu = f2i(rintf(row.x)), cv = f2i(rintf(row.y));
+            if (row.r <= 0.0f && p.rint == RI && cu >= RI && cu + RI < H && cv >= RI &&
+                cv + RI < W) {
+                const size_t c = static_cast<size_t>(cu) * W + cv;
+                characterize_inner<T, RI>(GlobalRow{src.p + c, W}, Rw + c, W, row);
+                rows[i].row = row;
+                continue;
+            }
+        }
         characterize_row<T, RI, true>(src, Rw, H, W, p.circle, row);
         rows[i].row = row;
     }
@@ -1452,8 +1465,9 @@ hipError_t gt_init_tables() {
 hipError_t gt_launch_characterize_det(const float *proc, const void *raw, int dtype,
                                       uint32_t nframes, const GtKParams &p, gt_detection *out,
                                       uint32_t out_cap, const uint32_t *ocount, hipStream_t s) {
+    // about one lane per expected detection (one per ~1000 pixels)
     const size_t hw = static_cast<size_t>(p.H) * p.W;
-    uint32_t blocks = static_cast<uint32_t>(hw / 65536);
+    uint32_t blocks = static_cast<uint32_t>(hw / 262144);
     blocks = blocks < 1 ? 1 : (blocks > 64 ? 64 : blocks);
     const dim3 grid(blocks, nframes, 1);
 #define GT_CHAR(TT)                                                                            \

@@ -217,13 +217,20 @@ gt_status run_pipeline(gt_context *c, const void *d_frames, int32_t dtype, uint3
             continue;
         }
         if (c->tiled) {
-            // local max, walk and characterize in one kernel
+            // local max and walk in one kernel, the walks it hands over, then
+            // characterize over the frame's compacted detections (whole
+            // wavefronts of detections instead of a few lanes per tile)
+            GtKParams kw = c->kp;
+            kw.characterize = 0;
             GT_HIP(gt_launch_zero_u32(c->d_scount, static_cast<size_t>(nf) * GT_COUNTER_STRIDE, s));
-            GT_HIP(gt_launch_detect_tile(c->d_proc, frames, dtype, nf, c->kp, out, cap,
+            GT_HIP(gt_launch_detect_tile(c->d_proc, frames, dtype, nf, kw, out, cap,
                                          c->d_ocount, c->d_slow, c->slow_cap, c->d_scount, f0, s));
             if (ev) GT_HIP(hipEventRecord(ev[2], s));
-            GT_HIP(gt_launch_walk_slow(c->d_proc, frames, dtype, nf, c->kp, c->d_slow, c->slow_cap,
+            GT_HIP(gt_launch_walk_slow(c->d_proc, frames, dtype, nf, kw, c->d_slow, c->slow_cap,
                                        c->d_scount, out, cap, c->d_ocount, f0, s));
+            if (c->kp.characterize)
+                GT_HIP(gt_launch_characterize_det(c->d_proc, frames, dtype, nf, c->kp, out, cap,
+                                                  c->d_ocount, s));
         } else {
             GT_HIP(gt_launch_localmax(c->d_proc, nf, c->kp, c->d_cand, c->cand_cap, c->d_ccount, s));
             if (ev) GT_HIP(hipEventRecord(ev[2], s));
