@@ -845,7 +845,7 @@ __global__ __launch_bounds__(FT_THREADS, FT_MIN_WAVES) void k_detect_tile(
     constexpr bool kFixed = KT > 0 && RI > 0;
     constexpr TileGeom gc = tile_geom(KT > 0 ? KT : 1, RI > 0 ? RI : 1);
     const TileGeom g = kFixed ? gc : g_rt;
-    extern __shared__ float lds[];
+    extern __shared__ __align__(16) float lds[];
     __shared__ uint32_t s_cnt[FT_WAVES];
     __shared__ uint64_t s_masks[FT_WAVES][FT_WROWS];
     constexpr bool kInt = sizeof(T) <= 2;  // integer frames: no NaN in the processed frame
