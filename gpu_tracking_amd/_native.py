@@ -73,6 +73,11 @@ class GtOutput(C.Structure):
 
 
 # numpy view of gt_detection (48 bytes)
+class GtFrameDesc(C.Structure):
+    _fields_ = [("data", C.c_void_p), ("nrows", C.c_uint32), ("ncols", C.c_uint32),
+                ("frame", C.c_uint32), ("pad", C.c_uint32)]
+
+
 DETECTION_FIELDS = [
     ("x", "<f4"), ("y", "<f4"), ("mass", "<f4"), ("r", "<f4"), ("max_intensity", "<f4"),
     ("Rg", "<f4"), ("raw_mass", "<f4"), ("signal", "<f4"), ("ecc", "<f4"), ("count", "<f4"),
@@ -84,7 +89,9 @@ EXPORTS = [
     "gt_params_default", "gt_abi_version", "gt_last_error", "gt_context_create",
     "gt_context_destroy", "gt_detect_device", "gt_execute", "gt_output_free",
     "gt_column_names", "gt_filter_close", "gt_doughnut", "gt_set_interrupt", "gt_progress",
-    "gt_set_timing", "gt_stage_times", "gt_prune_blobs", "gt_link",
+    "gt_set_timing", "gt_stage_times", "gt_prune_blobs", "gt_link", "gt_batch_create",
+    "gt_batch_destroy", "gt_batch_detect", "gt_batch_tune", "gt_batch_get_schedule",
+    "gt_batch_set_graph",
 ]
 
 _lib = None
@@ -136,6 +143,20 @@ def lib() -> C.CDLL:
     L.gt_prune_blobs.restype = C.c_int
     L.gt_link.argtypes = [C.c_void_p, C.c_uint64, C.c_float, C.c_uint32, C.c_void_p]
     L.gt_link.restype = C.c_int
+    L.gt_batch_create.argtypes = [P(GtParams), C.c_uint64, C.c_uint64, C.c_uint32, P(C.c_void_p)]
+    L.gt_batch_create.restype = C.c_int
+    L.gt_batch_destroy.argtypes = [C.c_void_p]
+    L.gt_batch_destroy.restype = None
+    L.gt_batch_detect.argtypes = [C.c_void_p, P(GtFrameDesc), C.c_uint32, C.c_int32, C.c_void_p,
+                                  C.c_uint32, C.c_void_p, C.c_void_p]
+    L.gt_batch_detect.restype = C.c_int
+    L.gt_batch_tune.argtypes = [C.c_void_p, P(GtFrameDesc), C.c_uint32, C.c_int32, C.c_void_p,
+                                C.c_uint32, C.c_void_p, P(C.c_float)]
+    L.gt_batch_tune.restype = C.c_int
+    L.gt_batch_get_schedule.argtypes = [C.c_void_p, C.c_void_p, P(C.c_uint64)]
+    L.gt_batch_get_schedule.restype = C.c_int
+    L.gt_batch_set_graph.argtypes = [C.c_void_p, C.c_int32]
+    L.gt_batch_set_graph.restype = C.c_int
     L.gt_set_interrupt.argtypes = [C.c_void_p, C.c_int32]
     L.gt_set_interrupt.restype = C.c_int
     L.gt_progress.argtypes = [C.c_void_p, P(C.c_uint64), P(C.c_uint64)]

@@ -378,3 +378,98 @@ def link(to_link, search_range, memory=0):
         out["particle"] = ids
         return out
     return ids
+
+
+class Batch:
+    """Batched detection of frames of different sizes (u8/u16, Trackpy style):
+    every launch runs a persistent kernel per stage over a table of frames,
+    replayed from a HIP graph; results equal Context.detect_device's bit for
+    bit.  No counterpart in the reference (one GpuState per frame size)."""
+
+    def __init__(self, params: N.GtParams, max_frame_pixels: int,
+                 max_launch_pixels: int = 1 << 28, max_slots: int = 256):
+        self._lib = N.lib()
+        self._h = C.c_void_p()
+        N.check(self._lib.gt_batch_create(C.byref(params), int(max_frame_pixels),
+                                          int(max_launch_pixels), int(max_slots),
+                                          C.byref(self._h)))
+
+    def close(self) -> None:
+        if self._h:
+            self._lib.gt_batch_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @staticmethod
+    def _descs(frames, frame_ids):
+        import torch
+        n = len(frames)
+        descs = (N.GtFrameDesc * n)()
+        code = None
+        for i, f in enumerate(frames):
+            if not f.is_cuda or f.dim() != 2 or not f.is_contiguous():
+                raise ValueError("frames must be contiguous 2-D device tensors")
+            c = N.GT_U16 if f.dtype in (torch.int16, torch.uint16) else (
+                N.GT_U8 if f.dtype == torch.uint8 else None)
+            if c is None:
+                raise TypeError("batched frames must be u8 or u16 (int16 tensors read as u16)")
+            if code is not None and c != code:
+                raise TypeError("all frames of a batch must have one pixel type")
+            code = c
+            descs[i].data = f.data_ptr()
+            descs[i].nrows, descs[i].ncols = f.shape
+            descs[i].frame = int(frame_ids[i]) if frame_ids is not None else i
+        return descs, code
+
+    def detect(self, frames, frame_ids=None, out=None, counts=None,
+               cap_per_frame: int = 8192, stream=None):
+        """frames: list of 2-D device tensors.  Returns (rows (n, cap, 12) int32
+        words, counts (n,) int32); rows of frames[i] at out[i]."""
+        import torch
+        descs, code = self._descs(frames, frame_ids)
+        n = len(frames)
+        dev = frames[0].device
+        if out is None:
+            out = torch.empty((n, cap_per_frame, 12), dtype=torch.int32, device=dev)
+        if counts is None:
+            counts = torch.zeros((n,), dtype=torch.int32, device=dev)
+        if stream is None:
+            stream = torch.cuda.current_stream(dev).cuda_stream
+        N.check(self._lib.gt_batch_detect(self._h, descs, n, code, out.data_ptr(), out.shape[1],
+                                          counts.data_ptr(), stream))
+        return out, counts
+
+    def tune(self, frames, cap_per_frame: int = 8192) -> float:
+        """Autotunes the persistent grid sizes on sample frames; returns the
+        best time (ms) of one pass over them."""
+        import torch
+        descs, code = self._descs(frames, None)
+        n = len(frames)
+        out = torch.empty((n, cap_per_frame, 12), dtype=torch.int32, device=frames[0].device)
+        counts = torch.zeros((n,), dtype=torch.int32, device=frames[0].device)
+        torch.cuda.synchronize()
+        ms = C.c_float()
+        N.check(self._lib.gt_batch_tune(self._h, descs, n, code, out.data_ptr(), cap_per_frame,
+                                        counts.data_ptr(), C.byref(ms)))
+        return float(ms.value)
+
+    def schedule(self) -> dict:
+        a = (C.c_uint32 * 4)()
+        launches = C.c_uint64()
+        N.check(self._lib.gt_batch_get_schedule(self._h, a, C.byref(launches)))
+        return {"bandpass_blocks": a[0], "detect_blocks": a[1], "walk_blocks": a[2],
+                "slot_stride": a[3], "launches_last_call": launches.value}
+
+    def set_graph(self, enable: bool) -> None:
+        N.check(self._lib.gt_batch_set_graph(self._h, int(bool(enable))))

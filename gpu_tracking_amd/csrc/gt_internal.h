@@ -58,5 +58,19 @@ hipError_t gt_launch_walk_log(const float *proc, const void *raw, int dtype, uin
 // Per-frame detection counts, 0xFFFFFFFF where the candidate list overflowed.
 hipError_t gt_launch_copy_counts_checked(const uint32_t *ocount, const uint32_t *ccount,
                                          uint32_t ccap, uint32_t *dst, uint32_t n, hipStream_t s);
+// Batched launches over frames of different sizes (u8/u16 frames, R <= 7).
+// Grid sizes of the persistent stages; the autotuner picks them.
+struct GtBatchSchedule {
+    uint32_t bp_blocks;    // persistent bandpass blocks (multiple of 8)
+    uint32_t dt_blocks;    // persistent detect-tile blocks (multiple of 8)
+    uint32_t walk_blocks;  // blocks per slot for slow walks and characterize
+    uint32_t slot_blocks;  // slot stride of those kernels
+};
+hipError_t gt_launch_batch(const void *unused, int dtype, const GtKParams &p,
+                           const GtBatchHeader *hdr, const GtBatchFrame *fs, uint32_t max_slots,
+                           float *proc, gt_detection *out, uint32_t out_cap, uint32_t *ocount,
+                           uint2 *slow, uint32_t slow_cap, uint32_t *scount, uint32_t *counts,
+                           const GtBatchSchedule &sch, hipStream_t s);
+void gt_batch_tiles(int H, int W, uint32_t *bp_tiles, uint32_t *dt_tiles);
 // Fills the constant tables the kernels read (once per device).
 hipError_t gt_init_tables();

@@ -186,33 +186,34 @@ __device__ __forceinline__ float div_const(float a, float b, float rb) {
     return fmaf(r, rb, q);
 }
 
+template <int R>
+struct BpLds {
+    static constexpr int RH = B2_TH + 2 * R;
+    static constexpr int RWP = (B2_TW + 2 * R + 3) & ~3;   // staged row pitch, 16-byte aligned
+    float Sraw[RH * RWP];
+    float Sg[RH * B2_TW];
+    float Sc[RH * B2_TW];
+};
+
+// One 64 x 32 output tile of one frame (f, out: the frame's input and
+// processed planes, p.H x p.W).
 template <typename T, int R>
-__global__ __launch_bounds__(B2_THREADS) void k_bandpass_int(const T *__restrict__ frames,
-                                                             float *__restrict__ proc,
-                                                             GtKParams p, int nframes) {
+__device__ __forceinline__ void bandpass_int_tile(const T *__restrict__ f, float *__restrict__ out,
+                                                  const GtKParams &p, int x0, int y0,
+                                                  BpLds<R> &L) {
     constexpr int NT = 2 * R + 1;
-    constexpr int RH = B2_TH + 2 * R;
-    constexpr int RWP = (B2_TW + 2 * R + 3) & ~3;   // staged row pitch, 16-byte aligned
+    constexpr int RH = BpLds<R>::RH;
+    constexpr int RWP = BpLds<R>::RWP;
     constexpr int NV = (NT + 3 + 3) / 4;            // float4 reads per row-pass item
     static_assert(B2_TW - 4 + 4 * NV <= RWP, "row-pass reads stay inside the staged row");
     static_assert(B2_TH % B2_ROWS == 0 && (B2_TH / B2_ROWS) * B2_TW == B2_THREADS, "mapping");
-    __shared__ __align__(16) float Sraw[RH * RWP];
-    __shared__ __align__(16) float Sg[RH * B2_TW];
-    __shared__ __align__(16) float Sc[RH * B2_TW];
+    float *Sraw = L.Sraw, *Sg = L.Sg, *Sc = L.Sc;
     const int H = p.H, W = p.W;
-    const size_t HW = static_cast<size_t>(H) * W;
-    const int ntx = (W + B2_TW - 1) / B2_TW, nty = (H + B2_TH - 1) / B2_TH;
-    const int tile = xcd_tile(ntx * nty * nframes);
-    if (tile >= ntx * nty * nframes) return;
-    const int fr = tile / (ntx * nty), trest = tile - fr * (ntx * nty);
-    const int x0 = (trest % ntx) * B2_TW, y0 = (trest / ntx) * B2_TH;
-    const T *f = frames + static_cast<size_t>(fr) * HW;
-    float *out = proc + static_cast<size_t>(fr) * HW;
     const int tid = threadIdx.x;
     constexpr int PAIRS = (B2_TW + 2 * R + 1) / 2;  // u16 pairs per staged row
     static_assert(2 * PAIRS <= RWP, "pairs fit the staged row");
     if (sizeof(T) == 2 && (R % 2) == 0 && (W % 2) == 0 &&
-        (reinterpret_cast<uintptr_t>(frames) & 3u) == 0 && y0 - R >= 0 && y0 + B2_TH + R <= H &&
+        (reinterpret_cast<uintptr_t>(f) & 3u) == 0 && y0 - R >= 0 && y0 + B2_TH + R <= H &&
         x0 - R >= 0 && x0 - R + 2 * PAIRS <= W) {
         // interior tile of a u16 frame: the staged region is whole in the
         // frame and starts on an even column, so rows load as u32 pairs
@@ -340,6 +341,22 @@ __global__ __launch_bounds__(B2_THREADS) void k_bandpass_int(const T *__restrict
         const int gy = gy0 + r;
         if (gy < H && gx < W) out[static_cast<size_t>(gy) * W + gx] = res;
     }
+}
+
+template <typename T, int R>
+__global__ __launch_bounds__(B2_THREADS) void k_bandpass_int(const T *__restrict__ frames,
+                                                             float *__restrict__ proc,
+                                                             GtKParams p, int nframes) {
+    __shared__ __align__(16) BpLds<R> L;
+    const int H = p.H, W = p.W;
+    const size_t HW = static_cast<size_t>(H) * W;
+    const int ntx = (W + B2_TW - 1) / B2_TW, nty = (H + B2_TH - 1) / B2_TH;
+    const int tile = xcd_tile(ntx * nty * nframes);
+    if (tile >= ntx * nty * nframes) return;
+    const int fr = tile / (ntx * nty), trest = tile - fr * (ntx * nty);
+    const int x0 = (trest % ntx) * B2_TW, y0 = (trest / ntx) * B2_TH;
+    bandpass_int_tile<T, R>(frames + static_cast<size_t>(fr) * HW,
+                            proc + static_cast<size_t>(fr) * HW, p, x0, y0, L);
 }
 
 // ---------------------------------------------------------------------------
@@ -836,18 +853,19 @@ __device__ __forceinline__ bool finish_walk(const GtKParams &p, const float *__r
     return true;
 }
 
+// One FT_TH x FT_TW tile (origin x0, y0) of one frame: P, Rw are its
+// processed and raw planes (p.H x p.W); its rows go to fout (counted in
+// *counter), walks handed over to fslow (counted in *slow_counter); `frame`
+// is the rows' frame field.
 template <typename T, int KT, int RI>
-__global__ __launch_bounds__(FT_THREADS, FT_MIN_WAVES) void k_detect_tile(
-    const float *__restrict__ proc, const T *__restrict__ raw, GtKParams p, TileGeom g_rt,
-    gt_detection *__restrict__ out, uint32_t out_cap, uint32_t *__restrict__ ocount,
-    uint2 *__restrict__ slow, uint32_t slow_cap, uint32_t *__restrict__ scount, uint32_t frame0,
-    int nframes) {
+__device__ __forceinline__ void detect_tile_body(
+    const float *__restrict__ P, const T *__restrict__ Rw, const GtKParams &p, const TileGeom &g,
+    int x0, int y0, uint32_t frame, gt_detection *__restrict__ fout, uint32_t out_cap,
+    uint32_t *__restrict__ counter, uint2 *__restrict__ fslow, uint32_t slow_cap,
+    uint32_t *__restrict__ slow_counter, float *lds, uint32_t *s_cnt,
+    uint64_t (*s_masks)[FT_WROWS]) {
     constexpr bool kFixed = KT > 0 && RI > 0;
     constexpr TileGeom gc = tile_geom(KT > 0 ? KT : 1, RI > 0 ? RI : 1);
-    const TileGeom g = kFixed ? gc : g_rt;
-    extern __shared__ __align__(16) float lds[];
-    __shared__ uint32_t s_cnt[FT_WAVES];
-    __shared__ uint64_t s_masks[FT_WAVES][FT_WROWS];
     constexpr bool kInt = sizeof(T) <= 2;  // integer frames: no NaN in the processed frame
     const int k = KT > 0 ? KT : p.k;
     const int a = g.a;
@@ -858,14 +876,6 @@ __global__ __launch_bounds__(FT_THREADS, FT_MIN_WAVES) void k_detect_tile(
     float *Hm = lds + RH * RW;         // MR x FT_TW; later the candidate list
     uint16_t *s_cand = reinterpret_cast<uint16_t *>(Hm);
     const int H = p.H, W = p.W;
-    const size_t HW = static_cast<size_t>(H) * W;
-    const int ntx = (W + FT_TW - 1) / FT_TW, nty = (H + FT_TH - 1) / FT_TH;
-    const int tile = xcd_tile(ntx * nty * nframes);
-    if (tile >= ntx * nty * nframes) return;
-    const int fr = tile / (ntx * nty), trest = tile - fr * (ntx * nty);
-    const int x0 = (trest % ntx) * FT_TW, y0 = (trest / ntx) * FT_TH;
-    const float *P = proc + static_cast<size_t>(fr) * HW;
-    const T *Rw = raw + static_cast<size_t>(fr) * HW;
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int oy = y0 - g.ht, ox = x0 - g.hl;
 
@@ -1021,11 +1031,6 @@ __global__ __launch_bounds__(FT_THREADS, FT_MIN_WAVES) void k_detect_tile(
     // 4-5. walks
     const int ylo = oy + rint, yhi = oy + RH - 1 - rint;
     const int xlo = ox + rint, xhi = ox + g.width - 1 - rint;
-    const uint32_t frame = frame0 + fr;
-    uint32_t *counter = &ocount[fr * GT_COUNTER_STRIDE];
-    uint32_t *slow_counter = &scount[fr * GT_COUNTER_STRIDE];
-    uint2 *fslow = slow + static_cast<size_t>(fr) * slow_cap;
-    gt_detection *fout = out + static_cast<size_t>(fr) * out_cap;
     constexpr int PITCH = kFixed ? gc.pitch : 0;
     for (uint32_t c0 = static_cast<uint32_t>(wave) * 64u; c0 < ncand; c0 += FT_THREADS) {
         const uint32_t ci = c0 + static_cast<uint32_t>(lane);
@@ -1071,6 +1076,32 @@ __global__ __launch_bounds__(FT_THREADS, FT_MIN_WAVES) void k_detect_tile(
         const uint32_t pos = wave_append(keep, counter);
         if (keep && pos < out_cap) fout[pos] = d;
     }
+}
+
+template <typename T, int KT, int RI>
+__global__ __launch_bounds__(FT_THREADS, FT_MIN_WAVES) void k_detect_tile(
+    const float *__restrict__ proc, const T *__restrict__ raw, GtKParams p, TileGeom g_rt,
+    gt_detection *__restrict__ out, uint32_t out_cap, uint32_t *__restrict__ ocount,
+    uint2 *__restrict__ slow, uint32_t slow_cap, uint32_t *__restrict__ scount, uint32_t frame0,
+    int nframes) {
+    constexpr bool kFixed = KT > 0 && RI > 0;
+    constexpr TileGeom gc = tile_geom(KT > 0 ? KT : 1, RI > 0 ? RI : 1);
+    const TileGeom g = kFixed ? gc : g_rt;
+    extern __shared__ __align__(16) float lds[];
+    __shared__ uint32_t s_cnt[FT_WAVES];
+    __shared__ uint64_t s_masks[FT_WAVES][FT_WROWS];
+    const int H = p.H, W = p.W;
+    const size_t HW = static_cast<size_t>(H) * W;
+    const int ntx = (W + FT_TW - 1) / FT_TW, nty = (H + FT_TH - 1) / FT_TH;
+    const int tile = xcd_tile(ntx * nty * nframes);
+    if (tile >= ntx * nty * nframes) return;
+    const int fr = tile / (ntx * nty), trest = tile - fr * (ntx * nty);
+    const int x0 = (trest % ntx) * FT_TW, y0 = (trest / ntx) * FT_TH;
+    detect_tile_body<T, KT, RI>(proc + static_cast<size_t>(fr) * HW, raw + static_cast<size_t>(fr) * HW,
+                                p, g, x0, y0, frame0 + fr, out + static_cast<size_t>(fr) * out_cap,
+                                out_cap, &ocount[fr * GT_COUNTER_STRIDE],
+                                slow + static_cast<size_t>(fr) * slow_cap, slow_cap,
+                                &scount[fr * GT_COUNTER_STRIDE], lds, s_cnt, s_masks);
 }
 
 // Continues the walks k_detect_tile handed over: each entry is the seed and
@@ -1876,4 +1907,266 @@ hipError_t gt_launch_copy_counts_checked(const uint32_t *ocount, const uint32_t 
                                          uint32_t ccap, uint32_t *dst, uint32_t n, hipStream_t s) {
     k_copy_counts_checked<<<(n + 255) / 256, 256, 0, s>>>(ocount, ccount, ccap, dst, n);
     return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Batched launches: the frames of a launch may differ in size.  A table of
+// GtBatchFrame slots (device memory, read through a pointer so a captured
+// graph can be replayed for another launch) gives each slot's frame address,
+// size, processed-plane offset and first tile; every stage is one persistent
+// launch whose blocks walk the launch's tiles in XCD-contiguous ranges and
+// run the same per-tile bodies as the one-size kernels.
+namespace {
+
+// Slot of tile t: the last slot whose first tile is <= t.
+template <bool kDetect>
+__device__ __forceinline__ int batch_slot(const GtBatchFrame *__restrict__ fs, int n, uint32_t t) {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        const uint32_t t0 = kDetect ? fs[mid].dt_tile0 : fs[mid].bp_tile0;
+        if (t0 <= t) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// Tiles of a persistent block: XCD x = blockIdx % 8 owns the contiguous range
+// [x * per, (x + 1) * per) and its blocks take every nb-th tile of it.
+struct PersistentTiles {
+    uint32_t t, end, step;
+    __device__ PersistentTiles(uint32_t total) {
+        const uint32_t nb = gridDim.x / kXcds;
+        const uint32_t per = (total + kXcds - 1) / kXcds;
+        const uint32_t x = blockIdx.x % kXcds, j = blockIdx.x / kXcds;
+        t = x * per + j;
+        end = min((x + 1) * per, total);
+        step = nb;
+    }
+};
+
+template <typename T, int R>
+__global__ __launch_bounds__(B2_THREADS) void k_bandpass_int_batch(
+    float *__restrict__ proc, GtKParams p, const GtBatchHeader *__restrict__ hdr,
+    const GtBatchFrame *__restrict__ fs) {
+    __shared__ __align__(16) BpLds<R> L;
+    const int n = static_cast<int>(hdr->nslots);
+    for (PersistentTiles it(hdr->bp_tiles); it.t < it.end; it.t += it.step) {
+        const int slot = batch_slot<false>(fs, n, it.t);
+        const GtBatchFrame e = fs[slot];
+        GtKParams q = p;
+        q.H = e.H;
+        q.W = e.W;
+        const int ntx = (e.W + B2_TW - 1) / B2_TW;
+        const int local = static_cast<int>(it.t - e.bp_tile0);
+        const int x0 = (local % ntx) * B2_TW, y0 = (local / ntx) * B2_TH;
+        bandpass_int_tile<T, R>(reinterpret_cast<const T *>(e.in_ptr), proc + e.proc_off, q, x0,
+                                y0, L);
+        __syncthreads();
+    }
+}
+
+template <typename T, int KT, int RI>
+__global__ __launch_bounds__(FT_THREADS, FT_MIN_WAVES) void k_detect_tile_batch(
+    const float *__restrict__ proc, GtKParams p, TileGeom g_rt,
+    const GtBatchHeader *__restrict__ hdr, const GtBatchFrame *__restrict__ fs,
+    gt_detection *__restrict__ out, uint32_t out_cap, uint32_t *__restrict__ ocount,
+    uint2 *__restrict__ slow, uint32_t slow_cap, uint32_t *__restrict__ scount) {
+    constexpr bool kFixed = KT > 0 && RI > 0;
+    constexpr TileGeom gc = tile_geom(KT > 0 ? KT : 1, RI > 0 ? RI : 1);
+    const TileGeom g = kFixed ? gc : g_rt;
+    extern __shared__ __align__(16) float lds[];
+    __shared__ uint32_t s_cnt[FT_WAVES];
+    __shared__ uint64_t s_masks[FT_WAVES][FT_WROWS];
+    const int n = static_cast<int>(hdr->nslots);
+    for (PersistentTiles it(hdr->dt_tiles); it.t < it.end; it.t += it.step) {
+        const int slot = batch_slot<true>(fs, n, it.t);
+        const GtBatchFrame e = fs[slot];
+        GtKParams q = p;
+        q.H = e.H;
+        q.W = e.W;
+        const int ntx = (e.W + FT_TW - 1) / FT_TW;
+        const int local = static_cast<int>(it.t - e.dt_tile0);
+        const int x0 = (local % ntx) * FT_TW, y0 = (local / ntx) * FT_TH;
+        // rows go to the output slot; counters and slow lists are per launch slot
+        detect_tile_body<T, KT, RI>(proc + e.proc_off, reinterpret_cast<const T *>(e.in_ptr), q, g,
+                                    x0, y0, e.frame, out + static_cast<size_t>(e.out_idx) * out_cap,
+                                    out_cap, &ocount[slot * GT_COUNTER_STRIDE],
+                                    slow + static_cast<size_t>(slot) * slow_cap, slow_cap,
+                                    &scount[slot * GT_COUNTER_STRIDE], lds, s_cnt, s_masks);
+        __syncthreads();
+    }
+}
+
+// Slow walks and characterize: slots over blockIdx.y (strided), detections
+// over blockIdx.x.
+template <typename T, int RI>
+__global__ __launch_bounds__(WK_THREADS) void k_walk_slow_batch(
+    const float *__restrict__ proc, GtKParams p, const GtBatchHeader *__restrict__ hdr,
+    const GtBatchFrame *__restrict__ fs, const uint2 *__restrict__ slow, uint32_t slow_cap,
+    const uint32_t *__restrict__ scount, gt_detection *__restrict__ out, uint32_t out_cap,
+    uint32_t *__restrict__ ocount) {
+    const int nslots = static_cast<int>(hdr->nslots);
+    const uint32_t lane = __lane_id();
+    const uint32_t stride = gridDim.x * WK_THREADS;
+    for (int slot = blockIdx.y; slot < nslots; slot += gridDim.y) {
+        const GtBatchFrame e = fs[slot];
+        GtKParams q = p;
+        q.H = e.H;
+        q.W = e.W;
+        const int H = e.H, W = e.W;
+        const float *P = proc + e.proc_off;
+        const T *Rw = reinterpret_cast<const T *>(e.in_ptr);
+        const uint32_t n = min(scount[slot * GT_COUNTER_STRIDE], slow_cap);
+        const uint2 *fsl = slow + static_cast<size_t>(slot) * slow_cap;
+        gt_detection *fout = out + static_cast<size_t>(e.out_idx) * out_cap;
+        uint32_t *counter = &ocount[slot * GT_COUNTER_STRIDE];
+        const int rint = RI > 0 ? RI : q.rint;
+        for (uint32_t i0 = blockIdx.x * WK_THREADS + (threadIdx.x & ~63u); i0 < n; i0 += stride) {
+            const uint32_t i = i0 + lane;
+            bool keep = false;
+            gt_detection d;
+            if (i < n) {
+                const uint2 en = fsl[i];
+                int pu = static_cast<int>(en.y / static_cast<uint32_t>(W));
+                int pv = static_cast<int>(en.y - static_cast<uint32_t>(pu) * static_cast<uint32_t>(W));
+                WalkRes res = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+                for (uint32_t itr = FT_WALK_STEPS; itr < q.max_iter; ++itr) {
+                    res = walk_eval<RI, 0>(q, P, nullptr, 0, 0, 0, 1, 0, 1, 0, pu, pv);
+                    if (!walk_move(res, pu, pv, H, W, rint, q.shift_thr)) break;
+                }
+                keep = finish_walk<T, RI, 0, false>(q, P, Rw, nullptr, 0, 0, 0, 1, 0, 1, 0, res, pu,
+                                                    pv, en.x, e.frame, d);
+            }
+            const uint32_t pos = wave_append(keep, counter);
+            if (keep && pos < out_cap) fout[pos] = d;
+        }
+    }
+}
+
+template <typename T, int RI>
+__global__ __launch_bounds__(WK_THREADS) void k_characterize_det_batch(
+    const float *__restrict__ proc, GtKParams p, const GtBatchHeader *__restrict__ hdr,
+    const GtBatchFrame *__restrict__ fs, gt_detection *__restrict__ out, uint32_t out_cap,
+    const uint32_t *__restrict__ ocount) {
+    const int nslots = static_cast<int>(hdr->nslots);
+    for (int slot = blockIdx.y; slot < nslots; slot += gridDim.y) {
+        const GtBatchFrame e = fs[slot];
+        const int H = e.H, W = e.W;
+        const GlobalSrc src{proc + e.proc_off, W};
+        const T *Rw = reinterpret_cast<const T *>(e.in_ptr);
+        const uint32_t n = min(ocount[slot * GT_COUNTER_STRIDE], out_cap);
+        gt_detection *rows = out + static_cast<size_t>(e.out_idx) * out_cap;
+        for (uint32_t i = blockIdx.x * WK_THREADS + threadIdx.x; i < n; i += gridDim.x * WK_THREADS) {
+            gt_result_row row = rows[i].row;
+            if constexpr (RI > 0) {
+                const int cu = f2i(rintf(row.x)), cv = f2i(rintf(row.y));
+                if (row.r <= 0.0f && p.rint == RI && cu >= RI && cu + RI < H && cv >= RI &&
+                    cv + RI < W) {
+                    const size_t c = static_cast<size_t>(cu) * W + cv;
+                    characterize_inner<T, RI>(GlobalRow{src.p + c, W}, Rw + c, W, row);
+                    rows[i].row = row;
+                    continue;
+                }
+            }
+            characterize_row<T, RI, true>(src, Rw, H, W, p.circle, row);
+            rows[i].row = row;
+        }
+    }
+}
+
+__global__ void k_copy_counts_batch(const uint32_t *__restrict__ strided,
+                                    const GtBatchHeader *__restrict__ hdr,
+                                    const GtBatchFrame *__restrict__ fs, uint32_t *__restrict__ dst) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < hdr->nslots) dst[fs[i].out_idx] = strided[i * GT_COUNTER_STRIDE];
+}
+
+template <typename T>
+hipError_t launch_batch_t(const GtKParams &p, const GtBatchHeader *hdr, const GtBatchFrame *fs,
+                          uint32_t max_slots, float *proc, gt_detection *out, uint32_t out_cap,
+                          uint32_t *ocount, uint2 *slow, uint32_t slow_cap, uint32_t *scount,
+                          uint32_t *counts, const GtBatchSchedule &sch, hipStream_t s) {
+    if (p.R < 1 || p.R > 7 || !p.fast_div) return hipErrorInvalidValue;
+    hipError_t e = gt_launch_zero_u32(ocount, static_cast<size_t>(max_slots) * GT_COUNTER_STRIDE, s);
+    if (e != hipSuccess) return e;
+    e = gt_launch_zero_u32(scount, static_cast<size_t>(max_slots) * GT_COUNTER_STRIDE, s);
+    if (e != hipSuccess) return e;
+    const dim3 gbp(sch.bp_blocks);
+    switch (p.R) {
+    case 1: k_bandpass_int_batch<T, 1><<<gbp, B2_THREADS, 0, s>>>(proc, p, hdr, fs); break;
+    case 2: k_bandpass_int_batch<T, 2><<<gbp, B2_THREADS, 0, s>>>(proc, p, hdr, fs); break;
+    case 3: k_bandpass_int_batch<T, 3><<<gbp, B2_THREADS, 0, s>>>(proc, p, hdr, fs); break;
+    case 4: k_bandpass_int_batch<T, 4><<<gbp, B2_THREADS, 0, s>>>(proc, p, hdr, fs); break;
+    case 5: k_bandpass_int_batch<T, 5><<<gbp, B2_THREADS, 0, s>>>(proc, p, hdr, fs); break;
+    case 6: k_bandpass_int_batch<T, 6><<<gbp, B2_THREADS, 0, s>>>(proc, p, hdr, fs); break;
+    default: k_bandpass_int_batch<T, 7><<<gbp, B2_THREADS, 0, s>>>(proc, p, hdr, fs); break;
+    }
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    GtKParams kw = p;
+    kw.characterize = 0;
+    const TileGeom g = tile_geom(p.k, p.rint);
+    const size_t lds = sizeof(float) * (static_cast<size_t>(g.rows) * g.pitch +
+                                        static_cast<size_t>(FT_TH + p.k - 1) * FT_TW);
+    const dim3 gdt(sch.dt_blocks);
+#define GT_TILEB(KV, RV)                                                                     \
+    k_detect_tile_batch<T, KV, RV><<<gdt, FT_THREADS, lds, s>>>(proc, kw, g, hdr, fs, out,      \
+                                                                out_cap, ocount, slow, slow_cap, \
+                                                                scount)
+    if (p.rint == 4 && p.k == 9) GT_TILEB(9, 4);
+    else if (p.rint == 4 && p.k == 12) GT_TILEB(12, 4);
+    else if (p.rint == 4) GT_TILEB(0, 4);
+    else GT_TILEB(0, 0);
+#undef GT_TILEB
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    const dim3 gw(sch.walk_blocks, sch.slot_blocks);
+    if (p.rint == 4)
+        k_walk_slow_batch<T, 4><<<gw, WK_THREADS, 0, s>>>(proc, kw, hdr, fs, slow, slow_cap, scount,
+                                                          out, out_cap, ocount);
+    else
+        k_walk_slow_batch<T, 0><<<gw, WK_THREADS, 0, s>>>(proc, kw, hdr, fs, slow, slow_cap, scount,
+                                                          out, out_cap, ocount);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    if (p.characterize) {
+        if (p.rint == 4)
+            k_characterize_det_batch<T, 4><<<gw, WK_THREADS, 0, s>>>(proc, p, hdr, fs, out, out_cap,
+                                                                     ocount);
+        else
+            k_characterize_det_batch<T, 0><<<gw, WK_THREADS, 0, s>>>(proc, p, hdr, fs, out, out_cap,
+                                                                     ocount);
+        e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    k_copy_counts_batch<<<(max_slots + 255) / 256, 256, 0, s>>>(ocount, hdr, fs, counts);
+    return hipGetLastError();
+}
+
+}  // namespace
+
+size_t gt_batch_tile_lds_bytes(const GtKParams &p) { return gt_tile_lds_bytes(p); }
+
+hipError_t gt_launch_batch(const void *unused, int dtype, const GtKParams &p,
+                           const GtBatchHeader *hdr, const GtBatchFrame *fs, uint32_t max_slots,
+                           float *proc, gt_detection *out, uint32_t out_cap, uint32_t *ocount,
+                           uint2 *slow, uint32_t slow_cap, uint32_t *scount, uint32_t *counts,
+                           const GtBatchSchedule &sch, hipStream_t s) {
+    (void)unused;
+    switch (dtype) {
+    case GT_U16:
+        return launch_batch_t<uint16_t>(p, hdr, fs, max_slots, proc, out, out_cap, ocount, slow,
+                                        slow_cap, scount, counts, sch, s);
+    case GT_U8:
+        return launch_batch_t<uint8_t>(p, hdr, fs, max_slots, proc, out, out_cap, ocount, slow,
+                                       slow_cap, scount, counts, sch, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+// Tile counts of one frame in the batched kernels.
+void gt_batch_tiles(int H, int W, uint32_t *bp_tiles, uint32_t *dt_tiles) {
+    *bp_tiles = static_cast<uint32_t>(((W + B2_TW - 1) / B2_TW) * ((H + B2_TH - 1) / B2_TH));
+    *dt_tiles = static_cast<uint32_t>(((W + FT_TW - 1) / FT_TW) * ((H + FT_TH - 1) / FT_TH));
 }

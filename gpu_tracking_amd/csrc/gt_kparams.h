@@ -58,3 +58,21 @@ struct GtLogTable {
     float n3[GT_LOG_MAX_SCALES];       // 1 / (sqrt(2 pi) sigma sigma^2)
     float taps[GT_LOG_MAX_TAPS];
 };
+
+// Batched launches: frames of different sizes in one launch.  One entry per
+// frame slot of a launch; tile ranges are prefix sums over the slots.
+struct GtBatchFrame {
+    uint64_t in_ptr;      // device address of the frame (dtype pixels, H x W)
+    uint64_t proc_off;    // element offset of its processed plane in the scratch pool
+    int32_t H, W;
+    uint32_t bp_tile0;    // first bandpass tile of the slot in the launch
+    uint32_t dt_tile0;    // first detect tile
+    uint32_t out_idx;     // output slot: rows at out[out_idx * cap], count at counts[out_idx]
+    uint32_t frame;       // the frame field of its detections
+};
+struct GtBatchHeader {
+    uint32_t nslots;
+    uint32_t bp_tiles;    // total bandpass tiles of the launch
+    uint32_t dt_tiles;    // total detect tiles
+    uint32_t pad;
+};

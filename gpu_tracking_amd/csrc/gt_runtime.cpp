@@ -38,6 +38,21 @@ gt_status fail(gt_status st, const char *fmt, ...) {
     return st;
 }
 
+}  // namespace
+
+// Shared with gt_batch.cpp: records the message for gt_last_error.
+gt_status gt_runtime_fail(gt_status st, const char *fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+    return st;
+}
+
+namespace {
+
 #define GT_HIP(call)                                                                       \
     do {                                                                                   \
         hipError_t e_ = (call);                                                            \

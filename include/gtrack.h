@@ -225,6 +225,43 @@ gt_status gt_prune_blobs(gt_result_row *rows, uint64_t n, float overlap_threshol
 gt_status gt_link(const float *points, uint64_t n, float search_range, uint32_t memory,
                   uint64_t *ids);
 
+/* ---- Batched detection over frames of different sizes ------------------
+ * No counterpart in the reference: one GpuState serves one frame size
+ * (src/execute_gpu.rs:1074-1080).  A batch context holds the parameters
+ * (Trackpy style, u8/u16 frames) and scratch for launches of up to
+ * max_launch_pixels pixels in up to max_slots frames; gt_batch_detect runs
+ * any mix of frame sizes through one persistent launch per stage per launch,
+ * replaying a HIP graph of the stage sequence.  Rows of frames[i] land at
+ * d_out[i * cap_per_frame], their count at d_counts[i]; each row's `frame`
+ * field is frames[i].frame.  Results equal gt_detect_device's on the same
+ * frames bit for bit. */
+typedef struct gt_frame_desc {
+    const void *data;      /* device address of the frame (row-major) */
+    uint32_t nrows, ncols;
+    uint32_t frame;        /* frame field of its detections */
+    uint32_t pad;
+} gt_frame_desc;
+
+typedef struct gt_batch gt_batch;
+
+gt_status gt_batch_create(const gt_params *params, uint64_t max_frame_pixels,
+                          uint64_t max_launch_pixels, uint32_t max_slots, gt_batch **out);
+void gt_batch_destroy(gt_batch *b);
+gt_status gt_batch_detect(gt_batch *b, const gt_frame_desc *frames, uint32_t n, int32_t dtype,
+                          gt_detection *d_out, uint32_t cap_per_frame, uint32_t *d_counts,
+                          void *stream);
+/* Autotuner: times candidate persistent grid sizes on the sample frames
+ * (coordinate search, warm-up + one timed run each) and keeps the fastest
+ * schedule for later calls; *best_ms receives its time. */
+gt_status gt_batch_tune(gt_batch *b, const gt_frame_desc *frames, uint32_t n, int32_t dtype,
+                        gt_detection *d_out, uint32_t cap_per_frame, uint32_t *d_counts,
+                        float *best_ms);
+/* The schedule in use (bandpass blocks, detect blocks, walk blocks per slot,
+ * slot stride) and the number of launches of the last call. */
+gt_status gt_batch_get_schedule(gt_batch *b, uint32_t *sched4, uint64_t *launches);
+/* Graph replay on (default) or direct launches. */
+gt_status gt_batch_set_graph(gt_batch *b, int32_t enable);
+
 /* Interruption and progress (the Arc<AtomicBool> / Arc<Mutex<(usize,
  * Option<usize>)>> pair of execute_gpu, src/execute_gpu.rs:1094-1099,
  * :974-977).  Safe to call from any thread while gt_execute runs. */

@@ -1,0 +1,127 @@
+"""Batched detection of frames of different sizes (persistent kernels over a
+frame table, graph replay, autotuned grids) against separate one-size runs:
+rows and counts must be identical bit for bit, per frame."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def torch_dev(gpu_ok):
+    import torch
+    return torch
+
+
+SIZES = [(96, 128), (200, 150), (256, 256), (131, 77), (64, 64)]
+PARAMS = dict(separation=7, minmass=400.0, characterize=True)
+
+
+def _videos(torch, nframes=3):
+    from gpu_tracking_amd import synth
+    vids = []
+    for k, (H, W) in enumerate(SIZES):
+        s = synth.spec("full", nrows=H, ncols=W, n_particles=max(4, H * W // 1000))
+        v = synth.video_numpy(s, nframes, seed=400 + k)
+        vids.append(torch.from_numpy(v.view(np.int16)).cuda())
+    return vids
+
+
+def _separate(torch, vids, cap):
+    import gpu_tracking_amd as gt
+    res = []
+    for v in vids:
+        with gt.Context(gt.make_params(9, **PARAMS), v.shape[1], v.shape[2]) as ctx:
+            out, counts = ctx.detect_device(v, cap_per_frame=cap)
+            torch.cuda.synchronize()
+            res.append((out.cpu().numpy(), counts.cpu().numpy()))
+    return res
+
+
+def _frames(vids):
+    frames, ids, where = [], [], []
+    T = vids[0].shape[0]
+    for t in range(T):                        # interleave videos: mixed sizes per launch
+        for k, v in enumerate(vids):
+            frames.append(v[t])
+            ids.append(t)
+            where.append((k, t))
+    return frames, ids, where
+
+
+def _check(out, counts, sep, where):
+    o = out.cpu().numpy()
+    c = counts.cpu().numpy()
+    for i, (k, t) in enumerate(where):
+        ro, rc = sep[k]
+        assert c[i] == rc[t], f"frame {i} (video {k}, t {t}): count {c[i]} vs {rc[t]}"
+        # rows arrive in atomic order: compare them ordered by seed (word 10)
+        a = o[i, :c[i]].view(np.uint32).reshape(c[i], 12)
+        b = ro[t, :rc[t]].view(np.uint32).reshape(rc[t], 12)
+        a = a[np.argsort(a[:, 10], kind="stable")]
+        b = b[np.argsort(b[:, 10], kind="stable")]
+        assert np.array_equal(a, b), f"frame {i} (video {k}, t {t}) rows differ"
+
+
+@pytest.mark.parametrize("graph,launch_px,slots", [(True, 1 << 28, 256), (False, 1 << 28, 256),
+                                                   (True, 100_000, 4)])
+def test_batch_matches_separate_runs(torch_dev, graph, launch_px, slots):
+    import gpu_tracking_amd as gt
+    vids = _videos(torch_dev)
+    cap = 2048
+    sep = _separate(torch_dev, vids, cap)
+    frames, ids, where = _frames(vids)
+    with gt.Batch(gt.make_params(9, **PARAMS), max_frame_pixels=256 * 256,
+                  max_launch_pixels=launch_px, max_slots=slots) as b:
+        b.set_graph(graph)
+        out, counts = b.detect(frames, frame_ids=ids, cap_per_frame=cap)
+        torch_dev.cuda.synchronize()
+        _check(out, counts, sep, where)
+        if launch_px < (1 << 28):
+            assert b.schedule()["launches_last_call"] > 1
+
+
+def test_batch_graph_replays_stay_exact(torch_dev):
+    # several calls (each capturing and replaying a graph) with eager work in
+    # between; every replay must equal the separate runs
+    import gpu_tracking_amd as gt
+    vids = _videos(torch_dev, nframes=2)
+    cap = 2048
+    sep = _separate(torch_dev, vids, cap)
+    frames, ids, where = _frames(vids)
+    with gt.Batch(gt.make_params(9, **PARAMS), max_frame_pixels=256 * 256,
+                  max_launch_pixels=150_000, max_slots=8) as b:
+        for rep in range(3):
+            out, counts = b.detect(frames, frame_ids=ids, cap_per_frame=cap)
+            torch_dev.cuda.synchronize()
+            _check(out, counts, sep, where)
+            junk = torch_dev.ones(1 << 20, device="cuda") * rep    # eager work between replays
+            torch_dev.cuda.synchronize()
+            del junk
+
+
+def test_batch_autotune_keeps_results(torch_dev):
+    import gpu_tracking_amd as gt
+    vids = _videos(torch_dev, nframes=2)
+    cap = 2048
+    sep = _separate(torch_dev, vids, cap)
+    frames, ids, where = _frames(vids)
+    with gt.Batch(gt.make_params(9, **PARAMS), max_frame_pixels=256 * 256) as b:
+        ms = b.tune(frames, cap_per_frame=cap)
+        assert ms > 0
+        sch = b.schedule()
+        assert sch["bandpass_blocks"] % 8 == 0 and sch["detect_blocks"] % 8 == 0
+        out, counts = b.detect(frames, frame_ids=ids, cap_per_frame=cap)
+        torch_dev.cuda.synchronize()
+        _check(out, counts, sep, where)
+
+
+def test_batch_rejects_bad_input(torch_dev):
+    import gpu_tracking_amd as gt
+    with gt.Batch(gt.make_params(9, **PARAMS), max_frame_pixels=64 * 64) as b:
+        big = torch_dev.zeros((128, 128), dtype=torch_dev.int16, device="cuda")
+        with pytest.raises(gt.GtError):
+            b.detect([big])
+        f32 = torch_dev.zeros((32, 32), dtype=torch_dev.float32, device="cuda")
+        with pytest.raises(TypeError):
+            b.detect([f32])
