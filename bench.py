@@ -12,7 +12,15 @@ HIP events around each stage on the launch stream), and the CPU oracle's
 frames/s on a bounded sample of the same frames on the host cores, plus a
 bit-for-bit parity check of that sample.
 
-    python bench.py [--steps K] [--warmup W] [--workload full|small|log]
+    python bench.py [--steps K] [--warmup W] [--workload full|small|log|batch]
+
+Workload "batch": 64 seeded videos, frame size drawn from {256x256, 512x384,
+1024x1024, 2048x1024}, length uniform in [200, 1000] frames (scaled by
+--batch-scale), u16, full-workload generator at 1 particle per 1000 px^2, the
+full workload's parameters; all videos go through one batch context
+(persistent kernels over a frame table, graph replay, autotuned grids) and,
+for comparison, through one one-size context per video; the two must agree
+per frame (counts and an order-independent checksum of every row).
 
 Workload "log": 500 synthetic 2048x2048 float32 frames, 3000 particles per
 frame (sigma ~ U[1, 5], A ~ U[200, 2000], background 50 + N(0, 5^2)), LoG
@@ -155,15 +163,153 @@ def _cpu_baseline_log(video_host, params, budget_s, gpu_rows):
     }, done, mismatched
 
 
+BATCH_SIZES = [(256, 256), (512, 384), (1024, 1024), (2048, 1024)]
+
+
+def _row_checksums(torch, out, counts):
+    """Per frame: sum over its rows of a hash of the row's 12 words (order
+    independent), as int64."""
+    n, cap, _ = out.shape
+    mult = torch.tensor([0x9E3779B1 + 2 * k for k in range(12)], dtype=torch.int64,
+                        device=out.device)
+    res = []
+    for a in range(0, n, 512):
+        w = out[a:a + 512].to(torch.int64) & 0xFFFFFFFF
+        h = ((w * mult) & 0xFFFFFFFFFFFF).sum(dim=2)                   # (chunk, cap)
+        valid = torch.arange(cap, device=out.device)[None, :] < \
+            counts[a:a + 512, None].to(torch.int64)
+        res.append((h * valid).sum(dim=1))
+    return torch.cat(res)
+
+
+def run_batch_workload(args):
+    import numpy as np
+    import torch
+
+    import gpu_tracking_amd as gt
+    from gpu_tracking_amd import synth
+    rng = np.random.default_rng(20261203 + 4000)
+    nvid = 64
+    size_idx = rng.integers(0, len(BATCH_SIZES), nvid)
+    lengths = rng.integers(200, 1001, nvid)
+    lengths = np.maximum(4, np.round(lengths * args.batch_scale).astype(int))
+    params = dict(separation=7, minmass=400.0, characterize=True)
+    p = gt.make_params(9, **params)
+    print(f"[bench] rendering {nvid} videos, {int(lengths.sum())} frames", file=sys.stderr,
+          flush=True)
+    vids = []
+    for v in range(nvid):
+        H, W = BATCH_SIZES[size_idx[v]]
+        sp = synth.spec("full", nrows=H, ncols=W, n_particles=H * W // 1000)
+        vids.append(synth.video_torch(sp, int(lengths[v]), seed=20261203 + 4000 + v,
+                                      device="cuda"))
+        if v % 8 == 7:
+            print(f"[bench] rendered {v + 1} videos", file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    frames = [f for vid in vids for f in vid.unbind(0)]
+    ids = [t for vid in vids for t in range(vid.shape[0])]
+    npx = sum(f.numel() for f in frames)
+    nf = len(frames)
+    cap = 4096
+    out_b = torch.empty((nf, cap, 12), dtype=torch.int32, device="cuda")
+    cnt_b = torch.zeros((nf,), dtype=torch.int32, device="cuda")
+    stream = torch.cuda.current_stream().cuda_stream
+    with gt.Batch(p, max_frame_pixels=2048 * 1024, max_launch_pixels=256 << 20,
+                  max_slots=1024) as b:
+        # autotune on the first 128 Mpx of frames (mixed sizes)
+        acc, k = 0, 0
+        while k < nf and acc < (128 << 20):
+            acc += frames[k].numel()
+            k += 1
+        print(f"[bench] autotuning on {k} frames", file=sys.stderr, flush=True)
+        tuned_ms = b.tune(frames[:k], cap_per_frame=cap)
+        sched = b.schedule()
+        for _ in range(max(1, args.warmup)):
+            b.detect(frames, frame_ids=ids, out=out_b, counts=cnt_b, cap_per_frame=cap,
+                     stream=stream)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            b.detect(frames, frame_ids=ids, out=out_b, counts=cnt_b, cap_per_frame=cap,
+                     stream=stream)
+        torch.cuda.synchronize()
+        dt_b = (time.perf_counter() - t0) / args.steps
+        launches = b.schedule()["launches_last_call"]
+        # the same launches without the graph, for comparison
+        b.set_graph(False)
+        b.detect(frames, frame_ids=ids, out=out_b, counts=cnt_b, cap_per_frame=cap, stream=stream)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            b.detect(frames, frame_ids=ids, out=out_b, counts=cnt_b, cap_per_frame=cap,
+                     stream=stream)
+        torch.cuda.synchronize()
+        dt_direct = (time.perf_counter() - t0) / args.steps
+    print(f"[bench] batched {nf / dt_b:.0f} frames/s ({launches} launches)", file=sys.stderr,
+          flush=True)
+    # separate one-size runs, one context per video
+    out_s = torch.empty((nf, cap, 12), dtype=torch.int32, device="cuda")
+    cnt_s = torch.zeros((nf,), dtype=torch.int32, device="cuda")
+    ctxs = [gt.Context(p, vid.shape[1], vid.shape[2]) for vid in vids]
+    offs = np.concatenate([[0], np.cumsum([vid.shape[0] for vid in vids])])
+
+    def separate():
+        for v, vid in enumerate(vids):
+            ctxs[v].detect_device(vid, out=out_s[offs[v]:offs[v + 1]],
+                                  counts=cnt_s[offs[v]:offs[v + 1]], cap_per_frame=cap,
+                                  stream=stream)
+
+    separate()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        separate()
+    torch.cuda.synchronize()
+    dt_s = (time.perf_counter() - t0) / args.steps
+    for c in ctxs:
+        c.close()
+    if int(cnt_b.max().item()) > cap:
+        raise SystemExit("detection capacity exceeded")
+    same_counts = bool(torch.equal(cnt_b, cnt_s))
+    same_rows = bool(torch.equal(_row_checksums(torch, out_b, cnt_b),
+                                 _row_checksums(torch, out_s, cnt_s)))
+    fps = nf / dt_b
+    achieved = 10.0 * npx / dt_b / 1e9
+    res = {
+        "metric": "frames/s (batched uneven videos, detect+characterize, u16)",
+        "value": round(fps, 1), "unit": "frames/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(dt_b * 1e3, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded, rendered on the GPU)",
+        "config": {"workload": f"batch: {nvid} videos, sizes {BATCH_SIZES}, "
+                               f"{nf} frames, {npx / 1e9:.2f} Gpx, 1 particle / 1000 px^2, "
+                               "diameter=9, separation=7, minmass=400, characterize=True"},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "whole batched pipeline (10 B/px algorithmic)"},
+        "pixels_per_s": round(npx / dt_b, 1),
+        "separate_runs": {"frames_per_s": round(nf / dt_s, 1), "ms_per_pass": round(dt_s * 1e3, 3),
+                          "speedup_of_batched": round(dt_s / dt_b, 3)},
+        "schedule": sched, "tuned_sample_ms": round(tuned_ms, 3), "launches": launches,
+        "direct_launches_frames_per_s": round(nf / dt_direct, 1),
+        "parity": {"frames": nf, "counts_identical": same_counts,
+                   "row_checksums_identical": same_rows},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(res), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="full", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="full", choices=sorted(WORKLOADS) + ["batch"])
     ap.add_argument("--frames", type=int, default=0, help="override the frame count")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--batch-scale", type=float, default=1.0,
+                    help="batch workload: scale of the video lengths")
     ap.add_argument("--e2e-frames", type=int, default=256,
                     help="frames of the end-to-end run (host frames in, DataFrame columns out, "
                          "with linking; plain workloads only; 0 = skip)")
@@ -175,6 +321,10 @@ def main():
         print("bench.py: this build runs on one GPU; multi-GPU runs are not part of it "
               "(frames shard across GPUs with no exchange; see the design notes)", file=sys.stderr)
         sys.exit(2)
+
+    if args.workload == "batch":
+        run_batch_workload(args)
+        return
 
     import numpy as np
     import torch

@@ -382,9 +382,10 @@ def link(to_link, search_range, memory=0):
 
 class Batch:
     """Batched detection of frames of different sizes (u8/u16, Trackpy style):
-    every launch runs a persistent kernel per stage over a table of frames,
-    replayed from a HIP graph; results equal Context.detect_device's bit for
-    bit.  No counterpart in the reference (one GpuState per frame size)."""
+    every launch runs one kernel per stage over a table of frames of any
+    sizes, replayed from a HIP graph; results equal Context.detect_device's
+    bit for bit.  No counterpart in the reference (one GpuState per frame
+    size)."""
 
     def __init__(self, params: N.GtParams, max_frame_pixels: int,
                  max_launch_pixels: int = 1 << 28, max_slots: int = 256):
@@ -451,8 +452,8 @@ class Batch:
         return out, counts
 
     def tune(self, frames, cap_per_frame: int = 8192) -> float:
-        """Autotunes the persistent grid sizes on sample frames; returns the
-        best time (ms) of one pass over them."""
+        """Autotunes the schedule (pixels per launch, slow-walk grid) on sample
+        frames; returns the best time (ms) of one pass over them."""
         import torch
         descs, code = self._descs(frames, None)
         n = len(frames)
@@ -465,11 +466,11 @@ class Batch:
         return float(ms.value)
 
     def schedule(self) -> dict:
-        a = (C.c_uint32 * 4)()
+        a = (C.c_uint64 * 4)()
         launches = C.c_uint64()
         N.check(self._lib.gt_batch_get_schedule(self._h, a, C.byref(launches)))
-        return {"bandpass_blocks": a[0], "detect_blocks": a[1], "walk_blocks": a[2],
-                "slot_stride": a[3], "launches_last_call": launches.value}
+        return {"pixels_per_launch": a[0], "walk_blocks": a[1], "max_frames_per_launch": a[2],
+                "max_pixels_per_launch": a[3], "launches_last_call": launches.value}
 
     def set_graph(self, enable: bool) -> None:
         N.check(self._lib.gt_batch_set_graph(self._h, int(bool(enable))))

@@ -58,18 +58,21 @@ hipError_t gt_launch_walk_log(const float *proc, const void *raw, int dtype, uin
 // Per-frame detection counts, 0xFFFFFFFF where the candidate list overflowed.
 hipError_t gt_launch_copy_counts_checked(const uint32_t *ocount, const uint32_t *ccount,
                                          uint32_t ccap, uint32_t *dst, uint32_t n, hipStream_t s);
-// Batched launches over frames of different sizes (u8/u16 frames, R <= 7).
-// Grid sizes of the persistent stages; the autotuner picks them.
+// Batched launches over frames of different sizes (u8/u16 frames, R <= 7):
+// grid y = frame slot of the launch, grid x = tile of that frame.
 struct GtBatchSchedule {
-    uint32_t bp_blocks;    // persistent bandpass blocks (multiple of 8)
-    uint32_t dt_blocks;    // persistent detect-tile blocks (multiple of 8)
-    uint32_t walk_blocks;  // blocks per slot for slow walks and characterize
-    uint32_t slot_blocks;  // slot stride of those kernels
+    uint32_t bp_blocks;     // bandpass tiles of the largest frame (multiple of 8)
+    uint32_t dt_blocks;     // detect tiles of the largest frame (multiple of 8)
+    uint32_t walk_blocks;   // blocks per frame for slow walks and characterize
+    uint32_t launch_slots;  // frames per launch at most (grid y of every stage)
 };
-hipError_t gt_launch_batch(const void *unused, int dtype, const GtKParams &p,
-                           const GtBatchHeader *hdr, const GtBatchFrame *fs, uint32_t max_slots,
-                           float *proc, gt_detection *out, uint32_t out_cap, uint32_t *ocount,
-                           uint2 *slow, uint32_t slow_cap, uint32_t *scount, uint32_t *counts,
+// One launch: k_select_launch copies launch *cur's entries of `all` into
+// (hdr, fs), the stages run, k_next_launch advances *cur.
+hipError_t gt_launch_batch(const GtBatchLaunch *launches, uint32_t *cur,
+                           const GtBatchFrame *all, int dtype, const GtKParams &p,
+                           GtBatchHeader *hdr, GtBatchFrame *fs, uint32_t max_slots, float *proc,
+                           gt_detection *out, uint32_t out_cap, uint32_t *ocount, uint2 *slow,
+                           uint32_t slow_cap, uint32_t *scount, uint32_t *counts,
                            const GtBatchSchedule &sch, hipStream_t s);
 void gt_batch_tiles(int H, int W, uint32_t *bp_tiles, uint32_t *dt_tiles);
 // Fills the constant tables the kernels read (once per device).

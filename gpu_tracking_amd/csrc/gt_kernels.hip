@@ -199,8 +199,8 @@ struct BpLds {
 // processed planes, p.H x p.W).
 template <typename T, int R>
 __device__ __forceinline__ void bandpass_int_tile(const T *__restrict__ f, float *__restrict__ out,
-                                                  const GtKParams &p, int x0, int y0,
-                                                  BpLds<R> &L) {
+                                                  const GtKParams &p, int H, int W, int x0,
+                                                  int y0, BpLds<R> &L) {
     constexpr int NT = 2 * R + 1;
     constexpr int RH = BpLds<R>::RH;
     constexpr int RWP = BpLds<R>::RWP;
@@ -208,7 +208,6 @@ __device__ __forceinline__ void bandpass_int_tile(const T *__restrict__ f, float
     static_assert(B2_TW - 4 + 4 * NV <= RWP, "row-pass reads stay inside the staged row");
     static_assert(B2_TH % B2_ROWS == 0 && (B2_TH / B2_ROWS) * B2_TW == B2_THREADS, "mapping");
     float *Sraw = L.Sraw, *Sg = L.Sg, *Sc = L.Sc;
-    const int H = p.H, W = p.W;
     const int tid = threadIdx.x;
     constexpr int PAIRS = (B2_TW + 2 * R + 1) / 2;  // u16 pairs per staged row
     static_assert(2 * PAIRS <= RWP, "pairs fit the staged row");
@@ -356,7 +355,7 @@ __global__ __launch_bounds__(B2_THREADS) void k_bandpass_int(const T *__restrict
     const int fr = tile / (ntx * nty), trest = tile - fr * (ntx * nty);
     const int x0 = (trest % ntx) * B2_TW, y0 = (trest / ntx) * B2_TH;
     bandpass_int_tile<T, R>(frames + static_cast<size_t>(fr) * HW,
-                            proc + static_cast<size_t>(fr) * HW, p, x0, y0, L);
+                            proc + static_cast<size_t>(fr) * HW, p, H, W, x0, y0, L);
 }
 
 // ---------------------------------------------------------------------------
@@ -747,10 +746,10 @@ struct GlobalRow {
 // One walk evaluation at (pu, pv) from the best available source; PITCH is
 // the staged region's compile-time row pitch (0: none or not known).
 template <int RI, int PITCH>
-__device__ __forceinline__ WalkRes walk_eval(const GtKParams &p, const float *__restrict__ P,
-                                             const float *Sp, int oy, int ox, int RW, int ylo,
-                                             int yhi, int xlo, int xhi, int pu, int pv) {
-    const int H = p.H, W = p.W;
+__device__ __forceinline__ WalkRes walk_eval(const GtKParams &p, int H, int W,
+                                             const float *__restrict__ P, const float *Sp, int oy,
+                                             int ox, int RW, int ylo, int yhi, int xlo, int xhi,
+                                             int pu, int pv) {
     if constexpr (RI > 0) {
         const bool in_frame = pu >= RI && pu + RI < H && pv >= RI && pv + RI < W;
         if (in_frame) {
@@ -818,7 +817,8 @@ __device__ __forceinline__ void characterize_inner(const B &base, const T *__res
 // Finish a walk at integer position (pu, pv) with last evaluation `res`:
 // minmass test, then the row, characterized when asked.
 template <typename T, int RI, int PITCH = 0, bool kChar = true>
-__device__ __forceinline__ bool finish_walk(const GtKParams &p, const float *__restrict__ P,
+__device__ __forceinline__ bool finish_walk(const GtKParams &p, int H, int W,
+                                            const float *__restrict__ P,
                                             const T *__restrict__ Rw, const float *Sp, int oy,
                                             int ox, int RW, int ylo, int yhi, int xlo, int xhi,
                                             WalkRes res, int pu, int pv, uint32_t seed,
@@ -831,24 +831,24 @@ __device__ __forceinline__ bool finish_walk(const GtKParams &p, const float *__r
         const int cu = f2i(rintf(d.row.x)), cv = f2i(rintf(d.row.y));
         const bool in_lds = Sp && cu >= ylo && cu <= yhi && cv >= xlo && cv <= xhi;
         if constexpr (RI > 0) {
-            if (cu >= RI && cu + RI < p.H && cv >= RI && cv + RI < p.W) {
-                const T *rc = Rw + static_cast<size_t>(cu) * p.W + cv;
+            if (cu >= RI && cu + RI < H && cv >= RI && cv + RI < W) {
+                const T *rc = Rw + static_cast<size_t>(cu) * W + cv;
                 if constexpr (PITCH > 0) {
                     if (in_lds) {
                         characterize_inner<T, RI>(LdsRow<PITCH>{Sp + (cu - oy) * PITCH + (cv - ox)},
-                                                  rc, p.W, d.row);
+                                                  rc, W, d.row);
                         return true;
                     }
                 }
-                characterize_inner<T, RI>(GlobalRow{P + static_cast<size_t>(cu) * p.W + cv, p.W},
-                                          rc, p.W, d.row);
+                characterize_inner<T, RI>(GlobalRow{P + static_cast<size_t>(cu) * W + cv, W},
+                                          rc, W, d.row);
                 return true;
             }
         }
         if (in_lds)
-            characterize_row<T, RI>(LdsSrc{Sp, oy, ox, RW}, Rw, p.H, p.W, p.circle, d.row);
+            characterize_row<T, RI>(LdsSrc{Sp, oy, ox, RW}, Rw, H, W, p.circle, d.row);
         else
-            characterize_row<T, RI>(GlobalSrc{P, p.W}, Rw, p.H, p.W, p.circle, d.row);
+            characterize_row<T, RI>(GlobalSrc{P, W}, Rw, H, W, p.circle, d.row);
     }
     return true;
 }
@@ -859,8 +859,9 @@ __device__ __forceinline__ bool finish_walk(const GtKParams &p, const float *__r
 // is the rows' frame field.
 template <typename T, int KT, int RI>
 __device__ __forceinline__ void detect_tile_body(
-    const float *__restrict__ P, const T *__restrict__ Rw, const GtKParams &p, const TileGeom &g,
-    int x0, int y0, uint32_t frame, gt_detection *__restrict__ fout, uint32_t out_cap,
+    const float *__restrict__ P, const T *__restrict__ Rw, const GtKParams &p, int H, int W,
+    const TileGeom &g, int x0, int y0, uint32_t frame, gt_detection *__restrict__ fout,
+    uint32_t out_cap,
     uint32_t *__restrict__ counter, uint2 *__restrict__ fslow, uint32_t slow_cap,
     uint32_t *__restrict__ slow_counter, float *lds, uint32_t *s_cnt,
     uint64_t (*s_masks)[FT_WROWS]) {
@@ -875,7 +876,6 @@ __device__ __forceinline__ void detect_tile_body(
     float *Sp = lds;                   // RH x pitch, origin (y0 - ht, x0 - hl)
     float *Hm = lds + RH * RW;         // MR x FT_TW; later the candidate list
     uint16_t *s_cand = reinterpret_cast<uint16_t *>(Hm);
-    const int H = p.H, W = p.W;
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int oy = y0 - g.ht, ox = x0 - g.hl;
 
@@ -1048,14 +1048,14 @@ __device__ __forceinline__ void detect_tile_body(
             bool moving = true;
             uint32_t it = 0;
             for (; it < p.max_iter && it < FT_WALK_STEPS; ++it) {
-                res = walk_eval<RI, PITCH>(p, P, Sp, oy, ox, RW, ylo, yhi, xlo, xhi, pu, pv);
+                res = walk_eval<RI, PITCH>(p, H, W, P, Sp, oy, ox, RW, ylo, yhi, xlo, xhi, pu, pv);
                 if (!walk_move(res, pu, pv, H, W, rint, p.shift_thr)) {
                     moving = false;
                     break;
                 }
             }
             if (moving && it < p.max_iter) defer = true;
-            else keep = finish_walk<T, RI, PITCH, false>(p, P, Rw, Sp, oy, ox, RW, ylo, yhi, xlo, xhi, res, pu,
+            else keep = finish_walk<T, RI, PITCH, false>(p, H, W, P, Rw, Sp, oy, ox, RW, ylo, yhi, xlo, xhi, res, pu,
                                            pv, seed, frame, d);
         }
         // hand long walks to k_walk_slow; if its list is full, walk on here
@@ -1066,10 +1066,10 @@ __device__ __forceinline__ void detect_tile_body(
             } else {
                 WalkRes res = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
                 for (uint32_t it = FT_WALK_STEPS; it < p.max_iter; ++it) {
-                    res = walk_eval<RI, PITCH>(p, P, Sp, oy, ox, RW, ylo, yhi, xlo, xhi, pu, pv);
+                    res = walk_eval<RI, PITCH>(p, H, W, P, Sp, oy, ox, RW, ylo, yhi, xlo, xhi, pu, pv);
                     if (!walk_move(res, pu, pv, H, W, rint, p.shift_thr)) break;
                 }
-                keep = finish_walk<T, RI, PITCH, false>(p, P, Rw, Sp, oy, ox, RW, ylo, yhi, xlo, xhi, res, pu,
+                keep = finish_walk<T, RI, PITCH, false>(p, H, W, P, Rw, Sp, oy, ox, RW, ylo, yhi, xlo, xhi, res, pu,
                                           pv, seed, frame, d);
             }
         }
@@ -1098,7 +1098,7 @@ __global__ __launch_bounds__(FT_THREADS, FT_MIN_WAVES) void k_detect_tile(
     const int fr = tile / (ntx * nty), trest = tile - fr * (ntx * nty);
     const int x0 = (trest % ntx) * FT_TW, y0 = (trest / ntx) * FT_TH;
     detect_tile_body<T, KT, RI>(proc + static_cast<size_t>(fr) * HW, raw + static_cast<size_t>(fr) * HW,
-                                p, g, x0, y0, frame0 + fr, out + static_cast<size_t>(fr) * out_cap,
+                                p, H, W, g, x0, y0, frame0 + fr, out + static_cast<size_t>(fr) * out_cap,
                                 out_cap, &ocount[fr * GT_COUNTER_STRIDE],
                                 slow + static_cast<size_t>(fr) * slow_cap, slow_cap,
                                 &scount[fr * GT_COUNTER_STRIDE], lds, s_cnt, s_masks);
@@ -1139,10 +1139,10 @@ __global__ __launch_bounds__(WK_THREADS) void k_walk_slow(const float *__restric
             int pv = static_cast<int>(e.y - static_cast<uint32_t>(pu) * static_cast<uint32_t>(W));
             WalkRes res = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
             for (uint32_t it = FT_WALK_STEPS; it < p.max_iter; ++it) {
-                res = walk_eval<RI, 0>(p, P, nullptr, 0, 0, 0, 1, 0, 1, 0, pu, pv);
+                res = walk_eval<RI, 0>(p, H, W, P, nullptr, 0, 0, 0, 1, 0, 1, 0, pu, pv);
                 if (!walk_move(res, pu, pv, H, W, rint, p.shift_thr)) break;
             }
-            keep = finish_walk<T, RI, 0, false>(p, P, Rw, nullptr, 0, 0, 0, 1, 0, 1, 0, res, pu, pv,
+            keep = finish_walk<T, RI, 0, false>(p, H, W, P, Rw, nullptr, 0, 0, 0, 1, 0, 1, 0, res, pu, pv,
                                                 e.x, frame0 + fr, d);
         }
         const uint32_t pos = wave_append(keep, counter);
@@ -1918,52 +1918,34 @@ hipError_t gt_launch_copy_counts_checked(const uint32_t *ocount, const uint32_t 
 // run the same per-tile bodies as the one-size kernels.
 namespace {
 
-// Slot of tile t: the last slot whose first tile is <= t.
-template <bool kDetect>
-__device__ __forceinline__ int batch_slot(const GtBatchFrame *__restrict__ fs, int n, uint32_t t) {
-    int lo = 0, hi = n - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        const uint32_t t0 = kDetect ? fs[mid].dt_tile0 : fs[mid].bp_tile0;
-        if (t0 <= t) lo = mid;
-        else hi = mid - 1;
-    }
-    return lo;
+// Tile of this block in a table launch: blockIdx.y is the frame slot and
+// blockIdx.x its tile, remapped so that each XCD (blockIdx.x % 8 for a
+// launch whose rows of blocks are multiples of 8 wide) works through one
+// contiguous range of the frame's tiles; blocks past the frame's tiles return
+// at once.  The grid is sized for the largest frame of a call, so one captured
+// graph serves every launch.
+__device__ __forceinline__ bool table_tile(uint32_t ntiles, uint32_t &t) {
+    const uint32_t per = (ntiles + kXcds - 1) / kXcds;
+    const uint32_t x = blockIdx.x % kXcds, j = blockIdx.x / kXcds;
+    if (j >= per) return false;
+    t = x * per + j;
+    return t < ntiles;
 }
-
-// Tiles of a persistent block: XCD x = blockIdx % 8 owns the contiguous range
-// [x * per, (x + 1) * per) and its blocks take every nb-th tile of it.
-struct PersistentTiles {
-    uint32_t t, end, step;
-    __device__ PersistentTiles(uint32_t total) {
-        const uint32_t nb = gridDim.x / kXcds;
-        const uint32_t per = (total + kXcds - 1) / kXcds;
-        const uint32_t x = blockIdx.x % kXcds, j = blockIdx.x / kXcds;
-        t = x * per + j;
-        end = min((x + 1) * per, total);
-        step = nb;
-    }
-};
 
 template <typename T, int R>
 __global__ __launch_bounds__(B2_THREADS) void k_bandpass_int_batch(
     float *__restrict__ proc, GtKParams p, const GtBatchHeader *__restrict__ hdr,
     const GtBatchFrame *__restrict__ fs) {
     __shared__ __align__(16) BpLds<R> L;
-    const int n = static_cast<int>(hdr->nslots);
-    for (PersistentTiles it(hdr->bp_tiles); it.t < it.end; it.t += it.step) {
-        const int slot = batch_slot<false>(fs, n, it.t);
-        const GtBatchFrame e = fs[slot];
-        GtKParams q = p;
-        q.H = e.H;
-        q.W = e.W;
-        const int ntx = (e.W + B2_TW - 1) / B2_TW;
-        const int local = static_cast<int>(it.t - e.bp_tile0);
-        const int x0 = (local % ntx) * B2_TW, y0 = (local / ntx) * B2_TH;
-        bandpass_int_tile<T, R>(reinterpret_cast<const T *>(e.in_ptr), proc + e.proc_off, q, x0,
-                                y0, L);
-        __syncthreads();
-    }
+    if (blockIdx.y >= hdr->nslots) return;
+    const GtBatchFrame e = fs[blockIdx.y];
+    const int ntx = (e.W + B2_TW - 1) / B2_TW, nty = (e.H + B2_TH - 1) / B2_TH;
+    uint32_t t;
+    if (!table_tile(static_cast<uint32_t>(ntx * nty), t)) return;
+    const int local = static_cast<int>(t);
+    const int x0 = (local % ntx) * B2_TW, y0 = (local / ntx) * B2_TH;
+    bandpass_int_tile<T, R>(reinterpret_cast<const T *>(e.in_ptr), proc + e.proc_off, p, e.H, e.W,
+                            x0, y0, L);
 }
 
 template <typename T, int KT, int RI>
@@ -1978,24 +1960,20 @@ __global__ __launch_bounds__(FT_THREADS, FT_MIN_WAVES) void k_detect_tile_batch(
     extern __shared__ __align__(16) float lds[];
     __shared__ uint32_t s_cnt[FT_WAVES];
     __shared__ uint64_t s_masks[FT_WAVES][FT_WROWS];
-    const int n = static_cast<int>(hdr->nslots);
-    for (PersistentTiles it(hdr->dt_tiles); it.t < it.end; it.t += it.step) {
-        const int slot = batch_slot<true>(fs, n, it.t);
-        const GtBatchFrame e = fs[slot];
-        GtKParams q = p;
-        q.H = e.H;
-        q.W = e.W;
-        const int ntx = (e.W + FT_TW - 1) / FT_TW;
-        const int local = static_cast<int>(it.t - e.dt_tile0);
-        const int x0 = (local % ntx) * FT_TW, y0 = (local / ntx) * FT_TH;
-        // rows go to the output slot; counters and slow lists are per launch slot
-        detect_tile_body<T, KT, RI>(proc + e.proc_off, reinterpret_cast<const T *>(e.in_ptr), q, g,
-                                    x0, y0, e.frame, out + static_cast<size_t>(e.out_idx) * out_cap,
-                                    out_cap, &ocount[slot * GT_COUNTER_STRIDE],
-                                    slow + static_cast<size_t>(slot) * slow_cap, slow_cap,
-                                    &scount[slot * GT_COUNTER_STRIDE], lds, s_cnt, s_masks);
-        __syncthreads();
-    }
+    const uint32_t slot = blockIdx.y;
+    if (slot >= hdr->nslots) return;
+    const GtBatchFrame e = fs[slot];
+    const int ntx = (e.W + FT_TW - 1) / FT_TW, nty = (e.H + FT_TH - 1) / FT_TH;
+    uint32_t t;
+    if (!table_tile(static_cast<uint32_t>(ntx * nty), t)) return;
+    const int local = static_cast<int>(t);
+    const int x0 = (local % ntx) * FT_TW, y0 = (local / ntx) * FT_TH;
+    // rows go to the output slot; counters and slow lists are per launch slot
+    detect_tile_body<T, KT, RI>(proc + e.proc_off, reinterpret_cast<const T *>(e.in_ptr), p, e.H,
+                                e.W, g, x0, y0, e.frame, out + static_cast<size_t>(e.out_idx) * out_cap,
+                                out_cap, &ocount[slot * GT_COUNTER_STRIDE],
+                                slow + static_cast<size_t>(slot) * slow_cap, slow_cap,
+                                &scount[slot * GT_COUNTER_STRIDE], lds, s_cnt, s_masks);
 }
 
 // Slow walks and characterize: slots over blockIdx.y (strided), detections
@@ -2011,9 +1989,6 @@ __global__ __launch_bounds__(WK_THREADS) void k_walk_slow_batch(
     const uint32_t stride = gridDim.x * WK_THREADS;
     for (int slot = blockIdx.y; slot < nslots; slot += gridDim.y) {
         const GtBatchFrame e = fs[slot];
-        GtKParams q = p;
-        q.H = e.H;
-        q.W = e.W;
         const int H = e.H, W = e.W;
         const float *P = proc + e.proc_off;
         const T *Rw = reinterpret_cast<const T *>(e.in_ptr);
@@ -2021,7 +1996,7 @@ __global__ __launch_bounds__(WK_THREADS) void k_walk_slow_batch(
         const uint2 *fsl = slow + static_cast<size_t>(slot) * slow_cap;
         gt_detection *fout = out + static_cast<size_t>(e.out_idx) * out_cap;
         uint32_t *counter = &ocount[slot * GT_COUNTER_STRIDE];
-        const int rint = RI > 0 ? RI : q.rint;
+        const int rint = RI > 0 ? RI : p.rint;
         for (uint32_t i0 = blockIdx.x * WK_THREADS + (threadIdx.x & ~63u); i0 < n; i0 += stride) {
             const uint32_t i = i0 + lane;
             bool keep = false;
@@ -2031,11 +2006,11 @@ __global__ __launch_bounds__(WK_THREADS) void k_walk_slow_batch(
                 int pu = static_cast<int>(en.y / static_cast<uint32_t>(W));
                 int pv = static_cast<int>(en.y - static_cast<uint32_t>(pu) * static_cast<uint32_t>(W));
                 WalkRes res = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
-                for (uint32_t itr = FT_WALK_STEPS; itr < q.max_iter; ++itr) {
-                    res = walk_eval<RI, 0>(q, P, nullptr, 0, 0, 0, 1, 0, 1, 0, pu, pv);
-                    if (!walk_move(res, pu, pv, H, W, rint, q.shift_thr)) break;
+                for (uint32_t itr = FT_WALK_STEPS; itr < p.max_iter; ++itr) {
+                    res = walk_eval<RI, 0>(p, H, W, P, nullptr, 0, 0, 0, 1, 0, 1, 0, pu, pv);
+                    if (!walk_move(res, pu, pv, H, W, rint, p.shift_thr)) break;
                 }
-                keep = finish_walk<T, RI, 0, false>(q, P, Rw, nullptr, 0, 0, 0, 1, 0, 1, 0, res, pu,
+                keep = finish_walk<T, RI, 0, false>(p, H, W, P, Rw, nullptr, 0, 0, 0, 1, 0, 1, 0, res, pu,
                                                     pv, en.x, e.frame, d);
             }
             const uint32_t pos = wave_append(keep, counter);
@@ -2075,6 +2050,26 @@ __global__ __launch_bounds__(WK_THREADS) void k_characterize_det_batch(
     }
 }
 
+// First and last kernels of a launch: copy launch *cur's entries into the
+// fixed table the stage kernels read, and advance *cur afterwards.
+__global__ void k_select_launch(const GtBatchLaunch *__restrict__ launches,
+                                const uint32_t *__restrict__ cur,
+                                const GtBatchFrame *__restrict__ all,
+                                GtBatchHeader *__restrict__ hdr, GtBatchFrame *__restrict__ fs) {
+    const GtBatchLaunch L = launches[*cur];
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < L.nslots) fs[i] = all[L.first + i];
+    if (i == 0) {
+        GtBatchHeader h = {};
+        h.nslots = L.nslots;
+        *hdr = h;
+    }
+}
+
+__global__ void k_next_launch(uint32_t *__restrict__ cur) {
+    if (threadIdx.x == 0) *cur = *cur + 1u;
+}
+
 __global__ void k_copy_counts_batch(const uint32_t *__restrict__ strided,
                                     const GtBatchHeader *__restrict__ hdr,
                                     const GtBatchFrame *__restrict__ fs, uint32_t *__restrict__ dst) {
@@ -2083,16 +2078,21 @@ __global__ void k_copy_counts_batch(const uint32_t *__restrict__ strided,
 }
 
 template <typename T>
-hipError_t launch_batch_t(const GtKParams &p, const GtBatchHeader *hdr, const GtBatchFrame *fs,
+hipError_t launch_batch_t(const GtBatchLaunch *launches, uint32_t *cur, const GtBatchFrame *all,
+                          const GtKParams &p, GtBatchHeader *hdr, GtBatchFrame *fs,
                           uint32_t max_slots, float *proc, gt_detection *out, uint32_t out_cap,
                           uint32_t *ocount, uint2 *slow, uint32_t slow_cap, uint32_t *scount,
                           uint32_t *counts, const GtBatchSchedule &sch, hipStream_t s) {
     if (p.R < 1 || p.R > 7 || !p.fast_div) return hipErrorInvalidValue;
-    hipError_t e = gt_launch_zero_u32(ocount, static_cast<size_t>(max_slots) * GT_COUNTER_STRIDE, s);
+    k_select_launch<<<(max_slots + 255) / 256, 256, 0, s>>>(launches, cur, all, hdr, fs);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    e = gt_launch_zero_u32(ocount, static_cast<size_t>(max_slots) * GT_COUNTER_STRIDE, s);
     if (e != hipSuccess) return e;
     e = gt_launch_zero_u32(scount, static_cast<size_t>(max_slots) * GT_COUNTER_STRIDE, s);
     if (e != hipSuccess) return e;
-    const dim3 gbp(sch.bp_blocks);
+    // x: tiles of the call's largest frame (multiple of 8); y: frames of a launch
+    const dim3 gbp(sch.bp_blocks, sch.launch_slots);
     switch (p.R) {
     case 1: k_bandpass_int_batch<T, 1><<<gbp, B2_THREADS, 0, s>>>(proc, p, hdr, fs); break;
     case 2: k_bandpass_int_batch<T, 2><<<gbp, B2_THREADS, 0, s>>>(proc, p, hdr, fs); break;
@@ -2109,7 +2109,7 @@ hipError_t launch_batch_t(const GtKParams &p, const GtBatchHeader *hdr, const Gt
     const TileGeom g = tile_geom(p.k, p.rint);
     const size_t lds = sizeof(float) * (static_cast<size_t>(g.rows) * g.pitch +
                                         static_cast<size_t>(FT_TH + p.k - 1) * FT_TW);
-    const dim3 gdt(sch.dt_blocks);
+    const dim3 gdt(sch.dt_blocks, sch.launch_slots);
 #define GT_TILEB(KV, RV)                                                                     \
     k_detect_tile_batch<T, KV, RV><<<gdt, FT_THREADS, lds, s>>>(proc, kw, g, hdr, fs, out,      \
                                                                 out_cap, ocount, slow, slow_cap, \
@@ -2121,7 +2121,7 @@ hipError_t launch_batch_t(const GtKParams &p, const GtBatchHeader *hdr, const Gt
 #undef GT_TILEB
     e = hipGetLastError();
     if (e != hipSuccess) return e;
-    const dim3 gw(sch.walk_blocks, sch.slot_blocks);
+    const dim3 gw(sch.walk_blocks, sch.launch_slots);
     if (p.rint == 4)
         k_walk_slow_batch<T, 4><<<gw, WK_THREADS, 0, s>>>(proc, kw, hdr, fs, slow, slow_cap, scount,
                                                           out, out_cap, ocount);
@@ -2141,6 +2141,9 @@ hipError_t launch_batch_t(const GtKParams &p, const GtBatchHeader *hdr, const Gt
         if (e != hipSuccess) return e;
     }
     k_copy_counts_batch<<<(max_slots + 255) / 256, 256, 0, s>>>(ocount, hdr, fs, counts);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    k_next_launch<<<1, 64, 0, s>>>(cur);
     return hipGetLastError();
 }
 
@@ -2148,19 +2151,19 @@ hipError_t launch_batch_t(const GtKParams &p, const GtBatchHeader *hdr, const Gt
 
 size_t gt_batch_tile_lds_bytes(const GtKParams &p) { return gt_tile_lds_bytes(p); }
 
-hipError_t gt_launch_batch(const void *unused, int dtype, const GtKParams &p,
-                           const GtBatchHeader *hdr, const GtBatchFrame *fs, uint32_t max_slots,
-                           float *proc, gt_detection *out, uint32_t out_cap, uint32_t *ocount,
-                           uint2 *slow, uint32_t slow_cap, uint32_t *scount, uint32_t *counts,
+hipError_t gt_launch_batch(const GtBatchLaunch *launches, uint32_t *cur,
+                           const GtBatchFrame *all, int dtype, const GtKParams &p,
+                           GtBatchHeader *hdr, GtBatchFrame *fs, uint32_t max_slots, float *proc,
+                           gt_detection *out, uint32_t out_cap, uint32_t *ocount, uint2 *slow,
+                           uint32_t slow_cap, uint32_t *scount, uint32_t *counts,
                            const GtBatchSchedule &sch, hipStream_t s) {
-    (void)unused;
     switch (dtype) {
     case GT_U16:
-        return launch_batch_t<uint16_t>(p, hdr, fs, max_slots, proc, out, out_cap, ocount, slow,
-                                        slow_cap, scount, counts, sch, s);
+        return launch_batch_t<uint16_t>(launches, cur, all, p, hdr, fs, max_slots, proc, out,
+                                        out_cap, ocount, slow, slow_cap, scount, counts, sch, s);
     case GT_U8:
-        return launch_batch_t<uint8_t>(p, hdr, fs, max_slots, proc, out, out_cap, ocount, slow,
-                                       slow_cap, scount, counts, sch, s);
+        return launch_batch_t<uint8_t>(launches, cur, all, p, hdr, fs, max_slots, proc, out,
+                                       out_cap, ocount, slow, slow_cap, scount, counts, sch, s);
     default: return hipErrorInvalidValue;
     }
 }

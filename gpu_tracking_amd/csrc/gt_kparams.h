@@ -60,19 +60,21 @@ struct GtLogTable {
 };
 
 // Batched launches: frames of different sizes in one launch.  One entry per
-// frame slot of a launch; tile ranges are prefix sums over the slots.
+// frame slot of a launch (grid y of every stage).
 struct GtBatchFrame {
     uint64_t in_ptr;      // device address of the frame (dtype pixels, H x W)
     uint64_t proc_off;    // element offset of its processed plane in the scratch pool
     int32_t H, W;
-    uint32_t bp_tile0;    // first bandpass tile of the slot in the launch
-    uint32_t dt_tile0;    // first detect tile
     uint32_t out_idx;     // output slot: rows at out[out_idx * cap], count at counts[out_idx]
     uint32_t frame;       // the frame field of its detections
 };
 struct GtBatchHeader {
+    uint32_t nslots;      // frames of the launch
+    uint32_t pad[3];
+};
+// A call's launches: launch k uses entries [first, first + nslots) of the
+// call's entry array; a device counter says which launch runs next.
+struct GtBatchLaunch {
+    uint32_t first;
     uint32_t nslots;
-    uint32_t bp_tiles;    // total bandpass tiles of the launch
-    uint32_t dt_tiles;    // total detect tiles
-    uint32_t pad;
 };

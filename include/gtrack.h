@@ -230,9 +230,11 @@ gt_status gt_link(const float *points, uint64_t n, float search_range, uint32_t 
  * (src/execute_gpu.rs:1074-1080).  A batch context holds the parameters
  * (Trackpy style, u8/u16 frames) and scratch for launches of up to
  * max_launch_pixels pixels in up to max_slots frames; gt_batch_detect runs
- * any mix of frame sizes through one persistent launch per stage per launch,
- * replaying a HIP graph of the stage sequence.  Rows of frames[i] land at
- * d_out[i * cap_per_frame], their count at d_counts[i]; each row's `frame`
+ * any mix of frame sizes through one kernel per stage per launch (grid y =
+ * frame of the launch's table, grid x = tile), uploading every launch's table
+ * at once and replaying a HIP graph of the stage sequence once per launch.
+ * Rows of frames[i] land at d_out[i * cap_per_frame], their count at
+ * d_counts[i]; each row's `frame`
  * field is frames[i].frame.  Results equal gt_detect_device's on the same
  * frames bit for bit. */
 typedef struct gt_frame_desc {
@@ -250,15 +252,17 @@ void gt_batch_destroy(gt_batch *b);
 gt_status gt_batch_detect(gt_batch *b, const gt_frame_desc *frames, uint32_t n, int32_t dtype,
                           gt_detection *d_out, uint32_t cap_per_frame, uint32_t *d_counts,
                           void *stream);
-/* Autotuner: times candidate persistent grid sizes on the sample frames
- * (coordinate search, warm-up + one timed run each) and keeps the fastest
- * schedule for later calls; *best_ms receives its time. */
+/* Autotuner: times candidate schedules on the sample frames (pixels per
+ * launch, then the slow-walk / characterize grid; coordinate search, warm-up
+ * + one timed run each) and keeps the fastest for later calls; *best_ms
+ * receives its time. */
 gt_status gt_batch_tune(gt_batch *b, const gt_frame_desc *frames, uint32_t n, int32_t dtype,
                         gt_detection *d_out, uint32_t cap_per_frame, uint32_t *d_counts,
                         float *best_ms);
-/* The schedule in use (bandpass blocks, detect blocks, walk blocks per slot,
- * slot stride) and the number of launches of the last call. */
-gt_status gt_batch_get_schedule(gt_batch *b, uint32_t *sched4, uint64_t *launches);
+/* The schedule in use (pixels per launch, slow-walk blocks per frame, frames
+ * per launch at most, the context's pixel budget per launch) and the number
+ * of launches of the last call. */
+gt_status gt_batch_get_schedule(gt_batch *b, uint64_t *sched4, uint64_t *launches);
 /* Graph replay on (default) or direct launches. */
 gt_status gt_batch_set_graph(gt_batch *b, int32_t enable);
 
