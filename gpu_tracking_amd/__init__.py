@@ -9,10 +9,11 @@ libgtrack.so), with the reference's Python entry points on top:
 The package needs the built library; importing the entry points raises when
 it is missing (no CPU fallback).
 """
-from .api import (Batch, Context, LoG, batch, characterize_points,  # noqa: F401
+from .api import (Batch, CentroidNet, Context, LoG, batch, characterize_points,  # noqa: F401
                   detections_to_numpy, link, make_log_params, make_params)
 from ._native import GtError, column_names, default_params  # noqa: F401
 
-__all__ = ["batch", "characterize_points", "LoG", "link", "Context", "Batch", "make_params",
+__all__ = ["batch", "characterize_points", "LoG", "link", "Context", "Batch", "CentroidNet",
+           "make_params",
            "make_log_params", "GtError",
            "column_names", "default_params", "detections_to_numpy"]

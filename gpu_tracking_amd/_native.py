@@ -91,8 +91,13 @@ EXPORTS = [
     "gt_column_names", "gt_filter_close", "gt_doughnut", "gt_set_interrupt", "gt_progress",
     "gt_set_timing", "gt_stage_times", "gt_prune_blobs", "gt_link", "gt_batch_create",
     "gt_batch_destroy", "gt_batch_detect", "gt_batch_tune", "gt_batch_get_schedule",
-    "gt_batch_set_graph",
+    "gt_batch_set_graph", "gt_cnn_create", "gt_cnn_destroy", "gt_cnn_set_params",
+    "gt_cnn_get_params", "gt_cnn_adam_steps", "gt_cnn_forward", "gt_cnn_gradient", "gt_cnn_step",
+    "gt_cnn_train", "gt_cnn_make_samples", "gt_context_set_cnn", "gt_context_cnn_stats",
 ]
+
+CNN_NPARAM = 14147
+CNN_PATCH_PITCH = 128
 
 _lib = None
 
@@ -166,6 +171,26 @@ def lib() -> C.CDLL:
     L.gt_stage_times.argtypes = [C.c_void_p, P(C.c_double), C.c_int32, P(C.c_uint32),
                                  P(C.c_uint32)]
     L.gt_stage_times.restype = C.c_int32
+    vp, u64, u32, f32 = C.c_void_p, C.c_uint64, C.c_uint32, C.c_float
+    sigs = {
+        "gt_cnn_create": [u64, P(vp)],
+        "gt_cnn_set_params": [vp, vp],
+        "gt_cnn_get_params": [vp, vp],
+        "gt_cnn_adam_steps": [vp, P(u64)],
+        "gt_cnn_forward": [vp, vp, u64, vp, vp],
+        "gt_cnn_gradient": [vp, vp, vp, vp, u32, vp, P(f32)],
+        "gt_cnn_step": [vp, vp, vp, vp, u32, f32, P(f32)],
+        "gt_cnn_train": [vp, vp, vp, u64, u32, u32, f32, u64, vp],
+        "gt_cnn_make_samples": [vp, vp, C.c_int32, u32, vp, vp, u64, P(u64), P(u64), vp],
+        "gt_context_set_cnn": [vp, vp, f32, f32],
+        "gt_context_cnn_stats": [vp, P(u64), P(u64), C.c_int32],
+    }
+    for name, args in sigs.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = C.c_int
+    L.gt_cnn_destroy.argtypes = [vp]
+    L.gt_cnn_destroy.restype = None
     _lib = L
     return L
 

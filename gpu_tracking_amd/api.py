@@ -115,6 +115,17 @@ def _common(p: N.GtParams, a: dict) -> None:
     p.shift_threshold = float(a["shift_threshold"])
 
 
+def _torch_code(frames) -> int:
+    import torch
+    if frames.dtype in (torch.int16, torch.uint16):
+        return N.GT_U16
+    if frames.dtype == torch.uint8:
+        return N.GT_U8
+    if frames.dtype == torch.float32:
+        return N.GT_F32
+    raise TypeError(f"unsupported dtype {frames.dtype}")
+
+
 class Context:
     """One detection context (setup_state) for frames of a fixed size."""
 
@@ -196,14 +207,7 @@ class Context:
             frames = frames.unsqueeze(0)
         if tuple(frames.shape[1:]) != self.shape or not frames.is_contiguous():
             raise ValueError("frames must be contiguous with shape (T,) + context shape")
-        if frames.dtype in (torch.int16, torch.uint16):
-            code = N.GT_U16
-        elif frames.dtype == torch.uint8:
-            code = N.GT_U8
-        elif frames.dtype == torch.float32:
-            code = N.GT_F32
-        else:
-            raise TypeError(f"unsupported dtype {frames.dtype}")
+        code = _torch_code(frames)
         T = frames.shape[0]
         cap = cap_per_frame or max(1024, (self.shape[0] * self.shape[1]) // 16)
         if out is None:
@@ -215,6 +219,49 @@ class Context:
         N.check(self._lib.gt_detect_device(self._h, frames.data_ptr(), code, T, out.data_ptr(),
                                            cap, counts.data_ptr(), stream))
         return out, counts
+
+    def set_learned(self, net: Optional["CentroidNet"], conf_logit: float = 0.0,
+                    tol: float = 0.05) -> None:
+        """Detect with the network's centroid proposals (None: the plain
+        walk).  A proposal is kept when its logit >= conf_logit and one walk
+        evaluation at its centre neither moves nor lands more than tol px
+        away; every other candidate takes the full walk."""
+        N.check(self._lib.gt_context_set_cnn(self._h, None if net is None else net.handle,
+                                             float(conf_logit), float(tol)))
+        self._net = net
+
+    def learned_stats(self, reset: bool = False):
+        """(accepted proposals, candidates) since the last reset."""
+        a, c = C.c_uint64(0), C.c_uint64(0)
+        N.check(self._lib.gt_context_cnn_stats(self._h, C.byref(a), C.byref(c), int(reset)))
+        return a.value, c.value
+
+    def training_samples(self, frames, cap: Optional[int] = None, stream=None):
+        """Training pairs from device frames (T, Y, X): every candidate's
+        normalised patch (bfloat16 (n, 128), entries 121.. zero) and target
+        (float32 (n, 4): end row - candidate row, end column - candidate
+        column, kept by minmass, candidate raster index as uint32 bits) from
+        the plain method's full walk.
+        Returns (patches, targets, candidates seen)."""
+        import torch
+        if frames.dim() == 2:
+            frames = frames.unsqueeze(0)
+        code = _torch_code(frames)
+        if tuple(frames.shape[1:]) != self.shape or not frames.is_contiguous():
+            raise ValueError("frames must be contiguous with shape (T,) + context shape")
+        T = frames.shape[0]
+        if cap is None:
+            cap = T * max(1024, (self.shape[0] * self.shape[1]) // 32)
+        patches = torch.zeros((cap, N.CNN_PATCH_PITCH), dtype=torch.bfloat16, device=frames.device)
+        targets = torch.zeros((cap, 4), dtype=torch.float32, device=frames.device)
+        nw, nt = C.c_uint64(0), C.c_uint64(0)
+        if stream is None:
+            stream = torch.cuda.current_stream(frames.device).cuda_stream
+        N.check(self._lib.gt_cnn_make_samples(self._h, frames.data_ptr(), code, T,
+                                              patches.data_ptr(), targets.data_ptr(), cap,
+                                              C.byref(nw), C.byref(nt), stream))
+        n = nw.value
+        return patches[:n], targets[:n], nt.value
 
     def set_timing(self, enable: bool) -> None:
         N.check(self._lib.gt_set_timing(self._h, int(enable)))
@@ -261,8 +308,13 @@ def batch(file_or_array, diameter, **kwargs):
     reset = kwargs.pop("linker_reset_points", None)
     kwargs.pop("tqdm", None)
     kwargs.pop("channel", None)
+    learned = kwargs.pop("learned", None)
+    learned_conf = kwargs.pop("learned_conf", 0.0)
+    learned_tol = kwargs.pop("learned_tol", 0.05)
     p = make_params(diameter, **kwargs)
     with Context(p, video.shape[1], video.shape[2]) as ctx:
+        if learned is not None:
+            ctx.set_learned(learned, learned_conf, learned_tol)
         data, names, types = ctx.execute(video, keys=frames, linker_reset_points=reset)
     return _to_dataframe(data, names, types)
 
@@ -474,3 +526,131 @@ class Batch:
 
     def set_graph(self, enable: bool) -> None:
         N.check(self._lib.gt_batch_set_graph(self._h, int(bool(enable))))
+
+
+class CentroidNet:
+    """Learned centroid proposal (no counterpart in the reference): a small
+    CNN that predicts where the centroid walk from a candidate pixel ends.
+    Its float32 parameters (torch layouts w1 b1 w2 b2 w3 b3 wfc bfc, 14147
+    values), Adam state and bf16 working copies live on the GPU; training
+    runs the library's forward/backward kernels."""
+
+    NPARAM = N.CNN_NPARAM
+
+    def __init__(self, seed: int = 0, params=None):
+        self._lib = N.lib()
+        h = C.c_void_p()
+        N.check(self._lib.gt_cnn_create(int(seed), C.byref(h)))
+        self._h = h
+        if params is not None:
+            self.params = params
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) and self._h.value:
+            self._lib.gt_cnn_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def handle(self) -> C.c_void_p:
+        return self._h
+
+    @property
+    def params(self) -> np.ndarray:
+        out = np.empty(self.NPARAM, np.float32)
+        N.check(self._lib.gt_cnn_get_params(self._h, out.ctypes.data))
+        return out
+
+    @params.setter
+    def params(self, p) -> None:
+        a = np.ascontiguousarray(np.asarray(p, np.float32).reshape(-1))
+        if a.shape != (self.NPARAM,):
+            raise ValueError(f"expected {self.NPARAM} parameters, got {a.size}")
+        N.check(self._lib.gt_cnn_set_params(self._h, a.ctypes.data))
+
+    @property
+    def adam_steps(self) -> int:
+        v = C.c_uint64(0)
+        N.check(self._lib.gt_cnn_adam_steps(self._h, C.byref(v)))
+        return v.value
+
+    def save(self, path: str) -> None:
+        np.save(path, self.params)
+
+    @classmethod
+    def load(cls, path: str) -> "CentroidNet":
+        return cls(params=np.load(path, allow_pickle=False))
+
+    @staticmethod
+    def _check_samples(patches, targets=None):
+        import torch
+        if not patches.is_cuda or patches.dtype != torch.bfloat16 or patches.dim() != 2 \
+                or patches.shape[1] != N.CNN_PATCH_PITCH or not patches.is_contiguous():
+            raise ValueError("patches must be a contiguous CUDA bfloat16 tensor (n, 128)")
+        if targets is not None and (targets.dtype != torch.float32 or targets.shape !=
+                                    (patches.shape[0], 4) or not targets.is_contiguous()):
+            raise ValueError("targets must be a contiguous float32 tensor (n, 4)")
+
+    def forward(self, patches, stream=None):
+        """(n, 4) float32 device tensor of (dy, dx, logit, 0)."""
+        import torch
+        self._check_samples(patches)
+        out = torch.empty((patches.shape[0], 4), dtype=torch.float32, device=patches.device)
+        if stream is None:
+            stream = torch.cuda.current_stream(patches.device).cuda_stream
+        N.check(self._lib.gt_cnn_forward(self._h, patches.data_ptr(), patches.shape[0],
+                                         out.data_ptr(), stream))
+        return out
+
+    def _idx(self, patches, idx):
+        import torch
+        if idx is None:
+            idx = torch.arange(patches.shape[0], dtype=torch.int32, device=patches.device)
+        idx = idx.to(device=patches.device, dtype=torch.int32).contiguous()
+        if idx.numel() and (int(idx.min()) < 0 or int(idx.max()) >= patches.shape[0]):
+            raise IndexError("sample index out of range")
+        torch.cuda.synchronize(patches.device)
+        return idx
+
+    def gradient(self, patches, targets, idx=None):
+        """Gradient (numpy float32 (14147,)) and loss of one batch."""
+        self._check_samples(patches, targets)
+        idx = self._idx(patches, idx)
+        g = np.empty(self.NPARAM, np.float32)
+        L = C.c_float(0)
+        N.check(self._lib.gt_cnn_gradient(self._h, patches.data_ptr(), targets.data_ptr(),
+                                          idx.data_ptr(), idx.numel(), g.ctypes.data,
+                                          C.byref(L)))
+        return g, L.value
+
+    def step(self, patches, targets, idx=None, lr: float = 1e-3) -> float:
+        """One Adam step on a batch; returns the batch loss before it."""
+        self._check_samples(patches, targets)
+        idx = self._idx(patches, idx)
+        L = C.c_float(0)
+        N.check(self._lib.gt_cnn_step(self._h, patches.data_ptr(), targets.data_ptr(),
+                                      idx.data_ptr(), idx.numel(), float(lr), C.byref(L)))
+        return L.value
+
+    def train(self, patches, targets, epochs: int = 1, batch: int = 4096, lr: float = 1e-3,
+              seed: int = 0) -> np.ndarray:
+        """Adam over shuffled batches; returns the mean batch loss per epoch."""
+        import torch
+        self._check_samples(patches, targets)
+        torch.cuda.synchronize(patches.device)
+        losses = np.zeros(int(epochs), np.float32)
+        N.check(self._lib.gt_cnn_train(self._h, patches.data_ptr(), targets.data_ptr(),
+                                       patches.shape[0], int(epochs), int(batch), float(lr),
+                                       int(seed), losses.ctypes.data))
+        return losses

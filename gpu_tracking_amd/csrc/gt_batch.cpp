@@ -335,15 +335,16 @@ gt_status gt_batch_detect(gt_batch *b, const gt_frame_desc *frames, uint32_t n, 
     if (st != GT_OK) return st;
     // the work runs on the context's own stream (a graph cannot be captured on
     // a legacy default stream), ordered after and before the caller's stream
+    // (NULL: the default stream)
     hipStream_t user = static_cast<hipStream_t>(stream);
-    if (user && user != b->stream) {
+    if (user != b->stream) {
         GB_HIP(hipEventRecord(b->ev_in, user));
         GB_HIP(hipStreamWaitEvent(b->stream, b->ev_in, 0));
     }
     st = run_batch(b, frames, n, dtype, d_out, cap_per_frame, d_counts, b->stream, b->walk_blocks,
                    b->launch_px, b->use_graph != 0);
     if (st != GT_OK) return st;
-    if (user && user != b->stream) {
+    if (user != b->stream) {
         GB_HIP(hipEventRecord(b->ev_out, b->stream));
         GB_HIP(hipStreamWaitEvent(user, b->ev_out, 0));
     }

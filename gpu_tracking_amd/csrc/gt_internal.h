@@ -75,5 +75,21 @@ hipError_t gt_launch_batch(const GtBatchLaunch *launches, uint32_t *cur,
                            uint32_t slow_cap, uint32_t *scount, uint32_t *counts,
                            const GtBatchSchedule &sch, hipStream_t s);
 void gt_batch_tiles(int H, int W, uint32_t *bp_tiles, uint32_t *dt_tiles);
+// Learned proposal path: candidate pixels per frame (local-maximum test of
+// the tile kernel), the verifying walk over the network's proposals
+// (stats[0] += accepted, stats[1] += candidates), and the training targets
+// (full walk from every candidate, flat index offs[f] + i).
+hipError_t gt_launch_detect_cand(const float *proc, int dtype, uint32_t nframes,
+                                 const GtKParams &p, uint32_t *cand, uint32_t cap,
+                                 uint32_t *ccount, hipStream_t s);
+hipError_t gt_launch_walk_learned(const float *proc, uint32_t nframes, const GtKParams &p,
+                                  const uint32_t *cand, uint32_t cap, const uint32_t *ccount,
+                                  const float4 *props, float conf_logit, float tol,
+                                  gt_detection *out, uint32_t out_cap, uint32_t *ocount,
+                                  unsigned long long *stats, uint32_t frame0, hipStream_t s);
+hipError_t gt_launch_cnn_label(const float *proc, uint32_t nframes, const GtKParams &p,
+                               const uint32_t *cand, uint32_t cap, const uint32_t *ccount,
+                               const uint32_t *offs, uint32_t maxn, float4 *targets,
+                               hipStream_t s);
 // Fills the constant tables the kernels read (once per device).
 hipError_t gt_init_tables();

@@ -856,15 +856,17 @@ __device__ __forceinline__ bool finish_walk(const GtKParams &p, int H, int W,
 // One FT_TH x FT_TW tile (origin x0, y0) of one frame: P, Rw are its
 // processed and raw planes (p.H x p.W); its rows go to fout (counted in
 // *counter), walks handed over to fslow (counted in *slow_counter); `frame`
-// is the rows' frame field.
-template <typename T, int KT, int RI>
+// is the rows' frame field.  With kCand the tile only lists its candidate
+// pixels (raster index) in fcand, counted in *slow_counter (capacity
+// slow_cap), for the learned proposal and its verifying walk.
+template <typename T, int KT, int RI, bool kCand = false>
 __device__ __forceinline__ void detect_tile_body(
     const float *__restrict__ P, const T *__restrict__ Rw, const GtKParams &p, int H, int W,
     const TileGeom &g, int x0, int y0, uint32_t frame, gt_detection *__restrict__ fout,
     uint32_t out_cap,
     uint32_t *__restrict__ counter, uint2 *__restrict__ fslow, uint32_t slow_cap,
     uint32_t *__restrict__ slow_counter, float *lds, uint32_t *s_cnt,
-    uint64_t (*s_masks)[FT_WROWS]) {
+    uint64_t (*s_masks)[FT_WROWS], uint32_t *__restrict__ fcand = nullptr) {
     constexpr bool kFixed = KT > 0 && RI > 0;
     constexpr TileGeom gc = tile_geom(KT > 0 ? KT : 1, RI > 0 ? RI : 1);
     constexpr bool kInt = sizeof(T) <= 2;  // integer frames: no NaN in the processed frame
@@ -1026,6 +1028,21 @@ __device__ __forceinline__ void detect_tile_body(
         off += static_cast<uint32_t>(__popcll(mk));
     }
     __syncthreads();
+    if constexpr (kCand) {
+        for (uint32_t c0 = static_cast<uint32_t>(wave) * 64u; c0 < ncand; c0 += FT_THREADS) {
+            const uint32_t ci = c0 + static_cast<uint32_t>(lane);
+            const bool v = ci < ncand;
+            uint32_t seed = 0;
+            if (v) {
+                const int li = s_cand[ci];
+                const int ty = li / FT_TW, cx = li - ty * FT_TW;
+                seed = static_cast<uint32_t>(y0 + ty) * static_cast<uint32_t>(W) + (x0 + cx);
+            }
+            const uint32_t pos = wave_append(v, slow_counter);
+            if (v && pos < slow_cap) fcand[pos] = seed;
+        }
+        return;
+    }
     if (static_cast<uint32_t>(wave) * 64u >= ncand) return;  // nothing left for this wave
 
     // 4-5. walks
@@ -1102,6 +1119,31 @@ __global__ __launch_bounds__(FT_THREADS, FT_MIN_WAVES) void k_detect_tile(
                                 out_cap, &ocount[fr * GT_COUNTER_STRIDE],
                                 slow + static_cast<size_t>(fr) * slow_cap, slow_cap,
                                 &scount[fr * GT_COUNTER_STRIDE], lds, s_cnt, s_masks);
+}
+
+// Candidate pixels of every tile (the local-maximum test of k_detect_tile),
+// listed per frame: cand[f * cap + i], count in ccount[f * GT_COUNTER_STRIDE].
+template <typename T, int KT, int RI>
+__global__ __launch_bounds__(FT_THREADS, FT_MIN_WAVES) void k_detect_cand(
+    const float *__restrict__ proc, GtKParams p, TileGeom g_rt, uint32_t *__restrict__ cand,
+    uint32_t cap, uint32_t *__restrict__ ccount, int nframes) {
+    constexpr bool kFixed = KT > 0 && RI > 0;
+    constexpr TileGeom gc = tile_geom(KT > 0 ? KT : 1, RI > 0 ? RI : 1);
+    const TileGeom g = kFixed ? gc : g_rt;
+    extern __shared__ __align__(16) float lds[];
+    __shared__ uint32_t s_cnt[FT_WAVES];
+    __shared__ uint64_t s_masks[FT_WAVES][FT_WROWS];
+    const int H = p.H, W = p.W;
+    const size_t HW = static_cast<size_t>(H) * W;
+    const int ntx = (W + FT_TW - 1) / FT_TW, nty = (H + FT_TH - 1) / FT_TH;
+    const int tile = xcd_tile(ntx * nty * nframes);
+    if (tile >= ntx * nty * nframes) return;
+    const int fr = tile / (ntx * nty), trest = tile - fr * (ntx * nty);
+    const int x0 = (trest % ntx) * FT_TW, y0 = (trest / ntx) * FT_TH;
+    detect_tile_body<T, KT, RI, true>(proc + static_cast<size_t>(fr) * HW, nullptr, p, H, W, g,
+                                      x0, y0, 0, nullptr, 0, nullptr, nullptr, cap,
+                                      &ccount[fr * GT_COUNTER_STRIDE], lds, s_cnt, s_masks,
+                                      cand + static_cast<size_t>(fr) * cap);
 }
 
 // Continues the walks k_detect_tile handed over: each entry is the seed and
@@ -1182,6 +1224,134 @@ __global__ __launch_bounds__(WK_THREADS) void k_characterize_det(const float *__
         }
         characterize_row<T, RI, true>(src, Rw, H, W, p.circle, row);
         rows[i].row = row;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Learned proposal: the walk's end position predicted per candidate (d =
+// (dy, dx) from the candidate pixel, confidence logit), checked by one walk
+// evaluation.
+
+// Integer offset of the walk's final centre for a predicted end offset d:
+// the walk moves one pixel per step while the centroid offset exceeds thr,
+// so it stops at the first centre within thr of the end position, the one
+// nearest the start: sign(d) * max(0, ceil(|d| - thr)).
+__device__ __forceinline__ int proposal_step(float d, float thr) {
+    const float a = fabsf(d) - thr;
+    const int k = a > 0.0f ? static_cast<int>(ceilf(a)) : 0;
+    return d < 0.0f ? -k : k;
+}
+
+// One lane per candidate: a proposal with logit >= conf_logit is accepted
+// when one walk evaluation at its centre does not move and lands within tol
+// (both axes) of the predicted end position; otherwise the full walk runs
+// from the candidate.  Rows as k_walk (minmass; characterize afterwards).
+// stats[0] += accepted proposals, stats[1] += candidates.
+template <int RI>
+__global__ __launch_bounds__(WK_THREADS) void k_walk_learned(
+    const float *__restrict__ proc, GtKParams p, const uint32_t *__restrict__ cand,
+    uint32_t cap, const uint32_t *__restrict__ ccount, const float4 *__restrict__ props,
+    float conf_logit, float tol, gt_detection *__restrict__ out, uint32_t out_cap,
+    uint32_t *__restrict__ ocount, unsigned long long *__restrict__ stats, uint32_t frame0) {
+    const int fr = blockIdx.y;
+    const int H = p.H, W = p.W;
+    const size_t HW = static_cast<size_t>(H) * W;
+    const float *P = proc + static_cast<size_t>(fr) * HW;
+    const uint32_t n = min(ccount[fr * GT_COUNTER_STRIDE], cap);
+    const uint32_t *fc = cand + static_cast<size_t>(fr) * cap;
+    const float4 *fp = props + static_cast<size_t>(fr) * cap;
+    gt_detection *fout = out + static_cast<size_t>(fr) * out_cap;
+    uint32_t *counter = &ocount[fr * GT_COUNTER_STRIDE];
+    const int rint = RI > 0 ? RI : p.rint;
+    const uint32_t lane = __lane_id();
+    const uint32_t stride = gridDim.x * WK_THREADS;
+    uint32_t n_acc = 0;
+    for (uint32_t i0 = blockIdx.x * WK_THREADS + (threadIdx.x & ~63u); i0 < n; i0 += stride) {
+        const uint32_t i = i0 + lane;
+        bool keep = false;
+        gt_detection d;
+        if (i < n) {
+            const uint32_t seed = fc[i];
+            const int cu = static_cast<int>(seed / static_cast<uint32_t>(W));
+            const int cv = static_cast<int>(seed - static_cast<uint32_t>(cu) * static_cast<uint32_t>(W));
+            const float4 pr = fp[i];
+            WalkRes res = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+            int pu = cu, pv = cv;
+            bool accepted = false;
+            if (pr.z >= conf_logit && fabsf(pr.x) < 64.0f && fabsf(pr.y) < 64.0f) {
+                // the centres the walk can reach from the candidate
+                const int qu = min(max(cu + proposal_step(pr.x, p.shift_thr), min(cu, rint)),
+                                   max(cu, H - 1 - rint));
+                const int qv = min(max(cv + proposal_step(pr.y, p.shift_thr), min(cv, rint)),
+                                   max(cv, W - 1 - rint));
+                const WalkRes r = walk_eval<RI, 0>(p, H, W, P, nullptr, 0, 0, 0, 1, 0, 1, 0, qu, qv);
+                int tu = qu, tv = qv;
+                const bool moves = walk_move(r, tu, tv, H, W, rint, p.shift_thr);
+                const float ey = r.x + static_cast<float>(qu) - (static_cast<float>(cu) + pr.x);
+                const float ex = r.y + static_cast<float>(qv) - (static_cast<float>(cv) + pr.y);
+                if (!moves && fabsf(ey) <= tol && fabsf(ex) <= tol) {
+                    accepted = true;
+                    res = r;
+                    pu = qu;
+                    pv = qv;
+                }
+            }
+            if (!accepted) {
+                for (uint32_t it = 0; it < p.max_iter; ++it) {
+                    res = walk_eval<RI, 0>(p, H, W, P, nullptr, 0, 0, 0, 1, 0, 1, 0, pu, pv);
+                    if (!walk_move(res, pu, pv, H, W, rint, p.shift_thr)) break;
+                }
+            }
+            n_acc += accepted ? 1u : 0u;
+            keep = finish_walk<uint16_t, RI, 0, false>(p, H, W, P, nullptr, nullptr, 0, 0, 0, 1, 0,
+                                                       1, 0, res, pu, pv, seed, frame0 + fr, d);
+        }
+        const uint32_t pos = wave_append(keep, counter);
+        if (keep && pos < out_cap) fout[pos] = d;
+    }
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) n_acc += __shfl_xor(n_acc, off);
+    if (lane == 0 && n_acc) atomicAdd(&stats[0], static_cast<unsigned long long>(n_acc));
+    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&stats[1], static_cast<unsigned long long>(n));
+}
+
+// Training targets: the full walk from every candidate (the plain method),
+// target (end row - candidate row, end column - candidate column, kept by
+// minmass, the candidate's raster index as uint32 bits) at flat index
+// offs[f] + i (dropped at maxn and beyond).  A walk ending without a
+// centroid (zero mass) gets offset (0, 0) and label 0.
+template <int RI>
+__global__ __launch_bounds__(WK_THREADS) void k_cnn_label(
+    const float *__restrict__ proc, GtKParams p, const uint32_t *__restrict__ cand, uint32_t cap,
+    const uint32_t *__restrict__ ccount, const uint32_t *__restrict__ offs, uint32_t maxn,
+    float4 *__restrict__ targets) {
+    const int fr = blockIdx.y;
+    const int H = p.H, W = p.W;
+    const float *P = proc + static_cast<size_t>(fr) * H * W;
+    const uint32_t n = min(ccount[fr * GT_COUNTER_STRIDE], cap);
+    const uint32_t *fc = cand + static_cast<size_t>(fr) * cap;
+    const int rint = RI > 0 ? RI : p.rint;
+    for (uint32_t i = blockIdx.x * WK_THREADS + threadIdx.x; i < n; i += gridDim.x * WK_THREADS) {
+        const uint32_t dst = offs[fr] + i;
+        if (dst >= maxn) break;
+        const uint32_t seed = fc[i];
+        const int cu = static_cast<int>(seed / static_cast<uint32_t>(W));
+        const int cv = static_cast<int>(seed - static_cast<uint32_t>(cu) * static_cast<uint32_t>(W));
+        int pu = cu, pv = cv;
+        WalkRes res = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+        for (uint32_t it = 0; it < p.max_iter; ++it) {
+            res = walk_eval<RI, 0>(p, H, W, P, nullptr, 0, 0, 0, 1, 0, 1, 0, pu, pv);
+            if (!walk_move(res, pu, pv, H, W, rint, p.shift_thr)) break;
+        }
+        res.x = res.x + static_cast<float>(pu);
+        res.y = res.y + static_cast<float>(pv);
+        // a disk without mass has no centroid (0 / 0): such a candidate
+        // trains the confidence only (offset 0, label 0)
+        const float ty = res.x - static_cast<float>(cu), tx = res.y - static_cast<float>(cv);
+        const bool fin = isfinite(ty) && isfinite(tx);
+        const bool kept = fin && passes_minmass(p, res);
+        targets[dst] = make_float4(fin ? ty : 0.0f, fin ? tx : 0.0f, kept ? 1.0f : 0.0f,
+                                   __uint_as_float(seed));
     }
 }
 
@@ -1425,6 +1595,75 @@ hipError_t gt_launch_walk_slow(const float *proc, const void *raw, int dtype, ui
                                   slow_cap, scount, out, out_cap, ocount, frame0, s);
     default: return hipErrorInvalidValue;
     }
+}
+
+namespace {
+template <typename T>
+hipError_t launch_detect_cand_t(const float *proc, uint32_t nframes, const GtKParams &p,
+                                uint32_t *cand, uint32_t cap, uint32_t *ccount, hipStream_t s) {
+    const TileGeom g = tile_geom(p.k, p.rint);
+    const size_t ntiles = static_cast<size_t>((p.W + FT_TW - 1) / FT_TW) *
+                          ((p.H + FT_TH - 1) / FT_TH) * nframes;
+    const dim3 grid(xcd_grid(ntiles));
+    const int nf = static_cast<int>(nframes);
+    const size_t lds = sizeof(float) * (static_cast<size_t>(g.rows) * g.pitch +
+                                        static_cast<size_t>(FT_TH + p.k - 1) * FT_TW);
+#define GT_CAND(KV, RV) \
+    k_detect_cand<T, KV, RV><<<grid, FT_THREADS, lds, s>>>(proc, p, g, cand, cap, ccount, nf)
+    if (p.rint == 4 && p.k == 9) GT_CAND(9, 4);
+    else if (p.rint == 4 && p.k == 12) GT_CAND(12, 4);
+    else if (p.rint == 4) GT_CAND(0, 4);
+    else GT_CAND(0, 0);
+#undef GT_CAND
+    return hipGetLastError();
+}
+
+dim3 walk_grid(const GtKParams &p, uint32_t nframes) {
+    const size_t hw = static_cast<size_t>(p.H) * p.W;
+    uint32_t blocks = static_cast<uint32_t>((hw / 512 + WK_THREADS - 1) / WK_THREADS);
+    blocks = blocks < 1 ? 1 : (blocks > 256 ? 256 : blocks);
+    return dim3(blocks, nframes, 1);
+}
+}  // namespace
+
+hipError_t gt_launch_detect_cand(const float *proc, int dtype, uint32_t nframes,
+                                 const GtKParams &p, uint32_t *cand, uint32_t cap,
+                                 uint32_t *ccount, hipStream_t s) {
+    switch (dtype) {
+    case GT_U16: return launch_detect_cand_t<uint16_t>(proc, nframes, p, cand, cap, ccount, s);
+    case GT_F32: return launch_detect_cand_t<float>(proc, nframes, p, cand, cap, ccount, s);
+    case GT_U8: return launch_detect_cand_t<uint8_t>(proc, nframes, p, cand, cap, ccount, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t gt_launch_walk_learned(const float *proc, uint32_t nframes, const GtKParams &p,
+                                  const uint32_t *cand, uint32_t cap, const uint32_t *ccount,
+                                  const float4 *props, float conf_logit, float tol,
+                                  gt_detection *out, uint32_t out_cap, uint32_t *ocount,
+                                  unsigned long long *stats, uint32_t frame0, hipStream_t s) {
+    const dim3 grid = walk_grid(p, nframes);
+    if (p.rint == 4)
+        k_walk_learned<4><<<grid, WK_THREADS, 0, s>>>(proc, p, cand, cap, ccount, props,
+                                                      conf_logit, tol, out, out_cap, ocount,
+                                                      stats, frame0);
+    else
+        k_walk_learned<0><<<grid, WK_THREADS, 0, s>>>(proc, p, cand, cap, ccount, props,
+                                                      conf_logit, tol, out, out_cap, ocount,
+                                                      stats, frame0);
+    return hipGetLastError();
+}
+
+hipError_t gt_launch_cnn_label(const float *proc, uint32_t nframes, const GtKParams &p,
+                               const uint32_t *cand, uint32_t cap, const uint32_t *ccount,
+                               const uint32_t *offs, uint32_t maxn, float4 *targets,
+                               hipStream_t s) {
+    const dim3 grid = walk_grid(p, nframes);
+    if (p.rint == 4)
+        k_cnn_label<4><<<grid, WK_THREADS, 0, s>>>(proc, p, cand, cap, ccount, offs, maxn, targets);
+    else
+        k_cnn_label<0><<<grid, WK_THREADS, 0, s>>>(proc, p, cand, cap, ccount, offs, maxn, targets);
+    return hipGetLastError();
 }
 
 hipError_t gt_launch_copy_counts(const uint32_t *strided, uint32_t *dst, uint32_t n,

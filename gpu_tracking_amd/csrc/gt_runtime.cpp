@@ -21,6 +21,7 @@
 #include <unordered_map>
 #include <vector>
 
+#include "gt_cnn.h"
 #include "gt_internal.h"
 #include "gt_host.h"
 
@@ -94,6 +95,14 @@ struct gt_context {
     bool log = false;              // LoG style
     GtLogTable *d_logtab = nullptr;  // LoG scale table (device copy)
     int32_t log_Rm = 0;
+
+    // learned centroid proposal (tiled Trackpy style only)
+    const gt_cnn *cnn = nullptr;   // caller's network; null = plain walk
+    float cnn_conf = 0.0f;         // logit threshold
+    float cnn_tol = 0.05f;         // verification tolerance (px)
+    float4 *d_props = nullptr;     // chunk * cand_cap proposals
+    unsigned long long *d_stats = nullptr;  // accepted, candidates
+    uint32_t *d_offs = nullptr;    // sample generation: chunk + 1 offsets, then the total
 
     // gt_execute staging: two slots, so one chunk's detections come back
     // and are post-processed while the next chunk runs on the GPU
@@ -175,6 +184,9 @@ void free_all(gt_context *c) {
     if (c->d_slow) (void)hipFree(c->d_slow);
     if (c->d_scount) (void)hipFree(c->d_scount);
     if (c->d_logtab) (void)hipFree(c->d_logtab);
+    if (c->d_props) (void)hipFree(c->d_props);
+    if (c->d_stats) (void)hipFree(c->d_stats);
+    if (c->d_offs) (void)hipFree(c->d_offs);
     for (auto &sl : c->slot) {
         if (sl.d_frames) (void)hipFree(sl.d_frames);
         if (sl.d_rows) (void)hipFree(sl.d_rows);
@@ -231,7 +243,25 @@ gt_status run_pipeline(gt_context *c, const void *d_frames, int32_t dtype, uint3
             if (ev) GT_HIP(hipEventRecord(ev[3], s));
             continue;
         }
-        if (c->tiled) {
+        if (c->tiled && c->cnn) {
+            // candidates, the network's proposals for all of them, one
+            // verifying walk evaluation each (full walk where it fails)
+            GtKParams kw = c->kp;
+            kw.characterize = 0;
+            GT_HIP(gt_launch_zero_u32(c->d_ccount, static_cast<size_t>(nf) * GT_COUNTER_STRIDE, s));
+            GT_HIP(gt_launch_detect_cand(c->d_proc, dtype, nf, kw, c->d_cand, c->cand_cap,
+                                         c->d_ccount, s));
+            if (ev) GT_HIP(hipEventRecord(ev[2], s));
+            GT_HIP(gt_cnn_launch_forward_frames(gt_cnn_packed(c->cnn), c->d_proc,
+                                                static_cast<int>(c->H), static_cast<int>(c->W), nf,
+                                                c->d_cand, c->cand_cap, c->d_ccount, c->d_props, s));
+            GT_HIP(gt_launch_walk_learned(c->d_proc, nf, kw, c->d_cand, c->cand_cap, c->d_ccount,
+                                          c->d_props, c->cnn_conf, c->cnn_tol, out, cap,
+                                          c->d_ocount, c->d_stats, f0, s));
+            if (c->kp.characterize)
+                GT_HIP(gt_launch_characterize_det(c->d_proc, frames, dtype, nf, c->kp, out, cap,
+                                                  c->d_ocount, s));
+        } else if (c->tiled) {
             // local max and walk in one kernel, the walks it hands over, then
             // characterize over the frame's compacted detections (whole
             // wavefronts of detections instead of a few lanes per tile)
@@ -367,7 +397,7 @@ gt_status gt_detect_device(gt_context *c, const void *d_frames, int32_t dtype, u
     if (nframes == 0) return fail(GT_ERR_EMPTY_ITERATOR, "the provided video doesn't contain any frames.");
     if (!d_frames || !d_out || !d_counts || cap_per_frame == 0)
         return fail(GT_ERR_INVALID_ARGUMENT, "null device buffer or zero capacity");
-    hipStream_t s = stream ? static_cast<hipStream_t>(stream) : c->stream;
+    hipStream_t s = static_cast<hipStream_t>(stream);  // NULL: the default stream
     return run_pipeline(c, d_frames, dtype, nframes, d_out, cap_per_frame, d_counts, s);
 }
 
@@ -606,6 +636,89 @@ gt_status gt_execute(gt_context *c, const void *frames, int32_t dtype, uint64_t 
     std::string ferr;
     gt_status fst = post.finish(out, &ferr);
     if (fst != GT_OK) return fail(fst, "%s", ferr.c_str());
+    return GT_OK;
+}
+
+gt_status gt_context_set_cnn(gt_context *c, const gt_cnn *net, float conf_logit, float tol) {
+    if (!c) return fail(GT_ERR_INVALID_ARGUMENT, "null context");
+    if (!net) {
+        c->cnn = nullptr;
+        return GT_OK;
+    }
+    if (!c->tiled || c->log)
+        return fail(GT_ERR_INVALID_ARGUMENT,
+                    "the learned proposal runs in the tiled Trackpy-style path only");
+    if (!std::isfinite(conf_logit) && conf_logit != INFINITY)
+        return fail(GT_ERR_INVALID_ARGUMENT, "confidence threshold must not be NaN or -inf");
+    if (!(tol >= 0.0f) || !std::isfinite(tol))
+        return fail(GT_ERR_INVALID_ARGUMENT, "tolerance must be finite and >= 0");
+    GT_HIP(hipStreamSynchronize(c->stream));
+    const size_t n = static_cast<size_t>(c->chunk) * c->cand_cap;
+    if (!c->d_cand) GT_HIP(hipMalloc(&c->d_cand, sizeof(uint32_t) * n));
+    if (!c->d_props) GT_HIP(hipMalloc(&c->d_props, sizeof(float4) * n));
+    if (!c->d_stats) {
+        GT_HIP(hipMalloc(&c->d_stats, 2 * sizeof(unsigned long long)));
+        GT_HIP(hipMemset(c->d_stats, 0, 2 * sizeof(unsigned long long)));
+    }
+    c->cnn = net;
+    c->cnn_conf = conf_logit;
+    c->cnn_tol = tol;
+    return GT_OK;
+}
+
+gt_status gt_context_cnn_stats(gt_context *c, uint64_t *accepted, uint64_t *candidates,
+                               int32_t reset) {
+    if (!c || !accepted || !candidates) return fail(GT_ERR_INVALID_ARGUMENT, "null argument");
+    unsigned long long h[2] = {0, 0};
+    if (c->d_stats) {
+        GT_HIP(hipDeviceSynchronize());
+        GT_HIP(hipMemcpy(h, c->d_stats, sizeof(h), hipMemcpyDeviceToHost));
+        if (reset) GT_HIP(hipMemset(c->d_stats, 0, sizeof(h)));
+    }
+    *accepted = h[0];
+    *candidates = h[1];
+    return GT_OK;
+}
+
+gt_status gt_cnn_make_samples(gt_context *c, const void *d_frames, int32_t dtype,
+                              uint32_t nframes, uint16_t *d_patches, float *d_targets,
+                              uint64_t cap, uint64_t *n_written, uint64_t *n_total, void *stream) {
+    if (!c) return fail(GT_ERR_INVALID_ARGUMENT, "null context");
+    if (!dtype_size(dtype)) return fail(GT_ERR_INVALID_ARGUMENT, "unknown dtype %d", dtype);
+    if (nframes == 0) return fail(GT_ERR_EMPTY_ITERATOR, "the provided video doesn't contain any frames.");
+    if (!d_frames || !d_patches || !d_targets || !n_written)
+        return fail(GT_ERR_INVALID_ARGUMENT, "null argument");
+    if (!c->tiled || c->log)
+        return fail(GT_ERR_INVALID_ARGUMENT,
+                    "training samples come from the tiled Trackpy-style path only");
+    hipStream_t s = static_cast<hipStream_t>(stream);  // NULL: the default stream
+    const size_t HW = static_cast<size_t>(c->H) * c->W;
+    if (!c->d_cand) GT_HIP(hipMalloc(&c->d_cand, sizeof(uint32_t) * c->chunk * static_cast<size_t>(c->cand_cap)));
+    if (!c->d_offs) GT_HIP(hipMalloc(&c->d_offs, sizeof(uint32_t) * (c->chunk + 2)));
+    uint32_t *d_total = c->d_offs + c->chunk + 1;
+    GT_HIP(gt_launch_zero_u32(d_total, 1, s));
+    const uint32_t maxn = static_cast<uint32_t>(std::min<uint64_t>(cap, 0xFFFFFFFFull));
+    const size_t esz = dtype_size(dtype);
+    GtKParams kw = c->kp;
+    kw.characterize = 0;
+    for (uint32_t f0 = 0; f0 < nframes; f0 += c->chunk) {
+        const uint32_t nf = std::min(c->chunk, nframes - f0);
+        const char *frames = static_cast<const char *>(d_frames) + esz * HW * f0;
+        GT_HIP(gt_launch_zero_u32(c->d_ccount, static_cast<size_t>(nf) * GT_COUNTER_STRIDE, s));
+        GT_HIP(gt_launch_bandpass(frames, dtype, c->d_proc, nf, c->kp, s));
+        GT_HIP(gt_launch_detect_cand(c->d_proc, dtype, nf, kw, c->d_cand, c->cand_cap, c->d_ccount, s));
+        GT_HIP(gt_cnn_launch_sample_offsets(c->d_ccount, c->cand_cap, nf, d_total, c->d_offs, s));
+        GT_HIP(gt_cnn_launch_gather(c->d_proc, static_cast<int>(c->H), static_cast<int>(c->W), nf,
+                                    c->d_cand, c->cand_cap, c->d_ccount, c->d_offs, maxn,
+                                    d_patches, s));
+        GT_HIP(gt_launch_cnn_label(c->d_proc, nf, kw, c->d_cand, c->cand_cap, c->d_ccount,
+                                   c->d_offs, maxn, reinterpret_cast<float4 *>(d_targets), s));
+    }
+    uint32_t total = 0;
+    GT_HIP(hipMemcpyAsync(&total, d_total, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    GT_HIP(hipStreamSynchronize(s));
+    *n_written = std::min<uint64_t>(total, maxn);
+    if (n_total) *n_total = total;
     return GT_OK;
 }
 

@@ -162,7 +162,7 @@ void gt_context_destroy(gt_context *ctx);
  *   d_counts: nframes uint32; d_counts[f] = number of detections of frame f
  *             (it may exceed cap_per_frame, in which case only cap_per_frame
  *             rows were written and the caller must treat it as an error).
- *   stream:   hipStream_t to enqueue on (NULL = the context's own stream).
+ *   stream:   hipStream_t to enqueue on (NULL = the default stream).
  * Asynchronous: returns after enqueueing.  No per-frame D2H copy. */
 gt_status gt_detect_device(gt_context *ctx, const void *d_frames, int32_t dtype,
                            uint32_t nframes, gt_detection *d_out, uint32_t cap_per_frame,
@@ -282,6 +282,70 @@ gt_status gt_progress(gt_context *ctx, uint64_t *cur, uint64_t *total);
 gt_status gt_set_timing(gt_context *ctx, int32_t enable);
 int32_t gt_stage_times(gt_context *ctx, double *stage_ms, int32_t n, uint32_t *launches,
                        uint32_t *frames_per_launch);
+
+/* ---- Learned centroid proposal ---------------------------------------------
+ * No counterpart in the reference.  The reference's walk is the iterative
+ * centroid loop of src/shaders/walk.wgsl:104-198.  Here a small CNN predicts,
+ * for every candidate pixel, where that walk ends.  One walk evaluation at the
+ * predicted centre verifies the prediction; the full walk runs when the
+ * verification fails or the confidence is low.
+ * The network: the 11x11 processed patch around the candidate (divided by
+ * its largest |value|, bf16) -> conv3x3 1->16 -> 16->32 -> 32->32 (valid,
+ * ReLU, bf16 activations, fp32 accumulation on the matrix cores) -> mean
+ * pooling -> linear 32->3: (dy, dx, confidence logit).
+ * Parameters: a float32 vector of GT_CNN_NPARAM entries, torch layouts in the
+ * order w1 (16,1,3,3) b1 (16) w2 (32,16,3,3) b2 (32) w3 (32,32,3,3) b3 (32)
+ * wfc (3,32) bfc (3).
+ * Training samples: patches bf16 [n][GT_CNN_PATCH_PITCH] (entries 121..127
+ * zero) and targets float32 [n][4] = (end row - candidate row, end column -
+ * candidate column, 1 if the plain method keeps the candidate, the
+ * candidate's raster index as uint32 bits), all in device memory.  Loss: label * |d - t|^2 + BCE(logit, label), batch mean;
+ * Adam with torch.optim.Adam's arithmetic (betas 0.9, 0.999; eps 1e-8). */
+#define GT_CNN_NPARAM 14147
+#define GT_CNN_PATCH_PITCH 128
+typedef struct gt_cnn gt_cnn;
+/* A network with He-uniform weights drawn from `seed` and zero biases. */
+gt_status gt_cnn_create(uint64_t seed, gt_cnn **out);
+void gt_cnn_destroy(gt_cnn *net);
+/* Host float32[GT_CNN_NPARAM]; setting resets the Adam state. */
+gt_status gt_cnn_set_params(gt_cnn *net, const float *params);
+gt_status gt_cnn_get_params(gt_cnn *net, float *params);
+gt_status gt_cnn_adam_steps(gt_cnn *net, uint64_t *steps);
+/* d_out: float32 [n][4] = (dy, dx, logit, 0) for n stored patches;
+ * asynchronous on `stream` (NULL = the default stream). */
+gt_status gt_cnn_forward(gt_cnn *net, const uint16_t *d_patches, uint64_t n, float *d_out,
+                         void *stream);
+/* Gradient, step and train run on the network's own stream and return when
+ * done; the samples and indices must already be complete in device memory.
+ * Gradient (host float32[GT_CNN_NPARAM]) and loss of the batch of samples
+ * d_idx[0..nb) (device uint32 indices, each < the number of samples). */
+gt_status gt_cnn_gradient(gt_cnn *net, const uint16_t *d_patches, const float *d_targets,
+                          const uint32_t *d_idx, uint32_t nb, float *grad, float *loss);
+/* One Adam step on the batch d_idx[0..nb); *loss = the batch loss before it. */
+gt_status gt_cnn_step(gt_cnn *net, const uint16_t *d_patches, const float *d_targets,
+                      const uint32_t *d_idx, uint32_t nb, float lr, float *loss);
+/* `epochs` passes over the n samples in batches of `batch` (a fresh seeded
+ * shuffle per epoch); loss_per_epoch[e] = mean batch loss of epoch e. */
+gt_status gt_cnn_train(gt_cnn *net, const uint16_t *d_patches, const float *d_targets,
+                       uint64_t n, uint32_t epochs, uint32_t batch, float lr, uint64_t seed,
+                       float *loss_per_epoch);
+/* Training samples of every candidate of nframes device frames, using the
+ * context's detection parameters (tiled Trackpy style): the plain method's
+ * full walk from each candidate gives the target.  Writes at most cap
+ * samples; *n_written = samples written, *n_total = candidates seen.  Runs
+ * on `stream` (NULL = the default stream) and returns when done. */
+gt_status gt_cnn_make_samples(gt_context *ctx, const void *d_frames, int32_t dtype,
+                              uint32_t nframes, uint16_t *d_patches, float *d_targets,
+                              uint64_t cap, uint64_t *n_written, uint64_t *n_total, void *stream);
+/* Detect with the network's proposals (NULL: the plain walk again).
+ * Proposals with logit < conf_logit, and proposals whose verifying
+ * evaluation moves or lands more than tol px from the prediction, take the
+ * full walk.  The context keeps the pointer: the network must outlive its
+ * use, and is not trained while the context detects. */
+gt_status gt_context_set_cnn(gt_context *ctx, const gt_cnn *net, float conf_logit, float tol);
+/* Accepted proposals and candidates since the last reset. */
+gt_status gt_context_cnn_stats(gt_context *ctx, uint64_t *accepted, uint64_t *candidates,
+                               int32_t reset);
 
 #ifdef __cplusplus
 }
