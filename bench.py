@@ -22,6 +22,15 @@ full workload's parameters; all videos go through one batch context
 for comparison, through one one-size context per video; the two must agree
 per frame (counts and an order-independent checksum of every row).
 
+Workload "learned": the full workload's frames and parameters, detected with
+the learned centroid proposal (a bf16 CNN on the matrix cores proposes where
+each candidate's walk ends; one walk evaluation verifies it; the full walk
+runs where it fails).  The network is first trained on the GPU on about 2M
+(patch, walked end position) pairs from frames of the same generator with
+disjoint seeds.  Reports learned and plain frames/s, the fraction of
+particles within 1e-3 px of the full walk, and the network kernel's MFMA
+roofline.
+
 Workload "log": 500 synthetic 2048x2048 float32 frames, 3000 particles per
 frame (sigma ~ U[1, 5], A ~ U[200, 2000], background 50 + N(0, 5^2)), LoG
 style with radii 1.5..6 in 10 log-spaced scales, characterize on; one step is
@@ -299,12 +308,171 @@ def run_batch_workload(args):
     print(json.dumps(res), flush=True)
 
 
+# matrix-core multiply-adds of one forward pass of the centroid network:
+# conv2 49 x 32 x 144, conv3 25 x 32 x 288 (conv1 and the head run on the VALU)
+CNN_MFMA_FLOP = 2 * (49 * 32 * 144 + 25 * 32 * 288)
+MFMA_BF16_PEAK_TFLOPS = 2500.0
+
+
+def _match_rows(plain, learned, tol=1e-3):
+    """Particles of `plain` (per-frame rows) found in `learned` (same seed)
+    within tol px, and the row counts."""
+    ok = n = extra = 0
+    for a, b in zip(plain, learned):
+        bs = {int(r["seed"]): r for r in b}
+        for r in a:
+            n += 1
+            q = bs.pop(int(r["seed"]), None)
+            if q is not None and abs(q["x"] - r["x"]) <= tol and abs(q["y"] - r["y"]) <= tol:
+                ok += 1
+        extra += len(bs)
+    return ok, n, extra
+
+
+def run_learned_workload(args):
+    import numpy as np
+    import torch
+
+    import gpu_tracking_amd as gt
+    from gpu_tracking_amd import synth
+    s = synth.spec("full")
+    H, W = s.nrows, s.ncols
+    params = dict(separation=7, minmass=400.0, characterize=True)
+    p = gt.make_params(9, **params)
+    ctx = gt.Context(p, H, W)
+    # training pairs from the same generator, seeds disjoint from the evaluation video
+    want = args.learned_samples
+    pchunks, tchunks, got, seed, nfr = [], [], 0, 20261203 + 9000, 0
+    t0 = time.perf_counter()
+    while got < want:
+        v = synth.video_torch(s, 32, seed=seed, device="cuda")
+        P, T, _ = ctx.training_samples(v)
+        pchunks.append(P.clone())
+        tchunks.append(T.clone())
+        got += P.shape[0]
+        nfr += 32
+        seed += 1
+        del v, P, T
+    P = torch.cat(pchunks)[:want].contiguous()
+    T = torch.cat(tchunks)[:want].contiguous()
+    del pchunks, tchunks
+    torch.cuda.synchronize()
+    t_samples = time.perf_counter() - t0
+    print(f"[bench] {P.shape[0]} training pairs from {nfr} frames in {t_samples:.1f} s "
+          f"({float(T[:, 2].mean()):.3f} kept)", file=sys.stderr, flush=True)
+    net = gt.CentroidNet(seed=7)
+    t0 = time.perf_counter()
+    losses = net.train(P, T, epochs=args.learned_epochs, batch=4096, lr=1e-3, seed=11)
+    t_train = time.perf_counter() - t0
+    steps_per_epoch = (P.shape[0] + 4095) // 4096
+    print(f"[bench] trained {args.learned_epochs} epochs ({steps_per_epoch} steps each) in "
+          f"{t_train:.1f} s, losses {losses}", file=sys.stderr, flush=True)
+
+    # the network's forward kernel alone over stored patches (MFMA roofline)
+    nfw = min(P.shape[0], 1 << 20)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    net.forward(P[:nfw])
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(5):
+        net.forward(P[:nfw])
+    e1.record()
+    torch.cuda.synchronize()
+    fw_s = e0.elapsed_time(e1) * 1e-3 / 5
+    fw_tflops = nfw * CNN_MFMA_FLOP / fw_s / 1e12
+
+    T_eval = args.frames or 2000
+    frames = synth.video_torch(s, T_eval, seed=20261203 + 2000, device="cuda")
+    cap = 8192
+    out_p = torch.empty((T_eval, cap, 12), dtype=torch.int32, device="cuda")
+    cnt_p = torch.zeros((T_eval,), dtype=torch.int32, device="cuda")
+    out_l = torch.empty_like(out_p)
+    cnt_l = torch.zeros_like(cnt_p)
+    stream = torch.cuda.current_stream().cuda_stream
+
+    def timed(out, cnt):
+        for _ in range(args.warmup):
+            ctx.detect_device(frames, out=out, counts=cnt, cap_per_frame=cap, stream=stream)
+        torch.cuda.synchronize()
+        ctx.set_timing(True)
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            ctx.detect_device(frames, out=out, counts=cnt, cap_per_frame=cap, stream=stream)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        st = ctx.stage_times()
+        ctx.set_timing(False)
+        return T_eval * args.steps / dt, st
+
+    fps_plain, st_plain = timed(out_p, cnt_p)
+    nchk = min(T_eval, 256)
+    rows_p = gt.detections_to_numpy(out_p[:nchk], cnt_p[:nchk])
+    scopes = {}
+    for scope in ("all", "moving"):      # "moving" last: its rows stay in out_l
+        ctx.set_learned(net, scope=scope)
+        ctx.learned_stats(reset=True)
+        fps_l, st_l = timed(out_l, cnt_l)
+        acc, cands = ctx.learned_stats(reset=True)
+        ctx.set_learned(None)
+        rows_l = gt.detections_to_numpy(out_l[:nchk], cnt_l[:nchk])
+        ok, n, extra = _match_rows(rows_p, rows_l)
+        scopes[scope] = {
+            "frames_per_s": round(fps_l, 1), "speedup_vs_plain": round(fps_l / fps_plain, 4),
+            "stage_ms_per_step": [round(x, 3) for x in st_l[0]],
+            "proposals": {"accepted": acc, "listed": cands,
+                          "accepted_fraction": round(acc / max(cands, 1), 4),
+                          "listed_per_frame": round(cands / (T_eval * (args.steps + args.warmup)), 1)},
+            "accuracy": {"frames": nchk, "particles": n, "within_1e-3_px": ok,
+                         "fraction": round(ok / max(n, 1), 6), "extra_rows": extra}}
+        print(f"[bench] learned ({scope}) {fps_l:.0f} frames/s vs plain {fps_plain:.0f}; "
+              f"stages {st_l[0]}; {ok}/{n} within 1e-3 px", file=sys.stderr, flush=True)
+    fps_learned = scopes["moving"]["frames_per_s"]
+    cpu = None
+    if not args.no_cpu:
+        host = frames[:min(T_eval, 2048)].cpu().numpy().view(np.uint16)
+        rows = gt.detections_to_numpy(out_p[:host.shape[0]], cnt_p[:host.shape[0]])
+        print("[bench] CPU baseline", file=sys.stderr, flush=True)
+        cpu, _, _ = _cpu_baseline(host, dict(diameter=9, **params), args.cpu_seconds, rows)
+    ctx.close()
+    res = {
+        "metric": "frames/s (learned centroid proposal for moving walks, detect+characterize, "
+                  "1024^2 u16)",
+        "value": round(fps_learned, 1), "unit": "frames/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(T_eval / fps_learned * 1e3, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "bf16",
+        "data": "synthetic (seeded, rendered on the GPU)",
+        "config": {"workload": f"learned: {T_eval} frames {H}x{W} u16, {s.n_particles} "
+                               "particles/frame, diameter=9, separation=7, minmass=400, "
+                               "characterize=True; network trained on "
+                               f"{P.shape[0]} pairs from {nfr} frames of disjoint seeds"},
+        "plain_frames_per_s": round(fps_plain, 1),
+        "speedup_vs_plain": round(fps_learned / fps_plain, 4),
+        "plain_stage_ms_per_step": [round(x, 3) for x in st_plain[0]],
+        "scopes": scopes,
+        "accuracy": scopes["moving"]["accuracy"],
+        "training": {"pairs": int(P.shape[0]), "frames": nfr, "epochs": args.learned_epochs,
+                     "batch": 4096, "lr": 1e-3, "steps": steps_per_epoch * args.learned_epochs,
+                     "seconds": round(t_train, 2), "sample_seconds": round(t_samples, 2),
+                     "loss_per_epoch": [round(float(x), 5) for x in losses]},
+        "roofline": {"bound": "mfma", "achieved": round(fw_tflops, 2),
+                     "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(fw_tflops / MFMA_BF16_PEAK_TFLOPS, 5), "traffic": None,
+                     "kernel": "k_cnn_forward", "patches": nfw,
+                     "alg_flop_per_patch": CNN_MFMA_FLOP,
+                     "launch_ms": round(fw_s * 1e3, 4)},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(res), flush=True)
+    net.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="full", choices=sorted(WORKLOADS) + ["batch"])
+    ap.add_argument("--workload", default="full",
+                    choices=sorted(WORKLOADS) + ["batch", "learned"])
     ap.add_argument("--frames", type=int, default=0, help="override the frame count")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -313,6 +481,10 @@ def main():
     ap.add_argument("--e2e-frames", type=int, default=256,
                     help="frames of the end-to-end run (host frames in, DataFrame columns out, "
                          "with linking; plain workloads only; 0 = skip)")
+    ap.add_argument("--learned-samples", type=int, default=2_000_000,
+                    help="learned workload: training pairs")
+    ap.add_argument("--learned-epochs", type=int, default=4,
+                    help="learned workload: training epochs")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a 1 GiB device-to-device copy (a known-size kernel that "
                          "calibrates rocprof's FETCH_SIZE/WRITE_SIZE counters)")
@@ -324,6 +496,9 @@ def main():
 
     if args.workload == "batch":
         run_batch_workload(args)
+        return
+    if args.workload == "learned":
+        run_learned_workload(args)
         return
 
     import numpy as np

@@ -182,7 +182,7 @@ def lib() -> C.CDLL:
         "gt_cnn_step": [vp, vp, vp, vp, u32, f32, P(f32)],
         "gt_cnn_train": [vp, vp, vp, u64, u32, u32, f32, u64, vp],
         "gt_cnn_make_samples": [vp, vp, C.c_int32, u32, vp, vp, u64, P(u64), P(u64), vp],
-        "gt_context_set_cnn": [vp, vp, f32, f32],
+        "gt_context_set_cnn": [vp, vp, f32, f32, C.c_int32],
         "gt_context_cnn_stats": [vp, P(u64), P(u64), C.c_int32],
     }
     for name, args in sigs.items():

@@ -221,13 +221,19 @@ class Context:
         return out, counts
 
     def set_learned(self, net: Optional["CentroidNet"], conf_logit: float = 0.0,
-                    tol: float = 0.05) -> None:
+                    tol: float = 0.05, scope: str = "moving") -> None:
         """Detect with the network's centroid proposals (None: the plain
-        walk).  A proposal is kept when its logit >= conf_logit and one walk
-        evaluation at its centre neither moves nor lands more than tol px
-        away; every other candidate takes the full walk."""
+        walk).  scope "moving": the network proposes for the candidates whose
+        first walk step moves (a walk that stops at once is finished by that
+        step); "all": for every candidate.  A proposal is kept when its logit
+        >= conf_logit and one walk evaluation at its centre neither moves nor
+        lands more than tol px away; every other candidate finishes the full
+        walk."""
+        scopes = {"moving": 0, "all": 1}
+        if scope not in scopes:
+            raise ValueError(f"scope must be one of {sorted(scopes)}")
         N.check(self._lib.gt_context_set_cnn(self._h, None if net is None else net.handle,
-                                             float(conf_logit), float(tol)))
+                                             float(conf_logit), float(tol), scopes[scope]))
         self._net = net
 
     def learned_stats(self, reset: bool = False):
@@ -240,8 +246,8 @@ class Context:
         """Training pairs from device frames (T, Y, X): every candidate's
         normalised patch (bfloat16 (n, 128), entries 121.. zero) and target
         (float32 (n, 4): end row - candidate row, end column - candidate
-        column, kept by minmass, candidate raster index as uint32 bits) from
-        the plain method's full walk.
+        column, 1 if the walk converges, candidate raster index as uint32
+        bits) from the plain method's full walk.
         Returns (patches, targets, candidates seen)."""
         import torch
         if frames.dim() == 2:
@@ -311,10 +317,11 @@ def batch(file_or_array, diameter, **kwargs):
     learned = kwargs.pop("learned", None)
     learned_conf = kwargs.pop("learned_conf", 0.0)
     learned_tol = kwargs.pop("learned_tol", 0.05)
+    learned_scope = kwargs.pop("learned_scope", "moving")
     p = make_params(diameter, **kwargs)
     with Context(p, video.shape[1], video.shape[2]) as ctx:
         if learned is not None:
-            ctx.set_learned(learned, learned_conf, learned_tol)
+            ctx.set_learned(learned, learned_conf, learned_tol, learned_scope)
         data, names, types = ctx.execute(video, keys=frames, linker_reset_points=reset)
     return _to_dataframe(data, names, types)
 

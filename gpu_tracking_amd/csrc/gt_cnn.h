@@ -51,11 +51,12 @@ hipError_t gt_cnn_launch_pack(const float *params, void *pk, hipStream_t s);
 hipError_t gt_cnn_launch_forward_patches(const void *pk, const uint16_t *patches, uint32_t n,
                                          float4 *out, hipStream_t s);
 // Forward over the candidate lists of a chunk of frames (frame f: proc + f * H * W,
-// candidates cand[f * cap ...], count min(ccount[f * stride], cap)) ->
-// props[f * cap + i].
+// candidate i's pixel cand[(f * cap + i) * cstride], cstride 1 or 2, count
+// min(ccount[f * GT_COUNTER_STRIDE], cap)) -> props[f * cap + i].
 hipError_t gt_cnn_launch_forward_frames(const void *pk, const float *proc, int H, int W,
-                                        uint32_t nframes, const uint32_t *cand, uint32_t cap,
-                                        const uint32_t *ccount, float4 *props, hipStream_t s);
+                                        uint32_t nframes, const uint32_t *cand, uint32_t cstride,
+                                        uint32_t cap, const uint32_t *ccount, float4 *props,
+                                        hipStream_t s);
 // Stored patches of every candidate of a chunk, at flat index offs[f] + i
 // (offs from gt_cnn_launch_sample_offsets; indices >= maxn are dropped).
 hipError_t gt_cnn_launch_gather(const float *proc, int H, int W, uint32_t nframes,

@@ -336,7 +336,7 @@ __global__ __launch_bounds__(CT, 2) void k_cnn_forward(const uint8_t *__restrict
                                                        uint32_t n, const float *__restrict__ proc,
                                                        int H, int W,
                                                        const uint32_t *__restrict__ cand,
-                                                       uint32_t cap,
+                                                       uint32_t cstride, uint32_t cap,
                                                        const uint32_t *__restrict__ ccount,
                                                        float4 *__restrict__ out) {
     extern __shared__ __align__(16) uint8_t smem[];
@@ -352,7 +352,7 @@ __global__ __launch_bounds__(CT, 2) void k_cnn_forward(const uint8_t *__restrict
         const uint32_t f = blockIdx.y;
         count = min(ccount[f * GT_COUNTER_STRIDE], cap);
         P = proc + static_cast<size_t>(f) * H * W;
-        fc = cand + static_cast<size_t>(f) * cap;
+        fc = cand + static_cast<size_t>(f) * cap * cstride;
         fo = out + static_cast<size_t>(f) * cap;
     }
     WaveAct &A = Aall[wave];
@@ -364,7 +364,8 @@ __global__ __launch_bounds__(CT, 2) void k_cnn_forward(const uint8_t *__restrict
         const bool valid = j < count;
         if constexpr (kFrames) {
             uint32_t h0, h1;
-            gather_norm(P, H, W, valid ? fc[j] : 0u, valid, lane, h0, h1);
+            gather_norm(P, H, W, valid ? fc[static_cast<size_t>(j) * cstride] : 0u, valid, lane,
+                        h0, h1);
             A.x[lane] = bf16_val(h0);
             A.x[lane + 64] = bf16_val(h1);
         } else {
@@ -665,19 +666,21 @@ hipError_t gt_cnn_launch_forward_patches(const void *pk, const uint16_t *patches
     if (e != hipSuccess) return e;
     const uint32_t blocks = std::min<uint32_t>(512u, (n + 3) / 4);
     k_cnn_forward<false><<<blocks, CT, kFwdLds, s>>>(static_cast<const uint8_t *>(pk), patches, n,
-                                                     nullptr, 0, 0, nullptr, 0, nullptr, out);
+                                                     nullptr, 0, 0, nullptr, 1, 0, nullptr, out);
     return hipGetLastError();
 }
 
 hipError_t gt_cnn_launch_forward_frames(const void *pk, const float *proc, int H, int W,
-                                        uint32_t nframes, const uint32_t *cand, uint32_t cap,
-                                        const uint32_t *ccount, float4 *props, hipStream_t s) {
+                                        uint32_t nframes, const uint32_t *cand, uint32_t cstride,
+                                        uint32_t cap, const uint32_t *ccount, float4 *props,
+                                        hipStream_t s) {
     if (nframes == 0) return hipSuccess;
+    if (cstride != 1 && cstride != 2) return hipErrorInvalidValue;
     hipError_t e = allow_lds(k_cnn_forward<true>, kFwdLds);
     if (e != hipSuccess) return e;
     const uint32_t bx = std::max<uint32_t>(1u, std::min<uint32_t>(32u, 1024u / nframes));
     k_cnn_forward<true><<<dim3(bx, nframes), CT, kFwdLds, s>>>(
-        static_cast<const uint8_t *>(pk), nullptr, 0, proc, H, W, cand, cap, ccount, props);
+        static_cast<const uint8_t *>(pk), nullptr, 0, proc, H, W, cand, cstride, cap, ccount, props);
     return hipGetLastError();
 }
 

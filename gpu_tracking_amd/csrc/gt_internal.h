@@ -29,10 +29,13 @@ hipError_t gt_launch_characterize_rows(const float *proc, const void *raw, int d
                                        hipStream_t s);
 // Fused local max + walk + characterize over LDS tiles (the main path).
 size_t gt_tile_lds_bytes(const GtKParams &p);
+// walk_steps (1 or 2) steps run in the tile; walks still moving after them
+// go to the slow list as (seed, position).
 hipError_t gt_launch_detect_tile(const float *proc, const void *raw, int dtype, uint32_t nframes,
                                  const GtKParams &p, gt_detection *out, uint32_t out_cap,
                                  uint32_t *ocount, uint2 *slow, uint32_t slow_cap,
-                                 uint32_t *scount, uint32_t frame0, hipStream_t s);
+                                 uint32_t *scount, uint32_t frame0, hipStream_t s,
+                                 uint32_t walk_steps = 2);
 // Finishes the walks the tile kernel handed over (slow list: seed, position).
 hipError_t gt_launch_walk_slow(const float *proc, const void *raw, int dtype, uint32_t nframes,
                                const GtKParams &p, const uint2 *slow, uint32_t slow_cap,
@@ -82,11 +85,14 @@ void gt_batch_tiles(int H, int W, uint32_t *bp_tiles, uint32_t *dt_tiles);
 hipError_t gt_launch_detect_cand(const float *proc, int dtype, uint32_t nframes,
                                  const GtKParams &p, uint32_t *cand, uint32_t cap,
                                  uint32_t *ccount, hipStream_t s);
+// List entries: stride 1 = candidate pixel; stride 2 = (candidate pixel,
+// position after it0 walk steps), the slow list of the tile kernel.
 hipError_t gt_launch_walk_learned(const float *proc, uint32_t nframes, const GtKParams &p,
-                                  const uint32_t *cand, uint32_t cap, const uint32_t *ccount,
-                                  const float4 *props, float conf_logit, float tol,
-                                  gt_detection *out, uint32_t out_cap, uint32_t *ocount,
-                                  unsigned long long *stats, uint32_t frame0, hipStream_t s);
+                                  const uint32_t *cand, uint32_t stride, uint32_t it0,
+                                  uint32_t cap, const uint32_t *ccount, const float4 *props,
+                                  float conf_logit, float tol, gt_detection *out,
+                                  uint32_t out_cap, uint32_t *ocount, unsigned long long *stats,
+                                  uint32_t frame0, hipStream_t s);
 hipError_t gt_launch_cnn_label(const float *proc, uint32_t nframes, const GtKParams &p,
                                const uint32_t *cand, uint32_t cap, const uint32_t *ccount,
                                const uint32_t *offs, uint32_t maxn, float4 *targets,

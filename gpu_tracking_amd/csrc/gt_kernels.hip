@@ -858,8 +858,9 @@ __device__ __forceinline__ bool finish_walk(const GtKParams &p, int H, int W,
 // *counter), walks handed over to fslow (counted in *slow_counter); `frame`
 // is the rows' frame field.  With kCand the tile only lists its candidate
 // pixels (raster index) in fcand, counted in *slow_counter (capacity
-// slow_cap), for the learned proposal and its verifying walk.
-template <typename T, int KT, int RI, bool kCand = false>
+// slow_cap), for the learned proposal and its verifying walk.  kSteps walk
+// steps run in the tile before a walk still moving is handed over.
+template <typename T, int KT, int RI, bool kCand = false, uint32_t kSteps = FT_WALK_STEPS>
 __device__ __forceinline__ void detect_tile_body(
     const float *__restrict__ P, const T *__restrict__ Rw, const GtKParams &p, int H, int W,
     const TileGeom &g, int x0, int y0, uint32_t frame, gt_detection *__restrict__ fout,
@@ -1064,7 +1065,7 @@ __device__ __forceinline__ void detect_tile_body(
             WalkRes res = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
             bool moving = true;
             uint32_t it = 0;
-            for (; it < p.max_iter && it < FT_WALK_STEPS; ++it) {
+            for (; it < p.max_iter && it < kSteps; ++it) {
                 res = walk_eval<RI, PITCH>(p, H, W, P, Sp, oy, ox, RW, ylo, yhi, xlo, xhi, pu, pv);
                 if (!walk_move(res, pu, pv, H, W, rint, p.shift_thr)) {
                     moving = false;
@@ -1082,7 +1083,7 @@ __device__ __forceinline__ void detect_tile_body(
                 fslow[spos] = make_uint2(seed, static_cast<uint32_t>(pu) * static_cast<uint32_t>(W) + pv);
             } else {
                 WalkRes res = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
-                for (uint32_t it = FT_WALK_STEPS; it < p.max_iter; ++it) {
+                for (uint32_t it = kSteps; it < p.max_iter; ++it) {
                     res = walk_eval<RI, PITCH>(p, H, W, P, Sp, oy, ox, RW, ylo, yhi, xlo, xhi, pu, pv);
                     if (!walk_move(res, pu, pv, H, W, rint, p.shift_thr)) break;
                 }
@@ -1095,7 +1096,7 @@ __device__ __forceinline__ void detect_tile_body(
     }
 }
 
-template <typename T, int KT, int RI>
+template <typename T, int KT, int RI, uint32_t kSteps = FT_WALK_STEPS>
 __global__ __launch_bounds__(FT_THREADS, FT_MIN_WAVES) void k_detect_tile(
     const float *__restrict__ proc, const T *__restrict__ raw, GtKParams p, TileGeom g_rt,
     gt_detection *__restrict__ out, uint32_t out_cap, uint32_t *__restrict__ ocount,
@@ -1114,7 +1115,8 @@ __global__ __launch_bounds__(FT_THREADS, FT_MIN_WAVES) void k_detect_tile(
     if (tile >= ntx * nty * nframes) return;
     const int fr = tile / (ntx * nty), trest = tile - fr * (ntx * nty);
     const int x0 = (trest % ntx) * FT_TW, y0 = (trest / ntx) * FT_TH;
-    detect_tile_body<T, KT, RI>(proc + static_cast<size_t>(fr) * HW, raw + static_cast<size_t>(fr) * HW,
+    detect_tile_body<T, KT, RI, false, kSteps>(
+                                proc + static_cast<size_t>(fr) * HW, raw + static_cast<size_t>(fr) * HW,
                                 p, H, W, g, x0, y0, frame0 + fr, out + static_cast<size_t>(fr) * out_cap,
                                 out_cap, &ocount[fr * GT_COUNTER_STRIDE],
                                 slow + static_cast<size_t>(fr) * slow_cap, slow_cap,
@@ -1242,62 +1244,80 @@ __device__ __forceinline__ int proposal_step(float d, float thr) {
     return d < 0.0f ? -k : k;
 }
 
-// One lane per candidate: a proposal with logit >= conf_logit is accepted
-// when one walk evaluation at its centre does not move and lands within tol
-// (both axes) of the predicted end position; otherwise the full walk runs
-// from the candidate.  Rows as k_walk (minmass; characterize afterwards).
-// stats[0] += accepted proposals, stats[1] += candidates.
+// One lane per listed candidate: entry i of frame f is list[(f * cap + i) *
+// stride] (the candidate pixel) and, when stride is 2, the walk position
+// after its first it0 steps in the next word.  A proposal with logit >=
+// conf_logit is accepted when one walk evaluation at its centre does not
+// move and lands within tol (both axes) of the predicted end position;
+// otherwise the walk goes on from that position and step (the full walk).
+// Rows as k_walk (minmass; characterize afterwards).  stats[0] += accepted
+// proposals, stats[1] += listed candidates.
 template <int RI>
 __global__ __launch_bounds__(WK_THREADS) void k_walk_learned(
     const float *__restrict__ proc, GtKParams p, const uint32_t *__restrict__ cand,
-    uint32_t cap, const uint32_t *__restrict__ ccount, const float4 *__restrict__ props,
-    float conf_logit, float tol, gt_detection *__restrict__ out, uint32_t out_cap,
-    uint32_t *__restrict__ ocount, unsigned long long *__restrict__ stats, uint32_t frame0) {
+    uint32_t stride, uint32_t it0, uint32_t cap, const uint32_t *__restrict__ ccount,
+    const float4 *__restrict__ props, float conf_logit, float tol,
+    gt_detection *__restrict__ out, uint32_t out_cap, uint32_t *__restrict__ ocount,
+    unsigned long long *__restrict__ stats, uint32_t frame0) {
     const int fr = blockIdx.y;
     const int H = p.H, W = p.W;
     const size_t HW = static_cast<size_t>(H) * W;
     const float *P = proc + static_cast<size_t>(fr) * HW;
     const uint32_t n = min(ccount[fr * GT_COUNTER_STRIDE], cap);
-    const uint32_t *fc = cand + static_cast<size_t>(fr) * cap;
+    const uint32_t *fc = cand + static_cast<size_t>(fr) * cap * stride;
     const float4 *fp = props + static_cast<size_t>(fr) * cap;
     gt_detection *fout = out + static_cast<size_t>(fr) * out_cap;
     uint32_t *counter = &ocount[fr * GT_COUNTER_STRIDE];
     const int rint = RI > 0 ? RI : p.rint;
     const uint32_t lane = __lane_id();
-    const uint32_t stride = gridDim.x * WK_THREADS;
+    const uint32_t gstride = gridDim.x * WK_THREADS;
     uint32_t n_acc = 0;
-    for (uint32_t i0 = blockIdx.x * WK_THREADS + (threadIdx.x & ~63u); i0 < n; i0 += stride) {
+    for (uint32_t i0 = blockIdx.x * WK_THREADS + (threadIdx.x & ~63u); i0 < n; i0 += gstride) {
         const uint32_t i = i0 + lane;
         bool keep = false;
         gt_detection d;
         if (i < n) {
-            const uint32_t seed = fc[i];
+            const uint32_t seed = fc[static_cast<size_t>(i) * stride];
+            const uint32_t pos = fc[static_cast<size_t>(i) * stride + stride - 1];
             const int cu = static_cast<int>(seed / static_cast<uint32_t>(W));
             const int cv = static_cast<int>(seed - static_cast<uint32_t>(cu) * static_cast<uint32_t>(W));
             const float4 pr = fp[i];
             WalkRes res = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
-            int pu = cu, pv = cv;
-            bool accepted = false;
+            int pu = static_cast<int>(pos / static_cast<uint32_t>(W));
+            int pv = static_cast<int>(pos - static_cast<uint32_t>(pu) * static_cast<uint32_t>(W));
+            uint32_t it = it0;
+            bool done = false, accepted = false;
             if (pr.z >= conf_logit && fabsf(pr.x) < 64.0f && fabsf(pr.y) < 64.0f) {
                 // the centres the walk can reach from the candidate
                 const int qu = min(max(cu + proposal_step(pr.x, p.shift_thr), min(cu, rint)),
                                    max(cu, H - 1 - rint));
                 const int qv = min(max(cv + proposal_step(pr.y, p.shift_thr), min(cv, rint)),
                                    max(cv, W - 1 - rint));
-                const WalkRes r = walk_eval<RI, 0>(p, H, W, P, nullptr, 0, 0, 0, 1, 0, 1, 0, qu, qv);
-                int tu = qu, tv = qv;
-                const bool moves = walk_move(r, tu, tv, H, W, rint, p.shift_thr);
-                const float ey = r.x + static_cast<float>(qu) - (static_cast<float>(cu) + pr.x);
-                const float ex = r.y + static_cast<float>(qv) - (static_cast<float>(cv) + pr.y);
-                if (!moves && fabsf(ey) <= tol && fabsf(ex) <= tol) {
-                    accepted = true;
-                    res = r;
-                    pu = qu;
-                    pv = qv;
+                // a proposal away from a candidate whose walk has not started
+                // is taken only if the walk leaves the candidate: a walk
+                // that stops there ends there, wherever else is stable
+                if (it == 0 && (qu != cu || qv != cv) && p.max_iter > 0) {
+                    res = walk_eval<RI, 0>(p, H, W, P, nullptr, 0, 0, 0, 1, 0, 1, 0, pu, pv);
+                    it = 1;
+                    done = !walk_move(res, pu, pv, H, W, rint, p.shift_thr);
+                }
+                if (!done) {
+                    const WalkRes r =
+                        walk_eval<RI, 0>(p, H, W, P, nullptr, 0, 0, 0, 1, 0, 1, 0, qu, qv);
+                    int tu = qu, tv = qv;
+                    const bool moves = walk_move(r, tu, tv, H, W, rint, p.shift_thr);
+                    const float ey = r.x + static_cast<float>(qu) - (static_cast<float>(cu) + pr.x);
+                    const float ex = r.y + static_cast<float>(qv) - (static_cast<float>(cv) + pr.y);
+                    if (!moves && fabsf(ey) <= tol && fabsf(ex) <= tol) {
+                        accepted = done = true;
+                        res = r;
+                        pu = qu;
+                        pv = qv;
+                    }
                 }
             }
-            if (!accepted) {
-                for (uint32_t it = 0; it < p.max_iter; ++it) {
+            if (!done) {
+                for (; it < p.max_iter; ++it) {
                     res = walk_eval<RI, 0>(p, H, W, P, nullptr, 0, 0, 0, 1, 0, 1, 0, pu, pv);
                     if (!walk_move(res, pu, pv, H, W, rint, p.shift_thr)) break;
                 }
@@ -1316,10 +1336,12 @@ __global__ __launch_bounds__(WK_THREADS) void k_walk_learned(
 }
 
 // Training targets: the full walk from every candidate (the plain method),
-// target (end row - candidate row, end column - candidate column, kept by
-// minmass, the candidate's raster index as uint32 bits) at flat index
-// offs[f] + i (dropped at maxn and beyond).  A walk ending without a
-// centroid (zero mass) gets offset (0, 0) and label 0.
+// target (end row - candidate row, end column - candidate column, 1 if the
+// walk converged, the candidate's raster index as uint32 bits) at flat
+// index offs[f] + i (dropped at maxn and beyond).  A walk that does not
+// converge (max_iter reached, or no centroid: zero mass) gets offset (0, 0)
+// and label 0.  minmass plays no part: it is applied after the verifying
+// evaluation, exactly as after the walk.
 template <int RI>
 __global__ __launch_bounds__(WK_THREADS) void k_cnn_label(
     const float *__restrict__ proc, GtKParams p, const uint32_t *__restrict__ cand, uint32_t cap,
@@ -1339,18 +1361,21 @@ __global__ __launch_bounds__(WK_THREADS) void k_cnn_label(
         const int cv = static_cast<int>(seed - static_cast<uint32_t>(cu) * static_cast<uint32_t>(W));
         int pu = cu, pv = cv;
         WalkRes res = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+        bool stopped = false;
         for (uint32_t it = 0; it < p.max_iter; ++it) {
             res = walk_eval<RI, 0>(p, H, W, P, nullptr, 0, 0, 0, 1, 0, 1, 0, pu, pv);
-            if (!walk_move(res, pu, pv, H, W, rint, p.shift_thr)) break;
+            if (!walk_move(res, pu, pv, H, W, rint, p.shift_thr)) {
+                stopped = true;
+                break;
+            }
         }
         res.x = res.x + static_cast<float>(pu);
         res.y = res.y + static_cast<float>(pv);
-        // a disk without mass has no centroid (0 / 0): such a candidate
-        // trains the confidence only (offset 0, label 0)
+        // label 1: the walk converged (stopped by the move rule within
+        // max_iter, with a centroid); otherwise offset (0, 0), label 0
         const float ty = res.x - static_cast<float>(cu), tx = res.y - static_cast<float>(cv);
-        const bool fin = isfinite(ty) && isfinite(tx);
-        const bool kept = fin && passes_minmass(p, res);
-        targets[dst] = make_float4(fin ? ty : 0.0f, fin ? tx : 0.0f, kept ? 1.0f : 0.0f,
+        const bool ok = stopped && isfinite(ty) && isfinite(tx);
+        targets[dst] = make_float4(ok ? ty : 0.0f, ok ? tx : 0.0f, ok ? 1.0f : 0.0f,
                                    __uint_as_float(seed));
     }
 }
@@ -1511,7 +1536,7 @@ template <typename T>
 hipError_t launch_detect_tile_t(const float *proc, const T *raw, uint32_t nframes,
                                 const GtKParams &p, gt_detection *out, uint32_t out_cap,
                                 uint32_t *ocount, uint2 *slow, uint32_t slow_cap,
-                                uint32_t *scount, uint32_t frame0, hipStream_t s) {
+                                uint32_t *scount, uint32_t frame0, uint32_t steps, hipStream_t s) {
     const TileGeom g = tile_geom(p.k, p.rint);
     const size_t ntiles = static_cast<size_t>((p.W + FT_TW - 1) / FT_TW) *
                           ((p.H + FT_TH - 1) / FT_TH) * nframes;
@@ -1519,9 +1544,15 @@ hipError_t launch_detect_tile_t(const float *proc, const T *raw, uint32_t nframe
     const int nf = static_cast<int>(nframes);
     const size_t lds = sizeof(float) * (static_cast<size_t>(g.rows) * g.pitch +
                                         static_cast<size_t>(FT_TH + p.k - 1) * FT_TW);
-#define GT_TILE(KV, RV)                                                                         \
-    k_detect_tile<T, KV, RV><<<grid, FT_THREADS, lds, s>>>(proc, raw, p, g, out, out_cap, ocount, \
-                                                            slow, slow_cap, scount, frame0, nf)
+#define GT_TILE(KV, RV)                                                                          \
+    do {                                                                                         \
+        if (steps == 1)                                                                          \
+            k_detect_tile<T, KV, RV, 1><<<grid, FT_THREADS, lds, s>>>(                           \
+                proc, raw, p, g, out, out_cap, ocount, slow, slow_cap, scount, frame0, nf);       \
+        else                                                                                     \
+            k_detect_tile<T, KV, RV><<<grid, FT_THREADS, lds, s>>>(                              \
+                proc, raw, p, g, out, out_cap, ocount, slow, slow_cap, scount, frame0, nf);       \
+    } while (0)
     // diameter 9 with separation 7 (k = 9) or 9 (k = 12)
     if (p.rint == 4 && p.k == 9) {
         GT_TILE(9, 4);
@@ -1564,17 +1595,19 @@ __global__ void k_copy_counts(const uint32_t *__restrict__ strided, uint32_t *__
 hipError_t gt_launch_detect_tile(const float *proc, const void *raw, int dtype, uint32_t nframes,
                                  const GtKParams &p, gt_detection *out, uint32_t out_cap,
                                  uint32_t *ocount, uint2 *slow, uint32_t slow_cap,
-                                 uint32_t *scount, uint32_t frame0, hipStream_t s) {
+                                 uint32_t *scount, uint32_t frame0, hipStream_t s,
+                                 uint32_t walk_steps) {
+    if (walk_steps != 1 && walk_steps != FT_WALK_STEPS) return hipErrorInvalidValue;
     switch (dtype) {
     case GT_U16:
         return launch_detect_tile_t(proc, static_cast<const uint16_t *>(raw), nframes, p, out,
-                                    out_cap, ocount, slow, slow_cap, scount, frame0, s);
+                                    out_cap, ocount, slow, slow_cap, scount, frame0, walk_steps, s);
     case GT_F32:
         return launch_detect_tile_t(proc, static_cast<const float *>(raw), nframes, p, out,
-                                    out_cap, ocount, slow, slow_cap, scount, frame0, s);
+                                    out_cap, ocount, slow, slow_cap, scount, frame0, walk_steps, s);
     case GT_U8:
         return launch_detect_tile_t(proc, static_cast<const uint8_t *>(raw), nframes, p, out,
-                                    out_cap, ocount, slow, slow_cap, scount, frame0, s);
+                                    out_cap, ocount, slow, slow_cap, scount, frame0, walk_steps, s);
     default: return hipErrorInvalidValue;
     }
 }
@@ -1638,19 +1671,21 @@ hipError_t gt_launch_detect_cand(const float *proc, int dtype, uint32_t nframes,
 }
 
 hipError_t gt_launch_walk_learned(const float *proc, uint32_t nframes, const GtKParams &p,
-                                  const uint32_t *cand, uint32_t cap, const uint32_t *ccount,
-                                  const float4 *props, float conf_logit, float tol,
-                                  gt_detection *out, uint32_t out_cap, uint32_t *ocount,
-                                  unsigned long long *stats, uint32_t frame0, hipStream_t s) {
+                                  const uint32_t *cand, uint32_t stride, uint32_t it0,
+                                  uint32_t cap, const uint32_t *ccount, const float4 *props,
+                                  float conf_logit, float tol, gt_detection *out,
+                                  uint32_t out_cap, uint32_t *ocount, unsigned long long *stats,
+                                  uint32_t frame0, hipStream_t s) {
+    if (stride != 1 && stride != 2) return hipErrorInvalidValue;
     const dim3 grid = walk_grid(p, nframes);
     if (p.rint == 4)
-        k_walk_learned<4><<<grid, WK_THREADS, 0, s>>>(proc, p, cand, cap, ccount, props,
-                                                      conf_logit, tol, out, out_cap, ocount,
-                                                      stats, frame0);
+        k_walk_learned<4><<<grid, WK_THREADS, 0, s>>>(proc, p, cand, stride, it0, cap, ccount,
+                                                      props, conf_logit, tol, out, out_cap,
+                                                      ocount, stats, frame0);
     else
-        k_walk_learned<0><<<grid, WK_THREADS, 0, s>>>(proc, p, cand, cap, ccount, props,
-                                                      conf_logit, tol, out, out_cap, ocount,
-                                                      stats, frame0);
+        k_walk_learned<0><<<grid, WK_THREADS, 0, s>>>(proc, p, cand, stride, it0, cap, ccount,
+                                                      props, conf_logit, tol, out, out_cap,
+                                                      ocount, stats, frame0);
     return hipGetLastError();
 }
 

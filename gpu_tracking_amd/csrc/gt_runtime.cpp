@@ -100,6 +100,7 @@ struct gt_context {
     const gt_cnn *cnn = nullptr;   // caller's network; null = plain walk
     float cnn_conf = 0.0f;         // logit threshold
     float cnn_tol = 0.05f;         // verification tolerance (px)
+    int32_t cnn_scope = GT_CNN_MOVING;  // which candidates get a proposal
     float4 *d_props = nullptr;     // chunk * cand_cap proposals
     unsigned long long *d_stats = nullptr;  // accepted, candidates
     uint32_t *d_offs = nullptr;    // sample generation: chunk + 1 offsets, then the total
@@ -243,9 +244,30 @@ gt_status run_pipeline(gt_context *c, const void *d_frames, int32_t dtype, uint3
             if (ev) GT_HIP(hipEventRecord(ev[3], s));
             continue;
         }
-        if (c->tiled && c->cnn) {
-            // candidates, the network's proposals for all of them, one
-            // verifying walk evaluation each (full walk where it fails)
+        if (c->tiled && c->cnn && c->cnn_scope == GT_CNN_MOVING) {
+            // one walk step per candidate in the tile (a walk that stops
+            // there is finished: the plain result); the network's proposals
+            // for the walks still moving, one verifying evaluation each, the
+            // rest of the walk where it fails
+            GtKParams kw = c->kp;
+            kw.characterize = 0;
+            GT_HIP(gt_launch_zero_u32(c->d_scount, static_cast<size_t>(nf) * GT_COUNTER_STRIDE, s));
+            GT_HIP(gt_launch_detect_tile(c->d_proc, frames, dtype, nf, kw, out, cap, c->d_ocount,
+                                         c->d_slow, c->slow_cap, c->d_scount, f0, s, 1));
+            if (ev) GT_HIP(hipEventRecord(ev[2], s));
+            const uint32_t *list = reinterpret_cast<const uint32_t *>(c->d_slow);
+            GT_HIP(gt_cnn_launch_forward_frames(gt_cnn_packed(c->cnn), c->d_proc,
+                                                static_cast<int>(c->H), static_cast<int>(c->W), nf,
+                                                list, 2, c->slow_cap, c->d_scount, c->d_props, s));
+            GT_HIP(gt_launch_walk_learned(c->d_proc, nf, kw, list, 2, 1, c->slow_cap, c->d_scount,
+                                          c->d_props, c->cnn_conf, c->cnn_tol, out, cap,
+                                          c->d_ocount, c->d_stats, f0, s));
+            if (c->kp.characterize)
+                GT_HIP(gt_launch_characterize_det(c->d_proc, frames, dtype, nf, c->kp, out, cap,
+                                                  c->d_ocount, s));
+        } else if (c->tiled && c->cnn) {
+            // every candidate: the network's proposals for all of them, one
+            // verifying walk evaluation each (the full walk where it fails)
             GtKParams kw = c->kp;
             kw.characterize = 0;
             GT_HIP(gt_launch_zero_u32(c->d_ccount, static_cast<size_t>(nf) * GT_COUNTER_STRIDE, s));
@@ -254,10 +276,11 @@ gt_status run_pipeline(gt_context *c, const void *d_frames, int32_t dtype, uint3
             if (ev) GT_HIP(hipEventRecord(ev[2], s));
             GT_HIP(gt_cnn_launch_forward_frames(gt_cnn_packed(c->cnn), c->d_proc,
                                                 static_cast<int>(c->H), static_cast<int>(c->W), nf,
-                                                c->d_cand, c->cand_cap, c->d_ccount, c->d_props, s));
-            GT_HIP(gt_launch_walk_learned(c->d_proc, nf, kw, c->d_cand, c->cand_cap, c->d_ccount,
-                                          c->d_props, c->cnn_conf, c->cnn_tol, out, cap,
-                                          c->d_ocount, c->d_stats, f0, s));
+                                                c->d_cand, 1, c->cand_cap, c->d_ccount, c->d_props,
+                                                s));
+            GT_HIP(gt_launch_walk_learned(c->d_proc, nf, kw, c->d_cand, 1, 0, c->cand_cap,
+                                          c->d_ccount, c->d_props, c->cnn_conf, c->cnn_tol, out,
+                                          cap, c->d_ocount, c->d_stats, f0, s));
             if (c->kp.characterize)
                 GT_HIP(gt_launch_characterize_det(c->d_proc, frames, dtype, nf, c->kp, out, cap,
                                                   c->d_ocount, s));
@@ -639,12 +662,15 @@ gt_status gt_execute(gt_context *c, const void *frames, int32_t dtype, uint64_t 
     return GT_OK;
 }
 
-gt_status gt_context_set_cnn(gt_context *c, const gt_cnn *net, float conf_logit, float tol) {
+gt_status gt_context_set_cnn(gt_context *c, const gt_cnn *net, float conf_logit, float tol,
+                             int32_t scope) {
     if (!c) return fail(GT_ERR_INVALID_ARGUMENT, "null context");
     if (!net) {
         c->cnn = nullptr;
         return GT_OK;
     }
+    if (scope != GT_CNN_MOVING && scope != GT_CNN_ALL)
+        return fail(GT_ERR_INVALID_ARGUMENT, "unknown proposal scope %d", scope);
     if (!c->tiled || c->log)
         return fail(GT_ERR_INVALID_ARGUMENT,
                     "the learned proposal runs in the tiled Trackpy-style path only");
@@ -663,6 +689,7 @@ gt_status gt_context_set_cnn(gt_context *c, const gt_cnn *net, float conf_logit,
     c->cnn = net;
     c->cnn_conf = conf_logit;
     c->cnn_tol = tol;
+    c->cnn_scope = scope;
     return GT_OK;
 }
 

@@ -298,7 +298,7 @@ int32_t gt_stage_times(gt_context *ctx, double *stage_ms, int32_t n, uint32_t *l
  * wfc (3,32) bfc (3).
  * Training samples: patches bf16 [n][GT_CNN_PATCH_PITCH] (entries 121..127
  * zero) and targets float32 [n][4] = (end row - candidate row, end column -
- * candidate column, 1 if the plain method keeps the candidate, the
+ * candidate column, 1 if the walk converges within max_iterations, the
  * candidate's raster index as uint32 bits), all in device memory.  Loss: label * |d - t|^2 + BCE(logit, label), batch mean;
  * Adam with torch.optim.Adam's arithmetic (betas 0.9, 0.999; eps 1e-8). */
 #define GT_CNN_NPARAM 14147
@@ -338,11 +338,18 @@ gt_status gt_cnn_make_samples(gt_context *ctx, const void *d_frames, int32_t dty
                               uint32_t nframes, uint16_t *d_patches, float *d_targets,
                               uint64_t cap, uint64_t *n_written, uint64_t *n_total, void *stream);
 /* Detect with the network's proposals (NULL: the plain walk again).
- * Proposals with logit < conf_logit, and proposals whose verifying
- * evaluation moves or lands more than tol px from the prediction, take the
- * full walk.  The context keeps the pointer: the network must outlive its
- * use, and is not trained while the context detects. */
-gt_status gt_context_set_cnn(gt_context *ctx, const gt_cnn *net, float conf_logit, float tol);
+ * scope GT_CNN_MOVING: every candidate takes one walk step first; a walk
+ * that stops there is finished (the plain result), and the network proposes
+ * for the walks still moving.  GT_CNN_ALL: the network proposes for every
+ * candidate.  Proposals with logit < conf_logit, and proposals whose
+ * verifying evaluation moves or lands more than tol px from the prediction,
+ * take the rest of the full walk.  The context keeps the pointer: the
+ * network must outlive its use, and is not trained while the context
+ * detects. */
+#define GT_CNN_MOVING 0
+#define GT_CNN_ALL 1
+gt_status gt_context_set_cnn(gt_context *ctx, const gt_cnn *net, float conf_logit, float tol,
+                             int32_t scope);
 /* Accepted proposals and candidates since the last reset. */
 gt_status gt_context_cnn_stats(gt_context *ctx, uint64_t *accepted, uint64_t *candidates,
                                int32_t reset);

@@ -20,8 +20,8 @@ proposal is specified as, in numpy.
   relative to the candidate pixel.
 * Loss per sample: label * ((dy - ty)^2 + (dx - tx)^2) + BCE(logit, label),
   averaged over the batch; the position term is 0 where label is 0 (any
-  target offset, even a non-finite one).  (ty, tx) is the walked centroid minus the
-  candidate pixel; label is 1 when the plain method keeps the candidate.
+  target offset, even a non-finite one).  (ty, tx) is the walked centroid
+  minus the candidate pixel; label is 1 when the walk converges.
 * Backward, with the same rounding points as the kernels.  The gradients at
   the pre-activations of conv3 and conv2 are rounded to bfloat16, the inputs
   of the matrix-core products.  The conv1 gradient stays float32.  The ReLU
