@@ -91,7 +91,7 @@ EXPORTS = [
     "gt_column_names", "gt_filter_close", "gt_doughnut", "gt_set_interrupt", "gt_progress",
     "gt_set_timing", "gt_stage_times", "gt_prune_blobs", "gt_link", "gt_batch_create",
     "gt_batch_destroy", "gt_batch_detect", "gt_batch_tune", "gt_batch_get_schedule",
-    "gt_batch_set_graph", "gt_cnn_create", "gt_cnn_destroy", "gt_cnn_set_params",
+    "gt_batch_set_graph", "gt_frame_std", "gt_cnn_create", "gt_cnn_destroy", "gt_cnn_set_params",
     "gt_cnn_get_params", "gt_cnn_adam_steps", "gt_cnn_forward", "gt_cnn_gradient", "gt_cnn_step",
     "gt_cnn_train", "gt_cnn_make_samples", "gt_context_set_cnn", "gt_context_cnn_stats",
 ]
@@ -183,6 +183,7 @@ def lib() -> C.CDLL:
         "gt_cnn_train": [vp, vp, vp, u64, u32, u32, f32, u64, vp],
         "gt_cnn_make_samples": [vp, vp, C.c_int32, u32, vp, vp, u64, P(u64), P(u64), vp],
         "gt_context_set_cnn": [vp, vp, f32, f32, C.c_int32],
+        "gt_frame_std": [vp, vp, C.c_int32, u32, vp, vp],
         "gt_context_cnn_stats": [vp, P(u64), P(u64), C.c_int32],
     }
     for name, args in sigs.items():

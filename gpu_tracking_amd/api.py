@@ -220,6 +220,22 @@ class Context:
                                            cap, counts.data_ptr(), stream))
         return out, counts
 
+    def frame_std(self, frames, stream=None):
+        """The noise estimate the snr / minmass_snr rules use: each raw
+        frame's sample standard deviation (float32 device tensor (T,))."""
+        import torch
+        if frames.dim() == 2:
+            frames = frames.unsqueeze(0)
+        code = _torch_code(frames)
+        if tuple(frames.shape[1:]) != self.shape or not frames.is_contiguous():
+            raise ValueError("frames must be contiguous with shape (T,) + context shape")
+        out = torch.empty((frames.shape[0],), dtype=torch.float32, device=frames.device)
+        if stream is None:
+            stream = torch.cuda.current_stream(frames.device).cuda_stream
+        N.check(self._lib.gt_frame_std(self._h, frames.data_ptr(), code, frames.shape[0],
+                                       out.data_ptr(), stream))
+        return out
+
     def set_learned(self, net: Optional["CentroidNet"], conf_logit: float = 0.0,
                     tol: float = 0.05, scope: str = "moving") -> None:
         """Detect with the network's centroid proposals (None: the plain

@@ -265,6 +265,9 @@ gt_status gt_batch_create(const gt_params *params, uint64_t max_frame_pixels,
                                "0 < max_slots <= 65535");
     if (params->style != GT_STYLE_TRACKPY)
         return gt_runtime_fail(GT_ERR_INVALID_ARGUMENT, "batched detection runs the Trackpy style");
+    if (params->has_snr || params->has_minmass_snr)
+        return gt_runtime_fail(GT_ERR_INVALID_ARGUMENT,
+                               "batched detection does not apply the snr / minmass_snr rules");
     gt_derived der;
     std::string why;
     if (!gt_host_derive(params, 1, 1, &der, &why))

@@ -59,9 +59,12 @@ bool gt_host_derive(const gt_params *p, uint32_t H, uint32_t W, gt_derived *d, s
         *why = "unknown style";
         return false;
     }
-    if (p->has_snr || p->has_minmass_snr || p->has_adaptive_background) {
-        *why = "snr, minmass_snr and adaptive_background need the per-frame noise estimate, "
-               "which this build does not provide";
+    if (p->has_adaptive_background) {
+        *why = "adaptive_background is not available in this build";
+        return false;
+    }
+    if ((p->has_snr || p->has_minmass_snr) && static_cast<uint64_t>(H) * W <= 24) {
+        *why = "snr and minmass_snr need frames of more than 24 pixels (noise estimate)";
         return false;
     }
     if (p->has_illumination_sigma) {
@@ -159,9 +162,12 @@ bool gt_host_derive(const gt_params *p, uint32_t H, uint32_t W, gt_derived *d, s
     d->max_iterations = p->max_iterations;
     d->shift_threshold = p->shift_threshold;
     d->minmass = p->minmass;
-    d->snr = 0.0f;
-    d->minmass_snr = 0.0f;
+    // gpu_setup.rs:572-578 (rough_snr_factor is 0.5 only with
+    // adaptive_background, rejected above)
+    d->snr = p->has_snr ? p->snr : 0.0f;
+    d->minmass_snr = p->has_minmass_snr ? p->minmass_snr : 0.0f;
     d->rough = 1.0f;
+    d->need_std = (p->has_snr || p->has_minmass_snr) ? 1 : 0;
     d->truncate = p->truncate_preprocessed ? 1 : 0;
     d->characterize = p->characterize ? 1 : 0;
     d->filter_close = p->filter_close ? 1 : 0;
@@ -243,6 +249,9 @@ void gt_host_fill_kparams(const gt_derived *d, uint32_t H, uint32_t W, GtKParams
     kp->minmass_snr = d->minmass_snr;
     kp->image_std = 0.0f;
     kp->rough = d->rough;
+    kp->snr = d->snr;
+    kp->pad_ = 0;
+    kp->fstd = nullptr;   // set by the context when the noise estimate runs
     kp->characterize = d->characterize;
     kp->circle = static_cast<int32_t>(d->circle);
 }

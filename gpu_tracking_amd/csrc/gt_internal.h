@@ -97,5 +97,9 @@ hipError_t gt_launch_cnn_label(const float *proc, uint32_t nframes, const GtKPar
                                const uint32_t *cand, uint32_t cap, const uint32_t *ccount,
                                const uint32_t *offs, uint32_t maxn, float4 *targets,
                                hipStream_t s);
+// Noise estimate (mean and sample std, the snr rules' eight-way tree) of
+// nframes raw frames of n > 24 pixels; tmp holds nframes * ceil(n / 8) floats.
+hipError_t gt_launch_frame_std(const void *frames, int dtype, uint32_t nframes, size_t n,
+                               float *tmp, float *mean, float *std_out, hipStream_t s);
 // Fills the constant tables the kernels read (once per device).
 hipError_t gt_init_tables();

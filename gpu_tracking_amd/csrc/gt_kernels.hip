@@ -358,6 +358,16 @@ __global__ __launch_bounds__(B2_THREADS) void k_bandpass_int(const T *__restrict
                             proc + static_cast<size_t>(fr) * HW, p, H, W, x0, y0, L);
 }
 
+// The noise estimate of frame fr of a launch (the snr rules' std of the raw
+// frame), and the candidate threshold std * snr * rough_snr_factor
+// (extract_max.wgsl:37, logspace_max.wgsl:122).
+__device__ __forceinline__ float frame_std(const GtKParams &p, int fr) {
+    return p.fstd ? p.fstd[fr] : p.image_std;
+}
+__device__ __forceinline__ float frame_thr(const GtKParams &p, int fr) {
+    return p.fstd ? p.fstd[fr] * p.snr * p.rough : p.max_thr;
+}
+
 // ---------------------------------------------------------------------------
 // Local maximum: a pixel inside the margin is a candidate when no in-frame
 // value of its k x k window [-a, k-1-a]^2 is greater than it, and it is not
@@ -378,7 +388,7 @@ __global__ __launch_bounds__(LM_THREADS) void k_localmax_direct(const float *__r
     if (gy < H && gx < W && gy >= p.margin && gy < H - p.margin && gx >= p.margin &&
         gx < W - p.margin) {
         const float c = P[static_cast<size_t>(gy) * W + gx];
-        if (!(c < p.max_thr)) {
+        if (!(c < frame_thr(p, fr))) {
             flag = true;
             for (int i = 0; i < k && flag; ++i) {
                 const int yy = gy - a + i;
@@ -581,11 +591,11 @@ __device__ void characterize_row(const S &src, const T *__restrict__ Rw, int H, 
     row.signal = acc.signal;
 }
 
-// minmass rule (walk.wgsl:200-213)
-__device__ __forceinline__ bool passes_minmass(const GtKParams &p, const WalkRes &res) {
+// minmass rule (walk.wgsl:200-213); stdv is the frame's noise estimate
+__device__ __forceinline__ bool passes_minmass(const GtKParams &p, const WalkRes &res, float stdv) {
     float mm;
     if (p.minmass > 0.0f) mm = p.minmass;
-    else mm = p.minmass_snr * res.count * p.image_std * p.rough;
+    else mm = p.minmass_snr * res.count * stdv * p.rough;
     return res.mass >= mm;
 }
 
@@ -645,7 +655,7 @@ __global__ __launch_bounds__(WK_THREADS) void k_walk(const float *__restrict__ p
             }
             res.x = res.x + static_cast<float>(pu);
             res.y = res.y + static_cast<float>(pv);
-            keep = passes_minmass(p, res);
+            keep = passes_minmass(p, res, frame_std(p, fr));
             if (keep) {
                 fill_row(d, res, seed, frame0 + fr);
                 if (p.characterize) characterize_row<T, RI>(src, Rw, H, W, p.circle, d.row);
@@ -822,10 +832,10 @@ __device__ __forceinline__ bool finish_walk(const GtKParams &p, int H, int W,
                                             const T *__restrict__ Rw, const float *Sp, int oy,
                                             int ox, int RW, int ylo, int yhi, int xlo, int xhi,
                                             WalkRes res, int pu, int pv, uint32_t seed,
-                                            uint32_t frame, gt_detection &d) {
+                                            uint32_t frame, gt_detection &d, float stdv) {
     res.x = res.x + static_cast<float>(pu);
     res.y = res.y + static_cast<float>(pv);
-    if (!passes_minmass(p, res)) return false;
+    if (!passes_minmass(p, res, stdv)) return false;
     fill_row(d, res, seed, frame);
     if (kChar && p.characterize) {
         const int cu = f2i(rintf(d.row.x)), cv = f2i(rintf(d.row.y));
@@ -867,7 +877,7 @@ __device__ __forceinline__ void detect_tile_body(
     uint32_t out_cap,
     uint32_t *__restrict__ counter, uint2 *__restrict__ fslow, uint32_t slow_cap,
     uint32_t *__restrict__ slow_counter, float *lds, uint32_t *s_cnt,
-    uint64_t (*s_masks)[FT_WROWS], uint32_t *__restrict__ fcand = nullptr) {
+    uint64_t (*s_masks)[FT_WROWS], float thr, float stdv, uint32_t *__restrict__ fcand = nullptr) {
     constexpr bool kFixed = KT > 0 && RI > 0;
     constexpr TileGeom gc = tile_geom(KT > 0 ? KT : 1, RI > 0 ? RI : 1);
     constexpr bool kInt = sizeof(T) <= 2;  // integer frames: no NaN in the processed frame
@@ -973,7 +983,6 @@ __device__ __forceinline__ void detect_tile_body(
     //    `lane`: Hm row q feeds output row r when 0 <= q - r < k
     const int r0 = wave * FT_WROWS;
     const int margin = p.margin;
-    const float thr = p.max_thr;
     const int gx = x0 + lane;
     uint32_t nwave = 0;  // wave-uniform
     {
@@ -1074,7 +1083,7 @@ __device__ __forceinline__ void detect_tile_body(
             }
             if (moving && it < p.max_iter) defer = true;
             else keep = finish_walk<T, RI, PITCH, false>(p, H, W, P, Rw, Sp, oy, ox, RW, ylo, yhi, xlo, xhi, res, pu,
-                                           pv, seed, frame, d);
+                                           pv, seed, frame, d, stdv);
         }
         // hand long walks to k_walk_slow; if its list is full, walk on here
         const uint32_t spos = wave_append(defer, slow_counter);
@@ -1088,7 +1097,7 @@ __device__ __forceinline__ void detect_tile_body(
                     if (!walk_move(res, pu, pv, H, W, rint, p.shift_thr)) break;
                 }
                 keep = finish_walk<T, RI, PITCH, false>(p, H, W, P, Rw, Sp, oy, ox, RW, ylo, yhi, xlo, xhi, res, pu,
-                                          pv, seed, frame, d);
+                                          pv, seed, frame, d, stdv);
             }
         }
         const uint32_t pos = wave_append(keep, counter);
@@ -1120,7 +1129,8 @@ __global__ __launch_bounds__(FT_THREADS, FT_MIN_WAVES) void k_detect_tile(
                                 p, H, W, g, x0, y0, frame0 + fr, out + static_cast<size_t>(fr) * out_cap,
                                 out_cap, &ocount[fr * GT_COUNTER_STRIDE],
                                 slow + static_cast<size_t>(fr) * slow_cap, slow_cap,
-                                &scount[fr * GT_COUNTER_STRIDE], lds, s_cnt, s_masks);
+                                &scount[fr * GT_COUNTER_STRIDE], lds, s_cnt, s_masks,
+                                frame_thr(p, fr), frame_std(p, fr));
 }
 
 // Candidate pixels of every tile (the local-maximum test of k_detect_tile),
@@ -1145,6 +1155,7 @@ __global__ __launch_bounds__(FT_THREADS, FT_MIN_WAVES) void k_detect_cand(
     detect_tile_body<T, KT, RI, true>(proc + static_cast<size_t>(fr) * HW, nullptr, p, H, W, g,
                                       x0, y0, 0, nullptr, 0, nullptr, nullptr, cap,
                                       &ccount[fr * GT_COUNTER_STRIDE], lds, s_cnt, s_masks,
+                                      frame_thr(p, fr), frame_std(p, fr),
                                       cand + static_cast<size_t>(fr) * cap);
 }
 
@@ -1187,7 +1198,7 @@ __global__ __launch_bounds__(WK_THREADS) void k_walk_slow(const float *__restric
                 if (!walk_move(res, pu, pv, H, W, rint, p.shift_thr)) break;
             }
             keep = finish_walk<T, RI, 0, false>(p, H, W, P, Rw, nullptr, 0, 0, 0, 1, 0, 1, 0, res, pu, pv,
-                                                e.x, frame0 + fr, d);
+                                                e.x, frame0 + fr, d, frame_std(p, fr));
         }
         const uint32_t pos = wave_append(keep, counter);
         if (keep && pos < out_cap) fout[pos] = d;
@@ -1324,7 +1335,8 @@ __global__ __launch_bounds__(WK_THREADS) void k_walk_learned(
             }
             n_acc += accepted ? 1u : 0u;
             keep = finish_walk<uint16_t, RI, 0, false>(p, H, W, P, nullptr, nullptr, 0, 0, 0, 1, 0,
-                                                       1, 0, res, pu, pv, seed, frame0 + fr, d);
+                                                       1, 0, res, pu, pv, seed, frame0 + fr, d,
+                                                       frame_std(p, fr));
         }
         const uint32_t pos = wave_append(keep, counter);
         if (keep && pos < out_cap) fout[pos] = d;
@@ -1701,6 +1713,125 @@ hipError_t gt_launch_cnn_label(const float *proc, uint32_t nframes, const GtKPar
     return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------
+// Noise estimate of raw frames for the snr / minmass_snr rules
+// (src/utils.rs:298-670; sum_image_outplace.wgsl, sum_image_inplace.wgsl,
+// std_image.wgsl, last_reduction.wgsl).  Eight-way tree per frame: a level
+// of length len has stride s = ceil(len / 8) and out[i] = sum_{j<8} in[i +
+// s j] in j order; levels repeat while len > 24; the last values are summed
+// in order; mean = sum / f32(N), std = sqrt(sum of squared deviations /
+// f32(N - 1)).  NaN-marker pixels (bits 0xFFFFFFFF) add nothing.  Indices
+// past a level's length are skipped.
+namespace {
+
+__device__ __forceinline__ bool is_marker(float v) { return __float_as_uint(v) == 0xFFFFFFFFu; }
+
+// First level over the frame: the values (kVar false) or the squared
+// deviations from mean[f] (kVar true); tmp row f has pitch s.
+template <typename T, bool kVar>
+__global__ __launch_bounds__(256) void k_std_first(const T *__restrict__ frames, size_t n,
+                                                   uint32_t s, const float *__restrict__ mean,
+                                                   float *__restrict__ tmp) {
+    const uint32_t f = blockIdx.y;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= s) return;
+    const T *x = frames + static_cast<size_t>(f) * n;
+    const float m = kVar ? mean[f] : 0.0f;
+    float sum = 0.0f;
+#pragma unroll
+    for (uint32_t j = 0; j < 8; ++j) {
+        const size_t k = i + static_cast<size_t>(s) * j;
+        if (k < n) {
+            const float d = to_f32(x[k]);
+            if constexpr (kVar) {
+                float v = 0.0f;
+                if (!is_marker(d)) {
+                    const float e = d - m;
+                    v = e * e;
+                }
+                sum = sum + v;
+            } else {
+                if (!is_marker(d)) sum = sum + d;
+            }
+        }
+    }
+    tmp[static_cast<size_t>(f) * s + i] = sum;
+}
+
+// Next level in place: row f (pitch) of length len -> its first s values.
+// Thread i writes index i < s and reads i + s j: no index is written by one
+// thread and read by another.
+__global__ __launch_bounds__(256) void k_std_level(float *__restrict__ tmp, size_t pitch,
+                                                   uint32_t len, uint32_t s) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= s) return;
+    float *t = tmp + static_cast<size_t>(blockIdx.y) * pitch;
+    float sum = 0.0f;
+#pragma unroll
+    for (uint32_t j = 0; j < 8; ++j) {
+        const uint32_t k = i + s * j;
+        if (k < len) sum = sum + t[k];
+    }
+    t[i] = sum;
+}
+
+__global__ void k_std_last(const float *__restrict__ tmp, size_t pitch, uint32_t len, size_t n,
+                           int var, uint32_t nframes, float *__restrict__ out) {
+    const uint32_t f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= nframes) return;
+    const float *t = tmp + static_cast<size_t>(f) * pitch;
+    float sum = 0.0f;
+    for (uint32_t i = 0; i < len; ++i) sum = sum + t[i];
+    if (var) out[f] = sqrtf(sum / static_cast<float>(static_cast<uint32_t>(n - 1)));
+    else out[f] = sum / static_cast<float>(n);
+}
+
+template <typename T>
+hipError_t launch_frame_std_t(const T *frames, uint32_t nframes, size_t n, float *tmp,
+                              float *mean, float *std_out, hipStream_t s) {
+    const uint32_t s0 = static_cast<uint32_t>((n + 7) / 8);
+    const dim3 g0((s0 + 255) / 256, nframes);
+    for (int pass = 0; pass < 2; ++pass) {
+        if (pass == 0)
+            k_std_first<T, false><<<g0, 256, 0, s>>>(frames, n, s0, nullptr, tmp);
+        else
+            k_std_first<T, true><<<g0, 256, 0, s>>>(frames, n, s0, mean, tmp);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+        uint32_t len = s0;
+        while (len > 24) {
+            const uint32_t st = (len + 7) / 8;
+            k_std_level<<<dim3((st + 255) / 256, nframes), 256, 0, s>>>(tmp, s0, len, st);
+            e = hipGetLastError();
+            if (e != hipSuccess) return e;
+            len = st;
+        }
+        k_std_last<<<(nframes + 63) / 64, 64, 0, s>>>(tmp, s0, len, n, pass, nframes,
+                                                       pass == 0 ? mean : std_out);
+        e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+}  // namespace
+
+hipError_t gt_launch_frame_std(const void *frames, int dtype, uint32_t nframes, size_t n,
+                               float *tmp, float *mean, float *std_out, hipStream_t s) {
+    if (n <= 24 || nframes == 0) return hipErrorInvalidValue;
+    switch (dtype) {
+    case GT_U16:
+        return launch_frame_std_t(static_cast<const uint16_t *>(frames), nframes, n, tmp, mean,
+                                  std_out, s);
+    case GT_F32:
+        return launch_frame_std_t(static_cast<const float *>(frames), nframes, n, tmp, mean,
+                                  std_out, s);
+    case GT_U8:
+        return launch_frame_std_t(static_cast<const uint8_t *>(frames), nframes, n, tmp, mean,
+                                  std_out, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
 hipError_t gt_launch_copy_counts(const uint32_t *strided, uint32_t *dst, uint32_t n,
                                  hipStream_t s) {
     if (n == 0) return hipSuccess;
@@ -1942,7 +2073,9 @@ __global__ __launch_bounds__(LG_THREADS) void k_log_tile(const T *__restrict__ r
                                                          LogGeom g, int H, int W,
                                                          uint32_t *__restrict__ cand, uint32_t cap,
                                                          uint32_t *__restrict__ ccount,
-                                                         int nframes) {
+                                                         int nframes,
+                                                         const float *__restrict__ fstd, float snr,
+                                                         float rough) {
     extern __shared__ float lds[];
     const int ntx = (W + LG_TW - 1) / LG_TW, nty = (H + LG_TH - 1) / LG_TH;
     const int per_frame = ntx * nty;
@@ -1975,7 +2108,7 @@ __global__ __launch_bounds__(LG_THREADS) void k_log_tile(const T *__restrict__ r
     }
     __syncthreads();
     const int n = tab->n;
-    const float snr_thr = tab->snr_thr;
+    const float snr_thr = fstd ? fstd[fr] * snr * rough : tab->snr_thr;
     uint32_t *cnt = &ccount[fr * GT_COUNTER_STRIDE];
     uint32_t *fc = cand + static_cast<size_t>(fr) * cap;
     // scale-space maximum test of scale m on the tile's interior pixels
@@ -2086,7 +2219,7 @@ __global__ __launch_bounds__(WK_THREADS) void k_walk_log(const float *__restrict
             }
             res.x = res.x + static_cast<float>(pu);
             res.y = res.y + static_cast<float>(pv);
-            keep = passes_minmass(p, res);
+            keep = passes_minmass(p, res, frame_std(p, fr));
             if (keep) {
                 fill_row(d, res, seed, frame0 + fr);
                 d.row.r = r;
@@ -2121,7 +2254,8 @@ hipError_t launch_log_tile_t(const T *raw, const float *proc, uint32_t nframes, 
     const size_t ntiles = static_cast<size_t>((p.W + LG_TW - 1) / LG_TW) *
                           ((p.H + LG_TH - 1) / LG_TH) * nframes;
     k_log_tile<T><<<xcd_grid(ntiles), LG_THREADS, lds, s>>>(raw, proc, tab, g, p.H, p.W, cand, cap,
-                                                            ccount, static_cast<int>(nframes));
+                                                            ccount, static_cast<int>(nframes),
+                                                            p.fstd, p.snr, p.rough);
     return hipGetLastError();
 }
 
@@ -2247,7 +2381,8 @@ __global__ __launch_bounds__(FT_THREADS, FT_MIN_WAVES) void k_detect_tile_batch(
                                 e.W, g, x0, y0, e.frame, out + static_cast<size_t>(e.out_idx) * out_cap,
                                 out_cap, &ocount[slot * GT_COUNTER_STRIDE],
                                 slow + static_cast<size_t>(slot) * slow_cap, slow_cap,
-                                &scount[slot * GT_COUNTER_STRIDE], lds, s_cnt, s_masks);
+                                &scount[slot * GT_COUNTER_STRIDE], lds, s_cnt, s_masks,
+                                p.max_thr, p.image_std);
 }
 
 // Slow walks and characterize: slots over blockIdx.y (strided), detections
@@ -2285,7 +2420,7 @@ __global__ __launch_bounds__(WK_THREADS) void k_walk_slow_batch(
                     if (!walk_move(res, pu, pv, H, W, rint, p.shift_thr)) break;
                 }
                 keep = finish_walk<T, RI, 0, false>(p, H, W, P, Rw, nullptr, 0, 0, 0, 1, 0, 1, 0, res, pu,
-                                                    pv, en.x, e.frame, d);
+                                                    pv, en.x, e.frame, d, p.image_std);
             }
             const uint32_t pos = wave_append(keep, counter);
             if (keep && pos < out_cap) fout[pos] = d;

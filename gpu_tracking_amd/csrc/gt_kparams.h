@@ -39,6 +39,11 @@ struct GtKParams {
     float rough;
     int32_t characterize;
     int32_t circle;
+    float snr;
+    int32_t pad_;
+    // per-frame noise estimate of the launch's frames (device; null: the
+    // constants max_thr and image_std apply to every frame)
+    const float *fstd;
 };
 
 // LoG style: the radius scan and each scale's tap table (separable_log.wgsl),

@@ -105,6 +105,10 @@ struct gt_context {
     unsigned long long *d_stats = nullptr;  // accepted, candidates
     uint32_t *d_offs = nullptr;    // sample generation: chunk + 1 offsets, then the total
 
+    // noise estimate of each chunk's frames (snr / minmass_snr), kp.fstd = d_std
+    float *d_std_tmp = nullptr;    // chunk * ceil(HW / 8)
+    float *d_mean = nullptr, *d_std = nullptr;  // chunk
+
     // gt_execute staging: two slots, so one chunk's detections come back
     // and are post-processed while the next chunk runs on the GPU
     struct Slot {
@@ -133,9 +137,25 @@ struct gt_context {
 
 namespace {
 
+// Noise-estimate buffers (also used by gt_frame_std on contexts without the
+// snr rules).
+gt_status alloc_std(gt_context *c) {
+    if (c->d_std_tmp) return GT_OK;
+    const size_t HW = static_cast<size_t>(c->H) * c->W;
+    GT_HIP(hipMalloc(&c->d_std_tmp, sizeof(float) * ((HW + 7) / 8) * c->chunk));
+    GT_HIP(hipMalloc(&c->d_mean, sizeof(float) * c->chunk));
+    GT_HIP(hipMalloc(&c->d_std, sizeof(float) * c->chunk));
+    return GT_OK;
+}
+
 gt_status alloc_scratch(gt_context *c) {
     const size_t HW = static_cast<size_t>(c->H) * c->W;
     GT_HIP(hipMalloc(&c->d_proc, sizeof(float) * HW * c->chunk));
+    if (c->der.need_std) {
+        gt_status st = alloc_std(c);
+        if (st != GT_OK) return st;
+        c->kp.fstd = c->d_std;
+    }
     if (!c->tiled)
         GT_HIP(hipMalloc(&c->d_cand, sizeof(uint32_t) * static_cast<size_t>(c->cand_cap) * c->chunk));
     GT_HIP(hipMalloc(&c->d_ccount, sizeof(uint32_t) * GT_COUNTER_STRIDE * c->chunk));
@@ -188,6 +208,9 @@ void free_all(gt_context *c) {
     if (c->d_props) (void)hipFree(c->d_props);
     if (c->d_stats) (void)hipFree(c->d_stats);
     if (c->d_offs) (void)hipFree(c->d_offs);
+    if (c->d_std_tmp) (void)hipFree(c->d_std_tmp);
+    if (c->d_mean) (void)hipFree(c->d_mean);
+    if (c->d_std) (void)hipFree(c->d_std);
     for (auto &sl : c->slot) {
         if (sl.d_frames) (void)hipFree(sl.d_frames);
         if (sl.d_rows) (void)hipFree(sl.d_rows);
@@ -229,6 +252,8 @@ gt_status run_pipeline(gt_context *c, const void *d_frames, int32_t dtype, uint3
         if (!c->tiled || c->log)
             GT_HIP(gt_launch_zero_u32(c->d_ccount, static_cast<size_t>(nf) * GT_COUNTER_STRIDE, s));
         if (ev) GT_HIP(hipEventRecord(ev[0], s));
+        if (c->der.need_std)
+            GT_HIP(gt_launch_frame_std(frames, dtype, nf, HW, c->d_std_tmp, c->d_mean, c->d_std, s));
         GT_HIP(gt_launch_bandpass(frames, dtype, c->d_proc, nf, c->kp, s));
         if (ev) GT_HIP(hipEventRecord(ev[1], s));
         gt_detection *out = d_out + static_cast<size_t>(f0) * cap;
@@ -662,6 +687,26 @@ gt_status gt_execute(gt_context *c, const void *frames, int32_t dtype, uint64_t 
     return GT_OK;
 }
 
+gt_status gt_frame_std(gt_context *c, const void *d_frames, int32_t dtype, uint32_t nframes,
+                       float *d_std, void *stream) {
+    if (!c) return fail(GT_ERR_INVALID_ARGUMENT, "null context");
+    if (!dtype_size(dtype)) return fail(GT_ERR_INVALID_ARGUMENT, "unknown dtype %d", dtype);
+    if (nframes == 0) return fail(GT_ERR_EMPTY_ITERATOR, "the provided video doesn't contain any frames.");
+    if (!d_frames || !d_std) return fail(GT_ERR_INVALID_ARGUMENT, "null argument");
+    const size_t HW = static_cast<size_t>(c->H) * c->W;
+    if (HW <= 24) return fail(GT_ERR_INVALID_ARGUMENT, "the noise estimate needs more than 24 pixels");
+    gt_status st = alloc_std(c);
+    if (st != GT_OK) return st;
+    hipStream_t s = static_cast<hipStream_t>(stream);  // NULL: the default stream
+    const size_t esz = dtype_size(dtype);
+    for (uint32_t f0 = 0; f0 < nframes; f0 += c->chunk) {
+        const uint32_t nf = std::min(c->chunk, nframes - f0);
+        GT_HIP(gt_launch_frame_std(static_cast<const char *>(d_frames) + esz * HW * f0, dtype, nf,
+                                   HW, c->d_std_tmp, c->d_mean, d_std + f0, s));
+    }
+    return GT_OK;
+}
+
 gt_status gt_context_set_cnn(gt_context *c, const gt_cnn *net, float conf_logit, float tol,
                              int32_t scope) {
     if (!c) return fail(GT_ERR_INVALID_ARGUMENT, "null context");
@@ -732,6 +777,8 @@ gt_status gt_cnn_make_samples(gt_context *c, const void *d_frames, int32_t dtype
         const uint32_t nf = std::min(c->chunk, nframes - f0);
         const char *frames = static_cast<const char *>(d_frames) + esz * HW * f0;
         GT_HIP(gt_launch_zero_u32(c->d_ccount, static_cast<size_t>(nf) * GT_COUNTER_STRIDE, s));
+        if (c->der.need_std)
+            GT_HIP(gt_launch_frame_std(frames, dtype, nf, HW, c->d_std_tmp, c->d_mean, c->d_std, s));
         GT_HIP(gt_launch_bandpass(frames, dtype, c->d_proc, nf, c->kp, s));
         GT_HIP(gt_launch_detect_cand(c->d_proc, dtype, nf, kw, c->d_cand, c->cand_cap, c->d_ccount, s));
         GT_HIP(gt_cnn_launch_sample_offsets(c->d_ccount, c->cand_cap, nf, d_total, c->d_offs, s));

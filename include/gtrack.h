@@ -283,6 +283,15 @@ gt_status gt_set_timing(gt_context *ctx, int32_t enable);
 int32_t gt_stage_times(gt_context *ctx, double *stage_ms, int32_t n, uint32_t *launches,
                        uint32_t *frames_per_launch);
 
+/* The noise estimate the snr and minmass_snr rules use: the sample standard
+ * deviation of each raw frame (StdArray, src/utils.rs:298-670: an eight-way
+ * summation tree; NaN-marker pixels, bits 0xFFFFFFFF, are left out of the
+ * sums but not the pixel count), for nframes device frames of the context's
+ * size; d_std receives nframes floats.  Asynchronous on `stream` (NULL = the
+ * default stream). */
+gt_status gt_frame_std(gt_context *ctx, const void *d_frames, int32_t dtype, uint32_t nframes,
+                       float *d_std, void *stream);
+
 /* ---- Learned centroid proposal ---------------------------------------------
  * No counterpart in the reference.  The reference's walk is the iterative
  * centroid loop of src/shaders/walk.wgsl:104-198.  Here a small CNN predicts,

@@ -92,6 +92,7 @@ def lib():
         L.or_log_detect_frame.argtypes = [C.c_void_p, P(Params), C.c_void_p, C.c_uint32, C.c_int,
                                           C.c_float, C.c_void_p, C.c_size_t, C.c_void_p]
         L.or_log_detect_frame.restype = C.c_size_t
+        L.or_frame_std.argtypes = [C.c_void_p, C.c_size_t, C.c_void_p, C.c_void_p]
         _lib = L
     return _lib
 
@@ -326,3 +327,15 @@ def log_detect_frame(frame, p: Params, radii, characterize: bool = True,
     n = lib().or_log_detect_frame(_ptr(f), C.byref(p), _ptr(rr), len(rr), int(characterize),
                                   overlap_threshold, _ptr(rows), cap, C.byref(nc))
     return rows[:n].copy(), nc.value
+
+
+def frame_std(frame):
+    """(mean, std) noise estimate of a raw frame (f32 values; NaN-marker pixels
+    with bits 0xFFFFFFFF are excluded from the sums but not the divisors)."""
+    f = np.ascontiguousarray(frame, dtype=np.float32).reshape(-1)
+    if f.size <= 24:
+        raise ValueError("the noise estimate needs more than 24 pixels")
+    m = np.zeros(1, np.float32)
+    s = np.zeros(1, np.float32)
+    lib().or_frame_std(_ptr(f), f.size, _ptr(m), _ptr(s))
+    return float(m[0]), float(s[0])

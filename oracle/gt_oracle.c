@@ -761,3 +761,67 @@ size_t or_log_detect_frame(const float *frame, const or_params *p, const float *
     free(proc);
     return n;
 }
+
+static int or_is_marker(float v) {
+    uint32_t u;
+    memcpy(&u, &v, sizeof(u));
+    return u == 0xFFFFFFFFu;
+}
+
+/* one level of the eight-way tree, in place over t[0..len) */
+static size_t or_tree_level(float *t, size_t len) {
+    const size_t s = (len + 7) / 8;
+    for (size_t i = 0; i < s; i++) {
+        float sum = 0.0f;
+        for (size_t j = 0; j < 8; j++) {
+            const size_t k = i + s * j;
+            if (k < len) sum = sum + t[k];
+        }
+        t[i] = sum;
+    }
+    return s;
+}
+
+void or_frame_std(const float *x, size_t n, float *mean_out, float *std_out) {
+    const size_t s0 = (n + 7) / 8;
+    float *t = (float *)malloc(sizeof(float) * (s0 > 0 ? s0 : 1));
+    /* mean: sum_image_outplace (markers skipped), then the in-place levels */
+    for (size_t i = 0; i < s0; i++) {
+        float sum = 0.0f;
+        for (size_t j = 0; j < 8; j++) {
+            const size_t k = i + s0 * j;
+            if (k < n && !or_is_marker(x[k])) sum = sum + x[k];
+        }
+        t[i] = sum;
+    }
+    size_t len = s0;
+    while (len > 24) len = or_tree_level(t, len);
+    float m = 0.0f;
+    for (size_t i = 0; i < len; i++) m = m + t[i];
+    m = m / (float)n;
+    /* variance: std_image (squared deviations, 0 at markers) fused into the
+     * first in-place level over n values */
+    for (size_t i = 0; i < s0; i++) {
+        float sum = 0.0f;
+        for (size_t j = 0; j < 8; j++) {
+            const size_t k = i + s0 * j;
+            if (k < n) {
+                float v = 0.0f;
+                if (!or_is_marker(x[k])) {
+                    const float d = x[k] - m;
+                    v = d * d;
+                }
+                sum = sum + v;
+            }
+        }
+        t[i] = sum;
+    }
+    len = s0;
+    while (len > 24) len = or_tree_level(t, len);
+    float v = 0.0f;
+    for (size_t i = 0; i < len; i++) v = v + t[i];
+    v = v / (float)(uint32_t)(n - 1);
+    *mean_out = m;
+    *std_out = sqrtf(v);
+    free(t);
+}

@@ -139,6 +139,20 @@ size_t or_log_detect_frame(const float *frame, const or_params *p, const float *
                            or_row *rows, size_t cap, size_t *ncand_out);
 
 /* FP64 twin for error reporting: f64 bandpass (exp in f64, exact divisions). */
+/* Noise estimate of a raw frame for the snr / minmass_snr rules
+ * (src/utils.rs:298-670 StdArray: MeanArrayOutplace, std_image, MeanArray;
+ * shaders sum_image_outplace.wgsl:14-67, sum_image_inplace.wgsl:10-42,
+ * std_image.wgsl:16-30, last_reduction.wgsl:18-34).  Eight-way tree: a level
+ * of length len has stride s = ceil(len / 8) and out[i] = sum_{j<8} in[i +
+ * s j] in j order from 0.0; levels repeat while len > 24; the last values are
+ * summed in order.  The mean's first level skips NaN-marker pixels (bits
+ * 0xFFFFFFFF), the squared deviations are 0 for them; the divisors are f32(N)
+ * and f32(N - 1), the pixel count the runtime writes into the divisor buffer
+ * each frame (src/execute_gpu.rs:113-135).  Indices at or past a level's
+ * length are skipped (the reference's guard is vacuous there and reads
+ * stale or out-of-range values; identical whenever every level length is a
+ * multiple of 8).  N must exceed 24. */
+void or_frame_std(const float *frame, size_t n, float *mean, float *std);
 void or_bandpass_f64(const float *frame, const or_params *p, double *processed, double *scratch);
 /* FP64 centroid of the disk at integer position (pu, pv) of a f64 frame. */
 void or_centroid_f64(const double *processed, const or_params *p, int32_t pu, int32_t pv,
