@@ -393,9 +393,30 @@ gt_host_post::gt_host_post(const gt_params *p, const gt_derived *d, uint32_t H, 
 
 gt_status gt_host_post::process(std::vector<gt_detection> &rows, uint64_t frame_index,
                                 const float *raw, std::atomic<int32_t> *interrupt,
-                                std::string *err) {
+                                std::string *err, const float *post4) {
     (void)interrupt;
     (void)err;
+    if (post4 && d_.style == GT_STYLE_TRACKPY) {
+        // filter_close and the doughnut ran on the GPU: order the rows by
+        // seed carrying their results, keep the survivors
+        order_.resize(rows.size());
+        for (size_t i = 0; i < rows.size(); ++i) order_[i] = static_cast<uint32_t>(i);
+        std::sort(order_.begin(), order_.end(),
+                  [&](uint32_t a, uint32_t b) { return rows[a].seed < rows[b].seed; });
+        tmp_rows_.clear();
+        dough_.clear();
+        for (uint32_t i : order_) {
+            const float *q = post4 + 4 * static_cast<size_t>(i);
+            if (q[3] == 0.0f) continue;
+            tmp_rows_.push_back(rows[i].row);
+            if (d_.doughnut) {
+                dough_.push_back(q[0]);
+                dough_.push_back(q[1]);
+                dough_.push_back(q[2]);
+            }
+        }
+        return assemble(frame_index, interrupt, err);
+    }
     std::sort(rows.begin(), rows.end(),
               [](const gt_detection &a, const gt_detection &b) { return a.seed < b.seed; });
     tmp_rows_.resize(rows.size());
@@ -430,6 +451,14 @@ gt_status gt_host_post::process(std::vector<gt_detection> &rows, uint64_t frame_
                         d_.bg_radius, d_.gap_radius, dough_.data());
         }
     }
+    return assemble(frame_index, interrupt, err);
+}
+
+// Linking and the output rows of one frame (tmp_rows_ and dough_ hold the
+// frame's surviving rows in seed order and their doughnut columns).
+gt_status gt_host_post::assemble(uint64_t frame_index, std::atomic<int32_t> *interrupt,
+                                 std::string *err) {
+    const size_t n = tmp_rows_.size();
     if (linker_) {
         // linker_reset_points (execute_gpu.rs:792-797): a new linker, ids
         // start over
@@ -572,27 +601,35 @@ extern "C" gt_status gt_filter_close(const gt_result_row *rows, uint64_t n, floa
     return GT_OK;
 }
 
-extern "C" gt_status gt_doughnut(const float *raw, uint32_t H, uint32_t W, const gt_result_row *rows,
-                                 uint64_t n, float radius, float bg_radius, float gap_radius,
-                                 float *out3) {
-    if (n == 0) return GT_OK;
-    if (!raw || !rows || !out3) return GT_ERR_INVALID_ARGUMENT;
-    // circle_inds(radius) / annulus_inds(bg_radius, gap_radius + radius)
-    // (kernels.rs:138-171), in their i-major, j-minor order.
-    std::vector<std::pair<int32_t, int32_t>> sig, ring;
+// circle_inds(radius) / annulus_inds(bg_radius, gap_radius + radius)
+// (kernels.rs:138-171), in their i-major, j-minor order.
+void gt_host_doughnut_offsets(float radius, float bg_radius, float gap_radius,
+                              std::vector<std::pair<int32_t, int32_t>> *sig,
+                              std::vector<std::pair<int32_t, int32_t>> *ring) {
+    sig->clear();
+    ring->clear();
     const int32_t cb = f2i_sat(std::ceil(radius));
     const float rr = radius * radius;
     for (int32_t i = -cb; i <= cb; ++i)
         for (int32_t j = -cb; j <= cb; ++j)
-            if (static_cast<float>(i * i + j * j) <= rr) sig.emplace_back(i, j);
+            if (static_cast<float>(i * i + j * j) <= rr) sig->emplace_back(i, j);
     const float inner = gap_radius + radius;
     const int32_t ab = f2i_sat(std::ceil(bg_radius));
     const float or2 = bg_radius * bg_radius, ir2 = inner * inner;
     for (int32_t i = -ab; i <= ab; ++i)
         for (int32_t j = -ab; j <= ab; ++j) {
             const float q = static_cast<float>(i * i + j * j);
-            if (q <= or2 && q >= ir2) ring.emplace_back(i, j);
+            if (q <= or2 && q >= ir2) ring->emplace_back(i, j);
         }
+}
+
+extern "C" gt_status gt_doughnut(const float *raw, uint32_t H, uint32_t W, const gt_result_row *rows,
+                                 uint64_t n, float radius, float bg_radius, float gap_radius,
+                                 float *out3) {
+    if (n == 0) return GT_OK;
+    if (!raw || !rows || !out3) return GT_ERR_INVALID_ARGUMENT;
+    std::vector<std::pair<int32_t, int32_t>> sig, ring;
+    gt_host_doughnut_offsets(radius, bg_radius, gap_radius, &sig, &ring);
     std::vector<float> bg;
     bg.reserve(ring.size());
     for (uint64_t k = 0; k < n; ++k) {

@@ -6,6 +6,7 @@
 #include <atomic>
 #include <memory>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "gt_kparams.h"
@@ -51,6 +52,10 @@ void gt_host_fill_logtable(const gt_derived *d, GtLogTable *t);
 size_t gt_host_prune_blobs(gt_result_row *rows, size_t n, float overlap_threshold);
 uint32_t gt_host_column_names(const gt_params *p, const char **names, const char **types);
 void gt_host_widen(const void *src, int32_t dtype, size_t n, float *dst);
+// The doughnut's disk and ring offsets (i, j), i-major (kernels.rs:138-171).
+void gt_host_doughnut_offsets(float radius, float bg_radius, float gap_radius,
+                              std::vector<std::pair<int32_t, int32_t>> *sig,
+                              std::vector<std::pair<int32_t, int32_t>> *ring);
 
 // Which frames to run, in order, and (characterize_points) which positions
 // each of them gets (FrameSubsetter, src/linking.rs:94-196; the keys/frames
@@ -76,11 +81,16 @@ class gt_host_post {
     const std::string &error() const { return error_; }
     bool needs_raw() const { return d_.doughnut != 0; }
     // rows: the frame's detections in any order (sorted here by seed).
+    // post4, when given: per row (raw_mass, bg_median, corrected, keep) as the
+    // GPU computed them (Trackpy style): filter_close and the doughnut are
+    // taken from it instead of being computed here (raw may then be null).
     gt_status process(std::vector<gt_detection> &rows, uint64_t frame_index, const float *raw,
-                      std::atomic<int32_t> *interrupt, std::string *err);
+                      std::atomic<int32_t> *interrupt, std::string *err,
+                      const float *post4 = nullptr);
     gt_status finish(gt_output *out, std::string *err);
 
   private:
+    gt_status assemble(uint64_t frame_index, std::atomic<int32_t> *interrupt, std::string *err);
     gt_params p_;
     gt_derived d_;
     uint32_t H_, W_;
@@ -90,6 +100,7 @@ class gt_host_post {
     std::vector<gt_result_row> tmp_rows_;
     std::vector<uint8_t> keep_;
     std::vector<float> dough_;
+    std::vector<uint32_t> order_;
     // linking (search_range): one linker per segment between reset points
     std::vector<uint64_t> reset_;          // sorted frame indices
     std::unique_ptr<gt_linker> linker_;

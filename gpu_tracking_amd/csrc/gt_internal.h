@@ -101,5 +101,18 @@ hipError_t gt_launch_cnn_label(const float *proc, uint32_t nframes, const GtKPar
 // nframes raw frames of n > 24 pixels; tmp holds nframes * ceil(n / 8) floats.
 hipError_t gt_launch_frame_std(const void *frames, int dtype, uint32_t nframes, size_t n,
                                float *tmp, float *mean, float *std_out, hipStream_t s);
+// Post-processing of a chunk's rows (frame f: rows[f * cap ...], counts[f])
+// into post[f * cap + i] = (raw_mass, bg_median, corrected, keep):
+// filter_close sets keep (and zeroes the rest); the doughnut fills the first
+// three for kept rows.  offs: nsig disk offsets, then nring ring offsets
+// (i, j), i-major.
+hipError_t gt_launch_filter_close_rows(const gt_detection *rows, uint32_t cap,
+                                       const uint32_t *counts, uint32_t nframes, float separation,
+                                       float4 *post, hipStream_t s);
+size_t gt_doughnut_lds_bytes(uint32_t nsig, uint32_t nring);
+hipError_t gt_launch_doughnut_rows(const void *raw, int dtype, int H, int W,
+                                   const gt_detection *rows, uint32_t cap, const uint32_t *counts,
+                                   uint32_t nframes, const short2 *offs, uint32_t nsig,
+                                   uint32_t nring, float4 *post, hipStream_t s);
 // Fills the constant tables the kernels read (once per device).
 hipError_t gt_init_tables();

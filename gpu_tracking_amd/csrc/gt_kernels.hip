@@ -1714,6 +1714,182 @@ hipError_t gt_launch_cnn_label(const float *proc, uint32_t nframes, const GtKPar
 }
 
 // ---------------------------------------------------------------------------
+// Post-processing of a chunk's detections on the GPU (Trackpy style):
+// filter_close and doughnut photometry, one float4 per row: (raw_mass,
+// bg_median, raw_mass_corrected, keep).
+namespace {
+
+constexpr int FC_THREADS = 256;
+
+// filter_close_trackpy (execute_gpu.rs:556-590): row i of a frame is dropped
+// when a row j != i of the same frame lies at d^2 < separation^2 (d^2 =
+// (0 + dx^2) + dy^2 in f32, dx = x_j - x_i) with a larger mass, or the same
+// mass and a smaller seed (rows are ordered by seed).  Every pair is tested:
+// rows of a frame are staged through LDS in tiles.
+__global__ __launch_bounds__(FC_THREADS) void k_filter_close_rows(
+    const gt_detection *__restrict__ rows, uint32_t cap, const uint32_t *__restrict__ counts,
+    float r2, float4 *__restrict__ post) {
+    __shared__ float sx[FC_THREADS], sy[FC_THREADS], sm[FC_THREADS];
+    __shared__ uint32_t ss[FC_THREADS];
+    const uint32_t f = blockIdx.y;
+    const uint32_t n = min(counts[f], cap);
+    const gt_detection *fr = rows + static_cast<size_t>(f) * cap;
+    float4 *fp = post + static_cast<size_t>(f) * cap;
+    for (uint32_t i0 = blockIdx.x * FC_THREADS; i0 < n; i0 += gridDim.x * FC_THREADS) {
+        const uint32_t i = i0 + threadIdx.x;
+        float xi = 0.0f, yi = 0.0f, mi = 0.0f;
+        uint32_t si = 0;
+        bool fin = false;
+        if (i < n) {
+            xi = fr[i].row.x;
+            yi = fr[i].row.y;
+            mi = fr[i].row.mass;
+            si = fr[i].seed;
+            fin = isfinite(xi) && isfinite(yi);
+        }
+        bool drop = false;
+        for (uint32_t j0 = 0; j0 < n; j0 += FC_THREADS) {
+            __syncthreads();
+            const uint32_t j = j0 + threadIdx.x;
+            if (j < n) {
+                sx[threadIdx.x] = fr[j].row.x;
+                sy[threadIdx.x] = fr[j].row.y;
+                sm[threadIdx.x] = fr[j].row.mass;
+                ss[threadIdx.x] = fr[j].seed;
+            }
+            __syncthreads();
+            const uint32_t m = min(static_cast<uint32_t>(FC_THREADS), n - j0);
+            if (fin && !drop) {
+                for (uint32_t t = 0; t < m; ++t) {
+                    if (ss[t] == si) continue;       // the row itself (seeds are unique)
+                    const float ddx = sx[t] - xi, ddy = sy[t] - yi;
+                    float d2 = 0.0f;
+                    d2 = d2 + ddx * ddx;
+                    d2 = d2 + ddy * ddy;
+                    if (!(d2 < r2)) continue;
+                    const float mj = sm[t];
+                    if (mi < mj || (mi == mj && si > ss[t])) {
+                        drop = true;
+                        break;
+                    }
+                }
+            }
+        }
+        if (i < n) fp[i] = make_float4(0.0f, 0.0f, 0.0f, drop ? 0.0f : 1.0f);
+    }
+}
+
+// Doughnut photometry (execute_gpu.rs:717-790; stencils kernels.rs:138-171):
+// centre (round half away from zero); raw_mass = in-order f32 sum of the raw
+// frame over the disk offsets (i-major); bg_median = element len/2 of the
+// sorted in-frame ring values; corrected = raw_mass - median * count.  One
+// wave per kept row; the ring values are ranked against each other in LDS.
+template <typename T>
+__global__ __launch_bounds__(64) void k_doughnut_rows(
+    const T *__restrict__ raw, int H, int W, const gt_detection *__restrict__ rows, uint32_t cap,
+    const uint32_t *__restrict__ counts, const short2 *__restrict__ offs, uint32_t nsig,
+    uint32_t nring, float4 *__restrict__ post) {
+    extern __shared__ float dl[];
+    float *sig = dl;             // nsig
+    float *ring = dl + nsig;     // nring (ring values of in-frame offsets, compacted)
+    __shared__ uint32_t s_nv;
+    __shared__ float s_med;
+    const uint32_t f = blockIdx.y;
+    const uint32_t n = min(counts[f], cap);
+    const T *F = raw + static_cast<size_t>(f) * H * W;
+    const gt_detection *fr = rows + static_cast<size_t>(f) * cap;
+    float4 *fp = post + static_cast<size_t>(f) * cap;
+    const uint32_t lane = threadIdx.x;
+    for (uint32_t i = blockIdx.x; i < n; i += gridDim.x) {
+        if (fp[i].w == 0.0f) continue;       // dropped by filter_close (block-uniform)
+        const int64_t mu = f2i(roundf(fr[i].row.x)), mv = f2i(roundf(fr[i].row.y));
+        for (uint32_t k = lane; k < nsig; k += 64) {
+            const int64_t pu = mu + offs[k].x, pv = mv + offs[k].y;
+            sig[k] = (pu >= 0 && pu < H && pv >= 0 && pv < W) ? to_f32(F[pu * W + pv]) : 0.0f;
+        }
+        if (lane == 0) s_nv = 0;
+        __syncthreads();
+        for (uint32_t k = lane; k < nring; k += 64) {
+            const int64_t pu = mu + offs[nsig + k].x, pv = mv + offs[nsig + k].y;
+            if (pu >= 0 && pu < H && pv >= 0 && pv < W)
+                ring[atomicAdd(&s_nv, 1u)] = to_f32(F[pu * W + pv]);
+        }
+        __syncthreads();
+        const uint32_t nv = s_nv;
+        const uint32_t kth = nv / 2;
+        if (lane == 0) s_med = 0.0f;
+        __syncthreads();
+        for (uint32_t a = lane; a < nv; a += 64) {
+            const float v = ring[a];
+            uint32_t less = 0, leq = 0;
+            for (uint32_t b = 0; b < nv; ++b) {
+                const float w = ring[b];
+                less += w < v ? 1u : 0u;
+                leq += w <= v ? 1u : 0u;
+            }
+            if (less <= kth && kth < leq) s_med = v;   // equal values: the same float
+        }
+        __syncthreads();
+        if (lane == 0) {
+            float sm = 0.0f;
+            uint32_t ns = 0;
+            for (uint32_t k = 0; k < nsig; ++k) {
+                const int64_t pu = mu + offs[k].x, pv = mv + offs[k].y;
+                if (pu < 0 || pu >= H || pv < 0 || pv >= W) continue;
+                sm = sm + sig[k];
+                ++ns;
+            }
+            const float med = nv ? s_med : 0.0f;
+            fp[i] = make_float4(sm, med, sm - med * static_cast<float>(ns), 1.0f);
+        }
+        __syncthreads();
+    }
+}
+
+}  // namespace
+
+hipError_t gt_launch_filter_close_rows(const gt_detection *rows, uint32_t cap,
+                                       const uint32_t *counts, uint32_t nframes, float separation,
+                                       float4 *post, hipStream_t s) {
+    if (nframes == 0) return hipSuccess;
+    k_filter_close_rows<<<dim3(8, nframes), FC_THREADS, 0, s>>>(rows, cap, counts,
+                                                                separation * separation, post);
+    return hipGetLastError();
+}
+
+size_t gt_doughnut_lds_bytes(uint32_t nsig, uint32_t nring) {
+    return sizeof(float) * (static_cast<size_t>(nsig) + nring);
+}
+
+hipError_t gt_launch_doughnut_rows(const void *raw, int dtype, int H, int W,
+                                   const gt_detection *rows, uint32_t cap, const uint32_t *counts,
+                                   uint32_t nframes, const short2 *offs, uint32_t nsig,
+                                   uint32_t nring, float4 *post, hipStream_t s) {
+    if (nframes == 0) return hipSuccess;
+    const size_t lds = gt_doughnut_lds_bytes(nsig, nring);
+    const dim3 grid(256, nframes);
+#define GT_DOUGH(TT)                                                                          \
+    do {                                                                                      \
+        if (lds > 65536) {                                                                    \
+            hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_doughnut_rows<TT>), \
+                                               hipFuncAttributeMaxDynamicSharedMemorySize,     \
+                                               static_cast<int>(lds));                         \
+            if (e != hipSuccess) return e;                                                    \
+        }                                                                                     \
+        k_doughnut_rows<TT><<<grid, 64, lds, s>>>(static_cast<const TT *>(raw), H, W, rows,   \
+                                                  cap, counts, offs, nsig, nring, post);      \
+    } while (0)
+    switch (dtype) {
+    case GT_U16: GT_DOUGH(uint16_t); break;
+    case GT_F32: GT_DOUGH(float); break;
+    case GT_U8: GT_DOUGH(uint8_t); break;
+    default: return hipErrorInvalidValue;
+    }
+#undef GT_DOUGH
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
 // Noise estimate of raw frames for the snr / minmass_snr rules
 // (src/utils.rs:298-670; sum_image_outplace.wgsl, sum_image_inplace.wgsl,
 // std_image.wgsl, last_reduction.wgsl).  Eight-way tree per frame: a level

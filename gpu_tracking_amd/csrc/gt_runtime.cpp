@@ -120,6 +120,14 @@ struct gt_context {
         hipEvent_t done = nullptr;     // counts of the slot's chunk are on the host
     } slot[2];
     uint32_t det_cap = 0;
+    // filter_close and the doughnut on the GPU (Trackpy style, gt_execute):
+    // per slot one float4 per row slot (raw_mass, bg_median, corrected, keep)
+    bool gpu_post = false;
+    short2 *d_dough_offs = nullptr;     // disk offsets, then ring offsets
+    uint32_t dough_nsig = 0, dough_nring = 0;
+    float4 *d_post[2] = {nullptr, nullptr};
+    float *h_post = nullptr;            // pinned, 4 floats per read-back row
+    size_t h_post_cap = 0;
     hipStream_t copy_stream = nullptr;  // detection read-back
     gt_detection *h_rows = nullptr;     // pinned read-back buffer
     size_t h_rows_cap = 0;
@@ -172,6 +180,9 @@ gt_status alloc_scratch(gt_context *c) {
 gt_status alloc_execute_buffers(gt_context *c) {
     if (c->slot[0].d_frames) return GT_OK;
     const size_t HW = static_cast<size_t>(c->H) * c->W;
+    if (c->gpu_post)
+        for (auto &dp : c->d_post)
+            GT_HIP(hipMalloc(&dp, sizeof(float4) * static_cast<size_t>(c->det_cap) * c->chunk));
     for (auto &sl : c->slot) {
         GT_HIP(hipMalloc(&sl.d_frames, 4 * HW * c->chunk));
         GT_HIP(hipMalloc(&sl.d_rows, sizeof(gt_detection) * static_cast<size_t>(c->det_cap) * c->chunk));
@@ -186,6 +197,15 @@ gt_status alloc_execute_buffers(gt_context *c) {
 }
 
 gt_status reserve_host_rows(gt_context *c, size_t n) {
+    if (c->gpu_post && n > c->h_post_cap) {
+        if (c->h_post) (void)hipHostFree(c->h_post);
+        c->h_post = nullptr;
+        c->h_post_cap = 0;
+        const size_t cap = std::max<size_t>(n, 2 * c->h_rows_cap + 4096);
+        GT_HIP(hipHostMalloc(reinterpret_cast<void **>(&c->h_post), sizeof(float4) * cap,
+                             hipHostMallocDefault));
+        c->h_post_cap = cap;
+    }
     if (n <= c->h_rows_cap) return GT_OK;
     if (c->h_rows) (void)hipHostFree(c->h_rows);
     c->h_rows = nullptr;
@@ -220,6 +240,10 @@ void free_all(gt_context *c) {
         if (sl.done) (void)hipEventDestroy(sl.done);
     }
     if (c->h_rows) (void)hipHostFree(c->h_rows);
+    if (c->h_post) (void)hipHostFree(c->h_post);
+    for (float4 *dp : c->d_post)
+        if (dp) (void)hipFree(dp);
+    if (c->d_dough_offs) (void)hipFree(c->d_dough_offs);
     if (c->copy_stream) (void)hipStreamDestroy(c->copy_stream);
     for (hipEvent_t e : c->events) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -420,6 +444,36 @@ gt_status gt_context_create(const gt_params *params, uint32_t nrows, uint32_t nc
             if (e != hipSuccess) st = fail(GT_ERR_GPU_DEVICE, "LoG table upload failed: %s", hipGetErrorString(e));
         }
     }
+    if (st == GT_OK && der.style == GT_STYLE_TRACKPY && !characterize_points &&
+        (der.filter_close || der.doughnut)) {
+        // filter_close and the doughnut run on the GPU when the doughnut's
+        // stencils fit one wave's LDS
+        c->gpu_post = true;
+        if (der.doughnut) {
+            std::vector<std::pair<int32_t, int32_t>> sig, ring;
+            gt_host_doughnut_offsets(static_cast<float>(der.circle) / 2.0f, der.bg_radius,
+                                     der.gap_radius, &sig, &ring);
+            const int32_t lim = 32767;
+            bool fits = gt_doughnut_lds_bytes(static_cast<uint32_t>(sig.size()),
+                                              static_cast<uint32_t>(ring.size())) <= 65536;
+            for (const auto &o : sig) fits = fits && std::abs(o.first) <= lim && std::abs(o.second) <= lim;
+            for (const auto &o : ring) fits = fits && std::abs(o.first) <= lim && std::abs(o.second) <= lim;
+            if (!fits) {
+                c->gpu_post = false;
+            } else {
+                std::vector<short2> tab;
+                for (const auto &o : sig) tab.push_back(make_short2(static_cast<short>(o.first), static_cast<short>(o.second)));
+                for (const auto &o : ring) tab.push_back(make_short2(static_cast<short>(o.first), static_cast<short>(o.second)));
+                c->dough_nsig = static_cast<uint32_t>(sig.size());
+                c->dough_nring = static_cast<uint32_t>(ring.size());
+                hipError_t e = hipMalloc(&c->d_dough_offs, sizeof(short2) * std::max<size_t>(1, tab.size()));
+                if (e == hipSuccess && !tab.empty())
+                    e = hipMemcpy(c->d_dough_offs, tab.data(), sizeof(short2) * tab.size(),
+                                  hipMemcpyHostToDevice);
+                if (e != hipSuccess) st = fail(GT_ERR_GPU_DEVICE, "doughnut table upload failed: %s", hipGetErrorString(e));
+            }
+        }
+    }
     if (st != GT_OK) {
         std::string msg = g_last_error;
         free_all(c);
@@ -536,6 +590,7 @@ gt_status gt_execute(gt_context *c, const void *frames, int32_t dtype, uint64_t 
     struct Item {
         uint64_t frame;
         std::vector<gt_detection> rows;
+        std::vector<float> post;   // GPU filter_close / doughnut results, 4 per row
         const char *raw;
     };
     std::deque<Item> queue;
@@ -564,13 +619,14 @@ gt_status gt_execute(gt_context *c, const void *frames, int32_t dtype, uint64_t 
             }
             if (failed) continue;   // drain without work after an error
             const float *rawp = nullptr;
-            if (post.needs_raw()) {
+            if (post.needs_raw() && !c->gpu_post) {
                 rawf.resize(HW);
                 gt_host_widen(it.raw, dtype, HW, rawf.data());
                 rawp = rawf.data();
             }
             std::string perr;
-            gt_status ps = post.process(it.rows, it.frame, rawp, &c->interrupt, &perr);
+            gt_status ps = post.process(it.rows, it.frame, rawp, &c->interrupt, &perr,
+                                        c->gpu_post ? it.post.data() : nullptr);
             if (ps != GT_OK) {
                 std::lock_guard<std::mutex> lk(mu);
                 post_status = ps;
@@ -615,6 +671,17 @@ gt_status gt_execute(gt_context *c, const void *frames, int32_t dtype, uint64_t 
         else
             st2 = run_pipeline(c, sl.d_frames, dtype, nf, sl.d_rows, c->det_cap, sl.d_counts, c->stream);
         if (st2 != GT_OK) return st2;
+        if (c->gpu_post) {
+            float4 *dp = c->d_post[k % 2];
+            GT_HIP(gt_launch_filter_close_rows(sl.d_rows, c->det_cap, sl.d_counts, nf,
+                                               c->der.filter_close ? static_cast<float>(c->der.separation) : 0.0f,
+                                               dp, c->stream));
+            if (c->der.doughnut)
+                GT_HIP(gt_launch_doughnut_rows(sl.d_frames, dtype, static_cast<int>(c->H),
+                                               static_cast<int>(c->W), sl.d_rows, c->det_cap,
+                                               sl.d_counts, nf, c->d_dough_offs, c->dough_nsig,
+                                               c->dough_nring, dp, c->stream));
+        }
         GT_HIP(hipMemcpyAsync(sl.h_counts, sl.d_counts, sizeof(uint32_t) * nf, hipMemcpyDeviceToHost, c->stream));
         GT_HIP(hipEventRecord(sl.done, c->stream));
         return GT_OK;
@@ -641,6 +708,10 @@ gt_status gt_execute(gt_context *c, const void *frames, int32_t dtype, uint64_t 
             if (n)
                 GT_HIP(hipMemcpyAsync(c->h_rows + off, sl.d_rows + static_cast<size_t>(f) * c->det_cap,
                                       sizeof(gt_detection) * n, hipMemcpyDeviceToHost, c->copy_stream));
+            if (n && c->gpu_post)
+                GT_HIP(hipMemcpyAsync(c->h_post + 4 * off,
+                                      c->d_post[k % 2] + static_cast<size_t>(f) * c->det_cap,
+                                      sizeof(float4) * n, hipMemcpyDeviceToHost, c->copy_stream));
             off += n;
         }
         GT_HIP(hipStreamSynchronize(c->copy_stream));
@@ -650,6 +721,7 @@ gt_status gt_execute(gt_context *c, const void *frames, int32_t dtype, uint64_t 
             Item it;
             it.frame = plan.frames[base + f];
             it.rows.assign(c->h_rows + off, c->h_rows + off + n);
+            if (c->gpu_post) it.post.assign(c->h_post + 4 * off, c->h_post + 4 * (off + n));
             it.raw = static_cast<const char *>(frames) + esz * HW * it.frame;
             off += n;
             std::unique_lock<std::mutex> lk(mu);

@@ -20,7 +20,7 @@ def _video(T=4, seed=3, H=200, W=232, n=30):
 
 
 def _expected(oracle, video, diameter, separation, minmass, characterize, doughnut,
-              filter_close=True, frames=None):
+              filter_close=True, frames=None, bg_radius=None, gap_radius=0.0):
     T, H, W = video.shape
     p = oracle.derive_trackpy(H, W, diameter=diameter, separation=separation, minmass=minmass)
     out = []
@@ -33,7 +33,8 @@ def _expected(oracle, video, diameter, separation, minmass, characterize, doughn
         if characterize:
             cols += [rows["Rg"], rows["raw_mass"], rows["signal"], rows["ecc"]]
         if doughnut:
-            d = oracle.doughnut(video[t], rows, diameter / 2.0, float(diameter), 0.0)
+            d = oracle.doughnut(video[t], rows, diameter / 2.0,
+                                float(diameter) if bg_radius is None else bg_radius, gap_radius)
             cols += [d[:, 0], d[:, 1], d[:, 2]]
         out.append(np.stack(cols, 1).astype(np.float32) if len(rows) else
                    np.zeros((0, len(cols)), np.float32))
@@ -63,6 +64,30 @@ def test_batch_without_characterize_or_doughnut(gt, oracle):
     assert list(df.columns) == ["frame", "y", "x", "mass"]
     ref = _expected(oracle, v, 7, 6, 150.0, False, False, filter_close=False)
     assert np.array_equal(_frame_matrix(df).view(np.uint32), ref.view(np.uint32))
+
+
+@pytest.mark.parametrize("kw,dough,fc,dtype", [
+    (dict(doughnut_correction=False), False, True, np.uint16),
+    (dict(filter_close=False), True, False, np.uint16),
+    (dict(bg_radius=7.0, gap_radius=1.5), True, True, np.uint16),
+    (dict(), True, True, np.float32),
+    (dict(), True, True, np.uint8),
+])
+def test_batch_post_processing_variants(gt, oracle, kw, dough, fc, dtype):
+    # filter_close and the doughnut run on the GPU; both must equal the
+    # oracle's host restatement bit for bit
+    v = _video(T=3, seed=12, n=60)
+    if dtype == np.uint8:
+        v = (v // 64).astype(np.uint8)
+    else:
+        v = v.astype(dtype)
+    mm = 20.0 if dtype == np.uint8 else 300.0
+    df = gt.batch(v, 9, minmass=mm, **kw)
+    ref = _expected(oracle, v, 9, 10, mm, True, dough, filter_close=fc,
+                    bg_radius=kw.get("bg_radius"), gap_radius=kw.get("gap_radius", 0.0))
+    got = _frame_matrix(df)
+    assert got.shape == ref.shape and len(got) > 0
+    assert np.array_equal(got.view(np.uint32), ref.view(np.uint32))
 
 
 def test_batch_frames_subset_and_order(gt, oracle):
