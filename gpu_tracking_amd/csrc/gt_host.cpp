@@ -60,11 +60,20 @@ bool gt_host_derive(const gt_params *p, uint32_t H, uint32_t W, gt_derived *d, s
         return false;
     }
     if (p->has_adaptive_background) {
-        *why = "adaptive_background is not available in this build";
-        return false;
+        if (p->style != GT_STYLE_TRACKPY) {
+            *why = "adaptive_background runs in the Trackpy style only in this build";
+            return false;
+        }
+        if (p->adaptive_background == 0) {
+            // execute_gpu.rs:299-315 unwraps the first of zero passes
+            *why = "adaptive_background needs at least one pass";
+            return false;
+        }
     }
-    if ((p->has_snr || p->has_minmass_snr) && static_cast<uint64_t>(H) * W <= 24) {
-        *why = "snr and minmass_snr need frames of more than 24 pixels (noise estimate)";
+    if ((p->has_snr || p->has_minmass_snr || p->has_adaptive_background) &&
+        static_cast<uint64_t>(H) * W <= 24) {
+        *why = "snr, minmass_snr and adaptive_background need frames of more than 24 pixels "
+               "(noise estimate)";
         return false;
     }
     if (p->has_illumination_sigma) {
@@ -162,12 +171,12 @@ bool gt_host_derive(const gt_params *p, uint32_t H, uint32_t W, gt_derived *d, s
     d->max_iterations = p->max_iterations;
     d->shift_threshold = p->shift_threshold;
     d->minmass = p->minmass;
-    // gpu_setup.rs:572-578 (rough_snr_factor is 0.5 only with
-    // adaptive_background, rejected above)
+    // gpu_setup.rs:572-578: rough_snr_factor 0.5 with adaptive_background
     d->snr = p->has_snr ? p->snr : 0.0f;
     d->minmass_snr = p->has_minmass_snr ? p->minmass_snr : 0.0f;
-    d->rough = 1.0f;
-    d->need_std = (p->has_snr || p->has_minmass_snr) ? 1 : 0;
+    d->rough = p->has_adaptive_background ? 0.5f : 1.0f;
+    d->adaptive = p->has_adaptive_background ? p->adaptive_background : 0u;
+    d->need_std = (p->has_snr || p->has_minmass_snr || p->has_adaptive_background) ? 1 : 0;
     d->truncate = p->truncate_preprocessed ? 1 : 0;
     d->characterize = p->characterize ? 1 : 0;
     d->filter_close = p->filter_close ? 1 : 0;

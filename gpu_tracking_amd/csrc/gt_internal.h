@@ -114,5 +114,15 @@ hipError_t gt_launch_doughnut_rows(const void *raw, int dtype, int H, int W,
                                    const gt_detection *rows, uint32_t cap, const uint32_t *counts,
                                    uint32_t nframes, const short2 *offs, uint32_t nsig,
                                    uint32_t nring, float4 *post, hipStream_t s);
+// adaptive_background over a chunk's walked rows (out, cap, strided ocount):
+// n_iter passes of the acceptance rule with the noise re-estimated (std2)
+// on an f32 copy of the frames (masked, nframes * H * W) with accepted
+// particles marked; accepted rows are compacted in place and ocount updated.
+// p.fstd holds the first estimate; state: nframes * cap bytes; counter:
+// nframes.
+hipError_t gt_launch_adaptive(const void *frames, int dtype, uint32_t nframes, const GtKParams &p,
+                              uint32_t n_iter, gt_detection *out, uint32_t cap, uint32_t *ocount,
+                              float *std_tmp, float *mean, float *std2, float *masked,
+                              uint8_t *state, uint32_t *counter, hipStream_t s);
 // Fills the constant tables the kernels read (once per device).
 hipError_t gt_init_tables();

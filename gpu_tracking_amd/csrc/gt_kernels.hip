@@ -990,7 +990,26 @@ __device__ __forceinline__ void detect_tile_body(
 #pragma unroll
         for (int r = 0; r < FT_WROWS; ++r) wmax[r] = -INFINITY;
         const float *col = Hm + r0 * FT_TW + lane;
-        if constexpr (KT > 0) {
+        if constexpr (KT > FT_WROWS && kInt) {
+            // integer frames (no NaN: max is order-free): window r..r+KT-1 =
+            // suffix max of rows r..7 with prefix max of rows 8..r+KT-1
+            float h[FT_WROWS + KT - 1];
+#pragma unroll
+            for (int q = 0; q < FT_WROWS + KT - 1; ++q) h[q] = col[q * FT_TW];
+            float suf[FT_WROWS];
+            suf[FT_WROWS - 1] = h[FT_WROWS - 1];
+#pragma unroll
+            for (int q = FT_WROWS - 2; q >= 0; --q) suf[q] = fmaxf(h[q], suf[q + 1]);
+            float pre = h[FT_WROWS];
+#pragma unroll
+            for (int q = FT_WROWS + 1; q < FT_WROWS + KT - 1; ++q) {
+                pre = fmaxf(pre, h[q]);
+                const int r = q - (KT - 1);   // window r..q ends here
+                if (r >= 0 && r < FT_WROWS) wmax[r] = fmaxf(suf[r], pre);
+            }
+            // the window ending at row FT_WROWS itself (KT = FT_WROWS + 1)
+            if constexpr (KT - 1 == FT_WROWS) wmax[0] = fmaxf(suf[0], h[FT_WROWS]);
+        } else if constexpr (KT > 0) {
 #pragma unroll
             for (int q = 0; q < FT_WROWS + KT - 1; ++q) {
                 const float hq = col[q * FT_TW];
@@ -1951,20 +1970,25 @@ __global__ __launch_bounds__(256) void k_std_level(float *__restrict__ tmp, size
     t[i] = sum;
 }
 
+// div[f] (when given) replaces the pixel count as the divisor: mean div,
+// variance div - 1 (the adaptive filter's count of unmasked pixels)
 __global__ void k_std_last(const float *__restrict__ tmp, size_t pitch, uint32_t len, size_t n,
-                           int var, uint32_t nframes, float *__restrict__ out) {
+                           int var, uint32_t nframes, const uint32_t *__restrict__ div,
+                           float *__restrict__ out) {
     const uint32_t f = blockIdx.x * blockDim.x + threadIdx.x;
     if (f >= nframes) return;
     const float *t = tmp + static_cast<size_t>(f) * pitch;
     float sum = 0.0f;
     for (uint32_t i = 0; i < len; ++i) sum = sum + t[i];
-    if (var) out[f] = sqrtf(sum / static_cast<float>(static_cast<uint32_t>(n - 1)));
-    else out[f] = sum / static_cast<float>(n);
+    const uint32_t d = div ? div[f] : static_cast<uint32_t>(n);
+    if (var) out[f] = sqrtf(sum / static_cast<float>(d - 1u));
+    else out[f] = sum / static_cast<float>(div ? d : n);
 }
 
 template <typename T>
 hipError_t launch_frame_std_t(const T *frames, uint32_t nframes, size_t n, float *tmp,
-                              float *mean, float *std_out, hipStream_t s) {
+                              float *mean, float *std_out, hipStream_t s,
+                              const uint32_t *div = nullptr) {
     const uint32_t s0 = static_cast<uint32_t>((n + 7) / 8);
     const dim3 g0((s0 + 255) / 256, nframes);
     for (int pass = 0; pass < 2; ++pass) {
@@ -1982,7 +2006,7 @@ hipError_t launch_frame_std_t(const T *frames, uint32_t nframes, size_t n, float
             if (e != hipSuccess) return e;
             len = st;
         }
-        k_std_last<<<(nframes + 63) / 64, 64, 0, s>>>(tmp, s0, len, n, pass, nframes,
+        k_std_last<<<(nframes + 63) / 64, 64, 0, s>>>(tmp, s0, len, n, pass, nframes, div,
                                                        pass == 0 ? mean : std_out);
         e = hipGetLastError();
         if (e != hipSuccess) return e;
@@ -1990,6 +2014,154 @@ hipError_t launch_frame_std_t(const T *frames, uint32_t nframes, size_t n, float
     return hipSuccess;
 }
 }  // namespace
+
+namespace {
+// ---- adaptive_background (adaptive_filter.wgsl, execute_gpu.rs:290-330) ----
+template <typename T>
+__global__ void k_widen_frames(const T *__restrict__ src, size_t n, float *__restrict__ dst) {
+    for (size_t i = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; i < n;
+         i += static_cast<size_t>(gridDim.x) * blockDim.x)
+        dst[i] = to_f32(src[i]);
+}
+
+__global__ void k_fill_counter(uint32_t *__restrict__ c, uint32_t n, uint32_t v) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) c[i] = v;
+}
+
+// One pass over frame f's rows not yet accepted (pass 0: every row): a row
+// is accepted when mass >= (minmass, or minmass_snr * count * std) and
+// max_intensity >= snr * std; acceptance marks its walk disk (centre
+// rounded half to even) in the f32 copy of the frame with the NaN marker and
+// takes u32(count) off the frame's divisor.  state: 1 accepted, 0 not yet.
+__global__ __launch_bounds__(256) void k_adapt_filter(
+    const gt_detection *__restrict__ rows, uint32_t cap, const uint32_t *__restrict__ ocount,
+    GtKParams p, const float *__restrict__ fstd, uint32_t pass, float *__restrict__ masked,
+    uint8_t *__restrict__ state, uint32_t *__restrict__ counter) {
+    const uint32_t f = blockIdx.y;
+    const uint32_t n = min(ocount[f * GT_COUNTER_STRIDE], cap);
+    const int H = p.H, W = p.W;
+    const float sd = fstd[f];
+    float *M = masked + static_cast<size_t>(f) * H * W;
+    const gt_detection *fr = rows + static_cast<size_t>(f) * cap;
+    uint8_t *st = state + static_cast<size_t>(f) * cap;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        if (pass > 0 && st[i]) continue;
+        const gt_result_row row = fr[i].row;
+        float mm;
+        if (p.minmass > 0.0f) mm = p.minmass;
+        else mm = p.minmass_snr * row.count * sd;
+        if (row.mass < mm || row.max_intensity < p.snr * sd) {
+            st[i] = 0;
+            continue;
+        }
+        st[i] = 1;
+        const int rint = p.rint;
+        const int u = f2i(rintf(row.x)), v = f2i(rintf(row.y));
+        const float r = static_cast<float>(rint), r2 = r * r;
+        for (int a = -rint; a <= rint; ++a) {
+            const float x = static_cast<float>(a), x2 = x * x;
+            for (int b = -rint; b <= rint; ++b) {
+                const float y = static_cast<float>(b), y2 = y * y;
+                if (x2 + y2 > r2) continue;
+                const int pu = u + a, pv = v + b;
+                if (pu < 0 || pu >= H || pv < 0 || pv >= W) continue;
+                M[static_cast<size_t>(pu) * W + pv] = __uint_as_float(0xFFFFFFFFu);
+            }
+        }
+        atomicSub(&counter[f], static_cast<uint32_t>(row.count));
+    }
+}
+
+// Keeps frame f's accepted rows, in order, at the front of its slots; one
+// block per frame walks the rows in tiles (every read of a tile precedes its
+// writes, and a tile's destinations lie below its own end).  The frame's
+// count becomes the accepted count (a count above cap is left as it is: the
+// capacity error stands).
+__global__ __launch_bounds__(256) void k_adapt_compact(gt_detection *__restrict__ rows,
+                                                       uint32_t cap,
+                                                       uint32_t *__restrict__ ocount,
+                                                       const uint8_t *__restrict__ state) {
+    __shared__ uint32_t s_scan[256];
+    const uint32_t f = blockIdx.x;
+    const uint32_t nc = ocount[f * GT_COUNTER_STRIDE];
+    if (nc > cap) return;
+    gt_detection *fr = rows + static_cast<size_t>(f) * cap;
+    const uint8_t *st = state + static_cast<size_t>(f) * cap;
+    uint32_t off = 0;
+    for (uint32_t t0 = 0; t0 < nc; t0 += 256) {
+        const uint32_t i = t0 + threadIdx.x;
+        const bool keep = i < nc && st[i];
+        gt_detection d;
+        if (keep) d = fr[i];
+        s_scan[threadIdx.x] = keep ? 1u : 0u;
+        __syncthreads();
+        for (uint32_t o = 1; o < 256; o <<= 1) {
+            const uint32_t v = threadIdx.x >= o ? s_scan[threadIdx.x - o] : 0u;
+            __syncthreads();
+            s_scan[threadIdx.x] += v;
+            __syncthreads();
+        }
+        if (keep) fr[off + s_scan[threadIdx.x] - 1u] = d;
+        const uint32_t tot = s_scan[255];
+        __syncthreads();
+        off += tot;
+    }
+    if (threadIdx.x == 0) ocount[f * GT_COUNTER_STRIDE] = off;
+}
+
+template <typename T>
+hipError_t launch_adaptive_t(const T *frames, uint32_t nf, const GtKParams &p, uint32_t n_iter,
+                             gt_detection *out, uint32_t cap, uint32_t *ocount, float *std_tmp,
+                             float *mean, float *std2, float *masked, uint8_t *state,
+                             uint32_t *counter, hipStream_t s) {
+    const size_t HW = static_cast<size_t>(p.H) * p.W;
+    k_widen_frames<T><<<2048, 256, 0, s>>>(frames, HW * nf, masked);
+    k_fill_counter<<<(nf + 255) / 256, 256, 0, s>>>(counter, nf, static_cast<uint32_t>(HW));
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    const dim3 g(16, nf);
+    for (uint32_t it = 0; it < n_iter; ++it) {
+        const float *sd = p.fstd;
+        if (it > 0) {
+            e = launch_frame_std_t<float>(masked, nf, HW, std_tmp, mean, std2, s, counter);
+            if (e != hipSuccess) return e;
+            sd = std2;
+        }
+        k_adapt_filter<<<g, 256, 0, s>>>(out, cap, ocount, p, sd, it, masked, state, counter);
+        e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    if (n_iter == 0) {
+        // no pass: no row is accepted
+        k_fill_counter<<<(nf * GT_COUNTER_STRIDE + 255) / 256, 256, 0, s>>>(
+            ocount, nf * GT_COUNTER_STRIDE, 0u);
+        return hipGetLastError();
+    }
+    k_adapt_compact<<<nf, 256, 0, s>>>(out, cap, ocount, state);
+    return hipGetLastError();
+}
+}  // namespace
+
+hipError_t gt_launch_adaptive(const void *frames, int dtype, uint32_t nframes, const GtKParams &p,
+                              uint32_t n_iter, gt_detection *out, uint32_t cap, uint32_t *ocount,
+                              float *std_tmp, float *mean, float *std2, float *masked,
+                              uint8_t *state, uint32_t *counter, hipStream_t s) {
+    if (nframes == 0) return hipSuccess;
+    if (!p.fstd) return hipErrorInvalidValue;
+    switch (dtype) {
+    case GT_U16:
+        return launch_adaptive_t(static_cast<const uint16_t *>(frames), nframes, p, n_iter, out,
+                                 cap, ocount, std_tmp, mean, std2, masked, state, counter, s);
+    case GT_F32:
+        return launch_adaptive_t(static_cast<const float *>(frames), nframes, p, n_iter, out, cap,
+                                 ocount, std_tmp, mean, std2, masked, state, counter, s);
+    case GT_U8:
+        return launch_adaptive_t(static_cast<const uint8_t *>(frames), nframes, p, n_iter, out,
+                                 cap, ocount, std_tmp, mean, std2, masked, state, counter, s);
+    default: return hipErrorInvalidValue;
+    }
+}
 
 hipError_t gt_launch_frame_std(const void *frames, int dtype, uint32_t nframes, size_t n,
                                float *tmp, float *mean, float *std_out, hipStream_t s) {

@@ -108,6 +108,12 @@ struct gt_context {
     // noise estimate of each chunk's frames (snr / minmass_snr), kp.fstd = d_std
     float *d_std_tmp = nullptr;    // chunk * ceil(HW / 8)
     float *d_mean = nullptr, *d_std = nullptr;  // chunk
+    // adaptive_background: marked f32 copy of the chunk, re-estimates, the
+    // divisors, and a state byte per row slot (grown to chunk * cap)
+    float *d_adapt_masked = nullptr, *d_std2 = nullptr;
+    uint32_t *d_adapt_counter = nullptr;
+    uint8_t *d_adapt_state = nullptr;
+    size_t adapt_state_cap = 0;
 
     // gt_execute staging: two slots, so one chunk's detections come back
     // and are post-processed while the next chunk runs on the GPU
@@ -163,6 +169,11 @@ gt_status alloc_scratch(gt_context *c) {
         gt_status st = alloc_std(c);
         if (st != GT_OK) return st;
         c->kp.fstd = c->d_std;
+    }
+    if (c->der.adaptive) {
+        GT_HIP(hipMalloc(&c->d_adapt_masked, sizeof(float) * HW * c->chunk));
+        GT_HIP(hipMalloc(&c->d_std2, sizeof(float) * c->chunk));
+        GT_HIP(hipMalloc(&c->d_adapt_counter, sizeof(uint32_t) * c->chunk));
     }
     if (!c->tiled)
         GT_HIP(hipMalloc(&c->d_cand, sizeof(uint32_t) * static_cast<size_t>(c->cand_cap) * c->chunk));
@@ -229,6 +240,10 @@ void free_all(gt_context *c) {
     if (c->d_stats) (void)hipFree(c->d_stats);
     if (c->d_offs) (void)hipFree(c->d_offs);
     if (c->d_std_tmp) (void)hipFree(c->d_std_tmp);
+    if (c->d_adapt_masked) (void)hipFree(c->d_adapt_masked);
+    if (c->d_std2) (void)hipFree(c->d_std2);
+    if (c->d_adapt_counter) (void)hipFree(c->d_adapt_counter);
+    if (c->d_adapt_state) (void)hipFree(c->d_adapt_state);
     if (c->d_mean) (void)hipFree(c->d_mean);
     if (c->d_std) (void)hipFree(c->d_std);
     for (auto &sl : c->slot) {
@@ -247,6 +262,24 @@ void free_all(gt_context *c) {
     if (c->copy_stream) (void)hipStreamDestroy(c->copy_stream);
     for (hipEvent_t e : c->events) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
+}
+
+// adaptive_background passes over a chunk's walked rows (before characterize).
+gt_status run_adaptive(gt_context *c, const void *frames, int32_t dtype, uint32_t nf,
+                       gt_detection *out, uint32_t cap, hipStream_t s) {
+    const size_t need = static_cast<size_t>(c->chunk) * cap;
+    if (need > c->adapt_state_cap) {
+        GT_HIP(hipStreamSynchronize(s));
+        if (c->d_adapt_state) (void)hipFree(c->d_adapt_state);
+        c->d_adapt_state = nullptr;
+        c->adapt_state_cap = 0;
+        GT_HIP(hipMalloc(&c->d_adapt_state, need));
+        c->adapt_state_cap = need;
+    }
+    GT_HIP(gt_launch_adaptive(frames, dtype, nf, c->kp, c->der.adaptive, out, cap, c->d_ocount,
+                              c->d_std_tmp, c->d_mean, c->d_std2, c->d_adapt_masked,
+                              c->d_adapt_state, c->d_adapt_counter, s));
+    return GT_OK;
 }
 
 // Runs the three pipeline stages over `nframes` device frames in chunks.
@@ -345,6 +378,10 @@ gt_status run_pipeline(gt_context *c, const void *d_frames, int32_t dtype, uint3
             if (ev) GT_HIP(hipEventRecord(ev[2], s));
             GT_HIP(gt_launch_walk_slow(c->d_proc, frames, dtype, nf, kw, c->d_slow, c->slow_cap,
                                        c->d_scount, out, cap, c->d_ocount, f0, s));
+            if (c->der.adaptive) {
+                gt_status ast = run_adaptive(c, frames, dtype, nf, out, cap, s);
+                if (ast != GT_OK) return ast;
+            }
             if (c->kp.characterize)
                 GT_HIP(gt_launch_characterize_det(c->d_proc, frames, dtype, nf, c->kp, out, cap,
                                                   c->d_ocount, s));
@@ -353,6 +390,10 @@ gt_status run_pipeline(gt_context *c, const void *d_frames, int32_t dtype, uint3
             if (ev) GT_HIP(hipEventRecord(ev[2], s));
             GT_HIP(gt_launch_walk(c->d_proc, frames, dtype, nf, c->kp, c->d_cand, c->cand_cap,
                                   c->d_ccount, out, cap, c->d_ocount, f0, s));
+            if (c->der.adaptive) {
+                gt_status ast = run_adaptive(c, frames, dtype, nf, out, cap, s);
+                if (ast != GT_OK) return ast;
+            }
         }
         GT_HIP(gt_launch_copy_counts(c->d_ocount, d_counts + f0, nf, s));
         if (ev) GT_HIP(hipEventRecord(ev[3], s));
@@ -393,6 +434,9 @@ gt_status gt_context_create(const gt_params *params, uint32_t nrows, uint32_t nc
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
         return fail(GT_ERR_GPU_ADAPTER, "no HIP device is available");
+    if (der.adaptive && characterize_points)
+        return fail(GT_ERR_INVALID_ARGUMENT,
+                    "adaptive_background does not apply to characterize_points");
     gt_context *c = new (std::nothrow) gt_context();
     if (!c) return fail(GT_ERR_GPU_DEVICE, "out of host memory");
     c->params = *params;
@@ -788,9 +832,10 @@ gt_status gt_context_set_cnn(gt_context *c, const gt_cnn *net, float conf_logit,
     }
     if (scope != GT_CNN_MOVING && scope != GT_CNN_ALL)
         return fail(GT_ERR_INVALID_ARGUMENT, "unknown proposal scope %d", scope);
-    if (!c->tiled || c->log)
+    if (!c->tiled || c->log || c->der.adaptive)
         return fail(GT_ERR_INVALID_ARGUMENT,
-                    "the learned proposal runs in the tiled Trackpy-style path only");
+                    "the learned proposal runs in the tiled Trackpy-style path only "
+                    "(without adaptive_background)");
     if (!std::isfinite(conf_logit) && conf_logit != INFINITY)
         return fail(GT_ERR_INVALID_ARGUMENT, "confidence threshold must not be NaN or -inf");
     if (!(tol >= 0.0f) || !std::isfinite(tol))

@@ -93,6 +93,11 @@ def lib():
                                           C.c_float, C.c_void_p, C.c_size_t, C.c_void_p]
         L.or_log_detect_frame.restype = C.c_size_t
         L.or_frame_std.argtypes = [C.c_void_p, C.c_size_t, C.c_void_p, C.c_void_p]
+        L.or_frame_std_div.argtypes = [C.c_void_p, C.c_size_t, C.c_uint32, C.c_uint32,
+                                       C.c_void_p, C.c_void_p]
+        L.or_detect_frame_adaptive.argtypes = [C.c_void_p, P(Params), C.c_uint32, C.c_int,
+                                               C.c_void_p, C.c_size_t]
+        L.or_detect_frame_adaptive.restype = C.c_size_t
         _lib = L
     return _lib
 
@@ -339,3 +344,22 @@ def frame_std(frame):
     s = np.zeros(1, np.float32)
     lib().or_frame_std(_ptr(f), f.size, _ptr(m), _ptr(s))
     return float(m[0]), float(s[0])
+
+
+def frame_std_div(frame, div_mean, div_var):
+    """frame_std with explicit divisors (the adaptive filter's re-estimates)."""
+    f = np.ascontiguousarray(frame, dtype=np.float32).reshape(-1)
+    m = np.zeros(1, np.float32)
+    s = np.zeros(1, np.float32)
+    lib().or_frame_std_div(_ptr(f), f.size, int(div_mean), int(div_var), _ptr(m), _ptr(s))
+    return float(m[0]), float(s[0])
+
+
+def detect_frame_adaptive(frame, p: Params, n_iter: int, characterize: bool = True):
+    """Trackpy detection with adaptive_background = n_iter (p from
+    derive_trackpy(..., adaptive_background=True)); accepted rows."""
+    f = _f32(frame)
+    rows = np.zeros(max(f.size, 1), ROW_DTYPE)
+    n = lib().or_detect_frame_adaptive(_ptr(f), C.byref(p), int(n_iter), int(characterize),
+                                       _ptr(rows), rows.size)
+    return rows[:n].copy()

@@ -783,6 +783,11 @@ static size_t or_tree_level(float *t, size_t len) {
 }
 
 void or_frame_std(const float *x, size_t n, float *mean_out, float *std_out) {
+    or_frame_std_div(x, n, (uint32_t)n, (uint32_t)(n - 1), mean_out, std_out);
+}
+
+void or_frame_std_div(const float *x, size_t n, uint32_t div_mean, uint32_t div_var,
+                      float *mean_out, float *std_out) {
     const size_t s0 = (n + 7) / 8;
     float *t = (float *)malloc(sizeof(float) * (s0 > 0 ? s0 : 1));
     /* mean: sum_image_outplace (markers skipped), then the in-place levels */
@@ -798,7 +803,7 @@ void or_frame_std(const float *x, size_t n, float *mean_out, float *std_out) {
     while (len > 24) len = or_tree_level(t, len);
     float m = 0.0f;
     for (size_t i = 0; i < len; i++) m = m + t[i];
-    m = m / (float)n;
+    m = m / (float)div_mean;
     /* variance: std_image (squared deviations, 0 at markers) fused into the
      * first in-place level over n values */
     for (size_t i = 0; i < s0; i++) {
@@ -820,8 +825,83 @@ void or_frame_std(const float *x, size_t n, float *mean_out, float *std_out) {
     while (len > 24) len = or_tree_level(t, len);
     float v = 0.0f;
     for (size_t i = 0; i < len; i++) v = v + t[i];
-    v = v / (float)(uint32_t)(n - 1);
+    v = v / (float)div_var;
     *mean_out = m;
     *std_out = sqrtf(v);
     free(t);
+}
+
+/* adaptive_filter.wgsl:56-87 set_nan: the walk disk (radius (circle-1)/2,
+ * r <= 0 rows) around the row's centre (WGSL round: half to even) marked
+ * with the NaN marker */
+static void or_mask_row(float *fr, const or_params *p, const or_row *row) {
+    int32_t rint;
+    if (row->r > 0.0f) rint = or_f2i(ceilf(row->r));
+    else rint = (p->circle - 1) / 2;
+    const int32_t u = or_f2i(rintf(row->x)), v = or_f2i(rintf(row->y));
+    const float r = (float)rint, r2 = r * r;
+    uint32_t marker = 0xFFFFFFFFu;
+    float mk;
+    memcpy(&mk, &marker, sizeof(mk));
+    for (int32_t i = -rint; i <= rint; i++) {
+        const float x = (float)i, x2 = x * x;
+        for (int32_t j = -rint; j <= rint; j++) {
+            const float y = (float)j, y2 = y * y;
+            if (x2 + y2 > r2) continue;
+            const int32_t pu = u + i, pv = v + j;
+            if (pu < 0 || pu >= p->nrows || pv < 0 || pv >= p->ncols) continue;
+            fr[(size_t)pu * p->ncols + pv] = mk;
+        }
+    }
+}
+
+size_t or_detect_frame_adaptive(const float *frame, const or_params *p_in, uint32_t n_iter,
+                                int characterize, or_row *rows, size_t cap) {
+    or_params p = *p_in;
+    const size_t hw = (size_t)p.nrows * p.ncols;
+    float mean, sd;
+    or_frame_std(frame, hw, &mean, &sd);
+    p.image_std = sd;
+    float *proc = (float *)malloc(sizeof(float) * hw * 3);
+    uint32_t *cand = (uint32_t *)malloc(sizeof(uint32_t) * (hw ? hw : 1));
+    or_bandpass(frame, &p, proc, proc + hw);
+    const size_t nc = or_local_max(proc, &p, cand, hw, proc + hw);
+    or_row *pend = (or_row *)malloc(sizeof(or_row) * (nc ? nc : 1));
+    or_row *next = (or_row *)malloc(sizeof(or_row) * (nc ? nc : 1));
+    or_row *acc = (or_row *)malloc(sizeof(or_row) * (nc ? nc : 1));
+    size_t np = or_walk(proc, &p, cand, nc, pend), na = 0;
+    float *masked = (float *)malloc(sizeof(float) * (hw ? hw : 1));
+    memcpy(masked, frame, sizeof(float) * hw);
+    uint32_t counter = (uint32_t)hw;
+    float cur = sd;
+    for (uint32_t it = 0; it < n_iter; it++) {
+        if (it > 0) or_frame_std_div(masked, hw, counter, counter - 1u, &mean, &cur);
+        size_t nn = 0;
+        for (size_t k = 0; k < np; k++) {
+            const or_row *row = &pend[k];
+            float mm;
+            if (p.minmass > 0.0f) mm = p.minmass;
+            else mm = p.minmass_snr * row->count * cur;
+            if (row->mass < mm || row->max_intensity < p.snr * cur) {
+                next[nn++] = *row;
+            } else {
+                acc[na++] = *row;
+                or_mask_row(masked, &p, row);
+                counter -= (uint32_t)row->count;
+            }
+        }
+        memcpy(pend, next, sizeof(or_row) * nn);
+        np = nn;
+    }
+    if (characterize) or_characterize(proc, frame, &p, acc, na, 1, 0);
+    size_t out = na;
+    if (na <= cap) memcpy(rows, acc, sizeof(or_row) * na);
+    else out = (size_t)-1;
+    free(masked);
+    free(acc);
+    free(next);
+    free(pend);
+    free(cand);
+    free(proc);
+    return out;
 }

@@ -153,6 +153,23 @@ size_t or_log_detect_frame(const float *frame, const or_params *p, const float *
  * stale or out-of-range values; identical whenever every level length is a
  * multiple of 8).  N must exceed 24. */
 void or_frame_std(const float *frame, size_t n, float *mean, float *std);
+/* The same with explicit divisors (mean: div_mean, variance: div_var); the
+ * adaptive filter's re-estimates divide by the pixel count less the
+ * accepted particles' disk counts (adaptive_filter.wgsl:116). */
+void or_frame_std_div(const float *frame, size_t n, uint32_t div_mean, uint32_t div_var,
+                      float *mean, float *std);
+/* Trackpy detection with adaptive_background = n_iter (src/execute_gpu.rs:
+ * 290-330, adaptive_filter.wgsl:89-125): the noise estimate of the raw frame;
+ * local max and walk with rough_snr_factor 0.5 (p from or_derive_trackpy
+ * with adaptive_background); then n_iter passes over the rows not yet
+ * accepted: accepted when mass >= (minmass, or minmass_snr * count * std)
+ * and max_intensity >= snr * std (no rough factor), which marks the row's
+ * walk disk with the NaN marker in a copy of the frame and takes its count
+ * off the divisor; from the second pass on the estimate is recomputed on
+ * the marked copy.  Accepted rows (in acceptance order), characterized on
+ * the unmarked frame. */
+size_t or_detect_frame_adaptive(const float *frame, const or_params *p, uint32_t n_iter,
+                                int characterize, or_row *rows, size_t cap);
 void or_bandpass_f64(const float *frame, const or_params *p, double *processed, double *scratch);
 /* FP64 centroid of the disk at integer position (pu, pv) of a f64 frame. */
 void or_centroid_f64(const double *processed, const or_params *p, int32_t pu, int32_t pv,
