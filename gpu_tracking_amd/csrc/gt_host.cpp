@@ -5,6 +5,8 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <unordered_map>
 
 namespace {
@@ -220,6 +222,51 @@ void gt_host_fill_logtable(const gt_derived *d, GtLogTable *t) {
     }
 }
 
+namespace {
+
+template <class Fma>
+bool one_correction_exact_with(float b, Fma fma) {
+    const float rb = 1.0f / b;
+    for (uint32_t m = 0; m < (1u << 23); ++m) {
+        const uint32_t u = 0x3F800000u | m;
+        float a;
+        std::memcpy(&a, &u, 4);
+        const float q = a * rb;
+        const float r = fma(-q, b, a);
+        const float q1 = fma(r, rb, q);
+        if (q1 != a / b) return false;
+    }
+    return true;
+}
+
+__attribute__((target("fma"))) bool one_correction_exact_hw(float b) {
+    return one_correction_exact_with(b, [](float x, float y, float z) { return __builtin_fmaf(x, y, z); });
+}
+
+bool one_correction_exact_lib(float b) {
+    return one_correction_exact_with(b, [](float x, float y, float z) { return std::fma(x, y, z); });
+}
+
+}  // namespace
+
+bool gt_host_one_correction_exact(float b) {
+    if (!(b >= 1.0f && b < 16777216.0f)) return false;
+    static std::mutex mu;
+    static std::map<uint32_t, bool> cache;
+    uint32_t key;
+    std::memcpy(&key, &b, 4);
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        auto it = cache.find(key);
+        if (it != cache.end()) return it->second;
+    }
+    const bool ok = __builtin_cpu_supports("fma") ? one_correction_exact_hw(b)
+                                                  : one_correction_exact_lib(b);
+    std::lock_guard<std::mutex> lk(mu);
+    cache[key] = ok;
+    return ok;
+}
+
 void gt_host_fill_kparams(const gt_derived *d, uint32_t H, uint32_t W, GtKParams *kp) {
     std::memset(kp, 0, sizeof(*kp));
     kp->H = static_cast<int32_t>(H);
@@ -240,11 +287,15 @@ void gt_host_fill_kparams(const gt_derived *d, uint32_t H, uint32_t W, GtKParams
     kp->rgnorm = 1.0f / kp->gnorm;
     kp->rbox = 1.0f / kp->box_area;
     // Integer frames let the bandpass divide by the constant norms with a
-    // reciprocal and two fma corrections; that needs normal (not tiny)
-    // dividends, which holds when every nonzero weight is >= 2^-100.
+    // reciprocal and one fma correction: that needs dividends that are not
+    // tiny (every nonzero weight >= 2^-46) and the sequence checked exact for
+    // both divisors; otherwise the general kernel divides.
     kp->fast_div = 1;
     for (int32_t i = 0; i <= 2 * R; ++i)
-        if (kp->w[i] != 0.0f && kp->w[i] < 7.888609052210118e-31f) kp->fast_div = 0;
+        if (kp->w[i] != 0.0f && kp->w[i] < 1.4210854715202004e-14f) kp->fast_div = 0;
+    if (kp->fast_div && !(gt_host_one_correction_exact(kp->gnorm) &&
+                          gt_host_one_correction_exact(kp->box_area)))
+        kp->fast_div = 0;
     kp->truncate = d->truncate;
     kp->k = static_cast<int32_t>(d->dilation);
     kp->a = f2i_sat(static_cast<float>(kp->k) / 2.0f - 0.5f);

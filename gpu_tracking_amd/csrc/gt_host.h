@@ -45,6 +45,9 @@ struct gt_derived {
 void gt_host_params_default(gt_params *p);
 bool gt_host_derive(const gt_params *p, uint32_t H, uint32_t W, gt_derived *d, std::string *why);
 void gt_host_fill_kparams(const gt_derived *d, uint32_t H, uint32_t W, GtKParams *kp);
+// Whether q = RN(a * RN(1/b)), then RN(q + fma(-q, b, a) * RN(1/b)) equals
+// IEEE a / b for every a in [1, 2) (all 2^23 mantissas; cached per divisor).
+bool gt_host_one_correction_exact(float b);
 // The LoG scale table (radii, tap radii, tap weights and norms).
 void gt_host_fill_logtable(const gt_derived *d, GtLogTable *t);
 // LoG post-processing of one frame (execute_gpu.rs:699-712): stable sort by

@@ -23,7 +23,7 @@ struct GtKParams {
     float box_area;               // f32(s*s)
     float rgnorm;                 // RN(1 / gnorm)
     float rbox;                   // RN(1 / box_area)
-    int32_t fast_div;             // every nonzero tap weight >= 2^-100 (see k_bandpass_int)
+    int32_t fast_div;             // k_bandpass_int may divide by the norms with one correction
     int32_t truncate;             // truncate_preprocessed
     int32_t k;                    // dilation side
     int32_t a;                    // window start offset i32(k/2 - 0.5)

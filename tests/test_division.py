@@ -1,9 +1,11 @@
 """The integer-input bandpass kernel divides by its constant norms with a
-reciprocal and two fma corrections instead of IEEE division.  This checks
+reciprocal and one fma correction instead of IEEE division.  This checks
 that sequence against IEEE a / b on the CPU (same f32 operations, correctly
 rounded fma) for the divisors the pipeline uses: every integer dividend up to
-2^24 for the boxcar areas, whole binades and random samples for the Gaussian
-norms of several (smoothing_size, noise_size) pairs."""
+2^24 for the boxcar areas, whole binades (every mantissa) and random samples
+over 115 binades for the Gaussian norms of several (smoothing_size,
+noise_size) pairs.  The library repeats the whole-binade check for each
+context's divisors before it uses the sequence."""
 import os
 import subprocess
 
