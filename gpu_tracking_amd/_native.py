@@ -94,6 +94,8 @@ EXPORTS = [
     "gt_batch_set_graph", "gt_frame_std", "gt_cnn_create", "gt_cnn_destroy", "gt_cnn_set_params",
     "gt_cnn_get_params", "gt_cnn_adam_steps", "gt_cnn_forward", "gt_cnn_gradient", "gt_cnn_step",
     "gt_cnn_train", "gt_cnn_make_samples", "gt_context_set_cnn", "gt_context_cnn_stats",
+    "gt_file_open", "gt_file_close", "gt_file_info", "gt_file_read", "gt_file_channels",
+    "gt_file_mean", "gt_execute_file", "gt_write_tiff",
 ]
 
 CNN_NPARAM = 14147
@@ -185,6 +187,13 @@ def lib() -> C.CDLL:
         "gt_context_set_cnn": [vp, vp, f32, f32, C.c_int32],
         "gt_frame_std": [vp, vp, C.c_int32, u32, vp, vp],
         "gt_context_cnn_stats": [vp, P(u64), P(u64), C.c_int32],
+        "gt_file_open": [C.c_char_p, C.c_int32, u32, P(vp)],
+        "gt_file_info": [vp, P(u32), P(u32), P(u64), P(C.c_int32)],
+        "gt_file_read": [vp, u64, u64, vp],
+        "gt_file_mean": [vp, vp],
+        "gt_execute_file": [vp, vp, vp, u64, vp, u64, u32, C.c_int32, C.c_int32, vp, u64,
+                            P(GtOutput)],
+        "gt_write_tiff": [C.c_char_p, vp, u64, u32, u32],
     }
     for name, args in sigs.items():
         fn = getattr(L, name)
@@ -192,6 +201,10 @@ def lib() -> C.CDLL:
         fn.restype = C.c_int
     L.gt_cnn_destroy.argtypes = [vp]
     L.gt_cnn_destroy.restype = None
+    L.gt_file_close.argtypes = [vp]
+    L.gt_file_close.restype = None
+    L.gt_file_channels.argtypes = [vp, vp, u32]
+    L.gt_file_channels.restype = u32
     _lib = L
     return L
 

@@ -17,6 +17,7 @@ Fixes of reference wrapper bugs (documented divergences):
 from __future__ import annotations
 
 import ctypes as C
+import os
 import math
 from typing import Iterable, Optional
 
@@ -46,9 +47,95 @@ def _dtype_code(arr: np.ndarray) -> int:
     raise TypeError(f"unsupported frame dtype {arr.dtype}")
 
 
-def _as_video(file_or_array) -> np.ndarray:
-    if isinstance(file_or_array, (str, bytes)) or hasattr(file_or_array, "__fspath__"):
-        raise N.GtError(12, "file input is not supported in this build; pass a numpy array")
+_NP_OF_CODE = {N.GT_U8: np.uint8, N.GT_U16: np.uint16, N.GT_F32: np.float32}
+
+
+def _is_path(x) -> bool:
+    return isinstance(x, (str, bytes)) or hasattr(x, "__fspath__")
+
+
+class VideoFile:
+    """A video file read by the library (path_to_iter, src/execute_gpu.rs:
+    1186-1270): tif / tiff (multi-page), ets, or vsi (whose frames are in
+    _<stem>_/stack1/frame_t_0.ets beside it); `channel` picks the ETS
+    channel (default 0).  Frames keep their sample type (u8, u16, f32; i8 and
+    i16 read as f32).  shape = (frames, rows, cols)."""
+
+    def __init__(self, path, channel=None):
+        self._lib = N.lib()
+        self._h = C.c_void_p()
+        N.check(self._lib.gt_file_open(os.fsencode(path), int(channel is not None),
+                                       int(channel or 0), C.byref(self._h)))
+        nr, nc, nf, dt = C.c_uint32(), C.c_uint32(), C.c_uint64(), C.c_int32()
+        N.check(self._lib.gt_file_info(self._h, C.byref(nr), C.byref(nc), C.byref(nf),
+                                       C.byref(dt)))
+        self.shape = (int(nf.value), int(nr.value), int(nc.value))
+        self.code = int(dt.value)
+        self.dtype = np.dtype(_NP_OF_CODE[self.code])
+
+    @property
+    def handle(self) -> C.c_void_p:
+        return self._h
+
+    def __len__(self) -> int:
+        return self.shape[0]
+
+    def read(self, keys=None) -> np.ndarray:
+        """Frames `keys` (default: all, in order) as a (n, rows, cols) array of
+        the file's sample type."""
+        idx = range(self.shape[0]) if keys is None else [int(k) for k in keys]
+        out = np.empty((len(idx),) + self.shape[1:], self.dtype)
+        # runs of consecutive frames go in one call
+        i = 0
+        idx = list(idx)
+        while i < len(idx):
+            j = i + 1
+            while j < len(idx) and idx[j] == idx[j - 1] + 1:
+                j += 1
+            if idx[i] < 0:
+                raise N.GtError(7, f"Requested a frame outside the length\n of the video. Video "
+                                   f"length is {self.shape[0]}, requested frame {idx[i]}")
+            N.check(self._lib.gt_file_read(self._h, idx[i], j - i, out[i:j].ctypes.data))
+            i = j
+        return out
+
+    def mean(self) -> np.ndarray:
+        """mean_from_iter (src/execute_gpu.rs:819-847): per-pixel f32 sum in
+        frame order divided by the frame count."""
+        out = np.empty(self.shape[1:], np.float32)
+        N.check(self._lib.gt_file_mean(self._h, out.ctypes.data))
+        return out
+
+    def channels(self) -> list:
+        """ETS channels present in the file (empty for TIFF)."""
+        n = self._lib.gt_file_channels(self._h, None, 0)
+        buf = (C.c_uint32 * max(int(n), 1))()
+        self._lib.gt_file_channels(self._h, buf, n)
+        return [int(buf[i]) for i in range(n)]
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) and self._h.value:
+            self._lib.gt_file_close(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+def _as_video(file_or_array, channel=None):
+    if isinstance(file_or_array, VideoFile):
+        return file_or_array
+    if _is_path(file_or_array):
+        return VideoFile(file_or_array, channel)
     arr = np.asarray(file_or_array)
     if arr.ndim == 2:
         arr = arr[None]
@@ -163,9 +250,11 @@ class Context:
     def execute(self, video: np.ndarray, keys: Optional[Iterable[int]] = None,
                 points: Optional[np.ndarray] = None, points_has_frames: bool = False,
                 points_has_r: bool = False, linker_reset_points=None):
-        """gt_execute over a host video; returns (matrix, names, types)."""
-        video = np.ascontiguousarray(video)
-        if video.shape[1:] != self.shape:
+        """gt_execute over a host video (or gt_execute_file over a VideoFile);
+        returns (matrix, names, types)."""
+        if not isinstance(video, VideoFile):
+            video = np.ascontiguousarray(video)
+        if tuple(video.shape[1:]) != self.shape:
             raise N.GtError(4, f"dimension mismatch. provided dimensions are {self.shape}, "
                                f"but the frames are {video.shape[1:]}")
         keys_arr = None if keys is None else np.ascontiguousarray(np.asarray(list(keys), np.uint64))
@@ -173,14 +262,17 @@ class Context:
         reset = None if linker_reset_points is None else np.ascontiguousarray(
             np.asarray(list(linker_reset_points), np.uint64))
         out = N.GtOutput()
-        st = self._lib.gt_execute(
-            self._h, video.ctypes.data, _dtype_code(video), video.shape[0],
-            None if keys_arr is None else keys_arr.ctypes.data,
-            0 if keys_arr is None else len(keys_arr),
-            None if pts is None else pts.ctypes.data, 0 if pts is None else pts.shape[0],
-            0 if pts is None else pts.shape[1], int(points_has_frames), int(points_has_r),
-            None if reset is None else reset.ctypes.data, 0 if reset is None else len(reset),
-            C.byref(out))
+        rest = (None if keys_arr is None else keys_arr.ctypes.data,
+                0 if keys_arr is None else len(keys_arr),
+                None if pts is None else pts.ctypes.data, 0 if pts is None else pts.shape[0],
+                0 if pts is None else pts.shape[1], int(points_has_frames), int(points_has_r),
+                None if reset is None else reset.ctypes.data, 0 if reset is None else len(reset),
+                C.byref(out))
+        if isinstance(video, VideoFile):
+            st = self._lib.gt_execute_file(self._h, video.handle, *rest)
+        else:
+            st = self._lib.gt_execute(self._h, video.ctypes.data, _dtype_code(video),
+                                      video.shape[0], *rest)
         N.check(st)
         try:
             n, m = int(out.nrows), int(out.ncols)
@@ -325,11 +417,10 @@ def batch(file_or_array, diameter, **kwargs):
     Returns a pandas DataFrame with columns frame, y, x, mass, [Rg, raw,
     signal, ecc], [raw_mass, raw_bg_median, raw_mass_corrected],
     [particle, duration]."""
-    video = _as_video(file_or_array)
+    video = _as_video(file_or_array, kwargs.pop("channel", None))
     frames = kwargs.pop("frames", None)
     reset = kwargs.pop("linker_reset_points", None)
     kwargs.pop("tqdm", None)
-    kwargs.pop("channel", None)
     learned = kwargs.pop("learned", None)
     learned_conf = kwargs.pop("learned_conf", 0.0)
     learned_tol = kwargs.pop("learned_tol", 0.05)
@@ -367,11 +458,10 @@ def characterize_points(file_or_array, points_to_characterize, diameter=None, ha
         else:
             raise ValueError("Diameter needs to be specified if the supplied points don't have "
                              "associated radiuses")
-    video = _as_video(file_or_array)
+    video = _as_video(file_or_array, kwargs.pop("channel", None))
     frames = kwargs.pop("frames", None)
     reset = kwargs.pop("linker_reset_points", None)
     kwargs.pop("tqdm", None)
-    kwargs.pop("channel", None)
     p = make_params(diameter, **kwargs)
     # characterize_points_rust forces characterize and disables filter_close
     # (macros lib.rs:671-681); r is reported when the points carry it.
@@ -420,15 +510,61 @@ def LoG(file_or_array, min_radius, max_radius, **kwargs):
     Returns a pandas DataFrame with columns frame, y, x, mass, r, [Rg, raw,
     signal, ecc], [raw_mass, raw_bg_median, raw_mass_corrected],
     [particle, duration]; rows of a frame in descending r, then mass."""
-    video = _as_video(file_or_array)
+    video = _as_video(file_or_array, kwargs.pop("channel", None))
     frames = kwargs.pop("frames", None)
     reset = kwargs.pop("linker_reset_points", None)
     kwargs.pop("tqdm", None)
-    kwargs.pop("channel", None)
     p = make_log_params(min_radius, max_radius, **kwargs)
     with Context(p, video.shape[1], video.shape[2]) as ctx:
         data, names, types = ctx.execute(video, keys=frames, linker_reset_points=reset)
     return _to_dataframe(data, names, types)
+
+
+def load(path, frames=None, channel=None) -> np.ndarray:
+    """A video file as a (T, Y, X) float32 array (load_rust,
+    gpu_tracking_python/src/expansion.rs:758-800); `frames` selects and
+    orders frames."""
+    with VideoFile(path, channel) as f:
+        return f.read(frames).astype(np.float32, copy=False)
+
+
+def mean_from_file(path, channel=None) -> np.ndarray:
+    """The mean frame of a video file (mean_from_file_rust,
+    gpu_tracking_python/src/expansion.rs:895-905)."""
+    with VideoFile(path, channel) as f:
+        return f.mean()
+
+
+def parse_ets(path) -> dict:
+    """Every channel of an ETS file as {channel: (T, Y, X) uint16 array}
+    (gpu_tracking_python/src/expansion.rs:849-867)."""
+    with VideoFile(path) as f0:
+        chans = f0.channels()
+    out = {}
+    for ch in chans:
+        with VideoFile(path, ch) as f:
+            out[ch] = f.read()
+    return out
+
+
+def parse_ets_with_keys(path, keys, channel=None) -> np.ndarray:
+    """Frames `keys` of one ETS channel as a uint16 array
+    (gpu_tracking_python/src/expansion.rs:870-892)."""
+    with VideoFile(path, channel) as f:
+        return f.read(keys)
+
+
+def to_tiff(path, video) -> None:
+    """Writes a (T, Y, X) video as uncompressed 16-bit grey TIFF pages,
+    values cast as `f32 as u16` (gpu_tracking_python/src/expansion.rs:
+    925-950)."""
+    arr = np.ascontiguousarray(np.asarray(video, np.float32))
+    if arr.ndim == 2:
+        arr = arr[None]
+    if arr.ndim != 3:
+        raise ValueError(f"expected a (T, Y, X) array, got shape {arr.shape}")
+    N.check(N.lib().gt_write_tiff(os.fsencode(path), arr.ctypes.data, arr.shape[0],
+                                  arr.shape[1], arr.shape[2]))
 
 
 def link(to_link, search_range, memory=0):

@@ -24,6 +24,7 @@
 #include "gt_cnn.h"
 #include "gt_internal.h"
 #include "gt_host.h"
+#include "gt_io.h"
 
 namespace {
 
@@ -593,16 +594,30 @@ void gt_output_free(gt_output *out) {
     out->nrows = 0;
 }
 
-gt_status gt_execute(gt_context *c, const void *frames, int32_t dtype, uint64_t nframes,
-                     const uint64_t *keys, uint64_t nkeys, const float *points, uint64_t npoints,
-                     uint32_t point_cols, int32_t points_has_frames, int32_t points_has_r,
-                     const uint64_t *linker_reset_points, uint64_t nreset, gt_output *out) {
-    if (!c || !out) return fail(GT_ERR_INVALID_ARGUMENT, "null argument");
-    out->data = nullptr;
-    out->nrows = 0;
-    out->ncols = 0;
-    if (!dtype_size(dtype)) return fail(GT_ERR_INVALID_ARGUMENT, "unknown dtype %d", dtype);
-    if (nframes > 0 && !frames) return fail(GT_ERR_INVALID_ARGUMENT, "null frames");
+}  // extern "C"
+
+namespace {
+
+// Where gt_execute's frames come from: a host array or a video file.
+struct FrameSrc {
+    int32_t dtype = GT_U16;
+    uint64_t nframes = 0;
+    const char *host = nullptr;
+    const gt_file *file = nullptr;
+};
+
+// Threads that stage a chunk's frames (copies or file reads).
+unsigned io_threads() {
+    const unsigned hw = std::thread::hardware_concurrency();
+    return std::max(1u, std::min(8u, hw ? hw : 4u));
+}
+
+gt_status execute_frames(gt_context *c, const FrameSrc &src, const uint64_t *keys, uint64_t nkeys,
+                         const float *points, uint64_t npoints, uint32_t point_cols,
+                         int32_t points_has_frames, int32_t points_has_r,
+                         const uint64_t *linker_reset_points, uint64_t nreset, gt_output *out) {
+    const int32_t dtype = src.dtype;
+    const uint64_t nframes = src.nframes;
     c->interrupt.store(0);
     gt_status st = alloc_execute_buffers(c);
     if (st != GT_OK) return st;
@@ -636,7 +651,9 @@ gt_status gt_execute(gt_context *c, const void *frames, int32_t dtype, uint64_t 
         std::vector<gt_detection> rows;
         std::vector<float> post;   // GPU filter_close / doughnut results, 4 per row
         const char *raw;
+        std::vector<char> rawbuf;  // a file frame's copy when the host needs it
     };
+    const bool keep_raw = post.needs_raw() && !c->gpu_post;
     std::deque<Item> queue;
     std::mutex mu;
     std::condition_variable cv_item, cv_space;
@@ -663,7 +680,7 @@ gt_status gt_execute(gt_context *c, const void *frames, int32_t dtype, uint64_t 
             }
             if (failed) continue;   // drain without work after an error
             const float *rawp = nullptr;
-            if (post.needs_raw() && !c->gpu_post) {
+            if (keep_raw) {
                 rawf.resize(HW);
                 gt_host_widen(it.raw, dtype, HW, rawf.data());
                 rawp = rawf.data();
@@ -705,8 +722,33 @@ gt_status gt_execute(gt_context *c, const void *frames, int32_t dtype, uint64_t 
                 return fail(GT_ERR_FRAME_OUT_OF_BOUNDS,
                             "Requested a frame outside the length\n of the video. Video length is %llu, requested frame %llu",
                             static_cast<unsigned long long>(nframes), static_cast<unsigned long long>(idx));
-            std::memcpy(static_cast<char *>(sl.h_stage) + esz * HW * f,
-                        static_cast<const char *>(frames) + esz * HW * idx, esz * HW);
+        }
+        {
+            // frames are copied (host array) or decoded (file) straight into
+            // the pinned staging slot, frame f by thread f % nt (the slot's
+            // previous chunk has been drained)
+            const unsigned nt = std::min<unsigned>(io_threads(), nf);
+            std::vector<gt_status> sts(nt, GT_OK);
+            std::vector<std::string> whys(nt);
+            std::vector<std::thread> th;
+            for (unsigned t = 0; t < nt; ++t)
+                th.emplace_back([&, t]() {
+                    for (uint32_t f = t; f < nf; f += nt) {
+                        char *dst = static_cast<char *>(sl.h_stage) + esz * HW * f;
+                        if (src.host) {
+                            std::memcpy(dst, src.host + esz * HW * plan.frames[base + f], esz * HW);
+                            continue;
+                        }
+                        const gt_status rs = gt_io_read(src.file, plan.frames[base + f], dst, &whys[t]);
+                        if (rs != GT_OK) {
+                            sts[t] = rs;
+                            return;
+                        }
+                    }
+                });
+            for (auto &t : th) t.join();
+            for (unsigned t = 0; t < nt; ++t)
+                if (sts[t] != GT_OK) return fail(sts[t], "%s", whys[t].c_str());
         }
         GT_HIP(hipMemcpyAsync(sl.d_frames, sl.h_stage, esz * HW * nf, hipMemcpyHostToDevice, c->stream));
         gt_status st2;
@@ -766,7 +808,15 @@ gt_status gt_execute(gt_context *c, const void *frames, int32_t dtype, uint64_t 
             it.frame = plan.frames[base + f];
             it.rows.assign(c->h_rows + off, c->h_rows + off + n);
             if (c->gpu_post) it.post.assign(c->h_post + 4 * off, c->h_post + 4 * (off + n));
-            it.raw = static_cast<const char *>(frames) + esz * HW * it.frame;
+            if (src.host) {
+                it.raw = src.host + esz * HW * it.frame;
+            } else if (keep_raw) {
+                const char *st = static_cast<const char *>(sl.h_stage) + esz * HW * f;
+                it.rawbuf.assign(st, st + esz * HW);
+                it.raw = it.rawbuf.data();
+            } else {
+                it.raw = nullptr;
+            }
             off += n;
             std::unique_lock<std::mutex> lk(mu);
             cv_space.wait(lk, [&] { return queue.size() < max_queued || post_status != GT_OK; });
@@ -800,6 +850,124 @@ gt_status gt_execute(gt_context *c, const void *frames, int32_t dtype, uint64_t 
     std::string ferr;
     gt_status fst = post.finish(out, &ferr);
     if (fst != GT_OK) return fail(fst, "%s", ferr.c_str());
+    return GT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+gt_status gt_execute(gt_context *c, const void *frames, int32_t dtype, uint64_t nframes,
+                     const uint64_t *keys, uint64_t nkeys, const float *points, uint64_t npoints,
+                     uint32_t point_cols, int32_t points_has_frames, int32_t points_has_r,
+                     const uint64_t *linker_reset_points, uint64_t nreset, gt_output *out) {
+    if (!c || !out) return fail(GT_ERR_INVALID_ARGUMENT, "null argument");
+    out->data = nullptr;
+    out->nrows = 0;
+    out->ncols = 0;
+    if (!dtype_size(dtype)) return fail(GT_ERR_INVALID_ARGUMENT, "unknown dtype %d", dtype);
+    if (nframes > 0 && !frames) return fail(GT_ERR_INVALID_ARGUMENT, "null frames");
+    FrameSrc src;
+    src.dtype = dtype;
+    src.nframes = nframes;
+    src.host = static_cast<const char *>(frames);
+    return execute_frames(c, src, keys, nkeys, points, npoints, point_cols, points_has_frames,
+                          points_has_r, linker_reset_points, nreset, out);
+}
+
+gt_status gt_execute_file(gt_context *c, const gt_file *f, const uint64_t *keys, uint64_t nkeys,
+                          const float *points, uint64_t npoints, uint32_t point_cols,
+                          int32_t points_has_frames, int32_t points_has_r,
+                          const uint64_t *linker_reset_points, uint64_t nreset, gt_output *out) {
+    if (!c || !f || !out) return fail(GT_ERR_INVALID_ARGUMENT, "null argument");
+    out->data = nullptr;
+    out->nrows = 0;
+    out->ncols = 0;
+    uint32_t H, W;
+    gt_io_dims(f, &H, &W);
+    if (H != c->H || W != c->W)
+        return fail(GT_ERR_DIMENSION_MISMATCH,
+                    "the file's frames are %ux%u but the context was created for %ux%u", H, W,
+                    c->H, c->W);
+    FrameSrc src;
+    src.dtype = gt_io_dtype(f);
+    src.nframes = gt_io_nframes(f);
+    src.file = f;
+    return execute_frames(c, src, keys, nkeys, points, npoints, point_cols, points_has_frames,
+                          points_has_r, linker_reset_points, nreset, out);
+}
+
+gt_status gt_file_open(const char *path, int32_t has_channel, uint32_t channel, gt_file **out) {
+    if (!path || !out) return fail(GT_ERR_INVALID_ARGUMENT, "null argument");
+    std::string why;
+    const gt_status st = gt_io_open(path, has_channel, channel, out, &why);
+    if (st != GT_OK) return fail(st, "%s", why.c_str());
+    return GT_OK;
+}
+
+void gt_file_close(gt_file *f) { gt_io_close(f); }
+
+gt_status gt_file_info(const gt_file *f, uint32_t *nrows, uint32_t *ncols, uint64_t *nframes,
+                       int32_t *dtype) {
+    if (!f) return fail(GT_ERR_INVALID_ARGUMENT, "null file");
+    uint32_t H, W;
+    gt_io_dims(f, &H, &W);
+    if (nrows) *nrows = H;
+    if (ncols) *ncols = W;
+    if (nframes) *nframes = gt_io_nframes(f);
+    if (dtype) *dtype = gt_io_dtype(f);
+    return GT_OK;
+}
+
+gt_status gt_file_read(const gt_file *f, uint64_t first, uint64_t n, void *dst) {
+    if (!f || (n && !dst)) return fail(GT_ERR_INVALID_ARGUMENT, "null argument");
+    uint32_t H, W;
+    gt_io_dims(f, &H, &W);
+    const size_t bytes = dtype_size(gt_io_dtype(f)) * static_cast<size_t>(H) * W;
+    std::string why;
+    for (uint64_t i = 0; i < n; ++i) {
+        const gt_status st = gt_io_read(f, first + i, static_cast<char *>(dst) + bytes * i, &why);
+        if (st != GT_OK) return fail(st, "%s", why.c_str());
+    }
+    return GT_OK;
+}
+
+uint32_t gt_file_channels(const gt_file *f, uint32_t *channels, uint32_t cap) {
+    if (!f) return 0;
+    const std::vector<uint32_t> v = gt_io_channels(f);
+    for (size_t i = 0; i < v.size() && i < cap; ++i) channels[i] = v[i];
+    return static_cast<uint32_t>(v.size());
+}
+
+gt_status gt_file_mean(const gt_file *f, float *out) {
+    if (!f || !out) return fail(GT_ERR_INVALID_ARGUMENT, "null argument");
+    uint32_t H, W;
+    gt_io_dims(f, &H, &W);
+    const size_t HW = static_cast<size_t>(H) * W;
+    const int32_t dt = gt_io_dtype(f);
+    const size_t esz = dtype_size(dt);
+    const uint64_t n = gt_io_nframes(f);
+    std::vector<char> fr(esz * HW);
+    std::vector<float> wide(HW);
+    std::fill(out, out + HW, 0.0f);
+    std::string why;
+    for (uint64_t t = 0; t < n; ++t) {
+        const gt_status st = gt_io_read(f, t, fr.data(), &why);
+        if (st != GT_OK) return fail(st, "%s", why.c_str());
+        gt_host_widen(fr.data(), dt, HW, wide.data());
+        for (size_t i = 0; i < HW; ++i) out[i] += wide[i];
+    }
+    const float nf = static_cast<float>(n);
+    for (size_t i = 0; i < HW; ++i) out[i] /= nf;
+    return GT_OK;
+}
+
+gt_status gt_write_tiff(const char *path, const float *frames, uint64_t nframes, uint32_t nrows,
+                        uint32_t ncols) {
+    if (!path || (nframes && !frames)) return fail(GT_ERR_INVALID_ARGUMENT, "null argument");
+    std::string why;
+    const gt_status st = gt_io_write_tiff_u16(path, frames, nframes, nrows, ncols, &why);
+    if (st != GT_OK) return fail(st, "%s", why.c_str());
     return GT_OK;
 }
 

@@ -188,6 +188,52 @@ gt_status gt_execute(gt_context *ctx, const void *frames, int32_t dtype, uint64_
                      uint64_t nreset, gt_output *out);
 void gt_output_free(gt_output *out);
 
+/* ---- Video files -------------------------------------------------------
+ * path_to_iter (src/execute_gpu.rs:1186-1270): tif / tiff (multi-page; the
+ * reference decodes them with the tiff crate, src/decoderiter.rs:64-127),
+ * ets (MinimalETSParser / ETSIterator, src/decoderiter.rs:351-481) and vsi
+ * (the frames of x.vsi live in _x_/stack1/frame_t_0.ets beside it).  Frames
+ * decode into the caller's memory in their own sample type: u8 -> GT_U8,
+ * u16 -> GT_U16, f32 -> GT_F32, i8 / i16 widened to GT_F32 (the reference
+ * widens every type to f32 on the host, src/decoderiter.rs:16-19).
+ * TIFF: classic or BigTIFF, either byte order, strips or tiles, one sample
+ * per pixel, compression none / PackBits / LZW / Deflate, horizontal
+ * predictor; every page has page 0's size and type.  ETS: u16 data.
+ * Errors: GT_ERR_NO_EXTENSION, GT_ERR_UNSUPPORTED_FILEFORMAT,
+ * GT_ERR_FILE_NOT_FOUND, GT_ERR_CHANNEL_NOT_FOUND, GT_ERR_CAST (u32 / u64 /
+ * i32 / i64 / f64 samples), GT_ERR_READ, GT_ERR_DIMENSION_MISMATCH (pages of
+ * different sizes), GT_ERR_FRAME_OUT_OF_BOUNDS (gt_file_read past the end). */
+typedef struct gt_file gt_file;
+/* channel: the ETS channel when has_channel (default 0); ignored for TIFF. */
+gt_status gt_file_open(const char *path, int32_t has_channel, uint32_t channel, gt_file **out);
+void gt_file_close(gt_file *f);
+gt_status gt_file_info(const gt_file *f, uint32_t *nrows, uint32_t *ncols, uint64_t *nframes,
+                       int32_t *dtype);
+/* Frames first .. first+n-1 into dst: n * nrows * ncols samples of the
+ * file's dtype.  One file may be read from several threads at once. */
+gt_status gt_file_read(const gt_file *f, uint64_t first, uint64_t n, void *dst);
+/* The ETS channels of the file in ascending order (parse_ets,
+ * gpu_tracking_python/src/expansion.rs:849-867): returns their number and
+ * writes at most cap of them; 0 for TIFF. */
+uint32_t gt_file_channels(const gt_file *f, uint32_t *channels, uint32_t cap);
+/* mean_from_iter (src/execute_gpu.rs:819-847): per pixel, the f32 sum over
+ * the frames in order, divided by f32(nframes); out holds nrows * ncols. */
+gt_status gt_file_mean(const gt_file *f, float *out);
+/* execute_file (src/execute_gpu.rs:1354-1376): gt_execute over the file's
+ * frames, decoded by reader threads straight into the context's pinned
+ * staging while the previous chunk runs on the GPU (no f32 widening of
+ * integer frames).  The context must have the file's frame size. */
+gt_status gt_execute_file(gt_context *ctx, const gt_file *f, const uint64_t *keys, uint64_t nkeys,
+                          const float *points, uint64_t npoints, uint32_t point_cols,
+                          int32_t points_has_frames, int32_t points_has_r,
+                          const uint64_t *linker_reset_points, uint64_t nreset, gt_output *out);
+/* to_tiff (gpu_tracking_python/src/expansion.rs:925-950): one uncompressed
+ * 16-bit grey page per frame, values cast as Rust's `f32 as u16` (toward
+ * zero, saturating at 0 and 65535, NaN -> 0); width = ncols (the reference
+ * passes the row count as the width). */
+gt_status gt_write_tiff(const char *path, const float *frames, uint64_t nframes, uint32_t nrows,
+                        uint32_t ncols);
+
 /* Column names and types for a parameter set (column_names,
  * src/execute_gpu.rs:482-519).  Returns the column count; the arrays point
  * to static strings and hold at most 16 entries. */
