@@ -265,6 +265,10 @@ gt_status gt_batch_create(const gt_params *params, uint64_t max_frame_pixels,
                                "0 < max_slots <= 65535");
     if (params->style != GT_STYLE_TRACKPY)
         return gt_runtime_fail(GT_ERR_INVALID_ARGUMENT, "batched detection runs the Trackpy style");
+    if (params->has_illumination_sigma)
+        return gt_runtime_fail(GT_ERR_INVALID_ARGUMENT,
+                               "batched detection does not apply illumination correction (it "
+                               "is defined per video)");
     if (params->has_snr || params->has_minmass_snr || params->has_adaptive_background)
         return gt_runtime_fail(GT_ERR_INVALID_ARGUMENT,
                                "batched detection does not apply the snr / minmass_snr rules "

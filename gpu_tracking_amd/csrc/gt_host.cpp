@@ -79,8 +79,13 @@ bool gt_host_derive(const gt_params *p, uint32_t H, uint32_t W, gt_derived *d, s
         return false;
     }
     if (p->has_illumination_sigma) {
-        *why = "illumination correction is not available in this build";
-        return false;
+        // the smoothing's taps reach i32(4 sigma + 0.5) pixels each side
+        if (!(p->illumination_sigma > 0.0f) || !(p->illumination_sigma <= 65536.0f)) {
+            *why = "illumination_sigma must be positive and at most 65536";
+            return false;
+        }
+        d->illum_sigma = p->illumination_sigma;
+        d->illum_per_frame = p->illumination_correction_per_frame ? 1 : 0;
     }
     if (!(p->noise_size > 0.0f)) {
         *why = "noise_size must be positive";

@@ -29,6 +29,8 @@ struct gt_derived {
     float snr = 0.0f, minmass_snr = 0.0f, rough = 1.0f;
     int32_t need_std = 0;       // snr, minmass_snr or adaptive_background: noise estimate
     uint32_t adaptive = 0;      // adaptive_background passes (0: off)
+    float illum_sigma = 0.0f;   // illumination correction (0: off)
+    int32_t illum_per_frame = 0;
     int32_t truncate = 1;
     int32_t characterize = 0;
     int32_t filter_close = 1;

@@ -124,5 +124,20 @@ hipError_t gt_launch_adaptive(const void *frames, int dtype, uint32_t nframes, c
                               uint32_t n_iter, gt_detection *out, uint32_t cap, uint32_t *ocount,
                               float *std_tmp, float *mean, float *std2, float *masked,
                               uint8_t *state, uint32_t *counter, hipStream_t s);
+// Illumination correction (correct_illumination/*.wgsl).  acc[p] += f32 of
+// each frame's pixel p, frames in order (mean_from_iter's running sum).
+hipError_t gt_launch_frames_accumulate(const void *frames, int dtype, uint32_t nframes, size_t HW,
+                                       float *acc, hipStream_t s);
+// dst[i] = f32(frames[i]) for n pixels.
+hipError_t gt_launch_widen(const void *frames, int dtype, size_t n, float *dst, hipStream_t s);
+// x[i] = x[i] / d.
+hipError_t gt_launch_scale_div(float *x, size_t n, float d, hipStream_t s);
+// smooth_mean_frame's two passes over nframes frames: in -> tmp (along
+// columns) -> out (along rows); w holds 2 * rint + 1 taps.
+hipError_t gt_launch_illum_smooth(const float *in, float *tmp, float *out, int H, int W,
+                                  uint32_t nframes, const float *w, int rint, hipStream_t s);
+// out[t * HW + p] = f32(frames[t * HW + p]) / profile[t * pstride + p].
+hipError_t gt_launch_illum_divide(const void *frames, int dtype, uint32_t nframes, size_t HW,
+                                  const float *profile, size_t pstride, float *out, hipStream_t s);
 // Fills the constant tables the kernels read (once per device).
 hipError_t gt_init_tables();

@@ -2148,6 +2148,126 @@ hipError_t launch_adaptive_t(const T *frames, uint32_t nf, const GtKParams &p, u
 }
 }  // namespace
 
+// ---------------------------------------------------------------------------
+// Illumination correction.  The profile is the mean frame (mean_from_iter:
+// per pixel an f32 running sum in frame order, then / f32(T)) or the frame
+// itself, smoothed by smooth_mean_frame.wgsl's two passes; each frame is then
+// divided by it (correct_illumination.wgsl).  One lane per pixel; the taps
+// come from a device table.
+
+namespace {
+
+template <typename T>
+__global__ void k_frames_accumulate(const T *__restrict__ frames, uint32_t nframes, size_t HW,
+                                    float *__restrict__ acc) {
+    const size_t p = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (p >= HW) return;
+    float a = acc[p];
+    for (uint32_t t = 0; t < nframes; ++t) a = a + to_f32(frames[static_cast<size_t>(t) * HW + p]);
+    acc[p] = a;
+}
+
+__global__ void k_scale_div(float *__restrict__ x, size_t n, float d) {
+    for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+         i += static_cast<size_t>(gridDim.x) * blockDim.x)
+        x[i] = x[i] / d;
+}
+
+// One pass; dim 0 runs along a column (u = row), dim 1 along a row.  Taps
+// in order, out-of-frame taps skipped and left out of the norm.
+__global__ void k_illum_pass(const float *__restrict__ in, float *__restrict__ out, int H, int W,
+                             const float *__restrict__ w, int rint, int dim) {
+    const size_t HW = static_cast<size_t>(H) * W;
+    const size_t p = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (p >= HW) return;
+    const size_t fo = static_cast<size_t>(blockIdx.y) * HW;
+    const int r = static_cast<int>(p / W), c = static_cast<int>(p - static_cast<size_t>(r) * W);
+    const int u = dim == 0 ? r : c, bound = dim == 0 ? H : W;
+    const size_t stride = dim == 0 ? static_cast<size_t>(W) : 1;
+    const float *src = in + fo + p;
+    float gs = 0.0f, ns = 0.0f;
+    const int lo = max(-rint, -u), hi = min(rint, bound - 1 - u);
+    for (int i = lo; i <= hi; ++i) {
+        const float wi = w[i + rint];
+        gs = fmaf(wi, src[static_cast<long long>(i) * static_cast<long long>(stride)], gs);
+        ns = ns + wi;
+    }
+    out[fo + p] = gs / ns;
+}
+
+template <typename T>
+__global__ void k_illum_divide(const T *__restrict__ frames, uint32_t nframes, size_t HW,
+                               const float *__restrict__ profile, size_t pstride,
+                               float *__restrict__ out) {
+    const size_t n = HW * nframes;
+    for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+         i += static_cast<size_t>(gridDim.x) * blockDim.x) {
+        const size_t t = i / HW, pix = i - t * HW;
+        out[i] = to_f32(frames[i]) / profile[t * pstride + pix];
+    }
+}
+
+unsigned grid_for(size_t n, unsigned cap = 65535u * 4u) {
+    const size_t b = (n + 255) / 256;
+    return static_cast<unsigned>(b < 1 ? 1 : (b > cap ? cap : b));
+}
+
+}  // namespace
+
+hipError_t gt_launch_frames_accumulate(const void *frames, int dtype, uint32_t nframes, size_t HW,
+                                       float *acc, hipStream_t s) {
+    if (nframes == 0 || HW == 0) return hipSuccess;
+    const unsigned g = static_cast<unsigned>((HW + 255) / 256);
+    switch (dtype) {
+    case GT_U16: k_frames_accumulate<<<g, 256, 0, s>>>(static_cast<const uint16_t *>(frames), nframes, HW, acc); break;
+    case GT_F32: k_frames_accumulate<<<g, 256, 0, s>>>(static_cast<const float *>(frames), nframes, HW, acc); break;
+    case GT_U8: k_frames_accumulate<<<g, 256, 0, s>>>(static_cast<const uint8_t *>(frames), nframes, HW, acc); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t gt_launch_widen(const void *frames, int dtype, size_t n, float *dst, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    switch (dtype) {
+    case GT_U16: k_widen_frames<<<2048, 256, 0, s>>>(static_cast<const uint16_t *>(frames), n, dst); break;
+    case GT_F32: k_widen_frames<<<2048, 256, 0, s>>>(static_cast<const float *>(frames), n, dst); break;
+    case GT_U8: k_widen_frames<<<2048, 256, 0, s>>>(static_cast<const uint8_t *>(frames), n, dst); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t gt_launch_scale_div(float *x, size_t n, float d, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    k_scale_div<<<grid_for(n), 256, 0, s>>>(x, n, d);
+    return hipGetLastError();
+}
+
+hipError_t gt_launch_illum_smooth(const float *in, float *tmp, float *out, int H, int W,
+                                  uint32_t nframes, const float *w, int rint, hipStream_t s) {
+    const size_t HW = static_cast<size_t>(H) * W;
+    if (nframes == 0 || HW == 0) return hipSuccess;
+    const dim3 g(static_cast<unsigned>((HW + 255) / 256), nframes);
+    k_illum_pass<<<g, 256, 0, s>>>(in, tmp, H, W, w, rint, 0);
+    k_illum_pass<<<g, 256, 0, s>>>(tmp, out, H, W, w, rint, 1);
+    return hipGetLastError();
+}
+
+hipError_t gt_launch_illum_divide(const void *frames, int dtype, uint32_t nframes, size_t HW,
+                                  const float *profile, size_t pstride, float *out, hipStream_t s) {
+    const size_t n = HW * nframes;
+    if (n == 0) return hipSuccess;
+    const unsigned g = grid_for(n);
+    switch (dtype) {
+    case GT_U16: k_illum_divide<<<g, 256, 0, s>>>(static_cast<const uint16_t *>(frames), nframes, HW, profile, pstride, out); break;
+    case GT_F32: k_illum_divide<<<g, 256, 0, s>>>(static_cast<const float *>(frames), nframes, HW, profile, pstride, out); break;
+    case GT_U8: k_illum_divide<<<g, 256, 0, s>>>(static_cast<const uint8_t *>(frames), nframes, HW, profile, pstride, out); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
 hipError_t gt_launch_adaptive(const void *frames, int dtype, uint32_t nframes, const GtKParams &p,
                               uint32_t n_iter, gt_detection *out, uint32_t cap, uint32_t *ocount,
                               float *std_tmp, float *mean, float *std2, float *masked,

@@ -116,6 +116,16 @@ struct gt_context {
     uint8_t *d_adapt_state = nullptr;
     size_t adapt_state_cap = 0;
 
+    // illumination correction: the taps, the whole-video profile, the
+    // corrected f32 chunk the detection reads, and (per frame) the chunk's
+    // smoothing scratch and profiles
+    float *d_illum_w = nullptr;
+    int32_t illum_rint = 0;
+    float *d_profile = nullptr;     // HW (whole-video profile, or the running sum)
+    float *d_corr = nullptr;        // chunk * HW
+    float *d_illum_tmp = nullptr;   // chunk * HW
+    float *d_illum_prof = nullptr;  // chunk * HW (per frame)
+
     // gt_execute staging: two slots, so one chunk's detections come back
     // and are post-processed while the next chunk runs on the GPU
     struct Slot {
@@ -175,6 +185,25 @@ gt_status alloc_scratch(gt_context *c) {
         GT_HIP(hipMalloc(&c->d_adapt_masked, sizeof(float) * HW * c->chunk));
         GT_HIP(hipMalloc(&c->d_std2, sizeof(float) * c->chunk));
         GT_HIP(hipMalloc(&c->d_adapt_counter, sizeof(uint32_t) * c->chunk));
+    }
+    if (c->der.illum_sigma > 0.0f) {
+        const float sigma = c->der.illum_sigma;
+        const int32_t rint = static_cast<int32_t>(4.0f * sigma + 0.5f);
+        std::vector<float> w(2 * static_cast<size_t>(rint) + 1);
+        const float sigma2 = sigma * sigma;
+        for (int32_t i = -rint; i <= rint; ++i) {
+            const float x = static_cast<float>(i);
+            const float x2 = x * x;
+            w[i + rint] = std::exp(-x2 / (2.0f * sigma2));
+        }
+        c->illum_rint = rint;
+        GT_HIP(hipMalloc(&c->d_illum_w, sizeof(float) * w.size()));
+        GT_HIP(hipMemcpy(c->d_illum_w, w.data(), sizeof(float) * w.size(), hipMemcpyHostToDevice));
+        GT_HIP(hipMalloc(&c->d_profile, sizeof(float) * HW));
+        GT_HIP(hipMalloc(&c->d_corr, sizeof(float) * HW * c->chunk));
+        GT_HIP(hipMalloc(&c->d_illum_tmp, sizeof(float) * HW * c->chunk));
+        if (c->der.illum_per_frame)
+            GT_HIP(hipMalloc(&c->d_illum_prof, sizeof(float) * HW * c->chunk));
     }
     if (!c->tiled)
         GT_HIP(hipMalloc(&c->d_cand, sizeof(uint32_t) * static_cast<size_t>(c->cand_cap) * c->chunk));
@@ -247,6 +276,11 @@ void free_all(gt_context *c) {
     if (c->d_adapt_state) (void)hipFree(c->d_adapt_state);
     if (c->d_mean) (void)hipFree(c->d_mean);
     if (c->d_std) (void)hipFree(c->d_std);
+    if (c->d_illum_w) (void)hipFree(c->d_illum_w);
+    if (c->d_profile) (void)hipFree(c->d_profile);
+    if (c->d_corr) (void)hipFree(c->d_corr);
+    if (c->d_illum_tmp) (void)hipFree(c->d_illum_tmp);
+    if (c->d_illum_prof) (void)hipFree(c->d_illum_prof);
     for (auto &sl : c->slot) {
         if (sl.d_frames) (void)hipFree(sl.d_frames);
         if (sl.d_rows) (void)hipFree(sl.d_rows);
@@ -284,6 +318,61 @@ gt_status run_adaptive(gt_context *c, const void *frames, int32_t dtype, uint32_
 }
 
 // Runs the three pipeline stages over `nframes` device frames in chunks.
+// The whole-video illumination profile: start (zero the running sum), add
+// frames in order, finish (divide by the frame count, then smooth into
+// d_profile).  setup_illumination_profile, src/execute_gpu.rs:849-884, over
+// mean_from_iter (:819-847).
+gt_status illum_begin(gt_context *c, hipStream_t s) {
+    GT_HIP(gt_launch_zero_u32(reinterpret_cast<uint32_t *>(c->d_illum_tmp),
+                              static_cast<size_t>(c->H) * c->W, s));
+    return GT_OK;
+}
+gt_status illum_add(gt_context *c, const void *frames, int32_t dtype, uint32_t nf, hipStream_t s) {
+    GT_HIP(gt_launch_frames_accumulate(frames, dtype, nf, static_cast<size_t>(c->H) * c->W,
+                                       c->d_illum_tmp, s));
+    return GT_OK;
+}
+gt_status illum_finish(gt_context *c, uint64_t nframes, hipStream_t s) {
+    const size_t HW = static_cast<size_t>(c->H) * c->W;
+    GT_HIP(gt_launch_scale_div(c->d_illum_tmp, HW, static_cast<float>(nframes), s));
+    // the second pass's input (tmp) and output may not overlap the first's
+    GT_HIP(gt_launch_illum_smooth(c->d_illum_tmp, c->d_corr, c->d_profile, static_cast<int>(c->H),
+                                  static_cast<int>(c->W), 1, c->d_illum_w, c->illum_rint, s));
+    return GT_OK;
+}
+// The whole-video profile over device frames (gt_detect_device).
+gt_status illum_profile_device(gt_context *c, const void *d_frames, int32_t dtype,
+                               uint32_t nframes, hipStream_t s) {
+    const size_t HW = static_cast<size_t>(c->H) * c->W;
+    gt_status st = illum_begin(c, s);
+    if (st != GT_OK) return st;
+    const size_t esz = dtype_size(dtype);
+    for (uint32_t f0 = 0; f0 < nframes; f0 += c->chunk) {
+        const uint32_t nf = std::min(c->chunk, nframes - f0);
+        st = illum_add(c, static_cast<const char *>(d_frames) + esz * HW * f0, dtype, nf, s);
+        if (st != GT_OK) return st;
+    }
+    return illum_finish(c, nframes, s);
+}
+// The chunk's frames divided by the profile (per frame: by their own
+// smoothed copies) into d_corr, which the detection then reads as f32
+// (submit_work, src/execute_gpu.rs:138-166).
+gt_status illum_correct(gt_context *c, const void *frames, int32_t dtype, uint32_t nf,
+                        hipStream_t s) {
+    const size_t HW = static_cast<size_t>(c->H) * c->W;
+    if (c->der.illum_per_frame) {
+        // f32 copies of the frames, then their smoothed profiles
+        GT_HIP(gt_launch_widen(frames, dtype, HW * nf, c->d_corr, s));
+        GT_HIP(gt_launch_illum_smooth(c->d_corr, c->d_illum_tmp, c->d_illum_prof,
+                                      static_cast<int>(c->H), static_cast<int>(c->W), nf,
+                                      c->d_illum_w, c->illum_rint, s));
+        GT_HIP(gt_launch_illum_divide(frames, dtype, nf, HW, c->d_illum_prof, HW, c->d_corr, s));
+    } else {
+        GT_HIP(gt_launch_illum_divide(frames, dtype, nf, HW, c->d_profile, 0, c->d_corr, s));
+    }
+    return GT_OK;
+}
+
 gt_status run_pipeline(gt_context *c, const void *d_frames, int32_t dtype, uint32_t nframes,
                        gt_detection *d_out, uint32_t cap, uint32_t *d_counts, hipStream_t s) {
     const size_t HW = static_cast<size_t>(c->H) * c->W;
@@ -305,22 +394,29 @@ gt_status run_pipeline(gt_context *c, const void *d_frames, int32_t dtype, uint3
         const uint32_t f0 = ci * c->chunk;
         const uint32_t nf = std::min(c->chunk, nframes - f0);
         const char *frames = static_cast<const char *>(d_frames) + esz * HW * f0;
+        int32_t fdtype = dtype;
+        if (c->der.illum_sigma > 0.0f) {
+            gt_status ist = illum_correct(c, frames, dtype, nf, s);
+            if (ist != GT_OK) return ist;
+            frames = reinterpret_cast<const char *>(c->d_corr);
+            fdtype = GT_F32;
+        }
         hipEvent_t *ev = c->timing ? &c->events[4 * ci] : nullptr;
         GT_HIP(gt_launch_zero_u32(c->d_ocount, static_cast<size_t>(nf) * GT_COUNTER_STRIDE, s));
         if (!c->tiled || c->log)
             GT_HIP(gt_launch_zero_u32(c->d_ccount, static_cast<size_t>(nf) * GT_COUNTER_STRIDE, s));
         if (ev) GT_HIP(hipEventRecord(ev[0], s));
         if (c->der.need_std)
-            GT_HIP(gt_launch_frame_std(frames, dtype, nf, HW, c->d_std_tmp, c->d_mean, c->d_std, s));
-        GT_HIP(gt_launch_bandpass(frames, dtype, c->d_proc, nf, c->kp, s));
+            GT_HIP(gt_launch_frame_std(frames, fdtype, nf, HW, c->d_std_tmp, c->d_mean, c->d_std, s));
+        GT_HIP(gt_launch_bandpass(frames, fdtype, c->d_proc, nf, c->kp, s));
         if (ev) GT_HIP(hipEventRecord(ev[1], s));
         gt_detection *out = d_out + static_cast<size_t>(f0) * cap;
         if (c->log) {
             // scale-space maxima of all scales, then their walks
-            GT_HIP(gt_launch_log_tile(frames, dtype, c->d_proc, nf, c->kp, c->d_logtab, c->log_Rm,
+            GT_HIP(gt_launch_log_tile(frames, fdtype, c->d_proc, nf, c->kp, c->d_logtab, c->log_Rm,
                                       c->d_cand, c->cand_cap, c->d_ccount, s));
             if (ev) GT_HIP(hipEventRecord(ev[2], s));
-            GT_HIP(gt_launch_walk_log(c->d_proc, frames, dtype, nf, c->kp, c->d_logtab, c->d_cand,
+            GT_HIP(gt_launch_walk_log(c->d_proc, frames, fdtype, nf, c->kp, c->d_logtab, c->d_cand,
                                       c->cand_cap, c->d_ccount, out, cap, c->d_ocount, f0, s));
             GT_HIP(gt_launch_copy_counts_checked(c->d_ocount, c->d_ccount, c->cand_cap,
                                                  d_counts + f0, nf, s));
@@ -335,7 +431,7 @@ gt_status run_pipeline(gt_context *c, const void *d_frames, int32_t dtype, uint3
             GtKParams kw = c->kp;
             kw.characterize = 0;
             GT_HIP(gt_launch_zero_u32(c->d_scount, static_cast<size_t>(nf) * GT_COUNTER_STRIDE, s));
-            GT_HIP(gt_launch_detect_tile(c->d_proc, frames, dtype, nf, kw, out, cap, c->d_ocount,
+            GT_HIP(gt_launch_detect_tile(c->d_proc, frames, fdtype, nf, kw, out, cap, c->d_ocount,
                                          c->d_slow, c->slow_cap, c->d_scount, f0, s, 1));
             if (ev) GT_HIP(hipEventRecord(ev[2], s));
             const uint32_t *list = reinterpret_cast<const uint32_t *>(c->d_slow);
@@ -346,7 +442,7 @@ gt_status run_pipeline(gt_context *c, const void *d_frames, int32_t dtype, uint3
                                           c->d_props, c->cnn_conf, c->cnn_tol, out, cap,
                                           c->d_ocount, c->d_stats, f0, s));
             if (c->kp.characterize)
-                GT_HIP(gt_launch_characterize_det(c->d_proc, frames, dtype, nf, c->kp, out, cap,
+                GT_HIP(gt_launch_characterize_det(c->d_proc, frames, fdtype, nf, c->kp, out, cap,
                                                   c->d_ocount, s));
         } else if (c->tiled && c->cnn) {
             // every candidate: the network's proposals for all of them, one
@@ -354,7 +450,7 @@ gt_status run_pipeline(gt_context *c, const void *d_frames, int32_t dtype, uint3
             GtKParams kw = c->kp;
             kw.characterize = 0;
             GT_HIP(gt_launch_zero_u32(c->d_ccount, static_cast<size_t>(nf) * GT_COUNTER_STRIDE, s));
-            GT_HIP(gt_launch_detect_cand(c->d_proc, dtype, nf, kw, c->d_cand, c->cand_cap,
+            GT_HIP(gt_launch_detect_cand(c->d_proc, fdtype, nf, kw, c->d_cand, c->cand_cap,
                                          c->d_ccount, s));
             if (ev) GT_HIP(hipEventRecord(ev[2], s));
             GT_HIP(gt_cnn_launch_forward_frames(gt_cnn_packed(c->cnn), c->d_proc,
@@ -365,7 +461,7 @@ gt_status run_pipeline(gt_context *c, const void *d_frames, int32_t dtype, uint3
                                           c->d_ccount, c->d_props, c->cnn_conf, c->cnn_tol, out,
                                           cap, c->d_ocount, c->d_stats, f0, s));
             if (c->kp.characterize)
-                GT_HIP(gt_launch_characterize_det(c->d_proc, frames, dtype, nf, c->kp, out, cap,
+                GT_HIP(gt_launch_characterize_det(c->d_proc, frames, fdtype, nf, c->kp, out, cap,
                                                   c->d_ocount, s));
         } else if (c->tiled) {
             // local max and walk in one kernel, the walks it hands over, then
@@ -374,25 +470,25 @@ gt_status run_pipeline(gt_context *c, const void *d_frames, int32_t dtype, uint3
             GtKParams kw = c->kp;
             kw.characterize = 0;
             GT_HIP(gt_launch_zero_u32(c->d_scount, static_cast<size_t>(nf) * GT_COUNTER_STRIDE, s));
-            GT_HIP(gt_launch_detect_tile(c->d_proc, frames, dtype, nf, kw, out, cap,
+            GT_HIP(gt_launch_detect_tile(c->d_proc, frames, fdtype, nf, kw, out, cap,
                                          c->d_ocount, c->d_slow, c->slow_cap, c->d_scount, f0, s));
             if (ev) GT_HIP(hipEventRecord(ev[2], s));
-            GT_HIP(gt_launch_walk_slow(c->d_proc, frames, dtype, nf, kw, c->d_slow, c->slow_cap,
+            GT_HIP(gt_launch_walk_slow(c->d_proc, frames, fdtype, nf, kw, c->d_slow, c->slow_cap,
                                        c->d_scount, out, cap, c->d_ocount, f0, s));
             if (c->der.adaptive) {
-                gt_status ast = run_adaptive(c, frames, dtype, nf, out, cap, s);
+                gt_status ast = run_adaptive(c, frames, fdtype, nf, out, cap, s);
                 if (ast != GT_OK) return ast;
             }
             if (c->kp.characterize)
-                GT_HIP(gt_launch_characterize_det(c->d_proc, frames, dtype, nf, c->kp, out, cap,
+                GT_HIP(gt_launch_characterize_det(c->d_proc, frames, fdtype, nf, c->kp, out, cap,
                                                   c->d_ocount, s));
         } else {
             GT_HIP(gt_launch_localmax(c->d_proc, nf, c->kp, c->d_cand, c->cand_cap, c->d_ccount, s));
             if (ev) GT_HIP(hipEventRecord(ev[2], s));
-            GT_HIP(gt_launch_walk(c->d_proc, frames, dtype, nf, c->kp, c->d_cand, c->cand_cap,
+            GT_HIP(gt_launch_walk(c->d_proc, frames, fdtype, nf, c->kp, c->d_cand, c->cand_cap,
                                   c->d_ccount, out, cap, c->d_ocount, f0, s));
             if (c->der.adaptive) {
-                gt_status ast = run_adaptive(c, frames, dtype, nf, out, cap, s);
+                gt_status ast = run_adaptive(c, frames, fdtype, nf, out, cap, s);
                 if (ast != GT_OK) return ast;
             }
         }
@@ -545,6 +641,11 @@ gt_status gt_detect_device(gt_context *c, const void *d_frames, int32_t dtype, u
     if (!d_frames || !d_out || !d_counts || cap_per_frame == 0)
         return fail(GT_ERR_INVALID_ARGUMENT, "null device buffer or zero capacity");
     hipStream_t s = static_cast<hipStream_t>(stream);  // NULL: the default stream
+    if (c->der.illum_sigma > 0.0f && !c->der.illum_per_frame) {
+        // the profile of this call's frames (the video it is given)
+        gt_status ist = illum_profile_device(c, d_frames, dtype, nframes, s);
+        if (ist != GT_OK) return ist;
+    }
     return run_pipeline(c, d_frames, dtype, nframes, d_out, cap_per_frame, d_counts, s);
 }
 
@@ -711,6 +812,61 @@ gt_status execute_frames(gt_context *c, const FrameSrc &src, const uint64_t *key
 
     const size_t nplan = plan.frames.size();
     const size_t nchunks = (nplan + c->chunk - 1) / c->chunk;
+    // stage frames idx[0..nf) into slot sl's pinned buffer: copies (host
+    // array) or decodes (file), frame f by thread f % nt
+    auto stage = [&](gt_context::Slot &sl, const uint64_t *idx, uint32_t nf) -> gt_status {
+        const unsigned nt = std::min<unsigned>(io_threads(), nf);
+        std::vector<gt_status> sts(nt, GT_OK);
+        std::vector<std::string> whys(nt);
+        std::vector<std::thread> th;
+        for (unsigned t = 0; t < nt; ++t)
+            th.emplace_back([&, t]() {
+                for (uint32_t f = t; f < nf; f += nt) {
+                    char *dst = static_cast<char *>(sl.h_stage) + esz * HW * f;
+                    if (src.host) {
+                        std::memcpy(dst, src.host + esz * HW * idx[f], esz * HW);
+                        continue;
+                    }
+                    const gt_status rs = gt_io_read(src.file, idx[f], dst, &whys[t]);
+                    if (rs != GT_OK) {
+                        sts[t] = rs;
+                        return;
+                    }
+                }
+            });
+        for (auto &t : th) t.join();
+        for (unsigned t = 0; t < nt; ++t)
+            if (sts[t] != GT_OK) return fail(sts[t], "%s", whys[t].c_str());
+        return GT_OK;
+    };
+    if (c->der.illum_sigma > 0.0f && !c->der.illum_per_frame) {
+        // the whole video's mean frame, smoothed, before any detection
+        // (execute_provider, src/execute_gpu.rs:1324-1330): every frame of the
+        // source, whatever the keys
+        auto profile = [&]() -> gt_status {
+            gt_status ist = illum_begin(c, c->stream);
+            std::vector<uint64_t> idx(c->chunk);
+            for (uint64_t f0 = 0; f0 < nframes && ist == GT_OK; f0 += c->chunk) {
+                const uint32_t nf = static_cast<uint32_t>(std::min<uint64_t>(c->chunk, nframes - f0));
+                for (uint32_t f = 0; f < nf; ++f) idx[f] = f0 + f;
+                auto &sl = c->slot[0];
+                GT_HIP(hipStreamSynchronize(c->stream));   // the slot's last copy is done
+                ist = stage(sl, idx.data(), nf);
+                if (ist != GT_OK) return ist;
+                GT_HIP(hipMemcpyAsync(sl.d_frames, sl.h_stage, esz * HW * nf,
+                                      hipMemcpyHostToDevice, c->stream));
+                ist = illum_add(c, sl.d_frames, dtype, nf, c->stream);
+            }
+            if (ist == GT_OK) ist = illum_finish(c, nframes, c->stream);
+            GT_HIP(hipStreamSynchronize(c->stream));
+            return ist;
+        };
+        const gt_status ist = profile();
+        if (ist != GT_OK) {
+            finish_worker();
+            return ist;
+        }
+    }
     // stage chunk k in slot k % 2 and queue its GPU work
     auto submit = [&](size_t k) -> gt_status {
         auto &sl = c->slot[k % 2];
@@ -724,31 +880,9 @@ gt_status execute_frames(gt_context *c, const FrameSrc &src, const uint64_t *key
                             static_cast<unsigned long long>(nframes), static_cast<unsigned long long>(idx));
         }
         {
-            // frames are copied (host array) or decoded (file) straight into
-            // the pinned staging slot, frame f by thread f % nt (the slot's
-            // previous chunk has been drained)
-            const unsigned nt = std::min<unsigned>(io_threads(), nf);
-            std::vector<gt_status> sts(nt, GT_OK);
-            std::vector<std::string> whys(nt);
-            std::vector<std::thread> th;
-            for (unsigned t = 0; t < nt; ++t)
-                th.emplace_back([&, t]() {
-                    for (uint32_t f = t; f < nf; f += nt) {
-                        char *dst = static_cast<char *>(sl.h_stage) + esz * HW * f;
-                        if (src.host) {
-                            std::memcpy(dst, src.host + esz * HW * plan.frames[base + f], esz * HW);
-                            continue;
-                        }
-                        const gt_status rs = gt_io_read(src.file, plan.frames[base + f], dst, &whys[t]);
-                        if (rs != GT_OK) {
-                            sts[t] = rs;
-                            return;
-                        }
-                    }
-                });
-            for (auto &t : th) t.join();
-            for (unsigned t = 0; t < nt; ++t)
-                if (sts[t] != GT_OK) return fail(sts[t], "%s", whys[t].c_str());
+            // the slot's previous chunk has been drained
+            gt_status sst = stage(sl, plan.frames.data() + base, nf);
+            if (sst != GT_OK) return sst;
         }
         GT_HIP(hipMemcpyAsync(sl.d_frames, sl.h_stage, esz * HW * nf, hipMemcpyHostToDevice, c->stream));
         gt_status st2;
@@ -1117,8 +1251,15 @@ gt_status gt_host_characterize_points(gt_context *c, int slot, size_t base, uint
     std::memcpy(sl.h_counts, counts.data(), sizeof(uint32_t) * nf);
     GT_HIP(hipMemcpyAsync(sl.d_counts, sl.h_counts, sizeof(uint32_t) * nf, hipMemcpyHostToDevice,
                           c->stream));
-    GT_HIP(gt_launch_bandpass(sl.d_frames, dtype, c->d_proc, nf, c->kp, c->stream));
-    GT_HIP(gt_launch_characterize_rows(c->d_proc, sl.d_frames, dtype, nf, c->kp, sl.d_rows,
+    const void *frames = sl.d_frames;
+    if (c->der.illum_sigma > 0.0f) {
+        gt_status ist = illum_correct(c, frames, dtype, nf, c->stream);
+        if (ist != GT_OK) return ist;
+        frames = c->d_corr;
+        dtype = GT_F32;
+    }
+    GT_HIP(gt_launch_bandpass(frames, dtype, c->d_proc, nf, c->kp, c->stream));
+    GT_HIP(gt_launch_characterize_rows(c->d_proc, frames, dtype, nf, c->kp, sl.d_rows,
                                        c->det_cap, sl.d_counts, max_count, c->stream));
     return GT_OK;
 }

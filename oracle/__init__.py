@@ -98,6 +98,12 @@ def lib():
         L.or_detect_frame_adaptive.argtypes = [C.c_void_p, P(Params), C.c_uint32, C.c_int,
                                                C.c_void_p, C.c_size_t]
         L.or_detect_frame_adaptive.restype = C.c_size_t
+        L.or_illum_taps.argtypes = [C.c_float, C.c_void_p]
+        L.or_illum_taps.restype = C.c_int32
+        L.or_illum_smooth.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_float, C.c_void_p,
+                                      C.c_void_p]
+        L.or_frames_mean.argtypes = [C.c_void_p, C.c_size_t, C.c_size_t, C.c_void_p]
+        L.or_illum_divide.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p]
         _lib = L
     return _lib
 
@@ -363,3 +369,43 @@ def detect_frame_adaptive(frame, p: Params, n_iter: int, characterize: bool = Tr
     n = lib().or_detect_frame_adaptive(_ptr(f), C.byref(p), int(n_iter), int(characterize),
                                        _ptr(rows), rows.size)
     return rows[:n].copy()
+
+
+def illum_taps(sigma: float) -> np.ndarray:
+    """The illumination smoothing's taps (w[i + rint], i = -rint..rint)."""
+    rint = int(np.float32(4.0) * np.float32(sigma) + np.float32(0.5))
+    w = np.zeros(2 * rint + 1, np.float32)
+    r = lib().or_illum_taps(C.c_float(sigma), _ptr(w))
+    assert r == rint
+    return w
+
+
+def illum_smooth(frame, sigma: float) -> np.ndarray:
+    """smooth_mean_frame's two passes over one f32 frame."""
+    f = np.ascontiguousarray(frame, dtype=np.float32)
+    H, W = f.shape
+    out = np.zeros_like(f)
+    tmp = np.zeros_like(f)
+    lib().or_illum_smooth(_ptr(f), H, W, C.c_float(sigma), _ptr(out), _ptr(tmp))
+    return out
+
+
+def frames_mean(frames) -> np.ndarray:
+    """mean_from_iter: the per-pixel sequential f32 mean over frames."""
+    f = np.ascontiguousarray(frames, dtype=np.float32)
+    out = np.zeros(f.shape[1:], np.float32)
+    lib().or_frames_mean(_ptr(f), f.shape[0], out.size, _ptr(out))
+    return out
+
+
+def illum_correct(frames, sigma: float, per_frame: bool = False) -> np.ndarray:
+    """The frames the detection sees with illumination correction: each raw
+    frame (as f32) divided by the smoothed mean frame of the whole video, or
+    by its own smoothed copy when per_frame."""
+    f = np.ascontiguousarray(frames, dtype=np.float32)
+    out = np.empty_like(f)
+    prof = None if per_frame else illum_smooth(frames_mean(f), sigma)
+    for t in range(f.shape[0]):
+        pr = illum_smooth(f[t], sigma) if per_frame else prof
+        lib().or_illum_divide(_ptr(f[t]), _ptr(pr), f[t].size, _ptr(out[t]))
+    return out

@@ -905,3 +905,60 @@ size_t or_detect_frame_adaptive(const float *frame, const or_params *p_in, uint3
     free(proc);
     return out;
 }
+
+/* ---- illumination correction -------------------------------------------- */
+
+int32_t or_illum_taps(float sigma, float *w) {
+    const int32_t rint = (int32_t)(4.0f * sigma + 0.5f);
+    const float sigma2 = sigma * sigma;
+    for (int32_t i = -rint; i <= rint; i++) {
+        const float x = (float)i;
+        const float x2 = x * x;
+        w[i + rint] = expf(-x2 / (2.0f * sigma2));
+    }
+    return rint;
+}
+
+/* one pass of smooth_mean_frame.wgsl:38-72; dim 0 runs along a column (u is
+ * the row), dim 1 along a row (u is the column) */
+static void or_illum_pass(const float *in, float *out, int32_t H, int32_t W, const float *w,
+                          int32_t rint, int dim) {
+    for (int32_t r = 0; r < H; r++) {
+        for (int32_t c = 0; c < W; c++) {
+            const int32_t u = dim == 0 ? r : c;
+            const int32_t bound = dim == 0 ? H : W;
+            float gs = 0.0f, ns = 0.0f;
+            for (int32_t i = -rint; i <= rint; i++) {
+                const int32_t pu = u + i;
+                if (pu < 0 || pu >= bound) continue;
+                const float x = dim == 0 ? in[(size_t)pu * W + c] : in[(size_t)r * W + pu];
+                gs = fmaf(w[i + rint], x, gs);
+                ns = ns + w[i + rint];
+            }
+            float norm = ns;
+            if (dim == 0) norm = norm * 1.0f;     /* n_frames = 1 */
+            out[(size_t)r * W + c] = gs / norm;
+        }
+    }
+}
+
+void or_illum_smooth(const float *in, int32_t H, int32_t W, float sigma, float *out, float *tmp) {
+    const int32_t rmax = (int32_t)(4.0f * sigma + 0.5f);
+    float *w = (float *)malloc(sizeof(float) * (size_t)(2 * rmax + 1));
+    const int32_t rint = or_illum_taps(sigma, w);
+    or_illum_pass(in, tmp, H, W, w, rint, 0);
+    or_illum_pass(tmp, out, H, W, w, rint, 1);
+    free(w);
+}
+
+void or_frames_mean(const float *frames, size_t T, size_t n, float *out) {
+    for (size_t i = 0; i < n; i++) out[i] = 0.0f;
+    for (size_t t = 0; t < T; t++)
+        for (size_t i = 0; i < n; i++) out[i] = out[i] + frames[t * n + i];
+    const float nf = (float)T;
+    for (size_t i = 0; i < n; i++) out[i] = out[i] / nf;
+}
+
+void or_illum_divide(const float *frame, const float *profile, size_t n, float *out) {
+    for (size_t i = 0; i < n; i++) out[i] = frame[i] / profile[i];
+}

@@ -170,6 +170,21 @@ void or_frame_std_div(const float *frame, size_t n, uint32_t div_mean, uint32_t 
  * the unmarked frame. */
 size_t or_detect_frame_adaptive(const float *frame, const or_params *p, uint32_t n_iter,
                                 int characterize, or_row *rows, size_t cap);
+/* Illumination correction (src/shaders/correct_illumination/*.wgsl;
+ * setup_illumination_profile, src/execute_gpu.rs:849-884; submit_work,
+ * :138-166).  Taps: rint = i32(4 sigma + 0.5), w[i + rint] =
+ * exp(-(i*i) / (2 (sigma*sigma))) in f32; returns rint. */
+int32_t or_illum_taps(float sigma, float *w);
+/* smooth_mean_frame.wgsl run as SeparableConvolution<2> with n_frames = 1:
+ * pass 0 along columns (rows index u) into tmp, pass 1 along rows into out;
+ * per output, taps in order, out-of-frame taps skipped and left out of the
+ * norm: fmaf(w, x, sum), norm += w, result sum / norm. */
+void or_illum_smooth(const float *in, int32_t H, int32_t W, float sigma, float *out, float *tmp);
+/* mean_from_iter (src/execute_gpu.rs:819-847): per pixel, the f32 sum over
+ * frames in order, then / f32(T). */
+void or_frames_mean(const float *frames, size_t T, size_t n, float *out);
+/* correct_illumination.wgsl: out[i] = frame[i] / profile[i]. */
+void or_illum_divide(const float *frame, const float *profile, size_t n, float *out);
 void or_bandpass_f64(const float *frame, const or_params *p, double *processed, double *scratch);
 /* FP64 centroid of the disk at integer position (pu, pv) of a f64 frame. */
 void or_centroid_f64(const double *processed, const or_params *p, int32_t pu, int32_t pv,
