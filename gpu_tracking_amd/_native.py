@@ -95,7 +95,7 @@ EXPORTS = [
     "gt_cnn_get_params", "gt_cnn_adam_steps", "gt_cnn_forward", "gt_cnn_gradient", "gt_cnn_step",
     "gt_cnn_train", "gt_cnn_make_samples", "gt_context_set_cnn", "gt_context_cnn_stats",
     "gt_file_open", "gt_file_close", "gt_file_info", "gt_file_read", "gt_file_channels",
-    "gt_file_mean", "gt_execute_file", "gt_write_tiff",
+    "gt_file_mean", "gt_execute_file", "gt_write_tiff", "gt_batch_set_lanes", "gt_batch_get_lanes",
 ]
 
 CNN_NPARAM = 14147
@@ -194,6 +194,7 @@ def lib() -> C.CDLL:
         "gt_execute_file": [vp, vp, vp, u64, vp, u64, u32, C.c_int32, C.c_int32, vp, u64,
                             P(GtOutput)],
         "gt_write_tiff": [C.c_char_p, vp, u64, u32, u32],
+        "gt_batch_set_lanes": [vp, u32],
     }
     for name, args in sigs.items():
         fn = getattr(L, name)
@@ -205,6 +206,8 @@ def lib() -> C.CDLL:
     L.gt_file_close.restype = None
     L.gt_file_channels.argtypes = [vp, vp, u32]
     L.gt_file_channels.restype = u32
+    L.gt_batch_get_lanes.argtypes = [vp]
+    L.gt_batch_get_lanes.restype = u32
     _lib = L
     return L
 

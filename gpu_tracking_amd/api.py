@@ -663,8 +663,8 @@ class Batch:
         return out, counts
 
     def tune(self, frames, cap_per_frame: int = 8192) -> float:
-        """Autotunes the schedule (pixels per launch, slow-walk grid) on sample
-        frames; returns the best time (ms) of one pass over them."""
+        """Autotunes the schedule (pixels per launch, slow-walk grid, lanes) on
+        sample frames; returns the best time (ms) of one pass over them."""
         import torch
         descs, code = self._descs(frames, None)
         n = len(frames)
@@ -681,10 +681,15 @@ class Batch:
         launches = C.c_uint64()
         N.check(self._lib.gt_batch_get_schedule(self._h, a, C.byref(launches)))
         return {"pixels_per_launch": a[0], "walk_blocks": a[1], "max_frames_per_launch": a[2],
-                "max_pixels_per_launch": a[3], "launches_last_call": launches.value}
+                "max_pixels_per_launch": a[3], "launches_last_call": launches.value,
+                "lanes": int(self._lib.gt_batch_get_lanes(self._h))}
 
     def set_graph(self, enable: bool) -> None:
         N.check(self._lib.gt_batch_set_graph(self._h, int(bool(enable))))
+
+    def set_lanes(self, lanes: int) -> None:
+        """Streams (1..4) that consecutive launches are dealt to."""
+        N.check(self._lib.gt_batch_set_lanes(self._h, int(lanes)))
 
 
 class CentroidNet:

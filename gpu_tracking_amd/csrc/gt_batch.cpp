@@ -9,8 +9,10 @@
 // a kernel that selects the next launch's table on the device and one that
 // advances a device launch counter, is captured once into a HIP graph (again
 // only when the output buffers or grid sizes change) and replayed once per
-// launch with no host work in between.  gt_batch_tune picks the launch size
-// and the slow-walk / characterize grid by timing candidates.
+// launch with no host work in between.  Launches are dealt to up to four
+// lanes (stream, scratch and graph each) so that consecutive launches
+// overlap.  gt_batch_tune picks the launch size, the slow-walk /
+// characterize grid and the lane count by timing candidates.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -34,35 +36,24 @@ gt_status gt_runtime_fail(gt_status st, const char *fmt, ...);
             return gt_runtime_fail(GT_ERR_GPU_DEVICE, "%s failed: %s", #call, hipGetErrorString(e_)); \
     } while (0)
 
-struct gt_batch {
-    gt_params params;
-    gt_derived der;
-    GtKParams kp;
+// One lane: a stream with its own scratch, launch list and captured graph.
+// A call's launches are dealt to the lanes in turn (launch k to lane k % n),
+// so consecutive launches overlap as separate contexts on separate streams
+// do; each frame's rows and count still go to its own output slots.
+struct BatchLane {
     hipStream_t stream = nullptr;
-    uint64_t max_frame_pixels = 0, max_launch_pixels = 0;
-    uint32_t max_slots = 0;
     float *d_proc = nullptr;          // max_launch_pixels
     uint32_t *d_ocount = nullptr;     // max_slots * GT_COUNTER_STRIDE
     uint32_t *d_scount = nullptr;
     uint2 *d_slow = nullptr;          // max_slots * slow_cap
-    uint32_t slow_cap = 0;
     GtBatchHeader *d_hdr = nullptr;   // the running launch's table ...
     GtBatchFrame *d_fs = nullptr;     // ... max_slots entries
-    // a call's launch tables: all entries, the launches, the launch counter
-    GtBatchFrame *d_all = nullptr;
-    GtBatchLaunch *d_launches = nullptr;
-    uint32_t *d_cur = nullptr;
-    size_t all_cap = 0, launch_cap = 0;
-    char *h_stage = nullptr;          // pinned staging of the tables (+ the counter's zero)
-    size_t h_cap = 0;
-    hipEvent_t stage_free = nullptr;  // the last upload from h_stage has completed
-    hipEvent_t ev_in = nullptr, ev_out = nullptr;  // ordering against the caller's stream
-    uint32_t walk_blocks = 4;         // blocks per frame for slow walks and characterize
-    uint64_t launch_px = 0;           // pixels per launch in use (<= max_launch_pixels)
-    int32_t use_graph = 1;
-    uint64_t launches = 0;            // launches of the last call
-    // the captured stage sequence, kept while the call's arguments repeat
-    hipGraphExec_t gexec = nullptr;
+    GtBatchLaunch *d_launches = nullptr;  // this lane's launches of the call
+    size_t launch_cap = 0;
+    uint32_t *d_cur = nullptr;        // device launch counter
+    hipEvent_t done = nullptr;
+    // captured stage sequences, one per grid shape, kept while the call's
+    // arguments repeat
     struct GraphKey {
         gt_detection *out;
         uint32_t cap;
@@ -77,38 +68,90 @@ struct gt_batch {
                    dt_blocks == o.dt_blocks && walk_blocks == o.walk_blocks &&
                    launch_slots == o.launch_slots;
         }
-    } gkey{};
+    };
+    std::vector<std::pair<GraphKey, hipGraphExec_t>> graphs;
+};
+
+constexpr uint32_t kMaxLanes = 4;
+
+struct gt_batch {
+    gt_params params;
+    gt_derived der;
+    GtKParams kp;
+    hipStream_t stream = nullptr;     // uploads, fork and join of the lanes
+    uint64_t max_frame_pixels = 0, max_launch_pixels = 0;
+    uint32_t max_slots = 0;
+    uint32_t slow_cap = 0;
+    BatchLane lanes[kMaxLanes];
+    uint32_t nlanes = 1;              // lanes in use
+    // a call's launch tables: all entries (shared by the lanes)
+    GtBatchFrame *d_all = nullptr;
+    size_t all_cap = 0;
+    char *h_stage = nullptr;          // pinned staging of the tables (+ the counters' zeros)
+    size_t h_cap = 0;
+    hipEvent_t stage_free = nullptr;  // the last upload from h_stage has completed
+    hipEvent_t uploaded = nullptr;    // the call's tables are on the device
+    hipEvent_t ev_in = nullptr, ev_out = nullptr;  // ordering against the caller's stream
+    uint32_t walk_blocks = 4;         // blocks per frame for slow walks and characterize
+    uint64_t launch_px = 0;           // pixels per launch in use (<= max_launch_pixels)
+    int32_t use_graph = 1;
+    uint64_t launches = 0;            // launches of the last call
 };
 
 namespace {
 
+void free_lane(BatchLane &L) {
+    if (L.d_proc) (void)hipFree(L.d_proc);
+    if (L.d_ocount) (void)hipFree(L.d_ocount);
+    if (L.d_scount) (void)hipFree(L.d_scount);
+    if (L.d_slow) (void)hipFree(L.d_slow);
+    if (L.d_hdr) (void)hipFree(L.d_hdr);
+    if (L.d_fs) (void)hipFree(L.d_fs);
+    if (L.d_launches) (void)hipFree(L.d_launches);
+    if (L.d_cur) (void)hipFree(L.d_cur);
+    for (auto &kg : L.graphs) (void)hipGraphExecDestroy(kg.second);
+    if (L.done) (void)hipEventDestroy(L.done);
+    if (L.stream) (void)hipStreamDestroy(L.stream);
+    L = BatchLane();
+}
+
 void free_batch(gt_batch *b) {
-    if (b->d_proc) (void)hipFree(b->d_proc);
-    if (b->d_ocount) (void)hipFree(b->d_ocount);
-    if (b->d_scount) (void)hipFree(b->d_scount);
-    if (b->d_slow) (void)hipFree(b->d_slow);
-    if (b->d_hdr) (void)hipFree(b->d_hdr);
-    if (b->d_fs) (void)hipFree(b->d_fs);
+    for (BatchLane &L : b->lanes) free_lane(L);
     if (b->d_all) (void)hipFree(b->d_all);
-    if (b->d_launches) (void)hipFree(b->d_launches);
-    if (b->d_cur) (void)hipFree(b->d_cur);
     if (b->h_stage) (void)hipHostFree(b->h_stage);
-    if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
     if (b->stage_free) (void)hipEventDestroy(b->stage_free);
+    if (b->uploaded) (void)hipEventDestroy(b->uploaded);
     if (b->ev_in) (void)hipEventDestroy(b->ev_in);
     if (b->ev_out) (void)hipEventDestroy(b->ev_out);
     if (b->stream) (void)hipStreamDestroy(b->stream);
 }
 
-// Splits frames[0..n) into launches: consecutive frames while the frame
-// limit and the pixel budget last.  Returns the end index of the launch
+// Allocates lane l's stream and scratch on first use.
+gt_status alloc_lane(gt_batch *b, uint32_t l) {
+    BatchLane &L = b->lanes[l];
+    if (L.stream) return GT_OK;
+    GB_HIP(hipStreamCreateWithFlags(&L.stream, hipStreamNonBlocking));
+    GB_HIP(hipEventCreateWithFlags(&L.done, hipEventDisableTiming));
+    GB_HIP(hipMalloc(&L.d_proc, sizeof(float) * b->max_launch_pixels));
+    GB_HIP(hipMalloc(&L.d_ocount, sizeof(uint32_t) * GT_COUNTER_STRIDE * b->max_slots));
+    GB_HIP(hipMalloc(&L.d_scount, sizeof(uint32_t) * GT_COUNTER_STRIDE * b->max_slots));
+    GB_HIP(hipMalloc(&L.d_slow, sizeof(uint2) * static_cast<size_t>(b->slow_cap) * b->max_slots));
+    GB_HIP(hipMalloc(&L.d_hdr, sizeof(GtBatchHeader)));
+    GB_HIP(hipMalloc(&L.d_fs, sizeof(GtBatchFrame) * b->max_slots));
+    GB_HIP(hipMalloc(&L.d_cur, sizeof(uint32_t)));
+    return GT_OK;
+}
+
+// Splits frames[order[0..n)] into launches: consecutive frames while the
+// frame limit and the pixel budget last.  Returns the end index of the launch
 // starting at i.
-uint32_t launch_end(const gt_batch *b, const gt_frame_desc *frames, uint32_t n, uint32_t i,
-                    uint64_t launch_px) {
+uint32_t launch_end(const gt_batch *b, const gt_frame_desc *frames, const uint32_t *order,
+                    uint32_t n, uint32_t i, uint64_t launch_px) {
     uint64_t px = 0;
     uint32_t j = i;
     while (j < n && j - i < b->max_slots) {
-        const uint64_t hw = static_cast<uint64_t>(frames[j].nrows) * frames[j].ncols;
+        const gt_frame_desc &fj = frames[order[j]];
+        const uint64_t hw = static_cast<uint64_t>(fj.nrows) * fj.ncols;
         if (j > i && px + hw > launch_px) break;
         px += hw;
         ++j;
@@ -131,19 +174,41 @@ gt_status grow(T **p, size_t *cap, size_t need) {
 
 gt_status run_batch(gt_batch *b, const gt_frame_desc *frames, uint32_t n, int32_t dtype,
                     gt_detection *d_out, uint32_t cap, uint32_t *d_counts, hipStream_t s,
-                    uint32_t walk_blocks, uint64_t launch_px, bool graph) {
+                    uint32_t walk_blocks, uint64_t launch_px, bool graph, uint32_t nlanes) {
+    nlanes = std::max(1u, std::min(nlanes, kMaxLanes));
+    // the call's frames by size (stable), so that a launch holds frames of
+    // one size where it can and its grids fit them; results go to each
+    // frame's own output slot, so the order does not show
+    std::vector<uint32_t> order(n);
+    for (uint32_t i = 0; i < n; ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t c) {
+        const uint64_t pa = static_cast<uint64_t>(frames[a].nrows) * frames[a].ncols;
+        const uint64_t pc = static_cast<uint64_t>(frames[c].nrows) * frames[c].ncols;
+        if (pa != pc) return pa < pc;
+        return frames[a].ncols < frames[c].ncols;
+    });
     // the call's launches and their tables
     std::vector<GtBatchLaunch> launches;
     for (uint32_t i = 0; i < n;) {
-        const uint32_t j = launch_end(b, frames, n, i, launch_px);
+        const uint32_t j = launch_end(b, frames, order.data(), n, i, launch_px);
         launches.push_back(GtBatchLaunch{i, j - i});
         i = j;
     }
     const size_t nl = launches.size();
+    nlanes = static_cast<uint32_t>(std::min<size_t>(nlanes, nl));
+    for (uint32_t l = 0; l < nlanes; ++l) {
+        gt_status ast = alloc_lane(b, l);
+        if (ast != GT_OK) return ast;
+    }
+    // launch k goes to lane k % nlanes; lane l's list starts at lane_off[l]
+    std::vector<size_t> lane_n(nlanes, 0), lane_off(nlanes + 1, 0);
+    for (size_t k = 0; k < nl; ++k) ++lane_n[k % nlanes];
+    for (uint32_t l = 0; l < nlanes; ++l) lane_off[l + 1] = lane_off[l] + lane_n[l];
     // the pinned staging buffer and the device tables may still serve the
     // previous call's upload and launches
     GB_HIP(hipEventSynchronize(b->stage_free));
-    const size_t bytes = sizeof(GtBatchFrame) * n + sizeof(GtBatchLaunch) * nl + sizeof(uint32_t);
+    const size_t bytes = sizeof(GtBatchFrame) * n + sizeof(GtBatchLaunch) * nl +
+                         sizeof(uint32_t) * kMaxLanes;
     if (bytes > b->h_cap) {
         const size_t c = std::max(bytes, 2 * b->h_cap);
         if (b->h_stage) (void)hipHostFree(b->h_stage);
@@ -152,19 +217,28 @@ gt_status run_batch(gt_batch *b, const gt_frame_desc *frames, uint32_t n, int32_
         GB_HIP(hipHostMalloc(&b->h_stage, c, hipHostMallocDefault));
         b->h_cap = c;
     }
-    if (n > b->all_cap || nl > b->launch_cap) GB_HIP(hipStreamSynchronize(s));
+    bool regrow = n > b->all_cap;
+    for (uint32_t l = 0; l < nlanes; ++l) regrow = regrow || lane_n[l] > b->lanes[l].launch_cap;
+    // the lanes' previous work was joined into s: when s is idle, so are they
+    if (regrow) GB_HIP(hipStreamSynchronize(s));
     gt_status st = grow(&b->d_all, &b->all_cap, n);
-    if (st == GT_OK) st = grow(&b->d_launches, &b->launch_cap, nl);
+    for (uint32_t l = 0; l < nlanes && st == GT_OK; ++l)
+        st = grow(&b->lanes[l].d_launches, &b->lanes[l].launch_cap, lane_n[l]);
     if (st != GT_OK) return st;
     GtBatchFrame *e = reinterpret_cast<GtBatchFrame *>(b->h_stage);
     GtBatchLaunch *hl = reinterpret_cast<GtBatchLaunch *>(b->h_stage + sizeof(GtBatchFrame) * n);
     uint32_t *zero = reinterpret_cast<uint32_t *>(b->h_stage + sizeof(GtBatchFrame) * n +
                                                   sizeof(GtBatchLaunch) * nl);
-    uint32_t maxbp = 1, maxdt = 1, maxslots = 1;
-    for (const GtBatchLaunch &L : launches) {
+    // each launch's grids: x = tiles of its largest frame (multiple of 8),
+    // y = its frames
+    std::vector<GtBatchSchedule> lsch(nl);
+    for (size_t k = 0; k < nl; ++k) {
+        const GtBatchLaunch &L = launches[k];
         uint64_t off = 0;
-        for (uint32_t f = L.first; f < L.first + L.nslots; ++f) {
-            GtBatchFrame &x = e[f];
+        uint32_t maxbp = 1, maxdt = 1;
+        for (uint32_t q = L.first; q < L.first + L.nslots; ++q) {
+            const uint32_t f = order[q];
+            GtBatchFrame &x = e[q];
             x.in_ptr = reinterpret_cast<uint64_t>(frames[f].data);
             x.proc_off = off;
             x.H = static_cast<int32_t>(frames[f].nrows);
@@ -177,57 +251,84 @@ gt_status run_batch(gt_batch *b, const gt_frame_desc *frames, uint32_t n, int32_
             maxbp = std::max(maxbp, tb);
             maxdt = std::max(maxdt, td);
         }
-        maxslots = std::max(maxslots, L.nslots);
+        lsch[k].bp_blocks = (maxbp + 7) / 8 * 8;
+        lsch[k].dt_blocks = (maxdt + 7) / 8 * 8;
+        lsch[k].walk_blocks = walk_blocks;
+        lsch[k].launch_slots = L.nslots;
     }
-    std::copy(launches.begin(), launches.end(), hl);
-    *zero = 0;
+    std::vector<size_t> fill(lane_off.begin(), lane_off.end() - 1);
+    for (size_t k = 0; k < nl; ++k) hl[fill[k % nlanes]++] = launches[k];
+    for (uint32_t l = 0; l < kMaxLanes; ++l) zero[l] = 0;
     GB_HIP(hipMemcpyAsync(b->d_all, e, sizeof(GtBatchFrame) * n, hipMemcpyHostToDevice, s));
-    GB_HIP(hipMemcpyAsync(b->d_launches, hl, sizeof(GtBatchLaunch) * nl, hipMemcpyHostToDevice, s));
-    GB_HIP(hipMemcpyAsync(b->d_cur, zero, sizeof(uint32_t), hipMemcpyHostToDevice, s));
-    // grids: x = tiles of the call's largest frame, y = frames of its largest launch
-    GtBatchSchedule sch;
-    sch.bp_blocks = (maxbp + 7) / 8 * 8;
-    sch.dt_blocks = (maxdt + 7) / 8 * 8;
-    sch.walk_blocks = walk_blocks;
-    sch.launch_slots = maxslots;
-    auto enqueue = [&]() {
-        return gt_launch_batch(b->d_launches, b->d_cur, b->d_all, dtype, b->kp, b->d_hdr, b->d_fs,
-                               b->max_slots, b->d_proc, d_out, cap, b->d_ocount, b->d_slow,
-                               b->slow_cap, b->d_scount, d_counts, sch, s);
-    };
-    hipGraphExec_t gexec = nullptr;
-    const gt_batch::GraphKey key{d_out,           cap,           d_counts,        dtype,
-                                 b->d_all,        b->d_launches, sch.bp_blocks,   sch.dt_blocks,
-                                 sch.walk_blocks, sch.launch_slots};
-    if (graph && b->gexec && b->gkey == key) {
-        gexec = b->gexec;
-    } else if (graph) {
-        if (b->gexec) {
-            GB_HIP(hipStreamSynchronize(s));   // the old graph may still run
-            (void)hipGraphExecDestroy(b->gexec);
-            b->gexec = nullptr;
-        }
-        hipGraph_t g = nullptr;
-        GB_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-        hipError_t le = enqueue();
-        hipError_t ce = hipStreamEndCapture(s, &g);
-        if (le != hipSuccess || ce != hipSuccess) {
-            if (g) (void)hipGraphDestroy(g);
-            return gt_runtime_fail(GT_ERR_GPU_DEVICE, "graph capture of the batch stages failed: %s",
-                                   hipGetErrorString(le != hipSuccess ? le : ce));
-        }
-        hipError_t ie = hipGraphInstantiate(&gexec, g, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(g);
-        if (ie != hipSuccess)
-            return gt_runtime_fail(GT_ERR_GPU_DEVICE, "hipGraphInstantiate failed: %s",
-                                   hipGetErrorString(ie));
-        b->gexec = gexec;
-        b->gkey = key;
+    for (uint32_t l = 0; l < nlanes; ++l) {
+        GB_HIP(hipMemcpyAsync(b->lanes[l].d_launches, hl + lane_off[l],
+                              sizeof(GtBatchLaunch) * lane_n[l], hipMemcpyHostToDevice, s));
+        GB_HIP(hipMemcpyAsync(b->lanes[l].d_cur, zero + l, sizeof(uint32_t), hipMemcpyHostToDevice,
+                              s));
     }
-    for (size_t k = 0; k < nl; ++k) {
-        const hipError_t e1 = gexec ? hipGraphLaunch(gexec, s) : enqueue();
-        if (e1 != hipSuccess)
-            return gt_runtime_fail(GT_ERR_GPU_DEVICE, "batch launch failed: %s", hipGetErrorString(e1));
+    GB_HIP(hipEventRecord(b->uploaded, s));
+    for (uint32_t l = 0; l < nlanes; ++l) {
+        BatchLane &L = b->lanes[l];
+        hipStream_t ls = L.stream;
+        GB_HIP(hipStreamWaitEvent(ls, b->uploaded, 0));
+        // graphs of a previous call with other arguments are dropped (after
+        // the lane's earlier work, which may still replay them)
+        const BatchLane::GraphKey probe{d_out, cap, d_counts, dtype, b->d_all, L.d_launches,
+                                        0, 0, 0, 0};
+        bool stale = false;
+        for (auto &kg : L.graphs)
+            stale = stale || kg.first.out != probe.out || kg.first.cap != probe.cap ||
+                    kg.first.counts != probe.counts || kg.first.dtype != probe.dtype ||
+                    kg.first.all != probe.all || kg.first.launches != probe.launches ||
+                    kg.first.walk_blocks != walk_blocks;
+        if (stale) {
+            GB_HIP(hipStreamSynchronize(ls));
+            for (auto &kg : L.graphs) (void)hipGraphExecDestroy(kg.second);
+            L.graphs.clear();
+        }
+        for (size_t q = lane_off[l]; q < lane_off[l + 1]; ++q) {
+            // the launch this lane runs q-th: launch k with k % nlanes == l
+            const size_t k = (q - lane_off[l]) * nlanes + l;
+            const GtBatchSchedule &sch = lsch[k];
+            auto enqueue = [&]() {
+                return gt_launch_batch(L.d_launches, L.d_cur, b->d_all, dtype, b->kp, L.d_hdr,
+                                       L.d_fs, b->max_slots, L.d_proc, d_out, cap, L.d_ocount,
+                                       L.d_slow, b->slow_cap, L.d_scount, d_counts, sch, ls);
+            };
+            hipGraphExec_t gexec = nullptr;
+            if (graph) {
+                const BatchLane::GraphKey key{d_out,           cap,          d_counts,
+                                              dtype,           b->d_all,     L.d_launches,
+                                              sch.bp_blocks,   sch.dt_blocks, sch.walk_blocks,
+                                              sch.launch_slots};
+                for (auto &kg : L.graphs)
+                    if (kg.first == key) gexec = kg.second;
+                if (!gexec) {
+                    hipGraph_t g = nullptr;
+                    GB_HIP(hipStreamBeginCapture(ls, hipStreamCaptureModeThreadLocal));
+                    hipError_t le = enqueue();
+                    hipError_t ce = hipStreamEndCapture(ls, &g);
+                    if (le != hipSuccess || ce != hipSuccess) {
+                        if (g) (void)hipGraphDestroy(g);
+                        return gt_runtime_fail(GT_ERR_GPU_DEVICE,
+                                               "graph capture of the batch stages failed: %s",
+                                               hipGetErrorString(le != hipSuccess ? le : ce));
+                    }
+                    hipError_t ie = hipGraphInstantiate(&gexec, g, nullptr, nullptr, 0);
+                    (void)hipGraphDestroy(g);
+                    if (ie != hipSuccess)
+                        return gt_runtime_fail(GT_ERR_GPU_DEVICE, "hipGraphInstantiate failed: %s",
+                                               hipGetErrorString(ie));
+                    L.graphs.emplace_back(key, gexec);
+                }
+            }
+            const hipError_t e1 = gexec ? hipGraphLaunch(gexec, ls) : enqueue();
+            if (e1 != hipSuccess)
+                return gt_runtime_fail(GT_ERR_GPU_DEVICE, "batch launch failed: %s",
+                                       hipGetErrorString(e1));
+        }
+        GB_HIP(hipEventRecord(L.done, ls));
+        GB_HIP(hipStreamWaitEvent(s, L.done, 0));
     }
     // the staging buffer and tables are free again once this call's launches are done
     GB_HIP(hipEventRecord(b->stage_free, s));
@@ -305,17 +406,12 @@ gt_status gt_batch_create(const gt_params *params, uint64_t max_frame_pixels,
     };
     (void)(ok(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking), "hipStreamCreate") &&
            ok(gt_init_tables(), "table init") &&
-           ok(hipMalloc(&b->d_proc, sizeof(float) * max_launch_pixels), "hipMalloc") &&
-           ok(hipMalloc(&b->d_ocount, sizeof(uint32_t) * GT_COUNTER_STRIDE * max_slots), "hipMalloc") &&
-           ok(hipMalloc(&b->d_scount, sizeof(uint32_t) * GT_COUNTER_STRIDE * max_slots), "hipMalloc") &&
-           ok(hipMalloc(&b->d_slow, sizeof(uint2) * static_cast<size_t>(b->slow_cap) * max_slots), "hipMalloc") &&
-           ok(hipMalloc(&b->d_hdr, sizeof(GtBatchHeader)), "hipMalloc") &&
-           ok(hipMalloc(&b->d_fs, sizeof(GtBatchFrame) * max_slots), "hipMalloc") &&
-           ok(hipMalloc(&b->d_cur, sizeof(uint32_t)), "hipMalloc") &&
+           ok(hipEventCreateWithFlags(&b->uploaded, hipEventDisableTiming), "hipEventCreate") &&
            ok(hipEventCreateWithFlags(&b->stage_free, hipEventDisableTiming), "hipEventCreate") &&
            ok(hipEventRecord(b->stage_free, b->stream), "hipEventRecord") &&
            ok(hipEventCreateWithFlags(&b->ev_in, hipEventDisableTiming), "hipEventCreate") &&
            ok(hipEventCreateWithFlags(&b->ev_out, hipEventDisableTiming), "hipEventCreate"));
+    if (st == GT_OK) st = alloc_lane(b, 0);
     if (st != GT_OK) {
         free_batch(b);
         delete b;
@@ -328,6 +424,8 @@ gt_status gt_batch_create(const gt_params *params, uint64_t max_frame_pixels,
 void gt_batch_destroy(gt_batch *b) {
     if (!b) return;
     (void)hipStreamSynchronize(b->stream);
+    for (BatchLane &L : b->lanes)
+        if (L.stream) (void)hipStreamSynchronize(L.stream);
     free_batch(b);
     delete b;
 }
@@ -350,7 +448,7 @@ gt_status gt_batch_detect(gt_batch *b, const gt_frame_desc *frames, uint32_t n, 
         GB_HIP(hipStreamWaitEvent(b->stream, b->ev_in, 0));
     }
     st = run_batch(b, frames, n, dtype, d_out, cap_per_frame, d_counts, b->stream, b->walk_blocks,
-                   b->launch_px, b->use_graph != 0);
+                   b->launch_px, b->use_graph != 0, b->nlanes);
     if (st != GT_OK) return st;
     if (user != b->stream) {
         GB_HIP(hipEventRecord(b->ev_out, b->stream));
@@ -364,6 +462,16 @@ gt_status gt_batch_set_graph(gt_batch *b, int32_t enable) {
     b->use_graph = enable != 0;
     return GT_OK;
 }
+
+gt_status gt_batch_set_lanes(gt_batch *b, uint32_t nlanes) {
+    if (!b) return gt_runtime_fail(GT_ERR_INVALID_ARGUMENT, "null batch context");
+    if (nlanes < 1 || nlanes > kMaxLanes)
+        return gt_runtime_fail(GT_ERR_INVALID_ARGUMENT, "lanes must be between 1 and %u", kMaxLanes);
+    b->nlanes = nlanes;
+    return GT_OK;
+}
+
+uint32_t gt_batch_get_lanes(const gt_batch *b) { return b ? b->nlanes : 0; }
 
 gt_status gt_batch_get_schedule(gt_batch *b, uint64_t *sched4, uint64_t *launches) {
     if (!b || !sched4) return gt_runtime_fail(GT_ERR_INVALID_ARGUMENT, "null argument");
@@ -379,9 +487,9 @@ gt_status gt_batch_tune(gt_batch *b, const gt_frame_desc *frames, uint32_t n, in
                         gt_detection *d_out, uint32_t cap_per_frame, uint32_t *d_counts,
                         float *best_ms) {
     // Coordinate search: first the pixels per launch, then the slow-walk /
-    // characterize blocks per frame; each candidate runs the sample once to
-    // warm up and once timed with events; the fastest value of each knob is
-    // kept.
+    // characterize blocks per frame, then the lanes; each candidate runs the
+    // sample once to warm up and once timed with events; the fastest value of
+    // each knob is kept.
     if (!b) return gt_runtime_fail(GT_ERR_INVALID_ARGUMENT, "null batch context");
     if (n == 0 || !frames || !d_out || !d_counts || cap_per_frame == 0)
         return gt_runtime_fail(GT_ERR_INVALID_ARGUMENT, "empty sample or null argument");
@@ -393,13 +501,14 @@ gt_status gt_batch_tune(gt_batch *b, const gt_frame_desc *frames, uint32_t n, in
     hipEvent_t e0, e1;
     GB_HIP(hipEventCreate(&e0));
     GB_HIP(hipEventCreate(&e1));
+    uint32_t lanes = 1;
     auto time_it = [&](uint32_t wb, uint64_t lpx, float *ms) -> gt_status {
         gt_status s1 = run_batch(b, frames, n, dtype, d_out, cap_per_frame, d_counts, b->stream, wb,
-                                 lpx, false);
+                                 lpx, false, lanes);
         if (s1 != GT_OK) return s1;
         GB_HIP(hipEventRecord(e0, b->stream));
         s1 = run_batch(b, frames, n, dtype, d_out, cap_per_frame, d_counts, b->stream, wb, lpx,
-                       false);
+                       false, lanes);
         if (s1 != GT_OK) return s1;
         GB_HIP(hipEventRecord(e1, b->stream));
         GB_HIP(hipEventSynchronize(e1));
@@ -429,11 +538,23 @@ gt_status gt_batch_tune(gt_batch *b, const gt_frame_desc *frames, uint32_t n, in
             wb = walks[o];
         }
     }
+    const uint32_t lane_opts[] = {2, 4};
+    uint32_t best_lanes = 1;
+    for (int o = 0; o < 2 && st == GT_OK; ++o) {
+        lanes = lane_opts[o];
+        float ms = 0.0f;
+        st = time_it(wb, lpx, &ms);
+        if (st == GT_OK && ms < best) {
+            best = ms;
+            best_lanes = lanes;
+        }
+    }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (st != GT_OK) return st;
     b->walk_blocks = wb;
     b->launch_px = lpx;
+    b->nlanes = best_lanes;
     if (best_ms) *best_ms = best;
     return GT_OK;
 }

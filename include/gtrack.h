@@ -299,9 +299,9 @@ gt_status gt_batch_detect(gt_batch *b, const gt_frame_desc *frames, uint32_t n, 
                           gt_detection *d_out, uint32_t cap_per_frame, uint32_t *d_counts,
                           void *stream);
 /* Autotuner: times candidate schedules on the sample frames (pixels per
- * launch, then the slow-walk / characterize grid; coordinate search, warm-up
- * + one timed run each) and keeps the fastest for later calls; *best_ms
- * receives its time. */
+ * launch, then the slow-walk / characterize grid, then the lanes; coordinate
+ * search, warm-up + one timed run each) and keeps the fastest for later
+ * calls; *best_ms receives its time. */
 gt_status gt_batch_tune(gt_batch *b, const gt_frame_desc *frames, uint32_t n, int32_t dtype,
                         gt_detection *d_out, uint32_t cap_per_frame, uint32_t *d_counts,
                         float *best_ms);
@@ -311,6 +311,11 @@ gt_status gt_batch_tune(gt_batch *b, const gt_frame_desc *frames, uint32_t n, in
 gt_status gt_batch_get_schedule(gt_batch *b, uint64_t *sched4, uint64_t *launches);
 /* Graph replay on (default) or direct launches. */
 gt_status gt_batch_set_graph(gt_batch *b, int32_t enable);
+/* Lanes (1..4): launch k of a call runs on lane k % lanes, each lane a stream
+ * with its own scratch and graph, so consecutive launches overlap; results do
+ * not depend on the lane count.  Default 1; gt_batch_tune also picks it. */
+gt_status gt_batch_set_lanes(gt_batch *b, uint32_t lanes);
+uint32_t gt_batch_get_lanes(const gt_batch *b);
 
 /* Interruption and progress (the Arc<AtomicBool> / Arc<Mutex<(usize,
  * Option<usize>)>> pair of execute_gpu, src/execute_gpu.rs:1094-1099,

@@ -81,6 +81,26 @@ def test_batch_matches_separate_runs(torch_dev, graph, launch_px, slots):
             assert b.schedule()["launches_last_call"] > 1
 
 
+@pytest.mark.parametrize("lanes,graph", [(2, True), (3, True), (4, False)])
+def test_batch_lanes_match_separate_runs(torch_dev, lanes, graph):
+    # launches dealt to several streams; repeated calls replay each lane's graph
+    import gpu_tracking_amd as gt
+    vids = _videos(torch_dev)
+    cap = 2048
+    sep = _separate(torch_dev, vids, cap)
+    frames, ids, where = _frames(vids)
+    with gt.Batch(gt.make_params(9, **PARAMS), max_frame_pixels=256 * 256,
+                  max_launch_pixels=100_000, max_slots=4) as b:
+        b.set_graph(graph)
+        b.set_lanes(lanes)
+        for rep in range(2):
+            out, counts = b.detect(frames, frame_ids=ids, cap_per_frame=cap)
+            torch_dev.cuda.synchronize()
+            _check(out, counts, sep, where)
+        sch = b.schedule()
+        assert sch["lanes"] == lanes and sch["launches_last_call"] > lanes
+
+
 def test_batch_graph_replays_stay_exact(torch_dev):
     # several calls (each capturing and replaying a graph) with eager work in
     # between; every replay must equal the separate runs
