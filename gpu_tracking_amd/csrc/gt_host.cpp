@@ -62,10 +62,6 @@ bool gt_host_derive(const gt_params *p, uint32_t H, uint32_t W, gt_derived *d, s
         return false;
     }
     if (p->has_adaptive_background) {
-        if (p->style != GT_STYLE_TRACKPY) {
-            *why = "adaptive_background runs in the Trackpy style only in this build";
-            return false;
-        }
         if (p->adaptive_background == 0) {
             // execute_gpu.rs:299-315 unwraps the first of zero passes
             *why = "adaptive_background needs at least one pass";

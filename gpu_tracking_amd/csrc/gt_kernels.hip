@@ -2061,7 +2061,8 @@ __global__ __launch_bounds__(256) void k_adapt_filter(
             continue;
         }
         st[i] = 1;
-        const int rint = p.rint;
+        // LoG rows carry their radius (set_nan, adaptive_filter.wgsl:56-63)
+        const int rint = row.r > 0.0f ? f2i(ceilf(row.r)) : p.rint;
         const int u = f2i(rintf(row.x)), v = f2i(rintf(row.y));
         const float r = static_cast<float>(rint), r2 = r * r;
         for (int a = -rint; a <= rint; ++a) {

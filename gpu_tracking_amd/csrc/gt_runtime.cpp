@@ -317,7 +317,6 @@ gt_status run_adaptive(gt_context *c, const void *frames, int32_t dtype, uint32_
     return GT_OK;
 }
 
-// Runs the three pipeline stages over `nframes` device frames in chunks.
 // The whole-video illumination profile: start (zero the running sum), add
 // frames in order, finish (divide by the frame count, then smooth into
 // d_profile).  setup_illumination_profile, src/execute_gpu.rs:849-884, over
@@ -373,6 +372,7 @@ gt_status illum_correct(gt_context *c, const void *frames, int32_t dtype, uint32
     return GT_OK;
 }
 
+// Runs the three pipeline stages over `nframes` device frames in chunks.
 gt_status run_pipeline(gt_context *c, const void *d_frames, int32_t dtype, uint32_t nframes,
                        gt_detection *d_out, uint32_t cap, uint32_t *d_counts, hipStream_t s) {
     const size_t HW = static_cast<size_t>(c->H) * c->W;
@@ -412,12 +412,22 @@ gt_status run_pipeline(gt_context *c, const void *d_frames, int32_t dtype, uint3
         if (ev) GT_HIP(hipEventRecord(ev[1], s));
         gt_detection *out = d_out + static_cast<size_t>(f0) * cap;
         if (c->log) {
-            // scale-space maxima of all scales, then their walks
+            // scale-space maxima of all scales, then their walks (and, with
+            // adaptive_background, the acceptance passes before characterize)
             GT_HIP(gt_launch_log_tile(frames, fdtype, c->d_proc, nf, c->kp, c->d_logtab, c->log_Rm,
                                       c->d_cand, c->cand_cap, c->d_ccount, s));
             if (ev) GT_HIP(hipEventRecord(ev[2], s));
-            GT_HIP(gt_launch_walk_log(c->d_proc, frames, fdtype, nf, c->kp, c->d_logtab, c->d_cand,
+            GtKParams kw = c->kp;
+            if (c->der.adaptive) kw.characterize = 0;
+            GT_HIP(gt_launch_walk_log(c->d_proc, frames, fdtype, nf, kw, c->d_logtab, c->d_cand,
                                       c->cand_cap, c->d_ccount, out, cap, c->d_ocount, f0, s));
+            if (c->der.adaptive) {
+                gt_status ast = run_adaptive(c, frames, fdtype, nf, out, cap, s);
+                if (ast != GT_OK) return ast;
+                if (c->kp.characterize)
+                    GT_HIP(gt_launch_characterize_det(c->d_proc, frames, fdtype, nf, c->kp, out,
+                                                      cap, c->d_ocount, s));
+            }
             GT_HIP(gt_launch_copy_counts_checked(c->d_ocount, c->d_ccount, c->cand_cap,
                                                  d_counts + f0, nf, s));
             if (ev) GT_HIP(hipEventRecord(ev[3], s));

@@ -98,6 +98,10 @@ def lib():
         L.or_detect_frame_adaptive.argtypes = [C.c_void_p, P(Params), C.c_uint32, C.c_int,
                                                C.c_void_p, C.c_size_t]
         L.or_detect_frame_adaptive.restype = C.c_size_t
+        L.or_log_detect_frame_adaptive.argtypes = [C.c_void_p, P(Params), C.c_void_p,
+                                                   C.c_uint32, C.c_uint32, C.c_int, C.c_float,
+                                                   C.c_void_p, C.c_size_t]
+        L.or_log_detect_frame_adaptive.restype = C.c_size_t
         L.or_illum_taps.argtypes = [C.c_float, C.c_void_p]
         L.or_illum_taps.restype = C.c_int32
         L.or_illum_smooth.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_float, C.c_void_p,
@@ -338,6 +342,20 @@ def log_detect_frame(frame, p: Params, radii, characterize: bool = True,
     n = lib().or_log_detect_frame(_ptr(f), C.byref(p), _ptr(rr), len(rr), int(characterize),
                                   overlap_threshold, _ptr(rows), cap, C.byref(nc))
     return rows[:n].copy(), nc.value
+
+
+def log_detect_frame_adaptive(frame, p: Params, radii, n_iter: int, characterize: bool = True,
+                              overlap_threshold: float = 0.0):
+    """LoG frame with adaptive_background = n_iter (p from derive_log with
+    snr / minmass_snr / minmass set and rough_snr_factor 0.5): accepted rows,
+    characterized and pruned."""
+    f = _f32(frame)
+    rr = np.ascontiguousarray(radii, np.float32)
+    cap = f.size * max(len(rr), 1)
+    rows = np.zeros(cap, ROW_DTYPE)
+    n = lib().or_log_detect_frame_adaptive(_ptr(f), C.byref(p), _ptr(rr), len(rr), int(n_iter),
+                                           int(characterize), overlap_threshold, _ptr(rows), cap)
+    return rows[:n].copy()
 
 
 def frame_std(frame):

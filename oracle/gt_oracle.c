@@ -712,16 +712,18 @@ size_t or_prune_blobs(or_row *rows, size_t n, float overlap_threshold) {
     return m;
 }
 
-size_t or_log_detect_frame(const float *frame, const or_params *p, const float *radii,
-                           uint32_t n_radii, int characterize, float overlap_threshold,
-                           or_row *rows, size_t cap, size_t *ncand_out) {
-    /* execute_gpu.rs:237-286 (bandpass, then per scale the LoG plane, the
-     * scale-space maximum of the previous scale and its walk) and the LoG
-     * branch of post_process.  Rows before pruning are in (scale, raster)
-     * order; seed = scale * H * W + raster. */
+static size_t or_adaptive_passes(const float *frame, const or_params *pp, float sd,
+                                 uint32_t n_iter, or_row *pend, size_t np, or_row *acc);
+
+/* The walked LoG rows of every scale (uncharacterized unless asked), in
+ * (scale, raster) order with seed = scale * H * W + raster; proc receives the
+ * bandpassed frame (3 * H * W floats of room).  Returns the row array
+ * (caller frees) and its length in *total. */
+static or_row *or_log_rows(const float *frame, const or_params *p, const float *radii,
+                           uint32_t n_radii, int characterize, float *proc, size_t *total_out,
+                           size_t *ncand_out) {
     const int32_t H = p->nrows, W = p->ncols;
     const size_t hw = (size_t)H * W;
-    float *proc = (float *)malloc(sizeof(float) * hw * 3);
     or_bandpass(frame, p, proc, proc + hw);
     float *planes = (float *)malloc(sizeof(float) * hw * 3);
     float *t1 = (float *)malloc(sizeof(float) * hw), *t2 = (float *)malloc(sizeof(float) * hw);
@@ -749,15 +751,52 @@ size_t or_log_detect_frame(const float *frame, const or_params *p, const float *
         total += nr;
     }
     if (ncand_out) *ncand_out = ncand;
-    size_t n = or_prune_blobs(all, total, overlap_threshold);
-    if (n <= cap) memcpy(rows, all, sizeof(or_row) * n);
-    else n = (size_t)-1;
-    free(all);
     free(cr);
     free(cand);
     free(t1);
     free(t2);
     free(planes);
+    *total_out = total;
+    return all;
+}
+
+size_t or_log_detect_frame(const float *frame, const or_params *p, const float *radii,
+                           uint32_t n_radii, int characterize, float overlap_threshold,
+                           or_row *rows, size_t cap, size_t *ncand_out) {
+    /* execute_gpu.rs:237-286 (bandpass, then per scale the LoG plane, the
+     * scale-space maximum of the previous scale and its walk) and the LoG
+     * branch of post_process. */
+    const size_t hw = (size_t)p->nrows * p->ncols;
+    float *proc = (float *)malloc(sizeof(float) * hw * 3);
+    size_t total = 0;
+    or_row *all = or_log_rows(frame, p, radii, n_radii, characterize, proc, &total, ncand_out);
+    size_t n = or_prune_blobs(all, total, overlap_threshold);
+    if (n <= cap) memcpy(rows, all, sizeof(or_row) * n);
+    else n = (size_t)-1;
+    free(all);
+    free(proc);
+    return n;
+}
+
+size_t or_log_detect_frame_adaptive(const float *frame, const or_params *p_in, const float *radii,
+                                    uint32_t n_radii, uint32_t n_iter, int characterize,
+                                    float overlap_threshold, or_row *rows, size_t cap) {
+    or_params p = *p_in;
+    const size_t hw = (size_t)p.nrows * p.ncols;
+    float mean, sd;
+    or_frame_std(frame, hw, &mean, &sd);
+    p.image_std = sd;
+    float *proc = (float *)malloc(sizeof(float) * hw * 3);
+    size_t total = 0;
+    or_row *all = or_log_rows(frame, &p, radii, n_radii, 0, proc, &total, NULL);
+    or_row *acc = (or_row *)malloc(sizeof(or_row) * (total ? total : 1));
+    const size_t na = or_adaptive_passes(frame, &p, sd, n_iter, all, total, acc);
+    if (characterize) or_characterize(proc, frame, &p, acc, na, 1, 0);
+    size_t n = or_prune_blobs(acc, na, overlap_threshold);
+    if (n <= cap) memcpy(rows, acc, sizeof(or_row) * n);
+    else n = (size_t)-1;
+    free(acc);
+    free(all);
     free(proc);
     return n;
 }
@@ -855,21 +894,16 @@ static void or_mask_row(float *fr, const or_params *p, const or_row *row) {
     }
 }
 
-size_t or_detect_frame_adaptive(const float *frame, const or_params *p_in, uint32_t n_iter,
-                                int characterize, or_row *rows, size_t cap) {
-    or_params p = *p_in;
+/* The adaptive passes (execute_gpu.rs:288-330, adaptive_filter.wgsl:89-125)
+ * over walked rows pend[0..np) (consumed): accepted rows go to acc in
+ * acceptance order; returns their number. */
+static size_t or_adaptive_passes(const float *frame, const or_params *pp, float sd,
+                                 uint32_t n_iter, or_row *pend, size_t np, or_row *acc) {
+    const or_params p = *pp;
     const size_t hw = (size_t)p.nrows * p.ncols;
-    float mean, sd;
-    or_frame_std(frame, hw, &mean, &sd);
-    p.image_std = sd;
-    float *proc = (float *)malloc(sizeof(float) * hw * 3);
-    uint32_t *cand = (uint32_t *)malloc(sizeof(uint32_t) * (hw ? hw : 1));
-    or_bandpass(frame, &p, proc, proc + hw);
-    const size_t nc = or_local_max(proc, &p, cand, hw, proc + hw);
-    or_row *pend = (or_row *)malloc(sizeof(or_row) * (nc ? nc : 1));
-    or_row *next = (or_row *)malloc(sizeof(or_row) * (nc ? nc : 1));
-    or_row *acc = (or_row *)malloc(sizeof(or_row) * (nc ? nc : 1));
-    size_t np = or_walk(proc, &p, cand, nc, pend), na = 0;
+    or_row *next = (or_row *)malloc(sizeof(or_row) * (np ? np : 1));
+    size_t na = 0;
+    float mean;
     float *masked = (float *)malloc(sizeof(float) * (hw ? hw : 1));
     memcpy(masked, frame, sizeof(float) * hw);
     uint32_t counter = (uint32_t)hw;
@@ -893,13 +927,31 @@ size_t or_detect_frame_adaptive(const float *frame, const or_params *p_in, uint3
         memcpy(pend, next, sizeof(or_row) * nn);
         np = nn;
     }
+    free(masked);
+    free(next);
+    return na;
+}
+
+size_t or_detect_frame_adaptive(const float *frame, const or_params *p_in, uint32_t n_iter,
+                                int characterize, or_row *rows, size_t cap) {
+    or_params p = *p_in;
+    const size_t hw = (size_t)p.nrows * p.ncols;
+    float mean, sd;
+    or_frame_std(frame, hw, &mean, &sd);
+    p.image_std = sd;
+    float *proc = (float *)malloc(sizeof(float) * hw * 3);
+    uint32_t *cand = (uint32_t *)malloc(sizeof(uint32_t) * (hw ? hw : 1));
+    or_bandpass(frame, &p, proc, proc + hw);
+    const size_t nc = or_local_max(proc, &p, cand, hw, proc + hw);
+    or_row *pend = (or_row *)malloc(sizeof(or_row) * (nc ? nc : 1));
+    or_row *acc = (or_row *)malloc(sizeof(or_row) * (nc ? nc : 1));
+    const size_t np = or_walk(proc, &p, cand, nc, pend);
+    const size_t na = or_adaptive_passes(frame, &p, sd, n_iter, pend, np, acc);
     if (characterize) or_characterize(proc, frame, &p, acc, na, 1, 0);
     size_t out = na;
     if (na <= cap) memcpy(rows, acc, sizeof(or_row) * na);
     else out = (size_t)-1;
-    free(masked);
     free(acc);
-    free(next);
     free(pend);
     free(cand);
     free(proc);

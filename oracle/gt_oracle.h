@@ -170,7 +170,7 @@ void or_frame_std_div(const float *frame, size_t n, uint32_t div_mean, uint32_t 
  * the unmarked frame. */
 size_t or_detect_frame_adaptive(const float *frame, const or_params *p, uint32_t n_iter,
                                 int characterize, or_row *rows, size_t cap);
-/* Illumination correction (src/shaders/correct_illumination/*.wgsl;
+/* Illumination correction (the shaders in src/shaders/correct_illumination;
  * setup_illumination_profile, src/execute_gpu.rs:849-884; submit_work,
  * :138-166).  Taps: rint = i32(4 sigma + 0.5), w[i + rint] =
  * exp(-(i*i) / (2 (sigma*sigma))) in f32; returns rint. */
@@ -185,6 +185,14 @@ void or_illum_smooth(const float *in, int32_t H, int32_t W, float sigma, float *
 void or_frames_mean(const float *frames, size_t T, size_t n, float *out);
 /* correct_illumination.wgsl: out[i] = frame[i] / profile[i]. */
 void or_illum_divide(const float *frame, const float *profile, size_t n, float *out);
+/* LoG detection with adaptive_background = n_iter: the noise estimate of the
+ * raw frame sets the scale-space threshold (rough_snr_factor 0.5 in p); the
+ * walked rows of every scale go through the adaptive passes (a row with
+ * r > 0 marks its disk of radius ceil(r)), the accepted ones are
+ * characterized and pruned. */
+size_t or_log_detect_frame_adaptive(const float *frame, const or_params *p, const float *radii,
+                                    uint32_t n_radii, uint32_t n_iter, int characterize,
+                                    float overlap_threshold, or_row *rows, size_t cap);
 void or_bandpass_f64(const float *frame, const or_params *p, double *processed, double *scratch);
 /* FP64 centroid of the disk at integer position (pu, pv) of a f64 frame. */
 void or_centroid_f64(const double *processed, const or_params *p, int32_t pu, int32_t pv,

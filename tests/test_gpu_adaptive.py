@@ -1,6 +1,7 @@
-"""adaptive_background on the GPU against the oracle (or_detect_frame_adaptive):
-the accepted rows, characterized, must be bit-identical for several pass
-counts and rule settings."""
+"""adaptive_background on the GPU against the oracle (or_detect_frame_adaptive,
+or_log_detect_frame_adaptive): the accepted rows, characterized, must be
+bit-identical for several pass counts and rule settings, in the Trackpy and
+the LoG style (whose rows mark disks of their own radius)."""
 import numpy as np
 import pytest
 
@@ -60,8 +61,37 @@ def test_adaptive_through_batch_and_rejections(torch_dev):
     assert len(df) > 0
     with pytest.raises(gt.GtError):
         gt.batch(v, 9, adaptive_background=0)
-    with pytest.raises(gt.GtError):
-        gt.LoG(v, 1.5, 3.0, adaptive_background=1)
     with gt.Context(gt.make_params(9, adaptive_background=1, snr=1.0), 128, 160) as ctx:
         with pytest.raises(gt.GtError):
             ctx.set_learned(gt.CentroidNet(seed=1))
+
+
+@pytest.mark.parametrize("n_iter", [1, 2, 3])
+def test_log_adaptive_rows_bit_identical(torch_dev, n_iter):
+    import oracle
+    import gpu_tracking_amd as gt
+    from gpu_tracking_amd import synth
+    H, W = 160, 176
+    v = synth.video_numpy(synth.spec("log", nrows=H, ncols=W, n_particles=50), 2, 60 + n_iter)
+    p = gt.make_log_params(1.5, 4.0, n_radii=5, snr=1.5, minmass_snr=0.3,
+                           adaptive_background=n_iter, characterize=True)
+    with gt.Context(p, H, W) as ctx:
+        out, counts = ctx.detect_device(torch_dev.from_numpy(v).cuda(), cap_per_frame=H * W)
+        torch_dev.cuda.synchronize()
+        rows = gt.detections_to_numpy(out, counts)
+    op = oracle.derive_log(H, W, 4.0)
+    op.snr, op.minmass_snr, op.rough_snr_factor = 1.5, 0.3, 0.5
+    radii = oracle.log_radii(1.5, 4.0, 5, False)
+    total = 0
+    for t in range(v.shape[0]):
+        ref = np.sort(oracle.log_detect_frame_adaptive(v[t], op, radii, n_iter,
+                                                       overlap_threshold=1.0), order="seed")
+        _assert_rows_equal(rows[t], ref, f"LoG frame {t} n_iter {n_iter}")
+        total += len(ref)
+    assert total > 0
+    # through LoG() (pruning at the default overlap_threshold 0)
+    df = gt.LoG(v, 1.5, 4.0, n_radii=5, snr=1.5, minmass_snr=0.3, adaptive_background=n_iter,
+                doughnut_correction=False)
+    n_ref = sum(len(oracle.log_detect_frame_adaptive(v[t], op, radii, n_iter,
+                                                     overlap_threshold=0.0)) for t in range(2))
+    assert len(df) == n_ref

@@ -91,3 +91,76 @@ def test_passes_only_add_rows():
         assert prev <= s
         prev = s
     assert len(prev) > 0
+
+
+# ---- LoG style: rows carry r > 0 and mark disks of radius ceil(r) -------------
+
+def _log_case(seed, n=60):
+    from gpu_tracking_amd import synth
+    s = synth.spec("log", nrows=128, ncols=144, n_particles=n)
+    f = synth.video_numpy(s, 1, seed)[0].astype(np.float32)
+    p = oracle.derive_log(128, 144, 4.0)
+    p.snr, p.minmass_snr, p.rough_snr_factor = 1.5, 0.3, 0.5
+    return f, p, oracle.log_radii(1.5, 4.0, 5, False)
+
+
+def _log_rule(rows, p, std):
+    mm = np.float32(p.minmass_snr) * rows["count"] * np.float32(std)
+    return ~((rows["mass"] < mm) | (rows["max_intensity"] < np.float32(p.snr) * np.float32(std)))
+
+
+def _mask_r(fr, rows):
+    H, W = fr.shape
+    for row in rows:
+        rint = int(np.ceil(row["r"]))
+        r2 = np.float32(rint) * np.float32(rint)
+        u, v = int(np.rint(row["x"])), int(np.rint(row["y"]))
+        for i in range(-rint, rint + 1):
+            for j in range(-rint, rint + 1):
+                if np.float32(i * i) + np.float32(j * j) > r2:
+                    continue
+                if 0 <= u + i < H and 0 <= v + j < W:
+                    fr[u + i, v + j] = MARKER
+
+
+def test_log_one_pass_is_the_filtered_plain_detection():
+    f, p, radii = _log_case(5)
+    got = np.sort(oracle.log_detect_frame_adaptive(f, p, radii, 1, overlap_threshold=1.0),
+                  order="seed")
+    _, std = oracle.frame_std(f)
+    p.image_std = std
+    rows, _ = oracle.log_detect_frame(f, p, radii, characterize=True, overlap_threshold=1.0)
+    want = np.sort(rows[_log_rule(rows, p, std)], order="seed")
+    assert len(want) > 0 and np.all(want["r"] > 0)
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+
+
+def test_log_two_passes_match_a_restatement():
+    f, p, radii = _log_case(6, n=90)
+    _, std0 = oracle.frame_std(f)
+    p.image_std = std0
+    rows, _ = oracle.log_detect_frame(f, p, radii, characterize=True, overlap_threshold=1.0)
+    ok1 = _log_rule(rows, p, std0)
+    acc1, rest = rows[ok1], rows[~ok1]
+    masked = f.copy()
+    _mask_r(masked, acc1)
+    div = f.size - int(acc1["count"].astype(np.uint32).sum())
+    _, std1 = oracle.frame_std_div(masked, div, div - 1)
+    ok2 = _log_rule(rest, p, std1)
+    want = np.sort(np.concatenate([acc1, rest[ok2]]), order="seed")
+    got = np.sort(oracle.log_detect_frame_adaptive(f, p, radii, 2, overlap_threshold=1.0),
+                  order="seed")
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+    # larger blobs mark wider disks than the fixed walk radius would
+    assert np.any(np.ceil(acc1["r"]) >= 3)
+
+
+def test_log_passes_only_add_rows():
+    f, p, radii = _log_case(7, n=90)
+    prev = set()
+    for n in range(4):
+        s = set(oracle.log_detect_frame_adaptive(f, p, radii, n, overlap_threshold=1.0)["seed"]
+                .tolist())
+        assert prev <= s
+        prev = s
+    assert len(prev) > 0
