@@ -284,6 +284,11 @@ class Context:
             self._lib.gt_output_free(C.byref(out))
         return data, names, types
 
+    def set_lanes(self, lanes: int) -> None:
+        """Streams (1 or 2) the chunks of a detect_device call alternate
+        between (plain Trackpy-style detection); results do not depend on it."""
+        N.check(self._lib.gt_context_set_lanes(self._h, int(lanes)))
+
     # device-resident path -------------------------------------------------
     def detect_device(self, frames, out=None, counts=None, cap_per_frame: Optional[int] = None,
                       stream=None):

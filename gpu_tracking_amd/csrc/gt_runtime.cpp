@@ -126,6 +126,16 @@ struct gt_context {
     float *d_illum_tmp = nullptr;   // chunk * HW
     float *d_illum_prof = nullptr;  // chunk * HW (per frame)
 
+    // a second lane for the plain tiled path: odd chunks run on their own
+    // stream with their own scratch, so a chunk's slow walks and
+    // characterize overlap the next chunk's bandpass
+    int32_t lanes = 1;                 // lanes allowed (1 or 2)
+    hipStream_t lane_stream[2] = {nullptr, nullptr};
+    hipEvent_t lane_fork = nullptr, lane_done[2] = {nullptr, nullptr};
+    float *l1_proc = nullptr;
+    uint32_t *l1_ocount = nullptr, *l1_scount = nullptr;
+    uint2 *l1_slow = nullptr;
+
     // gt_execute staging: two slots, so one chunk's detections come back
     // and are post-processed while the next chunk runs on the GPU
     struct Slot {
@@ -276,6 +286,15 @@ void free_all(gt_context *c) {
     if (c->d_adapt_state) (void)hipFree(c->d_adapt_state);
     if (c->d_mean) (void)hipFree(c->d_mean);
     if (c->d_std) (void)hipFree(c->d_std);
+    if (c->l1_proc) (void)hipFree(c->l1_proc);
+    if (c->l1_ocount) (void)hipFree(c->l1_ocount);
+    if (c->l1_scount) (void)hipFree(c->l1_scount);
+    if (c->l1_slow) (void)hipFree(c->l1_slow);
+    for (int l = 0; l < 2; ++l) {
+        if (c->lane_done[l]) (void)hipEventDestroy(c->lane_done[l]);
+        if (c->lane_stream[l]) (void)hipStreamDestroy(c->lane_stream[l]);
+    }
+    if (c->lane_fork) (void)hipEventDestroy(c->lane_fork);
     if (c->d_illum_w) (void)hipFree(c->d_illum_w);
     if (c->d_profile) (void)hipFree(c->d_profile);
     if (c->d_corr) (void)hipFree(c->d_corr);
@@ -372,6 +391,29 @@ gt_status illum_correct(gt_context *c, const void *frames, int32_t dtype, uint32
     return GT_OK;
 }
 
+// Whether the second lane applies: the plain tiled path (no LoG, learned
+// proposal, noise estimate, adaptive passes or illumination correction,
+// whose scratch is single).
+bool lanes_apply(const gt_context *c) {
+    return c->lanes > 1 && c->tiled && !c->log && !c->cnn && !c->der.need_std &&
+           !c->der.adaptive && !(c->der.illum_sigma > 0.0f);
+}
+
+gt_status alloc_lane1(gt_context *c) {
+    if (c->l1_proc) return GT_OK;
+    const size_t HW = static_cast<size_t>(c->H) * c->W;
+    for (int l = 0; l < 2; ++l) {
+        GT_HIP(hipStreamCreateWithFlags(&c->lane_stream[l], hipStreamNonBlocking));
+        GT_HIP(hipEventCreateWithFlags(&c->lane_done[l], hipEventDisableTiming));
+    }
+    GT_HIP(hipEventCreateWithFlags(&c->lane_fork, hipEventDisableTiming));
+    GT_HIP(hipMalloc(&c->l1_ocount, sizeof(uint32_t) * GT_COUNTER_STRIDE * c->chunk));
+    GT_HIP(hipMalloc(&c->l1_scount, sizeof(uint32_t) * GT_COUNTER_STRIDE * c->chunk));
+    GT_HIP(hipMalloc(&c->l1_slow, sizeof(uint2) * static_cast<size_t>(c->slow_cap) * c->chunk));
+    GT_HIP(hipMalloc(&c->l1_proc, sizeof(float) * HW * c->chunk));
+    return GT_OK;
+}
+
 // Runs the three pipeline stages over `nframes` device frames in chunks.
 gt_status run_pipeline(gt_context *c, const void *d_frames, int32_t dtype, uint32_t nframes,
                        gt_detection *d_out, uint32_t cap, uint32_t *d_counts, hipStream_t s) {
@@ -390,47 +432,62 @@ gt_status run_pipeline(gt_context *c, const void *d_frames, int32_t dtype, uint3
             c->events.push_back(e);
         }
     }
+    // chunk ci runs on lane ci % 2 when the second lane applies; the lanes
+    // start after the work queued on s and s waits for both at the end
+    const bool two = lanes_apply(c) && nchunks > 1;
+    if (two) {
+        gt_status lst = alloc_lane1(c);
+        if (lst != GT_OK) return lst;
+        GT_HIP(hipEventRecord(c->lane_fork, s));
+        for (int l = 0; l < 2; ++l) GT_HIP(hipStreamWaitEvent(c->lane_stream[l], c->lane_fork, 0));
+    }
     for (uint32_t ci = 0; ci < nchunks; ++ci) {
+        const int lane = two ? static_cast<int>(ci & 1u) : 0;
+        hipStream_t cs = two ? c->lane_stream[lane] : s;
+        float *proc = lane ? c->l1_proc : c->d_proc;
+        uint32_t *ocount = lane ? c->l1_ocount : c->d_ocount;
+        uint32_t *scount = lane ? c->l1_scount : c->d_scount;
+        uint2 *slow = lane ? c->l1_slow : c->d_slow;
         const uint32_t f0 = ci * c->chunk;
         const uint32_t nf = std::min(c->chunk, nframes - f0);
         const char *frames = static_cast<const char *>(d_frames) + esz * HW * f0;
         int32_t fdtype = dtype;
         if (c->der.illum_sigma > 0.0f) {
-            gt_status ist = illum_correct(c, frames, dtype, nf, s);
+            gt_status ist = illum_correct(c, frames, dtype, nf, cs);
             if (ist != GT_OK) return ist;
             frames = reinterpret_cast<const char *>(c->d_corr);
             fdtype = GT_F32;
         }
         hipEvent_t *ev = c->timing ? &c->events[4 * ci] : nullptr;
-        GT_HIP(gt_launch_zero_u32(c->d_ocount, static_cast<size_t>(nf) * GT_COUNTER_STRIDE, s));
+        GT_HIP(gt_launch_zero_u32(ocount, static_cast<size_t>(nf) * GT_COUNTER_STRIDE, cs));
         if (!c->tiled || c->log)
-            GT_HIP(gt_launch_zero_u32(c->d_ccount, static_cast<size_t>(nf) * GT_COUNTER_STRIDE, s));
-        if (ev) GT_HIP(hipEventRecord(ev[0], s));
+            GT_HIP(gt_launch_zero_u32(c->d_ccount, static_cast<size_t>(nf) * GT_COUNTER_STRIDE, cs));
+        if (ev) GT_HIP(hipEventRecord(ev[0], cs));
         if (c->der.need_std)
-            GT_HIP(gt_launch_frame_std(frames, fdtype, nf, HW, c->d_std_tmp, c->d_mean, c->d_std, s));
-        GT_HIP(gt_launch_bandpass(frames, fdtype, c->d_proc, nf, c->kp, s));
-        if (ev) GT_HIP(hipEventRecord(ev[1], s));
+            GT_HIP(gt_launch_frame_std(frames, fdtype, nf, HW, c->d_std_tmp, c->d_mean, c->d_std, cs));
+        GT_HIP(gt_launch_bandpass(frames, fdtype, proc, nf, c->kp, cs));
+        if (ev) GT_HIP(hipEventRecord(ev[1], cs));
         gt_detection *out = d_out + static_cast<size_t>(f0) * cap;
         if (c->log) {
             // scale-space maxima of all scales, then their walks (and, with
             // adaptive_background, the acceptance passes before characterize)
-            GT_HIP(gt_launch_log_tile(frames, fdtype, c->d_proc, nf, c->kp, c->d_logtab, c->log_Rm,
-                                      c->d_cand, c->cand_cap, c->d_ccount, s));
-            if (ev) GT_HIP(hipEventRecord(ev[2], s));
+            GT_HIP(gt_launch_log_tile(frames, fdtype, proc, nf, c->kp, c->d_logtab, c->log_Rm,
+                                      c->d_cand, c->cand_cap, c->d_ccount, cs));
+            if (ev) GT_HIP(hipEventRecord(ev[2], cs));
             GtKParams kw = c->kp;
             if (c->der.adaptive) kw.characterize = 0;
-            GT_HIP(gt_launch_walk_log(c->d_proc, frames, fdtype, nf, kw, c->d_logtab, c->d_cand,
-                                      c->cand_cap, c->d_ccount, out, cap, c->d_ocount, f0, s));
+            GT_HIP(gt_launch_walk_log(proc, frames, fdtype, nf, kw, c->d_logtab, c->d_cand,
+                                      c->cand_cap, c->d_ccount, out, cap, ocount, f0, cs));
             if (c->der.adaptive) {
-                gt_status ast = run_adaptive(c, frames, fdtype, nf, out, cap, s);
+                gt_status ast = run_adaptive(c, frames, fdtype, nf, out, cap, cs);
                 if (ast != GT_OK) return ast;
                 if (c->kp.characterize)
-                    GT_HIP(gt_launch_characterize_det(c->d_proc, frames, fdtype, nf, c->kp, out,
-                                                      cap, c->d_ocount, s));
+                    GT_HIP(gt_launch_characterize_det(proc, frames, fdtype, nf, c->kp, out,
+                                                      cap, ocount, cs));
             }
-            GT_HIP(gt_launch_copy_counts_checked(c->d_ocount, c->d_ccount, c->cand_cap,
-                                                 d_counts + f0, nf, s));
-            if (ev) GT_HIP(hipEventRecord(ev[3], s));
+            GT_HIP(gt_launch_copy_counts_checked(ocount, c->d_ccount, c->cand_cap,
+                                                 d_counts + f0, nf, cs));
+            if (ev) GT_HIP(hipEventRecord(ev[3], cs));
             continue;
         }
         if (c->tiled && c->cnn && c->cnn_scope == GT_CNN_MOVING) {
@@ -440,70 +497,76 @@ gt_status run_pipeline(gt_context *c, const void *d_frames, int32_t dtype, uint3
             // rest of the walk where it fails
             GtKParams kw = c->kp;
             kw.characterize = 0;
-            GT_HIP(gt_launch_zero_u32(c->d_scount, static_cast<size_t>(nf) * GT_COUNTER_STRIDE, s));
-            GT_HIP(gt_launch_detect_tile(c->d_proc, frames, fdtype, nf, kw, out, cap, c->d_ocount,
-                                         c->d_slow, c->slow_cap, c->d_scount, f0, s, 1));
-            if (ev) GT_HIP(hipEventRecord(ev[2], s));
-            const uint32_t *list = reinterpret_cast<const uint32_t *>(c->d_slow);
-            GT_HIP(gt_cnn_launch_forward_frames(gt_cnn_packed(c->cnn), c->d_proc,
+            GT_HIP(gt_launch_zero_u32(scount, static_cast<size_t>(nf) * GT_COUNTER_STRIDE, cs));
+            GT_HIP(gt_launch_detect_tile(proc, frames, fdtype, nf, kw, out, cap, ocount,
+                                         slow, c->slow_cap, scount, f0, cs, 1));
+            if (ev) GT_HIP(hipEventRecord(ev[2], cs));
+            const uint32_t *list = reinterpret_cast<const uint32_t *>(slow);
+            GT_HIP(gt_cnn_launch_forward_frames(gt_cnn_packed(c->cnn), proc,
                                                 static_cast<int>(c->H), static_cast<int>(c->W), nf,
-                                                list, 2, c->slow_cap, c->d_scount, c->d_props, s));
-            GT_HIP(gt_launch_walk_learned(c->d_proc, nf, kw, list, 2, 1, c->slow_cap, c->d_scount,
+                                                list, 2, c->slow_cap, scount, c->d_props, cs));
+            GT_HIP(gt_launch_walk_learned(proc, nf, kw, list, 2, 1, c->slow_cap, scount,
                                           c->d_props, c->cnn_conf, c->cnn_tol, out, cap,
-                                          c->d_ocount, c->d_stats, f0, s));
+                                          ocount, c->d_stats, f0, cs));
             if (c->kp.characterize)
-                GT_HIP(gt_launch_characterize_det(c->d_proc, frames, fdtype, nf, c->kp, out, cap,
-                                                  c->d_ocount, s));
+                GT_HIP(gt_launch_characterize_det(proc, frames, fdtype, nf, c->kp, out, cap,
+                                                  ocount, cs));
         } else if (c->tiled && c->cnn) {
             // every candidate: the network's proposals for all of them, one
             // verifying walk evaluation each (the full walk where it fails)
             GtKParams kw = c->kp;
             kw.characterize = 0;
-            GT_HIP(gt_launch_zero_u32(c->d_ccount, static_cast<size_t>(nf) * GT_COUNTER_STRIDE, s));
-            GT_HIP(gt_launch_detect_cand(c->d_proc, fdtype, nf, kw, c->d_cand, c->cand_cap,
-                                         c->d_ccount, s));
-            if (ev) GT_HIP(hipEventRecord(ev[2], s));
-            GT_HIP(gt_cnn_launch_forward_frames(gt_cnn_packed(c->cnn), c->d_proc,
+            GT_HIP(gt_launch_zero_u32(c->d_ccount, static_cast<size_t>(nf) * GT_COUNTER_STRIDE, cs));
+            GT_HIP(gt_launch_detect_cand(proc, fdtype, nf, kw, c->d_cand, c->cand_cap,
+                                         c->d_ccount, cs));
+            if (ev) GT_HIP(hipEventRecord(ev[2], cs));
+            GT_HIP(gt_cnn_launch_forward_frames(gt_cnn_packed(c->cnn), proc,
                                                 static_cast<int>(c->H), static_cast<int>(c->W), nf,
                                                 c->d_cand, 1, c->cand_cap, c->d_ccount, c->d_props,
-                                                s));
-            GT_HIP(gt_launch_walk_learned(c->d_proc, nf, kw, c->d_cand, 1, 0, c->cand_cap,
+                                                cs));
+            GT_HIP(gt_launch_walk_learned(proc, nf, kw, c->d_cand, 1, 0, c->cand_cap,
                                           c->d_ccount, c->d_props, c->cnn_conf, c->cnn_tol, out,
-                                          cap, c->d_ocount, c->d_stats, f0, s));
+                                          cap, ocount, c->d_stats, f0, cs));
             if (c->kp.characterize)
-                GT_HIP(gt_launch_characterize_det(c->d_proc, frames, fdtype, nf, c->kp, out, cap,
-                                                  c->d_ocount, s));
+                GT_HIP(gt_launch_characterize_det(proc, frames, fdtype, nf, c->kp, out, cap,
+                                                  ocount, cs));
         } else if (c->tiled) {
             // local max and walk in one kernel, the walks it hands over, then
             // characterize over the frame's compacted detections (whole
             // wavefronts of detections instead of a few lanes per tile)
             GtKParams kw = c->kp;
             kw.characterize = 0;
-            GT_HIP(gt_launch_zero_u32(c->d_scount, static_cast<size_t>(nf) * GT_COUNTER_STRIDE, s));
-            GT_HIP(gt_launch_detect_tile(c->d_proc, frames, fdtype, nf, kw, out, cap,
-                                         c->d_ocount, c->d_slow, c->slow_cap, c->d_scount, f0, s));
-            if (ev) GT_HIP(hipEventRecord(ev[2], s));
-            GT_HIP(gt_launch_walk_slow(c->d_proc, frames, fdtype, nf, kw, c->d_slow, c->slow_cap,
-                                       c->d_scount, out, cap, c->d_ocount, f0, s));
+            GT_HIP(gt_launch_zero_u32(scount, static_cast<size_t>(nf) * GT_COUNTER_STRIDE, cs));
+            GT_HIP(gt_launch_detect_tile(proc, frames, fdtype, nf, kw, out, cap,
+                                         ocount, slow, c->slow_cap, scount, f0, cs));
+            if (ev) GT_HIP(hipEventRecord(ev[2], cs));
+            GT_HIP(gt_launch_walk_slow(proc, frames, fdtype, nf, kw, slow, c->slow_cap,
+                                       scount, out, cap, ocount, f0, cs));
             if (c->der.adaptive) {
-                gt_status ast = run_adaptive(c, frames, fdtype, nf, out, cap, s);
+                gt_status ast = run_adaptive(c, frames, fdtype, nf, out, cap, cs);
                 if (ast != GT_OK) return ast;
             }
             if (c->kp.characterize)
-                GT_HIP(gt_launch_characterize_det(c->d_proc, frames, fdtype, nf, c->kp, out, cap,
-                                                  c->d_ocount, s));
+                GT_HIP(gt_launch_characterize_det(proc, frames, fdtype, nf, c->kp, out, cap,
+                                                  ocount, cs));
         } else {
-            GT_HIP(gt_launch_localmax(c->d_proc, nf, c->kp, c->d_cand, c->cand_cap, c->d_ccount, s));
-            if (ev) GT_HIP(hipEventRecord(ev[2], s));
-            GT_HIP(gt_launch_walk(c->d_proc, frames, fdtype, nf, c->kp, c->d_cand, c->cand_cap,
-                                  c->d_ccount, out, cap, c->d_ocount, f0, s));
+            GT_HIP(gt_launch_localmax(proc, nf, c->kp, c->d_cand, c->cand_cap, c->d_ccount, cs));
+            if (ev) GT_HIP(hipEventRecord(ev[2], cs));
+            GT_HIP(gt_launch_walk(proc, frames, fdtype, nf, c->kp, c->d_cand, c->cand_cap,
+                                  c->d_ccount, out, cap, ocount, f0, cs));
             if (c->der.adaptive) {
-                gt_status ast = run_adaptive(c, frames, fdtype, nf, out, cap, s);
+                gt_status ast = run_adaptive(c, frames, fdtype, nf, out, cap, cs);
                 if (ast != GT_OK) return ast;
             }
         }
-        GT_HIP(gt_launch_copy_counts(c->d_ocount, d_counts + f0, nf, s));
-        if (ev) GT_HIP(hipEventRecord(ev[3], s));
+        GT_HIP(gt_launch_copy_counts(ocount, d_counts + f0, nf, cs));
+        if (ev) GT_HIP(hipEventRecord(ev[3], cs));
+    }
+    if (two) {
+        for (int l = 0; l < 2; ++l) {
+            GT_HIP(hipEventRecord(c->lane_done[l], c->lane_stream[l]));
+            GT_HIP(hipStreamWaitEvent(s, c->lane_done[l], 0));
+        }
     }
     if (c->timing) {
         c->n_timed_chunks = static_cast<int>(nchunks);
@@ -657,6 +720,13 @@ gt_status gt_detect_device(gt_context *c, const void *d_frames, int32_t dtype, u
         if (ist != GT_OK) return ist;
     }
     return run_pipeline(c, d_frames, dtype, nframes, d_out, cap_per_frame, d_counts, s);
+}
+
+gt_status gt_context_set_lanes(gt_context *c, int32_t lanes) {
+    if (!c) return fail(GT_ERR_INVALID_ARGUMENT, "null context");
+    if (lanes != 1 && lanes != 2) return fail(GT_ERR_INVALID_ARGUMENT, "lanes must be 1 or 2");
+    c->lanes = lanes;
+    return GT_OK;
 }
 
 gt_status gt_set_timing(gt_context *c, int32_t enable) {

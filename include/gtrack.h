@@ -168,6 +168,17 @@ gt_status gt_detect_device(gt_context *ctx, const void *d_frames, int32_t dtype,
                            uint32_t nframes, gt_detection *d_out, uint32_t cap_per_frame,
                            uint32_t *d_counts, void *stream);
 
+/* Lanes of gt_detect_device (1 or 2, default 1): with 2, the chunks of a call
+ * alternate between two streams with their own scratch (plain Trackpy-style
+ * detection without the noise estimate, adaptive passes, illumination
+ * correction or learned proposal), so one chunk's slow walks and
+ * characterize overlap the next chunk's bandpass; the work still starts
+ * after, and ends before, the caller's stream.  Results do not depend on it.
+ * Throughput rises a few percent; each stage's own time (gt_stage_times)
+ * then includes the other lane's concurrent work.  No reference
+ * counterpart. */
+gt_status gt_context_set_lanes(gt_context *ctx, int32_t lanes);
+
 /* execute_gpu over an in-memory video (execute_ndarray -> execute_provider ->
  * execute_gpu, src/execute_gpu.rs:1378,1272,886): frames are staged through
  * pinned host memory to the GPU in chunks, detected, read back (counts and
