@@ -24,6 +24,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <unordered_set>
 
 struct gt_file {
     int fd = -1;
@@ -98,7 +99,9 @@ size_t type_size(uint16_t t) {
 // The integer values of an entry (BYTE / SHORT / LONG / LONG8 / IFD types).
 bool entry_values(int fd, bool be, bool big, const Entry &e, std::vector<uint64_t> *v) {
     const size_t ts = type_size(e.type);
-    if (!ts || e.count > (1ull << 32)) return false;
+    // tag arrays hold at most one entry per strip or tile; 2^24 bounds any
+    // real image and keeps a malformed count from a huge allocation
+    if (!ts || e.count > (1ull << 24)) return false;
     const size_t n = static_cast<size_t>(e.count) * ts;
     std::vector<uint8_t> buf(n);
     const size_t inl = big ? 8 : 4;
@@ -152,10 +155,9 @@ gt_status open_tiff(gt_file *f, std::string *why) {
         return GT_ERR_READ;
     }
     int src0 = -1;
-    std::vector<uint64_t> seen;
+    std::unordered_set<uint64_t> seen;
     while (ifd != 0) {
-        if (std::find(seen.begin(), seen.end(), ifd) != seen.end()) break;   // cycle
-        seen.push_back(ifd);
+        if (!seen.insert(ifd).second) break;   // a cycle in the IFD chain
         uint8_t cb[8];
         const size_t csz = big ? 8 : 2, esz = big ? 20 : 12;
         if (!pread_all(f->fd, ifd, cb, csz)) {
@@ -384,6 +386,7 @@ bool decode_segment(const gt_file *f, const gt_file::Page &pg, size_t seg, uint8
         if (cnt < want) return false;
         return pread_all(f->fd, off, out, want);
     }
+    if (cnt > (static_cast<uint64_t>(want) + 1024) * 4 + (1u << 20)) return false;   // implausible
     tmp.resize(static_cast<size_t>(cnt));
     if (!pread_all(f->fd, off, tmp.data(), tmp.size())) return false;
     switch (pg.compression) {
