@@ -2,8 +2,8 @@
 
 ctypes wrapper over ``oracle/liboracle.so`` (built from ``gt_oracle.c`` by
 ``oracle/Makefile``).  Only ``tests/``, ``__graft_entry__.smoke()`` and
-``bench.py``'s CPU-baseline leg import this module, and only as the checker
-the HIP path is compared against.  The product package never imports it.
+``bench.py``'s CPU-baseline leg import this module, and only to check
+the HIP path against it.  The product package never imports it.
 
 Functions mirror the reference WGSL/Rust they restate (see gt_oracle.h for
 file:line citations).  Arrays are numpy; frames are (nrows, ncols) float32.

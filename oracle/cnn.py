@@ -1,7 +1,7 @@
 """CPU restatement of the learned centroid proposal -- TEST INFRASTRUCTURE ONLY.
 
-Only ``tests/`` and ``bench.py``'s checking legs import this module, as the
-checker the HIP kernels are compared against; the product never imports it.
+Only ``tests/`` and ``bench.py``'s checking legs import this module, to check
+the HIP kernels against it; the product never imports it.
 
 The reference has no learned path: this restates the network the learned
 proposal is specified as, in numpy.

@@ -3,7 +3,7 @@
  * pipeline.  TEST INFRASTRUCTURE ONLY: the product (libgtrack.so and the
  * gpu_tracking_amd package) never links, loads or calls this code.  Only
  * tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg use it, as
- * the checker the HIP path is compared against.
+ * the result the HIP path is checked against.
  *
  * Every function names the reference file:line it restates.  The reference's
  * arithmetic is WGSL compiled by naga/wgpu; the few places where WGSL leaves
