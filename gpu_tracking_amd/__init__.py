@@ -10,11 +10,11 @@ The package needs the built library; importing the entry points raises when
 it is missing (no CPU fallback).
 """
 from .api import (Batch, CentroidNet, Context, LoG, VideoFile, batch,  # noqa: F401
-                  characterize_points, detections_to_numpy, link, load, make_log_params,
+                  characterize_points, connect, detections_to_numpy, link, load, make_log_params,
                   make_params, mean_from_file, parse_ets, parse_ets_with_keys, to_tiff)
 from ._native import GtError, column_names, default_params  # noqa: F401
 
-__all__ = ["batch", "characterize_points", "LoG", "link", "Context", "Batch", "CentroidNet",
+__all__ = ["batch", "characterize_points", "LoG", "link", "connect", "Context", "Batch", "CentroidNet",
            "VideoFile", "load", "mean_from_file", "parse_ets", "parse_ets_with_keys", "to_tiff",
            "make_params",
            "make_log_params", "GtError",

@@ -89,7 +89,7 @@ EXPORTS = [
     "gt_params_default", "gt_abi_version", "gt_last_error", "gt_context_create",
     "gt_context_destroy", "gt_detect_device", "gt_execute", "gt_output_free",
     "gt_column_names", "gt_filter_close", "gt_doughnut", "gt_set_interrupt", "gt_progress",
-    "gt_set_timing", "gt_stage_times", "gt_prune_blobs", "gt_link", "gt_batch_create",
+    "gt_set_timing", "gt_stage_times", "gt_prune_blobs", "gt_link", "gt_connect", "gt_batch_create",
     "gt_batch_destroy", "gt_batch_detect", "gt_batch_tune", "gt_batch_get_schedule",
     "gt_batch_set_graph", "gt_frame_std", "gt_cnn_create", "gt_cnn_destroy", "gt_cnn_set_params",
     "gt_cnn_get_params", "gt_cnn_adam_steps", "gt_cnn_forward", "gt_cnn_gradient", "gt_cnn_step",
@@ -151,6 +151,9 @@ def lib() -> C.CDLL:
     L.gt_prune_blobs.restype = C.c_int
     L.gt_link.argtypes = [C.c_void_p, C.c_uint64, C.c_float, C.c_uint32, C.c_void_p]
     L.gt_link.restype = C.c_int
+    L.gt_connect.argtypes = [C.c_void_p, C.c_uint64, C.c_void_p, C.c_uint64, C.c_float,
+                             C.c_void_p, C.c_void_p]
+    L.gt_connect.restype = C.c_int
     L.gt_batch_create.argtypes = [P(GtParams), C.c_uint64, C.c_uint64, C.c_uint32, P(C.c_void_p)]
     L.gt_batch_create.restype = C.c_int
     L.gt_batch_destroy.argtypes = [C.c_void_p]

@@ -596,6 +596,47 @@ def link(to_link, search_range, memory=0):
     return ids
 
 
+def _link_rows(x, name):
+    import pandas as pd
+    arr = x[["frame", "y", "x"]].to_numpy() if isinstance(x, pd.DataFrame) else x
+    arr = np.ascontiguousarray(arr, dtype=np.float32)
+    if arr.ndim != 2 or arr.shape[1] != 3:
+        raise ValueError(f"{name}: expected an (N, 3) array of frame, y, x")
+    return arr
+
+
+def connect(to_link1, to_link2, search_range, merge=True):
+    """Links two sets of detections frame by frame (lib.py:108-158), e.g. two
+    channels, or detections against ground truth: each input is a DataFrame
+    with columns frame, y, x (sorted by frame) or an (N, 3) array of the same.
+    Points of set 2 linked to a point of set 1 share its "connect particle"
+    id; the others get ids of their own.  With DataFrames, merge=True returns
+    the outer merge of the two on "connect particle" (columns of set 1
+    suffixed _x, of set 2 _y; NaN where a point has no partner) and
+    merge=False both DataFrames with the column added.  With arrays, the
+    two id arrays are returned (the reference would call DataFrame.merge on
+    them)."""
+    import pandas as pd
+    a1 = _link_rows(to_link1, "to_link1")
+    a2 = _link_rows(to_link2, "to_link2")
+    ids1 = np.zeros(a1.shape[0], np.uint64)
+    ids2 = np.zeros(a2.shape[0], np.uint64)
+    N.check(N.lib().gt_connect(a1.ctypes.data, a1.shape[0], a2.ctypes.data, a2.shape[0],
+                               float(search_range), ids1.ctypes.data, ids2.ctypes.data))
+    frames = [isinstance(x, pd.DataFrame) for x in (to_link1, to_link2)]
+    if not all(frames):
+        out1 = to_link1.assign(**{"connect particle": ids1}) if frames[0] else ids1
+        out2 = to_link2.assign(**{"connect particle": ids2}) if frames[1] else ids2
+        return out1, out2
+    out1 = to_link1.copy()
+    out1["connect particle"] = ids1
+    out2 = to_link2.copy()
+    out2["connect particle"] = ids2
+    if not merge:
+        return out1, out2
+    return out1.merge(out2, how="outer", on="connect particle")
+
+
 class Batch:
     """Batched detection of frames of different sizes (u8/u16, Trackpy style):
     every launch runs one kernel per stage over a table of frames of any

@@ -814,36 +814,104 @@ extern "C" gt_status gt_prune_blobs(gt_result_row *rows, uint64_t n, float overl
     return GT_OK;
 }
 
-extern "C" gt_status gt_link(const float *points, uint64_t n, float search_range, uint32_t memory,
-                             uint64_t *ids) {
-    // linker_all over FrameSubsetter's linking groups (linking.rs:94-196,
-    // :818-831): rows [frame, y, x] sorted by frame; consecutive rows of one
-    // frame value form a frame, the first frame is frame 0 even when empty.
-    if (n && (!points || !ids)) return GT_ERR_INVALID_ARGUMENT;
-    if (!(search_range > 0.0f) || !std::isfinite(search_range)) return GT_ERR_INVALID_ARGUMENT;
-    gt_linker lk(search_range, memory);
-    std::vector<float> xy;
-    std::string err;
+namespace {
+
+// FrameSubsetter's linking groups (linking.rs:94-155) of rows [frame, y, x]:
+// consecutive rows of one frame value form a group; a group's key is its
+// frame value cast to an unsigned integer (Rust's saturating `as usize`);
+// when the first row's frame is above 0 an empty group keyed 0 comes first;
+// no rows, no groups.  Frames must not decrease.
+struct FrameGroup {
+    uint64_t key, begin, end;
+};
+
+uint64_t frame_key(float f) {
+    if (!(f > 0.0f)) return 0;
+    if (f >= 18446744073709551616.0f) return UINT64_MAX;
+    return static_cast<uint64_t>(f);
+}
+
+gt_status frame_groups(const float *points, uint64_t n, std::vector<FrameGroup> *out) {
+    out->clear();
     float cur = 0.0f;
     uint64_t begin = 0;
-    auto flush = [&](uint64_t end) -> gt_status {
-        xy.resize(2 * (end - begin));
-        for (uint64_t i = begin; i < end; ++i) {
-            xy[2 * (i - begin)] = points[3 * i + 1];
-            xy[2 * (i - begin) + 1] = points[3 * i + 2];
-        }
-        return lk.advance(xy.data(), end - begin, ids + begin, nullptr, &err);
-    };
     for (uint64_t i = 0; i < n; ++i) {
         const float fr = points[3 * i];
         if (fr > cur) {
-            gt_status st = flush(i);
-            if (st != GT_OK) return st;
+            out->push_back({frame_key(cur), begin, i});
             begin = i;
             cur = fr;
         }
         if (fr < cur) return GT_ERR_NON_SORTED_CHARACTERIZATION;
     }
-    if (n > begin) return flush(n);
+    if (n > begin) out->push_back({frame_key(cur), begin, n});
+    return GT_OK;
+}
+
+void group_xy(const float *points, const FrameGroup &g, std::vector<float> *xy) {
+    xy->resize(2 * (g.end - g.begin));
+    for (uint64_t i = g.begin; i < g.end; ++i) {
+        (*xy)[2 * (i - g.begin)] = points[3 * i + 1];
+        (*xy)[2 * (i - g.begin) + 1] = points[3 * i + 2];
+    }
+}
+
+}  // namespace
+
+extern "C" gt_status gt_link(const float *points, uint64_t n, float search_range, uint32_t memory,
+                             uint64_t *ids) {
+    // linker_all over FrameSubsetter's linking groups (linking.rs:818-831)
+    if (n && (!points || !ids)) return GT_ERR_INVALID_ARGUMENT;
+    if (!(search_range > 0.0f) || !std::isfinite(search_range)) return GT_ERR_INVALID_ARGUMENT;
+    std::vector<FrameGroup> groups;
+    gt_status st = frame_groups(points, n, &groups);
+    if (st != GT_OK) return st;
+    gt_linker lk(search_range, memory);
+    std::vector<float> xy;
+    std::string err;
+    for (const FrameGroup &g : groups) {
+        group_xy(points, g, &xy);
+        st = lk.advance(xy.data(), g.end - g.begin, ids + g.begin, nullptr, &err);
+        if (st != GT_OK) return st;
+    }
+    return GT_OK;
+}
+
+extern "C" gt_status gt_connect(const float *points1, uint64_t n1, const float *points2,
+                                uint64_t n2, float search_range, uint64_t *ids1, uint64_t *ids2) {
+    // connect_all (linking.rs:960-1012): the two sets' frame groups in
+    // ascending frame order; a frame in both sets is linked set 1 -> set 2
+    // (Linker::connect, :405-432), a frame in one set takes fresh ids
+    // (fill_base).  Where the reference's sync_iterators (:945-955) fills
+    // the frame that overtook the one being caught up to twice, that frame
+    // is filled once, when it is reached.
+    if ((n1 && (!points1 || !ids1)) || (n2 && (!points2 || !ids2))) return GT_ERR_INVALID_ARGUMENT;
+    if (!(search_range > 0.0f) || !std::isfinite(search_range)) return GT_ERR_INVALID_ARGUMENT;
+    std::vector<FrameGroup> g1, g2;
+    gt_status st = frame_groups(points1, n1, &g1);
+    if (st != GT_OK) return st;
+    st = frame_groups(points2, n2, &g2);
+    if (st != GT_OK) return st;
+    gt_linker lk(search_range, 0);
+    std::vector<float> xy1, xy2;
+    std::string err;
+    size_t i = 0, j = 0;
+    while (i < g1.size() || j < g2.size()) {
+        if (j == g2.size() || (i < g1.size() && g1[i].key < g2[j].key)) {
+            lk.fresh(g1[i].end - g1[i].begin, ids1 + g1[i].begin);
+            ++i;
+        } else if (i == g1.size() || g2[j].key < g1[i].key) {
+            lk.fresh(g2[j].end - g2[j].begin, ids2 + g2[j].begin);
+            ++j;
+        } else {
+            group_xy(points1, g1[i], &xy1);
+            group_xy(points2, g2[j], &xy2);
+            st = lk.connect(xy1.data(), g1[i].end - g1[i].begin, xy2.data(),
+                            g2[j].end - g2[j].begin, ids1 + g1[i].begin, ids2 + g2[j].begin, &err);
+            if (st != GT_OK) return st;
+            ++i;
+            ++j;
+        }
+    }
     return GT_OK;
 }

@@ -236,6 +236,21 @@ gt_status gt_linker::advance(const float *xy, size_t n, uint64_t *ids,
     return GT_OK;
 }
 
+gt_status gt_linker::connect(const float *xy1, size_t n1, const float *xy2, size_t n2,
+                             uint64_t *ids1, uint64_t *ids2, std::string *err) {
+    prev_.resize(n1);
+    for (size_t i = 0; i < n1; ++i) {
+        ids1[i] = part_idx_++;
+        prev_[i] = {xy1[2 * i], xy1[2 * i + 1], ids1[i]};
+    }
+    dest_.resize(n2);
+    for (size_t i = 0; i < n2; ++i) dest_[i] = {xy2[2 * i], xy2[2 * i + 1], 0};
+    std::vector<Entry> unused;
+    gt_status st = link(static_cast<size_t>(-1), ids2, &unused, nullptr, err);
+    prev_.clear();
+    return st;
+}
+
 std::vector<gt_linker::Book> gt_linker::finish() {
     std::vector<Book> books = books_;
     const size_t L = memvec_.size();

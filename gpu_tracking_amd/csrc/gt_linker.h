@@ -30,6 +30,16 @@ class gt_linker {
                       std::string *err);
     // Durations of every particle id so far (Linker::finish, :500-519).
     std::vector<Book> finish();
+    // Links two point sets of one frame (Linker::connect, :405-432): set 1's
+    // points take fresh ids in order (ids1), set 2's points the id of the
+    // set-1 point they link to or a fresh one (ids2).  Leaves the frame
+    // history alone.
+    gt_status connect(const float *xy1, size_t n1, const float *xy2, size_t n2, uint64_t *ids1,
+                      uint64_t *ids2, std::string *err);
+    // Fresh ids for n points of a frame with no partner (fill_base, :914-919).
+    void fresh(size_t n, uint64_t *ids) {
+        for (size_t i = 0; i < n; ++i) ids[i] = part_idx_++;
+    }
 
   private:
     gt_status link(size_t memory_start, uint64_t *ids, std::vector<Entry> *unused,

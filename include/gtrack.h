@@ -282,6 +282,23 @@ gt_status gt_prune_blobs(gt_result_row *rows, uint64_t n, float overlap_threshol
 gt_status gt_link(const float *points, uint64_t n, float search_range, uint32_t memory,
                   uint64_t *ids);
 
+/* Links two sets of detections frame by frame (connect_all,
+ * src/linking.rs:960-1012, bound as connect_rust at
+ * gpu_tracking_python/src/python_bindings.rs:293-311): points1 (n1 rows)
+ * and points2 (n2 rows) are [frame, y, x] rows, each sorted by frame and
+ * grouped as gt_link groups them.  In ascending frame order, a frame present
+ * in both sets is linked set 1 -> set 2 with memory 0 (Linker::connect,
+ * :405-432): set 1's points take fresh ids in row order, set 2's points the
+ * id of the set-1 point they link to, else a fresh id; a frame present in
+ * one set only takes fresh ids (fill_base, :914-919).  ids1[i] / ids2[i]
+ * receive the ids; one id counter serves both sets.  Where the reference's
+ * sync_iterators (:945-955) gives ids twice to a frame of the lagging set
+ * that skips past the frame it was catching up to (its id list then comes
+ * out longer than its rows), that frame takes its ids once, when reached.
+ * Errors as gt_link.  Host-only. */
+gt_status gt_connect(const float *points1, uint64_t n1, const float *points2, uint64_t n2,
+                     float search_range, uint64_t *ids1, uint64_t *ids2);
+
 /* ---- Batched detection over frames of different sizes ------------------
  * No counterpart in the reference: one GpuState serves one frame size
  * (src/execute_gpu.rs:1074-1080).  A batch context holds the parameters

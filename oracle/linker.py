@@ -201,3 +201,101 @@ def link_frames(points, search_range, memory):
     if group:
         out.extend(lk.advance(group))
     return out
+
+
+def frame_groups(points):
+    """FrameSubsetter's linking groups (linking.rs:94-155) of a (n, 3) float
+    array [frame, y, x]: consecutive rows of one frame value form a group,
+    keyed by the frame value cast to an unsigned integer; when the first row's
+    frame is above 0 an empty group keyed 0 comes first; no rows, no groups."""
+    pts = np.asarray(points, np.float32).reshape(-1, 3)
+    groups = []
+    cur = f32(0.0)
+    group = []
+    for row in pts:
+        fr = row[0]
+        if fr > cur:
+            groups.append((int(cur), group))
+            group = []
+            cur = fr
+        if fr < cur:
+            raise ValueError("not sorted by frame")
+        group.append((row[1], row[2]))
+    if group:
+        groups.append((int(cur), group))
+    return groups
+
+
+def connect_frames(points1, points2, search_range, fixed=True):
+    """connect_all (linking.rs:960-1012) with Linker::connect (:405-432),
+    fill_base and sync_iterators (:833-958): frames present in both sets are
+    linked (set 1's points take fresh ids, set 2's points the id of the set-1
+    point they link to or a fresh one); a frame present in one set only takes
+    fresh ids.  Ids count up in the order the frames are visited.
+
+    fixed=False follows sync_iterators literally: when the lagging set skips
+    past the frame it was catching up to (Ordering::Less, :945-955), the
+    overtaking frame's points get ids there and again when that frame is
+    later linked or filled, so that set's id list comes out longer than its
+    rows.  fixed=True (the product's behaviour) gives the skipped frame its
+    ids and carries the overtaking frame on, unfilled.
+    Returns (ids1, ids2) as lists."""
+    it = [iter(frame_groups(points1)), iter(frame_groups(points2))]
+    res = [[], []]
+    counter = [0]
+    r = f32(search_range)
+
+    def fill(side, item):
+        for _ in item[1]:
+            res[side].append(counter[0])
+            counter[0] += 1
+
+    def connect(a, b):
+        src = []
+        for p in a[1]:
+            src.append(((f32(p[0]), f32(p[1])), counter[0]))
+            counter[0] += 1
+        dest = [(f32(p[0]), f32(p[1])) for p in b[1]]
+        ids, _, counter[0] = link(src, dest, None, r, counter[0])
+        res[0].extend(e[1] for e in src)
+        res[1].extend(ids)
+
+    def sync(hit_side, hit):
+        # hit: the frame ahead, on side hit_side; the other side catches up
+        while True:
+            nxt = next(it[1 - hit_side], None)
+            if nxt is None:
+                fill(hit_side, hit)
+                return None
+            if hit[0] > nxt[0]:
+                fill(1 - hit_side, nxt)
+                continue
+            if hit[0] == nxt[0]:
+                return (hit, nxt) if hit_side == 0 else (nxt, hit)
+            if not fixed:
+                fill(1 - hit_side, nxt)
+            fill(hit_side, hit)
+            hit_side, hit = 1 - hit_side, nxt
+
+    while True:
+        a = next(it[0], None)
+        b = next(it[1], None)
+        if a is None and b is None:
+            break
+        if b is None:
+            fill(0, a)
+            continue
+        if a is None:
+            fill(1, b)
+            continue
+        if a[0] == b[0]:
+            pair = (a, b)
+        elif a[0] > b[0]:
+            fill(1, b)
+            pair = sync(0, a)
+        else:
+            fill(0, a)
+            pair = sync(1, b)
+        if pair is not None:
+            connect(*pair)
+    return res[0], res[1]

@@ -155,3 +155,100 @@ def test_gt_link_groups_and_errors():
     with pytest.raises(N.GtError) as e:
         gt.link(dense, 5.0)
     assert e.value.status == 18
+
+
+def _connect_sets(rng, nframes, p_frame, n, box=30.0):
+    """Two point sets over frames 0..nframes-1 (each frame present in a set
+    with probability p_frame), set 2 a jittered, thinned copy of set 1 plus
+    extra points where the frame is in both."""
+    s1, s2 = [], []
+    for f in range(nframes):
+        pts = rng.uniform(0, box, (rng.integers(0, n + 1), 2)).astype(np.float32)
+        in1, in2 = rng.random() < p_frame, rng.random() < p_frame
+        if in1:
+            s1 += [(f, y, x) for y, x in pts]
+        if in2:
+            keep = pts[rng.random(len(pts)) < 0.8] + rng.normal(0, 0.6, (1, 2)).astype(np.float32)
+            extra = rng.uniform(0, box, (rng.integers(0, 3), 2))
+            s2 += [(f, y, x) for y, x in np.concatenate([keep, extra])]
+    return (np.array(s1, np.float32).reshape(-1, 3), np.array(s2, np.float32).reshape(-1, 3))
+
+
+def test_oracle_connect_literal_double_fill():
+    # set 2 skips frame 2 and overtakes it with frame 3: the literal
+    # sync_iterators gives frame 3's point two ids (4 ids for 3 rows)
+    a = np.array([[0, 1, 1], [2, 5, 5]], np.float32)
+    b = np.array([[0, 1.2, 1], [1, 3, 3], [3, 7, 7]], np.float32)
+    i1, i2 = OL.connect_frames(a, b, 2.0, fixed=False)
+    assert len(i1) == 2 and len(i2) == 4
+    assert OL.connect_frames(a, b, 2.0) == ([0, 2], [0, 1, 3])
+
+
+@pytest.mark.parametrize("seed", range(16))
+def test_oracle_connect_pinned(seed):
+    rng = np.random.default_rng(100 + seed)
+    r = 2.0
+    s1, s2 = _connect_sets(rng, int(rng.integers(1, 7)), 0.7, 4)
+    i1, i2 = OL.connect_frames(s1, s2, r)
+    lit1, lit2 = OL.connect_frames(s1, s2, r, fixed=False)
+    assert len(i1) == len(s1) and len(i2) == len(s2)
+    if len(lit1) == len(s1) and len(lit2) == len(s2):
+        # where the literal restatement is well formed the two agree
+        assert (lit1, lit2) == (i1, i2)
+    assert len(set(i1)) == len(i1) and len(set(i2)) == len(i2)
+    # every id is handed out once, counting up with no gaps
+    assert sorted(set(i1) | set(i2)) == list(range(len(set(i1) | set(i2))))
+    r2 = float(np.float32(np.float32(r) * np.float32(r)))
+    pos1 = {i: row for i, row in zip(i1, s1)}
+    for f in sorted(set(s2[:, 0].astype(int))):
+        d = [tuple(row[1:]) for row in s2 if int(row[0]) == f]
+        ids = [i for i, row in zip(i2, s2) if int(row[0]) == f]
+        src_rows = [row for row in s1 if int(row[0]) == f]
+        src = [((row[1], row[2]), i) for i, row in zip(i1, s1) if int(row[0]) == f]
+        for i, q in zip(ids, d):
+            if i in pos1:   # linked: same frame, within the search range
+                p = pos1[i]
+                assert int(p[0]) == f and OL._d2(q, (p[1], p[2])) < np.float32(r2)
+        if src_rows and d and len(src) <= 4 and len(d) <= 4:
+            # set 1 is the source side, set 2 the destinations: the linked
+            # pairs are a minimum-cost assignment
+            idmap = {e[1]: k for k, e in enumerate(src)}
+            assign = [idmap.get(i) for i in ids]
+            assert abs(_cost(src, d, assign, r2) - _brute(src, d, r)) < 1e-4
+
+
+@pytest.mark.parametrize("seed", range(10))
+def test_gt_connect_matches_oracle(seed):
+    from gpu_tracking_amd import _native as N
+    rng = np.random.default_rng(300 + seed)
+    s1, s2 = _connect_sets(rng, int(rng.integers(1, 12)), [1.0, 0.8, 0.5][seed % 3], 12)
+    if seed == 9:
+        s2 = s2[:0]
+    ref1, ref2 = OL.connect_frames(s1, s2, 2.5)
+    ids1 = np.zeros(len(s1), np.uint64)
+    ids2 = np.zeros(len(s2), np.uint64)
+    N.check(N.lib().gt_connect(s1.ctypes.data, len(s1), s2.ctypes.data, len(s2), 2.5,
+                               ids1.ctypes.data, ids2.ctypes.data))
+    assert ids1.tolist() == ref1 and ids2.tolist() == ref2
+
+
+def test_connect_wrapper():
+    import gpu_tracking_amd as gt
+    import pandas as pd
+    from gpu_tracking_amd import _native as N
+    a = pd.DataFrame({"frame": [0, 0, 2], "y": [1.0, 10.0, 5.0], "x": [1.0, 1.0, 5.0],
+                      "mass": [5.0, 6.0, 7.0]})
+    b = pd.DataFrame({"frame": [0, 1, 2, 2], "y": [1.3, 3.0, 5.2, 20.0],
+                      "x": [1.0, 3.0, 5.0, 1.0], "mass": [1.0, 2.0, 3.0, 4.0]})
+    o1, o2 = gt.connect(a, b, 1.0, merge=False)
+    assert list(o1["connect particle"]) == [0, 1, 3]
+    assert list(o2["connect particle"]) == [0, 2, 3, 4]
+    m = gt.connect(a, b, 1.0)
+    assert len(m) == 5 and {"frame_x", "frame_y", "mass_x", "mass_y"} <= set(m.columns)
+    both = m.dropna()
+    assert sorted(both["mass_x"]) == [5.0, 7.0] and sorted(both["mass_y"]) == [1.0, 3.0]
+    i1, i2 = gt.connect(a[["frame", "y", "x"]].to_numpy(), b, 1.0)
+    assert i1.tolist() == [0, 1, 3] and list(i2["connect particle"]) == [0, 2, 3, 4]
+    with pytest.raises(N.GtError) as e:
+        gt.connect(np.array([[1, 0, 0], [0, 1, 1]], np.float32), b, 1.0)
+    assert e.value.status == 6
