@@ -214,7 +214,41 @@ __device__ __forceinline__ void bandpass_int_tile(const T *__restrict__ f, float
     const int tid = threadIdx.x;
     constexpr int PAIRS = (B2_TW + 2 * R + 1) / 2;  // u16 pairs per staged row
     static_assert(2 * PAIRS <= RWP, "pairs fit the staged row");
-    if (sizeof(T) == 2 && (R % 2) == 0 && (W % 2) == 0 &&
+    constexpr int QR = (B2_TW + 2 * R) / 4;         // u16 quads per staged row (R % 4 == 0)
+    constexpr int QROWS = B2_THREADS / QR;          // staged rows per pass of quad loads
+    constexpr int QPASS = (RH + QROWS - 1) / QROWS;
+    if (sizeof(T) == 2 && (R % 4) == 0 && (W % 4) == 0 &&
+        (reinterpret_cast<uintptr_t>(f) & 7u) == 0 && y0 - R >= 0 && y0 + B2_TH + R <= H &&
+        x0 - R >= 0 && x0 + B2_TW + R <= W) {
+        // interior tile of a u16 frame whose staged region starts on a
+        // multiple of 4 columns: each thread loads 8-byte quads at one
+        // 32-bit offset from a per-pass tile row pointer
+        static_assert(R % 4 != 0 || 4 * QR == RWP, "quads fill the staged row");
+        const char *tb = reinterpret_cast<const char *>(f + (static_cast<size_t>(y0 - R) * W +
+                                                             static_cast<size_t>(x0 - R)));
+        const int rq = tid / QR, cq = tid - rq * QR;
+        const uint32_t voff = static_cast<uint32_t>(rq) * (2u * static_cast<uint32_t>(W)) +
+                              8u * static_cast<uint32_t>(cq);
+        const bool active = tid < QROWS * QR;
+        uint2 vals[QPASS];
+#pragma unroll
+        for (int q = 0; q < QPASS; ++q) {
+            const char *tq = tb + static_cast<size_t>(q * QROWS) * (2u * static_cast<uint32_t>(W));
+            if (active && rq + q * QROWS < RH)
+                vals[q] = *reinterpret_cast<const uint2 *>(tq + voff);
+        }
+#pragma unroll
+        for (int q = 0; q < QPASS; ++q) {
+            if (active && rq + q * QROWS < RH) {
+                float4 v4;
+                v4.x = static_cast<float>(vals[q].x & 0xFFFFu);
+                v4.y = static_cast<float>(vals[q].x >> 16);
+                v4.z = static_cast<float>(vals[q].y & 0xFFFFu);
+                v4.w = static_cast<float>(vals[q].y >> 16);
+                *reinterpret_cast<float4 *>(Sraw + (rq + q * QROWS) * RWP + 4 * cq) = v4;
+            }
+        }
+    } else if (sizeof(T) == 2 && (R % 2) == 0 && (W % 2) == 0 &&
         (reinterpret_cast<uintptr_t>(f) & 3u) == 0 && y0 - R >= 0 && y0 + B2_TH + R <= H &&
         x0 - R >= 0 && x0 - R + 2 * PAIRS <= W) {
         // interior tile of a u16 frame: the staged region is whole in the
@@ -261,6 +295,8 @@ __device__ __forceinline__ void bandpass_int_tile(const T *__restrict__ f, float
     }
     __syncthreads();
     const float gnorm = p.gnorm, rgnorm = p.rgnorm;
+    // every column the tile's row pass reads lies in the frame
+    const bool xin = x0 - R >= 0 && x0 + B2_TW + R <= W;
     // Row pass: 4 adjacent columns of one staged row per item.
     for (int item = tid; item < RH * (B2_TW / 4); item += B2_THREADS) {
         const int ry = item / (B2_TW / 4), cx = (item % (B2_TW / 4)) * 4;
@@ -276,7 +312,12 @@ __device__ __forceinline__ void bandpass_int_tile(const T *__restrict__ f, float
         }
         float gs[4] = {0.0f, 0.0f, 0.0f, 0.0f};
         const int gx0 = x0 + cx - R;   // frame column of v[0]
-        if (gx0 >= 0 && gx0 + 3 + 2 * R < W) {
+        if (xin) {
+#pragma unroll
+            for (int t = 0; t < NT; ++t)
+#pragma unroll
+                for (int k = 0; k < 4; ++k) gs[k] = fmaf(p.w[t], v[k + t], gs[k]);
+        } else if (gx0 >= 0 && gx0 + 3 + 2 * R < W) {
 #pragma unroll
             for (int t = 0; t < NT; ++t)
 #pragma unroll
