@@ -297,9 +297,18 @@ __device__ __forceinline__ void bandpass_int_tile(const T *__restrict__ f, float
     const float gnorm = p.gnorm, rgnorm = p.rgnorm;
     // every column the tile's row pass reads lies in the frame
     const bool xin = x0 - R >= 0 && x0 + B2_TW + R <= W;
-    // Row pass: 4 adjacent columns of one staged row per item.
-    for (int item = tid; item < RH * (B2_TW / 4); item += B2_THREADS) {
-        const int ry = item / (B2_TW / 4), cx = (item % (B2_TW / 4)) * 4;
+    // Row pass: 4 adjacent columns of one staged row per item; item
+    // tid + q * B2_THREADS is row ry0 + q * RSTEP, columns cx..cx+3, so a
+    // thread's items differ by constant LDS offsets.
+    constexpr int IPR = B2_TW / 4;                  // items per staged row
+    constexpr int RSTEP = B2_THREADS / IPR;         // staged rows per pass
+    static_assert(B2_THREADS % IPR == 0, "whole rows per pass");
+    constexpr int RPASS = (RH + RSTEP - 1) / RSTEP;
+    const int ry0 = tid / IPR, cxi = (tid % IPR) * 4;
+#pragma unroll
+    for (int q = 0; q < RPASS; ++q) {
+        const int ry = ry0 + q * RSTEP, cx = cxi;
+        if (ry >= RH) continue;
         const float4 *src4 = reinterpret_cast<const float4 *>(Sraw + ry * RWP + cx);
         float v[4 * NV];
 #pragma unroll
@@ -350,8 +359,11 @@ __device__ __forceinline__ void bandpass_int_tile(const T *__restrict__ f, float
         *reinterpret_cast<float4 *>(Sc + ry * B2_TW + cx) = oc;
     }
     __syncthreads();
-    // Column pass: B2_ROWS consecutive rows of one column per thread.
-    const int cx = tid % B2_TW, r0 = (tid / B2_TW) * B2_ROWS;
+    // Column pass: B2_ROWS consecutive rows of one column per thread (the
+    // rows of a wavefront are wave-uniform: B2_TW is one wavefront wide).
+    static_assert(B2_TW == 64, "one wavefront per row group");
+    const int cx = tid % B2_TW;
+    const int r0 = __builtin_amdgcn_readfirstlane(tid / B2_TW) * B2_ROWS;
     float g[B2_ROWS + 2 * R], c[B2_ROWS + 2 * R];
 #pragma unroll
     for (int t = 0; t < B2_ROWS + 2 * R; ++t) {
@@ -366,6 +378,21 @@ __device__ __forceinline__ void bandpass_int_tile(const T *__restrict__ f, float
 #pragma unroll
     for (int t = 0; t < NT; ++t) csum = csum + c[t];
     float *orow = out + static_cast<size_t>(gy0) * W + gx;
+    if (inner && x0 + B2_TW <= W) {
+        // every tap and every output of the wave's rows lies in the frame
+#pragma unroll
+        for (int r = 0; r < B2_ROWS; ++r) {
+            if (r > 0) csum = csum - c[r - 1] + c[r - 1 + NT];
+            float gs = 0.0f;
+#pragma unroll
+            for (int t = 0; t < NT; ++t) gs = fmaf(p.w[t], g[r + t], gs);
+            float res = div_const(gs, gnorm, rgnorm) - div_const(csum, box, rbox);
+            if (p.truncate && res < kTrunc) res = 0.0f;
+            *orow = res;
+            orow += W;
+        }
+        return;
+    }
     const bool rows_in = gy0 + B2_ROWS <= H && gx < W;
 #pragma unroll
     for (int r = 0; r < B2_ROWS; ++r) {
