@@ -967,7 +967,31 @@ __device__ __forceinline__ void detect_tile_body(
 
     // 1. staging (the pitch padding is loaded too; it is never compared)
     const bool interior = oy >= 0 && ox >= 0 && oy + RH <= H && ox + RW <= W;
-    if constexpr (kFixed) {
+    constexpr int Q4 = gc.pitch / 4;              // float4 per staged row
+    constexpr int QROWS = FT_THREADS / Q4;        // staged rows per pass of float4 loads
+    constexpr int QP = (gc.rows + QROWS - 1) / QROWS;
+    static_assert(gc.pitch % 4 == 0, "staged rows are whole float4s");
+    if (kFixed && interior && (W & 3) == 0 && (reinterpret_cast<uintptr_t>(P) & 15u) == 0) {
+        // every staged row is 16-byte aligned in the frame: float4 loads at
+        // one 32-bit offset per thread from a per-pass tile row pointer
+        const float *tb = P + (static_cast<size_t>(oy) * W + static_cast<size_t>(ox));
+        const int rq = tid / Q4, cq = tid - rq * Q4;
+        const bool active = tid < QROWS * Q4;
+        float4 v4[QP];
+#pragma unroll
+        for (int q = 0; q < QP; ++q) {
+            // rows past the staged region (not stored) load its last row:
+            // every load stays in the tile and none is predicated
+            const int ry = min(rq + q * QROWS, gc.rows - 1);
+            const uint32_t off = static_cast<uint32_t>(ry) * static_cast<uint32_t>(W) +
+                                 4u * static_cast<uint32_t>(cq);
+            v4[q] = *reinterpret_cast<const float4 *>(tb + off);
+        }
+#pragma unroll
+        for (int q = 0; q < QP; ++q)
+            if (active && rq + q * QROWS < gc.rows)
+                *reinterpret_cast<float4 *>(Sp + (rq + q * QROWS) * gc.pitch + 4 * cq) = v4[q];
+    } else if constexpr (kFixed) {
         constexpr int CN = gc.rows * gc.pitch;
         constexpr int NL = (CN + FT_THREADS - 1) / FT_THREADS;
         float vals[NL];
