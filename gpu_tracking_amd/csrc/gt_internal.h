@@ -14,8 +14,15 @@
 
 // Kernel launchers (gt_kernels.hip).  All are asynchronous on `s`.
 hipError_t gt_launch_zero_u32(uint32_t *p, size_t n, hipStream_t s);
+// Up to two u32 arrays a launch clears before any later launch on its stream
+// reads them (counters cleared by the bandpass launch instead of launches of
+// their own).
+struct GtZero {
+    uint32_t *p[2];
+    uint32_t n[2];
+};
 hipError_t gt_launch_bandpass(const void *frames, int dtype, float *proc, uint32_t nframes,
-                              const GtKParams &p, hipStream_t s);
+                              const GtKParams &p, hipStream_t s, const GtZero *zero = nullptr);
 hipError_t gt_launch_localmax(const float *proc, uint32_t nframes, const GtKParams &p,
                               uint32_t *cand, uint32_t cand_cap, uint32_t *cand_count,
                               hipStream_t s);

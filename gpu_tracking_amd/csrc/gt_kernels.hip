@@ -418,8 +418,15 @@ __device__ __forceinline__ void bandpass_int_tile(const T *__restrict__ f, float
 template <typename T, int R>
 __global__ __launch_bounds__(B2_THREADS) void k_bandpass_int(const T *__restrict__ frames,
                                                              float *__restrict__ proc,
-                                                             GtKParams p, int nframes) {
+                                                             GtKParams p, int nframes,
+                                                             GtZero zero) {
     __shared__ __align__(16) BpLds<R> L;
+    if (blockIdx.x == 0) {
+        // the counters of the chunk's later stages
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+            for (uint32_t i = threadIdx.x; i < zero.n[a]; i += B2_THREADS) zero.p[a][i] = 0u;
+    }
     const int H = p.H, W = p.W;
     const size_t HW = static_cast<size_t>(H) * W;
     const int ntx = (W + B2_TW - 1) / B2_TW, nty = (H + B2_TH - 1) / B2_TH;
@@ -1562,7 +1569,7 @@ __global__ void k_zero_u32(uint32_t *p, size_t n) {
 
 template <typename T>
 hipError_t launch_bandpass_t(const T *frames, float *proc, uint32_t nframes, const GtKParams &p,
-                             hipStream_t s) {
+                             hipStream_t s, const GtZero &z) {
     if constexpr (sizeof(T) <= 2) {
         if (p.fast_div && p.R >= 1 && p.R <= 7) {
             const size_t ntiles = static_cast<size_t>((p.W + B2_TW - 1) / B2_TW) *
@@ -1570,17 +1577,19 @@ hipError_t launch_bandpass_t(const T *frames, float *proc, uint32_t nframes, con
             const dim3 g2(xcd_grid(ntiles));
             const int nf = static_cast<int>(nframes);
             switch (p.R) {
-            case 1: k_bandpass_int<T, 1><<<g2, B2_THREADS, 0, s>>>(frames, proc, p, nf); break;
-            case 2: k_bandpass_int<T, 2><<<g2, B2_THREADS, 0, s>>>(frames, proc, p, nf); break;
-            case 3: k_bandpass_int<T, 3><<<g2, B2_THREADS, 0, s>>>(frames, proc, p, nf); break;
-            case 4: k_bandpass_int<T, 4><<<g2, B2_THREADS, 0, s>>>(frames, proc, p, nf); break;
-            case 5: k_bandpass_int<T, 5><<<g2, B2_THREADS, 0, s>>>(frames, proc, p, nf); break;
-            case 6: k_bandpass_int<T, 6><<<g2, B2_THREADS, 0, s>>>(frames, proc, p, nf); break;
-            default: k_bandpass_int<T, 7><<<g2, B2_THREADS, 0, s>>>(frames, proc, p, nf); break;
+            case 1: k_bandpass_int<T, 1><<<g2, B2_THREADS, 0, s>>>(frames, proc, p, nf, z); break;
+            case 2: k_bandpass_int<T, 2><<<g2, B2_THREADS, 0, s>>>(frames, proc, p, nf, z); break;
+            case 3: k_bandpass_int<T, 3><<<g2, B2_THREADS, 0, s>>>(frames, proc, p, nf, z); break;
+            case 4: k_bandpass_int<T, 4><<<g2, B2_THREADS, 0, s>>>(frames, proc, p, nf, z); break;
+            case 5: k_bandpass_int<T, 5><<<g2, B2_THREADS, 0, s>>>(frames, proc, p, nf, z); break;
+            case 6: k_bandpass_int<T, 6><<<g2, B2_THREADS, 0, s>>>(frames, proc, p, nf, z); break;
+            default: k_bandpass_int<T, 7><<<g2, B2_THREADS, 0, s>>>(frames, proc, p, nf, z); break;
             }
             return hipGetLastError();
         }
     }
+    for (int a = 0; a < 2; ++a)
+        if (z.n[a]) k_zero_u32<<<(z.n[a] + 255) / 256, 256, 0, s>>>(z.p[a], z.n[a]);
     const dim3 grid((p.W + BP_TW - 1) / BP_TW, (p.H + BP_TH - 1) / BP_TH, nframes);
     const int RH = BP_TH + 2 * p.R, RW = BP_TW + 2 * p.R;
     const size_t lds = sizeof(float) * (static_cast<size_t>(RH) * RW + 2u * RH * BP_TW);
@@ -1635,11 +1644,12 @@ hipError_t gt_launch_zero_u32(uint32_t *p, size_t n, hipStream_t s) {
 }
 
 hipError_t gt_launch_bandpass(const void *frames, int dtype, float *proc, uint32_t nframes,
-                              const GtKParams &p, hipStream_t s) {
+                              const GtKParams &p, hipStream_t s, const GtZero *zero) {
+    const GtZero z = zero ? *zero : GtZero{{nullptr, nullptr}, {0u, 0u}};
     switch (dtype) {
-    case GT_U16: return launch_bandpass_t(static_cast<const uint16_t *>(frames), proc, nframes, p, s);
-    case GT_F32: return launch_bandpass_t(static_cast<const float *>(frames), proc, nframes, p, s);
-    case GT_U8: return launch_bandpass_t(static_cast<const uint8_t *>(frames), proc, nframes, p, s);
+    case GT_U16: return launch_bandpass_t(static_cast<const uint16_t *>(frames), proc, nframes, p, s, z);
+    case GT_F32: return launch_bandpass_t(static_cast<const float *>(frames), proc, nframes, p, s, z);
+    case GT_U8: return launch_bandpass_t(static_cast<const uint8_t *>(frames), proc, nframes, p, s, z);
     default: return hipErrorInvalidValue;
     }
 }

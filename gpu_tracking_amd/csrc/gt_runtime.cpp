@@ -459,13 +459,19 @@ gt_status run_pipeline(gt_context *c, const void *d_frames, int32_t dtype, uint3
             fdtype = GT_F32;
         }
         hipEvent_t *ev = c->timing ? &c->events[4 * ci] : nullptr;
-        GT_HIP(gt_launch_zero_u32(ocount, static_cast<size_t>(nf) * GT_COUNTER_STRIDE, cs));
+        // the plain tiled path's counters (rows, handed-over walks) are
+        // cleared by the bandpass launch
+        const bool plain_tiled = c->tiled && !c->cnn && !c->log;
+        const uint32_t ncnt = nf * GT_COUNTER_STRIDE;
+        const GtZero zero = {{ocount, scount}, {ncnt, ncnt}};
+        if (!plain_tiled)
+            GT_HIP(gt_launch_zero_u32(ocount, static_cast<size_t>(nf) * GT_COUNTER_STRIDE, cs));
         if (!c->tiled || c->log)
             GT_HIP(gt_launch_zero_u32(c->d_ccount, static_cast<size_t>(nf) * GT_COUNTER_STRIDE, cs));
         if (ev) GT_HIP(hipEventRecord(ev[0], cs));
         if (c->der.need_std)
             GT_HIP(gt_launch_frame_std(frames, fdtype, nf, HW, c->d_std_tmp, c->d_mean, c->d_std, cs));
-        GT_HIP(gt_launch_bandpass(frames, fdtype, proc, nf, c->kp, cs));
+        GT_HIP(gt_launch_bandpass(frames, fdtype, proc, nf, c->kp, cs, plain_tiled ? &zero : nullptr));
         if (ev) GT_HIP(hipEventRecord(ev[1], cs));
         gt_detection *out = d_out + static_cast<size_t>(f0) * cap;
         if (c->log) {
@@ -536,7 +542,6 @@ gt_status run_pipeline(gt_context *c, const void *d_frames, int32_t dtype, uint3
             // wavefronts of detections instead of a few lanes per tile)
             GtKParams kw = c->kp;
             kw.characterize = 0;
-            GT_HIP(gt_launch_zero_u32(scount, static_cast<size_t>(nf) * GT_COUNTER_STRIDE, cs));
             GT_HIP(gt_launch_detect_tile(proc, frames, fdtype, nf, kw, out, cap,
                                          ocount, slow, c->slow_cap, scount, f0, cs));
             if (ev) GT_HIP(hipEventRecord(ev[2], cs));
