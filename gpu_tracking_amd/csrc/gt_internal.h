@@ -51,10 +51,13 @@ hipError_t gt_launch_walk_slow(const float *proc, const void *raw, int dtype, ui
 hipError_t gt_launch_copy_counts(const uint32_t *strided, uint32_t *dst, uint32_t n,
                                  hipStream_t s);
 // Fills the characterize columns of every detection of the chunk (rows of
-// frame f at out[f * out_cap], count at ocount[f * GT_COUNTER_STRIDE]).
+// frame f at out[f * out_cap], count at ocount[f * GT_COUNTER_STRIDE]).  With
+// dcount, also copies the counts there (dcount[f] = ocount[f * stride]), as
+// gt_launch_copy_counts does.
 hipError_t gt_launch_characterize_det(const float *proc, const void *raw, int dtype,
                                       uint32_t nframes, const GtKParams &p, gt_detection *out,
-                                      uint32_t out_cap, const uint32_t *ocount, hipStream_t s);
+                                      uint32_t out_cap, const uint32_t *ocount, hipStream_t s,
+                                      uint32_t *dcount = nullptr);
 // LoG style: scale-space maxima of every scale of a chunk (planes kept in
 // LDS), then their walks.  Candidate seeds are scale * H * W + raster index.
 size_t gt_log_lds_bytes(int Rm);

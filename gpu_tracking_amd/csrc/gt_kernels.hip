@@ -1337,8 +1337,10 @@ __global__ __launch_bounds__(WK_THREADS) void k_characterize_det(const float *__
                                                                  GtKParams p,
                                                                  gt_detection *__restrict__ out,
                                                                  uint32_t out_cap,
-                                                                 const uint32_t *__restrict__ ocount) {
+                                                                 const uint32_t *__restrict__ ocount,
+                                                                 uint32_t *__restrict__ dcount) {
     const int fr = blockIdx.y;
+    if (dcount && blockIdx.x == 0 && threadIdx.x == 0) dcount[fr] = ocount[fr * GT_COUNTER_STRIDE];
     const int H = p.H, W = p.W;
     const size_t HW = static_cast<size_t>(H) * W;
     const GlobalSrc src{proc + static_cast<size_t>(fr) * HW, W};
@@ -2476,7 +2478,8 @@ hipError_t gt_init_tables() {
 
 hipError_t gt_launch_characterize_det(const float *proc, const void *raw, int dtype,
                                       uint32_t nframes, const GtKParams &p, gt_detection *out,
-                                      uint32_t out_cap, const uint32_t *ocount, hipStream_t s) {
+                                      uint32_t out_cap, const uint32_t *ocount, hipStream_t s,
+                                      uint32_t *dcount) {
     // about one lane per expected detection (one per ~1000 pixels)
     const size_t hw = static_cast<size_t>(p.H) * p.W;
     uint32_t blocks = static_cast<uint32_t>(hw / 262144);
@@ -2486,10 +2489,10 @@ hipError_t gt_launch_characterize_det(const float *proc, const void *raw, int dt
     do {                                                                                       \
         if (p.rint == 4)                                                                       \
             k_characterize_det<TT, 4><<<grid, WK_THREADS, 0, s>>>(                             \
-                proc, static_cast<const TT *>(raw), p, out, out_cap, ocount);                  \
+                proc, static_cast<const TT *>(raw), p, out, out_cap, ocount, dcount);          \
         else                                                                                   \
             k_characterize_det<TT, 0><<<grid, WK_THREADS, 0, s>>>(                             \
-                proc, static_cast<const TT *>(raw), p, out, out_cap, ocount);                  \
+                proc, static_cast<const TT *>(raw), p, out, out_cap, ocount, dcount);          \
     } while (0)
     switch (dtype) {
     case GT_U16: GT_CHAR(uint16_t); break;

@@ -551,9 +551,13 @@ gt_status run_pipeline(gt_context *c, const void *d_frames, int32_t dtype, uint3
                 gt_status ast = run_adaptive(c, frames, fdtype, nf, out, cap, cs);
                 if (ast != GT_OK) return ast;
             }
-            if (c->kp.characterize)
+            if (c->kp.characterize) {
+                // the counts are final here: characterize copies them too
                 GT_HIP(gt_launch_characterize_det(proc, frames, fdtype, nf, c->kp, out, cap,
-                                                  ocount, cs));
+                                                  ocount, cs, d_counts + f0));
+                if (ev) GT_HIP(hipEventRecord(ev[3], cs));
+                continue;
+            }
         } else {
             GT_HIP(gt_launch_localmax(proc, nf, c->kp, c->d_cand, c->cand_cap, c->d_ccount, cs));
             if (ev) GT_HIP(hipEventRecord(ev[2], cs));
