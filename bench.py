@@ -225,9 +225,10 @@ def run_batch_workload(args):
     stream = torch.cuda.current_stream().cuda_stream
     with gt.Batch(p, max_frame_pixels=2048 * 1024, max_launch_pixels=256 << 20,
                   max_slots=1024) as b:
-        # autotune on the first 128 Mpx of frames (mixed sizes)
+        # autotune on the first 1 Gpx of frames (mixed sizes, several launches,
+        # so that the lane count shows)
         acc, k = 0, 0
-        while k < nf and acc < (128 << 20):
+        while k < nf and acc < (1024 << 20):
             acc += frames[k].numel()
             k += 1
         print(f"[bench] autotuning on {k} frames", file=sys.stderr, flush=True)

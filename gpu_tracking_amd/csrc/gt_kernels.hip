@@ -3011,10 +3011,13 @@ template <typename T, int RI>
 __global__ __launch_bounds__(WK_THREADS) void k_characterize_det_batch(
     const float *__restrict__ proc, GtKParams p, const GtBatchHeader *__restrict__ hdr,
     const GtBatchFrame *__restrict__ fs, gt_detection *__restrict__ out, uint32_t out_cap,
-    const uint32_t *__restrict__ ocount) {
+    const uint32_t *__restrict__ ocount, uint32_t *__restrict__ counts) {
     const int nslots = static_cast<int>(hdr->nslots);
     for (int slot = blockIdx.y; slot < nslots; slot += gridDim.y) {
         const GtBatchFrame e = fs[slot];
+        // the counts are final here: copy them out (k_copy_counts_batch)
+        if (counts && blockIdx.x == 0 && threadIdx.x == 0)
+            counts[e.out_idx] = ocount[slot * GT_COUNTER_STRIDE];
         const int H = e.H, W = e.W;
         const GlobalSrc src{proc + e.proc_off, W};
         const T *Rw = reinterpret_cast<const T *>(e.in_ptr);
@@ -3043,10 +3046,17 @@ __global__ __launch_bounds__(WK_THREADS) void k_characterize_det_batch(
 __global__ void k_select_launch(const GtBatchLaunch *__restrict__ launches,
                                 const uint32_t *__restrict__ cur,
                                 const GtBatchFrame *__restrict__ all,
-                                GtBatchHeader *__restrict__ hdr, GtBatchFrame *__restrict__ fs) {
+                                GtBatchHeader *__restrict__ hdr, GtBatchFrame *__restrict__ fs,
+                                uint32_t *__restrict__ ocount, uint32_t *__restrict__ scount,
+                                uint32_t max_slots) {
     const GtBatchLaunch L = launches[*cur];
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < L.nslots) fs[i] = all[L.first + i];
+    if (i < max_slots) {
+        // the launch's counters (one per slot, GT_COUNTER_STRIDE apart)
+        ocount[i * GT_COUNTER_STRIDE] = 0u;
+        scount[i * GT_COUNTER_STRIDE] = 0u;
+    }
     if (i == 0) {
         GtBatchHeader h = {};
         h.nslots = L.nslots;
@@ -3054,6 +3064,7 @@ __global__ void k_select_launch(const GtBatchLaunch *__restrict__ launches,
     }
 }
 
+// Advances *cur once k_select_launch has read it (the launch's last kernel).
 __global__ void k_next_launch(uint32_t *__restrict__ cur) {
     if (threadIdx.x == 0) *cur = *cur + 1u;
 }
@@ -3072,12 +3083,9 @@ hipError_t launch_batch_t(const GtBatchLaunch *launches, uint32_t *cur, const Gt
                           uint32_t *ocount, uint2 *slow, uint32_t slow_cap, uint32_t *scount,
                           uint32_t *counts, const GtBatchSchedule &sch, hipStream_t s) {
     if (p.R < 1 || p.R > 7 || !p.fast_div) return hipErrorInvalidValue;
-    k_select_launch<<<(max_slots + 255) / 256, 256, 0, s>>>(launches, cur, all, hdr, fs);
+    k_select_launch<<<(max_slots + 255) / 256, 256, 0, s>>>(launches, cur, all, hdr, fs, ocount,
+                                                            scount, max_slots);
     hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    e = gt_launch_zero_u32(ocount, static_cast<size_t>(max_slots) * GT_COUNTER_STRIDE, s);
-    if (e != hipSuccess) return e;
-    e = gt_launch_zero_u32(scount, static_cast<size_t>(max_slots) * GT_COUNTER_STRIDE, s);
     if (e != hipSuccess) return e;
     // x: tiles of the call's largest frame (multiple of 8); y: frames of a launch
     const dim3 gbp(sch.bp_blocks, sch.launch_slots);
@@ -3121,14 +3129,13 @@ hipError_t launch_batch_t(const GtBatchLaunch *launches, uint32_t *cur, const Gt
     if (p.characterize) {
         if (p.rint == 4)
             k_characterize_det_batch<T, 4><<<gw, WK_THREADS, 0, s>>>(proc, p, hdr, fs, out, out_cap,
-                                                                     ocount);
+                                                                     ocount, counts);
         else
             k_characterize_det_batch<T, 0><<<gw, WK_THREADS, 0, s>>>(proc, p, hdr, fs, out, out_cap,
-                                                                     ocount);
-        e = hipGetLastError();
-        if (e != hipSuccess) return e;
+                                                                     ocount, counts);
+    } else {
+        k_copy_counts_batch<<<(max_slots + 255) / 256, 256, 0, s>>>(ocount, hdr, fs, counts);
     }
-    k_copy_counts_batch<<<(max_slots + 255) / 256, 256, 0, s>>>(ocount, hdr, fs, counts);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
     k_next_launch<<<1, 64, 0, s>>>(cur);

@@ -328,7 +328,7 @@ gt_status gt_batch_detect(gt_batch *b, const gt_frame_desc *frames, uint32_t n, 
                           void *stream);
 /* Autotuner: times candidate schedules on the sample frames (pixels per
  * launch, then the slow-walk / characterize grid, then the lanes; coordinate
- * search, warm-up + one timed run each) and keeps the fastest for later
+ * search, warm-up + the faster of two timed runs each) and keeps the fastest for later
  * calls; *best_ms receives its time. */
 gt_status gt_batch_tune(gt_batch *b, const gt_frame_desc *frames, uint32_t n, int32_t dtype,
                         gt_detection *d_out, uint32_t cap_per_frame, uint32_t *d_counts,
