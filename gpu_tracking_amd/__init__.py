@@ -11,11 +11,12 @@ it is missing (no CPU fallback).
 """
 from .api import (Batch, CentroidNet, Context, LoG, VideoFile, batch,  # noqa: F401
                   characterize_points, connect, detections_to_numpy, link, load, make_log_params,
-                  make_params, mean_from_file, parse_ets, parse_ets_with_keys, to_tiff)
+                  make_params, mean_from_file, parse_ets, parse_ets_with_keys, to_tiff,
+                  unique_particle_ids)
 from ._native import GtError, column_names, default_params  # noqa: F401
 
 __all__ = ["batch", "characterize_points", "LoG", "link", "connect", "Context", "Batch", "CentroidNet",
            "VideoFile", "load", "mean_from_file", "parse_ets", "parse_ets_with_keys", "to_tiff",
-           "make_params",
+           "make_params", "unique_particle_ids",
            "make_log_params", "GtError",
            "column_names", "default_params", "detections_to_numpy"]

@@ -637,6 +637,17 @@ def connect(to_link1, to_link2, search_range, merge=True):
     return out1.merge(out2, how="outer", on="connect particle")
 
 
+def unique_particle_ids(df, column="particle"):
+    """Makes the ids in `column` unique across runs by appending one random
+    UUID to every id of this DataFrame, as strings, in place (lib.py:251-253;
+    the reference calls the non-existent Series.as_type and adds a UUID
+    object to strings - fixed here).  Returns the DataFrame."""
+    import uuid
+    tag = str(uuid.uuid4())
+    df[column] = df[column].astype(str) + tag
+    return df
+
+
 class Batch:
     """Batched detection of frames of different sizes (u8/u16, Trackpy style):
     every launch runs one kernel per stage over a table of frames of any

@@ -252,3 +252,15 @@ def test_connect_wrapper():
     with pytest.raises(N.GtError) as e:
         gt.connect(np.array([[1, 0, 0], [0, 1, 1]], np.float32), b, 1.0)
     assert e.value.status == 6
+
+
+def test_unique_particle_ids():
+    import gpu_tracking_amd as gt
+    import pandas as pd
+    a = pd.DataFrame({"particle": [0, 1, 1]})
+    b = pd.DataFrame({"particle": [0, 1, 1]})
+    gt.unique_particle_ids(a)
+    gt.unique_particle_ids(b)
+    assert a["particle"][1] == a["particle"][2] and a["particle"][0] != a["particle"][1]
+    assert not set(a["particle"]) & set(b["particle"])
+    assert all(v.startswith(str(i)) for v, i in zip(a["particle"], [0, 1, 1]))
