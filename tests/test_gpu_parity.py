@@ -103,6 +103,20 @@ def test_ragged_shapes(torch_dev, oracle, H, W):
         _assert_rows_equal(got[t], ref, f"{H}x{W} frame {t}")
 
 
+@pytest.mark.parametrize("H,W", [(1, 1), (2, 9), (9, 9), (1, 300), (300, 1), (12, 13)])
+def test_tiny_frames(torch_dev, oracle, H, W):
+    # frames smaller than a tile or than the margin: every stage's bounds
+    # handling, and no rows where the margin leaves no pixel
+    rng = np.random.default_rng(H * 1000 + W)
+    v = rng.integers(90, 110, (2, H, W)).astype(np.uint16)
+    v[:, H // 2, W // 2] += 3000
+    prm = dict(diameter=9, separation=9, minmass=0.0, characterize=True)
+    got = _gpu_detect(torch_dev, v, cap=max(16, H * W), **prm)
+    for t in range(2):
+        ref, _ = _oracle_rows(oracle, v[t], **prm)
+        _assert_rows_equal(got[t], ref, f"{H}x{W} frame {t}")
+
+
 @pytest.mark.parametrize("dtype", ["f32", "u8"])
 def test_other_pixel_types(torch_dev, oracle, dtype):
     v = _video("small", 2, 77, nrows=128, ncols=192, n_particles=20)
