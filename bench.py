@@ -486,9 +486,10 @@ def main():
                     help="learned workload: training pairs")
     ap.add_argument("--learned-epochs", type=int, default=4,
                     help="learned workload: training epochs")
-    ap.add_argument("--two-lanes", action="store_true",
-                    help="also time the steps with the chunks alternating between two streams "
-                         "(after the timed region; reported as two_lanes)")
+    ap.add_argument("--no-lanes", action="store_true",
+                    help="skip timing the steps with the chunks alternating between two "
+                         "streams, and with each chunk's tail on a second stream (done after "
+                         "the timed region by default; reported as two_lanes and overlap)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a 1 GiB device-to-device copy (a known-size kernel that "
                          "calibrates rocprof's FETCH_SIZE/WRITE_SIZE counters)")
@@ -613,28 +614,33 @@ def main():
                         "peak_tflops": 78.6,
                         "frac": round(flop_frame * chunk / tk / 1e12 / 78.6, 4)}
 
-    lanes2 = None
-    if not is_log and args.two_lanes:
+    lanes2 = overlap = None
+    if not is_log and not args.no_lanes and args.workload in ("full", "small"):
         # the same steps with the chunks alternating between two streams
         # (outside the timed region above; kernel spans then overlap, so the
         # roofline is taken from the one-lane run)
         ref_counts = counts.clone()
         ref_sums = _row_checksums(torch, out, counts)
-        ctx.set_lanes(2)
-        step()
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for _ in range(args.steps):
+        def timed_mode():
             step()
-        torch.cuda.synchronize()
-        dt_l2 = time.perf_counter() - t0
-        same = bool(torch.equal(ref_counts, counts)) and \
-            bool(torch.equal(ref_sums, _row_checksums(torch, out, counts)))
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(args.steps):
+                step()
+            torch.cuda.synchronize()
+            dt = time.perf_counter() - t0
+            same = bool(torch.equal(ref_counts, counts)) and \
+                bool(torch.equal(ref_sums, _row_checksums(torch, out, counts)))
+            return {"value": round(T * args.steps / dt, 1), "unit": "frames/s",
+                    "identical_to_one_lane": same}
+
+        ctx.set_lanes(2)
+        lanes2 = timed_mode()
         ctx.set_lanes(1)
-        lanes2 = {"value": round(T * args.steps / dt_l2, 1), "unit": "frames/s",
-                  "identical_to_one_lane": same}
-        print(f"[bench] two lanes {lanes2['value']} frames/s (identical: {same})",
-              file=sys.stderr, flush=True)
+        ctx.set_overlap(True)
+        overlap = timed_mode()
+        ctx.set_overlap(False)
+        print(f"[bench] two lanes {lanes2}, overlap {overlap}", file=sys.stderr, flush=True)
 
     e2e = None
     if not is_log and args.e2e_frames > 0:
@@ -706,7 +712,7 @@ def main():
                                f"{s.n_particles} particles/frame, " +
                                ", ".join(f"{k}={v}" for k, v in params.items())},
         "roofline": roof, "cpu_baseline": cpu, "parity": parity, "end_to_end": e2e,
-        "two_lanes": lanes2,
+        "two_lanes": lanes2, "overlap": overlap,
         "mean_detections_per_frame": round(float(counts.float().mean().item()), 1),
     }
     print(json.dumps(res), flush=True)

@@ -96,7 +96,7 @@ EXPORTS = [
     "gt_cnn_train", "gt_cnn_make_samples", "gt_context_set_cnn", "gt_context_cnn_stats",
     "gt_file_open", "gt_file_close", "gt_file_info", "gt_file_read", "gt_file_channels",
     "gt_file_mean", "gt_execute_file", "gt_write_tiff", "gt_batch_set_lanes", "gt_batch_get_lanes",
-    "gt_context_set_lanes",
+    "gt_context_set_lanes", "gt_context_set_overlap",
 ]
 
 CNN_NPARAM = 14147
@@ -200,6 +200,7 @@ def lib() -> C.CDLL:
         "gt_write_tiff": [C.c_char_p, vp, u64, u32, u32],
         "gt_batch_set_lanes": [vp, u32],
         "gt_context_set_lanes": [vp, C.c_int32],
+        "gt_context_set_overlap": [vp, C.c_int32],
     }
     for name, args in sigs.items():
         fn = getattr(L, name)

@@ -289,6 +289,12 @@ class Context:
         between (plain Trackpy-style detection); results do not depend on it."""
         N.check(self._lib.gt_context_set_lanes(self._h, int(lanes)))
 
+    def set_overlap(self, enable: bool = True) -> None:
+        """Runs each chunk's slow walks and characterize on a second stream,
+        overlapped with the next chunk's bandpass and local maxima (plain
+        Trackpy-style detection, one lane); results do not depend on it."""
+        N.check(self._lib.gt_context_set_overlap(self._h, int(bool(enable))))
+
     # device-resident path -------------------------------------------------
     def detect_device(self, frames, out=None, counts=None, cap_per_frame: Optional[int] = None,
                       stream=None):

@@ -130,8 +130,13 @@ struct gt_context {
     // stream with their own scratch, so a chunk's slow walks and
     // characterize overlap the next chunk's bandpass
     int32_t lanes = 1;                 // lanes allowed (1 or 2)
+    // overlap: every chunk's bandpass and local maxima on the caller's
+    // stream, its slow walks and characterize on lane stream 1 (chunk
+    // scratch alternating between the two lanes' buffers)
+    int32_t overlap = 0;
     hipStream_t lane_stream[2] = {nullptr, nullptr};
     hipEvent_t lane_fork = nullptr, lane_done[2] = {nullptr, nullptr};
+    hipEvent_t ovl_det[2] = {nullptr, nullptr};
     float *l1_proc = nullptr;
     uint32_t *l1_ocount = nullptr, *l1_scount = nullptr;
     uint2 *l1_slow = nullptr;
@@ -292,6 +297,7 @@ void free_all(gt_context *c) {
     if (c->l1_slow) (void)hipFree(c->l1_slow);
     for (int l = 0; l < 2; ++l) {
         if (c->lane_done[l]) (void)hipEventDestroy(c->lane_done[l]);
+        if (c->ovl_det[l]) (void)hipEventDestroy(c->ovl_det[l]);
         if (c->lane_stream[l]) (void)hipStreamDestroy(c->lane_stream[l]);
     }
     if (c->lane_fork) (void)hipEventDestroy(c->lane_fork);
@@ -394,10 +400,12 @@ gt_status illum_correct(gt_context *c, const void *frames, int32_t dtype, uint32
 // Whether the second lane applies: the plain tiled path (no LoG, learned
 // proposal, noise estimate, adaptive passes or illumination correction,
 // whose scratch is single).
-bool lanes_apply(const gt_context *c) {
-    return c->lanes > 1 && c->tiled && !c->log && !c->cnn && !c->der.need_std &&
-           !c->der.adaptive && !(c->der.illum_sigma > 0.0f);
+bool plain_tiled_only(const gt_context *c) {
+    return c->tiled && !c->log && !c->cnn && !c->der.need_std && !c->der.adaptive &&
+           !(c->der.illum_sigma > 0.0f);
 }
+
+bool lanes_apply(const gt_context *c) { return c->lanes > 1 && plain_tiled_only(c); }
 
 gt_status alloc_lane1(gt_context *c) {
     if (c->l1_proc) return GT_OK;
@@ -405,6 +413,7 @@ gt_status alloc_lane1(gt_context *c) {
     for (int l = 0; l < 2; ++l) {
         GT_HIP(hipStreamCreateWithFlags(&c->lane_stream[l], hipStreamNonBlocking));
         GT_HIP(hipEventCreateWithFlags(&c->lane_done[l], hipEventDisableTiming));
+        GT_HIP(hipEventCreateWithFlags(&c->ovl_det[l], hipEventDisableTiming));
     }
     GT_HIP(hipEventCreateWithFlags(&c->lane_fork, hipEventDisableTiming));
     GT_HIP(hipMalloc(&c->l1_ocount, sizeof(uint32_t) * GT_COUNTER_STRIDE * c->chunk));
@@ -435,15 +444,20 @@ gt_status run_pipeline(gt_context *c, const void *d_frames, int32_t dtype, uint3
     // chunk ci runs on lane ci % 2 when the second lane applies; the lanes
     // start after the work queued on s and s waits for both at the end
     const bool two = lanes_apply(c) && nchunks > 1;
-    if (two) {
+    // overlap: chunk ci's slow walks and characterize run on lane stream 1
+    // while s goes on with chunk ci + 1; chunk ci uses lane ci % 2's scratch,
+    // and s waits for chunk ci's tail before chunk ci + 2 reuses it
+    const bool ovl = !two && c->overlap && plain_tiled_only(c) && nchunks > 1;
+    if (two || ovl) {
         gt_status lst = alloc_lane1(c);
         if (lst != GT_OK) return lst;
         GT_HIP(hipEventRecord(c->lane_fork, s));
         for (int l = 0; l < 2; ++l) GT_HIP(hipStreamWaitEvent(c->lane_stream[l], c->lane_fork, 0));
     }
     for (uint32_t ci = 0; ci < nchunks; ++ci) {
-        const int lane = two ? static_cast<int>(ci & 1u) : 0;
+        const int lane = (two || ovl) ? static_cast<int>(ci & 1u) : 0;
         hipStream_t cs = two ? c->lane_stream[lane] : s;
+        if (ovl && ci >= 2) GT_HIP(hipStreamWaitEvent(s, c->lane_done[lane], 0));
         float *proc = lane ? c->l1_proc : c->d_proc;
         uint32_t *ocount = lane ? c->l1_ocount : c->d_ocount;
         uint32_t *scount = lane ? c->l1_scount : c->d_scount;
@@ -545,19 +559,28 @@ gt_status run_pipeline(gt_context *c, const void *d_frames, int32_t dtype, uint3
             GT_HIP(gt_launch_detect_tile(proc, frames, fdtype, nf, kw, out, cap,
                                          ocount, slow, c->slow_cap, scount, f0, cs));
             if (ev) GT_HIP(hipEventRecord(ev[2], cs));
+            hipStream_t ts = cs;   // the chunk's tail: slow walks, characterize, counts
+            if (ovl) {
+                GT_HIP(hipEventRecord(c->ovl_det[lane], cs));
+                ts = c->lane_stream[1];
+                GT_HIP(hipStreamWaitEvent(ts, c->ovl_det[lane], 0));
+            }
             GT_HIP(gt_launch_walk_slow(proc, frames, fdtype, nf, kw, slow, c->slow_cap,
-                                       scount, out, cap, ocount, f0, cs));
+                                       scount, out, cap, ocount, f0, ts));
             if (c->der.adaptive) {
-                gt_status ast = run_adaptive(c, frames, fdtype, nf, out, cap, cs);
+                gt_status ast = run_adaptive(c, frames, fdtype, nf, out, cap, ts);
                 if (ast != GT_OK) return ast;
             }
             if (c->kp.characterize) {
                 // the counts are final here: characterize copies them too
                 GT_HIP(gt_launch_characterize_det(proc, frames, fdtype, nf, c->kp, out, cap,
-                                                  ocount, cs, d_counts + f0));
-                if (ev) GT_HIP(hipEventRecord(ev[3], cs));
-                continue;
+                                                  ocount, ts, d_counts + f0));
+            } else {
+                GT_HIP(gt_launch_copy_counts(ocount, d_counts + f0, nf, ts));
             }
+            if (ev) GT_HIP(hipEventRecord(ev[3], ts));
+            if (ovl) GT_HIP(hipEventRecord(c->lane_done[lane], ts));
+            continue;
         } else {
             GT_HIP(gt_launch_localmax(proc, nf, c->kp, c->d_cand, c->cand_cap, c->d_ccount, cs));
             if (ev) GT_HIP(hipEventRecord(ev[2], cs));
@@ -577,6 +600,8 @@ gt_status run_pipeline(gt_context *c, const void *d_frames, int32_t dtype, uint3
             GT_HIP(hipStreamWaitEvent(s, c->lane_done[l], 0));
         }
     }
+    if (ovl)
+        for (int l = 0; l < 2; ++l) GT_HIP(hipStreamWaitEvent(s, c->lane_done[l], 0));
     if (c->timing) {
         c->n_timed_chunks = static_cast<int>(nchunks);
         c->timed_chunk_frames = std::min(c->chunk, nframes);
@@ -735,6 +760,12 @@ gt_status gt_context_set_lanes(gt_context *c, int32_t lanes) {
     if (!c) return fail(GT_ERR_INVALID_ARGUMENT, "null context");
     if (lanes != 1 && lanes != 2) return fail(GT_ERR_INVALID_ARGUMENT, "lanes must be 1 or 2");
     c->lanes = lanes;
+    return GT_OK;
+}
+
+gt_status gt_context_set_overlap(gt_context *c, int32_t enable) {
+    if (!c) return fail(GT_ERR_INVALID_ARGUMENT, "null context");
+    c->overlap = enable != 0;
     return GT_OK;
 }
 

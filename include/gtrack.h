@@ -178,6 +178,14 @@ gt_status gt_detect_device(gt_context *ctx, const void *d_frames, int32_t dtype,
  * then includes the other lane's concurrent work.  No reference
  * counterpart. */
 gt_status gt_context_set_lanes(gt_context *ctx, int32_t lanes);
+/* Tail overlap of gt_detect_device (0 or 1, default 0; applies where two
+ * lanes would, and only with one lane): every chunk's bandpass and local
+ * maxima run on the caller's stream and its slow walks, characterize and
+ * count copy on a second stream, so that they overlap the next chunk's
+ * bandpass and local maxima; chunks alternate between two sets of scratch.
+ * The work still starts after, and ends before, the caller's stream.
+ * Results do not depend on it.  No reference counterpart. */
+gt_status gt_context_set_overlap(gt_context *ctx, int32_t enable);
 
 /* execute_gpu over an in-memory video (execute_ndarray -> execute_provider ->
  * execute_gpu, src/execute_gpu.rs:1378,1272,886): frames are staged through

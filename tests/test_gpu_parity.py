@@ -177,7 +177,8 @@ def test_capacity_overflow_reported(torch_dev):
 
 
 def test_two_lanes_give_the_same_rows(gpu_ok):
-    # chunks alternating between two streams with their own scratch
+    # chunks alternating between two streams with their own scratch, and the
+    # chunk tails on a second stream (overlap)
     import torch
     import gpu_tracking_amd as gt
     from gpu_tracking_amd import synth
@@ -185,12 +186,14 @@ def test_two_lanes_give_the_same_rows(gpu_ok):
     v = synth.video_numpy(synth.spec("full", nrows=H, ncols=W, n_particles=80), T, 91)
     d = torch.from_numpy(v.view(np.int16)).cuda()
     res = []
-    for lanes in (1, 2):
+    for lanes, overlap in ((1, False), (2, False), (1, True)):
         with gt.Context(gt.make_params(9, separation=7, minmass=300.0), H, W) as ctx:
             ctx.set_lanes(lanes)
+            ctx.set_overlap(overlap)
             out, counts = ctx.detect_device(d, cap_per_frame=4096)
             torch.cuda.synchronize()
             res.append(gt.detections_to_numpy(out, counts))
     assert sum(len(r) for r in res[0]) > 0
-    for a, b in zip(res[0], res[1]):
-        assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+    for other in res[1:]:
+        for a, b in zip(res[0], other):
+            assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
