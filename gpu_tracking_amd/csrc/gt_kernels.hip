@@ -192,8 +192,9 @@ __device__ __forceinline__ float div_const(float a, float b, float rb) {
 template <int R>
 struct BpLds {
     static constexpr int RH = B2_TH + 2 * R;
-    static constexpr int RWP = (B2_TW + 2 * R + 3) & ~3;   // staged row pitch, 16-byte aligned
-    float Sraw[RH * RWP];
+    static constexpr int RWP = (B2_TW + 2 * R + 3) & ~3;   // staged row pitch, 8-byte aligned
+    uint16_t Sraw[RH * RWP];   // the frame's samples (u8 / u16), converted when read
+    static_assert((RH * RWP * sizeof(uint16_t)) % 16 == 0, "Sg, Sc stay 16-byte aligned");
     float Sg[RH * B2_TW];
     float Sc[RH * B2_TW];
 };
@@ -210,7 +211,8 @@ __device__ __forceinline__ void bandpass_int_tile(const T *__restrict__ f, float
     constexpr int NV = (NT + 3 + 3) / 4;            // float4 reads per row-pass item
     static_assert(B2_TW - 4 + 4 * NV <= RWP, "row-pass reads stay inside the staged row");
     static_assert(B2_TH % B2_ROWS == 0 && (B2_TH / B2_ROWS) * B2_TW == B2_THREADS, "mapping");
-    float *Sraw = L.Sraw, *Sg = L.Sg, *Sc = L.Sc;
+    uint16_t *Sraw = L.Sraw;
+    float *Sg = L.Sg, *Sc = L.Sc;
     const int tid = threadIdx.x;
     constexpr int PAIRS = (B2_TW + 2 * R + 1) / 2;  // u16 pairs per staged row
     static_assert(2 * PAIRS <= RWP, "pairs fit the staged row");
@@ -239,14 +241,8 @@ __device__ __forceinline__ void bandpass_int_tile(const T *__restrict__ f, float
         }
 #pragma unroll
         for (int q = 0; q < QPASS; ++q) {
-            if (active && rq + q * QROWS < RH) {
-                float4 v4;
-                v4.x = static_cast<float>(vals[q].x & 0xFFFFu);
-                v4.y = static_cast<float>(vals[q].x >> 16);
-                v4.z = static_cast<float>(vals[q].y & 0xFFFFu);
-                v4.w = static_cast<float>(vals[q].y >> 16);
-                *reinterpret_cast<float4 *>(Sraw + (rq + q * QROWS) * RWP + 4 * cq) = v4;
-            }
+            if (active && rq + q * QROWS < RH)
+                *reinterpret_cast<uint2 *>(Sraw + (rq + q * QROWS) * RWP + 4 * cq) = vals[q];
         }
     } else if (sizeof(T) == 2 && (R % 2) == 0 && (W % 2) == 0 &&
         (reinterpret_cast<uintptr_t>(f) & 3u) == 0 && y0 - R >= 0 && y0 + B2_TH + R <= H &&
@@ -269,10 +265,7 @@ __device__ __forceinline__ void bandpass_int_tile(const T *__restrict__ f, float
             const int idx = tid + q * B2_THREADS;
             if (idx < CN) {
                 const int ry = idx / PAIRS, rx = idx - ry * PAIRS;
-                float2 v2;
-                v2.x = static_cast<float>(vals[q] & 0xFFFFu);
-                v2.y = static_cast<float>(vals[q] >> 16);
-                *reinterpret_cast<float2 *>(Sraw + ry * RWP + 2 * rx) = v2;
+                *reinterpret_cast<uint32_t *>(Sraw + ry * RWP + 2 * rx) = vals[q];
             }
         }
     } else {
@@ -290,7 +283,7 @@ __device__ __forceinline__ void bandpass_int_tile(const T *__restrict__ f, float
 #pragma unroll
         for (int q = 0; q < NL; ++q) {
             const int idx = tid + q * B2_THREADS;
-            if (idx < CN) Sraw[idx] = to_f32(vals[q]);
+            if (idx < CN) Sraw[idx] = static_cast<uint16_t>(vals[q]);
         }
     }
     __syncthreads();
@@ -309,15 +302,15 @@ __device__ __forceinline__ void bandpass_int_tile(const T *__restrict__ f, float
     for (int q = 0; q < RPASS; ++q) {
         const int ry = ry0 + q * RSTEP, cx = cxi;
         if (ry >= RH) continue;
-        const float4 *src4 = reinterpret_cast<const float4 *>(Sraw + ry * RWP + cx);
+        const uint2 *src4 = reinterpret_cast<const uint2 *>(Sraw + ry * RWP + cx);
         float v[4 * NV];
 #pragma unroll
         for (int k = 0; k < NV; ++k) {
-            const float4 t = src4[k];
-            v[4 * k + 0] = t.x;
-            v[4 * k + 1] = t.y;
-            v[4 * k + 2] = t.z;
-            v[4 * k + 3] = t.w;
+            const uint2 t = src4[k];
+            v[4 * k + 0] = static_cast<float>(t.x & 0xFFFFu);
+            v[4 * k + 1] = static_cast<float>(t.x >> 16);
+            v[4 * k + 2] = static_cast<float>(t.y & 0xFFFFu);
+            v[4 * k + 3] = static_cast<float>(t.y >> 16);
         }
         float gs[4] = {0.0f, 0.0f, 0.0f, 0.0f};
         const int gx0 = x0 + cx - R;   // frame column of v[0]
