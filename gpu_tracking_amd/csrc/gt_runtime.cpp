@@ -84,6 +84,7 @@ struct gt_context {
 
     // device scratch shared by gt_detect_device and gt_execute
     uint32_t chunk = 1;            // frames per pipeline chunk
+    uint32_t xchunk = 1;           // frames per staging slot of gt_execute (<= chunk)
     float *d_proc = nullptr;       // chunk * HW
     uint32_t *d_cand = nullptr;    // chunk * cand_cap
     uint32_t *d_ccount = nullptr;  // chunk * GT_COUNTER_STRIDE (candidates, untiled path)
@@ -238,13 +239,13 @@ gt_status alloc_execute_buffers(gt_context *c) {
     const size_t HW = static_cast<size_t>(c->H) * c->W;
     if (c->gpu_post)
         for (auto &dp : c->d_post)
-            GT_HIP(hipMalloc(&dp, sizeof(float4) * static_cast<size_t>(c->det_cap) * c->chunk));
+            GT_HIP(hipMalloc(&dp, sizeof(float4) * static_cast<size_t>(c->det_cap) * c->xchunk));
     for (auto &sl : c->slot) {
-        GT_HIP(hipMalloc(&sl.d_frames, 4 * HW * c->chunk));
-        GT_HIP(hipMalloc(&sl.d_rows, sizeof(gt_detection) * static_cast<size_t>(c->det_cap) * c->chunk));
-        GT_HIP(hipMalloc(&sl.d_counts, sizeof(uint32_t) * c->chunk));
-        GT_HIP(hipHostMalloc(&sl.h_stage, 4 * HW * c->chunk, hipHostMallocDefault));
-        GT_HIP(hipHostMalloc(reinterpret_cast<void **>(&sl.h_counts), sizeof(uint32_t) * c->chunk,
+        GT_HIP(hipMalloc(&sl.d_frames, 4 * HW * c->xchunk));
+        GT_HIP(hipMalloc(&sl.d_rows, sizeof(gt_detection) * static_cast<size_t>(c->det_cap) * c->xchunk));
+        GT_HIP(hipMalloc(&sl.d_counts, sizeof(uint32_t) * c->xchunk));
+        GT_HIP(hipHostMalloc(&sl.h_stage, 4 * HW * c->xchunk, hipHostMallocDefault));
+        GT_HIP(hipHostMalloc(reinterpret_cast<void **>(&sl.h_counts), sizeof(uint32_t) * c->xchunk,
                              hipHostMallocDefault));
         GT_HIP(hipEventCreateWithFlags(&sl.done, hipEventDisableTiming));
     }
@@ -669,13 +670,16 @@ gt_status gt_context_create(const gt_params *params, uint32_t nrows, uint32_t nc
         return fail(GT_ERR_GPU_DEVICE, "could not initialise the device tables");
     }
     const size_t HW = static_cast<size_t>(nrows) * ncols;
-    // Frames per chunk: bounded by a 4 GiB scratch budget and by 64 frames.
+    // Frames per chunk: bounded by a 4 GiB scratch budget and by 128 frames.
     // per frame of device scratch: processed plane and candidate list (the
     // two gt_execute staging slots, frame + detection buffer each, are
     // allocated on first use)
     const size_t per_frame = HW * (4 + 4);
     size_t chunk = (size_t(4) << 30) / (per_frame ? per_frame : 1);
-    c->chunk = static_cast<uint32_t>(std::max<size_t>(1, std::min<size_t>(chunk, 64)));
+    c->chunk = static_cast<uint32_t>(std::max<size_t>(1, std::min<size_t>(chunk, 128)));
+    // host frames are staged 64 at a time: smaller pinned slots, and the
+    // first slot of a video starts on the GPU sooner
+    c->xchunk = std::min<uint32_t>(c->chunk, 64);
     c->cand_cap = static_cast<uint32_t>(HW);
     c->det_cap = static_cast<uint32_t>(HW);
     gt_status st = alloc_scratch(c);
@@ -881,7 +885,7 @@ gt_status execute_frames(gt_context *c, const FrameSrc &src, const uint64_t *key
     bool producer_done = false;
     gt_status post_status = GT_OK;
     std::string post_err;
-    const size_t max_queued = 4 * static_cast<size_t>(c->chunk) + 8;
+    const size_t max_queued = 4 * static_cast<size_t>(c->xchunk) + 8;
     std::thread worker([&]() {
         std::vector<float> rawf;
         for (;;) {
@@ -931,7 +935,7 @@ gt_status execute_frames(gt_context *c, const FrameSrc &src, const uint64_t *key
     };
 
     const size_t nplan = plan.frames.size();
-    const size_t nchunks = (nplan + c->chunk - 1) / c->chunk;
+    const size_t nchunks = (nplan + c->xchunk - 1) / c->xchunk;
     // stage frames idx[0..nf) into slot sl's pinned buffer: copies (host
     // array) or decodes (file), frame f by thread f % nt
     auto stage = [&](gt_context::Slot &sl, const uint64_t *idx, uint32_t nf) -> gt_status {
@@ -965,9 +969,9 @@ gt_status execute_frames(gt_context *c, const FrameSrc &src, const uint64_t *key
         // source, whatever the keys
         auto profile = [&]() -> gt_status {
             gt_status ist = illum_begin(c, c->stream);
-            std::vector<uint64_t> idx(c->chunk);
-            for (uint64_t f0 = 0; f0 < nframes && ist == GT_OK; f0 += c->chunk) {
-                const uint32_t nf = static_cast<uint32_t>(std::min<uint64_t>(c->chunk, nframes - f0));
+            std::vector<uint64_t> idx(c->xchunk);
+            for (uint64_t f0 = 0; f0 < nframes && ist == GT_OK; f0 += c->xchunk) {
+                const uint32_t nf = static_cast<uint32_t>(std::min<uint64_t>(c->xchunk, nframes - f0));
                 for (uint32_t f = 0; f < nf; ++f) idx[f] = f0 + f;
                 auto &sl = c->slot[0];
                 GT_HIP(hipStreamSynchronize(c->stream));   // the slot's last copy is done
@@ -990,8 +994,8 @@ gt_status execute_frames(gt_context *c, const FrameSrc &src, const uint64_t *key
     // stage chunk k in slot k % 2 and queue its GPU work
     auto submit = [&](size_t k) -> gt_status {
         auto &sl = c->slot[k % 2];
-        const size_t base = k * c->chunk;
-        const uint32_t nf = static_cast<uint32_t>(std::min<size_t>(c->chunk, nplan - base));
+        const size_t base = k * c->xchunk;
+        const uint32_t nf = static_cast<uint32_t>(std::min<size_t>(c->xchunk, nplan - base));
         for (uint32_t f = 0; f < nf; ++f) {
             const uint64_t idx = plan.frames[base + f];
             if (idx >= nframes)
@@ -1029,8 +1033,8 @@ gt_status execute_frames(gt_context *c, const FrameSrc &src, const uint64_t *key
     // read chunk k's detections back and hand them to the post thread
     auto drain = [&](size_t k) -> gt_status {
         auto &sl = c->slot[k % 2];
-        const size_t base = k * c->chunk;
-        const uint32_t nf = static_cast<uint32_t>(std::min<size_t>(c->chunk, nplan - base));
+        const size_t base = k * c->xchunk;
+        const uint32_t nf = static_cast<uint32_t>(std::min<size_t>(c->xchunk, nplan - base));
         GT_HIP(hipEventSynchronize(sl.done));
         size_t total = 0;
         for (uint32_t f = 0; f < nf; ++f) {
