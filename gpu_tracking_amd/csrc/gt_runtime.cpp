@@ -670,13 +670,13 @@ gt_status gt_context_create(const gt_params *params, uint32_t nrows, uint32_t nc
         return fail(GT_ERR_GPU_DEVICE, "could not initialise the device tables");
     }
     const size_t HW = static_cast<size_t>(nrows) * ncols;
-    // Frames per chunk: bounded by a 4 GiB scratch budget and by 128 frames.
+    // Frames per chunk: bounded by a 4 GiB scratch budget and by 256 frames.
     // per frame of device scratch: processed plane and candidate list (the
     // two gt_execute staging slots, frame + detection buffer each, are
     // allocated on first use)
     const size_t per_frame = HW * (4 + 4);
     size_t chunk = (size_t(4) << 30) / (per_frame ? per_frame : 1);
-    c->chunk = static_cast<uint32_t>(std::max<size_t>(1, std::min<size_t>(chunk, 128)));
+    c->chunk = static_cast<uint32_t>(std::max<size_t>(1, std::min<size_t>(chunk, 256)));
     // host frames are staged 64 at a time: smaller pinned slots, and the
     // first slot of a video starts on the GPU sooner
     c->xchunk = std::min<uint32_t>(c->chunk, 64);
