@@ -13,10 +13,12 @@ from .api import (Batch, CentroidNet, Context, LoG, VideoFile, batch,  # noqa: F
                   characterize_points, connect, detections_to_numpy, link, load, make_log_params,
                   make_params, mean_from_file, parse_ets, parse_ets_with_keys, to_tiff,
                   unique_particle_ids)
-from ._native import GtError, column_names, default_params  # noqa: F401
+from ._native import (GtBaseError, GtConnectionError, GtError, GtFileNotFoundError,  # noqa: F401
+                      GtInterrupted, GtValueError, column_names, default_params)
 
 __all__ = ["batch", "characterize_points", "LoG", "link", "connect", "Context", "Batch", "CentroidNet",
            "VideoFile", "load", "mean_from_file", "parse_ets", "parse_ets_with_keys", "to_tiff",
            "make_params", "unique_particle_ids",
-           "make_log_params", "GtError",
+           "make_log_params", "GtError", "GtValueError", "GtFileNotFoundError",
+           "GtConnectionError", "GtInterrupted", "GtBaseError",
            "column_names", "default_params", "detections_to_numpy"]

@@ -105,11 +105,56 @@ CNN_PATCH_PITCH = 128
 _lib = None
 
 
-class GtError(RuntimeError):
+class GtError(BaseException):
+    """Base of every error the library raises; `status` is the gt_status code.
+
+    Each status is raised as a subclass that also derives from the built-in
+    exception the reference's bindings raise for that error variant
+    (gpu_tracking_python/src/python_bindings.rs:159-192), so code written
+    against the reference (``except ValueError``, ``except
+    FileNotFoundError``, Ctrl-C as KeyboardInterrupt) keeps working.  GtError
+    itself derives from BaseException only so that the interrupt class stays
+    outside ``except Exception``, as KeyboardInterrupt does.
+    """
+
     def __init__(self, status: int, message: str):
         super().__init__(f"{STATUS_NAMES.get(status, status)}: {message}")
         self.status = status
         self.message = message
+
+
+class GtInterrupted(GtError, KeyboardInterrupt):
+    """Error::Interrupted -> KeyboardInterrupt."""
+
+
+class GtConnectionError(GtError, ConnectionError):
+    """Error::GpuAdapterError / GpuDeviceError -> ConnectionError."""
+
+
+class GtBaseError(GtError):
+    """Error::ThreadError / PolledAfterTermination / TiffWrite -> BaseException."""
+
+
+class GtFileNotFoundError(GtError, FileNotFoundError):
+    """Error::FileNotFound / InvalidFileName -> FileNotFoundError."""
+
+
+class GtValueError(GtError, ValueError):
+    """Every other variant -> ValueError; also the two codes without a
+    reference counterpart (rejected argument, capacity exceeded)."""
+
+
+_STATUS_CLASS = {
+    19: GtInterrupted,
+    1: GtConnectionError, 2: GtConnectionError,
+    9: GtBaseError, 20: GtBaseError, 21: GtBaseError,
+    3: GtFileNotFoundError, 16: GtFileNotFoundError,
+}
+
+
+def error(status: int, message: str) -> GtError:
+    """The exception object for a gt_status code (see GtError)."""
+    return _STATUS_CLASS.get(status, GtValueError)(status, message)
 
 
 def lib() -> C.CDLL:
@@ -221,7 +266,7 @@ def lib() -> C.CDLL:
 def check(status: int) -> None:
     if status != 0:
         msg = lib().gt_last_error().decode(errors="replace")
-        raise GtError(status, msg)
+        raise error(status, msg)
 
 
 def default_params() -> GtParams:

@@ -17,6 +17,7 @@ Fixes of reference wrapper bugs (documented divergences):
 from __future__ import annotations
 
 import ctypes as C
+import threading
 import os
 import math
 from typing import Iterable, Optional
@@ -93,7 +94,7 @@ class VideoFile:
             while j < len(idx) and idx[j] == idx[j - 1] + 1:
                 j += 1
             if idx[i] < 0:
-                raise N.GtError(7, f"Requested a frame outside the length\n of the video. Video "
+                raise N.error(7, f"Requested a frame outside the length\n of the video. Video "
                                    f"length is {self.shape[0]}, requested frame {idx[i]}")
             N.check(self._lib.gt_file_read(self._h, idx[i], j - i, out[i:j].ctypes.data))
             i = j
@@ -255,7 +256,7 @@ class Context:
         if not isinstance(video, VideoFile):
             video = np.ascontiguousarray(video)
         if tuple(video.shape[1:]) != self.shape:
-            raise N.GtError(4, f"dimension mismatch. provided dimensions are {self.shape}, "
+            raise N.error(4, f"dimension mismatch. provided dimensions are {self.shape}, "
                                f"but the frames are {video.shape[1:]}")
         keys_arr = None if keys is None else np.ascontiguousarray(np.asarray(list(keys), np.uint64))
         pts = None if points is None else np.ascontiguousarray(points, np.float32)
@@ -269,10 +270,16 @@ class Context:
                 None if reset is None else reset.ctypes.data, 0 if reset is None else len(reset),
                 C.byref(out))
         if isinstance(video, VideoFile):
-            st = self._lib.gt_execute_file(self._h, video.handle, *rest)
+            st, interrupted = self._interruptible(self._lib.gt_execute_file, self._h,
+                                                  video.handle, *rest)
         else:
-            st = self._lib.gt_execute(self._h, video.ctypes.data, _dtype_code(video),
-                                      video.shape[0], *rest)
+            st, interrupted = self._interruptible(self._lib.gt_execute, self._h,
+                                                  video.ctypes.data, _dtype_code(video),
+                                                  video.shape[0], *rest)
+        if interrupted and st == 0:
+            # the call finished before it saw the flag: the interrupt still wins
+            self._lib.gt_output_free(C.byref(out))
+            raise N.error(19, "Keyboard Interrupt")
         N.check(st)
         try:
             n, m = int(out.nrows), int(out.ncols)
@@ -283,6 +290,30 @@ class Context:
         finally:
             self._lib.gt_output_free(C.byref(out))
         return data, names, types
+
+    def _interruptible(self, fn, *args) -> int:
+        """Runs a whole-video call on a worker thread while this thread
+        waits in 50 ms steps, so that Ctrl-C reaches Python during the call.
+        A KeyboardInterrupt sets the context's interrupt flag; the call then
+        ends with Error::Interrupted, raised as GtInterrupted (a
+        KeyboardInterrupt).  The reference's bindings do the same with a
+        worker thread polled every 50 ms (gpu_tracking_macros/src/lib.rs:75-129)."""
+        box = {}
+
+        def work():
+            box["st"] = fn(*args)
+
+        t = threading.Thread(target=work, name="gt_execute", daemon=True)
+        t.start()
+        interrupted = False
+        try:
+            while t.is_alive():
+                t.join(0.05)
+        except KeyboardInterrupt:
+            interrupted = True
+            self._lib.gt_set_interrupt(self._h, 1)
+            t.join()
+        return box["st"], interrupted
 
     def set_lanes(self, lanes: int) -> None:
         """Streams (1 or 2) the chunks of a detect_device call alternate
@@ -409,7 +440,7 @@ def detections_to_numpy(out, counts) -> list:
     for t in range(o.shape[0]):
         n = int(c[t])
         if n > o.shape[1]:
-            raise N.GtError(101, f"frame {t}: {n} detections above capacity {o.shape[1]}")
+            raise N.error(101, f"frame {t}: {n} detections above capacity {o.shape[1]}")
         a = np.ascontiguousarray(o[t, :n]).view(dt).reshape(-1)
         res.append(np.sort(a, order="seed"))
     return res

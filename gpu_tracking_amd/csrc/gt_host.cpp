@@ -617,7 +617,7 @@ gt_status gt_host_post::finish(gt_output *out, std::string *err) {
 extern "C" gt_status gt_filter_close(const gt_result_row *rows, uint64_t n, float separation,
                                      uint8_t *keep) {
     if (n == 0) return GT_OK;
-    if (!rows || !keep) return GT_ERR_INVALID_ARGUMENT;
+    if (!rows || !keep) return gt_runtime_fail(GT_ERR_INVALID_ARGUMENT, "null argument");
     const float r2 = separation * separation;
     // Uniform grid with cells of side `separation`; +-2 cells are searched so
     // rounding of x / separation near a cell edge cannot hide a neighbour.
@@ -688,7 +688,7 @@ extern "C" gt_status gt_doughnut(const float *raw, uint32_t H, uint32_t W, const
                                  uint64_t n, float radius, float bg_radius, float gap_radius,
                                  float *out3) {
     if (n == 0) return GT_OK;
-    if (!raw || !rows || !out3) return GT_ERR_INVALID_ARGUMENT;
+    if (!raw || !rows || !out3) return gt_runtime_fail(GT_ERR_INVALID_ARGUMENT, "null argument");
     std::vector<std::pair<int32_t, int32_t>> sig, ring;
     gt_host_doughnut_offsets(radius, bg_radius, gap_radius, &sig, &ring);
     std::vector<float> bg;
@@ -809,7 +809,7 @@ size_t gt_host_prune_blobs(gt_result_row *rows, size_t n, float overlap_threshol
 
 extern "C" gt_status gt_prune_blobs(gt_result_row *rows, uint64_t n, float overlap_threshold,
                                     uint64_t *n_out) {
-    if (!n_out || (n && !rows)) return GT_ERR_INVALID_ARGUMENT;
+    if (!n_out || (n && !rows)) return gt_runtime_fail(GT_ERR_INVALID_ARGUMENT, "null argument");
     *n_out = gt_host_prune_blobs(rows, static_cast<size_t>(n), overlap_threshold);
     return GT_OK;
 }
@@ -842,7 +842,10 @@ gt_status frame_groups(const float *points, uint64_t n, std::vector<FrameGroup> 
             begin = i;
             cur = fr;
         }
-        if (fr < cur) return GT_ERR_NON_SORTED_CHARACTERIZATION;
+        if (fr < cur)
+            return gt_runtime_fail(GT_ERR_NON_SORTED_CHARACTERIZATION,
+                                   "the points provided for characterization aren't sorted in "
+                                   "ascending frame number.");
     }
     if (n > begin) out->push_back({frame_key(cur), begin, n});
     return GT_OK;
@@ -861,8 +864,9 @@ void group_xy(const float *points, const FrameGroup &g, std::vector<float> *xy) 
 extern "C" gt_status gt_link(const float *points, uint64_t n, float search_range, uint32_t memory,
                              uint64_t *ids) {
     // linker_all over FrameSubsetter's linking groups (linking.rs:818-831)
-    if (n && (!points || !ids)) return GT_ERR_INVALID_ARGUMENT;
-    if (!(search_range > 0.0f) || !std::isfinite(search_range)) return GT_ERR_INVALID_ARGUMENT;
+    if (n && (!points || !ids)) return gt_runtime_fail(GT_ERR_INVALID_ARGUMENT, "null argument");
+    if (!(search_range > 0.0f) || !std::isfinite(search_range))
+        return gt_runtime_fail(GT_ERR_INVALID_ARGUMENT, "search_range must be positive and finite");
     std::vector<FrameGroup> groups;
     gt_status st = frame_groups(points, n, &groups);
     if (st != GT_OK) return st;
@@ -872,7 +876,7 @@ extern "C" gt_status gt_link(const float *points, uint64_t n, float search_range
     for (const FrameGroup &g : groups) {
         group_xy(points, g, &xy);
         st = lk.advance(xy.data(), g.end - g.begin, ids + g.begin, nullptr, &err);
-        if (st != GT_OK) return st;
+        if (st != GT_OK) return gt_runtime_fail(st, "%s", err.c_str());
     }
     return GT_OK;
 }
@@ -885,8 +889,10 @@ extern "C" gt_status gt_connect(const float *points1, uint64_t n1, const float *
     // (fill_base).  Where the reference's sync_iterators (:945-955) fills
     // the frame that overtook the one being caught up to twice, that frame
     // is filled once, when it is reached.
-    if ((n1 && (!points1 || !ids1)) || (n2 && (!points2 || !ids2))) return GT_ERR_INVALID_ARGUMENT;
-    if (!(search_range > 0.0f) || !std::isfinite(search_range)) return GT_ERR_INVALID_ARGUMENT;
+    if ((n1 && (!points1 || !ids1)) || (n2 && (!points2 || !ids2)))
+        return gt_runtime_fail(GT_ERR_INVALID_ARGUMENT, "null argument");
+    if (!(search_range > 0.0f) || !std::isfinite(search_range))
+        return gt_runtime_fail(GT_ERR_INVALID_ARGUMENT, "search_range must be positive and finite");
     std::vector<FrameGroup> g1, g2;
     gt_status st = frame_groups(points1, n1, &g1);
     if (st != GT_OK) return st;
@@ -908,7 +914,7 @@ extern "C" gt_status gt_connect(const float *points1, uint64_t n1, const float *
             group_xy(points2, g2[j], &xy2);
             st = lk.connect(xy1.data(), g1[i].end - g1[i].begin, xy2.data(),
                             g2[j].end - g2[j].begin, ids1 + g1[i].begin, ids2 + g2[j].begin, &err);
-            if (st != GT_OK) return st;
+            if (st != GT_OK) return gt_runtime_fail(st, "%s", err.c_str());
             ++i;
             ++j;
         }

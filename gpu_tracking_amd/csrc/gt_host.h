@@ -72,6 +72,10 @@ struct gt_host_plan {
     std::vector<gt_result_row> pts;
 };
 
+// Records the message gt_last_error() returns on this thread and returns st
+// (defined in gt_runtime.cpp).
+gt_status gt_runtime_fail(gt_status st, const char *fmt, ...);
+
 gt_status gt_host_plan_frames(int32_t characterize_points, uint64_t nframes, const uint64_t *keys,
                               uint64_t nkeys, const float *points, uint64_t npoints,
                               uint32_t point_cols, int32_t has_frames, int32_t has_r,

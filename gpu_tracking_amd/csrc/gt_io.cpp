@@ -235,7 +235,10 @@ gt_status open_tiff(gt_file *f, std::string *why) {
                    std::to_string(bits) + " bits";
             return GT_ERR_READ;
         }
-        if (W == 0 || H == 0 || W > 0xFFFFFFFFull || H > 0xFFFFFFFFull) {
+        // a frame is decoded into one buffer: its byte size must fit far
+        // below size_t (2^36 samples bounds any real frame)
+        if (W == 0 || H == 0 || W > 0xFFFFFFFFull || H > 0xFFFFFFFFull ||
+            W * H > (1ull << 36)) {
             *why = "TIFF image without a valid size";
             return GT_ERR_READ;
         }
@@ -252,6 +255,17 @@ gt_status open_tiff(gt_file *f, std::string *why) {
         if (pg.tiled) {
             if (pg.tw == 0 || pg.th == 0) {
                 *why = "tiled TIFF without a tile size";
+                return GT_ERR_READ;
+            }
+            // one tile is decoded into a buffer of tw * th samples: tiles
+            // larger than the image rounded up to a multiple of 256 (or
+            // above 2^28 samples) are malformed, and this bounds the
+            // product far from overflow
+            const uint64_t wmax = (W + 255) / 256 * 256, hmax = (H + 255) / 256 * 256;
+            if (pg.tw > wmax || pg.th > hmax ||
+                static_cast<uint64_t>(pg.tw) * pg.th > (1ull << 28)) {
+                *why = "TIFF tile of " + std::to_string(pg.tw) + " x " + std::to_string(pg.th) +
+                       " for a " + std::to_string(W) + " x " + std::to_string(H) + " image";
                 return GT_ERR_READ;
             }
             nseg = ((W + pg.tw - 1) / pg.tw) * ((H + pg.th - 1) / pg.th);
@@ -506,11 +520,17 @@ gt_status open_ets(gt_file *f, int32_t has_channel, uint32_t channel, std::strin
                                                                                            : GT_ERR_READ;
     }
     const uint32_t d0 = ld32(b + 28, false), d1 = ld32(b + 32, false);
-    if (d0 == 0 || d1 == 0) {
-        *why = "ETS frame of size zero";
+    if (d0 == 0 || d1 == 0 || static_cast<uint64_t>(d0) * d1 > (1ull << 36)) {
+        *why = "ETS frame of size " + std::to_string(d0) + " x " + std::to_string(d1);
         return GT_ERR_READ;
     }
     const size_t esz = 4 + 4 * static_cast<size_t>(ndim) + 8 + 8;
+    // the chunk table lies in the file: its length bounds the entry count
+    struct stat stt;
+    if (fstat(f->fd, &stt) != 0 || static_cast<uint64_t>(nused) * esz > static_cast<uint64_t>(stt.st_size)) {
+        *why = "ETS chunk table of " + std::to_string(nused) + " entries exceeds the file";
+        return GT_ERR_READ;
+    }
     std::vector<uint8_t> tab(esz * nused);
     if (nused && !rd(used, tab.data(), tab.size())) {
         *why = "truncated ETS chunk table";
@@ -520,6 +540,14 @@ gt_status open_ets(gt_file *f, int32_t has_channel, uint32_t channel, std::strin
         const uint8_t *p = tab.data() + esz * i + 4;
         const uint32_t t = ld32(p + 4 * 2, false), ch = ld32(p + 4 * 4, false);
         const int64_t off = static_cast<int64_t>(ld64(p + 4 * ndim, false));
+        // frames missing from the table read as errors later (the reference
+        // keeps such holes); the table of a channel stays within 2^20
+        // entries beyond the file's chunk count
+        if (static_cast<uint64_t>(t) >= static_cast<uint64_t>(nused) + (1u << 20)) {
+            *why = "ETS chunk " + std::to_string(i) + " has time index " + std::to_string(t) +
+                   " beyond the " + std::to_string(nused) + " chunks of the file";
+            return GT_ERR_READ;
+        }
         std::vector<int64_t> &e = f->ets_all[ch];
         // the reference's three cases (decoderiter.rs:388-396)
         if (static_cast<size_t>(t) + 1 == e.size()) {
@@ -568,8 +596,10 @@ std::string lower_ext(const std::string &path, std::string *stem, std::string *d
 
 }  // namespace
 
-gt_status gt_io_open(const char *path, int32_t has_channel, uint32_t channel, gt_file **out,
-                     std::string *why) {
+namespace {
+
+gt_status io_open(const char *path, int32_t has_channel, uint32_t channel, gt_file **out,
+                  std::string *why) {
     *out = nullptr;
     std::string stem, dir;
     const std::string p(path);
@@ -609,6 +639,21 @@ gt_status gt_io_open(const char *path, int32_t has_channel, uint32_t channel, gt
     return GT_OK;
 }
 
+}  // namespace
+
+// Sizes read from a file bound every allocation above, but a failed
+// allocation must still end as an error, not cross the C boundary.
+gt_status gt_io_open(const char *path, int32_t has_channel, uint32_t channel, gt_file **out,
+                     std::string *why) {
+    try {
+        return io_open(path, has_channel, channel, out, why);
+    } catch (const std::exception &e) {
+        *out = nullptr;
+        *why = std::string("could not open the file: ") + e.what();
+        return GT_ERR_READ;
+    }
+}
+
 void gt_io_close(gt_file *f) { delete f; }
 
 uint64_t gt_io_nframes(const gt_file *f) { return f->kind == 0 ? f->pages.size() : f->ets.size(); }
@@ -625,7 +670,14 @@ gt_status gt_io_read(const gt_file *f, uint64_t idx, void *dst, std::string *why
                std::to_string(n) + ", requested frame " + std::to_string(idx);
         return GT_ERR_FRAME_OUT_OF_BOUNDS;
     }
-    if (f->kind == 0) return read_tiff(f, idx, dst, why);
+    if (f->kind == 0) {
+        try {
+            return read_tiff(f, idx, dst, why);
+        } catch (const std::exception &e) {
+            *why = "could not decode frame " + std::to_string(idx) + ": " + e.what();
+            return GT_ERR_READ;
+        }
+    }
     const int64_t off = f->ets[static_cast<size_t>(idx)];
     if (off < 0) {
         *why = "frame " + std::to_string(idx) + " has no chunk in the ETS file";

@@ -259,3 +259,52 @@ def test_path_errors(tmp_path):
     with pytest.raises(GtError) as e:
         gt.VideoFile(_write(tmp_path, "bad.tif", b"XX*\0\0\0\0\0"))
     assert _status(e) == 15     # ReadError
+
+
+# ---- malformed files end as ReadError, never as a crash --------------------
+
+@pytest.mark.parametrize("be", [False, True])
+@pytest.mark.parametrize("tw,th", [(2 ** 31, 2 ** 31), (2 ** 31, 16), (0, 16), (4096, 4096)])
+def test_oversized_or_empty_tiles_are_read_errors(tmp_path, be, tw, th):
+    # a 1x1 f32 image in one 16x16 tile, whose tile-size tags are then
+    # rewritten (tw * th * 4 wraps to 0 in 64 bits for 2^31 x 2^31)
+    e = ">" if be else "<"
+    b = bytearray(tiff_bytes([np.ones((1, 1), np.float32)], be=be, tile=(16, 16), fmt=3))
+    for tag, val in ((322, tw), (323, th)):
+        idx = b.find(struct.pack(e + "HHI", tag, 4, 1))
+        assert idx > 0
+        struct.pack_into(e + "I", b, idx + 8, val)
+    p = _write(tmp_path, "t.tif", bytes(b))
+    with pytest.raises(ValueError) as err:
+        gt.VideoFile(p)
+    assert _status(err) == 15   # Error::ReadError
+
+
+def test_ets_time_index_beyond_the_table_is_a_read_error(tmp_path):
+    frs = [np.full((5, 6), 7, np.uint16)]
+
+    def far(chunks):
+        return [(2 ** 31, ch, off) for (t, ch, off) in chunks]
+
+    p = _write(tmp_path, "t.ets", ets_bytes({0: frs}, (6, 5), extra=far))
+    with pytest.raises(ValueError) as err:
+        gt.VideoFile(p)
+    assert _status(err) == 15
+
+
+def test_ets_chunk_count_beyond_the_file_is_a_read_error(tmp_path):
+    b = bytearray(ets_bytes({0: [np.zeros((5, 6), np.uint16)]}, (6, 5)))
+    struct.pack_into("<I", b, 40, 0xFFFFFFFF)   # used-chunk count
+    p = _write(tmp_path, "n.ets", bytes(b))
+    with pytest.raises(ValueError) as err:
+        gt.VideoFile(p)
+    assert _status(err) == 15
+
+
+def test_ets_frame_size_overflow_is_a_read_error(tmp_path):
+    b = bytearray(ets_bytes({0: [np.zeros((5, 6), np.uint16)]}, (6, 5)))
+    struct.pack_into("<II", b, 64 + 28, 0xFFFFFFFF, 0xFFFFFFFF)
+    p = _write(tmp_path, "s.ets", bytes(b))
+    with pytest.raises(ValueError) as err:
+        gt.VideoFile(p)
+    assert _status(err) == 15
