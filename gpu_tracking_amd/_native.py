@@ -53,6 +53,7 @@ class GtParams(C.Structure):
         ("noise_size", C.c_float),
         ("has_smoothing_size", C.c_int32), ("smoothing_size", C.c_uint32),
         ("illumination_correction_per_frame", C.c_int32),
+        ("threshold_rel", C.c_float),   # no reference counterpart (LoG; 0 = off)
     ]
 
 

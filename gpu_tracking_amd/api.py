@@ -518,7 +518,10 @@ def characterize_points(file_or_array, points_to_characterize, diameter=None, ha
     return _to_dataframe(data, names, types)
 
 
-_LOG_ONLY = dict(n_radii=10, log_spacing=False, overlap_threshold=0.0)
+# threshold_rel has no reference counterpart (default off): keep a scale-space
+# maximum only where its LoG value exceeds threshold_rel times the frame's
+# largest LoG value over every scale (skimage blob_log's threshold_rel)
+_LOG_ONLY = dict(n_radii=10, log_spacing=False, overlap_threshold=0.0, threshold_rel=0.0)
 _LOG_DEFAULTS = dict({k: v for k, v in _TRACKPY_DEFAULTS.items()
                       if k not in ("maxsize", "separation", "threshold", "invert", "percentile",
                                    "topn", "preprocess", "filter_close")}, **_LOG_ONLY)
@@ -540,6 +543,7 @@ def make_log_params(min_radius: float, max_radius: float, **kw) -> N.GtParams:
     p.n_radii = int(a["n_radii"])
     p.log_spacing = int(bool(a["log_spacing"]))
     p.overlap_threshold = float(a["overlap_threshold"])
+    p.threshold_rel = float(a["threshold_rel"] or 0.0)
     p.include_r_in_output = 1
     _common(p, a)
     return p

@@ -146,7 +146,16 @@ bool gt_host_derive(const gt_params *p, uint32_t H, uint32_t W, gt_derived *d, s
             }
         }
         d->overlap_threshold = p->overlap_threshold;
+        if (!(p->threshold_rel >= 0.0f) || !std::isfinite(p->threshold_rel)) {
+            *why = "threshold_rel must be a finite number >= 0";
+            return false;
+        }
+        d->threshold_rel = p->threshold_rel;
     } else {
+        if (p->threshold_rel != 0.0f) {
+            *why = "threshold_rel applies to the LoG style only";
+            return false;
+        }
         if (p->diameter == 0) {
             *why = "diameter must be at least 1";
             return false;
@@ -311,7 +320,7 @@ void gt_host_fill_kparams(const gt_derived *d, uint32_t H, uint32_t W, GtKParams
     kp->image_std = 0.0f;
     kp->rough = d->rough;
     kp->snr = d->snr;
-    kp->pad_ = 0;
+    kp->thr_rel = d->threshold_rel;
     kp->fstd = nullptr;   // set by the context when the noise estimate runs
     kp->characterize = d->characterize;
     kp->circle = static_cast<int32_t>(d->circle);

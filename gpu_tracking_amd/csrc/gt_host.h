@@ -41,6 +41,7 @@ struct gt_derived {
     // LoG style
     std::vector<float> radii;   // the radius scan (gpu_setup.rs:958-984)
     float overlap_threshold = 0.0f;
+    float threshold_rel = 0.0f; // 0: off
     int32_t bg_radius_given = 0;  // LoG: else the ring's outer radius is 2 r per row
 };
 

@@ -60,14 +60,19 @@ hipError_t gt_launch_characterize_det(const float *proc, const void *raw, int dt
                                       uint32_t *dcount = nullptr);
 // LoG style: scale-space maxima of every scale of a chunk (planes kept in
 // LDS), then their walks.  Candidate seeds are scale * H * W + raster index.
+// With cval (threshold_rel on) the tile kernel also stores each candidate's
+// LoG value at cval[f * cap + i] and the frame's largest LoG value (f32 bits)
+// at ccount[f * GT_COUNTER_STRIDE + 1], which must start at zero; the walk
+// kernel then skips candidates whose value is not above p.thr_rel times it.
 size_t gt_log_lds_bytes(int Rm);
 hipError_t gt_launch_log_tile(const void *frames, int dtype, const float *proc, uint32_t nframes,
                               const GtKParams &p, const GtLogTable *tab, int Rm, uint32_t *cand,
-                              uint32_t cap, uint32_t *ccount, hipStream_t s);
+                              uint32_t cap, uint32_t *ccount, float *cval, hipStream_t s);
 hipError_t gt_launch_walk_log(const float *proc, const void *raw, int dtype, uint32_t nframes,
                               const GtKParams &p, const GtLogTable *tab, const uint32_t *cand,
                               uint32_t cap, const uint32_t *ccount, gt_detection *out,
-                              uint32_t out_cap, uint32_t *ocount, uint32_t frame0, hipStream_t s);
+                              uint32_t out_cap, uint32_t *ocount, uint32_t frame0,
+                              const float *cval, hipStream_t s);
 // Per-frame detection counts, 0xFFFFFFFF where the candidate list overflowed.
 hipError_t gt_launch_copy_counts_checked(const uint32_t *ocount, const uint32_t *ccount,
                                          uint32_t ccap, uint32_t *dst, uint32_t n, hipStream_t s);

@@ -2647,7 +2647,7 @@ __global__ __launch_bounds__(LG_THREADS) void k_log_tile(const T *__restrict__ r
                                                          uint32_t *__restrict__ ccount,
                                                          int nframes,
                                                          const float *__restrict__ fstd, float snr,
-                                                         float rough) {
+                                                         float rough, float *__restrict__ cval) {
     extern __shared__ float lds[];
     const int ntx = (W + LG_TW - 1) / LG_TW, nty = (H + LG_TH - 1) / LG_TH;
     const int per_frame = ntx * nty;
@@ -2701,12 +2701,17 @@ __global__ __launch_bounds__(LG_THREADS) void k_log_tile(const T *__restrict__ r
                 if (flag) flag = !(P[static_cast<size_t>(gy) * W + gx] < snr_thr);
             }
             const uint32_t pos = wave_append(flag, cnt);
-            if (flag && pos < cap)
+            if (flag && pos < cap) {
                 fc[pos] = static_cast<uint32_t>(m) * static_cast<uint32_t>(HW) +
                           static_cast<uint32_t>(gy) * static_cast<uint32_t>(W) +
                           static_cast<uint32_t>(gx);
+                if (cval) cval[static_cast<size_t>(fr) * cap + pos] = Lm[r * LG_RC + c];
+            }
         }
     };
+    // threshold_rel: the largest LoG value of the tile's own pixels over
+    // every scale (each frame pixel is in one tile's interior)
+    float lmax = 0.0f;
     for (int k = 0; k < n; ++k) {
         const int R = tab->R[k], ntaps = 2 * R + 1;
         const float *tg = tab->taps + tab->off[k];
@@ -2738,12 +2743,25 @@ __global__ __launch_bounds__(LG_THREADS) void k_log_tile(const T *__restrict__ r
                 float l = 0.0f - a0[m] * n1;
                 l = l - a1[m] * n3;
                 Lk[r * LG_RC + c0 + m] = l;
+                if (cval) {
+                    const int c = c0 + m;
+                    if (r >= 1 && r <= LG_TH && c >= 1 && c <= LG_TW && y0 + r - 1 < H &&
+                        x0 + c - 1 < W)
+                        lmax = fmaxf(lmax, l);
+                }
             }
         }
         __syncthreads();
         if (k >= 1) test(k - 1);
     }
     test(n - 1);
+    if (cval) {
+        // one atomic per wave: non-negative floats order as their bit patterns
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) lmax = fmaxf(lmax, __shfl_xor(lmax, o));
+        if (__lane_id() == 0 && lmax > 0.0f)
+            atomicMax(&ccount[fr * GT_COUNTER_STRIDE + 1], __float_as_uint(lmax));
+    }
 }
 
 // Walk + characterize of the LoG candidates (walk.wgsl with r > 0: rint =
@@ -2759,10 +2777,15 @@ __global__ __launch_bounds__(WK_THREADS) void k_walk_log(const float *__restrict
                                                          gt_detection *__restrict__ out,
                                                          uint32_t out_cap,
                                                          uint32_t *__restrict__ ocount,
-                                                         uint32_t frame0) {
+                                                         uint32_t frame0,
+                                                         const float *__restrict__ cval) {
     const int fr = blockIdx.y;
     const int H = p.H, W = p.W;
     const uint32_t HW = static_cast<uint32_t>(H) * static_cast<uint32_t>(W);
+    // threshold_rel: the candidate's LoG value must exceed thr_rel times the
+    // frame's largest LoG value (k_log_tile's per-frame maximum)
+    const float rel_thr =
+        cval ? p.thr_rel * __uint_as_float(ccount[fr * GT_COUNTER_STRIDE + 1]) : 0.0f;
     const GlobalSrc src{proc + static_cast<size_t>(fr) * HW, W};
     const T *Rw = raw + static_cast<size_t>(fr) * HW;
     const uint32_t nc = ccount[fr * GT_COUNTER_STRIDE];
@@ -2775,7 +2798,7 @@ __global__ __launch_bounds__(WK_THREADS) void k_walk_log(const float *__restrict
         const uint32_t i = i0 + lane;
         bool keep = false;
         gt_detection d;
-        if (i < n) {
+        if (i < n && (!cval || cval[static_cast<size_t>(fr) * cap + i] > rel_thr)) {
             const uint32_t seed = fc[i];
             const uint32_t m = seed / HW;
             const uint32_t ras = seed - m * HW;
@@ -2814,7 +2837,7 @@ __global__ void k_copy_counts_checked(const uint32_t *__restrict__ ocount,
 template <typename T>
 hipError_t launch_log_tile_t(const T *raw, const float *proc, uint32_t nframes, const GtKParams &p,
                              const GtLogTable *tab, int Rm, uint32_t *cand, uint32_t cap,
-                             uint32_t *ccount, hipStream_t s) {
+                             uint32_t *ccount, float *cval, hipStream_t s) {
     const LogGeom g = log_geom(Rm);
     const size_t lds = log_lds_bytes(g);
     if (lds > 65536) {
@@ -2827,7 +2850,7 @@ hipError_t launch_log_tile_t(const T *raw, const float *proc, uint32_t nframes, 
                           ((p.H + LG_TH - 1) / LG_TH) * nframes;
     k_log_tile<T><<<xcd_grid(ntiles), LG_THREADS, lds, s>>>(raw, proc, tab, g, p.H, p.W, cand, cap,
                                                             ccount, static_cast<int>(nframes),
-                                                            p.fstd, p.snr, p.rough);
+                                                            p.fstd, p.snr, p.rough, cval);
     return hipGetLastError();
 }
 
@@ -2835,12 +2858,12 @@ template <typename T>
 hipError_t launch_walk_log_t(const float *proc, const T *raw, uint32_t nframes, const GtKParams &p,
                              const GtLogTable *tab, const uint32_t *cand, uint32_t cap,
                              const uint32_t *ccount, gt_detection *out, uint32_t out_cap,
-                             uint32_t *ocount, uint32_t frame0, hipStream_t s) {
+                             uint32_t *ocount, uint32_t frame0, const float *cval, hipStream_t s) {
     // enough blocks to cover a typical frame's list in one pass
     const uint32_t blocks = std::min<uint32_t>(1024u, std::max<uint32_t>(1u, cap / WK_THREADS / 4u));
     const dim3 grid(blocks, nframes, 1);
     k_walk_log<T><<<grid, WK_THREADS, 0, s>>>(proc, raw, p, tab, cand, cap, ccount, out, out_cap,
-                                              ocount, frame0);
+                                              ocount, frame0, cval);
     return hipGetLastError();
 }
 
@@ -2850,17 +2873,17 @@ size_t gt_log_lds_bytes(int Rm) { return log_lds_bytes(log_geom(Rm)); }
 
 hipError_t gt_launch_log_tile(const void *frames, int dtype, const float *proc, uint32_t nframes,
                               const GtKParams &p, const GtLogTable *tab, int Rm, uint32_t *cand,
-                              uint32_t cap, uint32_t *ccount, hipStream_t s) {
+                              uint32_t cap, uint32_t *ccount, float *cval, hipStream_t s) {
     switch (dtype) {
     case GT_U16:
         return launch_log_tile_t(static_cast<const uint16_t *>(frames), proc, nframes, p, tab, Rm,
-                                 cand, cap, ccount, s);
+                                 cand, cap, ccount, cval, s);
     case GT_F32:
         return launch_log_tile_t(static_cast<const float *>(frames), proc, nframes, p, tab, Rm,
-                                 cand, cap, ccount, s);
+                                 cand, cap, ccount, cval, s);
     case GT_U8:
         return launch_log_tile_t(static_cast<const uint8_t *>(frames), proc, nframes, p, tab, Rm,
-                                 cand, cap, ccount, s);
+                                 cand, cap, ccount, cval, s);
     default: return hipErrorInvalidValue;
     }
 }
@@ -2868,17 +2891,18 @@ hipError_t gt_launch_log_tile(const void *frames, int dtype, const float *proc, 
 hipError_t gt_launch_walk_log(const float *proc, const void *raw, int dtype, uint32_t nframes,
                               const GtKParams &p, const GtLogTable *tab, const uint32_t *cand,
                               uint32_t cap, const uint32_t *ccount, gt_detection *out,
-                              uint32_t out_cap, uint32_t *ocount, uint32_t frame0, hipStream_t s) {
+                              uint32_t out_cap, uint32_t *ocount, uint32_t frame0, const float *cval,
+                              hipStream_t s) {
     switch (dtype) {
     case GT_U16:
         return launch_walk_log_t(proc, static_cast<const uint16_t *>(raw), nframes, p, tab, cand,
-                                 cap, ccount, out, out_cap, ocount, frame0, s);
+                                 cap, ccount, out, out_cap, ocount, frame0, cval, s);
     case GT_F32:
         return launch_walk_log_t(proc, static_cast<const float *>(raw), nframes, p, tab, cand, cap,
-                                 ccount, out, out_cap, ocount, frame0, s);
+                                 ccount, out, out_cap, ocount, frame0, cval, s);
     case GT_U8:
         return launch_walk_log_t(proc, static_cast<const uint8_t *>(raw), nframes, p, tab, cand,
-                                 cap, ccount, out, out_cap, ocount, frame0, s);
+                                 cap, ccount, out, out_cap, ocount, frame0, cval, s);
     default: return hipErrorInvalidValue;
     }
 }

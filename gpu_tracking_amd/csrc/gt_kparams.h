@@ -40,7 +40,7 @@ struct GtKParams {
     int32_t characterize;
     int32_t circle;
     float snr;
-    int32_t pad_;
+    float thr_rel;                // LoG threshold_rel (0: off)
     // per-frame noise estimate of the launch's frames (device; null: the
     // constants max_thr and image_std apply to every frame)
     const float *fstd;

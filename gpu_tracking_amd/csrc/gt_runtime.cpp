@@ -97,6 +97,7 @@ struct gt_context {
     bool log = false;              // LoG style
     GtLogTable *d_logtab = nullptr;  // LoG scale table (device copy)
     int32_t log_Rm = 0;
+    float *d_cval = nullptr;       // LoG threshold_rel: chunk * cand_cap candidate values
 
     // learned centroid proposal (tiled Trackpy style only)
     const gt_cnn *cnn = nullptr;   // caller's network; null = plain walk
@@ -223,6 +224,8 @@ gt_status alloc_scratch(gt_context *c) {
     }
     if (!c->tiled)
         GT_HIP(hipMalloc(&c->d_cand, sizeof(uint32_t) * static_cast<size_t>(c->cand_cap) * c->chunk));
+    if (c->log && c->der.threshold_rel > 0.0f)
+        GT_HIP(hipMalloc(&c->d_cval, sizeof(float) * static_cast<size_t>(c->cand_cap) * c->chunk));
     GT_HIP(hipMalloc(&c->d_ccount, sizeof(uint32_t) * GT_COUNTER_STRIDE * c->chunk));
     GT_HIP(hipMalloc(&c->d_ocount, sizeof(uint32_t) * GT_COUNTER_STRIDE * c->chunk));
     if (c->tiled) {
@@ -277,6 +280,7 @@ gt_status reserve_host_rows(gt_context *c, size_t n) {
 void free_all(gt_context *c) {
     if (c->d_proc) (void)hipFree(c->d_proc);
     if (c->d_cand) (void)hipFree(c->d_cand);
+    if (c->d_cval) (void)hipFree(c->d_cval);
     if (c->d_ccount) (void)hipFree(c->d_ccount);
     if (c->d_ocount) (void)hipFree(c->d_ocount);
     if (c->d_slow) (void)hipFree(c->d_slow);
@@ -493,12 +497,13 @@ gt_status run_pipeline(gt_context *c, const void *d_frames, int32_t dtype, uint3
             // scale-space maxima of all scales, then their walks (and, with
             // adaptive_background, the acceptance passes before characterize)
             GT_HIP(gt_launch_log_tile(frames, fdtype, proc, nf, c->kp, c->d_logtab, c->log_Rm,
-                                      c->d_cand, c->cand_cap, c->d_ccount, cs));
+                                      c->d_cand, c->cand_cap, c->d_ccount, c->d_cval, cs));
             if (ev) GT_HIP(hipEventRecord(ev[2], cs));
             GtKParams kw = c->kp;
             if (c->der.adaptive) kw.characterize = 0;
             GT_HIP(gt_launch_walk_log(proc, frames, fdtype, nf, kw, c->d_logtab, c->d_cand,
-                                      c->cand_cap, c->d_ccount, out, cap, ocount, f0, cs));
+                                      c->cand_cap, c->d_ccount, out, cap, ocount, f0, c->d_cval,
+                                      cs));
             if (c->der.adaptive) {
                 gt_status ast = run_adaptive(c, frames, fdtype, nf, out, cap, cs);
                 if (ast != GT_OK) return ast;
@@ -674,7 +679,7 @@ gt_status gt_context_create(const gt_params *params, uint32_t nrows, uint32_t nc
     // per frame of device scratch: processed plane and candidate list (the
     // two gt_execute staging slots, frame + detection buffer each, are
     // allocated on first use)
-    const size_t per_frame = HW * (4 + 4);
+    const size_t per_frame = HW * (4 + 4 + (c->log && der.threshold_rel > 0.0f ? 4 : 0));
     size_t chunk = (size_t(4) << 30) / (per_frame ? per_frame : 1);
     c->chunk = static_cast<uint32_t>(std::max<size_t>(1, std::min<size_t>(chunk, 256)));
     // host frames are staged 64 at a time: smaller pinned slots, and the

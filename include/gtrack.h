@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-#define GT_ABI_VERSION 1
+#define GT_ABI_VERSION 2
 
 /* One code per variant of the reference's error enum (src/error.rs:6-94),
  * in declaration order, plus two codes that have no reference counterpart. */
@@ -107,6 +107,13 @@ typedef struct gt_params {
     float noise_size;
     int32_t has_smoothing_size; uint32_t smoothing_size;
     int32_t illumination_correction_per_frame;
+    /* No reference counterpart (0 = off, the default).  LoG style only: a
+     * scale-space maximum is kept when its LoG value exceeds threshold_rel
+     * times the largest LoG value of its frame over every scale and pixel
+     * (skimage blob_log's threshold_rel; the reference allocates a global
+     * maximum buffer for this but never writes it, src/gpu_setup.rs:481-488,
+     * src/shaders/log_style/logspace_max.wgsl:48-49). */
+    float threshold_rel;
 } gt_params;
 
 /* ResultRow (src/execute_gpu.rs:48-61): 10 x f32, x = row, y = column. */

@@ -27,6 +27,7 @@ class Params(C.Structure):
         ("max_iterations", C.c_uint32), ("threshold", C.c_float), ("minmass", C.c_float),
         ("margin", C.c_int32), ("snr", C.c_float), ("minmass_snr", C.c_float),
         ("rough_snr_factor", C.c_float), ("image_std", C.c_float), ("truncate", C.c_int32),
+        ("threshold_rel", C.c_float),
     ]
 
 
@@ -251,7 +252,8 @@ def centroid_f64(processed64, p: Params, pu: int, pv: int):
 
 def derive_log(nrows: int, ncols: int, max_radius: float, smoothing_size: int | None = None,
                noise_size: float = 1.0, minmass: float = 0.0, max_iterations: int = 10,
-               shift_threshold: float = 0.6, truncate_preprocessed: bool = True) -> Params:
+               shift_threshold: float = 0.6, truncate_preprocessed: bool = True,
+               threshold_rel: float = 0.0) -> Params:
     """gpu_setup.rs:549-557 + :562-579 (Log branch): smoothing side
     2 * u32(max_radius + 0.5) + 1 unless given; no circle, dilation or margin."""
     p = Params()
@@ -272,6 +274,7 @@ def derive_log(nrows: int, ncols: int, max_radius: float, smoothing_size: int | 
     p.rough_snr_factor = 1.0
     p.image_std = 0.0
     p.truncate = int(truncate_preprocessed)
+    p.threshold_rel = threshold_rel
     return p
 
 

@@ -56,6 +56,7 @@ void or_derive_trackpy(uint32_t diameter, uint32_t separation, int32_t smoothing
     out->rough_snr_factor = adaptive_background ? 0.5f : 1.0f;
     out->image_std = 0.0f;
     out->truncate = truncate;
+    out->threshold_rel = 0.0f;
 }
 
 void or_gauss_weights(int32_t s, float sigma2, float *w, float *norm) {
@@ -732,6 +733,21 @@ static or_row *or_log_rows(const float *frame, const or_params *p, const float *
     size_t total = 0, ncand = 0;
     or_row *all = NULL;
     const float sq2 = sqrtf(2.0f);
+    /* threshold_rel (no reference counterpart; the reference allocates a
+     * global_max buffer it never writes, gpu_setup.rs:481-488): a first pass
+     * takes the largest LoG value of the frame over every scale, then a
+     * candidate stays when its value exceeds threshold_rel times it. */
+    float rel_thr = 0.0f;
+    const int use_rel = p->threshold_rel > 0.0f;
+    if (use_rel) {
+        float gmax = 0.0f;
+        for (uint32_t m = 0; m < n_radii; m++) {
+            or_log_plane(frame, H, W, radii[m] / sq2, planes, t1, t2);
+            for (size_t i = 0; i < hw; i++)
+                if (planes[i] > gmax) gmax = planes[i];
+        }
+        rel_thr = p->threshold_rel * gmax;
+    }
     for (uint32_t m = 0; m < n_radii; m++) {
         /* planes needed: m-1, m, m+1 (ring of 3 by scale index) */
         if (m == 0) or_log_plane(frame, H, W, radii[0] / sq2, planes, t1, t2);
@@ -742,6 +758,12 @@ static or_row *or_log_rows(const float *frame, const or_params *p, const float *
         const float *top = m + 1 < n_radii ? planes + ((m + 1) % 3) * hw : NULL;
         const float thr = p->image_std * p->snr * p->rough_snr_factor;
         size_t nc = or_log_max(bot, mid, top, proc, H, W, thr, cand, hw);
+        if (use_rel) {
+            size_t kept = 0;
+            for (size_t c = 0; c < nc; c++)
+                if (mid[cand[c]] > rel_thr) cand[kept++] = cand[c];
+            nc = kept;
+        }
         ncand += nc;
         for (size_t c = 0; c < nc; c++) cr[c] = radii[m];
         all = (or_row *)realloc(all, sizeof(or_row) * (total + nc + 1));

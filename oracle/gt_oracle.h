@@ -42,6 +42,10 @@ typedef struct {
     float rough_snr_factor;
     float image_std;          /* 0 unless an snr criterion is active */
     int32_t truncate;         /* truncate_preprocessed */
+    /* LoG only, no reference counterpart (0 = off): a scale-space maximum is
+     * kept when its LoG value exceeds threshold_rel times the largest LoG
+     * value of the frame over every scale (skimage blob_log's rule). */
+    float threshold_rel;
 } or_params;
 
 /* ResultRow (src/execute_gpu.rs:48-61) plus the raster index of the

@@ -27,7 +27,7 @@ def test_library_exports_every_declared_symbol():
     for name in declared:
         assert hasattr(L, name), name
     assert sorted(N.EXPORTS) == declared
-    assert L.gt_abi_version() == 1
+    assert L.gt_abi_version() == 2
 
 
 def test_struct_layouts():

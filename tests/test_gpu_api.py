@@ -198,3 +198,22 @@ def test_batch_linker_reset_points(gt, oracle):
     # ids start over at the reset point
     assert b["particle"].min() == 0 and a["particle"].min() == 0
     assert set(b[b.frame == 3]["particle"]) == set(range(len(b[b.frame == 3])))
+
+
+def test_batch_across_staging_slots_matches_the_oracle(gt, oracle):
+    # gt_execute stages host frames 64 at a time in two alternating pinned
+    # slots; 150 frames take three slots, so slot reuse, the GPU doughnut and
+    # filter_close buffers and the linker's state all cross slot boundaries
+    from oracle import linker as OL
+    v = _video(T=150, seed=21, H=128, W=128, n=14)
+    print(f"[slots] batch() over {v.shape}", flush=True)
+    df = gt.batch(v, 9, minmass=300, search_range=5.0, memory=2)
+    assert list(df.columns) == ["frame", "y", "x", "mass", "Rg", "raw", "signal", "ecc",
+                                "raw_mass", "raw_bg_median", "raw_mass_corrected", "particle",
+                                "duration"]
+    ref = _expected(oracle, v, 9, 10, 300.0, True, True)
+    got = _frame_matrix(df)
+    assert got.shape[0] == ref.shape[0] and got.shape[0] > 150
+    assert set(df["frame"]) >= {0, 63, 64, 127, 128, 149}
+    assert np.array_equal(got[:, :-2].view(np.uint32), ref.view(np.uint32))
+    assert df["particle"].tolist() == OL.link_frames(ref[:, :3], 5.0, 2)

@@ -145,3 +145,52 @@ def test_batch_rejects_bad_input(torch_dev):
         f32 = torch_dev.zeros((32, 32), dtype=torch_dev.float32, device="cuda")
         with pytest.raises(TypeError):
             b.detect([f32])
+
+
+# --------------------------------------------------------------------------
+# The batched workload's own frame sizes in one call, every frame checked
+# against the oracle (not only against one-size HIP runs).  Sizes (rows,
+# columns) and generator as the batched workload: the full workload's
+# generator at 1 particle per 1000 px^2, its detection parameters.
+WORKLOAD_SIZES = [(256, 256), (512, 384), (1024, 1024), (2048, 1024)]
+FIELDS = ["x", "y", "mass", "r", "max_intensity", "Rg", "raw_mass", "signal", "ecc", "count"]
+
+
+@pytest.mark.parametrize("graph", [True, False])
+def test_batch_workload_sizes_match_the_oracle(torch_dev, oracle, graph):
+    import gpu_tracking_amd as gt
+    from gpu_tracking_amd import synth
+    frames, ids, host = [], [], []
+    for k, (H, W) in enumerate(WORKLOAD_SIZES):
+        s = synth.spec("full", nrows=H, ncols=W, n_particles=H * W // 1000)
+        v = synth.video_numpy(s, 2, seed=20261203 + 4000 + k)
+        d = torch_dev.from_numpy(v.view(np.int16)).cuda()
+        for t in range(2):
+            frames.append(d[t])
+            ids.append(t)
+            host.append(v[t])
+    # interleave sizes so that launches mix them
+    order = [0, 2, 4, 6, 1, 3, 5, 7]
+    frames = [frames[i] for i in order]
+    ids = [ids[i] for i in order]
+    host = [host[i] for i in order]
+    print(f"[batch oracle] graph={graph} sizes {[f.shape for f in frames]}", flush=True)
+    cap = 8192
+    with gt.Batch(gt.make_params(9, **PARAMS), max_frame_pixels=2048 * 1024,
+                  max_launch_pixels=3 << 20, max_slots=8) as b:
+        b.set_graph(graph)
+        out, counts = b.detect(frames, frame_ids=ids, cap_per_frame=cap)
+        torch_dev.cuda.synchronize()
+        assert b.schedule()["launches_last_call"] > 1
+        got = gt.detections_to_numpy(out, counts)
+    for i, fr in enumerate(host):
+        H, W = fr.shape
+        p = oracle.derive_trackpy(H, W, diameter=9, separation=7, minmass=400.0)
+        ref, _ = oracle.detect_frame(fr, p, characterize=True)
+        ref = np.sort(ref, order="seed")
+        g = got[i]
+        assert len(g) == len(ref) and len(ref) > 0, (i, fr.shape, len(g), len(ref))
+        assert np.array_equal(g["seed"], ref["seed"]), i
+        assert np.all(g["frame"] == ids[i])
+        for f in FIELDS:
+            assert np.array_equal(g[f].view(np.uint32), ref[f].view(np.uint32)), (i, f)

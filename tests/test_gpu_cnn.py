@@ -267,3 +267,35 @@ def test_set_learned_rejects_other_paths(torch_dev, trained):
             ctx.set_learned(net, tol=float("nan"))
         with pytest.raises(ValueError):
             ctx.set_learned(net, scope="some")
+
+
+@pytest.mark.parametrize("scope", ["moving", "all"])
+def test_learned_path_on_the_full_workload_matches_the_oracle(torch_dev, oracle, trained, scope):
+    # the evaluation shape and generator: 1024 x 1024 u16 frames of the full
+    # workload (1000 particles), against the oracle's full walk
+    import gpu_tracking_amd as gt
+    from gpu_tracking_amd import synth
+    net, _ = trained
+    v = synth.video_numpy(synth.spec("full"), 2, 20261203 + 5000)
+    print(f"[learned 1024] scope={scope} frames {v.shape}", flush=True)
+    frames = torch_dev.from_numpy(v.view(np.int16)).cuda()
+    p = gt.make_params(9, characterize=True, **PRM)
+    with gt.Context(p, 1024, 1024) as ctx:
+        ctx.set_learned(net, scope=scope)
+        learned = _rows(*ctx.detect_device(frames))
+        acc, cands = ctx.learned_stats(reset=True)
+    n_all = n_ok = 0
+    for t in range(v.shape[0]):
+        op = oracle.derive_trackpy(1024, 1024, diameter=9, **PRM)
+        ref, _ = oracle.detect_frame(v[t], op, characterize=True)
+        bs = {int(r["seed"]): r for r in learned[t]}
+        for r in ref:
+            n_all += 1
+            q = bs.get(int(r["seed"]))
+            if q is not None and abs(q["x"] - r["x"]) <= 1e-3 and abs(q["y"] - r["y"]) <= 1e-3:
+                n_ok += 1
+    frac = n_ok / n_all
+    print(f"\n1024^2 {scope}: accepted {acc}/{cands} proposals; {n_ok}/{n_all} oracle particles "
+          "within 1e-3 px")
+    assert acc > 0 and n_all > 1500
+    assert frac >= 0.999, frac

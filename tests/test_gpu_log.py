@@ -134,3 +134,37 @@ def test_log_capacity_overflow_reported(torch_dev):
         out, counts = ctx.detect_device(torch_dev.from_numpy(video).cuda(), cap_per_frame=8)
         torch_dev.cuda.synchronize()
         assert (int(counts[0].item()) & 0xFFFFFFFF) > 8
+
+
+# --------------------------------------------------------------------------
+# The LoG workload at its own shape: one 2048 x 2048 f32 frame of the LoG
+# workload generator (3000 particles, sigma U[1, 5], amplitude U[200, 2000],
+# background 50 + N(0, 5^2)); radii 1.5 .. 6, 10 log-spaced scales; without
+# and with threshold_rel = 0.05 (no reference counterpart).
+@pytest.fixture(scope="module")
+def log_workload(oracle):
+    from concurrent.futures import ThreadPoolExecutor
+    from gpu_tracking_amd import synth
+    frame = synth.video_numpy(synth.spec("log"), 1, 31)
+    radii = oracle.log_radii(1.5, 6.0, 10, True)
+
+    def ref(rel):
+        p = oracle.derive_log(2048, 2048, 6.0, threshold_rel=rel)
+        return oracle.log_detect_frame(frame[0], p, radii, overlap_threshold=1.0)
+
+    # the oracle's C code releases the GIL: both variants at once
+    with ThreadPoolExecutor(2) as ex:
+        refs = dict(zip((0.0, 0.05), ex.map(ref, (0.0, 0.05))))
+    return frame, refs
+
+
+@pytest.mark.parametrize("rel", [0.0, 0.05])
+def test_log_workload_2048_bit_identical(torch_dev, log_workload, rel):
+    print(f"[log workload] 2048x2048 f32, threshold_rel={rel}", flush=True)
+    frame, refs = log_workload
+    ref, nc = refs[rel]
+    got = _gpu_rows(torch_dev, frame, 1.5, 6.0, n_radii=10, log_spacing=True, threshold_rel=rel)
+    _assert_rows_equal(got[0], ref, f"threshold_rel={rel}")
+    if rel > 0:
+        # the rule keeps the particle-sized maxima and drops the noise
+        assert 1000 < len(ref) < len(refs[0.0][0]) // 10

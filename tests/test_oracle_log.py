@@ -274,3 +274,42 @@ def test_log_frame_finds_blobs_at_their_scale(oracle):
         assert d[k] < 1.0
         assert abs(rows["r"][k] - 2.0 * math.sqrt(2)) <= 1.0
     assert np.all(rows["r"] > 0) and nc >= len(pos)
+
+
+# --------------------------------------------------------------------------
+# threshold_rel (no reference counterpart: skimage blob_log's relative
+# threshold; the reference allocates a global_max it never writes,
+# gpu_setup.rs:481-488)
+@pytest.mark.parametrize("rel", [0.05, 0.3])
+def test_threshold_rel_brute_force(oracle, rel):
+    from gpu_tracking_amd import synth
+    H, W = 40, 44
+    s = synth.spec("log", nrows=H, ncols=W, n_particles=8, sigma=(1.0, 3.0),
+                   amp=("uniform", 200.0, 2000.0), noise="gauss:5", min_dist=8.0, edge=6.0)
+    frame = synth.video_numpy(s, 1, seed=9)[0]
+    radii = oracle.log_radii(1.5, 6.0, 5, True)
+    planes = [oracle.log_plane(frame, float(r) / math.sqrt(2.0)) for r in np.float32(radii)]
+    gmax = max(0.0, max(float(pl.max()) for pl in planes))
+    thr = np.float32(rel) * np.float32(gmax)
+    P = oracle.bandpass(frame, oracle.derive_log(H, W, 6.0))
+    want, unfiltered = set(), set()
+    for m in range(len(radii)):
+        B = planes[m - 1] if m > 0 else None
+        T = planes[m + 1] if m + 1 < len(radii) else None
+        for c in _brute_max(B, planes[m], T, P, 0.0).tolist():
+            unfiltered.add(m * H * W + c)
+            if planes[m].flat[c] > thr:
+                want.add(m * H * W + c)
+    # minmass 0 keeps every walk and overlap 1 prunes nothing, so the rows'
+    # seeds are the candidate set
+    p = oracle.derive_log(H, W, 6.0, threshold_rel=rel)
+    rows, nc = oracle.log_detect_frame(frame, p, radii, characterize=False, overlap_threshold=1.0)
+    assert nc == len(want) and set(rows["seed"].tolist()) == want
+    assert len(want) < len(unfiltered)          # the rule removed candidates
+    off, nc_off = oracle.log_detect_frame(frame, oracle.derive_log(H, W, 6.0), radii,
+                                          characterize=False, overlap_threshold=1.0)
+    assert nc_off == len(unfiltered) and set(off["seed"].tolist()) == unfiltered
+    # the kept candidates walk exactly as without the rule
+    keep = np.isin(off["seed"], rows["seed"])
+    assert np.array_equal(np.sort(off[keep], order="seed").view(np.uint32),
+                          np.sort(rows, order="seed").view(np.uint32))
