@@ -315,6 +315,66 @@ class Context:
             t.join()
         return box["st"], interrupted
 
+    # one frame at a time (submit_work / get_work) ----------------------------
+    def submit_frame(self, frame: np.ndarray, frame_idx: int, positions=None) -> None:
+        """gt_submit_frame: stage one host frame (and, for characterize_points
+        contexts, its positions: an (n, 2) [y, x] or (n, 3) [y, x, r] array)
+        and queue its detection; returns without waiting."""
+        frame = np.ascontiguousarray(frame)
+        if tuple(frame.shape) != self.shape:
+            raise N.error(4, f"dimension mismatch. provided dimensions are {self.shape}, "
+                             f"but frame {frame_idx} has shape {frame.shape}")
+        pos, npos, buf = None, 0, None
+        if positions is not None:
+            pa = np.asarray(positions, np.float32).reshape(-1, np.shape(positions)[-1])
+            buf = np.zeros((len(pa), 10), np.float32)     # ResultRow: x = row, y = column
+            buf[:, 0] = pa[:, 0]
+            buf[:, 1] = pa[:, 1]
+            if pa.shape[1] > 2:
+                buf[:, 3] = pa[:, 2]
+            pos, npos = buf.ctypes.data, len(pa)
+        N.check(self._lib.gt_submit_frame(self._h, frame.ctypes.data, _dtype_code(frame),
+                                          frame.size, int(frame_idx), pos, npos))
+
+    def fetch_frame(self, cap: int = 4096):
+        """gt_fetch_frame: (frame_idx, rows) of the oldest frame in flight;
+        rows is a structured array of the ResultRow fields in ascending seed
+        order (fetched again with a larger buffer when the frame has more
+        than cap rows)."""
+        idx, n = C.c_uint64(0), C.c_uint32(0)
+        dt = np.dtype([(f, "<f4") for f, _ in N.GtResultRow._fields_])
+        while True:
+            out = np.zeros(max(1, cap), dt)
+            st = self._lib.gt_fetch_frame(self._h, C.byref(idx), out.ctypes.data, cap,
+                                          C.byref(n))
+            if st == 101 and n.value > cap:
+                cap = n.value
+                continue
+            N.check(st)
+            return int(idx.value), out[:n.value].copy()
+
+    def set_inflight(self, depth: int) -> None:
+        """Frames submit_frame may have in flight (1..16, default 2)."""
+        N.check(self._lib.gt_context_set_inflight(self._h, int(depth)))
+
+    def stream(self, frames, positions=None, depth: int = 2):
+        """Detection of an iterable of host frames one at a time, as
+        execute_gpu's loop drives submit_work / get_work: up to `depth` frames
+        are submitted before the oldest is fetched.  Yields (frame_idx, rows)
+        in frame order.  positions: for characterize_points contexts, a
+        callable frame_idx -> positions array."""
+        self.set_inflight(depth)
+        inflight = 0
+        for k, fr in enumerate(frames):
+            if inflight == depth:
+                yield self.fetch_frame()
+                inflight -= 1
+            self.submit_frame(fr, k, None if positions is None else positions(k))
+            inflight += 1
+        while inflight:
+            yield self.fetch_frame()
+            inflight -= 1
+
     def set_lanes(self, lanes: int) -> None:
         """Streams (1 or 2) the chunks of a detect_device call alternate
         between (plain Trackpy-style detection); results do not depend on it."""

@@ -166,6 +166,24 @@ struct gt_context {
     gt_detection *h_rows = nullptr;     // pinned read-back buffer
     size_t h_rows_cap = 0;
 
+    // gt_submit_frame / gt_fetch_frame: a ring of frames in flight, each with
+    // its pinned frame, device frame, device rows and count; work runs on
+    // `stream` in submission order, read-back on ring_copy
+    struct RingSlot {
+        void *h_frame = nullptr;          // pinned, HW * 4 bytes
+        void *d_frame = nullptr;          // HW * 4 bytes
+        gt_detection *d_rows = nullptr;   // det_cap rows
+        uint32_t *d_count = nullptr;
+        uint32_t *h_count = nullptr;      // pinned
+        std::vector<gt_detection> h_pos;  // characterize_points positions of the frame
+        hipEvent_t done = nullptr;        // the count is on the host
+        uint64_t frame_idx = 0;
+    };
+    std::vector<RingSlot> ring;
+    uint32_t ring_depth = 2;
+    uint32_t ring_head = 0, ring_count = 0;  // oldest slot in flight, slots in flight
+    hipStream_t ring_copy = nullptr;
+
     // timing
     bool timing = false;
     std::vector<hipEvent_t> events;
@@ -277,6 +295,38 @@ gt_status reserve_host_rows(gt_context *c, size_t n) {
     return GT_OK;
 }
 
+void free_ring(gt_context *c) {
+    for (auto &r : c->ring) {
+        if (r.h_frame) (void)hipHostFree(r.h_frame);
+        if (r.d_frame) (void)hipFree(r.d_frame);
+        if (r.d_rows) (void)hipFree(r.d_rows);
+        if (r.d_count) (void)hipFree(r.d_count);
+        if (r.h_count) (void)hipHostFree(r.h_count);
+        if (r.done) (void)hipEventDestroy(r.done);
+    }
+    c->ring.clear();
+    if (c->ring_copy) (void)hipStreamDestroy(c->ring_copy);
+    c->ring_copy = nullptr;
+    c->ring_head = c->ring_count = 0;
+}
+
+gt_status alloc_ring(gt_context *c) {
+    if (!c->ring.empty()) return GT_OK;
+    const size_t HW = static_cast<size_t>(c->H) * c->W;
+    c->ring.resize(c->ring_depth);
+    for (auto &r : c->ring) {
+        GT_HIP(hipHostMalloc(&r.h_frame, 4 * HW, hipHostMallocDefault));
+        GT_HIP(hipMalloc(&r.d_frame, 4 * HW));
+        GT_HIP(hipMalloc(&r.d_rows, sizeof(gt_detection) * static_cast<size_t>(c->det_cap)));
+        GT_HIP(hipMalloc(&r.d_count, sizeof(uint32_t)));
+        GT_HIP(hipHostMalloc(reinterpret_cast<void **>(&r.h_count), sizeof(uint32_t),
+                             hipHostMallocDefault));
+        GT_HIP(hipEventCreateWithFlags(&r.done, hipEventDisableTiming));
+    }
+    GT_HIP(hipStreamCreateWithFlags(&c->ring_copy, hipStreamNonBlocking));
+    return GT_OK;
+}
+
 void free_all(gt_context *c) {
     if (c->d_proc) (void)hipFree(c->d_proc);
     if (c->d_cand) (void)hipFree(c->d_cand);
@@ -326,6 +376,7 @@ void free_all(gt_context *c) {
     if (c->d_dough_offs) (void)hipFree(c->d_dough_offs);
     if (c->copy_stream) (void)hipStreamDestroy(c->copy_stream);
     for (hipEvent_t e : c->events) (void)hipEventDestroy(e);
+    free_ring(c);
     if (c->stream) (void)hipStreamDestroy(c->stream);
 }
 
@@ -756,6 +807,8 @@ gt_status gt_detect_device(gt_context *c, const void *d_frames, int32_t dtype, u
     if (nframes == 0) return fail(GT_ERR_EMPTY_ITERATOR, "the provided video doesn't contain any frames.");
     if (!d_frames || !d_out || !d_counts || cap_per_frame == 0)
         return fail(GT_ERR_INVALID_ARGUMENT, "null device buffer or zero capacity");
+    if (c->ring_count)
+        return fail(GT_ERR_INVALID_ARGUMENT, "frames submitted with gt_submit_frame are in flight");
     hipStream_t s = static_cast<hipStream_t>(stream);  // NULL: the default stream
     if (c->der.illum_sigma > 0.0f && !c->der.illum_per_frame) {
         // the profile of this call's frames (the video it is given)
@@ -763,6 +816,134 @@ gt_status gt_detect_device(gt_context *c, const void *d_frames, int32_t dtype, u
         if (ist != GT_OK) return ist;
     }
     return run_pipeline(c, d_frames, dtype, nframes, d_out, cap_per_frame, d_counts, s);
+}
+
+// submit_work (src/execute_gpu.rs:95-422) for one host frame: staged into the
+// ring's pinned slot, then the copy, the pass sequence and the count's
+// read-back are queued on the context's stream.
+gt_status gt_submit_frame(gt_context *c, const void *px, int32_t dtype, uint64_t n_pixels,
+                          uint64_t frame_idx, const gt_result_row *pos, uint32_t npos) {
+    if (!c) return fail(GT_ERR_INVALID_ARGUMENT, "null context");
+    const size_t esz = dtype_size(dtype);
+    if (!esz) return fail(GT_ERR_INVALID_ARGUMENT, "unknown dtype %d", dtype);
+    if (!px) return fail(GT_ERR_INVALID_ARGUMENT, "null frame");
+    const size_t HW = static_cast<size_t>(c->H) * c->W;
+    if (n_pixels != HW)
+        return fail(GT_ERR_DIMENSION_MISMATCH,
+                    "dimension mismatch. provided dimensions are (%u, %u), but frame %llu contains "
+                    "%llu pixels",
+                    c->H, c->W, static_cast<unsigned long long>(frame_idx),
+                    static_cast<unsigned long long>(n_pixels));
+    if (c->characterize_points && (npos > 0 && !pos))
+        return fail(GT_ERR_INVALID_ARGUMENT, "null positions");
+    if (!c->characterize_points && (pos || npos))
+        return fail(GT_ERR_INVALID_ARGUMENT,
+                    "positions given to a context created without characterize_points");
+    if (c->characterize_points && npos > c->det_cap)
+        return fail(GT_ERR_CAPACITY, "%u positions for one frame exceed the capacity of %u", npos,
+                    c->det_cap);
+    if (c->der.illum_sigma > 0.0f && !c->der.illum_per_frame)
+        return fail(GT_ERR_INVALID_ARGUMENT,
+                    "illumination correction over the whole video needs every frame first "
+                    "(gt_execute); per-frame correction streams");
+    if (c->ring_count == c->ring_depth)
+        return fail(GT_ERR_INVALID_ARGUMENT, "%u frames are in flight: fetch one first",
+                    c->ring_count);
+    gt_status st = alloc_ring(c);
+    if (st != GT_OK) {
+        std::string msg = g_last_error;
+        free_ring(c);
+        return fail(st, "%s", msg.c_str());
+    }
+    gt_context::RingSlot &r = c->ring[(c->ring_head + c->ring_count) % c->ring_depth];
+    std::memcpy(r.h_frame, px, esz * HW);
+    GT_HIP(hipMemcpyAsync(r.d_frame, r.h_frame, esz * HW, hipMemcpyHostToDevice, c->stream));
+    if (c->characterize_points) {
+        // characterize at the given positions (submit_work's characterization
+        // branch, src/execute_gpu.rs:190-229): seed = position index
+        r.h_pos.resize(npos);
+        for (uint32_t i = 0; i < npos; ++i) {
+            r.h_pos[i].row = pos[i];
+            r.h_pos[i].seed = i;
+            r.h_pos[i].frame = static_cast<uint32_t>(frame_idx);
+        }
+        if (npos)
+            GT_HIP(hipMemcpyAsync(r.d_rows, r.h_pos.data(), sizeof(gt_detection) * npos,
+                                  hipMemcpyHostToDevice, c->stream));
+        *r.h_count = npos;
+        GT_HIP(hipMemcpyAsync(r.d_count, r.h_count, sizeof(uint32_t), hipMemcpyHostToDevice,
+                              c->stream));
+        const void *fr = r.d_frame;
+        int32_t fdt = dtype;
+        if (c->der.illum_sigma > 0.0f) {
+            gt_status ist = illum_correct(c, fr, dtype, 1, c->stream);
+            if (ist != GT_OK) return ist;
+            fr = c->d_corr;
+            fdt = GT_F32;
+        }
+        GT_HIP(gt_launch_bandpass(fr, fdt, c->d_proc, 1, c->kp, c->stream));
+        GT_HIP(gt_launch_characterize_rows(c->d_proc, fr, fdt, 1, c->kp, r.d_rows, c->det_cap,
+                                           r.d_count, npos, c->stream));
+    } else {
+        st = run_pipeline(c, r.d_frame, dtype, 1, r.d_rows, c->det_cap, r.d_count, c->stream);
+        if (st != GT_OK) return st;
+    }
+    GT_HIP(hipMemcpyAsync(r.h_count, r.d_count, sizeof(uint32_t), hipMemcpyDeviceToHost,
+                          c->stream));
+    GT_HIP(hipEventRecord(r.done, c->stream));
+    r.frame_idx = frame_idx;
+    ++c->ring_count;
+    return GT_OK;
+}
+
+// get_work (src/execute_gpu.rs:424-467): waits for the oldest frame in
+// flight and copies its rows out in ascending seed order.
+gt_status gt_fetch_frame(gt_context *c, uint64_t *frame_idx, gt_result_row *out, uint32_t cap,
+                         uint32_t *n) {
+    if (!c || !frame_idx || !n || (cap && !out)) return fail(GT_ERR_INVALID_ARGUMENT, "null argument");
+    if (c->ring_count == 0) return fail(GT_ERR_INVALID_ARGUMENT, "no frame is in flight");
+    gt_context::RingSlot &r = c->ring[c->ring_head];
+    GT_HIP(hipEventSynchronize(r.done));
+    const uint32_t cnt = *r.h_count;
+    *frame_idx = r.frame_idx;
+    auto pop = [&]() {
+        c->ring_head = (c->ring_head + 1) % c->ring_depth;
+        --c->ring_count;
+    };
+    if (cnt == 0xFFFFFFFFu || cnt > c->det_cap) {
+        pop();
+        *n = 0;
+        return fail(GT_ERR_CAPACITY, "frame %llu exceeded the candidate or detection capacity",
+                    static_cast<unsigned long long>(r.frame_idx));
+    }
+    *n = cnt;
+    if (cnt > cap)   // the frame stays in flight: fetch again with room for *n rows
+        return fail(GT_ERR_CAPACITY, "frame %llu has %u rows; the buffer holds %u",
+                    static_cast<unsigned long long>(r.frame_idx), cnt, cap);
+    gt_status st = reserve_host_rows(c, cnt);
+    if (st != GT_OK) return st;
+    if (cnt) {
+        GT_HIP(hipMemcpyAsync(c->h_rows, r.d_rows, sizeof(gt_detection) * cnt,
+                              hipMemcpyDeviceToHost, c->ring_copy));
+        GT_HIP(hipStreamSynchronize(c->ring_copy));
+        std::sort(c->h_rows, c->h_rows + cnt,
+                  [](const gt_detection &a, const gt_detection &b) { return a.seed < b.seed; });
+        for (uint32_t i = 0; i < cnt; ++i) out[i] = c->h_rows[i].row;
+    }
+    pop();
+    return GT_OK;
+}
+
+gt_status gt_context_set_inflight(gt_context *c, uint32_t depth) {
+    if (!c) return fail(GT_ERR_INVALID_ARGUMENT, "null context");
+    if (depth < 1 || depth > 16) return fail(GT_ERR_INVALID_ARGUMENT, "depth must be 1..16");
+    if (c->ring_count) return fail(GT_ERR_INVALID_ARGUMENT, "frames are in flight");
+    if (depth != c->ring_depth) {
+        (void)hipStreamSynchronize(c->stream);
+        free_ring(c);
+        c->ring_depth = depth;
+    }
+    return GT_OK;
 }
 
 gt_status gt_context_set_lanes(gt_context *c, int32_t lanes) {
@@ -848,6 +1029,8 @@ gt_status execute_frames(gt_context *c, const FrameSrc &src, const uint64_t *key
                          const uint64_t *linker_reset_points, uint64_t nreset, gt_output *out) {
     const int32_t dtype = src.dtype;
     const uint64_t nframes = src.nframes;
+    if (c->ring_count)
+        return fail(GT_ERR_INVALID_ARGUMENT, "frames submitted with gt_submit_frame are in flight");
     c->interrupt.store(0);
     gt_status st = alloc_execute_buffers(c);
     if (st != GT_OK) return st;

@@ -175,6 +175,41 @@ gt_status gt_detect_device(gt_context *ctx, const void *d_frames, int32_t dtype,
                            uint32_t nframes, gt_detection *d_out, uint32_t cap_per_frame,
                            uint32_t *d_counts, void *stream);
 
+/* ---- One frame at a time (the reference's per-frame pair) ---------------
+ * gt_submit_frame mirrors submit_work (src/execute_gpu.rs:95-105; its body
+ * :106-422): the host frame (n_pixels = nrows * ncols samples of `dtype`) is
+ * copied into a pinned staging slot before the call returns, then its copy
+ * to the GPU, the pass sequence and the read-back of its count are queued on
+ * the context's stream; the call does not wait for them.  frame_idx is
+ * handed back by gt_fetch_frame.  Contexts created with characterize_points
+ * take the frame's positions instead (pos[0..npos): x = row, y = column,
+ * r > 0 a radius), as submit_work's characterization branch (:190-229) with
+ * FrameSubsetter's per-frame points (src/linking.rs:94-196); other contexts
+ * take pos = NULL, npos = 0.  At most gt_context_set_inflight frames (default
+ * 2, the reference's staging ring depth, src/gpu_setup.rs:173-183) are in
+ * flight: submitting one more fails until a frame is fetched.
+ * Errors: GT_ERR_DIMENSION_MISMATCH when n_pixels differs from the context's
+ * frame size (:1074-1080); GT_ERR_INVALID_ARGUMENT when the ring is full, or
+ * for whole-video illumination correction (its profile needs every frame
+ * first: gt_execute).
+ * gt_fetch_frame mirrors get_work (:424-467): it blocks until the oldest
+ * frame in flight is done and writes its detections (the reference's
+ * ResultRow before the host post-processing: no filter_close, doughnut or
+ * linking) to out in ascending order of the raster index of the local
+ * maximum each walk started from (position order with characterize_points),
+ * with *n = their number and *frame_idx = the frame's submit index.  When
+ * *n > cap it returns GT_ERR_CAPACITY and the frame stays in flight, so a
+ * second call with a larger buffer gets it; a frame whose candidates or
+ * detections overflowed the context's capacity is dropped with
+ * GT_ERR_CAPACITY.  gt_detect_device, gt_execute and gt_execute_file fail
+ * while frames are in flight. */
+gt_status gt_submit_frame(gt_context *ctx, const void *px, int32_t dtype, uint64_t n_pixels,
+                          uint64_t frame_idx, const gt_result_row *pos, uint32_t npos);
+gt_status gt_fetch_frame(gt_context *ctx, uint64_t *frame_idx, gt_result_row *out, uint32_t cap,
+                         uint32_t *n);
+/* Frames in flight at most (1..16, default 2); only with none in flight. */
+gt_status gt_context_set_inflight(gt_context *ctx, uint32_t depth);
+
 /* Lanes of gt_detect_device (1 or 2, default 1): with 2, the chunks of a call
  * alternate between two streams with their own scratch (plain Trackpy-style
  * detection without the noise estimate, adaptive passes, illumination
