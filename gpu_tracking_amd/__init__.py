@@ -10,7 +10,7 @@ The package needs the built library; importing the entry points raises when
 it is missing (no CPU fallback).
 """
 from .api import (Batch, CentroidNet, Context, LoG, VideoFile, batch,  # noqa: F401
-                  characterize_points, connect, detections_to_numpy, link, load, make_log_params,
+                  characterize_points, connect, detect_device_static, detections_to_numpy, link, load, make_log_params,
                   make_params, mean_from_file, parse_ets, parse_ets_with_keys, to_tiff,
                   unique_particle_ids)
 from ._native import (GtBaseError, GtConnectionError, GtError, GtFileNotFoundError,  # noqa: F401
@@ -21,4 +21,4 @@ __all__ = ["batch", "characterize_points", "LoG", "link", "connect", "Context", 
            "make_params", "unique_particle_ids",
            "make_log_params", "GtError", "GtValueError", "GtFileNotFoundError",
            "GtConnectionError", "GtInterrupted", "GtBaseError",
-           "column_names", "default_params", "detections_to_numpy"]
+           "column_names", "default_params", "detections_to_numpy", "detect_device_static"]

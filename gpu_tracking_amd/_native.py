@@ -89,7 +89,7 @@ DETECTION_FIELDS = [
 EXPORTS = [
     "gt_params_default", "gt_abi_version", "gt_last_error", "gt_context_create",
     "gt_context_destroy", "gt_detect_device", "gt_submit_frame", "gt_fetch_frame",
-    "gt_context_set_inflight", "gt_execute", "gt_output_free",
+    "gt_context_set_inflight", "gt_detect_device_static", "gt_execute", "gt_output_free",
     "gt_column_names", "gt_filter_close", "gt_doughnut", "gt_set_interrupt", "gt_progress",
     "gt_set_timing", "gt_stage_times", "gt_prune_blobs", "gt_link", "gt_connect", "gt_batch_create",
     "gt_batch_destroy", "gt_batch_detect", "gt_batch_tune", "gt_batch_get_schedule",
@@ -189,6 +189,10 @@ def lib() -> C.CDLL:
     L.gt_fetch_frame.restype = C.c_int
     L.gt_context_set_inflight.argtypes = [C.c_void_p, C.c_uint32]
     L.gt_context_set_inflight.restype = C.c_int
+    L.gt_detect_device_static.argtypes = [P(GtParams), C.c_uint32, C.c_uint32, C.c_void_p,
+                                          C.c_int32, C.c_uint32, C.c_void_p, C.c_uint32,
+                                          C.c_void_p, C.c_void_p]
+    L.gt_detect_device_static.restype = C.c_int
     L.gt_execute.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.c_uint64, C.c_void_p,
                              C.c_uint64, C.c_void_p, C.c_uint64, C.c_uint32, C.c_int32,
                              C.c_int32, C.c_void_p, C.c_uint64, P(GtOutput)]

@@ -491,6 +491,33 @@ class Context:
         return [buf[i] for i in range(n)], nl.value, fpl.value
 
 
+def detect_device_static(params: N.GtParams, frames, out=None, counts=None,
+                         cap_per_frame: Optional[int] = None, stream=None):
+    """gt_detect_device_static: Context.detect_device's plain path without a
+    context, its scratch in the library's static device array (no device
+    allocation inside the call; `out` and `counts` may be passed in
+    preallocated).  Calls that overlap on one GPU race."""
+    import torch
+    if not frames.is_cuda:
+        raise ValueError("detect_device_static needs a device tensor")
+    if frames.dim() == 2:
+        frames = frames.unsqueeze(0)
+    if not frames.is_contiguous():
+        raise ValueError("frames must be contiguous (T, Y, X)")
+    T, H, W = frames.shape
+    cap = cap_per_frame or max(1024, (H * W) // 16)
+    if out is None:
+        out = torch.empty((T, cap, 12), dtype=torch.int32, device=frames.device)
+    if counts is None:
+        counts = torch.empty((T,), dtype=torch.int32, device=frames.device)
+    if stream is None:
+        stream = torch.cuda.current_stream(frames.device).cuda_stream
+    N.check(N.lib().gt_detect_device_static(C.byref(params), H, W, frames.data_ptr(),
+                                            _torch_code(frames), T, out.data_ptr(), cap,
+                                            counts.data_ptr(), stream))
+    return out, counts
+
+
 def detections_to_numpy(out, counts) -> list:
     """Device detections -> list of per-frame structured arrays sorted by seed."""
     dt = np.dtype(N.DETECTION_FIELDS)

@@ -58,6 +58,28 @@ hipError_t gt_launch_characterize_det(const float *proc, const void *raw, int dt
                                       uint32_t nframes, const GtKParams &p, gt_detection *out,
                                       uint32_t out_cap, const uint32_t *ocount, hipStream_t s,
                                       uint32_t *dcount = nullptr);
+// The plain tiled Trackpy path of one chunk (no noise estimate, adaptive
+// passes, illumination correction or learned proposal), shared by
+// gt_detect_device and the static-scratch entry point.  Scratch: the chunk's
+// processed planes, per-frame row and slow-walk counters (GT_COUNTER_STRIDE
+// apart, cleared by the head's first launch) and the slow-walk lists.
+struct GtPlainScratch {
+    float *proc;
+    uint32_t *ocount, *scount;
+    uint2 *slow;
+    uint32_t slow_cap;
+};
+// Head: bandpass (clearing the counters), then local max + in-tile walks;
+// `mid` (may be null) is recorded on s between the two.
+hipError_t gt_plain_head(const void *frames, int dtype, uint32_t nf, const GtKParams &p,
+                         const GtPlainScratch &sc, gt_detection *out, uint32_t cap,
+                         uint32_t frame0, hipStream_t s, hipEvent_t mid);
+// Tail: the handed-over walks, then characterize (with p.characterize) or
+// the count copy; counts[f] receives frame f's detection count.
+hipError_t gt_plain_tail(const void *frames, int dtype, uint32_t nf, const GtKParams &p,
+                         const GtPlainScratch &sc, gt_detection *out, uint32_t cap,
+                         uint32_t *counts, uint32_t frame0, hipStream_t s);
+
 // LoG style: scale-space maxima of every scale of a chunk (planes kept in
 // LDS), then their walks.  Candidate seeds are scale * H * W + raster index.
 // With cval (threshold_rel on) the tile kernel also stores each candidate's

@@ -3185,3 +3185,31 @@ void gt_batch_tiles(int H, int W, uint32_t *bp_tiles, uint32_t *dt_tiles) {
     *bp_tiles = static_cast<uint32_t>(((W + B2_TW - 1) / B2_TW) * ((H + B2_TH - 1) / B2_TH));
     *dt_tiles = static_cast<uint32_t>(((W + FT_TW - 1) / FT_TW) * ((H + FT_TH - 1) / FT_TH));
 }
+
+hipError_t gt_plain_head(const void *frames, int dtype, uint32_t nf, const GtKParams &p,
+                         const GtPlainScratch &sc, gt_detection *out, uint32_t cap,
+                         uint32_t frame0, hipStream_t s, hipEvent_t mid) {
+    const uint32_t ncnt = nf * GT_COUNTER_STRIDE;
+    const GtZero zero = {{sc.ocount, sc.scount}, {ncnt, ncnt}};
+    hipError_t e = gt_launch_bandpass(frames, dtype, sc.proc, nf, p, s, &zero);
+    if (e != hipSuccess) return e;
+    if (mid && (e = hipEventRecord(mid, s)) != hipSuccess) return e;
+    GtKParams kw = p;
+    kw.characterize = 0;   // characterize runs in the tail over compacted rows
+    return gt_launch_detect_tile(sc.proc, frames, dtype, nf, kw, out, cap, sc.ocount, sc.slow,
+                                 sc.slow_cap, sc.scount, frame0, s);
+}
+
+hipError_t gt_plain_tail(const void *frames, int dtype, uint32_t nf, const GtKParams &p,
+                         const GtPlainScratch &sc, gt_detection *out, uint32_t cap,
+                         uint32_t *counts, uint32_t frame0, hipStream_t s) {
+    GtKParams kw = p;
+    kw.characterize = 0;
+    hipError_t e = gt_launch_walk_slow(sc.proc, frames, dtype, nf, kw, sc.slow, sc.slow_cap,
+                                       sc.scount, out, cap, sc.ocount, frame0, s);
+    if (e != hipSuccess) return e;
+    if (p.characterize)   // the counts are final here: characterize copies them too
+        return gt_launch_characterize_det(sc.proc, frames, dtype, nf, p, out, cap, sc.ocount, s,
+                                          counts);
+    return gt_launch_copy_counts(sc.ocount, counts, nf, s);
+}

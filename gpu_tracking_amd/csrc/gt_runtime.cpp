@@ -532,8 +532,6 @@ gt_status run_pipeline(gt_context *c, const void *d_frames, int32_t dtype, uint3
         // the plain tiled path's counters (rows, handed-over walks) are
         // cleared by the bandpass launch
         const bool plain_tiled = c->tiled && !c->cnn && !c->log;
-        const uint32_t ncnt = nf * GT_COUNTER_STRIDE;
-        const GtZero zero = {{ocount, scount}, {ncnt, ncnt}};
         if (!plain_tiled)
             GT_HIP(gt_launch_zero_u32(ocount, static_cast<size_t>(nf) * GT_COUNTER_STRIDE, cs));
         if (!c->tiled || c->log)
@@ -541,9 +539,44 @@ gt_status run_pipeline(gt_context *c, const void *d_frames, int32_t dtype, uint3
         if (ev) GT_HIP(hipEventRecord(ev[0], cs));
         if (c->der.need_std)
             GT_HIP(gt_launch_frame_std(frames, fdtype, nf, HW, c->d_std_tmp, c->d_mean, c->d_std, cs));
-        GT_HIP(gt_launch_bandpass(frames, fdtype, proc, nf, c->kp, cs, plain_tiled ? &zero : nullptr));
-        if (ev) GT_HIP(hipEventRecord(ev[1], cs));
         gt_detection *out = d_out + static_cast<size_t>(f0) * cap;
+        if (plain_tiled) {
+            // bandpass (clearing the chunk's counters), local max and walk in
+            // one kernel; then the walks it handed over and characterize over
+            // the frame's compacted detections (whole wavefronts of
+            // detections instead of a few lanes per tile)
+            const GtPlainScratch sc = {proc, ocount, scount, slow, c->slow_cap};
+            GT_HIP(gt_plain_head(frames, fdtype, nf, c->kp, sc, out, cap, f0, cs,
+                                 ev ? ev[1] : nullptr));
+            if (ev) GT_HIP(hipEventRecord(ev[2], cs));
+            hipStream_t ts = cs;   // the chunk's tail: slow walks, characterize, counts
+            if (ovl) {
+                GT_HIP(hipEventRecord(c->ovl_det[lane], cs));
+                ts = c->lane_stream[1];
+                GT_HIP(hipStreamWaitEvent(ts, c->ovl_det[lane], 0));
+            }
+            if (!c->der.adaptive) {
+                GT_HIP(gt_plain_tail(frames, fdtype, nf, c->kp, sc, out, cap, d_counts + f0, f0,
+                                     ts));
+            } else {
+                GtKParams kw = c->kp;
+                kw.characterize = 0;
+                GT_HIP(gt_launch_walk_slow(proc, frames, fdtype, nf, kw, slow, c->slow_cap,
+                                           scount, out, cap, ocount, f0, ts));
+                gt_status ast = run_adaptive(c, frames, fdtype, nf, out, cap, ts);
+                if (ast != GT_OK) return ast;
+                if (c->kp.characterize)
+                    GT_HIP(gt_launch_characterize_det(proc, frames, fdtype, nf, c->kp, out, cap,
+                                                      ocount, ts, d_counts + f0));
+                else
+                    GT_HIP(gt_launch_copy_counts(ocount, d_counts + f0, nf, ts));
+            }
+            if (ev) GT_HIP(hipEventRecord(ev[3], ts));
+            if (ovl) GT_HIP(hipEventRecord(c->lane_done[lane], ts));
+            continue;
+        }
+        GT_HIP(gt_launch_bandpass(frames, fdtype, proc, nf, c->kp, cs));
+        if (ev) GT_HIP(hipEventRecord(ev[1], cs));
         if (c->log) {
             // scale-space maxima of all scales, then their walks (and, with
             // adaptive_background, the acceptance passes before characterize)
@@ -607,37 +640,6 @@ gt_status run_pipeline(gt_context *c, const void *d_frames, int32_t dtype, uint3
             if (c->kp.characterize)
                 GT_HIP(gt_launch_characterize_det(proc, frames, fdtype, nf, c->kp, out, cap,
                                                   ocount, cs));
-        } else if (c->tiled) {
-            // local max and walk in one kernel, the walks it hands over, then
-            // characterize over the frame's compacted detections (whole
-            // wavefronts of detections instead of a few lanes per tile)
-            GtKParams kw = c->kp;
-            kw.characterize = 0;
-            GT_HIP(gt_launch_detect_tile(proc, frames, fdtype, nf, kw, out, cap,
-                                         ocount, slow, c->slow_cap, scount, f0, cs));
-            if (ev) GT_HIP(hipEventRecord(ev[2], cs));
-            hipStream_t ts = cs;   // the chunk's tail: slow walks, characterize, counts
-            if (ovl) {
-                GT_HIP(hipEventRecord(c->ovl_det[lane], cs));
-                ts = c->lane_stream[1];
-                GT_HIP(hipStreamWaitEvent(ts, c->ovl_det[lane], 0));
-            }
-            GT_HIP(gt_launch_walk_slow(proc, frames, fdtype, nf, kw, slow, c->slow_cap,
-                                       scount, out, cap, ocount, f0, ts));
-            if (c->der.adaptive) {
-                gt_status ast = run_adaptive(c, frames, fdtype, nf, out, cap, ts);
-                if (ast != GT_OK) return ast;
-            }
-            if (c->kp.characterize) {
-                // the counts are final here: characterize copies them too
-                GT_HIP(gt_launch_characterize_det(proc, frames, fdtype, nf, c->kp, out, cap,
-                                                  ocount, ts, d_counts + f0));
-            } else {
-                GT_HIP(gt_launch_copy_counts(ocount, d_counts + f0, nf, ts));
-            }
-            if (ev) GT_HIP(hipEventRecord(ev[3], ts));
-            if (ovl) GT_HIP(hipEventRecord(c->lane_done[lane], ts));
-            continue;
         } else {
             GT_HIP(gt_launch_localmax(proc, nf, c->kp, c->d_cand, c->cand_cap, c->d_ccount, cs));
             if (ev) GT_HIP(hipEventRecord(ev[2], cs));

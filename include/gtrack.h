@@ -175,6 +175,25 @@ gt_status gt_detect_device(gt_context *ctx, const void *d_frames, int32_t dtype,
                            uint32_t nframes, gt_detection *d_out, uint32_t cap_per_frame,
                            uint32_t *d_counts, void *stream);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): gt_detect_device's plain Trackpy-style path, without a
+ * context, with its scratch (processed planes, counters, slow-walk lists) in
+ * a static 64 MiB device array that each GPU holds until the process ends.
+ * It calls no allocating routine; the caller passes the parameters, the
+ * frame size, the frames, the outputs and the stream, laid out as for
+ * gt_detect_device, and rows and counts equal gt_detect_device's bit for bit.
+ * A frame needs 4 B/px of processed plane plus 1 B/px of slow-walk list (and
+ * 512 bytes of counters): frames above about 13.4 Mpx (GT_ERR_CAPACITY), the
+ * LoG style, snr / minmass_snr / adaptive_background and illumination
+ * correction (GT_ERR_INVALID_ARGUMENT) are rejected.  All calls on one GPU
+ * share its array: calls that overlap on one GPU (two streams, two host
+ * threads) race and must be serialised by the caller. */
+gt_status gt_detect_device_static(const gt_params *params, uint32_t nrows, uint32_t ncols,
+                                  const void *d_frames, int32_t dtype, uint32_t nframes,
+                                  gt_detection *d_out, uint32_t cap_per_frame, uint32_t *d_counts,
+                                  void *stream);
+
 /* ---- One frame at a time (the reference's per-frame pair) ---------------
  * gt_submit_frame mirrors submit_work (src/execute_gpu.rs:95-105; its body
  * :106-422): the host frame (n_pixels = nrows * ncols samples of `dtype`) is
