@@ -59,16 +59,32 @@ WORKLOADS = {
 }
 
 
+def _host_cpus():
+    """(threads the CPU baseline uses, nproc, CPU model).  A one-GPU box
+    allots 16 host cores to its GPU, whatever nproc shows, so the baseline
+    runs one thread per allotted core."""
+    try:
+        ncores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        ncores = os.cpu_count() or 1
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return max(1, min(16, ncores)), os.cpu_count() or ncores, model
+
+
 def _cpu_baseline(video_host, params, budget_s, gpu_rows):
     import numpy as np
 
     import oracle
     T, H, W = video_host.shape
-    try:
-        ncores = len(os.sched_getaffinity(0))
-    except AttributeError:
-        ncores = os.cpu_count() or 1
-    nthreads = max(1, min(16, ncores))
+    nthreads, nproc, model = _host_cpus()
     op = oracle.derive_trackpy(H, W, diameter=params["diameter"], separation=params["separation"],
                                minmass=params["minmass"])
     done = 0
@@ -92,9 +108,71 @@ def _cpu_baseline(video_host, params, budget_s, gpu_rows):
             mismatched += 1
     return {
         "value": round(done / dt, 3), "unit": "frames/s", "cores": nthreads, "kind": "port",
+        "nproc": nproc, "cpu_model": model,
         "sample": f"first {done} frames of the workload, detect+characterize, oracle C code "
-                  f"(-O2, one frame per thread) on {nthreads} host threads, {dt:.1f} s",
+                  f"(-O2, one frame per thread) on {nthreads} host threads (the host cores a "
+                  f"one-GPU box allots; nproc {nproc}), {dt:.1f} s",
     }, done, mismatched
+
+
+def _fp64_error(video_host, params, gpu_rows, nframes=8):
+    """Error of the f32 pipeline (the GPU rows, bit-identical to the oracle)
+    against an FP64 recomputation: the bandpass in float64, then at the walk's
+    final integer position (the 3 x 3 neighbour of the rounded centre whose
+    float64 offset stays within the shift threshold) the float64 centroid and
+    mass over the same disk.  Checker code of the CPU-baseline leg."""
+    import numpy as np
+
+    import oracle
+    T, H, W = video_host.shape
+    op = oracle.derive_trackpy(H, W, diameter=params["diameter"],
+                               separation=params["separation"], minmass=params["minmass"])
+    r = (params["diameter"] - 1) // 2
+    ii, jj = np.mgrid[-r:r + 1, -r:r + 1]
+    keep = (ii * ii + jj * jj) <= r * r
+    di, dj = ii[keep], jj[keep]
+    cerr, merr, unresolved = [], [], 0
+    for t in range(min(nframes, T)):
+        P64 = oracle.bandpass_f64(video_host[t], op)
+        rows = gpu_rows[t]
+        if len(rows) == 0:
+            continue
+        x = rows["x"].astype(np.float64)
+        y = rows["y"].astype(np.float64)
+        m32 = rows["mass"].astype(np.float64)
+        best = np.full(len(rows), np.inf)
+        bm = np.full(len(rows), np.nan)
+        for du in (-1, 0, 1):
+            for dv in (-1, 0, 1):
+                pu = np.rint(x).astype(np.int64) + du
+                pv = np.rint(y).astype(np.int64) + dv
+                U = pu[:, None] + di[None]
+                V = pv[:, None] + dj[None]
+                ok = (U >= 0) & (U < H) & (V >= 0) & (V < W)
+                d = np.where(ok, P64[np.clip(U, 0, H - 1), np.clip(V, 0, W - 1)], 0.0)
+                m = d.sum(1)
+                with np.errstate(invalid="ignore", divide="ignore"):
+                    ox = (d * di[None]).sum(1) / m
+                    oy = (d * dj[None]).sum(1) / m
+                    e = np.maximum(np.abs(pu + ox - x), np.abs(pv + oy - y))
+                    stay = (np.abs(ox) <= params.get("shift_threshold", 0.6)) & \
+                        (np.abs(oy) <= params.get("shift_threshold", 0.6)) & (m > 0)
+                better = stay & (e < best)
+                best = np.where(better, e, best)
+                bm = np.where(better, m, bm)
+        found = np.isfinite(best)
+        unresolved += int((~found).sum())
+        cerr.append(best[found])
+        merr.append(np.abs(m32[found] - bm[found]) / bm[found])
+    c = np.concatenate(cerr) if cerr else np.zeros(0)
+    m = np.concatenate(merr) if merr else np.zeros(0)
+    return {"frames": min(nframes, T), "rows": int(len(c)), "unresolved_rows": unresolved,
+            "centroid_max_px": float(c.max()) if len(c) else None,
+            "centroid_p99_9_px": float(np.quantile(c, 0.999)) if len(c) else None,
+            "mass_max_rel": float(m.max()) if len(m) else None,
+            "mass_p99_9_rel": float(np.quantile(m, 0.999)) if len(m) else None,
+            "reference": "float64 bandpass and centroid at the walk's final position "
+                         "(the f32 rows are bit-identical to the oracle's)"}
 
 
 def _log_taps_per_pixel(params):
@@ -130,11 +208,7 @@ def _cpu_baseline_log(video_host, params, budget_s, gpu_rows):
 
     import oracle
     T, H, W = video_host.shape
-    try:
-        ncores = len(os.sched_getaffinity(0))
-    except AttributeError:
-        ncores = os.cpu_count() or 1
-    nthreads = max(1, min(16, ncores))
+    nthreads, nproc, model = _host_cpus()
     op = oracle.derive_log(H, W, params["max_radius"])
     radii = oracle.log_radii(params["min_radius"], params["max_radius"], params["n_radii"],
                              params["log_spacing"])
@@ -166,6 +240,7 @@ def _cpu_baseline_log(video_host, params, budget_s, gpu_rows):
             mismatched += 1
     return {
         "value": round(done / dt, 3), "unit": "frames/s", "cores": nthreads, "kind": "port",
+        "nproc": nproc, "cpu_model": model,
         "sample": f"first {done} frames of the workload, LoG detect+characterize (no pruning), "
                   f"oracle C code (-O2, one frame per thread) on {nthreads} host threads, "
                   f"{dt:.1f} s",
@@ -467,6 +542,158 @@ def run_learned_workload(args):
     net.close()
 
 
+def _copy_peak(torch):
+    """Device copy bandwidth (GB/s, read + write bytes) over 1 GiB: the
+    faster of torch's copy and elementwise-multiply kernels, best of three
+    each, timed with events on the current stream."""
+    src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    best = float("inf")
+    for op in (lambda: dst.copy_(src), lambda: torch.mul(src, 2.0, out=dst)):
+        op()
+        for _ in range(3):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            op()
+            e1.record()
+            torch.cuda.synchronize()
+            best = min(best, e0.elapsed_time(e1) * 1e-3)
+    del src, dst
+    return 2.0 * (1 << 30) / best / 1e9
+
+
+def _learned_summary(frames, params, nframes, steps):
+    """The learned centroid proposal on the main workload's frames: a
+    network trained on the GPU (2M pairs from frames of the same generator
+    with disjoint seeds, Adam lr 1e-3, batch 4096, 4 epochs), then plain and
+    learned detection of the same frames (speed-up, accuracy against the
+    plain walk) and the network kernel's matrix-core roofline."""
+    import torch
+
+    import gpu_tracking_amd as gt
+    from gpu_tracking_amd import synth
+    s = synth.spec("full")
+    H, W = frames.shape[1], frames.shape[2]
+    p = gt.make_params(params["diameter"], **{k: v for k, v in params.items() if k != "diameter"})
+    ctx = gt.Context(p, H, W)
+    want, got, seed, nfr = 2_000_000, 0, 20261203 + 9000, 0
+    pch, tch = [], []
+    t0 = time.perf_counter()
+    while got < want:
+        v = synth.video_torch(s, 32, seed=seed, device="cuda")
+        P, T, _ = ctx.training_samples(v)
+        pch.append(P.clone())
+        tch.append(T.clone())
+        got += P.shape[0]
+        nfr += 32
+        seed += 1
+        del v, P, T
+    P = torch.cat(pch)[:want].contiguous()
+    T = torch.cat(tch)[:want].contiguous()
+    del pch, tch
+    net = gt.CentroidNet(seed=7)
+    losses = net.train(P, T, epochs=4, batch=4096, lr=1e-3, seed=11)
+    t_train = time.perf_counter() - t0
+    nfw = min(P.shape[0], 1 << 20)
+    net.forward(P[:nfw])
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(3):
+        net.forward(P[:nfw])
+    e1.record()
+    torch.cuda.synchronize()
+    fw_s = e0.elapsed_time(e1) * 1e-3 / 3
+    del P, T
+    fr = frames[:nframes]
+    cap = 8192
+    out_p = torch.empty((nframes, cap, 12), dtype=torch.int32, device="cuda")
+    cnt_p = torch.zeros((nframes,), dtype=torch.int32, device="cuda")
+    out_l, cnt_l = torch.empty_like(out_p), torch.zeros_like(cnt_p)
+    stream = torch.cuda.current_stream().cuda_stream
+
+    def timed(out, cnt):
+        ctx.detect_device(fr, out=out, counts=cnt, cap_per_frame=cap, stream=stream)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            ctx.detect_device(fr, out=out, counts=cnt, cap_per_frame=cap, stream=stream)
+        torch.cuda.synchronize()
+        return nframes * steps / (time.perf_counter() - t0)
+
+    fps_plain = timed(out_p, cnt_p)
+    ctx.set_learned(net, scope="moving")
+    fps_l = timed(out_l, cnt_l)
+    acc, cands = ctx.learned_stats(reset=True)
+    ctx.set_learned(None)
+    nchk = min(nframes, 256)
+    ok, n, extra = _match_rows(gt.detections_to_numpy(out_p[:nchk], cnt_p[:nchk]),
+                               gt.detections_to_numpy(out_l[:nchk], cnt_l[:nchk]))
+    ctx.close()
+    net.close()
+    tflops = nfw * CNN_MFMA_FLOP / fw_s / 1e12
+    return {"frames": nframes, "scope": "moving", "plain_frames_per_s": round(fps_plain, 1),
+            "learned_frames_per_s": round(fps_l, 1),
+            "speedup_vs_plain": round(fps_l / fps_plain, 4),
+            "accuracy": {"frames": nchk, "particles": n, "within_1e-3_px_of_full_walk": ok,
+                         "fraction": round(ok / max(n, 1), 6), "extra_rows": extra},
+            "proposals_accepted_fraction": round(acc / max(cands, 1), 4),
+            "training": {"pairs": want, "frames": nfr, "epochs": 4, "batch": 4096, "lr": 1e-3,
+                         "seconds_with_sampling": round(t_train, 2),
+                         "final_loss": round(float(losses[-1]), 5)},
+            "roofline": {"bound": "mfma", "kernel": "k_cnn_forward",
+                         "achieved": round(tflops, 2), "peak": MFMA_BF16_PEAK_TFLOPS,
+                         "unit": "TFLOP/s", "frac": round(tflops / MFMA_BF16_PEAK_TFLOPS, 5)}}
+
+
+def _log_summary(nframes, steps):
+    """The LoG workload at its shape (2048 x 2048 f32 frames of its
+    generator, radii 1.5..6 in 10 log-spaced scales, characterize on), with
+    threshold_rel off and at 0.05: frames/s and the scale-space kernel's FP32
+    vector roofline."""
+    import torch
+
+    import gpu_tracking_amd as gt
+    from gpu_tracking_amd import synth
+    s = synth.spec("log")
+    H, W = s.nrows, s.ncols
+    frames = synth.video_torch(s, nframes, seed=20261203 + 3000, device="cuda")
+    stream = torch.cuda.current_stream().cuda_stream
+    res = {"frames": nframes, "shape": [H, W], "dtype": "f32"}
+    taps = _log_taps_per_pixel(LOG_PARAMS)
+    for rel in (0.0, 0.05):
+        kw = {k: v for k, v in LOG_PARAMS.items() if k not in ("min_radius", "max_radius")}
+        p = gt.make_log_params(LOG_PARAMS["min_radius"], LOG_PARAMS["max_radius"],
+                               threshold_rel=rel, **kw)
+        cap = (H * W) // 8 if rel == 0.0 else 16384
+        with gt.Context(p, H, W) as ctx:
+            out = torch.empty((nframes, cap, 12), dtype=torch.int32, device="cuda")
+            cnt = torch.zeros((nframes,), dtype=torch.int32, device="cuda")
+            ctx.detect_device(frames, out=out, counts=cnt, cap_per_frame=cap, stream=stream)
+            torch.cuda.synchronize()
+            ctx.set_timing(True)
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                ctx.detect_device(frames, out=out, counts=cnt, cap_per_frame=cap, stream=stream)
+            torch.cuda.synchronize()
+            dt = time.perf_counter() - t0
+            st, nl, _ = ctx.stage_times()
+            ctx.set_timing(False)
+        fps = nframes * steps / dt
+        flop = 8.0 * taps * H * W * nframes
+        tk = st[1] * 1e-3 if st else float("nan")
+        res[f"threshold_rel_{rel}"] = {
+            "frames_per_s": round(fps, 1),
+            "mean_rows_per_frame": round(float(cnt.float().mean().item()), 1),
+            "stage_ms_per_step": [round(x, 3) for x in st],
+            "pipeline_frac": round((4 + 8 + 8 * LOG_PARAMS["n_radii"]) * H * W * fps
+                                   / (HBM_PEAK_GBS * 1e9), 4),
+            "log_tile_valu": {"achieved_tflops": round(flop / tk / 1e12, 2),
+                              "peak_tflops": 78.6, "frac": round(flop / tk / 1e12 / 78.6, 4)}}
+        del out, cnt
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -490,6 +717,10 @@ def main():
                     help="skip timing the steps with the chunks alternating between two "
                          "streams, and with each chunk's tail on a second stream (done after "
                          "the timed region by default; reported as two_lanes and overlap)")
+    ap.add_argument("--no-learned", action="store_true",
+                    help="full workload: skip the learned-proposal summary")
+    ap.add_argument("--no-log", action="store_true",
+                    help="full workload: skip the LoG-workload summary")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a 1 GiB device-to-device copy (a known-size kernel that "
                          "calibrates rocprof's FETCH_SIZE/WRITE_SIZE counters)")
@@ -587,21 +818,44 @@ def main():
         pipe_bpp = 10
     k = int(np.argmax(stage_ms)) if stage_ms else 0
     kname = names[k]
+    # per launch: the step's T frames over nlaunch launches (the last one
+    # may hold fewer than `chunk`), so bytes and time are both averages
+    fpl = T / nlaunch if stage_ms else float(chunk)
     launch_ms = stage_ms[k] / nlaunch if stage_ms else float("nan")
-    bytes_launch = alg_bpp[kname] * H * W * chunk
+    bytes_launch = alg_bpp[kname] * H * W * fpl
     achieved = bytes_launch / (launch_ms * 1e-3) / 1e9 if stage_ms and alg_bpp[kname] else 0.0
     traffic = None
     tf = os.path.join(ROOT, "profiles", f"traffic_{kname}_{args.workload}.json")
     if os.path.exists(tf):
         with open(tf) as f:
             tj = json.load(f)
-        traffic = round(tj["bytes_per_frame"] * chunk)
+        traffic = round(tj["bytes_per_frame"] * fpl)
+    copy_gbs = _copy_peak(torch)
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-            "kernel": kname, "launch_ms": round(launch_ms, 4), "frames_per_launch": chunk,
+            "kernel": kname, "launch_ms": round(launch_ms, 4),
+            "frames_per_launch": round(fpl, 2), "launches_per_step": nlaunch,
+            "alg_bytes_per_frame": alg_bpp[kname] * H * W,
             "stage_ms_per_step": [round(x, 3) for x in stage_ms],
+            "copy_peak_gbs": round(copy_gbs, 1),
+            "frac_of_copy_peak": round(achieved / copy_gbs, 4),
             "pipeline_alg_bytes_per_frame": pipe_bpp * H * W,
-            "pipeline_frac": round(pipe_bpp * H * W * fps / (HBM_PEAK_GBS * 1e9), 4)}
+            "pipeline_frac": round(pipe_bpp * H * W * fps / (HBM_PEAK_GBS * 1e9), 4),
+            "pipeline_frac_of_copy_peak": round(pipe_bpp * H * W * fps / (copy_gbs * 1e9), 4)}
+    # the same kernel's mean dispatch time in the committed rocprof summary of
+    # this command (profiles/), for comparison with the event-timed figure
+    kf = os.path.join(ROOT, "profiles", f"kernel_times_{args.workload}.json")
+    if os.path.exists(kf) and stage_ms:
+        with open(kf) as f:
+            kj = json.load(f)
+        if kname in kj.get("kernels", {}):
+            kk = kj["kernels"][kname]
+            r_ach = alg_bpp[kname] * H * W * kk["frames_per_dispatch"] / (kk["avg_us"] * 1e-6) / 1e9
+            roof["rocprof"] = {"kernel": kk["name"], "avg_us": kk["avg_us"],
+                               "frames_per_dispatch": kk["frames_per_dispatch"],
+                               "achieved": round(r_ach, 1),
+                               "frac": round(r_ach / HBM_PEAK_GBS, 4),
+                               "source": os.path.relpath(kf, ROOT)}
     if is_log and stage_ms:
         # the scale-space kernel is FP32-FMA bound: 4 separable passes of
         # 2R+1 taps per pixel per scale (8 (2R+1) flop), against the vector
@@ -681,8 +935,20 @@ def main():
         print(f"[bench] end to end from a TIFF file {e2e['from_tiff']['value']} frames/s "
               f"(identical: {same})", file=sys.stderr, flush=True)
 
+    learned = log_sum = None
+    if args.workload == "full" and not args.no_learned:
+        print("[bench] learned-proposal summary", file=sys.stderr, flush=True)
+        learned = _learned_summary(frames, params, min(T, 512), 3)
+        print(f"[bench] learned {learned['speedup_vs_plain']}x plain, "
+              f"{learned['accuracy']['fraction']} within 1e-3 px", file=sys.stderr, flush=True)
+    if args.workload == "full" and not args.no_log:
+        print("[bench] LoG-workload summary", file=sys.stderr, flush=True)
+        log_sum = _log_summary(64, 3)
+        print(f"[bench] LoG {log_sum}", file=sys.stderr, flush=True)
+
     cpu = None
     parity = None
+    fp64 = None
     if not args.no_cpu:
         if is_log:
             rows = gt.detections_to_numpy(out[:64], counts[:64])
@@ -697,6 +963,7 @@ def main():
             host = frames[:nh].cpu().numpy().view(np.uint16)
             print("[bench] CPU baseline", file=sys.stderr, flush=True)
             cpu, nchk, bad = _cpu_baseline(host, params, args.cpu_seconds, rows)
+            fp64 = _fp64_error(host, params, rows)
         parity = {"frames_checked": nchk, "frames_bit_identical": nchk - bad}
 
     res = {
@@ -714,6 +981,7 @@ def main():
         "roofline": roof, "cpu_baseline": cpu, "parity": parity, "end_to_end": e2e,
         "two_lanes": lanes2, "overlap": overlap,
         "mean_detections_per_frame": round(float(counts.float().mean().item()), 1),
+        "fp64_error": fp64, "learned": learned, "log": log_sum,
     }
     print(json.dumps(res), flush=True)
 
