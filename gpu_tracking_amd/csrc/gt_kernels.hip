@@ -2507,48 +2507,59 @@ hipError_t gt_launch_characterize_det(const float *proc, const void *raw, int dt
 // H * W + raster index) go to a per-frame list; k_walk_log walks them.
 namespace {
 
-constexpr int LG_TW = 64, LG_TH = 14;
+constexpr int LG_TW = 64, LG_TH = 16;
 constexpr int LG_RR = LG_TH + 2, LG_RC = LG_TW + 2;  // plane region: tile + 1-pixel ring
-constexpr int LG_MV = 8;                             // rows per lane in the vertical pass
-constexpr int LG_MH = 6;                             // columns per lane in the horizontal pass
+constexpr int LG_MV = 6;                             // rows per item of the vertical pass
+constexpr int LG_MH = 3;                             // columns per item of the horizontal pass
 constexpr int LG_THREADS = 256;
-static_assert(LG_RR % LG_MV == 0, "vertical chunks must tile the region rows");
-static_assert(LG_RC % LG_MH == 0, "horizontal chunks must tile the region columns");
+constexpr int LG_WAVES = LG_THREADS / 64;
+constexpr int LG_TR = LG_TH / LG_WAVES;              // tile rows per lane in the maximum test
+constexpr int LG_RU = 24;                            // largest tap radius with unrolled passes
+// Four workgroups of LG_THREADS fit a CU's LDS at Rm = 17; their waves need
+// at most 128 VGPRs each.
+constexpr int LG_MIN_WAVES = 4;
+static_assert(LG_RR % LG_MV == 0, "vertical items must tile the region rows");
+static_assert(LG_RC % LG_MH == 0, "horizontal items must tile the region columns");
+static_assert(LG_TW == 64 && LG_TH % LG_WAVES == 0, "one tile column per lane in the test");
 
 // LDS layout for a largest tap radius Rm:
-//   raw   RRows x RP  (region rows + 2 Rm, plus LG_MV rows the sliding window
-//                      may load past the last tap; those are never used)
-//   Vd,Vg LG_RR x VP  (vertical-pass outputs, both tap sets; VP odd)
-//   L     3 x LG_RR x LG_RC (ring of the last three scales' planes)
+//   raw   NC columns of CP samples, column-major (NC = LG_RC + 2 Rm, CP =
+//         (LG_RR + 2 Rm) | 1): the vertical pass reads down a column at
+//         compile-time offsets, and lanes on neighbouring columns fall in
+//         different banks.  The last sample of a column is padding.
+//   Vd,Vg LG_RR rows of VP = NC | 1 (vertical-pass outputs, both tap sets)
+//   L     LG_RR x LG_RC (the current scale's plane)
 struct LogGeom {
-    int Rm, RP, RRows, VP;
+    int Rm, NC, CP, VP;
 };
 inline LogGeom log_geom(int Rm) {
     LogGeom g;
     g.Rm = Rm;
-    g.RP = LG_RC + 2 * Rm;
-    g.RRows = LG_RR + 2 * Rm + LG_MV;
-    g.VP = (LG_RC + 2 * Rm + LG_MH) | 1;
+    g.NC = LG_RC + 2 * Rm;
+    g.CP = (LG_RR + 2 * Rm) | 1;
+    g.VP = g.NC | 1;
     return g;
 }
 inline size_t log_lds_bytes(const LogGeom &g) {
-    return sizeof(float) * (static_cast<size_t>(g.RRows) * g.RP + 2u * LG_RR * g.VP +
-                            3u * LG_RR * LG_RC);
+    return sizeof(float) * (static_cast<size_t>(g.NC) * g.CP + 2u * LG_RR * g.VP +
+                            static_cast<size_t>(LG_RR) * LG_RC);
 }
 
-// Vertical 1-D pass of one scale for LG_MV consecutive outputs of one column:
-// ad = sum_i td[i] * x[i + m], ag = sum_i tg[i] * x[i + m], taps in order
-// i = 0..ntaps-1 (the reference's -R..R).  The inputs slide through a
+// Vertical 1-D pass of one scale for LG_MV consecutive outputs of one column
+// (samples colp[0], colp[1], ... down the column):
+// ad[m] = sum_i td[i] * x[i + m], ag[m] = sum_i tg[i] * x[i + m], taps in
+// order i = 0..ntaps-1 (the reference's -R..R).  The inputs slide through a
 // register window; inside a block of LG_MV taps the window index rotates at
-// compile time, so no register moves are needed there.
-__device__ __forceinline__ void log_vpass(const float *colp, int RP, int ntaps,
+// compile time, so no register moves are needed there.  Reads one sample
+// past the last one used.
+__device__ __forceinline__ void log_vpass(const float *colp, int ntaps,
                                           const float *__restrict__ tg,
                                           const float *__restrict__ td, float (&ag)[LG_MV],
                                           float (&ad)[LG_MV]) {
     float w[LG_MV];
 #pragma unroll
     for (int m = 0; m < LG_MV; ++m) {
-        w[m] = colp[m * RP];
+        w[m] = colp[m];
         ag[m] = 0.0f;
         ad[m] = 0.0f;
     }
@@ -2563,7 +2574,7 @@ __device__ __forceinline__ void log_vpass(const float *colp, int RP, int ntaps,
                 ad[m] = fmaf(d, v, ad[m]);
                 ag[m] = fmaf(g, v, ag[m]);
             }
-            w[k] = colp[(i + k + LG_MV) * RP];
+            w[k] = colp[i + k + LG_MV];
         }
     }
     for (; i < ntaps; ++i) {
@@ -2575,12 +2586,39 @@ __device__ __forceinline__ void log_vpass(const float *colp, int RP, int ntaps,
         }
 #pragma unroll
         for (int m = 0; m + 1 < LG_MV; ++m) w[m] = w[m + 1];
-        w[LG_MV - 1] = colp[(i + LG_MV) * RP];
+        w[LG_MV - 1] = colp[i + LG_MV];
+    }
+}
+
+// The same sums for a compile-time tap radius R: the whole window is loaded
+// first (one wait), then every tap's multiply-adds run back to back.
+template <int R>
+__device__ __forceinline__ void log_vpass_u(const float *colp, const float *__restrict__ tg,
+                                            const float *__restrict__ td, float (&ag)[LG_MV],
+                                            float (&ad)[LG_MV]) {
+    constexpr int NW = LG_MV + 2 * R;
+    float w[NW];
+#pragma unroll
+    for (int i = 0; i < NW; ++i) w[i] = colp[i];
+#pragma unroll
+    for (int m = 0; m < LG_MV; ++m) {
+        ag[m] = 0.0f;
+        ad[m] = 0.0f;
+    }
+#pragma unroll
+    for (int i = 0; i < 2 * R + 1; ++i) {
+        const float g = tg[i], d = td[i];
+#pragma unroll
+        for (int m = 0; m < LG_MV; ++m) {
+            ad[m] = fmaf(d, w[i + m], ad[m]);
+            ag[m] = fmaf(g, w[i + m], ag[m]);
+        }
     }
 }
 
 // Horizontal 1-D pass for LG_MH consecutive outputs of one row:
-// a0 = sum_j tg[j] * vd[j + m], a1 = sum_j td[j] * vg[j + m].
+// a0 = sum_j tg[j] * vd[j + m], a1 = sum_j td[j] * vg[j + m].  Reads one
+// value past the last one used.
 __device__ __forceinline__ void log_hpass(const float *vd, const float *vg, int ntaps,
                                           const float *__restrict__ tg,
                                           const float *__restrict__ td, float (&a0)[LG_MH],
@@ -2625,21 +2663,150 @@ __device__ __forceinline__ void log_hpass(const float *vd, const float *vg, int 
     }
 }
 
-// is_max_in_plane (logspace_max.wgsl:62-92) for a pixel whose 8 neighbours
-// are in the frame: no neighbour >= the centre.
-__device__ __forceinline__ bool log_plane_max(const float *L, int r, int c, float ctr) {
+// The same sums for a compile-time tap radius R, one tap set at a time with
+// its whole window in registers.
+template <int R>
+__device__ __forceinline__ void log_hpass_u(const float *vd, const float *vg,
+                                            const float *__restrict__ tg,
+                                            const float *__restrict__ td, float (&a0)[LG_MH],
+                                            float (&a1)[LG_MH]) {
+    constexpr int NW = LG_MH + 2 * R;
+    {
+        float w[NW];
 #pragma unroll
-    for (int a = -1; a <= 1; ++a)
+        for (int i = 0; i < NW; ++i) w[i] = vd[i];
 #pragma unroll
-        for (int b = -1; b <= 1; ++b) {
-            if (a == 0 && b == 0) continue;
-            if (L[(r + a) * LG_RC + (c + b)] >= ctr) return false;
+        for (int m = 0; m < LG_MH; ++m) a0[m] = 0.0f;
+#pragma unroll
+        for (int j = 0; j < 2 * R + 1; ++j) {
+            const float g = tg[j];
+#pragma unroll
+            for (int m = 0; m < LG_MH; ++m) a0[m] = fmaf(g, w[j + m], a0[m]);
         }
-    return true;
+    }
+    {
+        float w[NW];
+#pragma unroll
+        for (int i = 0; i < NW; ++i) w[i] = vg[i];
+#pragma unroll
+        for (int m = 0; m < LG_MH; ++m) a1[m] = 0.0f;
+#pragma unroll
+        for (int j = 0; j < 2 * R + 1; ++j) {
+            const float d = td[j];
+#pragma unroll
+            for (int m = 0; m < LG_MH; ++m) a1[m] = fmaf(d, w[j + m], a1[m]);
+        }
+    }
+}
+
+// Both passes of one scale (tap radius R; RT = R at compile time, or 0):
+// vertical pass into Vd/Vg, barrier, horizontal pass into the plane Lk with
+// L = (0 - R0) - R1 as the reference subtracts both passes from a zeroed
+// plane.  Items go to the waves in blocks of 64, the first block to wave
+// `rot`, so that the blocks past a multiple of the wave count land on
+// different waves in different workgroups.
+template <int RT>
+__device__ __forceinline__ void log_scale_passes(const float *Sraw, float *Vd, float *Vg,
+                                                 float *Lk, const LogGeom &g, int R,
+                                                 const float *__restrict__ tg,
+                                                 const float *__restrict__ td, float n1, float n3,
+                                                 int wave, int lane_in, int rot) {
+    // the item addresses are the same for every scale; the empty asm keeps
+    // the compiler from computing them once per tap radius ahead of the
+    // scale loop (one live register set per instantiation, spilled)
+    int lane = lane_in;
+    asm volatile("" : "+v"(lane));
+    const int Rr = RT > 0 ? RT : R;
+    const int ncv = LG_RC + 2 * Rr;  // vertical-pass columns: region columns -R .. LG_RC + R - 1
+    const int nv = (LG_RR / LG_MV) * ncv;
+    const int b0 = (wave - rot + LG_WAVES) % LG_WAVES;
+    for (int b = b0; b * 64 < nv; b += LG_WAVES) {
+        const int item = b * 64 + lane;
+        if (item < nv) {
+            const int q = item / ncv, vc = item - q * ncv, r0 = q * LG_MV;
+            const float *colp = Sraw + (vc + g.Rm - Rr) * g.CP + (r0 + g.Rm - Rr);
+            float ag[LG_MV], ad[LG_MV];
+            if constexpr (RT > 0) log_vpass_u<RT>(colp, tg, td, ag, ad);
+            else log_vpass(colp, 2 * Rr + 1, tg, td, ag, ad);
+#pragma unroll
+            for (int m = 0; m < LG_MV; ++m) {
+                Vd[(r0 + m) * g.VP + vc] = ad[m] * n3;
+                Vg[(r0 + m) * g.VP + vc] = ag[m] * n1;
+            }
+        }
+    }
+    __syncthreads();
+    constexpr int NH = LG_RR * (LG_RC / LG_MH);
+    for (int b = b0; b * 64 < NH; b += LG_WAVES) {
+        const int item = b * 64 + lane;
+        if (item < NH) {
+            const int r = item % LG_RR, c0 = (item / LG_RR) * LG_MH;
+            float a0[LG_MH], a1[LG_MH];
+            if constexpr (RT > 0)
+                log_hpass_u<RT>(Vd + r * g.VP + c0, Vg + r * g.VP + c0, tg, td, a0, a1);
+            else
+                log_hpass(Vd + r * g.VP + c0, Vg + r * g.VP + c0, 2 * Rr + 1, tg, td, a0, a1);
+#pragma unroll
+            for (int m = 0; m < LG_MH; ++m) {
+                float l = 0.0f - a0[m] * n1;
+                l = l - a1[m] * n3;
+                Lk[r * LG_RC + c0 + m] = l;
+            }
+        }
+    }
+}
+
+// Appends lane flags f[0..N-1] to a list guarded by *counter with one
+// atomicAdd per wavefront; pos[s] is flag s's slot (set where f[s]).  Must be
+// reached by every lane of the wave.
+template <int N>
+__device__ __forceinline__ void wave_append_n(const bool (&f)[N], uint32_t *counter,
+                                              uint32_t (&pos)[N]) {
+    uint64_t mk[N];
+    uint32_t tot = 0;
+#pragma unroll
+    for (int s = 0; s < N; ++s) {
+        mk[s] = __ballot(f[s]);
+        tot += static_cast<uint32_t>(__popcll(mk[s]));
+    }
+    if (tot == 0) return;
+    const uint32_t lane = __lane_id();
+    const uint64_t lt = (1ull << lane) - 1ull;
+    uint32_t base = 0u;
+    if (lane == 0) base = atomicAdd(counter, tot);
+    base = __shfl(base, 0);
+#pragma unroll
+    for (int s = 0; s < N; ++s) {
+        pos[s] = base + static_cast<uint32_t>(__popcll(mk[s] & lt));
+        base += static_cast<uint32_t>(__popcll(mk[s]));
+    }
+}
+
+// Centre value c[s] and largest of the 8 neighbours x[s] in plane Lk of the
+// lane's LG_TR tile pixels (column `lane`, tile rows tr0 .. tr0 + LG_TR - 1).
+// fmaxf skips NaN, so x is the largest non-NaN neighbour, NaN when all 8
+// are: "no neighbour >= c" (is_max_in_plane, logspace_max.wgsl:62-92) is
+// then !(x >= c) in every case.
+__device__ __forceinline__ void log_plane_stats(const float *Lk, int tr0, int lane,
+                                                float (&c)[LG_TR], float (&x)[LG_TR]) {
+    const float *b = Lk + tr0 * LG_RC + lane;  // region (tr0, lane): the first pixel's top-left
+    float v[LG_TR + 2][3];
+#pragma unroll
+    for (int i = 0; i < LG_TR + 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) v[i][j] = b[i * LG_RC + j];
+    float h[LG_TR + 2];
+#pragma unroll
+    for (int i = 0; i < LG_TR + 2; ++i) h[i] = fmaxf(fmaxf(v[i][0], v[i][1]), v[i][2]);
+#pragma unroll
+    for (int s = 0; s < LG_TR; ++s) {
+        c[s] = v[s + 1][1];
+        x[s] = fmaxf(fmaxf(h[s], h[s + 2]), fmaxf(v[s + 1][0], v[s + 1][2]));
+    }
 }
 
 template <typename T>
-__global__ __launch_bounds__(LG_THREADS) void k_log_tile(const T *__restrict__ raw,
+__global__ __launch_bounds__(LG_THREADS, LG_MIN_WAVES) void k_log_tile(const T *__restrict__ raw,
                                                          const float *__restrict__ proc,
                                                          const GtLogTable *__restrict__ tab,
                                                          LogGeom g, int H, int W,
@@ -2660,101 +2827,121 @@ __global__ __launch_bounds__(LG_THREADS) void k_log_tile(const T *__restrict__ r
     const size_t HW = static_cast<size_t>(H) * W;
     const T *f = raw + static_cast<size_t>(fr) * HW;
     const float *P = proc + static_cast<size_t>(fr) * HW;
-    const int Rm = g.Rm, RP = g.RP, VP = g.VP;
+    const int Rm = g.Rm;
     float *Sraw = lds;
-    float *Vd = Sraw + g.RRows * RP;
-    float *Vg = Vd + LG_RR * VP;
-    float *Lr = Vg + LG_RR * VP;
-    const int tid = threadIdx.x;
-    // raw tile: region origin (y0 - 1, x0 - 1) less the largest tap radius;
-    // outside the frame 0 (a zero sample adds nothing to a tap sum, as the
-    // reference's skipped out-of-frame taps)
-    const int oy = y0 - 1 - Rm, ox = x0 - 1 - Rm;
-    const int nload = (LG_RR + 2 * Rm) * RP;
-    for (int idx = tid; idx < nload; idx += LG_THREADS) {
-        const int ry = idx / RP, rx = idx - ry * RP;
-        const int gy = oy + ry, gx = ox + rx;
-        float v = 0.0f;
-        if (gy >= 0 && gy < H && gx >= 0 && gx < W) v = to_f32(f[static_cast<size_t>(gy) * W + gx]);
-        Sraw[idx] = v;
+    float *Vd = Sraw + g.NC * g.CP;
+    float *Vg = Vd + LG_RR * g.VP;
+    float *Lr = Vg + LG_RR * g.VP;
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    // raw tile, column-major: region origin (y0 - 1, x0 - 1) less the largest
+    // tap radius; outside the frame 0 (a zero sample adds nothing to a tap
+    // sum, as the reference's skipped out-of-frame taps).  Four rows per wave
+    // per pass, so that each lane has several loads in flight.
+    {
+        const int oy = y0 - 1 - Rm, ox = x0 - 1 - Rm;
+        const int nrows = LG_RR + 2 * Rm;
+        constexpr int U = 4;
+        for (int ry0 = wave * U; ry0 < nrows; ry0 += LG_WAVES * U) {
+            for (int rx = lane; rx < g.NC; rx += 64) {
+                const int gx = ox + rx;
+                const bool cok = gx >= 0 && gx < W;
+                float v[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const int gy = oy + ry0 + u;
+                    v[u] = 0.0f;
+                    if (cok && ry0 + u < nrows && gy >= 0 && gy < H)
+                        v[u] = to_f32(f[static_cast<size_t>(gy) * W + gx]);
+                }
+#pragma unroll
+                for (int u = 0; u < U; ++u)
+                    if (ry0 + u < nrows) Sraw[rx * g.CP + ry0 + u] = v[u];
+            }
+        }
     }
     __syncthreads();
     const int n = tab->n;
     const float snr_thr = fstd ? fstd[fr] * snr * rough : tab->snr_thr;
     uint32_t *cnt = &ccount[fr * GT_COUNTER_STRIDE];
     uint32_t *fc = cand + static_cast<size_t>(fr) * cap;
-    // scale-space maximum test of scale m on the tile's interior pixels
-    auto test = [&](int m) {
-        const float *Lm = Lr + (m % 3) * (LG_RR * LG_RC);
-        const float *Lb = m >= 1 ? Lr + ((m + 2) % 3) * (LG_RR * LG_RC) : nullptr;
-        const float *Lt = m + 1 < n ? Lr + ((m + 1) % 3) * (LG_RR * LG_RC) : nullptr;
-        // LG_TH * LG_TW is a multiple of 128, so each wave runs whole iterations
-        for (int pi = tid; pi < LG_TH * LG_TW; pi += LG_THREADS) {
-            const int r = 1 + pi / LG_TW, c = 1 + (pi % LG_TW);
-            const int gy = y0 + r - 1, gx = x0 + c - 1;
-            bool flag = gy >= 1 && gy <= H - 2 && gx >= 1 && gx <= W - 2;
-            if (flag) {
-                const float ctr = Lm[r * LG_RC + c];
-                flag = !(ctr <= 0.0f) && log_plane_max(Lm, r, c, ctr);
-                if (flag && Lb) flag = log_plane_max(Lb, r, c, ctr);
-                if (flag && Lt) flag = log_plane_max(Lt, r, c, ctr);
-                if (flag) flag = !(P[static_cast<size_t>(gy) * W + gx] < snr_thr);
-            }
-            const uint32_t pos = wave_append(flag, cnt);
-            if (flag && pos < cap) {
-                fc[pos] = static_cast<uint32_t>(m) * static_cast<uint32_t>(HW) +
-                          static_cast<uint32_t>(gy) * static_cast<uint32_t>(W) +
-                          static_cast<uint32_t>(gx);
-                if (cval) cval[static_cast<size_t>(fr) * cap + pos] = Lm[r * LG_RC + c];
+    const int rot = static_cast<int>(blockIdx.x % LG_WAVES);
+    // the maximum test: this lane's pixels are tile column `lane`, tile rows
+    // tr0 .. tr0 + LG_TR - 1; the centre and neighbour maximum of the
+    // previous scale (cp, xp) and the one before it (xpp) stay in registers
+    const int tr0 = wave * LG_TR;
+    const int gx = x0 + lane;
+    float cp[LG_TR], xp[LG_TR], xpp[LG_TR];
+#pragma unroll
+    for (int s = 0; s < LG_TR; ++s) cp[s] = xp[s] = xpp[s] = 0.0f;
+    // scale m's candidates: a positive centre with no neighbour >= it in its
+    // own plane, in scale m - 1's (m >= 1) and in scale m + 1's (xu, when
+    // `up`: m + 1 < n); pixels of the frame's one-pixel border never qualify
+    auto test = [&](int m, bool up, const float (&xu)[LG_TR]) {
+        bool fl[LG_TR];
+#pragma unroll
+        for (int s = 0; s < LG_TR; ++s) {
+            const int gy = y0 + tr0 + s;
+            const float ctr = cp[s];
+            bool flag = gy >= 1 && gy <= H - 2 && gx >= 1 && gx <= W - 2 && !(ctr <= 0.0f) &&
+                        !(xp[s] >= ctr);
+            if (m >= 1) flag = flag && !(xpp[s] >= ctr);
+            if (up) flag = flag && !(xu[s] >= ctr);
+            if (flag) flag = !(P[static_cast<size_t>(gy) * W + gx] < snr_thr);
+            fl[s] = flag;
+        }
+        uint32_t pos[LG_TR];
+        wave_append_n<LG_TR>(fl, cnt, pos);
+#pragma unroll
+        for (int s = 0; s < LG_TR; ++s) {
+            if (fl[s] && pos[s] < cap) {
+                const int gy = y0 + tr0 + s;
+                fc[pos[s]] = static_cast<uint32_t>(m) * static_cast<uint32_t>(HW) +
+                             static_cast<uint32_t>(gy) * static_cast<uint32_t>(W) +
+                             static_cast<uint32_t>(gx);
+                if (cval) cval[static_cast<size_t>(fr) * cap + pos[s]] = cp[s];
             }
         }
     };
     // threshold_rel: the largest LoG value of the tile's own pixels over
-    // every scale (each frame pixel is in one tile's interior)
+    // every scale (each frame pixel is in one tile)
     float lmax = 0.0f;
     for (int k = 0; k < n; ++k) {
-        const int R = tab->R[k], ntaps = 2 * R + 1;
+        const int R = tab->R[k];
         const float *tg = tab->taps + tab->off[k];
-        const float *td = tg + ntaps;
+        const float *td = tg + 2 * R + 1;
         const float n1 = tab->n1[k], n3 = tab->n3[k];
-        // vertical pass over columns -R .. LG_RC + R of the region
-        const int ncv = LG_RC + 2 * R;
-        for (int item = tid; item < (LG_RR / LG_MV) * ncv; item += LG_THREADS) {
-            const int q = item / ncv, vc = item - q * ncv, r0 = q * LG_MV;
-            const float *colp = Sraw + (r0 + Rm - R) * RP + vc + (Rm - R);
-            float ag[LG_MV], ad[LG_MV];
-            log_vpass(colp, RP, ntaps, tg, td, ag, ad);
-#pragma unroll
-            for (int m = 0; m < LG_MV; ++m) {
-                Vd[(r0 + m) * VP + vc] = ad[m] * n3;
-                Vg[(r0 + m) * VP + vc] = ag[m] * n1;
-            }
+        switch (R) {
+#define GT_LG_CASE(RV)                                                                            \
+    case RV:                                                                                      \
+        log_scale_passes<RV>(Sraw, Vd, Vg, Lr, g, R, tg, td, n1, n3, wave, lane, rot);           \
+        break;
+            GT_LG_CASE(1) GT_LG_CASE(2) GT_LG_CASE(3) GT_LG_CASE(4) GT_LG_CASE(5) GT_LG_CASE(6)
+            GT_LG_CASE(7) GT_LG_CASE(8) GT_LG_CASE(9) GT_LG_CASE(10) GT_LG_CASE(11)
+            GT_LG_CASE(12) GT_LG_CASE(13) GT_LG_CASE(14) GT_LG_CASE(15) GT_LG_CASE(16)
+            GT_LG_CASE(17) GT_LG_CASE(18) GT_LG_CASE(19) GT_LG_CASE(20) GT_LG_CASE(21)
+            GT_LG_CASE(22) GT_LG_CASE(23) GT_LG_CASE(24)
+#undef GT_LG_CASE
+        default:
+            log_scale_passes<0>(Sraw, Vd, Vg, Lr, g, R, tg, td, n1, n3, wave, lane, rot);
+            break;
         }
         __syncthreads();
-        // horizontal pass; L = (0 - R0) - R1 as the reference subtracts both
-        // passes from a zeroed plane
-        float *Lk = Lr + (k % 3) * (LG_RR * LG_RC);
-        for (int item = tid; item < LG_RR * (LG_RC / LG_MH); item += LG_THREADS) {
-            const int r = item % LG_RR, c0 = (item / LG_RR) * LG_MH;
-            float a0[LG_MH], a1[LG_MH];
-            log_hpass(Vd + r * VP + c0, Vg + r * VP + c0, ntaps, tg, td, a0, a1);
+        float c[LG_TR], x[LG_TR];
+        log_plane_stats(Lr, tr0, lane, c, x);
+        if (cval) {
 #pragma unroll
-            for (int m = 0; m < LG_MH; ++m) {
-                float l = 0.0f - a0[m] * n1;
-                l = l - a1[m] * n3;
-                Lk[r * LG_RC + c0 + m] = l;
-                if (cval) {
-                    const int c = c0 + m;
-                    if (r >= 1 && r <= LG_TH && c >= 1 && c <= LG_TW && y0 + r - 1 < H &&
-                        x0 + c - 1 < W)
-                        lmax = fmaxf(lmax, l);
-                }
-            }
+            for (int s = 0; s < LG_TR; ++s)
+                if (y0 + tr0 + s < H && gx < W) lmax = fmaxf(lmax, c[s]);
         }
-        __syncthreads();
-        if (k >= 1) test(k - 1);
+        if (k >= 1) test(k - 1, true, x);
+#pragma unroll
+        for (int s = 0; s < LG_TR; ++s) {
+            xpp[s] = xp[s];
+            xp[s] = x[s];
+            cp[s] = c[s];
+        }
     }
-    test(n - 1);
+    if (n > 0) test(n - 1, false, xp);
     if (cval) {
         // one atomic per wave: non-negative floats order as their bit patterns
 #pragma unroll
