@@ -2507,27 +2507,33 @@ hipError_t gt_launch_characterize_det(const float *proc, const void *raw, int dt
 // H * W + raster index) go to a per-frame list; k_walk_log walks them.
 namespace {
 
-constexpr int LG_TW = 64, LG_TH = 16;
+constexpr int LG_TW = 128, LG_TH = 16;
 constexpr int LG_RR = LG_TH + 2, LG_RC = LG_TW + 2;  // plane region: tile + 1-pixel ring
 constexpr int LG_MV = 6;                             // rows per item of the vertical pass
-constexpr int LG_MH = 3;                             // columns per item of the horizontal pass
-constexpr int LG_THREADS = 256;
+constexpr int LG_MH = 5;                             // columns per item of the horizontal pass
+constexpr int LG_NG = LG_RC / LG_MH;                 // horizontal items per region row
+constexpr int LG_THREADS = 512;
 constexpr int LG_WAVES = LG_THREADS / 64;
-constexpr int LG_TR = LG_TH / LG_WAVES;              // tile rows per lane in the maximum test
+constexpr int LG_CB = LG_TW / 64;                    // 64-column blocks of the tile
+constexpr int LG_TR = LG_TH * LG_CB / LG_WAVES;      // tile rows per lane in the maximum test
 constexpr int LG_RU = 24;                            // largest tap radius with unrolled passes
-// Four workgroups of LG_THREADS fit a CU's LDS at Rm = 17; their waves need
+// Two workgroups of LG_THREADS fit a CU's LDS at Rm = 17; their waves need
 // at most 128 VGPRs each.
 constexpr int LG_MIN_WAVES = 4;
 static_assert(LG_RR % LG_MV == 0, "vertical items must tile the region rows");
-static_assert(LG_RC % LG_MH == 0, "horizontal items must tile the region columns");
-static_assert(LG_TW == 64 && LG_TH % LG_WAVES == 0, "one tile column per lane in the test");
+static_assert(LG_RC % LG_MH == 0 && (LG_MH & 1), "odd horizontal items must tile the region columns");
+static_assert(LG_TW % 64 == 0 && LG_WAVES % LG_CB == 0 && LG_TR * (LG_WAVES / LG_CB) == LG_TH,
+              "one tile column per lane in the test");
 
 // LDS layout for a largest tap radius Rm:
 //   raw   NC columns of CP samples, column-major (NC = LG_RC + 2 Rm, CP =
 //         (LG_RR + 2 Rm) | 1): the vertical pass reads down a column at
 //         compile-time offsets, and lanes on neighbouring columns fall in
 //         different banks.  The last sample of a column is padding.
-//   Vd,Vg LG_RR rows of VP = NC | 1 (vertical-pass outputs, both tap sets)
+//   Vd,Vg LG_RR rows of VP >= NC + 1 with VP = LG_RC (mod 64): horizontal
+//         item i (row i / LG_NG, columns LG_MH * (i % LG_NG) ...) then starts
+//         at LG_MH * i (mod 64), so the 64 items of a wave start in 64
+//         different banks (LG_MH odd)
 //   L     LG_RR x LG_RC (the current scale's plane)
 struct LogGeom {
     int Rm, NC, CP, VP;
@@ -2537,7 +2543,7 @@ inline LogGeom log_geom(int Rm) {
     g.Rm = Rm;
     g.NC = LG_RC + 2 * Rm;
     g.CP = (LG_RR + 2 * Rm) | 1;
-    g.VP = g.NC | 1;
+    g.VP = g.NC + 1 + ((LG_RC - (g.NC + 1)) % 64 + 64) % 64;
     return g;
 }
 inline size_t log_lds_bytes(const LogGeom &g) {
@@ -2736,11 +2742,11 @@ __device__ __forceinline__ void log_scale_passes(const float *Sraw, float *Vd, f
         }
     }
     __syncthreads();
-    constexpr int NH = LG_RR * (LG_RC / LG_MH);
+    constexpr int NH = LG_RR * LG_NG;
     for (int b = b0; b * 64 < NH; b += LG_WAVES) {
         const int item = b * 64 + lane;
         if (item < NH) {
-            const int r = item % LG_RR, c0 = (item / LG_RR) * LG_MH;
+            const int r = item / LG_NG, c0 = (item - r * LG_NG) * LG_MH;
             float a0[LG_MH], a1[LG_MH];
             if constexpr (RT > 0)
                 log_hpass_u<RT>(Vd + r * g.VP + c0, Vg + r * g.VP + c0, tg, td, a0, a1);
@@ -2783,13 +2789,13 @@ __device__ __forceinline__ void wave_append_n(const bool (&f)[N], uint32_t *coun
 }
 
 // Centre value c[s] and largest of the 8 neighbours x[s] in plane Lk of the
-// lane's LG_TR tile pixels (column `lane`, tile rows tr0 .. tr0 + LG_TR - 1).
+// lane's LG_TR tile pixels (tile column tc, tile rows tr0 .. tr0 + LG_TR - 1).
 // fmaxf skips NaN, so x is the largest non-NaN neighbour, NaN when all 8
 // are: "no neighbour >= c" (is_max_in_plane, logspace_max.wgsl:62-92) is
 // then !(x >= c) in every case.
-__device__ __forceinline__ void log_plane_stats(const float *Lk, int tr0, int lane,
+__device__ __forceinline__ void log_plane_stats(const float *Lk, int tr0, int tc,
                                                 float (&c)[LG_TR], float (&x)[LG_TR]) {
-    const float *b = Lk + tr0 * LG_RC + lane;  // region (tr0, lane): the first pixel's top-left
+    const float *b = Lk + tr0 * LG_RC + tc;  // region (tr0, tc): the first pixel's top-left
     float v[LG_TR + 2][3];
 #pragma unroll
     for (int i = 0; i < LG_TR + 2; ++i)
@@ -2865,11 +2871,12 @@ __global__ __launch_bounds__(LG_THREADS, LG_MIN_WAVES) void k_log_tile(const T *
     uint32_t *cnt = &ccount[fr * GT_COUNTER_STRIDE];
     uint32_t *fc = cand + static_cast<size_t>(fr) * cap;
     const int rot = static_cast<int>(blockIdx.x % LG_WAVES);
-    // the maximum test: this lane's pixels are tile column `lane`, tile rows
+    // the maximum test: this lane's pixels are tile column tc, tile rows
     // tr0 .. tr0 + LG_TR - 1; the centre and neighbour maximum of the
     // previous scale (cp, xp) and the one before it (xpp) stay in registers
-    const int tr0 = wave * LG_TR;
-    const int gx = x0 + lane;
+    const int tr0 = (wave / LG_CB) * LG_TR;
+    const int tc = (wave % LG_CB) * 64 + lane;
+    const int gx = x0 + tc;
     float cp[LG_TR], xp[LG_TR], xpp[LG_TR];
 #pragma unroll
     for (int s = 0; s < LG_TR; ++s) cp[s] = xp[s] = xpp[s] = 0.0f;
@@ -2927,7 +2934,7 @@ __global__ __launch_bounds__(LG_THREADS, LG_MIN_WAVES) void k_log_tile(const T *
         }
         __syncthreads();
         float c[LG_TR], x[LG_TR];
-        log_plane_stats(Lr, tr0, lane, c, x);
+        log_plane_stats(Lr, tr0, tc, c, x);
         if (cval) {
 #pragma unroll
             for (int s = 0; s < LG_TR; ++s)
