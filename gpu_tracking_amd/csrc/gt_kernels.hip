@@ -2517,6 +2517,7 @@ constexpr int LG_WAVES = LG_THREADS / 64;
 constexpr int LG_CB = LG_TW / 64;                    // 64-column blocks of the tile
 constexpr int LG_TR = LG_TH * LG_CB / LG_WAVES;      // tile rows per lane in the maximum test
 constexpr int LG_RU = 24;                            // largest tap radius with unrolled passes
+constexpr int LG_BUF = 1024;                         // a tile's buffered candidates
 // Two workgroups of LG_THREADS fit a CU's LDS at Rm = 17; their waves need
 // at most 128 VGPRs each.
 constexpr int LG_MIN_WAVES = 4;
@@ -2535,6 +2536,8 @@ static_assert(LG_TW % 64 == 0 && LG_WAVES % LG_CB == 0 && LG_TR * (LG_WAVES / LG
 //         at LG_MH * i (mod 64), so the 64 items of a wave start in 64
 //         different banks (LG_MH odd)
 //   L     LG_RR x LG_RC (the current scale's plane)
+//   B     LG_BUF candidate seeds, then their LoG values: the tile's
+//         candidates, written to the frame's list as one block at the end
 struct LogGeom {
     int Rm, NC, CP, VP;
 };
@@ -2548,7 +2551,7 @@ inline LogGeom log_geom(int Rm) {
 }
 inline size_t log_lds_bytes(const LogGeom &g) {
     return sizeof(float) * (static_cast<size_t>(g.NC) * g.CP + 2u * LG_RR * g.VP +
-                            static_cast<size_t>(LG_RR) * LG_RC);
+                            static_cast<size_t>(LG_RR) * LG_RC + 2u * LG_BUF);
 }
 
 // Vertical 1-D pass of one scale for LG_MV consecutive outputs of one column
@@ -2838,7 +2841,11 @@ __global__ __launch_bounds__(LG_THREADS, LG_MIN_WAVES) void k_log_tile(const T *
     float *Vd = Sraw + g.NC * g.CP;
     float *Vg = Vd + LG_RR * g.VP;
     float *Lr = Vg + LG_RR * g.VP;
+    uint32_t *Bs = reinterpret_cast<uint32_t *>(Lr + LG_RR * LG_RC);
+    float *Bv = reinterpret_cast<float *>(Bs + LG_BUF);
+    __shared__ uint32_t s_fill, s_base;
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    if (tid == 0) s_fill = 0u;
     // raw tile, column-major: region origin (y0 - 1, x0 - 1) less the largest
     // tap radius; outside the frame 0 (a zero sample adds nothing to a tap
     // sum, as the reference's skipped out-of-frame taps).  Four rows per wave
@@ -2896,16 +2903,30 @@ __global__ __launch_bounds__(LG_THREADS, LG_MIN_WAVES) void k_log_tile(const T *
             if (flag) flag = !(P[static_cast<size_t>(gy) * W + gx] < snr_thr);
             fl[s] = flag;
         }
-        uint32_t pos[LG_TR];
-        wave_append_n<LG_TR>(fl, cnt, pos);
+        // into the tile's buffer; what does not fit goes to the frame's list
+        uint32_t pos[LG_TR], gpos[LG_TR];
+        bool of[LG_TR];
+        wave_append_n<LG_TR>(fl, &s_fill, pos);
 #pragma unroll
         for (int s = 0; s < LG_TR; ++s) {
-            if (fl[s] && pos[s] < cap) {
+            of[s] = fl[s] && pos[s] >= static_cast<uint32_t>(LG_BUF);
+            if (fl[s] && !of[s]) {
                 const int gy = y0 + tr0 + s;
-                fc[pos[s]] = static_cast<uint32_t>(m) * static_cast<uint32_t>(HW) +
+                Bs[pos[s]] = static_cast<uint32_t>(m) * static_cast<uint32_t>(HW) +
                              static_cast<uint32_t>(gy) * static_cast<uint32_t>(W) +
                              static_cast<uint32_t>(gx);
-                if (cval) cval[static_cast<size_t>(fr) * cap + pos[s]] = cp[s];
+                Bv[pos[s]] = cp[s];
+            }
+        }
+        wave_append_n<LG_TR>(of, cnt, gpos);
+#pragma unroll
+        for (int s = 0; s < LG_TR; ++s) {
+            if (of[s] && gpos[s] < cap) {
+                const int gy = y0 + tr0 + s;
+                fc[gpos[s]] = static_cast<uint32_t>(m) * static_cast<uint32_t>(HW) +
+                              static_cast<uint32_t>(gy) * static_cast<uint32_t>(W) +
+                              static_cast<uint32_t>(gx);
+                if (cval) cval[static_cast<size_t>(fr) * cap + gpos[s]] = cp[s];
             }
         }
     };
@@ -2949,6 +2970,19 @@ __global__ __launch_bounds__(LG_THREADS, LG_MIN_WAVES) void k_log_tile(const T *
         }
     }
     if (n > 0) test(n - 1, false, xp);
+    // the buffered candidates as one block of the frame's list (the walks
+    // then gather from one neighbourhood per block)
+    __syncthreads();
+    const uint32_t nb = min(s_fill, static_cast<uint32_t>(LG_BUF));
+    if (tid == 0) s_base = nb ? atomicAdd(cnt, nb) : 0u;
+    __syncthreads();
+    for (uint32_t i = tid; i < nb; i += LG_THREADS) {
+        const uint32_t gp = s_base + i;
+        if (gp < cap) {
+            fc[gp] = Bs[i];
+            if (cval) cval[static_cast<size_t>(fr) * cap + gp] = Bv[i];
+        }
+    }
     if (cval) {
         // one atomic per wave: non-negative floats order as their bit patterns
 #pragma unroll
