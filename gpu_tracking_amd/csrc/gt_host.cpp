@@ -229,6 +229,20 @@ void gt_host_fill_logtable(const gt_derived *d, GtLogTable *t) {
         t->n1[k] = 1.0f / (sqrt2pi * sigma);
         t->n3[k] = 1.0f / (sqrt2pi * sigma * sigma2);
         if (R > t->Rm) t->Rm = R;
+        // walk disk rows: the same float test as the kernels' per-pixel one
+        const float r = d->radii[k], r2 = r * r;
+        const int32_t rint = f2i_sat(std::ceil(r));
+        for (int32_t i = -GT_LOG_WR; i <= GT_LOG_WR; ++i) {
+            int32_t jm = -1;
+            if (rint <= GT_LOG_WR && i >= -rint && i <= rint) {
+                const float x = static_cast<float>(i), x2 = x * x;
+                for (int32_t j = 0; j <= rint; ++j) {
+                    const float y = static_cast<float>(j), y2 = y * y;
+                    if (!(x2 + y2 > r2)) jm = j;
+                }
+            }
+            t->wjm[k][i + GT_LOG_WR] = static_cast<int8_t>(jm);
+        }
     }
 }
 

@@ -2938,6 +2938,7 @@ __global__ __launch_bounds__(LG_THREADS, LG_MIN_WAVES) void k_log_tile(const T *
         const float *tg = tab->taps + tab->off[k];
         const float *td = tg + 2 * R + 1;
         const float n1 = tab->n1[k], n3 = tab->n3[k];
+        static_assert(LG_RU == 24, "the unrolled cases below are tap radii 1..24");
         switch (R) {
 #define GT_LG_CASE(RV)                                                                            \
     case RV:                                                                                      \
@@ -2995,6 +2996,71 @@ __global__ __launch_bounds__(LG_THREADS, LG_MIN_WAVES) void k_log_tile(const T *
 // Walk + characterize of the LoG candidates (walk.wgsl with r > 0: rint =
 // ceil(r), r2 = r*r, move band rint; characterize.wgsl at the same radius).
 // A frame whose candidate list overflowed reports count 0xFFFFFFFF.
+//
+// Disks wholly inside the frame run over the rows of the scale's disk table
+// (row i: columns -jm .. jm), with no per-pixel mask or bounds test: the same
+// pixels in the same order as get_center / characterize_row, so the same
+// bits.
+
+// get_center over a table disk around (u, v) inside the frame
+__device__ __forceinline__ WalkRes log_center_inner(const float *__restrict__ P, int W, int u,
+                                                    int v, int rint, const int8_t *jm) {
+    WalkRes res = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    for (int i = -rint; i <= rint; ++i) {
+        const int m = jm[i + GT_LOG_WR];
+        const float x = static_cast<float>(i);
+        const float *row = P + static_cast<size_t>(u + i) * W + v;
+        for (int j = -m; j <= m; ++j) {
+            const float d = row[j];
+            res.x = fmaf(d, x, res.x);
+            res.y = fmaf(d, static_cast<float>(j), res.y);
+            res.mass = res.mass + d;
+            res.count = res.count + 1.0f;
+            res.max = maxf_ref(res.max, d);
+        }
+    }
+    res.x = res.x / res.mass;
+    res.y = res.y / res.mass;
+    return res;
+}
+
+// characterize_row for a row of radius r > 0 (rint = ceil(r)) whose disk
+// around the rounded centre (u, v) is inside the frame, over the table disk
+template <typename T>
+__device__ __forceinline__ void log_characterize_inner(const float *__restrict__ P,
+                                                       const T *__restrict__ Rw, int W, int u,
+                                                       int v, int rint, const int8_t *jm,
+                                                       gt_result_row &row) {
+    CharAcc acc = {0.0f, 0.0f, 0.0, 0.0, 0.0};
+    for (int i = -rint; i <= rint; ++i) {
+        const int m = jm[i + GT_LOG_WR];
+        const float x = static_cast<float>(i), x2 = x * x;
+        const size_t ro = static_cast<size_t>(u + i) * W + v;
+        const float *prow = P + ro;
+        const T *rrow = Rw + ro;
+        for (int j = -m; j <= m; ++j) {
+            const float y = static_cast<float>(j), y2 = y * y;
+            const float mask = x2 + y2;
+            const float d = prow[j];
+            acc.raw_mass = acc.raw_mass + to_f32(rrow[j]);
+            acc.signal = maxf_ref(acc.signal, d);
+            if (i != 0 || j != 0) {
+                const double2 sc = sincos2(i, j);
+                acc.es = fma(static_cast<double>(d), sc.x, acc.es);
+                acc.ec = fma(static_cast<double>(d), sc.y, acc.ec);
+            } else {
+                acc.ec = acc.ec + static_cast<double>(d);
+            }
+            acc.rg = fma(static_cast<double>(d), static_cast<double>(mask), acc.rg);
+        }
+    }
+    const float den = row.mass - P[static_cast<size_t>(u) * W + v];
+    row.ecc = static_cast<float>(sqrt(acc.es * acc.es + acc.ec * acc.ec) / static_cast<double>(den));
+    row.Rg = static_cast<float>(sqrt(acc.rg / static_cast<double>(row.mass)));
+    row.raw_mass = acc.raw_mass;
+    row.signal = acc.signal;
+}
+
 template <typename T>
 __global__ __launch_bounds__(WK_THREADS) void k_walk_log(const float *__restrict__ proc,
                                                          const T *__restrict__ raw, GtKParams p,
@@ -3007,19 +3073,27 @@ __global__ __launch_bounds__(WK_THREADS) void k_walk_log(const float *__restrict
                                                          uint32_t *__restrict__ ocount,
                                                          uint32_t frame0,
                                                          const float *__restrict__ cval) {
+    constexpr int JW = 2 * GT_LOG_WR + 1;
+    __shared__ int8_t s_jm[GT_LOG_MAX_SCALES * JW];
     const int fr = blockIdx.y;
     const int H = p.H, W = p.W;
     const uint32_t HW = static_cast<uint32_t>(H) * static_cast<uint32_t>(W);
+    const int ns = tab->n;
+    for (int q = threadIdx.x; q < ns * JW; q += WK_THREADS)
+        s_jm[q] = tab->wjm[q / JW][q - (q / JW) * JW];
+    __syncthreads();
     // threshold_rel: the candidate's LoG value must exceed thr_rel times the
     // frame's largest LoG value (k_log_tile's per-frame maximum)
     const float rel_thr =
         cval ? p.thr_rel * __uint_as_float(ccount[fr * GT_COUNTER_STRIDE + 1]) : 0.0f;
-    const GlobalSrc src{proc + static_cast<size_t>(fr) * HW, W};
+    const float *P = proc + static_cast<size_t>(fr) * HW;
+    const GlobalSrc src{P, W};
     const T *Rw = raw + static_cast<size_t>(fr) * HW;
     const uint32_t nc = ccount[fr * GT_COUNTER_STRIDE];
     const uint32_t n = min(nc, cap);
     const uint32_t *fc = cand + static_cast<size_t>(fr) * cap;
     gt_detection *fout = out + static_cast<size_t>(fr) * out_cap;
+    const float stdv = frame_std(p, fr);
     const uint32_t lane = __lane_id();
     const uint32_t stride = gridDim.x * WK_THREADS;
     for (uint32_t i0 = blockIdx.x * WK_THREADS + (threadIdx.x & ~63u); i0 < n; i0 += stride) {
@@ -3035,18 +3109,29 @@ __global__ __launch_bounds__(WK_THREADS) void k_walk_log(const float *__restrict
             const float r = tab->radius[m];
             const int rint = f2i(ceilf(r));
             const float r2 = r * r;
+            const int8_t *jm = s_jm + m * JW;
+            const bool table = rint <= GT_LOG_WR;
             WalkRes res = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
             for (uint32_t it = 0; it < p.max_iter; ++it) {
-                res = get_center<0>(src, H, W, pu, pv, rint, r2);
+                if (table && pu >= rint && pu + rint < H && pv >= rint && pv + rint < W)
+                    res = log_center_inner(P, W, pu, pv, rint, jm);
+                else
+                    res = get_center<0>(src, H, W, pu, pv, rint, r2);
                 if (!walk_move(res, pu, pv, H, W, rint, p.shift_thr)) break;
             }
             res.x = res.x + static_cast<float>(pu);
             res.y = res.y + static_cast<float>(pv);
-            keep = passes_minmass(p, res, frame_std(p, fr));
+            keep = passes_minmass(p, res, stdv);
             if (keep) {
                 fill_row(d, res, seed, frame0 + fr);
                 d.row.r = r;
-                if (p.characterize) characterize_row<T>(src, Rw, H, W, p.circle, d.row);
+                if (p.characterize) {
+                    const int u = f2i(rintf(d.row.x)), v = f2i(rintf(d.row.y));
+                    if (table && u >= rint && u + rint < H && v >= rint && v + rint < W)
+                        log_characterize_inner<T>(P, Rw, W, u, v, rint, jm, d.row);
+                    else
+                        characterize_row<T>(src, Rw, H, W, p.circle, d.row);
+                }
             }
         }
         const uint32_t pos = wave_append(keep, &ocount[fr * GT_COUNTER_STRIDE]);

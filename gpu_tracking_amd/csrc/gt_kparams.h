@@ -51,6 +51,7 @@ struct GtKParams {
 #define GT_LOG_MAX_SCALES 64
 #define GT_LOG_MAX_R 48
 #define GT_LOG_MAX_TAPS (2 * GT_LOG_MAX_SCALES * (2 * GT_LOG_MAX_R + 1))
+#define GT_LOG_WR 24  // largest walk radius ceil(r) with a row table
 struct GtLogTable {
     int32_t n;                         // scales
     int32_t Rm;                        // largest tap radius over the scales
@@ -62,6 +63,11 @@ struct GtLogTable {
     float n1[GT_LOG_MAX_SCALES];       // 1 / (sqrt(2 pi) sigma)
     float n3[GT_LOG_MAX_SCALES];       // 1 / (sqrt(2 pi) sigma sigma^2)
     float taps[GT_LOG_MAX_TAPS];
+    // the walk and characterize disk of scale k, x^2 + y^2 <= radius^2 in
+    // float arithmetic: row i (|i| <= ceil(radius)) holds columns
+    // -wjm[k][i + GT_LOG_WR] .. wjm[k][i + GT_LOG_WR], none where it is -1;
+    // filled where ceil(radius) <= GT_LOG_WR
+    int8_t wjm[GT_LOG_MAX_SCALES][2 * GT_LOG_WR + 1];
 };
 
 // Batched launches: frames of different sizes in one launch.  One entry per
