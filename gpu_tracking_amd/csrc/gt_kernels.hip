@@ -2999,8 +2999,12 @@ __global__ __launch_bounds__(LG_THREADS, LG_MIN_WAVES) void k_log_tile(const T *
 //
 // Disks wholly inside the frame run over the rows of the scale's disk table
 // (row i: columns -jm .. jm), with no per-pixel mask or bounds test: the same
-// pixels in the same order as get_center / characterize_row, so the same
-// bits.
+// pixels in the same order as get_center / characterize_row.  A row is read
+// LW_C pixels at a time, so that its loads are in flight together; the
+// slots of the last group past the row end read the row's last pixel and
+// add an exact zero instead (every sum and moment here starts at +0 and
+// never becomes -0, and the maxima start at +0), so the bits are the same.
+constexpr int LW_C = 4;
 
 // get_center over a table disk around (u, v) inside the frame
 __device__ __forceinline__ WalkRes log_center_inner(const float *__restrict__ P, int W, int u,
@@ -3010,13 +3014,20 @@ __device__ __forceinline__ WalkRes log_center_inner(const float *__restrict__ P,
         const int m = jm[i + GT_LOG_WR];
         const float x = static_cast<float>(i);
         const float *row = P + static_cast<size_t>(u + i) * W + v;
-        for (int j = -m; j <= m; ++j) {
-            const float d = row[j];
-            res.x = fmaf(d, x, res.x);
-            res.y = fmaf(d, static_cast<float>(j), res.y);
-            res.mass = res.mass + d;
-            res.count = res.count + 1.0f;
-            res.max = maxf_ref(res.max, d);
+        for (int j0 = -m; j0 <= m; j0 += LW_C) {
+            float dv[LW_C];
+#pragma unroll
+            for (int q = 0; q < LW_C; ++q) dv[q] = row[min(j0 + q, m)];
+#pragma unroll
+            for (int q = 0; q < LW_C; ++q) {
+                const bool in = j0 + q <= m;
+                const float d = in ? dv[q] : 0.0f;
+                res.x = fmaf(d, x, res.x);
+                res.y = fmaf(d, static_cast<float>(j0 + q), res.y);
+                res.mass = res.mass + d;
+                res.count = res.count + (in ? 1.0f : 0.0f);
+                res.max = maxf_ref(res.max, d);
+            }
         }
     }
     res.x = res.x / res.mass;
@@ -3038,20 +3049,33 @@ __device__ __forceinline__ void log_characterize_inner(const float *__restrict__
         const size_t ro = static_cast<size_t>(u + i) * W + v;
         const float *prow = P + ro;
         const T *rrow = Rw + ro;
-        for (int j = -m; j <= m; ++j) {
-            const float y = static_cast<float>(j), y2 = y * y;
-            const float mask = x2 + y2;
-            const float d = prow[j];
-            acc.raw_mass = acc.raw_mass + to_f32(rrow[j]);
-            acc.signal = maxf_ref(acc.signal, d);
-            if (i != 0 || j != 0) {
-                const double2 sc = sincos2(i, j);
-                acc.es = fma(static_cast<double>(d), sc.x, acc.es);
-                acc.ec = fma(static_cast<double>(d), sc.y, acc.ec);
-            } else {
-                acc.ec = acc.ec + static_cast<double>(d);
+        for (int j0 = -m; j0 <= m; j0 += LW_C) {
+            float dv[LW_C], rv[LW_C];
+            double2 sv[LW_C];
+#pragma unroll
+            for (int q = 0; q < LW_C; ++q) {
+                const int j = min(j0 + q, m);
+                dv[q] = prow[j];
+                rv[q] = to_f32(rrow[j]);
+                sv[q] = sincos2(i, j);
             }
-            acc.rg = fma(static_cast<double>(d), static_cast<double>(mask), acc.rg);
+#pragma unroll
+            for (int q = 0; q < LW_C; ++q) {
+                const int j = j0 + q;
+                const bool in = j <= m;
+                const float y = static_cast<float>(j), y2 = y * y;
+                const float mask = x2 + y2;
+                const float d = in ? dv[q] : 0.0f;
+                acc.raw_mass = acc.raw_mass + (in ? rv[q] : 0.0f);
+                acc.signal = maxf_ref(acc.signal, d);
+                if (i != 0 || j != 0) {
+                    acc.es = fma(static_cast<double>(d), sv[q].x, acc.es);
+                    acc.ec = fma(static_cast<double>(d), sv[q].y, acc.ec);
+                } else {
+                    acc.ec = acc.ec + static_cast<double>(d);
+                }
+                acc.rg = fma(static_cast<double>(d), static_cast<double>(mask), acc.rg);
+            }
         }
     }
     const float den = row.mass - P[static_cast<size_t>(u) * W + v];
