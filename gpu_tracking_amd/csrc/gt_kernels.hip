@@ -61,6 +61,25 @@ inline unsigned xcd_grid(size_t ntiles) {
     return static_cast<unsigned>(per * kXcds);
 }
 
+// Appends the lanes with `fa` set to list A (*ca) and those with `fb` set to
+// list B (*cb) with one atomic instruction per wavefront (lane 0 adds to A,
+// lane 1 to B): one memory round trip for both.  Must be reached by every
+// lane of the wave.
+__device__ __forceinline__ void wave_append2(bool fa, uint32_t *ca, bool fb, uint32_t *cb,
+                                             uint32_t &pa, uint32_t &pb) {
+    const unsigned long long ma = __ballot(fa), mb = __ballot(fb);
+    const uint32_t lane = __lane_id();
+    const unsigned long long below = (1ull << lane) - 1ull;
+    uint32_t old = 0u;
+    if ((ma | mb) != 0ull) {
+        const uint32_t add = static_cast<uint32_t>(__popcll(lane == 0 ? ma : mb));
+        uint32_t *ctr = lane == 0 ? ca : cb;
+        if (lane < 2 && add != 0u) old = atomicAdd(ctr, add);
+    }
+    pa = __shfl(old, 0) + static_cast<uint32_t>(__popcll(ma & below));
+    pb = __shfl(old, 1) + static_cast<uint32_t>(__popcll(mb & below));
+}
+
 // Appends the lanes with `flag` set to a list guarded by *counter: one
 // atomicAdd per wavefront.  Must be reached by every lane of the wave.
 __device__ __forceinline__ uint32_t wave_append(bool flag, uint32_t *counter) {
@@ -936,6 +955,20 @@ __device__ __forceinline__ bool finish_walk(const GtKParams &p, int H, int W,
     return true;
 }
 
+// Workgroup barrier between the phases of a detect tile.
+struct SyncBarrier {
+    __device__ __forceinline__ static void sync() { __syncthreads(); }
+};
+
+template <typename T, int KT, int RI, bool kCand, uint32_t kSteps, class Bar>
+__device__ __forceinline__ void detect_compute(
+    const float *__restrict__ P, const T *__restrict__ Rw, const GtKParams &p, int H, int W,
+    const TileGeom &g, int x0, int y0, uint32_t frame, gt_detection *__restrict__ fout,
+    uint32_t out_cap, uint32_t *__restrict__ counter, uint2 *__restrict__ fslow,
+    uint32_t slow_cap, uint32_t *__restrict__ slow_counter, float *Sp, float *Hm,
+    uint32_t *s_cnt, float thr, float stdv,
+    uint32_t *__restrict__ fcand);
+
 // One FT_TH x FT_TW tile (origin x0, y0) of one frame: P, Rw are its
 // processed and raw planes (p.H x p.W); its rows go to fout (counted in
 // *counter), walks handed over to fslow (counted in *slow_counter); `frame`
@@ -950,19 +983,13 @@ __device__ __forceinline__ void detect_tile_body(
     uint32_t out_cap,
     uint32_t *__restrict__ counter, uint2 *__restrict__ fslow, uint32_t slow_cap,
     uint32_t *__restrict__ slow_counter, float *lds, uint32_t *s_cnt,
-    uint64_t (*s_masks)[FT_WROWS], float thr, float stdv, uint32_t *__restrict__ fcand = nullptr) {
+    float thr, float stdv, uint32_t *__restrict__ fcand = nullptr) {
     constexpr bool kFixed = KT > 0 && RI > 0;
     constexpr TileGeom gc = tile_geom(KT > 0 ? KT : 1, RI > 0 ? RI : 1);
-    constexpr bool kInt = sizeof(T) <= 2;  // integer frames: no NaN in the processed frame
-    const int k = KT > 0 ? KT : p.k;
-    const int a = g.a;
-    const int rint = RI > 0 ? RI : p.rint;
     const int RW = g.pitch, RH = g.rows;
-    const int MR = FT_TH + k - 1;      // rows of horizontal maxima the tile needs
     float *Sp = lds;                   // RH x pitch, origin (y0 - ht, x0 - hl)
     float *Hm = lds + RH * RW;         // MR x FT_TW; later the candidate list
-    uint16_t *s_cand = reinterpret_cast<uint16_t *>(Hm);
-    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int tid = threadIdx.x;
     const int oy = y0 - g.ht, ox = x0 - g.hl;
 
     // 1. staging (the pitch padding is loaded too; it is never compared)
@@ -1029,8 +1056,42 @@ __device__ __forceinline__ void detect_tile_body(
         }
     }
     __syncthreads();
+    detect_compute<T, KT, RI, kCand, kSteps, SyncBarrier>(
+        P, Rw, p, H, W, g, x0, y0, frame, fout, out_cap, counter, fslow, slow_cap, slow_counter,
+        Sp, Hm, s_cnt, thr, stdv, fcand);
+}
 
-    auto vmax = [](float x, float y) { return kInt ? fmaxf(x, y) : maxf_ref(x, y); };
+// Phases 2-5 of a detect tile whose processed region (TileGeom g, origin
+// (y0 - g.ht, x0 - g.hl)) is staged in Sp (out-of-frame entries -inf); Hm
+// holds the horizontal maxima and then the candidate list.  Bar::sync() is
+// the workgroup barrier between the phases.
+template <typename T, int KT, int RI, bool kCand, uint32_t kSteps, class Bar>
+__device__ __forceinline__ void detect_compute(
+    const float *__restrict__ P, const T *__restrict__ Rw, const GtKParams &p, int H, int W,
+    const TileGeom &g, int x0, int y0, uint32_t frame, gt_detection *__restrict__ fout,
+    uint32_t out_cap, uint32_t *__restrict__ counter, uint2 *__restrict__ fslow,
+    uint32_t slow_cap, uint32_t *__restrict__ slow_counter, float *Sp, float *Hm,
+    uint32_t *s_cnt, float thr, float stdv,
+    uint32_t *__restrict__ fcand) {
+    constexpr bool kFixed = KT > 0 && RI > 0;
+    constexpr TileGeom gc = tile_geom(KT > 0 ? KT : 1, RI > 0 ? RI : 1);
+    constexpr bool kInt = sizeof(T) <= 2;  // integer frames: no NaN in the processed frame
+    const int k = KT > 0 ? KT : p.k;
+    const int a = g.a;
+    const int rint = RI > 0 ? RI : p.rint;
+    const int RW = g.pitch, RH = g.rows;
+    const int MR = FT_TH + k - 1;      // rows of horizontal maxima the tile needs
+    uint16_t *s_cand = reinterpret_cast<uint16_t *>(Hm);
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int oy = y0 - g.ht, ox = x0 - g.hl;
+
+    // integer frames have no NaN: IEEE maximum (one instruction, no operand
+    // canonicalization); float frames keep the oracle's comparison
+    auto vmax = [](float x, float y) {
+        if constexpr (kInt) return __builtin_elementwise_maximum(x, y);
+        else return maxf_ref(x, y);
+    };
     // 2. horizontal maxima: Hm[m][c] = max_{j<k} Sp[ht - a + m][hl - a + c + j]
     //    (float frames keep the oracle's sequential order: NaN inputs)
     if constexpr (kFixed && kInt) {
@@ -1074,7 +1135,7 @@ __device__ __forceinline__ void detect_tile_body(
             Hm[item] = mx;
         }
     }
-    __syncthreads();
+    Bar::sync();
 
     // 3. vertical maxima and the candidate test for rows r0..r0+7 of column
     //    `lane`: Hm row q feeds output row r when 0 <= q - r < k
@@ -1082,6 +1143,7 @@ __device__ __forceinline__ void detect_tile_body(
     const int margin = p.margin;
     const int gx = x0 + lane;
     uint32_t nwave = 0;  // wave-uniform
+    uint64_t mks[FT_WROWS];
     {
         float wmax[FT_WROWS];
 #pragma unroll
@@ -1096,16 +1158,16 @@ __device__ __forceinline__ void detect_tile_body(
             float suf[FT_WROWS];
             suf[FT_WROWS - 1] = h[FT_WROWS - 1];
 #pragma unroll
-            for (int q = FT_WROWS - 2; q >= 0; --q) suf[q] = fmaxf(h[q], suf[q + 1]);
+            for (int q = FT_WROWS - 2; q >= 0; --q) suf[q] = vmax(h[q], suf[q + 1]);
             float pre = h[FT_WROWS];
 #pragma unroll
             for (int q = FT_WROWS + 1; q < FT_WROWS + KT - 1; ++q) {
-                pre = fmaxf(pre, h[q]);
+                pre = vmax(pre, h[q]);
                 const int r = q - (KT - 1);   // window r..q ends here
-                if (r >= 0 && r < FT_WROWS) wmax[r] = fmaxf(suf[r], pre);
+                if (r >= 0 && r < FT_WROWS) wmax[r] = vmax(suf[r], pre);
             }
             // the window ending at row FT_WROWS itself (KT = FT_WROWS + 1)
-            if constexpr (KT - 1 == FT_WROWS) wmax[0] = fmaxf(suf[0], h[FT_WROWS]);
+            if constexpr (KT - 1 == FT_WROWS) wmax[0] = vmax(suf[0], h[FT_WROWS]);
         } else if constexpr (KT > 0) {
 #pragma unroll
             for (int q = 0; q < FT_WROWS + KT - 1; ++q) {
@@ -1122,22 +1184,24 @@ __device__ __forceinline__ void detect_tile_body(
                     if (q - r >= 0 && q - r < k) wmax[r] = vmax(wmax[r], hq);
             }
         }
+        // the staged region covers every row and column of the tile, so the
+        // centre values load unconditionally (all in flight at once); the
+        // row test is wave-uniform, the column test per lane
+        float cv[FT_WROWS];
+#pragma unroll
+        for (int r = 0; r < FT_WROWS; ++r) cv[r] = Sp[(g.ht + r0 + r) * RW + g.hl + lane];
+        const bool col_ok = gx < W && gx >= margin && gx < W - margin;
 #pragma unroll
         for (int r = 0; r < FT_WROWS; ++r) {
             const int gy = y0 + r0 + r;
-            bool flag = false;
-            if (gy < H && gx < W && gy >= margin && gy < H - margin && gx >= margin &&
-                gx < W - margin) {
-                const float c = Sp[(g.ht + r0 + r) * RW + g.hl + lane];
-                flag = !(c < thr) && !(wmax[r] > c);
-            }
-            const uint64_t mk = __ballot(flag);
-            if (lane == 0) s_masks[wave][r] = mk;
-            nwave += static_cast<uint32_t>(__popcll(mk));
+            const bool row_ok = gy < H && gy >= margin && gy < H - margin;
+            const bool flag = row_ok && col_ok && !(cv[r] < thr) && !(wmax[r] > cv[r]);
+            mks[r] = __ballot(flag);
+            nwave += static_cast<uint32_t>(__popcll(mks[r]));
         }
     }
     if (lane == 0) s_cnt[wave] = nwave;
-    __syncthreads();  // Hm is dead from here on: the list takes its place
+    Bar::sync();  // Hm is dead from here on: the list takes its place
     uint32_t off = 0, ncand = 0;
 #pragma unroll
     for (int w = 0; w < FT_WAVES; ++w) {
@@ -1147,13 +1211,13 @@ __device__ __forceinline__ void detect_tile_body(
     }
 #pragma unroll
     for (int r = 0; r < FT_WROWS; ++r) {
-        const uint64_t mk = s_masks[wave][r];
+        const uint64_t mk = mks[r];
         if ((mk >> lane) & 1ull)
             s_cand[off + __popcll(mk & ((1ull << lane) - 1ull))] =
                 static_cast<uint16_t>((r0 + r) * FT_TW + lane);
         off += static_cast<uint32_t>(__popcll(mk));
     }
-    __syncthreads();
+    Bar::sync();
     if constexpr (kCand) {
         for (uint32_t c0 = static_cast<uint32_t>(wave) * 64u; c0 < ncand; c0 += FT_THREADS) {
             const uint32_t ci = c0 + static_cast<uint32_t>(lane);
@@ -1201,23 +1265,29 @@ __device__ __forceinline__ void detect_tile_body(
             else keep = finish_walk<T, RI, PITCH, false>(p, H, W, P, Rw, Sp, oy, ox, RW, ylo, yhi, xlo, xhi, res, pu,
                                            pv, seed, frame, d, stdv);
         }
-        // hand long walks to k_walk_slow; if its list is full, walk on here
-        const uint32_t spos = wave_append(defer, slow_counter);
-        if (defer) {
-            if (spos < slow_cap) {
-                fslow[spos] = make_uint2(seed, static_cast<uint32_t>(pu) * static_cast<uint32_t>(W) + pv);
-            } else {
+        // long walks go to k_walk_slow, finished ones to the rows: one atomic
+        // round trip for both lists
+        uint32_t spos, pos;
+        wave_append2(defer, slow_counter, keep, counter, spos, pos);
+        if (defer && spos < slow_cap)
+            fslow[spos] = make_uint2(seed, static_cast<uint32_t>(pu) * static_cast<uint32_t>(W) + pv);
+        if (keep && pos < out_cap) fout[pos] = d;
+        // the slow list is full: those walks go on here
+        const bool over = defer && spos >= slow_cap;
+        if (__ballot(over) != 0ull) {
+            bool keep2 = false;
+            if (over) {
                 WalkRes res = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
                 for (uint32_t it = kSteps; it < p.max_iter; ++it) {
                     res = walk_eval<RI, PITCH>(p, H, W, P, Sp, oy, ox, RW, ylo, yhi, xlo, xhi, pu, pv);
                     if (!walk_move(res, pu, pv, H, W, rint, p.shift_thr)) break;
                 }
-                keep = finish_walk<T, RI, PITCH, false>(p, H, W, P, Rw, Sp, oy, ox, RW, ylo, yhi, xlo, xhi, res, pu,
-                                          pv, seed, frame, d, stdv);
+                keep2 = finish_walk<T, RI, PITCH, false>(p, H, W, P, Rw, Sp, oy, ox, RW, ylo, yhi, xlo, xhi, res, pu,
+                                           pv, seed, frame, d, stdv);
             }
+            const uint32_t pos2 = wave_append(keep2, counter);
+            if (keep2 && pos2 < out_cap) fout[pos2] = d;
         }
-        const uint32_t pos = wave_append(keep, counter);
-        if (keep && pos < out_cap) fout[pos] = d;
     }
 }
 
@@ -1232,7 +1302,6 @@ __global__ __launch_bounds__(FT_THREADS, FT_MIN_WAVES) void k_detect_tile(
     const TileGeom g = kFixed ? gc : g_rt;
     extern __shared__ __align__(16) float lds[];
     __shared__ uint32_t s_cnt[FT_WAVES];
-    __shared__ uint64_t s_masks[FT_WAVES][FT_WROWS];
     const int H = p.H, W = p.W;
     const size_t HW = static_cast<size_t>(H) * W;
     const int ntx = (W + FT_TW - 1) / FT_TW, nty = (H + FT_TH - 1) / FT_TH;
@@ -1245,7 +1314,7 @@ __global__ __launch_bounds__(FT_THREADS, FT_MIN_WAVES) void k_detect_tile(
                                 p, H, W, g, x0, y0, frame0 + fr, out + static_cast<size_t>(fr) * out_cap,
                                 out_cap, &ocount[fr * GT_COUNTER_STRIDE],
                                 slow + static_cast<size_t>(fr) * slow_cap, slow_cap,
-                                &scount[fr * GT_COUNTER_STRIDE], lds, s_cnt, s_masks,
+                                &scount[fr * GT_COUNTER_STRIDE], lds, s_cnt,
                                 frame_thr(p, fr), frame_std(p, fr));
 }
 
@@ -1260,7 +1329,6 @@ __global__ __launch_bounds__(FT_THREADS, FT_MIN_WAVES) void k_detect_cand(
     const TileGeom g = kFixed ? gc : g_rt;
     extern __shared__ __align__(16) float lds[];
     __shared__ uint32_t s_cnt[FT_WAVES];
-    __shared__ uint64_t s_masks[FT_WAVES][FT_WROWS];
     const int H = p.H, W = p.W;
     const size_t HW = static_cast<size_t>(H) * W;
     const int ntx = (W + FT_TW - 1) / FT_TW, nty = (H + FT_TH - 1) / FT_TH;
@@ -1270,7 +1338,7 @@ __global__ __launch_bounds__(FT_THREADS, FT_MIN_WAVES) void k_detect_cand(
     const int x0 = (trest % ntx) * FT_TW, y0 = (trest / ntx) * FT_TH;
     detect_tile_body<T, KT, RI, true>(proc + static_cast<size_t>(fr) * HW, nullptr, p, H, W, g,
                                       x0, y0, 0, nullptr, 0, nullptr, nullptr, cap,
-                                      &ccount[fr * GT_COUNTER_STRIDE], lds, s_cnt, s_masks,
+                                      &ccount[fr * GT_COUNTER_STRIDE], lds, s_cnt,
                                       frame_thr(p, fr), frame_std(p, fr),
                                       cand + static_cast<size_t>(fr) * cap);
 }
@@ -3300,7 +3368,6 @@ __global__ __launch_bounds__(FT_THREADS, FT_MIN_WAVES) void k_detect_tile_batch(
     const TileGeom g = kFixed ? gc : g_rt;
     extern __shared__ __align__(16) float lds[];
     __shared__ uint32_t s_cnt[FT_WAVES];
-    __shared__ uint64_t s_masks[FT_WAVES][FT_WROWS];
     const uint32_t slot = blockIdx.y;
     if (slot >= hdr->nslots) return;
     const GtBatchFrame e = fs[slot];
@@ -3314,7 +3381,7 @@ __global__ __launch_bounds__(FT_THREADS, FT_MIN_WAVES) void k_detect_tile_batch(
                                 e.W, g, x0, y0, e.frame, out + static_cast<size_t>(e.out_idx) * out_cap,
                                 out_cap, &ocount[slot * GT_COUNTER_STRIDE],
                                 slow + static_cast<size_t>(slot) * slow_cap, slow_cap,
-                                &scount[slot * GT_COUNTER_STRIDE], lds, s_cnt, s_masks,
+                                &scount[slot * GT_COUNTER_STRIDE], lds, s_cnt,
                                 p.max_thr, p.image_std);
 }
 
