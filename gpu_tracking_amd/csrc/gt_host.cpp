@@ -271,7 +271,45 @@ bool one_correction_exact_lib(float b) {
     return one_correction_exact_with(b, [](float x, float y, float z) { return std::fma(x, y, z); });
 }
 
+template <class Fma>
+bool split_exact_with(float b, float rh, float rl, Fma fma) {
+    for (uint32_t m = 0; m < (1u << 23); ++m) {
+        const uint32_t u = 0x3F800000u | m;
+        float a;
+        std::memcpy(&a, &u, 4);
+        if (fma(a, rh, a * rl) != a / b) return false;
+    }
+    return true;
+}
+
+__attribute__((target("fma"))) bool split_exact_hw(float b, float rh, float rl) {
+    return split_exact_with(b, rh, rl, [](float x, float y, float z) { return __builtin_fmaf(x, y, z); });
+}
+
 }  // namespace
+
+bool gt_host_split_exact(float b, float *rh, float *rl) {
+    const double r = 1.0 / static_cast<double>(b);
+    *rh = static_cast<float>(r);
+    *rl = static_cast<float>(r - static_cast<double>(*rh));
+    if (!(b >= 1.0f && b < 16777216.0f)) return false;
+    static std::mutex mu;
+    static std::map<uint32_t, bool> cache;
+    uint32_t key;
+    std::memcpy(&key, &b, 4);
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        auto it = cache.find(key);
+        if (it != cache.end()) return it->second;
+    }
+    const float h = *rh, l = *rl;
+    const bool ok = __builtin_cpu_supports("fma")
+                        ? split_exact_hw(b, h, l)
+                        : split_exact_with(b, h, l, [](float x, float y, float z) { return std::fma(x, y, z); });
+    std::lock_guard<std::mutex> lk(mu);
+    cache[key] = ok;
+    return ok;
+}
 
 bool gt_host_one_correction_exact(float b) {
     if (!(b >= 1.0f && b < 16777216.0f)) return false;
@@ -320,6 +358,18 @@ void gt_host_fill_kparams(const gt_derived *d, uint32_t H, uint32_t W, GtKParams
     if (kp->fast_div && !(gt_host_one_correction_exact(kp->gnorm) &&
                           gt_host_one_correction_exact(kp->box_area)))
         kp->fast_div = 0;
+    // two operations per division where both norms pass the whole-binade
+    // check of the split reciprocal (the dividends' a * r_lo stay normal:
+    // nonzero weights >= 2^-46 as above)
+    float gh, gl, bh, bl;
+    if (kp->fast_div && gt_host_split_exact(kp->gnorm, &gh, &gl) &&
+        gt_host_split_exact(kp->box_area, &bh, &bl)) {
+        kp->fast_div = 2;
+        kp->rgnorm = gh;
+        kp->rgnorm_lo = gl;
+        kp->rbox = bh;
+        kp->rbox_lo = bl;
+    }
     kp->truncate = d->truncate;
     kp->k = static_cast<int32_t>(d->dilation);
     kp->a = f2i_sat(static_cast<float>(kp->k) / 2.0f - 0.5f);

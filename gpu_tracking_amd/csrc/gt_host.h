@@ -51,6 +51,10 @@ void gt_host_fill_kparams(const gt_derived *d, uint32_t H, uint32_t W, GtKParams
 // Whether q = RN(a * RN(1/b)), then RN(q + fma(-q, b, a) * RN(1/b)) equals
 // IEEE a / b for every a in [1, 2) (all 2^23 mantissas; cached per divisor).
 bool gt_host_one_correction_exact(float b);
+// Whether RN(a*rh + RN(a*rl)) (rh = RN(1/b), rl = RN(1/b - rh)) equals a / b
+// for every mantissa of a binade (then for every dividend whose a*rl is
+// normal); rh, rl receive the split reciprocal.
+bool gt_host_split_exact(float b, float *rh, float *rl);
 // The LoG scale table (radii, tap radii, tap weights and norms).
 void gt_host_fill_logtable(const gt_derived *d, GtLogTable *t);
 // LoG post-processing of one frame (execute_gpu.rs:699-712): stable sort by

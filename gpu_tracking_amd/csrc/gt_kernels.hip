@@ -19,6 +19,7 @@
 #include <math.h>
 
 #include <algorithm>
+#include <type_traits>
 
 #include "gt_internal.h"
 
@@ -207,6 +208,19 @@ __device__ __forceinline__ float div_const(float a, float b, float rb) {
     const float r = fmaf(-q, b, a);
     return fmaf(r, rb, q);
 }
+// a / b as RN(a * rh + RN(a * rl)), rh + rl = 1/b to about 48 bits: two
+// operations.  The same binade argument applies (a * rl stays normal for
+// these dividends), and the host selects it only for divisors whose whole
+// binade it has checked against IEEE division.
+__device__ __forceinline__ float div_split(float a, float rh, float rl) {
+    return fmaf(a, rh, a * rl);
+}
+// Division by a bandpass norm: the two-operation form when kD2.
+template <bool kD2>
+__device__ __forceinline__ float div_norm(float a, float b, float rb, float rl) {
+    if constexpr (kD2) return div_split(a, rb, rl);
+    else return div_const(a, b, rb);
+}
 
 template <int R>
 struct BpLds {
@@ -220,7 +234,7 @@ struct BpLds {
 
 // One 64 x 32 output tile of one frame (f, out: the frame's input and
 // processed planes, p.H x p.W).
-template <typename T, int R>
+template <typename T, int R, bool kD2>
 __device__ __forceinline__ void bandpass_int_tile(const T *__restrict__ f, float *__restrict__ out,
                                                   const GtKParams &p, int H, int W, int x0,
                                                   int y0, BpLds<R> &L) {
@@ -306,7 +320,7 @@ __device__ __forceinline__ void bandpass_int_tile(const T *__restrict__ f, float
         }
     }
     __syncthreads();
-    const float gnorm = p.gnorm, rgnorm = p.rgnorm;
+    const float gnorm = p.gnorm, rgnorm = p.rgnorm, rgnorm_lo = p.rgnorm_lo;
     // every column the tile's row pass reads lies in the frame
     const bool xin = x0 - R >= 0 && x0 + B2_TW + R <= W;
     // Row pass: 4 adjacent columns of one staged row per item; item
@@ -359,10 +373,10 @@ __device__ __forceinline__ void bandpass_int_tile(const T *__restrict__ f, float
 #pragma unroll
         for (int k = 1; k < 4; ++k) cs[k] = cs[k - 1] - v[k - 1] + v[k - 1 + NT];
         float4 og, oc;
-        og.x = div_const(gs[0], gnorm, rgnorm);
-        og.y = div_const(gs[1], gnorm, rgnorm);
-        og.z = div_const(gs[2], gnorm, rgnorm);
-        og.w = div_const(gs[3], gnorm, rgnorm);
+        og.x = div_norm<kD2>(gs[0], gnorm, rgnorm, rgnorm_lo);
+        og.y = div_norm<kD2>(gs[1], gnorm, rgnorm, rgnorm_lo);
+        og.z = div_norm<kD2>(gs[2], gnorm, rgnorm, rgnorm_lo);
+        og.w = div_norm<kD2>(gs[3], gnorm, rgnorm, rgnorm_lo);
         oc.x = cs[0];
         oc.y = cs[1];
         oc.z = cs[2];
@@ -385,7 +399,7 @@ __device__ __forceinline__ void bandpass_int_tile(const T *__restrict__ f, float
     const int gx = x0 + cx;
     const int gy0 = y0 + r0;           // frame row of output 0; its taps start at gy0 - R
     const bool inner = gy0 - R >= 0 && gy0 + B2_ROWS - 1 + R < H;
-    const float box = p.box_area, rbox = p.rbox;
+    const float box = p.box_area, rbox = p.rbox, rbox_lo = p.rbox_lo;
     float csum = 0.0f;
 #pragma unroll
     for (int t = 0; t < NT; ++t) csum = csum + c[t];
@@ -398,7 +412,7 @@ __device__ __forceinline__ void bandpass_int_tile(const T *__restrict__ f, float
             float gs = 0.0f;
 #pragma unroll
             for (int t = 0; t < NT; ++t) gs = fmaf(p.w[t], g[r + t], gs);
-            float res = div_const(gs, gnorm, rgnorm) - div_const(csum, box, rbox);
+            float res = div_norm<kD2>(gs, gnorm, rgnorm, rgnorm_lo) - div_norm<kD2>(csum, box, rbox, rbox_lo);
             if (p.truncate && res < kTrunc) res = 0.0f;
             *orow = res;
             orow += W;
@@ -420,14 +434,14 @@ __device__ __forceinline__ void bandpass_int_tile(const T *__restrict__ f, float
                 if (py >= 0 && py < H) gs = fmaf(p.w[t], g[r + t], gs);
             }
         }
-        float res = div_const(gs, gnorm, rgnorm) - div_const(csum, box, rbox);
+        float res = div_norm<kD2>(gs, gnorm, rgnorm, rgnorm_lo) - div_norm<kD2>(csum, box, rbox, rbox_lo);
         if (p.truncate && res < kTrunc) res = 0.0f;
         if (rows_in || (gy0 + r < H && gx < W)) *orow = res;
         orow += W;
     }
 }
 
-template <typename T, int R>
+template <typename T, int R, bool kD2>
 __global__ __launch_bounds__(B2_THREADS) void k_bandpass_int(const T *__restrict__ frames,
                                                              float *__restrict__ proc,
                                                              GtKParams p, int nframes,
@@ -446,7 +460,7 @@ __global__ __launch_bounds__(B2_THREADS) void k_bandpass_int(const T *__restrict
     if (tile >= ntx * nty * nframes) return;
     const int fr = tile / (ntx * nty), trest = tile - fr * (ntx * nty);
     const int x0 = (trest % ntx) * B2_TW, y0 = (trest / ntx) * B2_TH;
-    bandpass_int_tile<T, R>(frames + static_cast<size_t>(fr) * HW,
+    bandpass_int_tile<T, R, kD2>(frames + static_cast<size_t>(fr) * HW,
                             proc + static_cast<size_t>(fr) * HW, p, H, W, x0, y0, L);
 }
 
@@ -1630,6 +1644,26 @@ __global__ void k_zero_u32(uint32_t *p, size_t n) {
     if (i < n) p[i] = 0u;
 }
 
+// Calls f(integral_constant R, bool_constant two-operation division) for
+// the integer bandpass's radius 1..7 and division form (fast_div 2: the
+// two-operation quotient, checked for both norms).
+template <class F>
+void with_bandpass_form(const GtKParams &p, F f) {
+    auto by_r = [&](auto d2) {
+        switch (p.R) {
+        case 1: f(std::integral_constant<int, 1>{}, d2); break;
+        case 2: f(std::integral_constant<int, 2>{}, d2); break;
+        case 3: f(std::integral_constant<int, 3>{}, d2); break;
+        case 4: f(std::integral_constant<int, 4>{}, d2); break;
+        case 5: f(std::integral_constant<int, 5>{}, d2); break;
+        case 6: f(std::integral_constant<int, 6>{}, d2); break;
+        default: f(std::integral_constant<int, 7>{}, d2); break;
+        }
+    };
+    if (p.fast_div == 2) by_r(std::true_type{});
+    else by_r(std::false_type{});
+}
+
 template <typename T>
 hipError_t launch_bandpass_t(const T *frames, float *proc, uint32_t nframes, const GtKParams &p,
                              hipStream_t s, const GtZero &z) {
@@ -1639,15 +1673,10 @@ hipError_t launch_bandpass_t(const T *frames, float *proc, uint32_t nframes, con
                                   ((p.H + B2_TH - 1) / B2_TH) * nframes;
             const dim3 g2(xcd_grid(ntiles));
             const int nf = static_cast<int>(nframes);
-            switch (p.R) {
-            case 1: k_bandpass_int<T, 1><<<g2, B2_THREADS, 0, s>>>(frames, proc, p, nf, z); break;
-            case 2: k_bandpass_int<T, 2><<<g2, B2_THREADS, 0, s>>>(frames, proc, p, nf, z); break;
-            case 3: k_bandpass_int<T, 3><<<g2, B2_THREADS, 0, s>>>(frames, proc, p, nf, z); break;
-            case 4: k_bandpass_int<T, 4><<<g2, B2_THREADS, 0, s>>>(frames, proc, p, nf, z); break;
-            case 5: k_bandpass_int<T, 5><<<g2, B2_THREADS, 0, s>>>(frames, proc, p, nf, z); break;
-            case 6: k_bandpass_int<T, 6><<<g2, B2_THREADS, 0, s>>>(frames, proc, p, nf, z); break;
-            default: k_bandpass_int<T, 7><<<g2, B2_THREADS, 0, s>>>(frames, proc, p, nf, z); break;
-            }
+            with_bandpass_form(p, [&](auto r, auto d2) {
+                k_bandpass_int<T, decltype(r)::value, decltype(d2)::value>
+                    <<<g2, B2_THREADS, 0, s>>>(frames, proc, p, nf, z);
+            });
             return hipGetLastError();
         }
     }
@@ -3341,7 +3370,7 @@ __device__ __forceinline__ bool table_tile(uint32_t ntiles, uint32_t &t) {
     return t < ntiles;
 }
 
-template <typename T, int R>
+template <typename T, int R, bool kD2>
 __global__ __launch_bounds__(B2_THREADS) void k_bandpass_int_batch(
     float *__restrict__ proc, GtKParams p, const GtBatchHeader *__restrict__ hdr,
     const GtBatchFrame *__restrict__ fs) {
@@ -3353,7 +3382,7 @@ __global__ __launch_bounds__(B2_THREADS) void k_bandpass_int_batch(
     if (!table_tile(static_cast<uint32_t>(ntx * nty), t)) return;
     const int local = static_cast<int>(t);
     const int x0 = (local % ntx) * B2_TW, y0 = (local / ntx) * B2_TH;
-    bandpass_int_tile<T, R>(reinterpret_cast<const T *>(e.in_ptr), proc + e.proc_off, p, e.H, e.W,
+    bandpass_int_tile<T, R, kD2>(reinterpret_cast<const T *>(e.in_ptr), proc + e.proc_off, p, e.H, e.W,
                             x0, y0, L);
 }
 
@@ -3510,15 +3539,10 @@ hipError_t launch_batch_t(const GtBatchLaunch *launches, uint32_t *cur, const Gt
     if (e != hipSuccess) return e;
     // x: tiles of the call's largest frame (multiple of 8); y: frames of a launch
     const dim3 gbp(sch.bp_blocks, sch.launch_slots);
-    switch (p.R) {
-    case 1: k_bandpass_int_batch<T, 1><<<gbp, B2_THREADS, 0, s>>>(proc, p, hdr, fs); break;
-    case 2: k_bandpass_int_batch<T, 2><<<gbp, B2_THREADS, 0, s>>>(proc, p, hdr, fs); break;
-    case 3: k_bandpass_int_batch<T, 3><<<gbp, B2_THREADS, 0, s>>>(proc, p, hdr, fs); break;
-    case 4: k_bandpass_int_batch<T, 4><<<gbp, B2_THREADS, 0, s>>>(proc, p, hdr, fs); break;
-    case 5: k_bandpass_int_batch<T, 5><<<gbp, B2_THREADS, 0, s>>>(proc, p, hdr, fs); break;
-    case 6: k_bandpass_int_batch<T, 6><<<gbp, B2_THREADS, 0, s>>>(proc, p, hdr, fs); break;
-    default: k_bandpass_int_batch<T, 7><<<gbp, B2_THREADS, 0, s>>>(proc, p, hdr, fs); break;
-    }
+    with_bandpass_form(p, [&](auto r, auto d2) {
+        k_bandpass_int_batch<T, decltype(r)::value, decltype(d2)::value>
+            <<<gbp, B2_THREADS, 0, s>>>(proc, p, hdr, fs);
+    });
     e = hipGetLastError();
     if (e != hipSuccess) return e;
     GtKParams kw = p;

@@ -23,7 +23,12 @@ struct GtKParams {
     float box_area;               // f32(s*s)
     float rgnorm;                 // RN(1 / gnorm)
     float rbox;                   // RN(1 / box_area)
-    int32_t fast_div;             // k_bandpass_int may divide by the norms with one correction
+    // k_bandpass_int's division by the norms: 0 IEEE division (general
+    // kernel), 1 reciprocal with one fma correction, 2 RN(a*r + RN(a*r_lo))
+    // with r + r_lo = 1/norm to ~48 bits (rgnorm, rbox hold r then)
+    int32_t fast_div;
+    float rgnorm_lo;              // fast_div 2: RN(1/gnorm - rgnorm)
+    float rbox_lo;                // fast_div 2: RN(1/box_area - rbox)
     int32_t truncate;             // truncate_preprocessed
     int32_t k;                    // dilation side
     int32_t a;                    // window start offset i32(k/2 - 0.5)

@@ -1,7 +1,8 @@
-/* Checks the division sequence the integer-input bandpass kernel uses,
- *   q = a*rb; r = fma(-q, b, a); q = fma(r, rb, q)
+/* Checks the division sequences the integer-input bandpass kernel uses,
+ *   one correction:    q = a*rb; r = fma(-q, b, a); q = fma(r, rb, q)
+ *   split reciprocal:  q = fma(a, rh, RN(a*rl)), rh = RN(1/b), rl = RN(1/b - rh)
  * with rb = RN(1/b), against IEEE a / b.  Usage:
- *   div_check <divisor> <n_random> <int_limit> <binade_lo> <binade_hi>
+ *   div_check <divisor> <n_random> <int_limit> <binade_lo> <binade_hi> [split]
  * tests every integer a in [0, int_limit], every float in the binades
  * [2^binade_lo, 2^binade_hi), and n_random random floats in [2^-90, 2^25).
  * Prints the number of mismatches (0 expected) and exits non-zero on any. */
@@ -17,6 +18,13 @@ static float div_const(float a, float b, float rb) {
     return fmaf(r, rb, q);
 }
 
+static int use_split;
+static float rh, rl;
+
+static float quot(float a, float b, float rb) {
+    return use_split ? fmaf(a, rh, a * rl) : div_const(a, b, rb);
+}
+
 static uint32_t bits(float x) {
     uint32_t u;
     memcpy(&u, &x, 4);
@@ -30,10 +38,13 @@ int main(int argc, char **argv) {
     const long ilim = atol(argv[3]);
     const int blo = atoi(argv[4]), bhi = atoi(argv[5]);
     const float rb = 1.0f / b;
+    use_split = argc > 6 && strcmp(argv[6], "split") == 0;
+    rh = (float)(1.0 / (double)b);
+    rl = (float)(1.0 / (double)b - (double)rh);
     long bad = 0, checked = 0;
     for (long i = 0; i <= ilim; i++) {
         const float a = (float)i;
-        bad += bits(div_const(a, b, rb)) != bits(a / b);
+        bad += bits(quot(a, b, rb)) != bits(a / b);
         checked++;
     }
     for (int e = blo; e < bhi; e++) {
@@ -42,7 +53,7 @@ int main(int argc, char **argv) {
             float a;
             uint32_t u = bits(base) + m;
             memcpy(&a, &u, 4);
-            bad += bits(div_const(a, b, rb)) != bits(a / b);
+            bad += bits(quot(a, b, rb)) != bits(a / b);
             checked++;
         }
     }
@@ -55,7 +66,7 @@ int main(int argc, char **argv) {
         float a;
         uint32_t u = bits(base) + m;
         memcpy(&a, &u, 4);
-        bad += bits(div_const(a, b, rb)) != bits(a / b);
+        bad += bits(quot(a, b, rb)) != bits(a / b);
         checked++;
     }
     printf("%ld %ld\n", bad, checked);
