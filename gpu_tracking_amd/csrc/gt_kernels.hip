@@ -1032,6 +1032,31 @@ __device__ __forceinline__ void detect_tile_body(
         for (int q = 0; q < QP; ++q)
             if (active && rq + q * QROWS < gc.rows)
                 *reinterpret_cast<float4 *>(Sp + (rq + q * QROWS) * gc.pitch + 4 * cq) = v4[q];
+    } else if (kFixed && gc.hl % 4 == 0 && (W & 3) == 0 &&
+               (reinterpret_cast<uintptr_t>(P) & 15u) == 0) {
+        // border tile whose float4 groups start on multiples of 4 frame
+        // columns: each group is wholly inside or outside the frame; groups
+        // load from clamped coordinates (always in the frame) and the
+        // outside ones become -inf
+        const int rq = tid / Q4, cq = tid - rq * Q4;
+        const bool active = tid < QROWS * Q4;
+        const int gx = ox + 4 * cq;
+        const bool col_in = gx >= 0 && gx < W;
+        const int gxc = min(max(gx, 0), W - 4);
+        float4 v4[QP];
+#pragma unroll
+        for (int q = 0; q < QP; ++q) {
+            const int gy = oy + min(rq + q * QROWS, gc.rows - 1);
+            const int gyc = min(max(gy, 0), H - 1);
+            const float4 t = *reinterpret_cast<const float4 *>(
+                P + (static_cast<size_t>(gyc) * static_cast<size_t>(W) + static_cast<size_t>(gxc)));
+            const bool in = col_in && gy == gyc;
+            v4[q] = in ? t : make_float4(-INFINITY, -INFINITY, -INFINITY, -INFINITY);
+        }
+#pragma unroll
+        for (int q = 0; q < QP; ++q)
+            if (active && rq + q * QROWS < gc.rows)
+                *reinterpret_cast<float4 *>(Sp + (rq + q * QROWS) * gc.pitch + 4 * cq) = v4[q];
     } else if constexpr (kFixed) {
         constexpr int CN = gc.rows * gc.pitch;
         constexpr int NL = (CN + FT_THREADS - 1) / FT_THREADS;
@@ -1205,13 +1230,31 @@ __device__ __forceinline__ void detect_compute(
 #pragma unroll
         for (int r = 0; r < FT_WROWS; ++r) cv[r] = Sp[(g.ht + r0 + r) * RW + g.hl + lane];
         const bool col_ok = gx < W && gx >= margin && gx < W - margin;
+        if constexpr (kInt) {
+            // finite values: c passes iff c >= max(thr, window maximum); the
+            // rows inside the margin are a wave-uniform bit mask, the columns
+            // a ballot applied to each row's mask
+            const unsigned long long cols = __ballot(col_ok);
+            const int ylo = margin - (y0 + r0), yhi = min(H, H - margin) - (y0 + r0);
+            const int rlo = min(max(ylo, 0), FT_WROWS), rhi = min(max(yhi, 0), FT_WROWS);
+            const uint32_t rows = rhi > rlo ? (1u << rhi) - (1u << rlo) : 0u;
+            const float thr_u = __builtin_bit_cast(
+                float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, thr)));
 #pragma unroll
-        for (int r = 0; r < FT_WROWS; ++r) {
-            const int gy = y0 + r0 + r;
-            const bool row_ok = gy < H && gy >= margin && gy < H - margin;
-            const bool flag = row_ok && col_ok && !(cv[r] < thr) && !(wmax[r] > cv[r]);
-            mks[r] = __ballot(flag);
-            nwave += static_cast<uint32_t>(__popcll(mks[r]));
+            for (int r = 0; r < FT_WROWS; ++r) {
+                const float m = __builtin_elementwise_maximum(wmax[r], thr_u);
+                mks[r] = ((rows >> r) & 1u) ? (__ballot(!(m > cv[r])) & cols) : 0ull;
+                nwave += static_cast<uint32_t>(__popcll(mks[r]));
+            }
+        } else {
+#pragma unroll
+            for (int r = 0; r < FT_WROWS; ++r) {
+                const int gy = y0 + r0 + r;
+                const bool row_ok = gy < H && gy >= margin && gy < H - margin;
+                const bool flag = row_ok && col_ok && !(cv[r] < thr) && !(wmax[r] > cv[r]);
+                mks[r] = __ballot(flag);
+                nwave += static_cast<uint32_t>(__popcll(mks[r]));
+            }
         }
     }
     if (lane == 0) s_cnt[wave] = nwave;
@@ -1226,9 +1269,12 @@ __device__ __forceinline__ void detect_compute(
 #pragma unroll
     for (int r = 0; r < FT_WROWS; ++r) {
         const uint64_t mk = mks[r];
-        if ((mk >> lane) & 1ull)
-            s_cand[off + __popcll(mk & ((1ull << lane) - 1ull))] =
-                static_cast<uint16_t>((r0 + r) * FT_TW + lane);
+        // the lane's rank among the row's candidates (mbcnt: set bits of mk
+        // below the lane); lanes not in mk write nowhere
+        const uint32_t below = __builtin_amdgcn_mbcnt_hi(
+            static_cast<uint32_t>(mk >> 32), __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(mk), 0u));
+        const bool mine = (static_cast<uint32_t>(mk >> (lane & 32)) >> (lane & 31)) & 1u;
+        if (mine) s_cand[off + below] = static_cast<uint16_t>((r0 + r) * FT_TW + lane);
         off += static_cast<uint32_t>(__popcll(mk));
     }
     Bar::sync();
