@@ -52,6 +52,17 @@ __device__ __forceinline__ float maxf_ref(float a, float b) { return b > a ? b :
 // in the L2 its neighbour filled.  The launch has 8 * per workgroups; ids
 // that map past the last tile return at once.
 constexpr int kXcds = 8;
+// n / d and n % d for n < 2^22 and d >= 1: a float reciprocal estimate (off
+// by at most one for quotients below 2^22) and one correction.
+__device__ __forceinline__ uint32_t udiv22(uint32_t n, uint32_t d, uint32_t &rem) {
+    uint32_t q = static_cast<uint32_t>(static_cast<float>(n) *
+                                       __builtin_amdgcn_rcpf(static_cast<float>(d)));
+    int32_t r = static_cast<int32_t>(n - q * d);
+    if (r < 0) { --q; r += static_cast<int32_t>(d); }
+    else if (r >= static_cast<int32_t>(d)) { ++q; r -= static_cast<int32_t>(d); }
+    rem = static_cast<uint32_t>(r);
+    return q;
+}
 __device__ __forceinline__ int xcd_tile(int ntiles) {
     const int per = (ntiles + kXcds - 1) / kXcds;
     const int w = blockIdx.x;
@@ -821,7 +832,10 @@ namespace {
 // Centroid of the disk around (u, v) when the whole disk is inside the frame:
 // no per-pixel bounds test, every offset a compile-time constant.  Same
 // operations in the same order as get_center, so the same bits.
-template <int RI, class B>
+// kFin: every sample is finite.  Then the zero-offset taps fma(d, 0, acc) are
+// skipped: d * 0 is a zero and acc is never -0 (it starts at +0, and an exact
+// zero sum rounds to +0), so they leave acc unchanged.
+template <int RI, bool kFin = false, class B>
 __device__ __forceinline__ WalkRes get_center_inner(const B &base) {
     WalkRes res = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
     constexpr float r2 = static_cast<float>(RI) * static_cast<float>(RI);
@@ -832,8 +846,8 @@ __device__ __forceinline__ WalkRes get_center_inner(const B &base) {
             const float x = static_cast<float>(i), y = static_cast<float>(j);
             if (x * x + y * y > r2) continue;
             const float d = base(i, j);
-            res.x = fmaf(d, x, res.x);
-            res.y = fmaf(d, y, res.y);
+            if (!(kFin && i == 0)) res.x = fmaf(d, x, res.x);
+            if (!(kFin && j == 0)) res.y = fmaf(d, y, res.y);
             res.mass = res.mass + d;
             res.count = res.count + 1.0f;
             // res.max starts at +0 and is never NaN, so fmaxf agrees with the
@@ -861,7 +875,7 @@ struct GlobalRow {
 
 // One walk evaluation at (pu, pv) from the best available source; PITCH is
 // the staged region's compile-time row pitch (0: none or not known).
-template <int RI, int PITCH>
+template <int RI, int PITCH, bool kFin = false>
 __device__ __forceinline__ WalkRes walk_eval(const GtKParams &p, int H, int W,
                                              const float *__restrict__ P, const float *Sp, int oy,
                                              int ox, int RW, int ylo, int yhi, int xlo, int xhi,
@@ -871,9 +885,9 @@ __device__ __forceinline__ WalkRes walk_eval(const GtKParams &p, int H, int W,
         if (in_frame) {
             if constexpr (PITCH > 0) {
                 if (pu >= ylo && pu <= yhi && pv >= xlo && pv <= xhi)
-                    return get_center_inner<RI>(LdsRow<PITCH>{Sp + (pu - oy) * PITCH + (pv - ox)});
+                    return get_center_inner<RI, kFin>(LdsRow<PITCH>{Sp + (pu - oy) * PITCH + (pv - ox)});
             }
-            return get_center_inner<RI>(GlobalRow{P + static_cast<size_t>(pu) * W + pv, W});
+            return get_center_inner<RI, kFin>(GlobalRow{P + static_cast<size_t>(pu) * W + pv, W});
         }
     }
     if (pu >= ylo && pu <= yhi && pv >= xlo && pv <= xhi)
@@ -1315,7 +1329,7 @@ __device__ __forceinline__ void detect_compute(
             bool moving = true;
             uint32_t it = 0;
             for (; it < p.max_iter && it < kSteps; ++it) {
-                res = walk_eval<RI, PITCH>(p, H, W, P, Sp, oy, ox, RW, ylo, yhi, xlo, xhi, pu, pv);
+                res = walk_eval<RI, PITCH, kInt>(p, H, W, P, Sp, oy, ox, RW, ylo, yhi, xlo, xhi, pu, pv);
                 if (!walk_move(res, pu, pv, H, W, rint, p.shift_thr)) {
                     moving = false;
                     break;
@@ -1339,7 +1353,7 @@ __device__ __forceinline__ void detect_compute(
             if (over) {
                 WalkRes res = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
                 for (uint32_t it = kSteps; it < p.max_iter; ++it) {
-                    res = walk_eval<RI, PITCH>(p, H, W, P, Sp, oy, ox, RW, ylo, yhi, xlo, xhi, pu, pv);
+                    res = walk_eval<RI, PITCH, kInt>(p, H, W, P, Sp, oy, ox, RW, ylo, yhi, xlo, xhi, pu, pv);
                     if (!walk_move(res, pu, pv, H, W, rint, p.shift_thr)) break;
                 }
                 keep2 = finish_walk<T, RI, PITCH, false>(p, H, W, P, Rw, Sp, oy, ox, RW, ylo, yhi, xlo, xhi, res, pu,
@@ -1367,8 +1381,12 @@ __global__ __launch_bounds__(FT_THREADS, FT_MIN_WAVES) void k_detect_tile(
     const int ntx = (W + FT_TW - 1) / FT_TW, nty = (H + FT_TH - 1) / FT_TH;
     const int tile = xcd_tile(ntx * nty * nframes);
     if (tile >= ntx * nty * nframes) return;
-    const int fr = tile / (ntx * nty), trest = tile - fr * (ntx * nty);
-    const int x0 = (trest % ntx) * FT_TW, y0 = (trest / ntx) * FT_TH;
+    // tile -> (frame, tile row, tile column); the launch keeps ntiles < 2^22
+    uint32_t trest, tx;
+    const int fr = static_cast<int>(udiv22(static_cast<uint32_t>(tile),
+                                           static_cast<uint32_t>(ntx * nty), trest));
+    const int ty = static_cast<int>(udiv22(trest, static_cast<uint32_t>(ntx), tx));
+    const int x0 = static_cast<int>(tx) * FT_TW, y0 = ty * FT_TH;
     detect_tile_body<T, KT, RI, false, kSteps>(
                                 proc + static_cast<size_t>(fr) * HW, raw + static_cast<size_t>(fr) * HW,
                                 p, H, W, g, x0, y0, frame0 + fr, out + static_cast<size_t>(fr) * out_cap,
@@ -1438,7 +1456,7 @@ __global__ __launch_bounds__(WK_THREADS) void k_walk_slow(const float *__restric
             int pv = static_cast<int>(e.y - static_cast<uint32_t>(pu) * static_cast<uint32_t>(W));
             WalkRes res = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
             for (uint32_t it = FT_WALK_STEPS; it < p.max_iter; ++it) {
-                res = walk_eval<RI, 0>(p, H, W, P, nullptr, 0, 0, 0, 1, 0, 1, 0, pu, pv);
+                res = walk_eval<RI, 0, (sizeof(T) <= 2)>(p, H, W, P, nullptr, 0, 0, 0, 1, 0, 1, 0, pu, pv);
                 if (!walk_move(res, pu, pv, H, W, rint, p.shift_thr)) break;
             }
             keep = finish_walk<T, RI, 0, false>(p, H, W, P, Rw, nullptr, 0, 0, 0, 1, 0, 1, 0, res, pu, pv,
@@ -1813,6 +1831,25 @@ hipError_t launch_detect_tile_t(const float *proc, const T *raw, uint32_t nframe
                                 const GtKParams &p, gt_detection *out, uint32_t out_cap,
                                 uint32_t *ocount, uint2 *slow, uint32_t slow_cap,
                                 uint32_t *scount, uint32_t frame0, uint32_t steps, hipStream_t s) {
+    // the kernel decodes tile indices below 2^22: larger launches are split
+    // into frame groups
+    const size_t tpf = static_cast<size_t>((p.W + FT_TW - 1) / FT_TW) * ((p.H + FT_TH - 1) / FT_TH);
+    const size_t fmax = tpf ? std::max<size_t>(1, ((size_t(1) << 22) - 1) / tpf) : nframes;
+    if (nframes > fmax) {
+        const size_t HW = static_cast<size_t>(p.H) * p.W;
+        for (uint32_t f = 0; f < nframes; f += static_cast<uint32_t>(fmax)) {
+            const uint32_t nf = std::min<uint32_t>(static_cast<uint32_t>(fmax), nframes - f);
+            GtKParams pf = p;
+            if (pf.fstd) pf.fstd += f;   // the group's per-frame noise estimates
+            const hipError_t e = launch_detect_tile_t(
+                proc + f * HW, raw + f * HW, nf, pf, out + static_cast<size_t>(f) * out_cap, out_cap,
+                ocount + static_cast<size_t>(f) * GT_COUNTER_STRIDE,
+                slow + static_cast<size_t>(f) * slow_cap, slow_cap,
+                scount + static_cast<size_t>(f) * GT_COUNTER_STRIDE, frame0 + f, steps, s);
+            if (e != hipSuccess) return e;
+        }
+        return hipSuccess;
+    }
     const TileGeom g = tile_geom(p.k, p.rint);
     const size_t ntiles = static_cast<size_t>((p.W + FT_TW - 1) / FT_TW) *
                           ((p.H + FT_TH - 1) / FT_TH) * nframes;
@@ -3491,7 +3528,7 @@ __global__ __launch_bounds__(WK_THREADS) void k_walk_slow_batch(
                 int pv = static_cast<int>(en.y - static_cast<uint32_t>(pu) * static_cast<uint32_t>(W));
                 WalkRes res = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
                 for (uint32_t itr = FT_WALK_STEPS; itr < p.max_iter; ++itr) {
-                    res = walk_eval<RI, 0>(p, H, W, P, nullptr, 0, 0, 0, 1, 0, 1, 0, pu, pv);
+                    res = walk_eval<RI, 0, (sizeof(T) <= 2)>(p, H, W, P, nullptr, 0, 0, 0, 1, 0, 1, 0, pu, pv);
                     if (!walk_move(res, pu, pv, H, W, rint, p.shift_thr)) break;
                 }
                 keep = finish_walk<T, RI, 0, false>(p, H, W, P, Rw, nullptr, 0, 0, 0, 1, 0, 1, 0, res, pu,
