@@ -2,9 +2,12 @@
 //
 // The reference's GpuState serves one frame size (src/execute_gpu.rs:1074-1080);
 // a batch context instead takes any mix of frame sizes (same detection
-// parameters).  A call's frames are split into launches (a pixel budget and a
-// frame-count limit); every launch is one kernel per stage over the launch's
-// frame table (grid y = frame, grid x = tile of that frame).  All launch
+// parameters).  A call's frames, sorted by size, are split into launches (a
+// pixel budget, a frame-count limit and at most GT_BATCH_MAX_SEG runs of
+// equal size); every launch is one kernel per stage: the bandpass and detect
+// kernels take one queue of all the launch's tiles (frame by frame through
+// the size runs), the per-frame stages take grid y = frame.  A call whose
+// pixels fit the budget is one launch per stage.  All launch
 // tables of a call go to the device in one copy; the stage sequence, framed by
 // a kernel that selects the next launch's table on the device and one that
 // advances a device launch counter, is captured once into a HIP graph (again
@@ -61,12 +64,11 @@ struct BatchLane {
         int32_t dtype;
         GtBatchFrame *all;
         GtBatchLaunch *launches;
-        uint32_t bp_blocks, dt_blocks, walk_blocks, launch_slots;
+        uint32_t tile_blocks, walk_blocks, launch_slots;
         bool operator==(const GraphKey &o) const {
             return out == o.out && cap == o.cap && counts == o.counts && dtype == o.dtype &&
-                   all == o.all && launches == o.launches && bp_blocks == o.bp_blocks &&
-                   dt_blocks == o.dt_blocks && walk_blocks == o.walk_blocks &&
-                   launch_slots == o.launch_slots;
+                   all == o.all && launches == o.launches && tile_blocks == o.tile_blocks &&
+                   walk_blocks == o.walk_blocks && launch_slots == o.launch_slots;
         }
     };
     std::vector<std::pair<GraphKey, hipGraphExec_t>> graphs;
@@ -142,17 +144,22 @@ gt_status alloc_lane(gt_batch *b, uint32_t l) {
     return GT_OK;
 }
 
-// Splits frames[order[0..n)] into launches: consecutive frames while the
-// frame limit and the pixel budget last.  Returns the end index of the launch
-// starting at i.
+// Splits frames[order[0..n)] (sorted by size) into launches: consecutive
+// frames while the frame limit, the pixel budget and the segment limit (runs
+// of equal frame size) last.  Returns the end index of the launch starting at
+// i.
 uint32_t launch_end(const gt_batch *b, const gt_frame_desc *frames, const uint32_t *order,
                     uint32_t n, uint32_t i, uint64_t launch_px) {
     uint64_t px = 0;
-    uint32_t j = i;
+    uint32_t j = i, nseg = 0;
     while (j < n && j - i < b->max_slots) {
         const gt_frame_desc &fj = frames[order[j]];
         const uint64_t hw = static_cast<uint64_t>(fj.nrows) * fj.ncols;
         if (j > i && px + hw > launch_px) break;
+        const bool new_seg = j == i || fj.nrows != frames[order[j - 1]].nrows ||
+                             fj.ncols != frames[order[j - 1]].ncols;
+        if (new_seg && nseg == GT_BATCH_MAX_SEG) break;
+        nseg += new_seg ? 1u : 0u;
         px += hw;
         ++j;
     }
@@ -191,7 +198,25 @@ gt_status run_batch(gt_batch *b, const gt_frame_desc *frames, uint32_t n, int32_
     std::vector<GtBatchLaunch> launches;
     for (uint32_t i = 0; i < n;) {
         const uint32_t j = launch_end(b, frames, order.data(), n, i, launch_px);
-        launches.push_back(GtBatchLaunch{i, j - i});
+        GtBatchLaunch L = {};
+        L.first = i;
+        L.nslots = j - i;
+        // the launch's tile queue: one segment per run of equal frame size
+        for (uint32_t q = i; q < j; ++q) {
+            const gt_frame_desc &fq = frames[order[q]];
+            if (q == i || fq.nrows != frames[order[q - 1]].nrows ||
+                fq.ncols != frames[order[q - 1]].ncols) {
+                uint32_t tb, td;
+                gt_batch_tiles(static_cast<int>(fq.nrows), static_cast<int>(fq.ncols), &tb, &td);
+                GtBatchSeg &sg = L.seg[L.nseg++];
+                sg.first_tile = L.ntiles;
+                sg.first_slot = q - i;
+                sg.tpf = td;
+                sg.ntx = static_cast<uint32_t>((fq.ncols + 63) / 64);
+            }
+            L.ntiles += L.seg[L.nseg - 1].tpf;
+        }
+        launches.push_back(L);
         i = j;
     }
     const size_t nl = launches.size();
@@ -229,13 +254,12 @@ gt_status run_batch(gt_batch *b, const gt_frame_desc *frames, uint32_t n, int32_
     GtBatchLaunch *hl = reinterpret_cast<GtBatchLaunch *>(b->h_stage + sizeof(GtBatchFrame) * n);
     uint32_t *zero = reinterpret_cast<uint32_t *>(b->h_stage + sizeof(GtBatchFrame) * n +
                                                   sizeof(GtBatchLaunch) * nl);
-    // each launch's grids: x = tiles of its largest frame (multiple of 8),
-    // y = its frames
-    std::vector<GtBatchSchedule> lsch(nl);
+    // the frame tables; one grid shape for every launch of the call: the tile
+    // queue of its largest launch (multiple of 8) and its most frames
+    GtBatchSchedule sch_all = {8u, walk_blocks, 1u};
     for (size_t k = 0; k < nl; ++k) {
         const GtBatchLaunch &L = launches[k];
         uint64_t off = 0;
-        uint32_t maxbp = 1, maxdt = 1;
         for (uint32_t q = L.first; q < L.first + L.nslots; ++q) {
             const uint32_t f = order[q];
             GtBatchFrame &x = e[q];
@@ -246,15 +270,9 @@ gt_status run_batch(gt_batch *b, const gt_frame_desc *frames, uint32_t n, int32_
             x.out_idx = f;
             x.frame = frames[f].frame;
             off += static_cast<uint64_t>(x.H) * x.W;
-            uint32_t tb, td;
-            gt_batch_tiles(x.H, x.W, &tb, &td);
-            maxbp = std::max(maxbp, tb);
-            maxdt = std::max(maxdt, td);
         }
-        lsch[k].bp_blocks = (maxbp + 7) / 8 * 8;
-        lsch[k].dt_blocks = (maxdt + 7) / 8 * 8;
-        lsch[k].walk_blocks = walk_blocks;
-        lsch[k].launch_slots = L.nslots;
+        sch_all.tile_blocks = std::max(sch_all.tile_blocks, (L.ntiles + 7) / 8 * 8);
+        sch_all.launch_slots = std::max(sch_all.launch_slots, L.nslots);
     }
     std::vector<size_t> fill(lane_off.begin(), lane_off.end() - 1);
     for (size_t k = 0; k < nl; ++k) hl[fill[k % nlanes]++] = launches[k];
@@ -274,7 +292,7 @@ gt_status run_batch(gt_batch *b, const gt_frame_desc *frames, uint32_t n, int32_
         // graphs of a previous call with other arguments are dropped (after
         // the lane's earlier work, which may still replay them)
         const BatchLane::GraphKey probe{d_out, cap, d_counts, dtype, b->d_all, L.d_launches,
-                                        0, 0, 0, 0};
+                                        0, 0, 0};
         bool stale = false;
         for (auto &kg : L.graphs)
             stale = stale || kg.first.out != probe.out || kg.first.cap != probe.cap ||
@@ -287,9 +305,8 @@ gt_status run_batch(gt_batch *b, const gt_frame_desc *frames, uint32_t n, int32_
             L.graphs.clear();
         }
         for (size_t q = lane_off[l]; q < lane_off[l + 1]; ++q) {
-            // the launch this lane runs q-th: launch k with k % nlanes == l
-            const size_t k = (q - lane_off[l]) * nlanes + l;
-            const GtBatchSchedule &sch = lsch[k];
+            // every launch of the call replays the same grid shape
+            const GtBatchSchedule &sch = sch_all;
             auto enqueue = [&]() {
                 return gt_launch_batch(L.d_launches, L.d_cur, b->d_all, dtype, b->kp, L.d_hdr,
                                        L.d_fs, b->max_slots, L.d_proc, d_out, cap, L.d_ocount,
@@ -297,10 +314,9 @@ gt_status run_batch(gt_batch *b, const gt_frame_desc *frames, uint32_t n, int32_
             };
             hipGraphExec_t gexec = nullptr;
             if (graph) {
-                const BatchLane::GraphKey key{d_out,           cap,          d_counts,
-                                              dtype,           b->d_all,     L.d_launches,
-                                              sch.bp_blocks,   sch.dt_blocks, sch.walk_blocks,
-                                              sch.launch_slots};
+                const BatchLane::GraphKey key{d_out,           cap,           d_counts,
+                                              dtype,           b->d_all,      L.d_launches,
+                                              sch.tile_blocks, sch.walk_blocks, sch.launch_slots};
                 for (auto &kg : L.graphs)
                     if (kg.first == key) gexec = kg.second;
                 if (!gexec) {

@@ -101,10 +101,9 @@ hipError_t gt_launch_copy_counts_checked(const uint32_t *ocount, const uint32_t 
 // Batched launches over frames of different sizes (u8/u16 frames, R <= 7):
 // grid y = frame slot of the launch, grid x = tile of that frame.
 struct GtBatchSchedule {
-    uint32_t bp_blocks;     // bandpass tiles of the largest frame (multiple of 8)
-    uint32_t dt_blocks;     // detect tiles of the largest frame (multiple of 8)
+    uint32_t tile_blocks;   // queue tiles of the call's largest launch (multiple of 8)
     uint32_t walk_blocks;   // blocks per frame for slow walks and characterize
-    uint32_t launch_slots;  // frames per launch at most (grid y of every stage)
+    uint32_t launch_slots;  // frames per launch at most (grid y of the per-frame stages)
 };
 // One launch: k_select_launch copies launch *cur's entries of `all` into
 // (hdr, fs), the stages run, k_next_launch advances *cur.

@@ -3445,12 +3445,38 @@ namespace {
 // contiguous range of the frame's tiles; blocks past the frame's tiles return
 // at once.  The grid is sized for the largest frame of a call, so one captured
 // graph serves every launch.
-__device__ __forceinline__ bool table_tile(uint32_t ntiles, uint32_t &t) {
-    const uint32_t per = (ntiles + kXcds - 1) / kXcds;
+// The launch queue's tile of this workgroup (XCD-contiguous ranges of the
+// queue, as xcd_tile): its frame slot and the tile's index in that frame.
+// False for workgroups past the launch's tiles.
+__device__ __forceinline__ bool queue_tile(const GtBatchHeader *__restrict__ hdr, uint32_t &slot,
+                                           uint32_t &local, uint32_t &ntx) {
+    const uint32_t ntiles = hdr->ntiles;
+    const uint32_t per = (gridDim.x / kXcds);
     const uint32_t x = blockIdx.x % kXcds, j = blockIdx.x / kXcds;
-    if (j >= per) return false;
-    t = x * per + j;
-    return t < ntiles;
+    const uint32_t per_used = (ntiles + kXcds - 1) / kXcds;
+    // the grid holds every launch's queue (the host sizes it for the largest)
+    if (j >= per_used || per_used > per) return false;
+    const uint32_t t = x * per_used + j;
+    if (t >= ntiles) return false;
+    // every segment is read at once (one round of scalar loads); unused
+    // segments start past the launch's tiles
+    const GtBatchHeader h = *hdr;
+    uint32_t first_tile = h.seg[0].first_tile, first_slot = h.seg[0].first_slot;
+    uint32_t tpf = h.seg[0].tpf, nx = h.seg[0].ntx;
+#pragma unroll
+    for (int q = 1; q < GT_BATCH_MAX_SEG; ++q) {
+        if (h.seg[q].first_tile <= t) {
+            first_tile = h.seg[q].first_tile;
+            first_slot = h.seg[q].first_slot;
+            tpf = h.seg[q].tpf;
+            nx = h.seg[q].ntx;
+        }
+    }
+    uint32_t rem;
+    slot = first_slot + udiv22(t - first_tile, tpf, rem);
+    local = rem;
+    ntx = nx;
+    return true;
 }
 
 template <typename T, int R, bool kD2>
@@ -3458,13 +3484,12 @@ __global__ __launch_bounds__(B2_THREADS) void k_bandpass_int_batch(
     float *__restrict__ proc, GtKParams p, const GtBatchHeader *__restrict__ hdr,
     const GtBatchFrame *__restrict__ fs) {
     __shared__ __align__(16) BpLds<R> L;
-    if (blockIdx.y >= hdr->nslots) return;
-    const GtBatchFrame e = fs[blockIdx.y];
-    const int ntx = (e.W + B2_TW - 1) / B2_TW, nty = (e.H + B2_TH - 1) / B2_TH;
-    uint32_t t;
-    if (!table_tile(static_cast<uint32_t>(ntx * nty), t)) return;
-    const int local = static_cast<int>(t);
-    const int x0 = (local % ntx) * B2_TW, y0 = (local / ntx) * B2_TH;
+    uint32_t slot, local, ntx;
+    if (!queue_tile(hdr, slot, local, ntx)) return;
+    const GtBatchFrame e = fs[slot];
+    uint32_t tx;
+    const int ty = static_cast<int>(udiv22(local, ntx, tx));
+    const int x0 = static_cast<int>(tx) * B2_TW, y0 = ty * B2_TH;
     bandpass_int_tile<T, R, kD2>(reinterpret_cast<const T *>(e.in_ptr), proc + e.proc_off, p, e.H, e.W,
                             x0, y0, L);
 }
@@ -3480,14 +3505,12 @@ __global__ __launch_bounds__(FT_THREADS, FT_MIN_WAVES) void k_detect_tile_batch(
     const TileGeom g = kFixed ? gc : g_rt;
     extern __shared__ __align__(16) float lds[];
     __shared__ uint32_t s_cnt[FT_WAVES];
-    const uint32_t slot = blockIdx.y;
-    if (slot >= hdr->nslots) return;
+    uint32_t slot, local, ntx;
+    if (!queue_tile(hdr, slot, local, ntx)) return;
     const GtBatchFrame e = fs[slot];
-    const int ntx = (e.W + FT_TW - 1) / FT_TW, nty = (e.H + FT_TH - 1) / FT_TH;
-    uint32_t t;
-    if (!table_tile(static_cast<uint32_t>(ntx * nty), t)) return;
-    const int local = static_cast<int>(t);
-    const int x0 = (local % ntx) * FT_TW, y0 = (local / ntx) * FT_TH;
+    uint32_t tx;
+    const int ty = static_cast<int>(udiv22(local, ntx, tx));
+    const int x0 = static_cast<int>(tx) * FT_TW, y0 = ty * FT_TH;
     // rows go to the output slot; counters and slow lists are per launch slot
     detect_tile_body<T, KT, RI>(proc + e.proc_off, reinterpret_cast<const T *>(e.in_ptr), p, e.H,
                                 e.W, g, x0, y0, e.frame, out + static_cast<size_t>(e.out_idx) * out_cap,
@@ -3593,6 +3616,12 @@ __global__ void k_select_launch(const GtBatchLaunch *__restrict__ launches,
     if (i == 0) {
         GtBatchHeader h = {};
         h.nslots = L.nslots;
+        h.nseg = L.nseg;
+        h.ntiles = L.ntiles;
+        for (uint32_t q = 0; q < GT_BATCH_MAX_SEG; ++q) {
+            h.seg[q] = L.seg[q];
+            if (q >= L.nseg) h.seg[q].first_tile = 0xFFFFFFFFu;
+        }
         *hdr = h;
     }
 }
@@ -3620,8 +3649,9 @@ hipError_t launch_batch_t(const GtBatchLaunch *launches, uint32_t *cur, const Gt
                                                             scount, max_slots);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    // x: tiles of the call's largest frame (multiple of 8); y: frames of a launch
-    const dim3 gbp(sch.bp_blocks, sch.launch_slots);
+    // one queue of the launch's tiles over all its frames (blocks past the
+    // launch's tiles return)
+    const dim3 gbp(sch.tile_blocks);
     with_bandpass_form(p, [&](auto r, auto d2) {
         k_bandpass_int_batch<T, decltype(r)::value, decltype(d2)::value>
             <<<gbp, B2_THREADS, 0, s>>>(proc, p, hdr, fs);
@@ -3633,7 +3663,7 @@ hipError_t launch_batch_t(const GtBatchLaunch *launches, uint32_t *cur, const Gt
     const TileGeom g = tile_geom(p.k, p.rint);
     const size_t lds = sizeof(float) * (static_cast<size_t>(g.rows) * g.pitch +
                                         static_cast<size_t>(FT_TH + p.k - 1) * FT_TW);
-    const dim3 gdt(sch.dt_blocks, sch.launch_slots);
+    const dim3 gdt(sch.tile_blocks);
 #define GT_TILEB(KV, RV)                                                                     \
     k_detect_tile_batch<T, KV, RV><<<gdt, FT_THREADS, lds, s>>>(proc, kw, g, hdr, fs, out,      \
                                                                 out_cap, ocount, slow, slow_cap, \
@@ -3692,6 +3722,7 @@ hipError_t gt_launch_batch(const GtBatchLaunch *launches, uint32_t *cur,
 }
 
 // Tile counts of one frame in the batched kernels.
+static_assert(B2_TW == FT_TW && B2_TH == FT_TH, "one tile queue serves bandpass and detect");
 void gt_batch_tiles(int H, int W, uint32_t *bp_tiles, uint32_t *dt_tiles) {
     *bp_tiles = static_cast<uint32_t>(((W + B2_TW - 1) / B2_TW) * ((H + B2_TH - 1) / B2_TH));
     *dt_tiles = static_cast<uint32_t>(((W + FT_TW - 1) / FT_TW) * ((H + FT_TH - 1) / FT_TH));

@@ -84,13 +84,32 @@ struct GtBatchFrame {
     uint32_t out_idx;     // output slot: rows at out[out_idx * cap], count at counts[out_idx]
     uint32_t frame;       // the frame field of its detections
 };
+// A launch's tiles form one queue over all its frames: its frames sorted by
+// size make runs of equal size (segments), and tile t of the launch is tile
+// (t - first_tile) % tpf of slot first_slot + (t - first_tile) / tpf of the
+// last segment with first_tile <= t.  Bandpass and detect tiles are both
+// 64 x 32, so one queue serves both stages.
+#define GT_BATCH_MAX_SEG 8
+struct GtBatchSeg {
+    uint32_t first_tile;  // queue index of the segment's first tile
+    uint32_t first_slot;  // its first frame slot
+    uint32_t tpf;         // tiles per frame
+    uint32_t ntx;         // tile columns per frame
+};
 struct GtBatchHeader {
     uint32_t nslots;      // frames of the launch
-    uint32_t pad[3];
+    uint32_t nseg;        // segments (1 .. GT_BATCH_MAX_SEG)
+    uint32_t ntiles;      // tiles of the launch's queue
+    uint32_t pad;
+    GtBatchSeg seg[GT_BATCH_MAX_SEG];
 };
 // A call's launches: launch k uses entries [first, first + nslots) of the
-// call's entry array; a device counter says which launch runs next.
+// call's entry array with its segments; a device counter says which launch
+// runs next.
 struct GtBatchLaunch {
     uint32_t first;
     uint32_t nslots;
+    uint32_t nseg;
+    uint32_t ntiles;
+    GtBatchSeg seg[GT_BATCH_MAX_SEG];
 };

@@ -156,8 +156,8 @@ WORKLOAD_SIZES = [(256, 256), (512, 384), (1024, 1024), (2048, 1024)]
 FIELDS = ["x", "y", "mass", "r", "max_intensity", "Rg", "raw_mass", "signal", "ecc", "count"]
 
 
-@pytest.mark.parametrize("graph", [True, False])
-def test_batch_workload_sizes_match_the_oracle(torch_dev, oracle, graph):
+@pytest.mark.parametrize("graph,launch_px", [(True, 3 << 20), (False, 3 << 20), (True, 16 << 20)])
+def test_batch_workload_sizes_match_the_oracle(torch_dev, oracle, graph, launch_px):
     import gpu_tracking_amd as gt
     from gpu_tracking_amd import synth
     frames, ids, host = [], [], []
@@ -177,11 +177,14 @@ def test_batch_workload_sizes_match_the_oracle(torch_dev, oracle, graph):
     print(f"[batch oracle] graph={graph} sizes {[f.shape for f in frames]}", flush=True)
     cap = 8192
     with gt.Batch(gt.make_params(9, **PARAMS), max_frame_pixels=2048 * 1024,
-                  max_launch_pixels=3 << 20, max_slots=8) as b:
+                  max_launch_pixels=launch_px, max_slots=8) as b:
         b.set_graph(graph)
         out, counts = b.detect(frames, frame_ids=ids, cap_per_frame=cap)
         torch_dev.cuda.synchronize()
-        assert b.schedule()["launches_last_call"] > 1
+        # 6.6 Mpx in all: several launches under a 3 Mpx budget, one launch
+        # per stage (one tile queue over all four sizes) under 16 Mpx
+        launches = b.schedule()["launches_last_call"]
+        assert launches > 1 if launch_px < (6 << 20) else launches == 1
         got = gt.detections_to_numpy(out, counts)
     for i, fr in enumerate(host):
         H, W = fr.shape
@@ -194,3 +197,26 @@ def test_batch_workload_sizes_match_the_oracle(torch_dev, oracle, graph):
         assert np.all(g["frame"] == ids[i])
         for f in FIELDS:
             assert np.array_equal(g[f].view(np.uint32), ref[f].view(np.uint32)), (i, f)
+
+
+def test_batch_many_sizes_split_by_size_runs(torch_dev):
+    # ten frame sizes in one call: a launch holds at most 8 runs of equal
+    # size, so the call takes two launches even though its pixels fit one;
+    # rows equal separate one-size runs
+    import gpu_tracking_amd as gt
+    from gpu_tracking_amd import synth
+    sizes = [(64 + 16 * k, 96 + 8 * k) for k in range(10)]
+    vids = []
+    for k, (H, W) in enumerate(sizes):
+        sp = synth.spec("full", nrows=H, ncols=W, n_particles=max(4, H * W // 1000))
+        vids.append(torch_dev.from_numpy(synth.video_numpy(sp, 1, seed=900 + k).view(np.int16)).cuda())
+    cap = 2048
+    sep = _separate(torch_dev, vids, cap)
+    frames = [v[0] for v in vids]
+    where = [(k, 0) for k in range(len(vids))]
+    with gt.Batch(gt.make_params(9, **PARAMS), max_frame_pixels=1 << 20,
+                  max_launch_pixels=64 << 20, max_slots=64) as b:
+        out, counts = b.detect(frames, frame_ids=[0] * len(frames), cap_per_frame=cap)
+        torch_dev.cuda.synchronize()
+        assert b.schedule()["launches_last_call"] == 2
+    _check(out, counts, sep, where)
