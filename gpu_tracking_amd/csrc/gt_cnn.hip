@@ -31,11 +31,10 @@ typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int CT = 256;  // four waves
 
-// float32 -> bfloat16, round to nearest even (finite inputs)
+// float32 -> bfloat16, round to nearest even (finite inputs): the gfx950
+// conversion instruction (v_cvt_pk_bf16_f32)
 __device__ __forceinline__ uint32_t bf16_bits(float f) {
-    uint32_t u = __float_as_uint(f);
-    u += 0x7FFFu + ((u >> 16) & 1u);
-    return u >> 16;
+    return __builtin_bit_cast(uint16_t, static_cast<__bf16>(f));
 }
 __device__ __forceinline__ float bf16_val(uint32_t h) { return __uint_as_float(h << 16); }
 __device__ __forceinline__ float bf16_round(float f) { return bf16_val(bf16_bits(f)); }
@@ -59,6 +58,8 @@ __device__ __forceinline__ bf16x8 frag_pack(const uint32_t (&v)[8]) {
 __device__ __forceinline__ f32x4 mma(bf16x8 a, bf16x8 b, f32x4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
 }
+// Orders the wave's LDS accesses across it for the compiler (no instruction).
+__device__ __forceinline__ void wave_lds_fence() { asm volatile("" ::: "memory"); }
 __device__ __forceinline__ f32x4 zero4() {
     const f32x4 z = {0.0f, 0.0f, 0.0f, 0.0f};
     return z;
@@ -100,13 +101,12 @@ __device__ __forceinline__ void stage_weights(const uint8_t *__restrict__ pk, Bl
     for (int i = tid; i < CNN_F_N; i += CT) B.f[i] = fs[i];
 }
 
-// The two patch values lane `lane` owns (indices lane and lane + 64) of the
-// 11 x 11 patch around pixel `pix`, normalised by the patch's largest
-// |value| and rounded to bf16 (bit patterns); 0 outside the frame.
-__device__ __forceinline__ void gather_norm(const float *__restrict__ P, int H, int W,
-                                            uint32_t pix, bool valid, int lane, uint32_t &h0,
-                                            uint32_t &h1) {
-    float v0 = 0.0f, v1 = 0.0f;
+// The two raw patch values lane `lane` owns (indices lane and lane + 64) of
+// the 11 x 11 patch around pixel `pix`; 0 outside the frame.
+__device__ __forceinline__ void gather_raw(const float *__restrict__ P, int H, int W, uint32_t pix,
+                                           bool valid, int lane, float &v0, float &v1) {
+    v0 = 0.0f;
+    v1 = 0.0f;
     if (valid) {
         const int cu = static_cast<int>(pix / static_cast<uint32_t>(W));
         const int cv = static_cast<int>(pix - static_cast<uint32_t>(cu) * static_cast<uint32_t>(W));
@@ -120,23 +120,57 @@ __device__ __forceinline__ void gather_norm(const float *__restrict__ P, int H, 
             if (u >= 0 && u < H && v >= 0 && v < W) v1 = P[static_cast<size_t>(u) * W + v];
         }
     }
+}
+// The raw values normalised by the patch's largest |value| and rounded to
+// bf16 (bit patterns).
+__device__ __forceinline__ void normalise(float v0, float v1, int lane, uint32_t &h0, uint32_t &h1) {
     float mx = fmaxf(fabsf(v0), fabsf(v1));
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off));
     h0 = mx > 0.0f ? bf16_bits(v0 / mx) : 0u;
     h1 = (mx > 0.0f && lane + 64 < 121) ? bf16_bits(v1 / mx) : 0u;
 }
+__device__ __forceinline__ void gather_norm(const float *__restrict__ P, int H, int W,
+                                            uint32_t pix, bool valid, int lane, uint32_t &h0,
+                                            uint32_t &h1) {
+    float v0, v1;
+    gather_raw(P, H, W, pix, valid, lane, v0, v1);
+    normalise(v0, v1, lane, h0, h1);
+}
 
 // ---- forward phases (one wave, its own sample) ---------------------------
+// conv1 (M = 81 positions in 6 tiles, N = 16 channels, K = 9 taps padded to
+// 32) on the matrix cores: lane l's A row is position m = 16 mt + l % 16 with
+// taps 8 (l / 16) .. + 8 of its 3 x 3 window, its B column channel l % 16
+// with the same taps of w1 (bf16); the bias is added to the f32 sum.
 __device__ __forceinline__ void conv1(const BlockW &B, WaveAct &A, int lane) {
-    for (int idx = lane; idx < 81 * 16; idx += 64) {
-        const int pos = idx >> 4, c = idx & 15;
-        const int py = pos / 9, px = pos - py * 9;
-        float z = B.f[CNN_F_B1 + c];
+    const int r16 = lane & 15, kg = lane >> 4;
+    uint32_t wv[8];
 #pragma unroll
-        for (int t = 0; t < 9; ++t)
-            z = fmaf(B.f[CNN_F_W1 + c * 9 + t], A.x[(py + t / 3) * 11 + px + t % 3], z);
-        A.a1[idx] = static_cast<uint16_t>(bf16_bits(fmaxf(z, 0.0f)));
+    for (int e = 0; e < 8; ++e) {
+        const int t = kg * 8 + e;
+        wv[e] = t < 9 ? (__float_as_uint(B.f[CNN_F_W1 + r16 * 9 + t]) >> 16) : 0u;
+    }
+    const bf16x8 b = frag_pack(wv);
+    const float bias = B.f[CNN_F_B1 + r16];
+#pragma unroll
+    for (int mt = 0; mt < 6; ++mt) {
+        const int m = mt * 16 + r16;
+        const int py = m / 9, px = m - py * 9;
+        uint32_t xv[8];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const int t = kg * 8 + e;
+            xv[e] = (m < 81 && t < 9) ? (__float_as_uint(A.x[(py + t / 3) * 11 + px + t % 3]) >> 16)
+                                      : 0u;
+        }
+        const f32x4 d = mma(frag_pack(xv), b, zero4());
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int row = mt * 16 + kg * 4 + i;
+            if (row < 81)
+                A.a1[row * 16 + r16] = static_cast<uint16_t>(bf16_bits(fmaxf(d[i] + bias, 0.0f)));
+        }
     }
 }
 
@@ -184,6 +218,87 @@ __device__ __forceinline__ void conv3(const BlockW &B, WaveAct &A, int lane) {
             if (mv) a = frag_ld(&A.a2[((my + ky) * 7 + mx + kx) * 32 + kg * 8]);
             c0 = mma(a, frag_ld(&B.w3p[r16 * CNN_K3 + s * 32 + kg * 8]), c0);
             c1 = mma(a, frag_ld(&B.w3p[(16 + r16) * CNN_K3 + s * 32 + kg * 8]), c1);
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int row = mt * 16 + kg * 4 + i;
+            if (row < 25) {
+                A.a3[row * 32 + r16] = bf16_round(fmaxf(c0[i] + B.f[CNN_F_B3 + r16], 0.0f));
+                A.a3[row * 32 + 16 + r16] = bf16_round(fmaxf(c1[i] + B.f[CNN_F_B3 + 16 + r16], 0.0f));
+            }
+        }
+    }
+}
+
+// The conv2 and conv3 B operands of this lane, held in registers across the
+// samples of the forward kernel (the weights do not change between samples,
+// so only the activations are read from LDS per sample).
+struct WeightFrags {
+    bf16x8 w2[5][2];
+    bf16x8 w3[9][2];
+};
+__device__ __forceinline__ void load_weight_frags(const BlockW &B, int lane, WeightFrags &F) {
+    const int r16 = lane & 15, kg = lane >> 4;
+#pragma unroll
+    for (int st = 0; st < 5; ++st) {
+        const int k0 = st * 32 + kg * 8;
+        F.w2[st][0] = frag_ld(&B.w2p[r16 * CNN_K2 + k0]);
+        F.w2[st][1] = frag_ld(&B.w2p[(16 + r16) * CNN_K2 + k0]);
+    }
+#pragma unroll
+    for (int st = 0; st < 9; ++st) {
+        F.w3[st][0] = frag_ld(&B.w3p[r16 * CNN_K3 + st * 32 + kg * 8]);
+        F.w3[st][1] = frag_ld(&B.w3p[(16 + r16) * CNN_K3 + st * 32 + kg * 8]);
+    }
+}
+
+// conv2 and conv3 as above with the B operands from registers (the same
+// products in the same order).
+__device__ __forceinline__ void conv2_rf(const BlockW &B, const WeightFrags &F, WaveAct &A,
+                                         int lane) {
+    const int r16 = lane & 15, kg = lane >> 4;
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt) {
+        const int m = mt * 16 + r16;
+        const bool mv = m < 49;
+        const int my = m / 7, mx = m - my * 7;
+        f32x4 c0 = zero4(), c1 = zero4();
+#pragma unroll
+        for (int st = 0; st < 5; ++st) {
+            const int k0 = st * 32 + kg * 8, tap = k0 >> 4, ch = k0 & 15;
+            bf16x8 a = frag_zero();
+            if (mv && tap < 9) a = frag_ld(&A.a1[((my + tap / 3) * 9 + mx + tap % 3) * 16 + ch]);
+            c0 = mma(a, F.w2[st][0], c0);
+            c1 = mma(a, F.w2[st][1], c1);
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int row = mt * 16 + kg * 4 + i;
+            if (row < 49) {
+                A.a2[row * 32 + r16] =
+                    static_cast<uint16_t>(bf16_bits(fmaxf(c0[i] + B.f[CNN_F_B2 + r16], 0.0f)));
+                A.a2[row * 32 + 16 + r16] =
+                    static_cast<uint16_t>(bf16_bits(fmaxf(c1[i] + B.f[CNN_F_B2 + 16 + r16], 0.0f)));
+            }
+        }
+    }
+}
+__device__ __forceinline__ void conv3_rf(const BlockW &B, const WeightFrags &F, WaveAct &A,
+                                         int lane) {
+    const int r16 = lane & 15, kg = lane >> 4;
+#pragma unroll
+    for (int mt = 0; mt < 2; ++mt) {
+        const int m = mt * 16 + r16;
+        const bool mv = m < 25;
+        const int my = m / 5, mx = m - my * 5;
+        f32x4 c0 = zero4(), c1 = zero4();
+#pragma unroll
+        for (int st = 0; st < 9; ++st) {
+            const int ky = st / 3, kx = st - ky * 3;
+            bf16x8 a = frag_zero();
+            if (mv) a = frag_ld(&A.a2[((my + ky) * 7 + mx + kx) * 32 + kg * 8]);
+            c0 = mma(a, F.w3[st][0], c0);
+            c1 = mma(a, F.w3[st][1], c1);
         }
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
@@ -359,29 +474,74 @@ __global__ __launch_bounds__(CT, 2) void k_cnn_forward(const uint8_t *__restrict
     const uint32_t per = 4u * gridDim.x;
     const uint32_t iters = (count + per - 1) / per;   // uniform across the block
     __syncthreads();
+    WeightFrags F;
+    load_weight_frags(B, lane, F);
+    // sample of iteration it: j(it); its inputs are loaded one iteration
+    // ahead (frames: the candidate pixel two ahead), so the loads' latency
+    // overlaps the previous sample's network
+    auto jof = [&](uint32_t it) {
+        return (it * gridDim.x + blockIdx.x) * 4u + static_cast<uint32_t>(wave);
+    };
+    float v0 = 0.0f, v1 = 0.0f;      // inputs of the current sample
+    uint32_t pix_next = 0u;          // frames: candidate pixel of the next sample
+    if (iters > 0) {
+        const uint32_t j0 = jof(0);
+        if constexpr (kFrames) {
+            const uint32_t pix0 = j0 < count ? fc[static_cast<size_t>(j0) * cstride] : 0u;
+            gather_raw(P, H, W, pix0, j0 < count, lane, v0, v1);
+            const uint32_t j1 = jof(1);
+            pix_next = j1 < count ? fc[static_cast<size_t>(j1) * cstride] : 0u;
+        } else {
+            if (j0 < count) {
+                v0 = bf16_val(patches[static_cast<size_t>(j0) * CNN_PATCH_PITCH + lane]);
+                if (lane + 64 < 121)
+                    v1 = bf16_val(patches[static_cast<size_t>(j0) * CNN_PATCH_PITCH + lane + 64]);
+            }
+        }
+    }
     for (uint32_t it = 0; it < iters; ++it) {
-        const uint32_t j = (it * gridDim.x + blockIdx.x) * 4u + static_cast<uint32_t>(wave);
+        const uint32_t j = jof(it);
         const bool valid = j < count;
         if constexpr (kFrames) {
             uint32_t h0, h1;
-            gather_norm(P, H, W, valid ? fc[static_cast<size_t>(j) * cstride] : 0u, valid, lane,
-                        h0, h1);
+            normalise(v0, v1, lane, h0, h1);
             A.x[lane] = bf16_val(h0);
             A.x[lane + 64] = bf16_val(h1);
         } else {
-            for (int i = lane; i < 128; i += 64)
-                A.x[i] = (valid && i < 121) ? bf16_val(patches[static_cast<size_t>(j) * CNN_PATCH_PITCH + i]) : 0.0f;
+            A.x[lane] = v0;
+            A.x[lane + 64] = v1;
         }
-        __syncthreads();
+        // the next sample's inputs
+        const uint32_t jn = jof(it + 1);
+        const bool vn = it + 1 < iters && jn < count;
+        if constexpr (kFrames) {
+            gather_raw(P, H, W, pix_next, vn, lane, v0, v1);
+            const uint32_t jnn = jof(it + 2);
+            pix_next = (it + 2 < iters && jnn < count) ? fc[static_cast<size_t>(jnn) * cstride] : 0u;
+        } else {
+            v0 = 0.0f;
+            v1 = 0.0f;
+            if (vn) {
+                v0 = bf16_val(patches[static_cast<size_t>(jn) * CNN_PATCH_PITCH + lane]);
+                if (lane + 64 < 121)
+                    v1 = bf16_val(patches[static_cast<size_t>(jn) * CNN_PATCH_PITCH + lane + 64]);
+            }
+        }
+        // every phase's activations are the wave's own: the wave's LDS
+        // operations complete in order, so a compiler barrier (no
+        // workgroup barrier) separates the phases and the four waves run
+        // independently
+        wave_lds_fence();
         conv1(B, A, lane);
-        __syncthreads();
-        conv2(B, A, lane);
-        __syncthreads();
-        conv3(B, A, lane);
-        __syncthreads();
+        wave_lds_fence();
+        conv2_rf(B, F, A, lane);
+        wave_lds_fence();
+        conv3_rf(B, F, A, lane);
+        wave_lds_fence();
         float o[3];
         head(B, A, lane, o);
         if (valid && lane == 0) fo[j] = make_float4(o[0], o[1], o[2], 0.0f);
+        wave_lds_fence();
     }
 }
 
