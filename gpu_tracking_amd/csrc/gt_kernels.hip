@@ -3276,10 +3276,18 @@ __global__ __launch_bounds__(WK_THREADS) void k_walk_log(const float *__restrict
                                                          uint32_t out_cap,
                                                          uint32_t *__restrict__ ocount,
                                                          uint32_t frame0,
-                                                         const float *__restrict__ cval) {
+                                                         const float *__restrict__ cval,
+                                                         int nframes, uint32_t wb) {
     constexpr int JW = 2 * GT_LOG_WR + 1;
     __shared__ int8_t s_jm[GT_LOG_MAX_SCALES * JW];
-    const int fr = blockIdx.y;
+    // block w of the launch serves part b of frame fr's list; consecutive
+    // parts of a frame (neighbouring candidates, whose disks overlap) go to
+    // one XCD, so its L2 serves the overlaps
+    const uint32_t total = static_cast<uint32_t>(nframes) * wb;
+    const uint32_t w = static_cast<uint32_t>(xcd_tile(static_cast<int>(total)));
+    if (w >= total) return;
+    const int fr = static_cast<int>(w / wb);
+    const uint32_t b = w - static_cast<uint32_t>(fr) * wb;
     const int H = p.H, W = p.W;
     const uint32_t HW = static_cast<uint32_t>(H) * static_cast<uint32_t>(W);
     const int ns = tab->n;
@@ -3299,12 +3307,14 @@ __global__ __launch_bounds__(WK_THREADS) void k_walk_log(const float *__restrict
     gt_detection *fout = out + static_cast<size_t>(fr) * out_cap;
     const float stdv = frame_std(p, fr);
     const uint32_t lane = __lane_id();
-    const uint32_t stride = gridDim.x * WK_THREADS;
-    for (uint32_t i0 = blockIdx.x * WK_THREADS + (threadIdx.x & ~63u); i0 < n; i0 += stride) {
+    // this block's contiguous share of the list, in whole wavefronts
+    const uint32_t per = ((n + wb - 1) / wb + 63u) & ~63u;
+    const uint32_t lo = b * per, hi = min(n, lo + per);
+    for (uint32_t i0 = lo + (threadIdx.x & ~63u); i0 < hi; i0 += WK_THREADS) {
         const uint32_t i = i0 + lane;
         bool keep = false;
         gt_detection d;
-        if (i < n && (!cval || cval[static_cast<size_t>(fr) * cap + i] > rel_thr)) {
+        if (i < hi && (!cval || cval[static_cast<size_t>(fr) * cap + i] > rel_thr)) {
             const uint32_t seed = fc[i];
             const uint32_t m = seed / HW;
             const uint32_t ras = seed - m * HW;
@@ -3376,11 +3386,12 @@ hipError_t launch_walk_log_t(const float *proc, const T *raw, uint32_t nframes, 
                              const GtLogTable *tab, const uint32_t *cand, uint32_t cap,
                              const uint32_t *ccount, gt_detection *out, uint32_t out_cap,
                              uint32_t *ocount, uint32_t frame0, const float *cval, hipStream_t s) {
-    // enough blocks to cover a typical frame's list in one pass
-    const uint32_t blocks = std::min<uint32_t>(1024u, std::max<uint32_t>(1u, cap / WK_THREADS / 4u));
-    const dim3 grid(blocks, nframes, 1);
-    k_walk_log<T><<<grid, WK_THREADS, 0, s>>>(proc, raw, p, tab, cand, cap, ccount, out, out_cap,
-                                              ocount, frame0, cval);
+    // parts per frame: enough blocks to cover a typical frame's list in one
+    // pass of each block's wavefronts
+    const uint32_t wb = std::min<uint32_t>(1024u, std::max<uint32_t>(1u, cap / WK_THREADS / 4u));
+    k_walk_log<T><<<xcd_grid(static_cast<size_t>(wb) * nframes), WK_THREADS, 0, s>>>(
+        proc, raw, p, tab, cand, cap, ccount, out, out_cap, ocount, frame0, cval,
+        static_cast<int>(nframes), wb);
     return hipGetLastError();
 }
 
