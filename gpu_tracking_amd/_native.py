@@ -93,7 +93,7 @@ EXPORTS = [
     "gt_column_names", "gt_filter_close", "gt_doughnut", "gt_set_interrupt", "gt_progress",
     "gt_set_timing", "gt_stage_times", "gt_prune_blobs", "gt_link", "gt_connect", "gt_batch_create",
     "gt_batch_destroy", "gt_batch_detect", "gt_batch_tune", "gt_batch_get_schedule",
-    "gt_batch_set_graph", "gt_frame_std", "gt_cnn_create", "gt_cnn_destroy", "gt_cnn_set_params",
+    "gt_batch_get_schedule_info", "gt_batch_set_graph", "gt_frame_std", "gt_cnn_create", "gt_cnn_destroy", "gt_cnn_set_params",
     "gt_cnn_get_params", "gt_cnn_adam_steps", "gt_cnn_forward", "gt_cnn_gradient", "gt_cnn_step",
     "gt_cnn_train", "gt_cnn_make_samples", "gt_context_set_cnn", "gt_context_cnn_stats",
     "gt_file_open", "gt_file_close", "gt_file_info", "gt_file_read", "gt_file_channels",
@@ -273,6 +273,8 @@ def lib() -> C.CDLL:
     L.gt_file_channels.restype = u32
     L.gt_batch_get_lanes.argtypes = [vp]
     L.gt_batch_get_lanes.restype = u32
+    L.gt_batch_get_schedule_info.argtypes = [vp, vp]
+    L.gt_batch_get_schedule_info.restype = C.c_int
     _lib = L
     return L
 

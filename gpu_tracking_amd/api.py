@@ -862,12 +862,20 @@ class Batch:
         return float(ms.value)
 
     def schedule(self) -> dict:
-        a = (C.c_uint64 * 4)()
-        launches = C.c_uint64()
-        N.check(self._lib.gt_batch_get_schedule(self._h, a, C.byref(launches)))
-        return {"pixels_per_launch": a[0], "walk_blocks": a[1], "max_frames_per_launch": a[2],
-                "max_pixels_per_launch": a[3], "launches_last_call": launches.value,
-                "lanes": int(self._lib.gt_batch_get_lanes(self._h))}
+        """The schedule in use and the last call's grid
+        (gt_batch_get_schedule_info)."""
+        info = _BatchScheduleInfo()
+        N.check(self._lib.gt_batch_get_schedule_info(self._h, C.byref(info)))
+        return {"pixels_per_launch": info.pixels_per_launch,
+                "walk_blocks": info.walk_blocks,
+                "max_frames_per_launch": info.max_frames_per_launch,
+                "max_pixels_per_launch": info.max_pixels_per_launch,
+                "launches_last_call": info.launches, "lanes": info.lanes,
+                "tile_shape": (info.tile_rows, info.tile_cols),
+                "tile_blocks_last_call": info.tile_blocks,
+                "frames_per_launch_last_call": info.launch_slots,
+                "max_size_runs_per_launch_last_call": info.max_segments,
+                "graph": bool(info.graph)}
 
     def set_graph(self, enable: bool) -> None:
         N.check(self._lib.gt_batch_set_graph(self._h, int(bool(enable))))
@@ -875,6 +883,16 @@ class Batch:
     def set_lanes(self, lanes: int) -> None:
         """Streams (1..4) that consecutive launches are dealt to."""
         N.check(self._lib.gt_batch_set_lanes(self._h, int(lanes)))
+
+
+class _BatchScheduleInfo(C.Structure):
+    # gt_batch_schedule_info (include/gtrack.h)
+    _fields_ = [("pixels_per_launch", C.c_uint64), ("max_pixels_per_launch", C.c_uint64),
+                ("walk_blocks", C.c_uint32), ("max_frames_per_launch", C.c_uint32),
+                ("lanes", C.c_uint32), ("tile_rows", C.c_uint32), ("tile_cols", C.c_uint32),
+                ("launches", C.c_uint32), ("tile_blocks", C.c_uint32),
+                ("launch_slots", C.c_uint32), ("max_segments", C.c_uint32),
+                ("graph", C.c_int32)]
 
 
 class CentroidNet:

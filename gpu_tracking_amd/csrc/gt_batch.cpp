@@ -98,6 +98,8 @@ struct gt_batch {
     uint64_t launch_px = 0;           // pixels per launch in use (<= max_launch_pixels)
     int32_t use_graph = 1;
     uint64_t launches = 0;            // launches of the last call
+    GtBatchSchedule last_sch = {0u, 0u, 0u};  // grid shape of the last call
+    uint32_t last_max_seg = 0;        // most size runs in one launch of the last call
 };
 
 namespace {
@@ -257,6 +259,7 @@ gt_status run_batch(gt_batch *b, const gt_frame_desc *frames, uint32_t n, int32_
     // the frame tables; one grid shape for every launch of the call: the tile
     // queue of its largest launch (multiple of 8) and its most frames
     GtBatchSchedule sch_all = {8u, walk_blocks, 1u};
+    uint32_t max_seg = 0;
     for (size_t k = 0; k < nl; ++k) {
         const GtBatchLaunch &L = launches[k];
         uint64_t off = 0;
@@ -273,6 +276,7 @@ gt_status run_batch(gt_batch *b, const gt_frame_desc *frames, uint32_t n, int32_
         }
         sch_all.tile_blocks = std::max(sch_all.tile_blocks, (L.ntiles + 7) / 8 * 8);
         sch_all.launch_slots = std::max(sch_all.launch_slots, L.nslots);
+        max_seg = std::max(max_seg, L.nseg);
     }
     std::vector<size_t> fill(lane_off.begin(), lane_off.end() - 1);
     for (size_t k = 0; k < nl; ++k) hl[fill[k % nlanes]++] = launches[k];
@@ -349,6 +353,8 @@ gt_status run_batch(gt_batch *b, const gt_frame_desc *frames, uint32_t n, int32_
     // the staging buffer and tables are free again once this call's launches are done
     GB_HIP(hipEventRecord(b->stage_free, s));
     b->launches = nl;
+    b->last_sch = sch_all;
+    b->last_max_seg = max_seg;
     return GT_OK;
 }
 
@@ -496,6 +502,25 @@ gt_status gt_batch_get_schedule(gt_batch *b, uint64_t *sched4, uint64_t *launche
     sched4[2] = b->max_slots;
     sched4[3] = b->max_launch_pixels;
     if (launches) *launches = b->launches;
+    return GT_OK;
+}
+
+gt_status gt_batch_get_schedule_info(gt_batch *b, gt_batch_schedule_info *out) {
+    if (!b || !out) return gt_runtime_fail(GT_ERR_INVALID_ARGUMENT, "null argument");
+    gt_batch_schedule_info r = {};
+    r.pixels_per_launch = b->launch_px;
+    r.max_pixels_per_launch = b->max_launch_pixels;
+    r.walk_blocks = b->walk_blocks;
+    r.max_frames_per_launch = b->max_slots;
+    r.lanes = b->nlanes;
+    r.tile_rows = gt_batch_tile_rows();
+    r.tile_cols = gt_batch_tile_cols();
+    r.launches = static_cast<uint32_t>(b->launches);
+    r.tile_blocks = b->last_sch.tile_blocks;
+    r.launch_slots = b->last_sch.launch_slots;
+    r.max_segments = b->last_max_seg;
+    r.graph = b->use_graph;
+    *out = r;
     return GT_OK;
 }
 

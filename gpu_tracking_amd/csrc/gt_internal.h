@@ -114,6 +114,9 @@ hipError_t gt_launch_batch(const GtBatchLaunch *launches, uint32_t *cur,
                            uint32_t slow_cap, uint32_t *scount, uint32_t *counts,
                            const GtBatchSchedule &sch, hipStream_t s);
 void gt_batch_tiles(int H, int W, uint32_t *bp_tiles, uint32_t *dt_tiles);
+// Rows and columns of the tile the batched bandpass and detect kernels share.
+uint32_t gt_batch_tile_rows();
+uint32_t gt_batch_tile_cols();
 // Learned proposal path: candidate pixels per frame (local-maximum test of
 // the tile kernel), the verifying walk over the network's proposals
 // (stats[0] += accepted, stats[1] += candidates), and the training targets

@@ -3734,6 +3734,8 @@ hipError_t gt_launch_batch(const GtBatchLaunch *launches, uint32_t *cur,
 
 // Tile counts of one frame in the batched kernels.
 static_assert(B2_TW == FT_TW && B2_TH == FT_TH, "one tile queue serves bandpass and detect");
+uint32_t gt_batch_tile_rows() { return static_cast<uint32_t>(FT_TH); }
+uint32_t gt_batch_tile_cols() { return static_cast<uint32_t>(FT_TW); }
 void gt_batch_tiles(int H, int W, uint32_t *bp_tiles, uint32_t *dt_tiles) {
     *bp_tiles = static_cast<uint32_t>(((W + B2_TW - 1) / B2_TW) * ((H + B2_TH - 1) / B2_TH));
     *dt_tiles = static_cast<uint32_t>(((W + FT_TW - 1) / FT_TW) * ((H + FT_TH - 1) / FT_TH));

@@ -406,6 +406,26 @@ gt_status gt_batch_tune(gt_batch *b, const gt_frame_desc *frames, uint32_t n, in
  * per launch at most, the context's pixel budget per launch) and the number
  * of launches of the last call. */
 gt_status gt_batch_get_schedule(gt_batch *b, uint64_t *sched4, uint64_t *launches);
+/* The whole schedule: the knobs above, the lanes, the tile shape of the
+ * bandpass and detect kernels (fixed: 64 columns, one per lane of a
+ * wavefront, by 32 rows; 16 and 64 rows were measured slower at every batch
+ * frame size) and the last call's grid (tile workgroups per launch, frames
+ * per launch, most runs of one frame size in a launch).  No reference
+ * counterpart: the reference has no batched path. */
+typedef struct gt_batch_schedule_info {
+    uint64_t pixels_per_launch;     /* pixel budget per launch in use */
+    uint64_t max_pixels_per_launch; /* the context's limit */
+    uint32_t walk_blocks;           /* slow-walk / characterize blocks per frame */
+    uint32_t max_frames_per_launch;
+    uint32_t lanes;
+    uint32_t tile_rows, tile_cols;
+    uint32_t launches;              /* launches of the last call */
+    uint32_t tile_blocks;           /* tile workgroups per launch, last call */
+    uint32_t launch_slots;          /* frames per launch the grids cover, last call */
+    uint32_t max_segments;          /* most size runs in one launch, last call */
+    int32_t graph;                  /* graph replay on */
+} gt_batch_schedule_info;
+gt_status gt_batch_get_schedule_info(gt_batch *b, gt_batch_schedule_info *out);
 /* Graph replay on (default) or direct launches. */
 gt_status gt_batch_set_graph(gt_batch *b, int32_t enable);
 /* Lanes (1..4): launch k of a call runs on lane k % lanes, each lane a stream

@@ -218,5 +218,14 @@ def test_batch_many_sizes_split_by_size_runs(torch_dev):
                   max_launch_pixels=64 << 20, max_slots=64) as b:
         out, counts = b.detect(frames, frame_ids=[0] * len(frames), cap_per_frame=cap)
         torch_dev.cuda.synchronize()
-        assert b.schedule()["launches_last_call"] == 2
+        sch = b.schedule()
+        assert sch["launches_last_call"] == 2
+        # the first launch holds 8 runs of one frame each; the grid covers it
+        assert sch["max_size_runs_per_launch_last_call"] == 8
+        assert sch["frames_per_launch_last_call"] == 8
+        assert sch["tile_shape"] == (32, 64)
+        # a launch of 8 frames has at least the 8 smallest frames' tiles
+        tiles = sorted(((H + 31) // 32) * ((W + 63) // 64) for H, W in sizes)
+        assert sch["tile_blocks_last_call"] >= sum(tiles[:8])
+        assert sch["tile_blocks_last_call"] % 8 == 0
     _check(out, counts, sep, where)
