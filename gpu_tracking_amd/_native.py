@@ -217,10 +217,10 @@ def lib() -> C.CDLL:
     L.gt_batch_create.restype = C.c_int
     L.gt_batch_destroy.argtypes = [C.c_void_p]
     L.gt_batch_destroy.restype = None
-    L.gt_batch_detect.argtypes = [C.c_void_p, P(GtFrameDesc), C.c_uint32, C.c_int32, C.c_void_p,
+    L.gt_batch_detect.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32, C.c_int32, C.c_void_p,
                                   C.c_uint32, C.c_void_p, C.c_void_p]
     L.gt_batch_detect.restype = C.c_int
-    L.gt_batch_tune.argtypes = [C.c_void_p, P(GtFrameDesc), C.c_uint32, C.c_int32, C.c_void_p,
+    L.gt_batch_tune.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32, C.c_int32, C.c_void_p,
                                 C.c_uint32, C.c_void_p, P(C.c_float)]
     L.gt_batch_tune.restype = C.c_int
     L.gt_batch_get_schedule.argtypes = [C.c_void_p, C.c_void_p, P(C.c_uint64)]

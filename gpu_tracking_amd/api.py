@@ -810,13 +810,14 @@ class Batch:
 
     @staticmethod
     def _descs(frames, frame_ids):
+        """gt_frame_desc array for `frames`: 2-D frames, or 3-D videos whose
+        frames join the list in order (built without a per-frame loop)."""
         import torch
-        n = len(frames)
-        descs = (N.GtFrameDesc * n)()
+        ptrs, rows, cols = [], [], []
         code = None
-        for i, f in enumerate(frames):
-            if not f.is_cuda or f.dim() != 2 or not f.is_contiguous():
-                raise ValueError("frames must be contiguous 2-D device tensors")
+        for f in frames:
+            if not f.is_cuda or f.dim() not in (2, 3) or not f.is_contiguous():
+                raise ValueError("frames must be contiguous 2-D (or 3-D video) device tensors")
             c = N.GT_U16 if f.dtype in (torch.int16, torch.uint16) else (
                 N.GT_U8 if f.dtype == torch.uint8 else None)
             if c is None:
@@ -824,18 +825,34 @@ class Batch:
             if code is not None and c != code:
                 raise TypeError("all frames of a batch must have one pixel type")
             code = c
-            descs[i].data = f.data_ptr()
-            descs[i].nrows, descs[i].ncols = f.shape
-            descs[i].frame = int(frame_ids[i]) if frame_ids is not None else i
-        return descs, code
+            if f.dim() == 2:
+                ptrs.append(np.array([f.data_ptr()], dtype=np.uint64))
+                rows.append(np.array([f.shape[0]], dtype=np.uint32))
+                cols.append(np.array([f.shape[1]], dtype=np.uint32))
+            else:
+                t, h, w = f.shape
+                step = h * w * f.element_size()
+                ptrs.append(np.uint64(f.data_ptr()) + np.arange(t, dtype=np.uint64) * np.uint64(step))
+                rows.append(np.full(t, h, dtype=np.uint32))
+                cols.append(np.full(t, w, dtype=np.uint32))
+        n = int(sum(len(x) for x in ptrs))
+        arr = np.zeros(n, dtype=_FRAME_DESC_DTYPE)
+        if n:
+            arr["data"] = np.concatenate(ptrs)
+            arr["nrows"] = np.concatenate(rows)
+            arr["ncols"] = np.concatenate(cols)
+            arr["frame"] = (np.asarray(frame_ids, dtype=np.uint32) if frame_ids is not None
+                            else np.arange(n, dtype=np.uint32))
+        return arr, code
 
     def detect(self, frames, frame_ids=None, out=None, counts=None,
                cap_per_frame: int = 8192, stream=None):
-        """frames: list of 2-D device tensors.  Returns (rows (n, cap, 12) int32
-        words, counts (n,) int32); rows of frames[i] at out[i]."""
+        """frames: list of 2-D device tensors (frames) or 3-D ones (videos,
+        whose frames join the list in order).  Returns (rows (n, cap, 12)
+        int32 words, counts (n,) int32); rows of the i-th frame at out[i]."""
         import torch
         descs, code = self._descs(frames, frame_ids)
-        n = len(frames)
+        n = len(descs)
         dev = frames[0].device
         if out is None:
             out = torch.empty((n, cap_per_frame, 12), dtype=torch.int32, device=dev)
@@ -843,8 +860,8 @@ class Batch:
             counts = torch.zeros((n,), dtype=torch.int32, device=dev)
         if stream is None:
             stream = torch.cuda.current_stream(dev).cuda_stream
-        N.check(self._lib.gt_batch_detect(self._h, descs, n, code, out.data_ptr(), out.shape[1],
-                                          counts.data_ptr(), stream))
+        N.check(self._lib.gt_batch_detect(self._h, descs.ctypes.data, n, code, out.data_ptr(),
+                                          out.shape[1], counts.data_ptr(), stream))
         return out, counts
 
     def tune(self, frames, cap_per_frame: int = 8192) -> float:
@@ -852,13 +869,13 @@ class Batch:
         sample frames; returns the best time (ms) of one pass over them."""
         import torch
         descs, code = self._descs(frames, None)
-        n = len(frames)
+        n = len(descs)
         out = torch.empty((n, cap_per_frame, 12), dtype=torch.int32, device=frames[0].device)
         counts = torch.zeros((n,), dtype=torch.int32, device=frames[0].device)
         torch.cuda.synchronize()
         ms = C.c_float()
-        N.check(self._lib.gt_batch_tune(self._h, descs, n, code, out.data_ptr(), cap_per_frame,
-                                        counts.data_ptr(), C.byref(ms)))
+        N.check(self._lib.gt_batch_tune(self._h, descs.ctypes.data, n, code, out.data_ptr(),
+                                        cap_per_frame, counts.data_ptr(), C.byref(ms)))
         return float(ms.value)
 
     def schedule(self) -> dict:
@@ -883,6 +900,11 @@ class Batch:
     def set_lanes(self, lanes: int) -> None:
         """Streams (1..4) that consecutive launches are dealt to."""
         N.check(self._lib.gt_batch_set_lanes(self._h, int(lanes)))
+
+
+# gt_frame_desc (include/gtrack.h) as a numpy record
+_FRAME_DESC_DTYPE = np.dtype([("data", "<u8"), ("nrows", "<u4"), ("ncols", "<u4"),
+                              ("frame", "<u4"), ("pad", "<u4")])
 
 
 class _BatchScheduleInfo(C.Structure):

@@ -89,9 +89,13 @@ struct gt_batch {
     // a call's launch tables: all entries (shared by the lanes)
     GtBatchFrame *d_all = nullptr;
     size_t all_cap = 0;
-    char *h_stage = nullptr;          // pinned staging of the tables (+ the counters' zeros)
-    size_t h_cap = 0;
-    hipEvent_t stage_free = nullptr;  // the last upload from h_stage has completed
+    // pinned staging of the tables (+ the counters' zeros), two buffers used
+    // by alternate calls: a call fills one while the previous call's launches
+    // (uploaded from the other) still run
+    char *h_stage[2] = {nullptr, nullptr};
+    size_t h_cap[2] = {0, 0};
+    hipEvent_t stage_free[2] = {nullptr, nullptr};  // the last call using buffer i is done
+    uint32_t stage_next = 0;          // buffer of the next call
     hipEvent_t uploaded = nullptr;    // the call's tables are on the device
     hipEvent_t ev_in = nullptr, ev_out = nullptr;  // ordering against the caller's stream
     uint32_t walk_blocks = 4;         // blocks per frame for slow walks and characterize
@@ -122,8 +126,10 @@ void free_lane(BatchLane &L) {
 void free_batch(gt_batch *b) {
     for (BatchLane &L : b->lanes) free_lane(L);
     if (b->d_all) (void)hipFree(b->d_all);
-    if (b->h_stage) (void)hipHostFree(b->h_stage);
-    if (b->stage_free) (void)hipEventDestroy(b->stage_free);
+    for (int i = 0; i < 2; ++i) {
+        if (b->h_stage[i]) (void)hipHostFree(b->h_stage[i]);
+        if (b->stage_free[i]) (void)hipEventDestroy(b->stage_free[i]);
+    }
     if (b->uploaded) (void)hipEventDestroy(b->uploaded);
     if (b->ev_in) (void)hipEventDestroy(b->ev_in);
     if (b->ev_out) (void)hipEventDestroy(b->ev_out);
@@ -231,30 +237,34 @@ gt_status run_batch(gt_batch *b, const gt_frame_desc *frames, uint32_t n, int32_
     std::vector<size_t> lane_n(nlanes, 0), lane_off(nlanes + 1, 0);
     for (size_t k = 0; k < nl; ++k) ++lane_n[k % nlanes];
     for (uint32_t l = 0; l < nlanes; ++l) lane_off[l + 1] = lane_off[l] + lane_n[l];
-    // the pinned staging buffer and the device tables may still serve the
-    // previous call's upload and launches
-    GB_HIP(hipEventSynchronize(b->stage_free));
+    // this call's staging buffer may still serve the upload of the call
+    // before the previous one; the device tables are written by uploads
+    // ordered on s after the previous call's launches
+    const uint32_t sb = b->stage_next;
+    b->stage_next ^= 1u;
+    GB_HIP(hipEventSynchronize(b->stage_free[sb]));
     const size_t bytes = sizeof(GtBatchFrame) * n + sizeof(GtBatchLaunch) * nl +
                          sizeof(uint32_t) * kMaxLanes;
-    if (bytes > b->h_cap) {
-        const size_t c = std::max(bytes, 2 * b->h_cap);
-        if (b->h_stage) (void)hipHostFree(b->h_stage);
-        b->h_stage = nullptr;
-        b->h_cap = 0;
-        GB_HIP(hipHostMalloc(&b->h_stage, c, hipHostMallocDefault));
-        b->h_cap = c;
+    if (bytes > b->h_cap[sb]) {
+        const size_t c = std::max(bytes, 2 * b->h_cap[sb]);
+        if (b->h_stage[sb]) (void)hipHostFree(b->h_stage[sb]);
+        b->h_stage[sb] = nullptr;
+        b->h_cap[sb] = 0;
+        GB_HIP(hipHostMalloc(&b->h_stage[sb], c, hipHostMallocDefault));
+        b->h_cap[sb] = c;
     }
+    char *stage = b->h_stage[sb];
     bool regrow = n > b->all_cap;
     for (uint32_t l = 0; l < nlanes; ++l) regrow = regrow || lane_n[l] > b->lanes[l].launch_cap;
-    // the lanes' previous work was joined into s: when s is idle, so are they
-    if (regrow) GB_HIP(hipStreamSynchronize(s));
+    // tables grown while the previous call may still read them: wait for it
+    if (regrow) GB_HIP(hipEventSynchronize(b->stage_free[sb ^ 1u]));
     gt_status st = grow(&b->d_all, &b->all_cap, n);
     for (uint32_t l = 0; l < nlanes && st == GT_OK; ++l)
         st = grow(&b->lanes[l].d_launches, &b->lanes[l].launch_cap, lane_n[l]);
     if (st != GT_OK) return st;
-    GtBatchFrame *e = reinterpret_cast<GtBatchFrame *>(b->h_stage);
-    GtBatchLaunch *hl = reinterpret_cast<GtBatchLaunch *>(b->h_stage + sizeof(GtBatchFrame) * n);
-    uint32_t *zero = reinterpret_cast<uint32_t *>(b->h_stage + sizeof(GtBatchFrame) * n +
+    GtBatchFrame *e = reinterpret_cast<GtBatchFrame *>(stage);
+    GtBatchLaunch *hl = reinterpret_cast<GtBatchLaunch *>(stage + sizeof(GtBatchFrame) * n);
+    uint32_t *zero = reinterpret_cast<uint32_t *>(stage + sizeof(GtBatchFrame) * n +
                                                   sizeof(GtBatchLaunch) * nl);
     // the frame tables; one grid shape for every launch of the call: the tile
     // queue of its largest launch (multiple of 8) and its most frames
@@ -281,6 +291,9 @@ gt_status run_batch(gt_batch *b, const gt_frame_desc *frames, uint32_t n, int32_
     std::vector<size_t> fill(lane_off.begin(), lane_off.end() - 1);
     for (size_t k = 0; k < nl; ++k) hl[fill[k % nlanes]++] = launches[k];
     for (uint32_t l = 0; l < kMaxLanes; ++l) zero[l] = 0;
+    // the device tables are rewritten after the previous call's launches,
+    // whichever stream it ran on
+    GB_HIP(hipStreamWaitEvent(s, b->stage_free[sb ^ 1u], 0));
     GB_HIP(hipMemcpyAsync(b->d_all, e, sizeof(GtBatchFrame) * n, hipMemcpyHostToDevice, s));
     for (uint32_t l = 0; l < nlanes; ++l) {
         GB_HIP(hipMemcpyAsync(b->lanes[l].d_launches, hl + lane_off[l],
@@ -350,8 +363,9 @@ gt_status run_batch(gt_batch *b, const gt_frame_desc *frames, uint32_t n, int32_
         GB_HIP(hipEventRecord(L.done, ls));
         GB_HIP(hipStreamWaitEvent(s, L.done, 0));
     }
-    // the staging buffer and tables are free again once this call's launches are done
-    GB_HIP(hipEventRecord(b->stage_free, s));
+    // this call's staging buffer is free again, and the next call may rewrite
+    // the tables, once its launches are done
+    GB_HIP(hipEventRecord(b->stage_free[sb], s));
     b->launches = nl;
     b->last_sch = sch_all;
     b->last_max_seg = max_seg;
@@ -429,8 +443,10 @@ gt_status gt_batch_create(const gt_params *params, uint64_t max_frame_pixels,
     (void)(ok(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking), "hipStreamCreate") &&
            ok(gt_init_tables(), "table init") &&
            ok(hipEventCreateWithFlags(&b->uploaded, hipEventDisableTiming), "hipEventCreate") &&
-           ok(hipEventCreateWithFlags(&b->stage_free, hipEventDisableTiming), "hipEventCreate") &&
-           ok(hipEventRecord(b->stage_free, b->stream), "hipEventRecord") &&
+           ok(hipEventCreateWithFlags(&b->stage_free[0], hipEventDisableTiming), "hipEventCreate") &&
+           ok(hipEventCreateWithFlags(&b->stage_free[1], hipEventDisableTiming), "hipEventCreate") &&
+           ok(hipEventRecord(b->stage_free[0], b->stream), "hipEventRecord") &&
+           ok(hipEventRecord(b->stage_free[1], b->stream), "hipEventRecord") &&
            ok(hipEventCreateWithFlags(&b->ev_in, hipEventDisableTiming), "hipEventCreate") &&
            ok(hipEventCreateWithFlags(&b->ev_out, hipEventDisableTiming), "hipEventCreate"));
     if (st == GT_OK) st = alloc_lane(b, 0);

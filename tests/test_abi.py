@@ -169,3 +169,13 @@ def test_log_parameters_validated():
     p = gt.make_log_params(1.5, 40.0, smoothing_size=9)     # sigma 28: taps beyond 48
     assert N.lib().gt_context_create(C.byref(p), 64, 64, 0, C.byref(h)) == 100
     assert b"radius" in N.lib().gt_last_error()
+
+
+def test_frame_desc_record_matches_header_struct():
+    # the numpy record the batch wrapper fills is laid out as gt_frame_desc
+    import ctypes as C
+    from gpu_tracking_amd import api
+    dt = api._FRAME_DESC_DTYPE
+    assert dt.itemsize == C.sizeof(N.GtFrameDesc) == 24
+    for name, _ in N.GtFrameDesc._fields_:
+        assert dt.fields[name][1] == getattr(N.GtFrameDesc, name).offset, name

@@ -229,3 +229,23 @@ def test_batch_many_sizes_split_by_size_runs(torch_dev):
         assert sch["tile_blocks_last_call"] >= sum(tiles[:8])
         assert sch["tile_blocks_last_call"] % 8 == 0
     _check(out, counts, sep, where)
+
+
+def test_batch_videos_as_3d_tensors(torch_dev):
+    # 3-D videos in the list: their frames join the call in order, with the
+    # rows of separate one-size runs; a 2-D frame may sit between them
+    import gpu_tracking_amd as gt
+    vids = _videos(torch_dev)
+    cap = 2048
+    sep = _separate(torch_dev, vids, cap)
+    items = [vids[0], vids[1][1], vids[2], vids[3], vids[4]]
+    where = ([(0, t) for t in range(3)] + [(1, 1)] + [(2, t) for t in range(3)] +
+             [(3, t) for t in range(3)] + [(4, t) for t in range(3)])
+    ids = [t for _, t in where]
+    print(f"[batch 3-D] {len(where)} frames from {len(items)} items", flush=True)
+    with gt.Batch(gt.make_params(9, **PARAMS), max_frame_pixels=256 * 256,
+                  max_launch_pixels=1 << 20, max_slots=64) as b:
+        out, counts = b.detect(items, frame_ids=ids, cap_per_frame=cap)
+        torch_dev.cuda.synchronize()
+        assert out.shape[0] == len(where)
+    _check(out, counts, sep, where)
