@@ -1,8 +1,8 @@
 // gt_cnn.hip -- learned centroid proposal on the gfx950 matrix cores.
 //
 // One block = four waves; each wave carries one sample (candidate) through
-// the network with its activations in LDS:
-//   conv1 (1 -> 16, 9 MACs per output) on the VALU,
+// the network with its activations in its own LDS region:
+//   conv1 (M = 81 positions, N = 16, K = 9 taps padded to 32),
 //   conv2 (M = 49 positions, N = 32, K = 9 taps x 16 ch) and
 //   conv3 (M = 25, N = 32, K = 9 taps x 32 ch) as implicit-im2col GEMMs of
 //   v_mfma_f32_16x16x32_bf16 (lane l holds A row l%16, k 8*(l/16)..+8;
