@@ -728,13 +728,21 @@ gt_status gt_context_create(const gt_params *params, uint32_t nrows, uint32_t nc
         return fail(GT_ERR_GPU_DEVICE, "could not initialise the device tables");
     }
     const size_t HW = static_cast<size_t>(nrows) * ncols;
-    // Frames per chunk: bounded by a 4 GiB scratch budget and by 256 frames.
-    // per frame of device scratch: processed plane and candidate list (the
-    // two gt_execute staging slots, frame + detection buffer each, are
-    // allocated on first use)
-    const size_t per_frame = HW * (4 + 4 + (c->log && der.threshold_rel > 0.0f ? 4 : 0));
+    // Frames per chunk: bounded by a 4 GiB scratch budget and by 512 frames
+    // (longer chunks spread the slow walks' and characterize's tails over
+    // more frames).  Per frame of device scratch, in bytes per pixel: the
+    // processed plane (4); the slow-walk list (1: HW/8 entries of 8 B) on the
+    // tiled path, else the candidate list (4) and, with threshold_rel, its
+    // values (4); adaptive_background's masked plane (4); illumination
+    // correction's planes (8, 12 per frame).  The two gt_execute staging
+    // slots, frame + detection buffer each, are allocated on first use.
+    size_t bpp = 4 + (c->tiled && !c->log ? 1 : 4);
+    if (c->log && der.threshold_rel > 0.0f) bpp += 4;
+    if (der.adaptive) bpp += 4;
+    if (der.illum_sigma > 0.0f) bpp += der.illum_per_frame ? 12 : 8;
+    const size_t per_frame = HW * bpp;
     size_t chunk = (size_t(4) << 30) / (per_frame ? per_frame : 1);
-    c->chunk = static_cast<uint32_t>(std::max<size_t>(1, std::min<size_t>(chunk, 256)));
+    c->chunk = static_cast<uint32_t>(std::max<size_t>(1, std::min<size_t>(chunk, 512)));
     // host frames are staged 64 at a time: smaller pinned slots, and the
     // first slot of a video starts on the GPU sooner
     c->xchunk = std::min<uint32_t>(c->chunk, 64);

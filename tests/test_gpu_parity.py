@@ -178,13 +178,13 @@ def test_capacity_overflow_reported(torch_dev):
 
 def test_two_lanes_give_the_same_rows(torch_dev, oracle):
     # chunks alternating between two streams with their own scratch, and the
-    # chunk tails on a second stream (overlap).  A context takes up to 256
-    # frames per chunk: 800 frames of 64 x 96 make four chunks, so both lanes
+    # chunk tails on a second stream (overlap).  A context takes up to 512
+    # frames per chunk: 1600 frames of 64 x 96 make four chunks, so both lanes
     # and the overlap really run; frames of chunks 1..3 are also checked
     # against the oracle.
     import gpu_tracking_amd as gt
     from gpu_tracking_amd import synth
-    H, W, T = 64, 96, 800
+    H, W, T = 64, 96, 1600
     v = synth.video_numpy(synth.spec("full", nrows=H, ncols=W, n_particles=8), T, 91)
     d = torch_dev.from_numpy(v.view(np.int16)).cuda()
     res = []
@@ -196,12 +196,12 @@ def test_two_lanes_give_the_same_rows(torch_dev, oracle):
             out, counts = ctx.detect_device(d, cap_per_frame=512)
             torch_dev.cuda.synchronize()
             ms, launches, per_launch = ctx.stage_times()
-            assert (launches, per_launch) == (4, 256), (launches, per_launch)
+            assert (launches, per_launch) == (4, 512), (launches, per_launch)
             res.append(gt.detections_to_numpy(out, counts))
     assert sum(len(r) for r in res[0]) > 0
     for other in res[1:]:
         for a, b in zip(res[0], other):
             assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
-    for t in (300, 555, 799):
+    for t in (600, 1111, 1599):
         ref, _ = _oracle_rows(oracle, v[t], diameter=9, separation=7, minmass=300.0)
         _assert_rows_equal(res[1][t], ref, f"frame {t}")
