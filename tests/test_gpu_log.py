@@ -126,6 +126,20 @@ def test_log_overlap_threshold_variants(torch_dev, oracle):
         assert np.array_equal(df["mass"].to_numpy(np.float32), ref["mass"])
 
 
+def test_log_tile_buffer_spill_bit_identical(torch_dev, oracle):
+    # pure noise at 20 small scales: about 1360 scale-space maxima per
+    # 128 x 16 tile, more than a tile buffers, so candidates spill to the top
+    # of the list and are walked from global memory; the rest walk from the
+    # tiles' staged planes.  Rows equal the oracle's either way.
+    rng = np.random.default_rng(5)
+    frame = (50.0 + rng.normal(0.0, 5.0, (64, 256))).astype(np.float32)
+    print("[log spill] 64x256 noise, 20 scales", flush=True)
+    ref, nc = _oracle_rows(oracle, frame, 1.0, 3.0, n_radii=20)
+    assert nc > 8 * 1024
+    got = _gpu_rows(torch_dev, frame[None], 1.0, 3.0, n_radii=20, log_spacing=False)
+    _assert_rows_equal(got[0], ref, "spill")
+
+
 def test_log_capacity_overflow_reported(torch_dev):
     video = _video(1, 9, H=128, W=128, n=20)
     import gpu_tracking_amd as gt
