@@ -89,14 +89,29 @@ struct alignas(16) WaveGrad {
     float dz1[81 * 16];      // [q][c]
 };
 
-constexpr size_t kFwdLds = sizeof(BlockW) + 4 * sizeof(WaveAct);
-constexpr size_t kTrainLds = kFwdLds + 4 * sizeof(WaveGrad) + 32 * sizeof(float);
+// The forward kernel's staged weights: only the f32 tensors (conv1, the
+// biases, the head); its conv2 / conv3 B fragments are read from global
+// memory straight into registers.  Keeps its LDS at about 43 KB: three
+// blocks per CU.
+struct alignas(16) FwdW {
+    float f[CNN_F_N];
+};
+
+constexpr size_t kFwdLds = sizeof(FwdW) + 4 * sizeof(WaveAct);
+constexpr size_t kTrainActs = sizeof(BlockW) + 4 * sizeof(WaveAct);   // the train kernel's
+constexpr size_t kTrainLds = kTrainActs + 4 * sizeof(WaveGrad) + 32 * sizeof(float);
 
 __device__ __forceinline__ void stage_weights(const uint8_t *__restrict__ pk, BlockW &B, int tid) {
     constexpr int NV = static_cast<int>((CNN_PK_W2T - CNN_PK_W2P) / 16);  // w2p and w3p
     const u32x4 *src = reinterpret_cast<const u32x4 *>(pk + CNN_PK_W2P);
     u32x4 *dst = reinterpret_cast<u32x4 *>(B.w2p);
     for (int i = tid; i < NV; i += CT) dst[i] = src[i];
+    const float *fs = reinterpret_cast<const float *>(pk + CNN_PK_F32);
+    for (int i = tid; i < CNN_F_N; i += CT) B.f[i] = fs[i];
+}
+
+__device__ __forceinline__ void stage_fwd_weights(const uint8_t *__restrict__ pk, FwdW &B,
+                                                  int tid) {
     const float *fs = reinterpret_cast<const float *>(pk + CNN_PK_F32);
     for (int i = tid; i < CNN_F_N; i += CT) B.f[i] = fs[i];
 }
@@ -143,7 +158,8 @@ __device__ __forceinline__ void gather_norm(const float *__restrict__ P, int H, 
 // 32) on the matrix cores: lane l's A row is position m = 16 mt + l % 16 with
 // taps 8 (l / 16) .. + 8 of its 3 x 3 window, its B column channel l % 16
 // with the same taps of w1 (bf16); the bias is added to the f32 sum.
-__device__ __forceinline__ void conv1(const BlockW &B, WaveAct &A, int lane) {
+template <class WB>
+__device__ __forceinline__ void conv1(const WB &B, WaveAct &A, int lane) {
     const int r16 = lane & 15, kg = lane >> 4;
     uint32_t wv[8];
 #pragma unroll
@@ -237,24 +253,30 @@ struct WeightFrags {
     bf16x8 w2[5][2];
     bf16x8 w3[9][2];
 };
-__device__ __forceinline__ void load_weight_frags(const BlockW &B, int lane, WeightFrags &F) {
+// The lane's conv2 / conv3 B fragments from the packed weights in global
+// memory (the w2p / w3p layouts stage_weights copies to LDS).
+__device__ __forceinline__ void load_weight_frags(const uint8_t *__restrict__ pk, int lane,
+                                                  WeightFrags &F) {
+    const uint16_t *w2p = reinterpret_cast<const uint16_t *>(pk + CNN_PK_W2P);
+    const uint16_t *w3p = reinterpret_cast<const uint16_t *>(pk + CNN_PK_W3P);
     const int r16 = lane & 15, kg = lane >> 4;
 #pragma unroll
     for (int st = 0; st < 5; ++st) {
         const int k0 = st * 32 + kg * 8;
-        F.w2[st][0] = frag_ld(&B.w2p[r16 * CNN_K2 + k0]);
-        F.w2[st][1] = frag_ld(&B.w2p[(16 + r16) * CNN_K2 + k0]);
+        F.w2[st][0] = frag_ld(&w2p[r16 * CNN_K2 + k0]);
+        F.w2[st][1] = frag_ld(&w2p[(16 + r16) * CNN_K2 + k0]);
     }
 #pragma unroll
     for (int st = 0; st < 9; ++st) {
-        F.w3[st][0] = frag_ld(&B.w3p[r16 * CNN_K3 + st * 32 + kg * 8]);
-        F.w3[st][1] = frag_ld(&B.w3p[(16 + r16) * CNN_K3 + st * 32 + kg * 8]);
+        F.w3[st][0] = frag_ld(&w3p[r16 * CNN_K3 + st * 32 + kg * 8]);
+        F.w3[st][1] = frag_ld(&w3p[(16 + r16) * CNN_K3 + st * 32 + kg * 8]);
     }
 }
 
 // conv2 and conv3 as above with the B operands from registers (the same
 // products in the same order).
-__device__ __forceinline__ void conv2_rf(const BlockW &B, const WeightFrags &F, WaveAct &A,
+template <class WB>
+__device__ __forceinline__ void conv2_rf(const WB &B, const WeightFrags &F, WaveAct &A,
                                          int lane) {
     const int r16 = lane & 15, kg = lane >> 4;
 #pragma unroll
@@ -283,7 +305,8 @@ __device__ __forceinline__ void conv2_rf(const BlockW &B, const WeightFrags &F, 
         }
     }
 }
-__device__ __forceinline__ void conv3_rf(const BlockW &B, const WeightFrags &F, WaveAct &A,
+template <class WB>
+__device__ __forceinline__ void conv3_rf(const WB &B, const WeightFrags &F, WaveAct &A,
                                          int lane) {
     const int r16 = lane & 15, kg = lane >> 4;
 #pragma unroll
@@ -312,7 +335,8 @@ __device__ __forceinline__ void conv3_rf(const BlockW &B, const WeightFrags &F, 
 }
 
 // Mean pooling and the linear head; every lane gets o[0..2].
-__device__ __forceinline__ void head(const BlockW &B, WaveAct &A, int lane, float (&o)[3]) {
+template <class WB>
+__device__ __forceinline__ void head(const WB &B, WaveAct &A, int lane, float (&o)[3]) {
     float g = 0.0f;
     if (lane < 32) {
         for (int m = 0; m < 25; ++m) g = g + A.a3[m * 32 + lane];
@@ -455,10 +479,10 @@ __global__ __launch_bounds__(CT, 2) void k_cnn_forward(const uint8_t *__restrict
                                                        const uint32_t *__restrict__ ccount,
                                                        float4 *__restrict__ out) {
     extern __shared__ __align__(16) uint8_t smem[];
-    BlockW &B = *reinterpret_cast<BlockW *>(smem);
-    WaveAct *Aall = reinterpret_cast<WaveAct *>(smem + sizeof(BlockW));
+    FwdW &B = *reinterpret_cast<FwdW *>(smem);
+    WaveAct *Aall = reinterpret_cast<WaveAct *>(smem + sizeof(FwdW));
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-    stage_weights(pk, B, tid);
+    stage_fwd_weights(pk, B, tid);
     uint32_t count = n;
     const float *P = proc;
     const uint32_t *fc = cand;
@@ -475,7 +499,7 @@ __global__ __launch_bounds__(CT, 2) void k_cnn_forward(const uint8_t *__restrict
     const uint32_t iters = (count + per - 1) / per;   // uniform across the block
     __syncthreads();
     WeightFrags F;
-    load_weight_frags(B, lane, F);
+    load_weight_frags(pk, lane, F);
     // sample of iteration it: j(it); its inputs are loaded one iteration
     // ahead (frames: the candidate pixel two ahead), so the loads' latency
     // overlaps the previous sample's network
@@ -554,8 +578,8 @@ __global__ __launch_bounds__(CT, 1) void k_cnn_train(const uint8_t *__restrict__
     extern __shared__ __align__(16) uint8_t smem[];
     BlockW &B = *reinterpret_cast<BlockW *>(smem);
     WaveAct *Aall = reinterpret_cast<WaveAct *>(smem + sizeof(BlockW));
-    WaveGrad *Gall = reinterpret_cast<WaveGrad *>(smem + kFwdLds);
-    float *s_do = reinterpret_cast<float *>(smem + kFwdLds + 4 * sizeof(WaveGrad));  // [4][4]
+    WaveGrad *Gall = reinterpret_cast<WaveGrad *>(smem + kTrainActs);
+    float *s_do = reinterpret_cast<float *>(smem + kTrainActs + 4 * sizeof(WaveGrad));  // [4][4]
     float *s_loss = s_do + 16;
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     stage_weights(pk, B, tid);
