@@ -1,5 +1,5 @@
 // gt_linker.cpp -- the Linker of src/linking.rs (:346-521, link at :564-816)
-// in C++, with a uniform-grid neighbour search in place of the kd-tree.
+// in C++, with a search over strips of rows in place of the kd-tree.
 //
 // A source's candidate destinations are listed in ascending destination
 // index (the reference lists them in kd-tree order); that fixes the
@@ -11,14 +11,6 @@
 #include <cmath>
 #include <limits>
 
-namespace {
-
-int64_t cell_of(float v, double cell) {
-    return static_cast<int64_t>(std::floor(static_cast<double>(v) / cell));
-}
-
-}  // namespace
-
 gt_linker::gt_linker(float search_range, uint32_t memory)
     : radius_(search_range), memory_(memory) {
     for (uint32_t i = 0; i < memory_; ++i) memvec_.emplace_back();
@@ -29,94 +21,151 @@ gt_status gt_linker::link(size_t memory_start, uint64_t *ids, std::vector<Entry>
     const std::vector<Entry> &src = prev_;
     const size_t ns = src.size(), nd = dest_.size();
     const float r2 = radius_ * radius_;
-    s2d_.assign(ns, {});
-    d2s_.assign(nd, {});
+    s2d_off_.assign(ns + 1, 0u);
+    s2d_.clear();
     // candidate pairs: squared distance (0 + dx^2) + dy^2 < r^2, dx = dest -
-    // source (within_radius_rd2, linking.rs:208-247); grid cells slightly
-    // wider than the radius so rounding cannot hide a pair one cell further
+    // source (within_radius_rd2, linking.rs:208-247).  The destinations are
+    // ordered by strip (floor of row / w, w slightly wider than the radius so
+    // rounding cannot hide a pair one strip further) and then by index; a
+    // source's partners lie in the one run covering its strip and the two
+    // beside it.
     if (ns && nd && radius_ > 0.0f) {
-        const double cell = static_cast<double>(radius_) * 1.0001;
-        std::unordered_map<int64_t, std::vector<uint32_t>> grid;
-        grid.reserve(nd);
-        auto key = [](int64_t a, int64_t b) { return (a << 32) ^ (b & 0xFFFFFFFFll); };
+        const double inv = 1.0 / (static_cast<double>(radius_) * 1.0001);
+        double kmin = std::numeric_limits<double>::infinity(), kmax = -kmin;
+        dkey_.resize(nd);
+        size_t nfin = 0;
         for (size_t d = 0; d < nd; ++d) {
-            if (!std::isfinite(dest_[d].x) || !std::isfinite(dest_[d].y)) continue;
-            grid[key(cell_of(dest_[d].x, cell), cell_of(dest_[d].y, cell))].push_back(
-                static_cast<uint32_t>(d));
+            if (!std::isfinite(dest_[d].x) || !std::isfinite(dest_[d].y)) {
+                dkey_[d] = std::numeric_limits<double>::quiet_NaN();
+                continue;
+            }
+            const double k = std::floor(static_cast<double>(dest_[d].x) * inv);
+            dkey_[d] = k;
+            kmin = std::min(kmin, k);
+            kmax = std::max(kmax, k);
+            ++nfin;
+        }
+        strip_.resize(nfin);
+        const bool dense = nfin && kmax - kmin <= 4.0 * static_cast<double>(nd) + 4096.0;
+        if (dense) {
+            // counting sort by strip; strip k is strip_[soff_[k - kmin] .. soff_[k - kmin + 1])
+            const size_t S = static_cast<size_t>(kmax - kmin) + 1;
+            soff_.assign(S + 1, 0u);
+            for (size_t d = 0; d < nd; ++d)
+                if (dkey_[d] == dkey_[d]) ++soff_[static_cast<size_t>(dkey_[d] - kmin) + 1];
+            for (size_t k = 0; k < S; ++k) soff_[k + 1] += soff_[k];
+            fill_.assign(soff_.begin(), soff_.end() - 1);
+            for (size_t d = 0; d < nd; ++d)
+                if (dkey_[d] == dkey_[d])
+                    strip_[fill_[static_cast<size_t>(dkey_[d] - kmin)]++] = {
+                        dest_[d].x, dest_[d].y, static_cast<uint32_t>(d)};
+        } else {
+            order_.clear();
+            for (size_t d = 0; d < nd; ++d)
+                if (dkey_[d] == dkey_[d]) order_.emplace_back(dkey_[d], static_cast<uint32_t>(d));
+            std::sort(order_.begin(), order_.end());
+            skey_.resize(nfin);
+            for (size_t i = 0; i < nfin; ++i) {
+                const uint32_t d = order_[i].second;
+                skey_[i] = order_[i].first;
+                strip_[i] = {dest_[d].x, dest_[d].y, d};
+            }
         }
         for (size_t s = 0; s < ns; ++s) {
-            if (!std::isfinite(src[s].x) || !std::isfinite(src[s].y)) continue;
-            const int64_t cx = cell_of(src[s].x, cell), cy = cell_of(src[s].y, cell);
-            auto &lst = s2d_[s];
-            for (int64_t a = -1; a <= 1; ++a)
-                for (int64_t b = -1; b <= 1; ++b) {
-                    auto it = grid.find(key(cx + a, cy + b));
-                    if (it == grid.end()) continue;
-                    for (uint32_t d : it->second) {
-                        const float dx = dest_[d].x - src[s].x, dy = dest_[d].y - src[s].y;
-                        float d2 = 0.0f;
-                        d2 = d2 + dx * dx;
-                        d2 = d2 + dy * dy;
-                        if (d2 < r2) lst.emplace_back(d, d2);
-                    }
-                }
-            std::sort(lst.begin(), lst.end(),
-                      [](const std::pair<uint32_t, float> &a, const std::pair<uint32_t, float> &b) {
-                          return a.first < b.first;
-                      });
-            for (const auto &e : lst) d2s_[e.first].emplace_back(static_cast<uint32_t>(s), e.second);
+            s2d_off_[s] = static_cast<uint32_t>(s2d_.size());
+            if (!nfin || !std::isfinite(src[s].x) || !std::isfinite(src[s].y)) continue;
+            const double k = std::floor(static_cast<double>(src[s].x) * inv);
+            size_t lo, hi;
+            if (dense) {
+                const double a = std::max(k - 1.0, kmin), b = std::min(k + 1.0, kmax);
+                if (a > b) continue;
+                lo = soff_[static_cast<size_t>(a - kmin)];
+                hi = soff_[static_cast<size_t>(b - kmin) + 1];
+            } else {
+                lo = static_cast<size_t>(std::lower_bound(skey_.begin(), skey_.end(), k - 1.0) -
+                                         skey_.begin());
+                hi = static_cast<size_t>(std::upper_bound(skey_.begin(), skey_.end(), k + 1.0) -
+                                         skey_.begin());
+            }
+            near_.clear();
+            for (size_t q = lo; q < hi; ++q) {
+                const float dx = strip_[q].x - src[s].x, dy = strip_[q].y - src[s].y;
+                float d2 = 0.0f;
+                d2 = d2 + dx * dx;
+                d2 = d2 + dy * dy;
+                if (d2 < r2) near_.push_back({strip_[q].idx, d2});
+            }
+            // ascending destination index (a handful of entries)
+            for (size_t i = 1; i < near_.size(); ++i)
+                for (size_t j = i; j > 0 && near_[j - 1].idx > near_[j].idx; --j)
+                    std::swap(near_[j - 1], near_[j]);
+            s2d_.insert(s2d_.end(), near_.begin(), near_.end());
         }
     }
+    s2d_off_[ns] = static_cast<uint32_t>(s2d_.size());
+    // each destination's sources, ascending
+    d2s_off_.assign(nd + 1, 0u);
+    for (const Cand &c : s2d_) ++d2s_off_[c.idx + 1];
+    for (size_t d = 0; d < nd; ++d) d2s_off_[d + 1] += d2s_off_[d];
+    d2s_.resize(s2d_.size());
+    fill_.assign(nd, 0u);
+    for (size_t s = 0; s < ns; ++s)
+        for (uint32_t q = s2d_off_[s]; q < s2d_off_[s + 1]; ++q) {
+            const Cand &c = s2d_[q];
+            d2s_[d2s_off_[c.idx] + fill_[c.idx]++] = {static_cast<uint32_t>(s), c.d2};
+        }
     // connected components by depth-first search from each destination, in
     // the visiting order of the reference's recursion
     vis_src_.assign(ns, 0);
     vis_dest_.assign(nd, 0);
     out_.assign(nd, -1);
-    std::vector<std::vector<uint32_t>> subnets;   // destinations of each subnetwork
+    out_src_.assign(nd, -1);
+    sub_dests_.clear();
+    sub_off_.assign(1, 0u);
     size_t load = 0;
-    struct Frame {
-        uint32_t node;
-        uint8_t kind;   // 0 source, 1 destination
-        uint32_t next;
-    };
-    std::vector<Frame> stack;
-    std::vector<uint32_t> psrc, pdest;
+    stack_.clear();
     for (size_t i = 0; i < nd; ++i) {
-        psrc.clear();
-        pdest.clear();
+        psrc_.clear();
+        pdest_.clear();
         if (!vis_dest_[i]) {
             vis_dest_[i] = 1;
-            pdest.push_back(static_cast<uint32_t>(i));
-            stack.push_back({static_cast<uint32_t>(i), 1, 0});
-            while (!stack.empty()) {
-                Frame &f = stack.back();
+            pdest_.push_back(static_cast<uint32_t>(i));
+            stack_.push_back({static_cast<uint32_t>(i), 1, 0});
+            while (!stack_.empty()) {
+                Frame &f = stack_.back();
                 if (f.kind == 1) {
-                    const auto &adj = d2s_[f.node];
-                    if (f.next >= adj.size()) { stack.pop_back(); continue; }
-                    const uint32_t s = adj[f.next++].first;
+                    const uint32_t lo = d2s_off_[f.node], n = d2s_off_[f.node + 1] - lo;
+                    if (f.next >= n) { stack_.pop_back(); continue; }
+                    const uint32_t s = d2s_[lo + f.next++].idx;
                     if (!vis_src_[s]) {
                         vis_src_[s] = 1;
-                        psrc.push_back(s);
-                        stack.push_back({s, 0, 0});
+                        psrc_.push_back(s);
+                        stack_.push_back({s, 0, 0});
                     }
                 } else {
-                    const auto &adj = s2d_[f.node];
-                    if (f.next >= adj.size()) { stack.pop_back(); continue; }
-                    const uint32_t d = adj[f.next++].first;
+                    const uint32_t lo = s2d_off_[f.node], n = s2d_off_[f.node + 1] - lo;
+                    if (f.next >= n) { stack_.pop_back(); continue; }
+                    const uint32_t d = s2d_[lo + f.next++].idx;
                     if (!vis_dest_[d]) {
                         vis_dest_[d] = 1;
-                        pdest.push_back(d);
-                        stack.push_back({d, 1, 0});
+                        pdest_.push_back(d);
+                        stack_.push_back({d, 1, 0});
                     }
                 }
             }
         }
-        if (pdest.size() == 1) {
-            if (psrc.size() == 1) out_[i] = static_cast<int64_t>(src[psrc[0]].id);
-            else if (psrc.size() > 1) subnets.push_back(pdest);
-        } else if (pdest.size() > 1) {
-            load += psrc.size() * pdest.size();
-            subnets.push_back(pdest);
+        if (pdest_.size() == 1) {
+            if (psrc_.size() == 1) {
+                out_[i] = static_cast<int64_t>(src[psrc_[0]].id);
+                out_src_[i] = psrc_[0];
+            } else if (psrc_.size() > 1) {
+                sub_dests_.insert(sub_dests_.end(), pdest_.begin(), pdest_.end());
+                sub_off_.push_back(static_cast<uint32_t>(sub_dests_.size()));
+            }
+        } else if (pdest_.size() > 1) {
+            load += psrc_.size() * pdest_.size();
+            sub_dests_.insert(sub_dests_.end(), pdest_.begin(), pdest_.end());
+            sub_off_.push_back(static_cast<uint32_t>(sub_dests_.size()));
         }
     }
     if (load > 5000) {
@@ -126,70 +175,72 @@ gt_status gt_linker::link(size_t memory_start, uint64_t *ids, std::vector<Entry>
     }
     // each subnetwork: exhaustive search for the assignment of minimum total
     // squared distance, r^2 for a destination left unlinked
-    for (const auto &dests : subnets) {
-        used_.clear();
-        nulls_.clear();
+    for (size_t k = 0; k + 1 < sub_off_.size(); ++k) {
+        cur_.clear();
         score_ = 0.0f;
         best_ = std::numeric_limits<float>::infinity();
         interrupted_ = false;
-        solve(0, dests, interrupt);
+        solve(0, sub_dests_.data() + sub_off_[k], sub_off_[k + 1] - sub_off_[k], interrupt);
         if (interrupted_) {
             *err = "Keyboard Interrupt, my custom version";
             return GT_ERR_INTERRUPTED;
         }
     }
-    // unlinked destinations start new particles, in destination order
-    std::unordered_set<uint64_t> taken;
-    taken.reserve(nd * 2 + 1);
+    // unlinked destinations start new particles, in destination order; the
+    // sources nobody linked to (ids are unique among the sources) are unused
+    taken_.assign(ns, 0);
     for (size_t d = 0; d < nd; ++d) {
         if (out_[d] < 0) out_[d] = static_cast<int64_t>(part_idx_++);
+        else taken_[static_cast<size_t>(out_src_[d])] = 1;
         ids[d] = static_cast<uint64_t>(out_[d]);
-        taken.insert(ids[d]);
     }
     unused->clear();
     const size_t lim = std::min(memory_start, ns);
     for (size_t s = 0; s < lim; ++s)
-        if (!taken.count(src[s].id)) unused->push_back(src[s]);
+        if (!taken_[s]) unused->push_back(src[s]);
     return GT_OK;
 }
 
-// recurse2 (linking.rs:694-774)
-bool gt_linker::solve(size_t progress, const std::vector<uint32_t> &dests,
+// recurse2 (linking.rs:694-774); cur_ holds the partial assignment (a
+// destination and its source, or -1 for none), and a better complete one
+// is written to out_ / out_src_
+bool gt_linker::solve(size_t progress, const uint32_t *dests, size_t nd_sub,
                       std::atomic<int32_t> *interrupt) {
     const uint32_t cur = dests[progress];
-    const auto &cands = d2s_[cur];
+    const uint32_t lo = d2s_off_[cur], nc = d2s_off_[cur + 1] - lo;
     const float r2 = radius_ * radius_;
-    for (size_t c = 0; c <= cands.size(); ++c) {
+    for (uint32_t c = 0; c <= nc; ++c) {
         if (interrupt && interrupt->load(std::memory_order_relaxed)) {
             interrupted_ = true;
             return false;
         }
         if (score_ > best_) return true;
-        const bool is_null = c == cands.size();
-        const float cost = is_null ? r2 : cands[c].second;
+        const bool is_null = c == nc;
+        const float cost = is_null ? r2 : d2s_[lo + c].d2;
         score_ = score_ + cost;
-        uint64_t ident = 0;
+        int64_t s = -1;
         if (!is_null) {
-            ident = prev_[cands[c].first].id;
-            if (used_.count(ident)) {
+            s = d2s_[lo + c].idx;
+            bool used = false;
+            for (const auto &e : cur_) used = used || e.second == s;
+            if (used) {
                 score_ = score_ - cost;
                 continue;
             }
-            used_.emplace(ident, cur);
-        } else {
-            nulls_.insert(cur);
         }
-        if (progress + 1 == dests.size()) {
+        cur_.emplace_back(cur, s);
+        if (progress + 1 == nd_sub) {
             if (score_ < best_) {
                 best_ = score_;
-                for (const auto &kv : used_) out_[kv.second] = static_cast<int64_t>(kv.first);
-                for (uint32_t v : nulls_) out_[v] = -1;
+                for (const auto &e : cur_) {
+                    out_[e.first] = e.second >= 0 ? static_cast<int64_t>(prev_[e.second].id) : -1;
+                    out_src_[e.first] = e.second;
+                }
             }
         } else {
-            if (!solve(progress + 1, dests, interrupt)) return false;
+            if (!solve(progress + 1, dests, nd_sub, interrupt)) return false;
         }
-        if (!is_null) used_.erase(ident);
-        else nulls_.erase(cur);
+        cur_.pop_back();
         score_ = score_ - cost;
     }
     return true;
@@ -200,12 +251,17 @@ gt_status gt_linker::advance(const float *xy, size_t n, uint64_t *ids,
     size_t memory_start = static_cast<size_t>(-1);
     if (memory_ > 0) {
         // remembered particles join the sources unless already among them
-        std::unordered_set<uint64_t> seen;
-        seen.reserve(prev_.size() * 2 + 1);
-        for (const auto &e : prev_) seen.insert(e.id);
+        // (every id is below part_idx_; a particle's mark is the tag of the
+        // last frame that listed it)
+        if (mark_.size() < part_idx_) mark_.resize(part_idx_, 0);
+        const uint64_t tag = ++mark_tag_;
+        for (const auto &e : prev_) mark_[e.id] = tag;
         for (const auto &vec : memvec_)
             for (const auto &e : vec)
-                if (seen.insert(e.id).second) prev_.push_back(e);
+                if (mark_[e.id] != tag) {
+                    mark_[e.id] = tag;
+                    prev_.push_back(e);
+                }
         memory_start = prev_n_;
     }
     dest_.resize(n);

@@ -6,8 +6,6 @@
 #include <cstdint>
 #include <deque>
 #include <string>
-#include <unordered_map>
-#include <unordered_set>
 #include <vector>
 
 #include "gtrack.h"
@@ -44,7 +42,7 @@ class gt_linker {
   private:
     gt_status link(size_t memory_start, uint64_t *ids, std::vector<Entry> *unused,
                    std::atomic<int32_t> *interrupt, std::string *err);
-    bool solve(size_t progress, const std::vector<uint32_t> &dests,
+    bool solve(size_t progress, const uint32_t *dests, size_t nd_sub,
                std::atomic<int32_t> *interrupt);
 
     float radius_;
@@ -56,14 +54,45 @@ class gt_linker {
     uint64_t part_idx_ = 0;
     std::vector<Book> books_;
 
-    // per-link scratch
+    // per-link scratch, kept across frames (no per-frame allocation once the
+    // largest frame has been seen)
+    struct Cand {
+        uint32_t idx;   // the other end (destination of a source, or source of a destination)
+        float d2;       // squared distance
+    };
     std::vector<Entry> dest_;
-    std::vector<std::vector<std::pair<uint32_t, float>>> s2d_, d2s_;
-    std::vector<uint8_t> vis_src_, vis_dest_;
+    // destinations ordered by strip of rows, then index: dkey_ (strip of each
+    // destination), strip_ (the ordered points), soff_ (each strip's run when
+    // the strips are few) or skey_ (the ordered strips, else)
+    struct Pt {
+        float x, y;
+        uint32_t idx;
+    };
+    std::vector<double> dkey_, skey_;
+    std::vector<Pt> strip_;
+    std::vector<uint32_t> soff_;
+    std::vector<std::pair<double, uint32_t>> order_;
+    // candidate lists in compressed rows: s2d_ (per source, ascending
+    // destination) and d2s_ (per destination, ascending source)
+    std::vector<uint32_t> s2d_off_, d2s_off_, fill_;
+    std::vector<Cand> s2d_, d2s_;
+    std::vector<Cand> near_;
+    std::vector<uint8_t> vis_src_, vis_dest_, taken_;
     std::vector<int64_t> out_;            // particle id per destination, -1 = none
-    // subnetwork search state
+    std::vector<int64_t> out_src_;        // source linked to each destination, -1 = none
+    std::vector<uint32_t> sub_dests_, sub_off_;   // subnetworks: their destinations
+    std::vector<uint32_t> psrc_, pdest_;
+    struct Frame {
+        uint32_t node;
+        uint8_t kind;   // 0 source, 1 destination
+        uint32_t next;
+    };
+    std::vector<Frame> stack_;
+    std::vector<uint64_t> mark_;   // per particle id: tag of the frame that last listed it
+    uint64_t mark_tag_ = 0;
+    // subnetwork search state: the current partial assignment (destination,
+    // source or -1 for none) and the best one so far
     float score_ = 0.0f, best_ = 0.0f;
-    std::unordered_map<uint64_t, uint32_t> used_;
-    std::unordered_set<uint32_t> nulls_;
+    std::vector<std::pair<uint32_t, int64_t>> cur_, best_asg_;
     bool interrupted_ = false;
 };

@@ -137,6 +137,25 @@ def test_gt_link_matches_oracle(seed, n, T, step, blink, sr, mem):
     assert ids.tolist() == ref
 
 
+@pytest.mark.parametrize("seed,box,shift,sr", [
+    (10, 2.0e5, 0.0, 3.0),       # strips far more than points: the sorted-strip search
+    (11, 120.0, -5000.0, 4.0),   # negative rows and columns
+    (12, 120.0, 0.0, 0.05),      # a radius much smaller than the spacing
+    (13, 60.0, 0.0, 1.0e4),      # a radius spanning the field: one strip
+])
+def test_gt_link_matches_oracle_strip_layouts(seed, box, shift, sr):
+    from gpu_tracking_amd import _native as N
+    rng = np.random.default_rng(seed)
+    n = 6 if sr > 100 else 40
+    frames, _ = _tracks(rng, n, 8, 1.0, 0.1, box=box)
+    rows = _as_rows(frames)
+    rows[:, 1:] += np.float32(shift)
+    ref = OL.link_frames(rows, sr, 1)
+    ids = np.zeros(len(rows), np.uint64)
+    N.check(N.lib().gt_link(rows.ctypes.data, len(rows), sr, 1, ids.ctypes.data))
+    assert ids.tolist() == ref
+
+
 def test_gt_link_groups_and_errors():
     import gpu_tracking_amd as gt
     import pandas as pd
