@@ -175,6 +175,38 @@ def _fp64_error(video_host, params, gpu_rows, nframes=8):
                          "(the f32 rows are bit-identical to the oracle's)"}
 
 
+def _e2e_oracle_check(video_host, params, data, names, search_range, memory, nframes=2):
+    """The end-to-end rows of the first frames (detect, characterize,
+    filter_close, doughnut) and their particle ids against the oracle's: a
+    frame's ids depend only on it and the frames before it.  Checker code of
+    the CPU-baseline leg."""
+    import numpy as np
+
+    import oracle
+    from oracle import linker as OL
+    T, H, W = video_host.shape
+    d = params["diameter"]
+    op = oracle.derive_trackpy(H, W, diameter=d, separation=params["separation"],
+                               minmass=params["minmass"])
+    ref = []
+    for t in range(nframes):
+        rows, _ = oracle.detect_frame(video_host[t], op, characterize=True)
+        rows = np.sort(rows, order="seed")
+        rows = rows[oracle.filter_close(rows, float(params["separation"]))]
+        dn = oracle.doughnut(video_host[t], rows, d / 2.0, float(d), 0.0)
+        ref.append(np.stack([np.full(len(rows), t, np.float32), rows["x"], rows["y"],
+                             rows["mass"], rows["Rg"], rows["raw_mass"], rows["signal"],
+                             rows["ecc"], dn[:, 0], dn[:, 1], dn[:, 2]], 1).astype(np.float32))
+    ref = np.concatenate(ref, 0)
+    got = data[data[:, list(names).index("frame")] < nframes]
+    ids = got[:, list(names).index("particle")].astype(np.int64).tolist()
+    same_rows = got.shape[0] == ref.shape[0] and np.array_equal(
+        np.ascontiguousarray(got[:, :ref.shape[1]]).view(np.uint32), ref.view(np.uint32))
+    same_ids = same_rows and ids == OL.link_frames(ref[:, :3], search_range, memory)
+    return {"frames": nframes, "rows": int(ref.shape[0]), "rows_bit_identical": bool(same_rows),
+            "particle_ids_identical": bool(same_ids)}
+
+
 def _log_taps_per_pixel(params):
     """Sum over the scales of 2R+1, R = i32(4 sigma + 0.5), sigma = r / sqrt(2)
     (the radius scan of gpu_setup.rs:958-984 in f32)."""
@@ -707,7 +739,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--batch-scale", type=float, default=1.0,
                     help="batch workload: scale of the video lengths")
-    ap.add_argument("--e2e-frames", type=int, default=256,
+    ap.add_argument("--e2e-frames", type=int, default=1024,
                     help="frames of the end-to-end run (host frames in, DataFrame columns out, "
                          "with linking; plain workloads only; 0 = skip)")
     ap.add_argument("--learned-samples", type=int, default=2_000_000,
@@ -916,22 +948,27 @@ def main():
                "rows": int(data.shape[0]), "columns": list(names),
                "includes": "host frames -> pinned staging -> H2D -> detection -> D2H -> "
                            "filter_close, doughnut, linking (search_range 5, memory 2)"}
+        e2e_sample = (host[:2].copy(), data, list(names))
         print(f"[bench] end to end {e2e['value']} frames/s", file=sys.stderr, flush=True)
-        # the same frames from a multi-page u16 TIFF (read into the pinned
-        # staging by the library's reader threads; the file is in the page
-        # cache after it is written)
+        # the first frames (at most 256) from a multi-page u16 TIFF, read into
+        # the pinned staging by the library's reader threads (the file is in
+        # the page cache after it is written); every column but duration,
+        # which depends on later frames, equals the array run's
         import tempfile
+        nt = min(n, 256)
         with tempfile.TemporaryDirectory() as td:
             path = os.path.join(td, "bench.tif")
-            gt.to_tiff(path, host.astype(np.float32))
+            gt.to_tiff(path, host[:nt].astype(np.float32))
             with gt.VideoFile(path) as vf, gt.Context(p2, H, W) as ctx3:
-                ctx3.execute(vf, keys=range(min(n, 8)))
+                ctx3.execute(vf, keys=range(min(nt, 8)))
                 t0 = time.perf_counter()
                 data3, _, _ = ctx3.execute(vf)
                 dt3 = time.perf_counter() - t0
-        same = data3.shape == data.shape and np.array_equal(data3.view(np.uint32),
-                                                           data.view(np.uint32))
-        e2e["from_tiff"] = {"value": round(n / dt3, 1), "unit": "frames/s",
+        head = data[data[:, 0] < nt]
+        same = data3.shape == head.shape and np.array_equal(
+            np.ascontiguousarray(data3[:, :-1]).view(np.uint32),
+            np.ascontiguousarray(head[:, :-1]).view(np.uint32))
+        e2e["from_tiff"] = {"frames": nt, "value": round(nt / dt3, 1), "unit": "frames/s",
                             "identical_to_array_run": bool(same)}
         print(f"[bench] end to end from a TIFF file {e2e['from_tiff']['value']} frames/s "
               f"(identical: {same})", file=sys.stderr, flush=True)
@@ -965,6 +1002,11 @@ def main():
             print("[bench] CPU baseline", file=sys.stderr, flush=True)
             cpu, nchk, bad = _cpu_baseline(host, params, args.cpu_seconds, rows)
             fp64 = _fp64_error(host, params, rows)
+            if e2e is not None:
+                e2e["oracle_check"] = _e2e_oracle_check(e2e_sample[0], params, e2e_sample[1],
+                                                        e2e_sample[2], 5.0, 2)
+                print(f"[bench] end to end against the oracle {e2e['oracle_check']}",
+                      file=sys.stderr, flush=True)
         parity = {"frames_checked": nchk, "frames_bit_identical": nchk - bad}
 
     res = {
