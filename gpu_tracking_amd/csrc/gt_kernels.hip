@@ -236,11 +236,16 @@ __device__ __forceinline__ float div_norm(float a, float b, float rb, float rl) 
 template <int R>
 struct BpLds {
     static constexpr int RH = B2_TH + 2 * R;
-    static constexpr int RWP = (B2_TW + 2 * R + 3) & ~3;   // staged row pitch, 8-byte aligned
-    uint16_t Sraw[RH * RWP];   // the frame's samples (u8 / u16), converted when read
-    static_assert((RH * RWP * sizeof(uint16_t)) % 16 == 0, "Sg, Sc stay 16-byte aligned");
-    float Sg[RH * B2_TW];
-    float Sc[RH * B2_TW];
+    static constexpr int RWP = (B2_TW + 2 * R + 3) & ~3;   // staged row pitch, 16-byte rows
+    // the staged samples, then (once every row pass has read them) the row
+    // pass's Gaussian and boxcar sums
+    union {
+        float Sraw[RH * RWP];   // the frame's samples (u8 / u16), converted when staged
+        struct {
+            float Sg[RH * B2_TW];
+            float Sc[RH * B2_TW];
+        };
+    };
 };
 
 // One 64 x 32 output tile of one frame (f, out: the frame's input and
@@ -255,7 +260,7 @@ __device__ __forceinline__ void bandpass_int_tile(const T *__restrict__ f, float
     constexpr int NV = (NT + 3 + 3) / 4;            // float4 reads per row-pass item
     static_assert(B2_TW - 4 + 4 * NV <= RWP, "row-pass reads stay inside the staged row");
     static_assert(B2_TH % B2_ROWS == 0 && (B2_TH / B2_ROWS) * B2_TW == B2_THREADS, "mapping");
-    uint16_t *Sraw = L.Sraw;
+    float *Sraw = L.Sraw;
     float *Sg = L.Sg, *Sc = L.Sc;
     const int tid = threadIdx.x;
     constexpr int PAIRS = (B2_TW + 2 * R + 1) / 2;  // u16 pairs per staged row
@@ -286,7 +291,11 @@ __device__ __forceinline__ void bandpass_int_tile(const T *__restrict__ f, float
 #pragma unroll
         for (int q = 0; q < QPASS; ++q) {
             if (active && rq + q * QROWS < RH)
-                *reinterpret_cast<uint2 *>(Sraw + (rq + q * QROWS) * RWP + 4 * cq) = vals[q];
+                *reinterpret_cast<float4 *>(Sraw + (rq + q * QROWS) * RWP + 4 * cq) =
+                    make_float4(static_cast<float>(vals[q].x & 0xFFFFu),
+                                static_cast<float>(vals[q].x >> 16),
+                                static_cast<float>(vals[q].y & 0xFFFFu),
+                                static_cast<float>(vals[q].y >> 16));
         }
     } else if (sizeof(T) == 2 && (R % 2) == 0 && (W % 2) == 0 &&
         (reinterpret_cast<uintptr_t>(f) & 3u) == 0 && y0 - R >= 0 && y0 + B2_TH + R <= H &&
@@ -309,7 +318,9 @@ __device__ __forceinline__ void bandpass_int_tile(const T *__restrict__ f, float
             const int idx = tid + q * B2_THREADS;
             if (idx < CN) {
                 const int ry = idx / PAIRS, rx = idx - ry * PAIRS;
-                *reinterpret_cast<uint32_t *>(Sraw + ry * RWP + 2 * rx) = vals[q];
+                *reinterpret_cast<float2 *>(Sraw + ry * RWP + 2 * rx) =
+                    make_float2(static_cast<float>(vals[q] & 0xFFFFu),
+                                static_cast<float>(vals[q] >> 16));
             }
         }
     } else {
@@ -327,7 +338,7 @@ __device__ __forceinline__ void bandpass_int_tile(const T *__restrict__ f, float
 #pragma unroll
         for (int q = 0; q < NL; ++q) {
             const int idx = tid + q * B2_THREADS;
-            if (idx < CN) Sraw[idx] = static_cast<uint16_t>(vals[q]);
+            if (idx < CN) Sraw[idx] = static_cast<float>(vals[q]);
         }
     }
     __syncthreads();
@@ -342,19 +353,20 @@ __device__ __forceinline__ void bandpass_int_tile(const T *__restrict__ f, float
     static_assert(B2_THREADS % IPR == 0, "whole rows per pass");
     constexpr int RPASS = (RH + RSTEP - 1) / RSTEP;
     const int ry0 = tid / IPR, cxi = (tid % IPR) * 4;
+    float4 keep_g[RPASS], keep_c[RPASS];
 #pragma unroll
     for (int q = 0; q < RPASS; ++q) {
         const int ry = ry0 + q * RSTEP, cx = cxi;
         if (ry >= RH) continue;
-        const uint2 *src4 = reinterpret_cast<const uint2 *>(Sraw + ry * RWP + cx);
+        const float4 *src4 = reinterpret_cast<const float4 *>(Sraw + ry * RWP + cx);
         float v[4 * NV];
 #pragma unroll
         for (int k = 0; k < NV; ++k) {
-            const uint2 t = src4[k];
-            v[4 * k + 0] = static_cast<float>(t.x & 0xFFFFu);
-            v[4 * k + 1] = static_cast<float>(t.x >> 16);
-            v[4 * k + 2] = static_cast<float>(t.y & 0xFFFFu);
-            v[4 * k + 3] = static_cast<float>(t.y >> 16);
+            const float4 t = src4[k];
+            v[4 * k + 0] = t.x;
+            v[4 * k + 1] = t.y;
+            v[4 * k + 2] = t.z;
+            v[4 * k + 3] = t.w;
         }
         float gs[4] = {0.0f, 0.0f, 0.0f, 0.0f};
         const int gx0 = x0 + cx - R;   // frame column of v[0]
@@ -369,13 +381,18 @@ __device__ __forceinline__ void bandpass_int_tile(const T *__restrict__ f, float
 #pragma unroll
                 for (int k = 0; k < 4; ++k) gs[k] = fmaf(p.w[t], v[k + t], gs[k]);
         } else {
+            // a tap outside the frame is skipped; fmaf(w, 0, s) == s for
+            // these non-negative sums, so such samples become zeros
+            float vm[NT + 3];
+#pragma unroll
+            for (int j = 0; j < NT + 3; ++j) {
+                const int px = gx0 + j;
+                vm[j] = px >= 0 && px < W ? v[j] : 0.0f;
+            }
 #pragma unroll
             for (int t = 0; t < NT; ++t)
 #pragma unroll
-                for (int k = 0; k < 4; ++k) {
-                    const int px = gx0 + k + t;
-                    if (px >= 0 && px < W) gs[k] = fmaf(p.w[t], v[k + t], gs[k]);
-                }
+                for (int k = 0; k < 4; ++k) gs[k] = fmaf(p.w[t], vm[k + t], gs[k]);
         }
         float cs[4];
         cs[0] = 0.0f;
@@ -392,8 +409,17 @@ __device__ __forceinline__ void bandpass_int_tile(const T *__restrict__ f, float
         oc.y = cs[1];
         oc.z = cs[2];
         oc.w = cs[3];
-        *reinterpret_cast<float4 *>(Sg + ry * B2_TW + cx) = og;
-        *reinterpret_cast<float4 *>(Sc + ry * B2_TW + cx) = oc;
+        keep_g[q] = og;
+        keep_c[q] = oc;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < RPASS; ++q) {
+        const int ry = ry0 + q * RSTEP;
+        if (ry < RH) {
+            *reinterpret_cast<float4 *>(Sg + ry * B2_TW + cxi) = keep_g[q];
+            *reinterpret_cast<float4 *>(Sc + ry * B2_TW + cxi) = keep_c[q];
+        }
     }
     __syncthreads();
     // Column pass: B2_ROWS consecutive rows of one column per thread (the
@@ -414,7 +440,8 @@ __device__ __forceinline__ void bandpass_int_tile(const T *__restrict__ f, float
     float csum = 0.0f;
 #pragma unroll
     for (int t = 0; t < NT; ++t) csum = csum + c[t];
-    float *orow = out + static_cast<size_t>(gy0) * W + gx;
+    // the wave's row pointer is uniform; each lane stores at column cx of it
+    float *orow = out + static_cast<size_t>(gy0) * W + x0;
     if (inner && x0 + B2_TW <= W) {
         // every tap and every output of the wave's rows lies in the frame
 #pragma unroll
@@ -425,7 +452,7 @@ __device__ __forceinline__ void bandpass_int_tile(const T *__restrict__ f, float
             for (int t = 0; t < NT; ++t) gs = fmaf(p.w[t], g[r + t], gs);
             float res = div_norm<kD2>(gs, gnorm, rgnorm, rgnorm_lo) - div_norm<kD2>(csum, box, rbox, rbox_lo);
             if (p.truncate && res < kTrunc) res = 0.0f;
-            *orow = res;
+            orow[cx] = res;
             orow += W;
         }
         return;
@@ -447,7 +474,7 @@ __device__ __forceinline__ void bandpass_int_tile(const T *__restrict__ f, float
         }
         float res = div_norm<kD2>(gs, gnorm, rgnorm, rgnorm_lo) - div_norm<kD2>(csum, box, rbox, rbox_lo);
         if (p.truncate && res < kTrunc) res = 0.0f;
-        if (rows_in || (gy0 + r < H && gx < W)) *orow = res;
+        if (rows_in || (gy0 + r < H && gx < W)) orow[cx] = res;
         orow += W;
     }
 }
