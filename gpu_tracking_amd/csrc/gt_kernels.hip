@@ -354,64 +354,66 @@ __device__ __forceinline__ void bandpass_int_tile(const T *__restrict__ f, float
     constexpr int RPASS = (RH + RSTEP - 1) / RSTEP;
     const int ry0 = tid / IPR, cxi = (tid % IPR) * 4;
     float4 keep_g[RPASS], keep_c[RPASS];
+    // kEdge: the tile touches the frame's left or right edge (its own copy of
+    // the pass, so the interior tiles' code carries no edge handling)
+    auto row_pass = [&](auto edge) {
+        constexpr bool kEdge = decltype(edge)::value;
 #pragma unroll
-    for (int q = 0; q < RPASS; ++q) {
-        const int ry = ry0 + q * RSTEP, cx = cxi;
-        if (ry >= RH) continue;
-        const float4 *src4 = reinterpret_cast<const float4 *>(Sraw + ry * RWP + cx);
-        float v[4 * NV];
+        for (int q = 0; q < RPASS; ++q) {
+            const int ry = ry0 + q * RSTEP, cx = cxi;
+            if (ry >= RH) continue;
+            const float4 *src4 = reinterpret_cast<const float4 *>(Sraw + ry * RWP + cx);
+            float v[4 * NV];
 #pragma unroll
-        for (int k = 0; k < NV; ++k) {
-            const float4 t = src4[k];
-            v[4 * k + 0] = t.x;
-            v[4 * k + 1] = t.y;
-            v[4 * k + 2] = t.z;
-            v[4 * k + 3] = t.w;
-        }
-        float gs[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-        const int gx0 = x0 + cx - R;   // frame column of v[0]
-        if (xin) {
-#pragma unroll
-            for (int t = 0; t < NT; ++t)
-#pragma unroll
-                for (int k = 0; k < 4; ++k) gs[k] = fmaf(p.w[t], v[k + t], gs[k]);
-        } else if (gx0 >= 0 && gx0 + 3 + 2 * R < W) {
-#pragma unroll
-            for (int t = 0; t < NT; ++t)
-#pragma unroll
-                for (int k = 0; k < 4; ++k) gs[k] = fmaf(p.w[t], v[k + t], gs[k]);
-        } else {
-            // a tap outside the frame is skipped; fmaf(w, 0, s) == s for
-            // these non-negative sums, so such samples become zeros
-            float vm[NT + 3];
-#pragma unroll
-            for (int j = 0; j < NT + 3; ++j) {
-                const int px = gx0 + j;
-                vm[j] = px >= 0 && px < W ? v[j] : 0.0f;
+            for (int k = 0; k < NV; ++k) {
+                const float4 t = src4[k];
+                v[4 * k + 0] = t.x;
+                v[4 * k + 1] = t.y;
+                v[4 * k + 2] = t.z;
+                v[4 * k + 3] = t.w;
             }
+            float gs[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+            const int gx0 = x0 + cx - R;   // frame column of v[0]
+            if (!kEdge || (gx0 >= 0 && gx0 + 3 + 2 * R < W)) {
 #pragma unroll
-            for (int t = 0; t < NT; ++t)
+                for (int t = 0; t < NT; ++t)
 #pragma unroll
-                for (int k = 0; k < 4; ++k) gs[k] = fmaf(p.w[t], vm[k + t], gs[k]);
+                    for (int k = 0; k < 4; ++k) gs[k] = fmaf(p.w[t], v[k + t], gs[k]);
+            } else {
+                // a tap outside the frame is skipped; fmaf(w, 0, s) == s for
+                // these non-negative sums, so such samples become zeros
+                float vm[NT + 3];
+#pragma unroll
+                for (int j = 0; j < NT + 3; ++j) {
+                    const int px = gx0 + j;
+                    vm[j] = px >= 0 && px < W ? v[j] : 0.0f;
+                }
+#pragma unroll
+                for (int t = 0; t < NT; ++t)
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) gs[k] = fmaf(p.w[t], vm[k + t], gs[k]);
+            }
+            float cs[4];
+            cs[0] = 0.0f;
+#pragma unroll
+            for (int t = 0; t < NT; ++t) cs[0] = cs[0] + v[t];
+#pragma unroll
+            for (int k = 1; k < 4; ++k) cs[k] = cs[k - 1] - v[k - 1] + v[k - 1 + NT];
+            float4 og, oc;
+            og.x = div_norm<kD2>(gs[0], gnorm, rgnorm, rgnorm_lo);
+            og.y = div_norm<kD2>(gs[1], gnorm, rgnorm, rgnorm_lo);
+            og.z = div_norm<kD2>(gs[2], gnorm, rgnorm, rgnorm_lo);
+            og.w = div_norm<kD2>(gs[3], gnorm, rgnorm, rgnorm_lo);
+            oc.x = cs[0];
+            oc.y = cs[1];
+            oc.z = cs[2];
+            oc.w = cs[3];
+            keep_g[q] = og;
+            keep_c[q] = oc;
         }
-        float cs[4];
-        cs[0] = 0.0f;
-#pragma unroll
-        for (int t = 0; t < NT; ++t) cs[0] = cs[0] + v[t];
-#pragma unroll
-        for (int k = 1; k < 4; ++k) cs[k] = cs[k - 1] - v[k - 1] + v[k - 1 + NT];
-        float4 og, oc;
-        og.x = div_norm<kD2>(gs[0], gnorm, rgnorm, rgnorm_lo);
-        og.y = div_norm<kD2>(gs[1], gnorm, rgnorm, rgnorm_lo);
-        og.z = div_norm<kD2>(gs[2], gnorm, rgnorm, rgnorm_lo);
-        og.w = div_norm<kD2>(gs[3], gnorm, rgnorm, rgnorm_lo);
-        oc.x = cs[0];
-        oc.y = cs[1];
-        oc.z = cs[2];
-        oc.w = cs[3];
-        keep_g[q] = og;
-        keep_c[q] = oc;
-    }
+    };
+    if (xin) row_pass(std::false_type{});
+    else row_pass(std::true_type{});
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < RPASS; ++q) {
