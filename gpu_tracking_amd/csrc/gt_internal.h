@@ -148,6 +148,11 @@ hipError_t gt_launch_frame_std(const void *frames, int dtype, uint32_t nframes, 
 hipError_t gt_launch_filter_close_rows(const gt_detection *rows, uint32_t cap,
                                        const uint32_t *counts, uint32_t nframes, float separation,
                                        float4 *post, hipStream_t s);
+// Packs each frame's rows (and post values, when post is non-null) of a
+// chunk one after another into orows / opost.
+hipError_t gt_launch_pack_rows(const gt_detection *rows, uint32_t cap, const uint32_t *counts,
+                               uint32_t nframes, const float4 *post, gt_detection *orows,
+                               float4 *opost, hipStream_t s);
 size_t gt_doughnut_lds_bytes(uint32_t nsig, uint32_t nring);
 hipError_t gt_launch_doughnut_rows(const void *raw, int dtype, int H, int W,
                                    const gt_detection *rows, uint32_t cap, const uint32_t *counts,

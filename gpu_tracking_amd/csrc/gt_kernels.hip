@@ -2187,6 +2187,51 @@ hipError_t gt_launch_filter_close_rows(const gt_detection *rows, uint32_t cap,
     return hipGetLastError();
 }
 
+namespace {
+
+// Packs frame f's rows (rows[f * cap ..], min(counts[f], cap) of them) and
+// their post-processing values after those of frames 0..f-1, so that a
+// chunk's rows read back in one copy.  One workgroup per frame; the frame's
+// offset is the sum of the counts before it (a chunk has few frames).
+constexpr int PK_THREADS = 256;
+static_assert(sizeof(gt_detection) % sizeof(uint32_t) == 0, "rows copy as words");
+__global__ __launch_bounds__(PK_THREADS) void k_pack_rows(const gt_detection *__restrict__ rows,
+                                                          uint32_t cap,
+                                                          const uint32_t *__restrict__ counts,
+                                                          const float4 *__restrict__ post,
+                                                          gt_detection *__restrict__ orows,
+                                                          float4 *__restrict__ opost) {
+    const uint32_t f = blockIdx.x;
+    uint32_t off = 0;
+    for (uint32_t g = threadIdx.x; g < f; g += PK_THREADS) off += min(counts[g], cap);
+    for (int o = 32; o >= 1; o >>= 1) off += __shfl_xor(off, o);
+    __shared__ uint32_t s_off[PK_THREADS / 64];
+    if ((threadIdx.x & 63u) == 0) s_off[threadIdx.x >> 6] = off;
+    __syncthreads();
+    off = 0;
+#pragma unroll
+    for (int w = 0; w < PK_THREADS / 64; ++w) off += s_off[w];
+    const uint32_t n = min(counts[f], cap);
+    constexpr uint32_t WPR = sizeof(gt_detection) / sizeof(uint32_t);
+    const uint32_t *src = reinterpret_cast<const uint32_t *>(rows + static_cast<size_t>(f) * cap);
+    uint32_t *dst = reinterpret_cast<uint32_t *>(orows + off);
+    for (uint32_t i = threadIdx.x; i < n * WPR; i += PK_THREADS) dst[i] = src[i];
+    if (post) {
+        const float4 *ps = post + static_cast<size_t>(f) * cap;
+        for (uint32_t i = threadIdx.x; i < n; i += PK_THREADS) opost[off + i] = ps[i];
+    }
+}
+
+}  // namespace
+
+hipError_t gt_launch_pack_rows(const gt_detection *rows, uint32_t cap, const uint32_t *counts,
+                               uint32_t nframes, const float4 *post, gt_detection *orows,
+                               float4 *opost, hipStream_t s) {
+    if (nframes == 0) return hipSuccess;
+    k_pack_rows<<<nframes, PK_THREADS, 0, s>>>(rows, cap, counts, post, orows, opost);
+    return hipGetLastError();
+}
+
 size_t gt_doughnut_lds_bytes(uint32_t nsig, uint32_t nring) {
     return sizeof(float) * (static_cast<size_t>(nsig) + nring);
 }

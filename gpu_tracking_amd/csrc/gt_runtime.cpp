@@ -151,6 +151,14 @@ struct gt_context {
         uint32_t *d_counts = nullptr;  // chunk
         void *h_stage = nullptr;       // pinned, chunk * HW * 4 bytes
         uint32_t *h_counts = nullptr;  // pinned, chunk
+        gt_detection *d_packed = nullptr;  // the chunk's rows one frame after another
+        float4 *d_packed_post = nullptr;   // and their post values (gpu_post)
+        // pinned copies of the packed rows (and post values): the first
+        // pk_copied rows are read back in stream order right after the
+        // chunk's kernels, the rest (if any) by drain
+        gt_detection *h_pk = nullptr;
+        float4 *h_pk_post = nullptr;
+        size_t pk_cap = 0, pk_copied = 0;
         hipEvent_t done = nullptr;     // counts of the slot's chunk are on the host
     } slot[2];
     uint32_t det_cap = 0;
@@ -164,6 +172,7 @@ struct gt_context {
     size_t h_post_cap = 0;
     hipStream_t copy_stream = nullptr;  // detection read-back
     gt_detection *h_rows = nullptr;     // pinned read-back buffer
+    size_t pk_guess = 65536;            // rows read back with a chunk's kernels
     size_t h_rows_cap = 0;
 
     // gt_submit_frame / gt_fetch_frame: a ring of frames in flight, each with
@@ -265,12 +274,43 @@ gt_status alloc_execute_buffers(gt_context *c) {
         GT_HIP(hipMalloc(&sl.d_frames, 4 * HW * c->xchunk));
         GT_HIP(hipMalloc(&sl.d_rows, sizeof(gt_detection) * static_cast<size_t>(c->det_cap) * c->xchunk));
         GT_HIP(hipMalloc(&sl.d_counts, sizeof(uint32_t) * c->xchunk));
+        GT_HIP(hipMalloc(&sl.d_packed, sizeof(gt_detection) * static_cast<size_t>(c->det_cap) * c->xchunk));
+        if (c->gpu_post)
+            GT_HIP(hipMalloc(&sl.d_packed_post, sizeof(float4) * static_cast<size_t>(c->det_cap) * c->xchunk));
         GT_HIP(hipHostMalloc(&sl.h_stage, 4 * HW * c->xchunk, hipHostMallocDefault));
         GT_HIP(hipHostMalloc(reinterpret_cast<void **>(&sl.h_counts), sizeof(uint32_t) * c->xchunk,
                              hipHostMallocDefault));
         GT_HIP(hipEventCreateWithFlags(&sl.done, hipEventDisableTiming));
     }
     GT_HIP(hipStreamCreateWithFlags(&c->copy_stream, hipStreamNonBlocking));
+    return GT_OK;
+}
+
+// Grows slot sl's pinned read-back buffers to hold n rows, keeping the first
+// `keep` rows (no copy into them may be in flight).
+gt_status reserve_slot_rows(gt_context *c, gt_context::Slot &sl, size_t n, size_t keep) {
+    if (n <= sl.pk_cap) return GT_OK;
+    const size_t cap = std::max<size_t>(n, sl.pk_cap + sl.pk_cap / 2);
+    gt_detection *r = nullptr;
+    float4 *q = nullptr;
+    GT_HIP(hipHostMalloc(reinterpret_cast<void **>(&r), sizeof(gt_detection) * cap, hipHostMallocDefault));
+    if (c->gpu_post) {
+        const hipError_t e = hipHostMalloc(reinterpret_cast<void **>(&q), sizeof(float4) * cap,
+                                           hipHostMallocDefault);
+        if (e != hipSuccess) {
+            (void)hipHostFree(r);
+            return fail(GT_ERR_GPU_DEVICE, "pinned allocation failed: %s", hipGetErrorString(e));
+        }
+    }
+    if (keep) {
+        std::memcpy(r, sl.h_pk, sizeof(gt_detection) * keep);
+        if (q) std::memcpy(q, sl.h_pk_post, sizeof(float4) * keep);
+    }
+    if (sl.h_pk) (void)hipHostFree(sl.h_pk);
+    if (sl.h_pk_post) (void)hipHostFree(sl.h_pk_post);
+    sl.h_pk = r;
+    sl.h_pk_post = q;
+    sl.pk_cap = cap;
     return GT_OK;
 }
 
@@ -365,6 +405,10 @@ void free_all(gt_context *c) {
         if (sl.d_frames) (void)hipFree(sl.d_frames);
         if (sl.d_rows) (void)hipFree(sl.d_rows);
         if (sl.d_counts) (void)hipFree(sl.d_counts);
+        if (sl.d_packed) (void)hipFree(sl.d_packed);
+        if (sl.d_packed_post) (void)hipFree(sl.d_packed_post);
+        if (sl.h_pk) (void)hipHostFree(sl.h_pk);
+        if (sl.h_pk_post) (void)hipHostFree(sl.h_pk_post);
         if (sl.h_stage) (void)hipHostFree(sl.h_stage);
         if (sl.h_counts) (void)hipHostFree(sl.h_counts);
         if (sl.done) (void)hipEventDestroy(sl.done);
@@ -1189,8 +1233,8 @@ gt_status execute_frames(gt_context *c, const FrameSrc &src, const uint64_t *key
             return ist;
         }
     }
-    // stage chunk k in slot k % 2 and queue its GPU work
-    auto submit = [&](size_t k) -> gt_status {
+    // stage chunk k's frames in slot k % 2's pinned buffer (host work)
+    auto stage_chunk = [&](size_t k) -> gt_status {
         auto &sl = c->slot[k % 2];
         const size_t base = k * c->xchunk;
         const uint32_t nf = static_cast<uint32_t>(std::min<size_t>(c->xchunk, nplan - base));
@@ -1201,11 +1245,14 @@ gt_status execute_frames(gt_context *c, const FrameSrc &src, const uint64_t *key
                             "Requested a frame outside the length\n of the video. Video length is %llu, requested frame %llu",
                             static_cast<unsigned long long>(nframes), static_cast<unsigned long long>(idx));
         }
-        {
-            // the slot's previous chunk has been drained
-            gt_status sst = stage(sl, plan.frames.data() + base, nf);
-            if (sst != GT_OK) return sst;
-        }
+        // the slot's previous chunk has been drained
+        return stage(sl, plan.frames.data() + base, nf);
+    };
+    // queue chunk k's copy to the device and its GPU work
+    auto submit = [&](size_t k) -> gt_status {
+        auto &sl = c->slot[k % 2];
+        const size_t base = k * c->xchunk;
+        const uint32_t nf = static_cast<uint32_t>(std::min<size_t>(c->xchunk, nplan - base));
         GT_HIP(hipMemcpyAsync(sl.d_frames, sl.h_stage, esz * HW * nf, hipMemcpyHostToDevice, c->stream));
         gt_status st2;
         if (c->characterize_points)
@@ -1224,6 +1271,23 @@ gt_status execute_frames(gt_context *c, const FrameSrc &src, const uint64_t *key
                                                sl.d_counts, nf, c->d_dough_offs, c->dough_nsig,
                                                c->dough_nring, dp, c->stream));
         }
+        // the chunk's rows (and post values) one frame after another, read
+        // back in one copy each by drain
+        GT_HIP(gt_launch_pack_rows(sl.d_rows, c->det_cap, sl.d_counts, nf,
+                                   c->gpu_post ? c->d_post[k % 2] : nullptr, sl.d_packed,
+                                   sl.d_packed_post, c->stream));
+        // read back as many rows as recent chunks held, in stream order (a
+        // copy on another stream would queue behind the next chunk's upload)
+        sl.pk_copied = std::min<size_t>(c->pk_guess, static_cast<size_t>(c->det_cap) * nf);
+        {
+            gt_status rs = reserve_slot_rows(c, sl, sl.pk_copied, 0);
+            if (rs != GT_OK) return rs;
+        }
+        GT_HIP(hipMemcpyAsync(sl.h_pk, sl.d_packed, sizeof(gt_detection) * sl.pk_copied,
+                              hipMemcpyDeviceToHost, c->stream));
+        if (c->gpu_post)
+            GT_HIP(hipMemcpyAsync(sl.h_pk_post, sl.d_packed_post, sizeof(float4) * sl.pk_copied,
+                                  hipMemcpyDeviceToHost, c->stream));
         GT_HIP(hipMemcpyAsync(sl.h_counts, sl.d_counts, sizeof(uint32_t) * nf, hipMemcpyDeviceToHost, c->stream));
         GT_HIP(hipEventRecord(sl.done, c->stream));
         return GT_OK;
@@ -1242,28 +1306,27 @@ gt_status execute_frames(gt_context *c, const FrameSrc &src, const uint64_t *key
                             static_cast<unsigned long long>(plan.frames[base + f]), n, c->det_cap);
             total += n;
         }
-        gt_status rs = reserve_host_rows(c, total);
-        if (rs != GT_OK) return rs;
-        size_t off = 0;
-        for (uint32_t f = 0; f < nf; ++f) {
-            const uint32_t n = sl.h_counts[f];
-            if (n)
-                GT_HIP(hipMemcpyAsync(c->h_rows + off, sl.d_rows + static_cast<size_t>(f) * c->det_cap,
-                                      sizeof(gt_detection) * n, hipMemcpyDeviceToHost, c->copy_stream));
-            if (n && c->gpu_post)
-                GT_HIP(hipMemcpyAsync(c->h_post + 4 * off,
-                                      c->d_post[k % 2] + static_cast<size_t>(f) * c->det_cap,
-                                      sizeof(float4) * n, hipMemcpyDeviceToHost, c->copy_stream));
-            off += n;
+        if (total > sl.pk_copied) {
+            // more rows than the copy queued with the chunk: the rest now
+            gt_status rs = reserve_slot_rows(c, sl, total, sl.pk_copied);
+            if (rs != GT_OK) return rs;
+            const size_t rest = total - sl.pk_copied;
+            GT_HIP(hipMemcpyAsync(sl.h_pk + sl.pk_copied, sl.d_packed + sl.pk_copied,
+                                  sizeof(gt_detection) * rest, hipMemcpyDeviceToHost, c->copy_stream));
+            if (c->gpu_post)
+                GT_HIP(hipMemcpyAsync(sl.h_pk_post + sl.pk_copied, sl.d_packed_post + sl.pk_copied,
+                                      sizeof(float4) * rest, hipMemcpyDeviceToHost, c->copy_stream));
+            GT_HIP(hipStreamSynchronize(c->copy_stream));
         }
-        GT_HIP(hipStreamSynchronize(c->copy_stream));
-        off = 0;
+        c->pk_guess = std::max(c->pk_guess, total + total / 4);
+        const float *pk_post = reinterpret_cast<const float *>(sl.h_pk_post);
+        size_t off = 0;
         for (uint32_t f = 0; f < nf; ++f) {
             const uint32_t n = sl.h_counts[f];
             Item it;
             it.frame = plan.frames[base + f];
-            it.rows.assign(c->h_rows + off, c->h_rows + off + n);
-            if (c->gpu_post) it.post.assign(c->h_post + 4 * off, c->h_post + 4 * (off + n));
+            it.rows.assign(sl.h_pk + off, sl.h_pk + off + n);
+            if (c->gpu_post) it.post.assign(pk_post + 4 * off, pk_post + 4 * (off + n));
             if (src.host) {
                 it.raw = src.host + esz * HW * it.frame;
             } else if (keep_raw) {
@@ -1284,16 +1347,20 @@ gt_status execute_frames(gt_context *c, const FrameSrc &src, const uint64_t *key
     };
 
     gt_status lst = GT_OK;
-    if (nchunks) lst = submit(0);
+    if (nchunks) lst = stage_chunk(0);
+    if (nchunks && lst == GT_OK) lst = submit(0);
     for (size_t k = 0; k < nchunks && lst == GT_OK; ++k) {
         if (c->interrupt.load()) {
             lst = fail(GT_ERR_INTERRUPTED, "Keyboard Interrupt, my custom version");
             break;
         }
         if (post_failed()) break;
+        // chunk k+1 is staged on the host and queued while the GPU works on
+        // chunk k (slot (k+1) % 2 last held chunk k-1, drained in the
+        // previous pass)
         if (k + 1 < nchunks) {
-            // slot (k+1) % 2 last held chunk k-1, drained in the previous pass
-            lst = submit(k + 1);
+            lst = stage_chunk(k + 1);
+            if (lst == GT_OK) lst = submit(k + 1);
             if (lst != GT_OK) break;
         }
         lst = drain(k);
