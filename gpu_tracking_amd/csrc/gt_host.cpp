@@ -533,13 +533,15 @@ gt_status gt_host_post::process(std::vector<gt_detection> &rows, uint64_t frame_
     if (post4 && d_.style == GT_STYLE_TRACKPY) {
         // filter_close and the doughnut ran on the GPU: order the rows by
         // seed carrying their results, keep the survivors
-        order_.resize(rows.size());
-        for (size_t i = 0; i < rows.size(); ++i) order_[i] = static_cast<uint32_t>(i);
-        std::sort(order_.begin(), order_.end(),
-                  [&](uint32_t a, uint32_t b) { return rows[a].seed < rows[b].seed; });
+        // (seed, position) keys: a frame's seeds are distinct pixels
+        keys_.resize(rows.size());
+        for (size_t i = 0; i < rows.size(); ++i)
+            keys_[i] = (static_cast<uint64_t>(rows[i].seed) << 32) | static_cast<uint32_t>(i);
+        std::sort(keys_.begin(), keys_.end());
         tmp_rows_.clear();
         dough_.clear();
-        for (uint32_t i : order_) {
+        for (uint64_t key : keys_) {
+            const uint32_t i = static_cast<uint32_t>(key);
             const float *q = post4 + 4 * static_cast<size_t>(i);
             if (q[3] == 0.0f) continue;
             tmp_rows_.push_back(rows[i].row);
@@ -610,30 +612,37 @@ gt_status gt_host_post::assemble(uint64_t frame_index, std::atomic<int32_t> *int
         if (ls != GT_OK) return ls;
     }
     const float fidx = static_cast<float>(frame_index);
+    const size_t base = out_.size();
+    out_.resize(base + n * ncols_);
+    float *o = out_.data() + base;
     for (size_t i = 0; i < n; ++i) {
         const gt_result_row &r = tmp_rows_[i];
-        out_.push_back(fidx);
-        out_.push_back(r.x);
-        out_.push_back(r.y);
-        out_.push_back(r.mass);
-        if (d_.include_r) out_.push_back(r.r);
+        *o++ = fidx;
+        *o++ = r.x;
+        *o++ = r.y;
+        *o++ = r.mass;
+        if (d_.include_r) *o++ = r.r;
         if (d_.characterize) {
-            out_.push_back(r.Rg);
-            out_.push_back(r.raw_mass);
-            out_.push_back(r.signal);
-            out_.push_back(r.ecc);
+            *o++ = r.Rg;
+            *o++ = r.raw_mass;
+            *o++ = r.signal;
+            *o++ = r.ecc;
         }
         if (d_.doughnut) {
-            out_.push_back(dough_[3 * i + 0]);
-            out_.push_back(dough_[3 * i + 1]);
-            out_.push_back(dough_[3 * i + 2]);
+            *o++ = dough_[3 * i + 0];
+            *o++ = dough_[3 * i + 1];
+            *o++ = dough_[3 * i + 2];
         }
         if (linker_) {
-            out_.push_back(static_cast<float>(ids_[i]));
-            out_.push_back(0.0f);    // duration, filled in by finish()
-            row_seg_.push_back(static_cast<uint32_t>(seg_books_.size()));
+            *o++ = static_cast<float>(ids_[i]);
+            *o++ = 0.0f;    // duration, filled in by finish()
         }
     }
+    if (o != out_.data() + out_.size()) {
+        *err = "internal error: the row layout does not match the column list";
+        return GT_ERR_INVALID_ARGUMENT;
+    }
+    if (linker_) row_seg_.resize(row_seg_.size() + n, static_cast<uint32_t>(seg_books_.size()));
     return GT_OK;
 }
 

@@ -115,7 +115,7 @@ class gt_host_post {
     std::vector<gt_result_row> tmp_rows_;
     std::vector<uint8_t> keep_;
     std::vector<float> dough_;
-    std::vector<uint32_t> order_;
+    std::vector<uint64_t> keys_;
     // linking (search_range): one linker per segment between reset points
     std::vector<uint64_t> reset_;          // sorted frame indices
     std::unique_ptr<gt_linker> linker_;
