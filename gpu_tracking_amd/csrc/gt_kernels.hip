@@ -2130,44 +2130,79 @@ __global__ __launch_bounds__(64) void k_doughnut_rows(
     const gt_detection *fr = rows + static_cast<size_t>(f) * cap;
     float4 *fp = post + static_cast<size_t>(f) * cap;
     const uint32_t lane = threadIdx.x;
+    // the largest offset of either stencil (the ring's outer radius)
+    int rmax = 0;
+    for (uint32_t k = lane; k < nsig + nring; k += 64)
+        rmax = max(rmax, max(abs(static_cast<int>(offs[k].x)), abs(static_cast<int>(offs[k].y))));
+    for (int o = 32; o >= 1; o >>= 1) rmax = max(rmax, __shfl_xor(rmax, o));
     for (uint32_t i = blockIdx.x; i < n; i += gridDim.x) {
         if (fp[i].w == 0.0f) continue;       // dropped by filter_close (block-uniform)
         const int64_t mu = f2i(roundf(fr[i].row.x)), mv = f2i(roundf(fr[i].row.y));
-        for (uint32_t k = lane; k < nsig; k += 64) {
-            const int64_t pu = mu + offs[k].x, pv = mv + offs[k].y;
-            sig[k] = (pu >= 0 && pu < H && pv >= 0 && pv < W) ? to_f32(F[pu * W + pv]) : 0.0f;
+        // every offset lands in the frame: no bounds tests, the ring in
+        // offset order
+        const bool all_in = mu - rmax >= 0 && mu + rmax < H && mv - rmax >= 0 && mv + rmax < W;
+        uint32_t nv;
+        if (all_in) {
+            for (uint32_t k = lane; k < nsig; k += 64)
+                sig[k] = to_f32(F[(mu + offs[k].x) * W + (mv + offs[k].y)]);
+            for (uint32_t k = lane; k < nring; k += 64)
+                ring[k] = to_f32(F[(mu + offs[nsig + k].x) * W + (mv + offs[nsig + k].y)]);
+            nv = nring;
+        } else {
+            for (uint32_t k = lane; k < nsig; k += 64) {
+                const int64_t pu = mu + offs[k].x, pv = mv + offs[k].y;
+                sig[k] = (pu >= 0 && pu < H && pv >= 0 && pv < W) ? to_f32(F[pu * W + pv]) : 0.0f;
+            }
+            if (lane == 0) s_nv = 0;
+            __syncthreads();
+            for (uint32_t k = lane; k < nring; k += 64) {
+                const int64_t pu = mu + offs[nsig + k].x, pv = mv + offs[nsig + k].y;
+                if (pu >= 0 && pu < H && pv >= 0 && pv < W)
+                    ring[atomicAdd(&s_nv, 1u)] = to_f32(F[pu * W + pv]);
+            }
+            __syncthreads();
+            nv = s_nv;
         }
-        if (lane == 0) s_nv = 0;
-        __syncthreads();
-        for (uint32_t k = lane; k < nring; k += 64) {
-            const int64_t pu = mu + offs[nsig + k].x, pv = mv + offs[nsig + k].y;
-            if (pu >= 0 && pu < H && pv >= 0 && pv < W)
-                ring[atomicAdd(&s_nv, 1u)] = to_f32(F[pu * W + pv]);
-        }
-        __syncthreads();
-        const uint32_t nv = s_nv;
         const uint32_t kth = nv / 2;
         if (lane == 0) s_med = 0.0f;
         __syncthreads();
-        for (uint32_t a = lane; a < nv; a += 64) {
-            const float v = ring[a];
-            uint32_t less = 0, leq = 0;
+        // rank each ring value against all of them, up to four values per
+        // lane against one pass over the ring
+        for (uint32_t a0 = 0; a0 < nv; a0 += 4 * 64) {
+            float v[4];
+            uint32_t less[4] = {0u, 0u, 0u, 0u}, leq[4] = {0u, 0u, 0u, 0u};
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const uint32_t a = a0 + lane + 64u * j;
+                v[j] = a < nv ? ring[a] : 0.0f;
+            }
             for (uint32_t b = 0; b < nv; ++b) {
                 const float w = ring[b];
-                less += w < v ? 1u : 0u;
-                leq += w <= v ? 1u : 0u;
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    less[j] += w < v[j] ? 1u : 0u;
+                    leq[j] += w <= v[j] ? 1u : 0u;
+                }
             }
-            if (less <= kth && kth < leq) s_med = v;   // equal values: the same float
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                if (a0 + lane + 64u * j < nv && less[j] <= kth && kth < leq[j])
+                    s_med = v[j];   // equal values: the same float
         }
         __syncthreads();
         if (lane == 0) {
             float sm = 0.0f;
             uint32_t ns = 0;
-            for (uint32_t k = 0; k < nsig; ++k) {
-                const int64_t pu = mu + offs[k].x, pv = mv + offs[k].y;
-                if (pu < 0 || pu >= H || pv < 0 || pv >= W) continue;
-                sm = sm + sig[k];
-                ++ns;
+            if (all_in) {
+                for (uint32_t k = 0; k < nsig; ++k) sm = sm + sig[k];
+                ns = nsig;
+            } else {
+                for (uint32_t k = 0; k < nsig; ++k) {
+                    const int64_t pu = mu + offs[k].x, pv = mv + offs[k].y;
+                    if (pu < 0 || pu >= H || pv < 0 || pv >= W) continue;
+                    sm = sm + sig[k];
+                    ++ns;
+                }
             }
             const float med = nv ? s_med : 0.0f;
             fp[i] = make_float4(sm, med, sm - med * static_cast<float>(ns), 1.0f);
