@@ -270,17 +270,18 @@ class Context:
                 None if reset is None else reset.ctypes.data, 0 if reset is None else len(reset),
                 C.byref(out))
         if isinstance(video, VideoFile):
-            st, interrupted = self._interruptible(self._lib.gt_execute_file, self._h,
+            st, interrupted, msg = self._interruptible(self._lib.gt_execute_file, self._h,
                                                   video.handle, *rest)
         else:
-            st, interrupted = self._interruptible(self._lib.gt_execute, self._h,
+            st, interrupted, msg = self._interruptible(self._lib.gt_execute, self._h,
                                                   video.ctypes.data, _dtype_code(video),
                                                   video.shape[0], *rest)
         if interrupted and st == 0:
             # the call finished before it saw the flag: the interrupt still wins
             self._lib.gt_output_free(C.byref(out))
             raise N.error(19, "Keyboard Interrupt")
-        N.check(st)
+        if st != 0:
+            raise N.error(st, msg)
         try:
             n, m = int(out.nrows), int(out.ncols)
             names = [out.col_names[i].decode() for i in range(m)]
@@ -291,7 +292,7 @@ class Context:
             self._lib.gt_output_free(C.byref(out))
         return data, names, types
 
-    def _interruptible(self, fn, *args) -> int:
+    def _interruptible(self, fn, *args):
         """Runs a whole-video call on a worker thread while this thread
         waits in 50 ms steps, so that Ctrl-C reaches Python during the call.
         A KeyboardInterrupt sets the context's interrupt flag; the call then
@@ -302,6 +303,8 @@ class Context:
 
         def work():
             box["st"] = fn(*args)
+            # the library keeps the message per thread: read it on this one
+            box["msg"] = self._lib.gt_last_error().decode(errors="replace") if box["st"] else ""
 
         t = threading.Thread(target=work, name="gt_execute", daemon=True)
         t.start()
@@ -313,7 +316,7 @@ class Context:
             interrupted = True
             self._lib.gt_set_interrupt(self._h, 1)
             t.join()
-        return box["st"], interrupted
+        return box["st"], interrupted, box["msg"]
 
     # one frame at a time (submit_work / get_work) ----------------------------
     def submit_frame(self, frame: np.ndarray, frame_idx: int, positions=None) -> None:

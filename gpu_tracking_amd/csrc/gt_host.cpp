@@ -596,20 +596,29 @@ gt_status gt_host_post::assemble(uint64_t frame_index, std::atomic<int32_t> *int
                                  std::string *err) {
     const size_t n = tmp_rows_.size();
     if (linker_) {
-        // linker_reset_points (execute_gpu.rs:792-797): a new linker, ids
-        // start over
-        if (std::binary_search(reset_.begin(), reset_.end(), frame_index)) {
-            seg_books_.push_back(linker_->finish());
-            linker_.reset(new gt_linker(p_.search_range, p_.has_memory ? p_.memory : 0u));
-        }
-        xy_.resize(2 * n);
+        // the frame's positions go to the linking thread (started with the
+        // first frame); an error it met ends the run here
+        LinkJob job;
+        job.frame = frame_index;
+        job.row0 = ncols_ ? out_.size() / ncols_ : 0;
+        job.xy.resize(2 * n);
         for (size_t i = 0; i < n; ++i) {
-            xy_[2 * i] = tmp_rows_[i].x;
-            xy_[2 * i + 1] = tmp_rows_[i].y;
+            job.xy[2 * i] = tmp_rows_[i].x;
+            job.xy[2 * i + 1] = tmp_rows_[i].y;
         }
-        ids_.resize(n);
-        gt_status ls = linker_->advance(xy_.data(), n, ids_.data(), interrupt, err);
-        if (ls != GT_OK) return ls;
+        {
+            std::lock_guard<std::mutex> lk(link_mu_);
+            if (link_status_ != GT_OK) {
+                *err = link_err_;
+                return link_status_;
+            }
+            link_q_.push_back(std::move(job));
+        }
+        if (!link_thread_.joinable()) {
+            link_interrupt_ = interrupt;
+            link_thread_ = std::thread([this] { link_loop(); });
+        }
+        link_cv_.notify_one();
     }
     const float fidx = static_cast<float>(frame_index);
     const size_t base = out_.size();
@@ -634,30 +643,89 @@ gt_status gt_host_post::assemble(uint64_t frame_index, std::atomic<int32_t> *int
             *o++ = dough_[3 * i + 2];
         }
         if (linker_) {
-            *o++ = static_cast<float>(ids_[i]);
-            *o++ = 0.0f;    // duration, filled in by finish()
+            *o++ = 0.0f;    // particle and duration, filled in by finish()
+            *o++ = 0.0f;
         }
     }
     if (o != out_.data() + out_.size()) {
         *err = "internal error: the row layout does not match the column list";
         return GT_ERR_INVALID_ARGUMENT;
     }
-    if (linker_) row_seg_.resize(row_seg_.size() + n, static_cast<uint32_t>(seg_books_.size()));
     return GT_OK;
 }
 
+gt_host_post::~gt_host_post() {
+    std::string ignored;
+    (void)stop_linking(&ignored);
+}
+
+void gt_host_post::link_loop() {
+    std::vector<uint64_t> ids;
+    for (;;) {
+        LinkJob job;
+        {
+            std::unique_lock<std::mutex> lk(link_mu_);
+            link_cv_.wait(lk, [&] { return !link_q_.empty() || link_stop_; });
+            if (link_q_.empty()) return;
+            job = std::move(link_q_.front());
+            link_q_.pop_front();
+            if (link_status_ != GT_OK) continue;   // drain after an error
+        }
+        // linker_reset_points (execute_gpu.rs:792-797): a new linker, ids
+        // start over
+        if (std::binary_search(reset_.begin(), reset_.end(), job.frame)) {
+            seg_books_.push_back(linker_->finish());
+            linker_.reset(new gt_linker(p_.search_range, p_.has_memory ? p_.memory : 0u));
+        }
+        const size_t n = job.xy.size() / 2;
+        ids.resize(n);
+        std::string err;
+        const gt_status ls = linker_->advance(job.xy.data(), n, ids.data(), link_interrupt_, &err);
+        if (ls != GT_OK) {
+            std::lock_guard<std::mutex> lk(link_mu_);
+            link_status_ = ls;
+            link_err_ = err;
+            continue;
+        }
+        if (row_ids_.size() < job.row0 + n) {
+            row_ids_.resize(job.row0 + n, 0);
+            row_seg_.resize(job.row0 + n, 0);
+        }
+        for (size_t i = 0; i < n; ++i) {
+            row_ids_[job.row0 + i] = ids[i];
+            row_seg_[job.row0 + i] = static_cast<uint32_t>(seg_books_.size());
+        }
+    }
+}
+
+gt_status gt_host_post::stop_linking(std::string *err) {
+    if (!link_thread_.joinable()) return GT_OK;
+    {
+        std::lock_guard<std::mutex> lk(link_mu_);
+        link_stop_ = true;
+    }
+    link_cv_.notify_all();
+    link_thread_.join();
+    if (link_status_ != GT_OK) *err = link_err_;
+    return link_status_;
+}
+
 gt_status gt_host_post::finish(gt_output *out, std::string *err) {
-    (void)err;
     if (linker_) {
-        // duration column (execute_gpu.rs:1165-1181), the last column.  Each
-        // segment between reset points keeps its own durations (the
-        // reference reads every row's id in the last segment's table).
+        const gt_status ls = stop_linking(err);
+        if (ls != GT_OK) return ls;
+        // particle column, then the duration column (execute_gpu.rs:
+        // 1165-1181), the last two.  Each segment between reset points keeps
+        // its own durations (the reference reads every row's id in the last
+        // segment's table).
         seg_books_.push_back(linker_->finish());
         const uint32_t nc = ncols_;
-        for (size_t r = 0; r < row_seg_.size(); ++r) {
+        const size_t nrows = nc ? out_.size() / nc : 0;
+        for (size_t r = 0; r < nrows && r < row_ids_.size(); ++r) {
             float *row = &out_[r * nc];
             const auto &books = seg_books_[row_seg_[r]];
-            const size_t id = static_cast<size_t>(row[nc - 2]);
+            const uint64_t id = row_ids_[r];
+            row[nc - 2] = static_cast<float>(id);
             row[nc - 1] = id < books.size() ? static_cast<float>(books[id].duration) : 0.0f;
         }
     }

@@ -4,8 +4,12 @@
 #pragma once
 
 #include <atomic>
+#include <condition_variable>
+#include <deque>
 #include <memory>
+#include <mutex>
 #include <string>
+#include <thread>
 #include <utility>
 #include <vector>
 
@@ -92,6 +96,9 @@ class gt_host_post {
   public:
     gt_host_post(const gt_params *p, const gt_derived *d, uint32_t H, uint32_t W,
                  const uint64_t *reset_points, uint64_t nreset);
+    ~gt_host_post();
+    gt_host_post(const gt_host_post &) = delete;
+    gt_host_post &operator=(const gt_host_post &) = delete;
     bool ok() const { return error_.empty(); }
     const std::string &error() const { return error_; }
     bool needs_raw() const { return d_.doughnut != 0; }
@@ -106,6 +113,17 @@ class gt_host_post {
 
   private:
     gt_status assemble(uint64_t frame_index, std::atomic<int32_t> *interrupt, std::string *err);
+    // Linking runs on its own thread, frame by frame in order, beside the
+    // per-frame work: each frame's positions and first output row go to it
+    // as a job; the particle ids land in row_ids_ and are written into the
+    // output by finish().
+    struct LinkJob {
+        uint64_t frame;
+        size_t row0;
+        std::vector<float> xy;
+    };
+    void link_loop();
+    gt_status stop_linking(std::string *err);
     gt_params p_;
     gt_derived d_;
     uint32_t H_, W_;
@@ -121,8 +139,15 @@ class gt_host_post {
     std::unique_ptr<gt_linker> linker_;
     std::vector<std::vector<gt_linker::Book>> seg_books_;   // finished segments
     std::vector<uint32_t> row_seg_;        // segment of each output row
-    std::vector<uint64_t> ids_;
-    std::vector<float> xy_;
+    std::vector<uint64_t> row_ids_;        // particle id of each output row
+    std::thread link_thread_;
+    std::mutex link_mu_;
+    std::condition_variable link_cv_;
+    std::deque<LinkJob> link_q_;
+    bool link_stop_ = false;
+    gt_status link_status_ = GT_OK;        // under link_mu_
+    std::string link_err_;                 // under link_mu_
+    std::atomic<int32_t> *link_interrupt_ = nullptr;
 };
 
 // Implemented in gt_runtime.cpp (needs the device).

@@ -103,6 +103,8 @@ def test_batch_errors(gt):
     with pytest.raises(gt.GtError) as e:
         gt.batch(v, 9, frames=[0, 7])
     assert e.value.status == 7  # FrameOutOfBounds
+    # the message of the call's own thread, not "no error"
+    assert "outside the length" in str(e.value)
     with pytest.raises(gt.GtError) as e:
         gt.batch(v[:0], 9)
     assert e.value.status == 5  # EmptyIterator
@@ -187,6 +189,23 @@ def test_batch_with_linking(gt, oracle):
         first.setdefault(i, f)
         last[i] = f
     assert df["duration"].tolist() == [last[i] - first[i] + 1 for i in ids]
+
+
+def test_batch_linking_too_dense_raises(gt):
+    # a search range that joins every particle of a crowded frame into one
+    # subnetwork: TooDenseToLink (linking.rs: load above 5000), raised from
+    # the linking thread at the end of the run, with the reference's message
+    from gpu_tracking_amd import _native as N
+    v = _video(T=3, seed=14, H=128, W=128, n=90)
+    print("[dense] batch() over", v.shape, flush=True)
+    with pytest.raises(N.GtError) as e:
+        gt.batch(v, 9, minmass=0, search_range=200.0, doughnut_correction=False)
+    assert e.value.status == 18
+    assert "computational load during linking is abnormally high" in str(e.value)
+    # the same context still links a sparse video afterwards
+    df = gt.batch(_video(T=3, seed=15, n=20), 9, minmass=300, search_range=5.0,
+                  doughnut_correction=False)
+    assert list(df.columns)[-2:] == ["particle", "duration"]
 
 
 def test_batch_linker_reset_points(gt, oracle):
