@@ -191,6 +191,22 @@ def test_batch_with_linking(gt, oracle):
     assert df["duration"].tolist() == [last[i] - first[i] + 1 for i in ids]
 
 
+def test_batch_read_back_beyond_the_first_copy(gt, oracle):
+    # gt_execute reads a chunk's packed rows back with its kernels, as many as
+    # recent chunks held (65536 at first), and the rest in a second copy: a
+    # 64-frame chunk of noise maxima (thousands per frame) takes that path,
+    # the 6-frame chunk after it the grown first copy
+    v = _video(T=70, seed=17, H=512, W=512, n=200)
+    print(f"[readback] batch() over {v.shape}", flush=True)
+    df = gt.batch(v, 7, separation=6, minmass=0, characterize=False,
+                  doughnut_correction=False, filter_close=False)
+    ref = _expected(oracle, v, 7, 6, 0.0, False, False, filter_close=False)
+    got = _frame_matrix(df)
+    assert len(got) > 65536 + 64 and (got[:, 0] < 64).sum() > 65536
+    assert got.shape == ref.shape
+    assert np.array_equal(got.view(np.uint32), ref.view(np.uint32))
+
+
 def test_batch_linking_too_dense_raises(gt):
     # a search range that joins every particle of a crowded frame into one
     # subnetwork: TooDenseToLink (linking.rs: load above 5000), raised from
