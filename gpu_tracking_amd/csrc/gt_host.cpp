@@ -622,7 +622,10 @@ gt_status gt_host_post::assemble(uint64_t frame_index, std::atomic<int32_t> *int
     }
     const float fidx = static_cast<float>(frame_index);
     const size_t base = out_.size();
-    out_.resize(base + n * ncols_);
+    if (!out_.resize(base + n * ncols_)) {
+        *err = "out of host memory for the output rows";
+        return GT_ERR_GPU_DEVICE;
+    }
     float *o = out_.data() + base;
     for (size_t i = 0; i < n; ++i) {
         const gt_result_row &r = tmp_rows_[i];
@@ -756,9 +759,8 @@ gt_status gt_host_post::finish(gt_output *out, std::string *err) {
     const uint32_t nc = ncols_;
     out->ncols = nc;
     out->nrows = nc ? out_.size() / nc : 0;
-    out->data = static_cast<float *>(std::malloc(sizeof(float) * (out_.size() ? out_.size() : 1)));
-    if (!out->data) return GT_ERR_GPU_DEVICE;
-    if (!out_.empty()) std::memcpy(out->data, out_.data(), sizeof(float) * out_.size());
+    if (!out_.size() && !out_.resize(1)) return GT_ERR_GPU_DEVICE;   // a block to free, even empty
+    out->data = out_.release();
     out->col_names = names;
     out->col_types = types;
     return GT_OK;

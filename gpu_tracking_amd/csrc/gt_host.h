@@ -3,7 +3,9 @@
 // post-processing of execute_gpu (filter_close, doughnut, output assembly).
 #pragma once
 
+#include <algorithm>
 #include <atomic>
+#include <cstdlib>
 #include <condition_variable>
 #include <deque>
 #include <memory>
@@ -128,7 +130,37 @@ class gt_host_post {
     gt_derived d_;
     uint32_t H_, W_;
     uint32_t ncols_ = 0;
-    std::vector<float> out_;
+    // The output block, row-major, grown with realloc (large blocks move
+    // without copying) and handed to gt_output as it is.
+    struct OutBuf {
+        float *p = nullptr;
+        size_t n = 0, cap = 0;
+        OutBuf() = default;
+        OutBuf(const OutBuf &) = delete;
+        OutBuf &operator=(const OutBuf &) = delete;
+        ~OutBuf() { std::free(p); }
+        size_t size() const { return n; }
+        float *data() { return p; }
+        float &operator[](size_t i) { return p[i]; }
+        bool resize(size_t m) {   // new elements are left unset
+            if (m > cap) {
+                const size_t c = std::max(m, cap + cap / 2 + 4096);
+                float *q = static_cast<float *>(std::realloc(p, sizeof(float) * c));
+                if (!q) return false;
+                p = q;
+                cap = c;
+            }
+            n = m;
+            return true;
+        }
+        float *release() {
+            float *q = p;
+            p = nullptr;
+            n = cap = 0;
+            return q;
+        }
+    };
+    OutBuf out_;
     std::string error_;
     std::vector<gt_result_row> tmp_rows_;
     std::vector<uint8_t> keep_;
