@@ -125,9 +125,28 @@ gt_status gt_linker::link(size_t memory_start, uint64_t *ids, std::vector<Entry>
     size_t load = 0;
     stack_.clear();
     for (size_t i = 0; i < nd; ++i) {
+        if (vis_dest_[i]) continue;
+        // the common components, without the stack: a destination with no
+        // candidate (alone), and a destination and a source that are each
+        // other's only candidate (linked); the search below finds the same
+        const uint32_t nsrc = d2s_off_[i + 1] - d2s_off_[i];
+        if (nsrc == 0) {
+            vis_dest_[i] = 1;
+            continue;
+        }
+        if (nsrc == 1) {
+            const uint32_t s = d2s_[d2s_off_[i]].idx;
+            if (s2d_off_[s + 1] - s2d_off_[s] == 1) {
+                vis_dest_[i] = 1;
+                vis_src_[s] = 1;
+                out_[i] = static_cast<int64_t>(src[s].id);
+                out_src_[i] = s;
+                continue;
+            }
+        }
         psrc_.clear();
         pdest_.clear();
-        if (!vis_dest_[i]) {
+        {
             vis_dest_[i] = 1;
             pdest_.push_back(static_cast<uint32_t>(i));
             stack_.push_back({static_cast<uint32_t>(i), 1, 0});
